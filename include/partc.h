@@ -1,0 +1,214 @@
+/*
+ * partc.h -- C ABI of the MI355X (gfx950) sparse tensor contraction engine.
+ *
+ * The reference (ssomesh/parTC) has no FFI: its boundary is the C++ header
+ * API that main.cpp calls.  Each entry point below replaces one reference
+ * interface; the citation on each names it.  All functions return PTC_OK (0)
+ * or a negative ptc_status; nothing calls exit() (the reference exits on
+ * every error, e.g. tensor.hpp:116-119, fksCuckoo.hpp:439-441).
+ *
+ * Conventions
+ *  - Coordinates are uint32, 0-based, nnz x d row-major ("hedges", as in the
+ *    reference Tensor, tensor.hpp:19-26); values are float64.
+ *  - dim_tc is the reference's mode permutation: its first
+ *    num_key_dimensions entries are the key modes of a table built on the
+ *    tensor (fksCuckoo.hpp:33-41, :664).
+ *  - A ptc_ctx owns one HIP device and one HIP stream; call it from one host
+ *    thread at a time.
+ *  - Counts are 64-bit (the reference's 32-bit counts overflow above 2^32
+ *    multiply-adds, fksCuckooTC.hpp:156).
+ */
+#ifndef PARTC_H
+#define PARTC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PTC_MAX_MODES 8
+
+typedef enum {
+  PTC_OK = 0,
+  PTC_ERR_ARG = -1,         /* invalid argument / shape / permutation       */
+  PTC_ERR_HIP = -2,         /* HIP runtime error                            */
+  PTC_ERR_NOMEM = -3,       /* device or host allocation failed             */
+  PTC_ERR_EMPTY = -4,       /* no contributing pair: the reference prints
+                               "tensor contraction not possible!!" and exits
+                               (tensor.hpp:116-119)                          */
+  PTC_ERR_RANGE = -5,       /* index/count range exceeded                   */
+  PTC_ERR_UNSUPPORTED = -6, /* a key space wider than 64 bits               */
+  PTC_ERR_IO = -7,          /* file could not be opened/parsed              */
+  PTC_ERR_NODEV = -8,       /* no HIP device                                */
+  PTC_ERR_INTERNAL = -9     /* a device-side consistency check failed       */
+} ptc_status;
+
+/* ---------------------------------------------------------------- tensors */
+
+/* Host view of a COO tensor, field for field the reference struct Tensor
+ * (tensor.hpp:19-26).  N is 64-bit here. */
+typedef struct {
+  uint32_t d;
+  uint32_t num_key_dimensions;
+  uint64_t N;
+  const uint32_t* hedges;   /* N*d, row-major                */
+  const uint32_t* dims;     /* d (may be NULL: max index + 1) */
+  const double* vals;       /* N                             */
+  const uint32_t* dim_tc;   /* d, a permutation of 0..d-1    */
+} ptc_tensor_view;
+
+typedef struct ptc_ctx ptc_ctx;
+typedef struct ptc_dtensor ptc_dtensor;     /* device-resident COO tensor   */
+typedef struct ptc_hashtable ptc_hashtable; /* device key table (FKS role)  */
+typedef struct ptc_mlp ptc_mlp;             /* learned row-size predictor   */
+
+/* Context: one device, one stream.  device = -1 picks the current device. */
+int ptc_create(ptc_ctx** ctx, int device);
+void ptc_destroy(ptc_ctx* ctx);
+const char* ptc_last_error(const ptc_ctx* ctx);
+int ptc_synchronize(ptc_ctx* ctx);
+/* Raw hipStream_t of the context, for callers that order their own work. */
+void* ptc_stream(ptc_ctx* ctx);
+
+/* Copy a host tensor to the device (the reference keeps every tensor in host
+ * malloc memory; main.cpp:59-116).  Per-mode maxima are computed here. */
+int ptc_dtensor_upload(ptc_ctx* ctx, const ptc_tensor_view* t, ptc_dtensor** out);
+/* Wrap device arrays that the caller owns (not freed by ptc_dtensor_free).
+ * mode_max may be NULL: the maxima are then reduced on the device. */
+int ptc_dtensor_wrap(ptc_ctx* ctx, uint32_t d, uint32_t num_key_dimensions, uint64_t N,
+                     const uint32_t* d_hedges, const double* d_vals, const uint32_t* dim_tc,
+                     const uint32_t* dims, const uint32_t* mode_max, ptc_dtensor** out);
+/* Re-label the key modes of a device tensor without copying data (what
+ * main.cpp:68-109 does with a full copy to build A_copy). */
+int ptc_dtensor_set_keys(ptc_dtensor* t, uint32_t num_key_dimensions, const uint32_t* dim_tc);
+int ptc_dtensor_info(const ptc_dtensor* t, uint32_t* d, uint64_t* N, uint32_t* num_key_dimensions);
+/* Device pointers of a tensor (hedges, vals; vals32 for FP32 outputs). */
+int ptc_dtensor_device_ptrs(const ptc_dtensor* t, const uint32_t** hedges, const double** vals,
+                            const float** vals32);
+/* Copy a device tensor back.  hedges: N*d uint32, vals: N float64 (for an
+ * FP32 result the values are widened).  dims may be NULL. */
+int ptc_dtensor_download(ptc_ctx* ctx, const ptc_dtensor* t, uint32_t* hedges, double* vals,
+                         uint32_t* dims);
+int ptc_dtensor_download_f32(ptc_ctx* ctx, const ptc_dtensor* t, float* vals);
+void ptc_dtensor_free(ptc_dtensor* t);
+
+/* ------------------------------------------------------------ .meta spec */
+
+/* Parse a .meta file: two free-text lines, then c, then d mode ids whose
+ * first c are the contracted modes.  Replaces Tensor::populate_meta_info
+ * (tensor.hpp:151-173); fails instead of exiting. */
+int ptc_meta_parse(const char* path, uint32_t d, uint32_t* c_out, uint32_t* dim_tc_out);
+
+/* ------------------------------------------------------------- key table */
+
+/* Group the nonzeros of t by their key (the first num_key_dimensions modes of
+ * dim_tc): a device hash over the keys, built by wavefront-cooperative
+ * atomicCAS insertion, then a count -> scan -> scatter that lays out every
+ * key's nonzeros contiguously ("slices").  Replaces buildHashTable
+ * (fksCuckoo.hpp:1044-1060) together with reorder_tensor (fksCuckooTC.hpp:
+ * 31-77), which the reference runs inside tc_fksCuckoo.  The table keeps a
+ * reference to t; t must outlive it. */
+int ptc_hashtable_build(ptc_ctx* ctx, const ptc_dtensor* t, ptc_hashtable** out);
+/* Number of distinct keys and nonzeros grouped. */
+int ptc_hashtable_info(const ptc_hashtable* h, uint64_t* num_keys, uint64_t* num_items);
+/* Probe n keys (n x nk uint32 host array, key modes in dim_tc order).
+ * slice_len[i] = number of nonzeros with that key (0 if absent);
+ * slice_ids (optional, capacity cap) receives the concatenated original
+ * nonzero ids of every found slice, slice_off[i] (n+1) their offsets.
+ * Replaces Hash_Table::find_element_new (fksCuckoo.hpp:867-978). */
+int ptc_hashtable_probe(ptc_ctx* ctx, const ptc_hashtable* h, uint64_t n, const uint32_t* keys,
+                        uint64_t* slice_len, uint64_t* slice_off, uint32_t* slice_ids,
+                        uint64_t cap);
+void ptc_hashtable_free(ptc_hashtable* h);
+
+/* ------------------------------------------------------------ contraction */
+
+typedef enum {
+  PTC_METHOD_TWO_PASS = 0, /* exact symbolic count -> scan -> numeric        */
+  PTC_METHOD_LEARNED = 1,  /* MLP-sized single pass, exact fallback rows     */
+  PTC_METHOD_UPPER_BOUND = 2 /* single pass sized by min(flops, columns)     */
+} ptc_method;
+
+typedef enum {
+  PTC_VAL_F64 = 0,        /* default and oracle path                          */
+  PTC_VAL_F32_TWOSUM = 1  /* FP32 values, TwoSum-compensated accumulation    */
+} ptc_value_type;
+
+typedef enum {
+  PTC_ACC_AUTO = 0,
+  PTC_ACC_DENSE = 1,      /* direct-mapped (perfect-hash) LDS accumulator     */
+  PTC_ACC_HASH = 2,       /* open-addressing LDS hash accumulator            */
+  PTC_ACC_SPILL = 3       /* HBM hash accumulator (heavy rows)               */
+} ptc_acc_mode;
+
+typedef struct {
+  int method;             /* ptc_method                                      */
+  int value_type;         /* ptc_value_type                                  */
+  int acc_mode;           /* ptc_acc_mode; AUTO lets the tuner decide        */
+  uint32_t dense_max_cols;/* largest column space held densely in LDS (0 = default) */
+  uint32_t lds_hash_max;  /* largest row nnz held in an LDS hash (0 = default) */
+  uint64_t chunk_flops;   /* target multiply-adds per workgroup (0 = default) */
+  double predict_scale;   /* learned path: capacity = scale * prediction (0 = default 1.15) */
+  const ptc_mlp* mlp;     /* learned path: predictor (NULL = built-in)       */
+  int deterministic;      /* 1: one wavefront per row in A order (bit-stable values) */
+} ptc_options;
+
+typedef struct {
+  uint64_t nnz_a, nnz_b;
+  uint64_t rows;          /* nonempty rows of A (distinct free keys)         */
+  uint64_t flops;         /* useful multiply-adds F = sum_k |A_k| |B_k|      */
+  uint64_t nnz_out;
+  uint64_t fallback_rows; /* learned path: rows whose prediction overflowed  */
+  uint64_t heavy_rows;    /* rows split over several workgroups              */
+  uint64_t spill_rows;    /* rows accumulated in HBM                          */
+  double ms_build_b, ms_build_a, ms_preprocess, ms_symbolic, ms_numeric, ms_total;
+} ptc_stats;
+
+void ptc_options_default(ptc_options* opt);
+
+/* O = A x B over the contracted modes.  Replaces tc_fksCuckoo
+ * (fksCuckooTC.hpp:80-371) with the same preconditions: A is free-first
+ * (A.num_key_dimensions = d_A - c, A.dim_tc = [free..., contracted...]),
+ * B is contracted-first (B.num_key_dimensions = c); HT_A / HT_B were built
+ * on A / B; A's contracted modes pair position-wise with B's.  The output
+ * has d = (d_A - c) + (d_B - c) modes: A's free modes then B's free modes;
+ * dims are zero and dim_tc is the identity (tensor.hpp:106-149).  A
+ * contraction with no contributing pair returns PTC_ERR_EMPTY. */
+int ptc_tc_fksCuckoo(ptc_ctx* ctx, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B,
+                     const ptc_dtensor* A, const ptc_dtensor* B, const ptc_options* opt,
+                     ptc_dtensor** O, ptc_stats* stats);
+
+/* Convenience: both tensors carry .meta-style dim_tc (contracted modes
+ * first, num_key_dimensions = c, as main.cpp:65 leaves A).  Builds both
+ * tables and contracts, as main.cpp:68-118 does.  Replaces the driver path
+ * and matricize_spGEMM (matricizedspGEMM.hpp:172-603), whose result it
+ * equals. */
+int ptc_contract(ptc_ctx* ctx, const ptc_dtensor* A, const ptc_dtensor* B,
+                 const ptc_options* opt, ptc_dtensor** O, ptc_stats* stats);
+
+/* matricize_spGEMM (matricizedspGEMM.hpp:172) and matricize_sort_spGEMM
+ * (matricizedspGEMM_sorting.hpp:194; the reference exits before computing
+ * it, :754): both tensors contracted-first; same result as ptc_contract. */
+int ptc_matricize_spGEMM(ptc_ctx* ctx, const ptc_dtensor* A, const ptc_dtensor* B,
+                         ptc_dtensor** O, ptc_stats* stats);
+int ptc_matricize_sort_spGEMM(ptc_ctx* ctx, const ptc_dtensor* A, const ptc_dtensor* B,
+                              ptc_dtensor** O, ptc_stats* stats);
+
+/* ------------------------------------------------------ synthetic tensors */
+
+enum { PTC_DIST_UNIFORM = 0, PTC_DIST_ZIPF = 1 };
+
+/* Seeded unique-coordinate COO tensor (host arrays sized nnz*d and nnz). */
+int ptc_gen_coo(uint64_t seed, uint32_t d, const uint32_t* dims, uint64_t nnz, int dist,
+                double alpha, uint32_t* idx_out, double* val_out);
+/* Shape of one job of a batch of uneven contractions. */
+int ptc_gen_job_shape(uint64_t seed, uint32_t dim_lo, uint32_t dim_hi, uint64_t nnz_lo,
+                      uint64_t nnz_hi, uint32_t* d_out, uint32_t* dims_out, uint64_t* nnz_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PARTC_H */

@@ -1,0 +1,503 @@
+"""partc_amd -- Python host interface of the MI355X sparse tensor contraction
+engine (libpartc.so, C ABI in include/partc.h).
+
+The names mirror the reference's C++ API (ssomesh/parTC):
+
+    Tensor                 struct Tensor (tensor.hpp:19-174), numpy-backed
+    Tensor.populate_meta_info   tensor.hpp:151-173 (.meta: 2 text lines, c, d mode ids)
+    make_free_first        the A_copy construction of main.cpp:68-109
+    get_sparse_tensor      sparse_tensor_generator.hpp:4-88 (B := A)
+    buildHashTable         fksCuckoo.hpp:1044-1060 (+ reorder_tensor, fksCuckooTC.hpp:31-77)
+    tc_fksCuckoo           fksCuckooTC.hpp:80-371
+    matricize_spGEMM       matricizedspGEMM.hpp:172-603
+    matricize_sort_spGEMM  matricizedspGEMM_sorting.hpp:194 (the reference exits
+                           before computing it; here it returns the product)
+
+Every compute call runs the HIP kernels of libpartc.so on the GPU; there is
+no CPU fallback.  A missing library or device raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+from pathlib import Path
+from typing import Optional
+
+import numpy as np
+
+_PKG = Path(__file__).resolve().parent
+LIB_PATH = _PKG / "libpartc.so"
+MAX_MODES = 8
+
+PTC_OK = 0
+ERRORS = {
+    -1: "invalid argument",
+    -2: "HIP runtime error",
+    -3: "out of memory",
+    -4: "tensor contraction not possible",
+    -5: "range exceeded",
+    -6: "unsupported",
+    -7: "I/O error",
+    -8: "no HIP device",
+    -9: "internal consistency check failed",
+}
+
+METHOD_TWO_PASS, METHOD_LEARNED, METHOD_UPPER_BOUND = 0, 1, 2
+VAL_F64, VAL_F32_TWOSUM = 0, 1
+ACC_AUTO, ACC_DENSE, ACC_HASH, ACC_SPILL = 0, 1, 2, 3
+DIST_UNIFORM, DIST_ZIPF = 0, 1
+
+
+class PtcError(RuntimeError):
+    def __init__(self, code: int, msg: str = ""):
+        self.code = code
+        super().__init__(f"{ERRORS.get(code, 'error')} ({code}){': ' + msg if msg else ''}")
+
+
+class TensorView(C.Structure):
+    _fields_ = [
+        ("d", C.c_uint32),
+        ("num_key_dimensions", C.c_uint32),
+        ("N", C.c_uint64),
+        ("hedges", C.POINTER(C.c_uint32)),
+        ("dims", C.POINTER(C.c_uint32)),
+        ("vals", C.POINTER(C.c_double)),
+        ("dim_tc", C.POINTER(C.c_uint32)),
+    ]
+
+
+class Options(C.Structure):
+    _fields_ = [
+        ("method", C.c_int),
+        ("value_type", C.c_int),
+        ("acc_mode", C.c_int),
+        ("dense_max_cols", C.c_uint32),
+        ("lds_hash_max", C.c_uint32),
+        ("chunk_flops", C.c_uint64),
+        ("predict_scale", C.c_double),
+        ("mlp", C.c_void_p),
+        ("deterministic", C.c_int),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("nnz_a", C.c_uint64),
+        ("nnz_b", C.c_uint64),
+        ("rows", C.c_uint64),
+        ("flops", C.c_uint64),
+        ("nnz_out", C.c_uint64),
+        ("fallback_rows", C.c_uint64),
+        ("heavy_rows", C.c_uint64),
+        ("spill_rows", C.c_uint64),
+        ("ms_build_b", C.c_double),
+        ("ms_build_a", C.c_double),
+        ("ms_preprocess", C.c_double),
+        ("ms_symbolic", C.c_double),
+        ("ms_numeric", C.c_double),
+        ("ms_total", C.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib: Optional[C.CDLL] = None
+
+
+def lib() -> C.CDLL:
+    """Load libpartc.so (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise ImportError(f"{LIB_PATH} is missing: run `python -c 'import __graft_entry__ as g; g.build()'`")
+    L = C.CDLL(str(LIB_PATH))
+    vp, u32, u64, i32 = C.c_void_p, C.c_uint32, C.c_uint64, C.c_int
+    P32, P64, PD = C.POINTER(C.c_uint32), C.POINTER(C.c_uint64), C.POINTER(C.c_double)
+    sig = {
+        "ptc_create": (i32, [C.POINTER(vp), i32]),
+        "ptc_destroy": (None, [vp]),
+        "ptc_last_error": (C.c_char_p, [vp]),
+        "ptc_synchronize": (i32, [vp]),
+        "ptc_stream": (vp, [vp]),
+        "ptc_dtensor_upload": (i32, [vp, C.POINTER(TensorView), C.POINTER(vp)]),
+        "ptc_dtensor_wrap": (i32, [vp, u32, u32, u64, vp, vp, P32, P32, P32, C.POINTER(vp)]),
+        "ptc_dtensor_set_keys": (i32, [vp, u32, P32]),
+        "ptc_dtensor_info": (i32, [vp, P32, P64, P32]),
+        "ptc_dtensor_device_ptrs": (i32, [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp)]),
+        "ptc_dtensor_download": (i32, [vp, vp, P32, PD, P32]),
+        "ptc_dtensor_download_f32": (i32, [vp, vp, C.POINTER(C.c_float)]),
+        "ptc_dtensor_free": (None, [vp]),
+        "ptc_meta_parse": (i32, [C.c_char_p, u32, P32, P32]),
+        "ptc_hashtable_build": (i32, [vp, vp, C.POINTER(vp)]),
+        "ptc_hashtable_info": (i32, [vp, P64, P64]),
+        "ptc_hashtable_probe": (i32, [vp, vp, u64, P32, P64, P64, P32, u64]),
+        "ptc_hashtable_free": (None, [vp]),
+        "ptc_options_default": (None, [C.POINTER(Options)]),
+        "ptc_tc_fksCuckoo": (i32, [vp, vp, vp, vp, vp, C.POINTER(Options), C.POINTER(vp), C.POINTER(Stats)]),
+        "ptc_contract": (i32, [vp, vp, vp, C.POINTER(Options), C.POINTER(vp), C.POINTER(Stats)]),
+        "ptc_matricize_spGEMM": (i32, [vp, vp, vp, C.POINTER(vp), C.POINTER(Stats)]),
+        "ptc_matricize_sort_spGEMM": (i32, [vp, vp, vp, C.POINTER(vp), C.POINTER(Stats)]),
+        "ptc_gen_coo": (i32, [u64, u32, P32, u64, i32, C.c_double, P32, PD]),
+        "ptc_gen_job_shape": (i32, [u64, u32, u32, u64, u64, P32, P32, P64]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _p32(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_uint32))
+
+
+def _p64(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_uint64))
+
+
+def _pd(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+# --------------------------------------------------------------- host tensor
+
+
+@dataclass
+class Tensor:
+    """Host COO tensor with the reference's fields (tensor.hpp:19-26)."""
+
+    hedges_array: np.ndarray  # (N, d) uint32, 0-based
+    val_array: np.ndarray  # (N,) float64
+    dimensions_array: Optional[np.ndarray] = None  # (d,) uint32
+    dim_tc: Optional[np.ndarray] = None  # (d,) uint32 permutation, key modes first
+    num_key_dimensions: int = 0
+
+    def __post_init__(self):
+        h = np.ascontiguousarray(self.hedges_array, dtype=np.uint32)
+        if h.ndim == 1:
+            h = h.reshape(-1, 1)
+        self.hedges_array = h
+        self.val_array = np.ascontiguousarray(self.val_array, dtype=np.float64).reshape(-1)
+        if self.val_array.shape[0] != h.shape[0]:
+            raise ValueError("hedges and values disagree on N")
+        if self.dimensions_array is None:
+            self.dimensions_array = (h.max(axis=0) + 1 if h.shape[0] else np.zeros(h.shape[1])).astype(np.uint32)
+        self.dimensions_array = np.ascontiguousarray(self.dimensions_array, dtype=np.uint32)
+        if self.dim_tc is None:
+            self.dim_tc = np.arange(h.shape[1], dtype=np.uint32)
+        self.dim_tc = np.ascontiguousarray(self.dim_tc, dtype=np.uint32)
+
+    @property
+    def d(self) -> int:
+        return int(self.hedges_array.shape[1])
+
+    @property
+    def N(self) -> int:
+        return int(self.hedges_array.shape[0])
+
+    def populate_meta_info(self, filename: str) -> None:
+        """Read a .meta file: two free-text lines, c, then d mode ids whose
+        first c are the contracted modes (tensor.hpp:151-173)."""
+        c = C.c_uint32()
+        perm = np.zeros(MAX_MODES, dtype=np.uint32)
+        rc = lib().ptc_meta_parse(str(filename).encode(), self.d, C.byref(c), _p32(perm))
+        if rc != PTC_OK:
+            raise PtcError(rc, f"cannot parse {filename}")
+        self.num_key_dimensions = int(c.value)
+        self.dim_tc = perm[: self.d].copy()
+
+    def view(self) -> TensorView:
+        return TensorView(self.d, int(self.num_key_dimensions), self.N, _p32(self.hedges_array),
+                          _p32(self.dimensions_array), _pd(self.val_array), _p32(self.dim_tc))
+
+    def sorted(self) -> "Tensor":
+        """Copy with nonzeros in lexicographic coordinate order (the output
+        order of a contraction is unspecified, fksCuckooTC.hpp:345)."""
+        if self.N == 0:
+            return self
+        order = np.lexsort(self.hedges_array.T[::-1])
+        return Tensor(self.hedges_array[order], self.val_array[order], self.dimensions_array, self.dim_tc,
+                      self.num_key_dimensions)
+
+
+def make_free_first(A: Tensor) -> Tensor:
+    """A_copy of main.cpp:68-109: same data, key modes = A's free modes."""
+    c = int(A.num_key_dimensions)
+    perm = np.concatenate([A.dim_tc[c:], A.dim_tc[:c]]).astype(np.uint32)
+    return Tensor(A.hedges_array, A.val_array, A.dimensions_array, perm, A.d - c)
+
+
+def get_sparse_tensor(A: Tensor) -> Tensor:
+    """B := A with A's key split (sparse_tensor_generator.hpp:4-88, factor 1)."""
+    return Tensor(A.hedges_array.copy(), A.val_array.copy(), A.dimensions_array.copy(), A.dim_tc.copy(),
+                  A.num_key_dimensions)
+
+
+# ------------------------------------------------------------ device objects
+
+
+def _check(rc: int, ctx: Optional["Context"] = None) -> None:
+    if rc != PTC_OK:
+        msg = ctx.last_error() if ctx is not None else ""
+        raise PtcError(rc, msg)
+
+
+class Context:
+    """One device and one HIP stream (ptc_create)."""
+
+    def __init__(self, device: int = -1):
+        self._h = C.c_void_p()
+        rc = lib().ptc_create(C.byref(self._h), device)
+        if rc != PTC_OK:
+            raise PtcError(rc, "ptc_create")
+
+    @property
+    def handle(self):
+        return self._h
+
+    def last_error(self) -> str:
+        s = lib().ptc_last_error(self._h)
+        return s.decode() if s else ""
+
+    def synchronize(self) -> None:
+        _check(lib().ptc_synchronize(self._h), self)
+
+    def stream(self) -> int:
+        return int(lib().ptc_stream(self._h) or 0)
+
+    def close(self) -> None:
+        if self._h:
+            lib().ptc_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default_ctx: Optional[Context] = None
+
+
+def default_context() -> Context:
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context()
+    return _default_ctx
+
+
+class DeviceTensor:
+    """A COO tensor resident in HBM (ptc_dtensor)."""
+
+    def __init__(self, ctx: Context, handle: C.c_void_p, keep=None):
+        self.ctx = ctx
+        self._h = handle
+        self._keep = keep
+
+    @classmethod
+    def upload(cls, t: Tensor, ctx: Optional[Context] = None) -> "DeviceTensor":
+        ctx = ctx or default_context()
+        h = C.c_void_p()
+        v = t.view()
+        _check(lib().ptc_dtensor_upload(ctx.handle, C.byref(v), C.byref(h)), ctx)
+        return cls(ctx, h)
+
+    def info(self):
+        d, n, nk = C.c_uint32(), C.c_uint64(), C.c_uint32()
+        _check(lib().ptc_dtensor_info(self._h, C.byref(d), C.byref(n), C.byref(nk)))
+        return int(d.value), int(n.value), int(nk.value)
+
+    def set_keys(self, num_key_dimensions: int, dim_tc) -> None:
+        perm = np.ascontiguousarray(dim_tc, dtype=np.uint32)
+        _check(lib().ptc_dtensor_set_keys(self._h, num_key_dimensions, _p32(perm)), self.ctx)
+
+    def download(self) -> Tensor:
+        d, n, nk = self.info()
+        h = np.empty((n, d), dtype=np.uint32)
+        v = np.empty(n, dtype=np.float64)
+        dims = np.zeros(MAX_MODES, dtype=np.uint32)
+        _check(lib().ptc_dtensor_download(self.ctx.handle, self._h, _p32(h), _pd(v), _p32(dims)), self.ctx)
+        return Tensor(h, v, dims[:d].copy(), np.arange(d, dtype=np.uint32), nk)
+
+    def download_f32(self) -> np.ndarray:
+        d, n, nk = self.info()
+        v = np.empty(n, dtype=np.float32)
+        _check(lib().ptc_dtensor_download_f32(self.ctx.handle, self._h, v.ctypes.data_as(C.POINTER(C.c_float))),
+               self.ctx)
+        return v
+
+    def free(self) -> None:
+        if self._h:
+            lib().ptc_dtensor_free(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class HashTable:
+    """Device key table over a tensor's key modes (buildHashTable)."""
+
+    def __init__(self, ctx: Context, handle: C.c_void_p, tensor: DeviceTensor):
+        self.ctx = ctx
+        self._h = handle
+        self.tensor = tensor
+
+    def info(self):
+        k, n = C.c_uint64(), C.c_uint64()
+        _check(lib().ptc_hashtable_info(self._h, C.byref(k), C.byref(n)))
+        return int(k.value), int(n.value)
+
+    def find(self, keys: np.ndarray, with_ids: bool = True):
+        """Probe (n, nk) keys: slice lengths and, optionally, the original
+        nonzero ids of each slice (find_element_new, fksCuckoo.hpp:867)."""
+        keys = np.ascontiguousarray(keys, dtype=np.uint32)
+        if keys.ndim == 1:
+            keys = keys.reshape(1, -1)
+        n = keys.shape[0]
+        lens = np.zeros(n, dtype=np.uint64)
+        offs = np.zeros(n + 1, dtype=np.uint64)
+        _check(lib().ptc_hashtable_probe(self.ctx.handle, self._h, n, _p32(keys), _p64(lens), _p64(offs),
+                                         None, 0), self.ctx)
+        if not with_ids:
+            return lens
+        total = int(offs[-1])
+        ids = np.zeros(max(total, 1), dtype=np.uint32)
+        _check(lib().ptc_hashtable_probe(self.ctx.handle, self._h, n, _p32(keys), _p64(lens), _p64(offs),
+                                         _p32(ids), total), self.ctx)
+        return [ids[int(offs[i]):int(offs[i + 1])] for i in range(n)]
+
+    def free(self) -> None:
+        if self._h:
+            lib().ptc_hashtable_free(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def _as_device(t, ctx: Context) -> DeviceTensor:
+    return t if isinstance(t, DeviceTensor) else DeviceTensor.upload(t, ctx)
+
+
+def buildHashTable(T, ctx: Optional[Context] = None) -> HashTable:
+    """Group T's nonzeros by its key modes (T.dim_tc[:num_key_dimensions])
+    on the device (fksCuckoo.hpp:1044-1060 with reorder_tensor)."""
+    ctx = ctx or default_context()
+    dT = _as_device(T, ctx)
+    h = C.c_void_p()
+    _check(lib().ptc_hashtable_build(ctx.handle, dT._h, C.byref(h)), ctx)
+    return HashTable(ctx, h, dT)
+
+
+def options(method: int = METHOD_TWO_PASS, value_type: int = VAL_F64, acc_mode: int = ACC_AUTO,
+            dense_max_cols: int = 0, lds_hash_max: int = 0, chunk_flops: int = 0, predict_scale: float = 0.0,
+            mlp=None, deterministic: int = 0) -> Options:
+    o = Options()
+    lib().ptc_options_default(C.byref(o))
+    o.method = method
+    o.value_type = value_type
+    o.acc_mode = acc_mode
+    o.dense_max_cols = dense_max_cols
+    o.lds_hash_max = lds_hash_max
+    o.chunk_flops = chunk_flops
+    if predict_scale:
+        o.predict_scale = predict_scale
+    o.mlp = mlp._h if mlp is not None else None
+    o.deterministic = deterministic
+    return o
+
+
+def tc_fksCuckoo(HT_A: HashTable, HT_B: HashTable, A=None, B=None, *, download: bool = True,
+                 stats: Optional[Stats] = None, **opts):
+    """O = A x B (fksCuckooTC.hpp:80).  A free-first, B contracted-first; the
+    tables must have been built on A and B."""
+    ctx = HT_A.ctx
+    o = options(**opts)
+    out = C.c_void_p()
+    st = stats if stats is not None else Stats()
+    _check(lib().ptc_tc_fksCuckoo(ctx.handle, HT_A._h, HT_B._h, HT_A.tensor._h, HT_B.tensor._h, C.byref(o),
+                                  C.byref(out), C.byref(st)), ctx)
+    dO = DeviceTensor(ctx, out)
+    return dO.download() if download else dO
+
+
+def contract(A, B, ctx: Optional[Context] = None, *, download: bool = True, stats: Optional[Stats] = None,
+             **opts):
+    """Both tensors with .meta-style keys (contracted modes first); the
+    driver path of main.cpp:59-118 (A_copy, HT_B then HT_A, contract)."""
+    ctx = ctx or default_context()
+    dA, dB = _as_device(A, ctx), _as_device(B, ctx)
+    o = options(**opts)
+    out = C.c_void_p()
+    st = stats if stats is not None else Stats()
+    _check(lib().ptc_contract(ctx.handle, dA._h, dB._h, C.byref(o), C.byref(out), C.byref(st)), ctx)
+    dO = DeviceTensor(ctx, out)
+    return dO.download() if download else dO
+
+
+def matricize_spGEMM(A, B, ctx: Optional[Context] = None) -> Tensor:
+    """matricizedspGEMM.hpp:172 -- both tensors contracted-first."""
+    ctx = ctx or default_context()
+    dA, dB = _as_device(A, ctx), _as_device(B, ctx)
+    out = C.c_void_p()
+    _check(lib().ptc_matricize_spGEMM(ctx.handle, dA._h, dB._h, C.byref(out), None), ctx)
+    return DeviceTensor(ctx, out).download()
+
+
+def matricize_sort_spGEMM(A, B, ctx: Optional[Context] = None) -> Tensor:
+    """matricizedspGEMM_sorting.hpp:194 -- same product."""
+    ctx = ctx or default_context()
+    dA, dB = _as_device(A, ctx), _as_device(B, ctx)
+    out = C.c_void_p()
+    _check(lib().ptc_matricize_sort_spGEMM(ctx.handle, dA._h, dB._h, C.byref(out), None), ctx)
+    return DeviceTensor(ctx, out).download()
+
+
+# --------------------------------------------------------- synthetic inputs
+
+
+def gen_coo(seed: int, dims, nnz: int, dist: int = DIST_ZIPF, alpha: float = 1.1):
+    """Seeded unique-coordinate COO (SplitMix64/xoshiro256**, truncated Zipf
+    by inverse CDF or uniform; values U(0,1])."""
+    dims_a = np.ascontiguousarray(dims, dtype=np.uint32)
+    d = dims_a.shape[0]
+    h = np.empty((nnz, d), dtype=np.uint32)
+    v = np.empty(nnz, dtype=np.float64)
+    rc = lib().ptc_gen_coo(seed, d, _p32(dims_a), nnz, dist, alpha, _p32(h), _pd(v))
+    if rc != PTC_OK:
+        raise PtcError(rc, "ptc_gen_coo")
+    return h, v
+
+
+def gen_job_shape(seed: int, dim_lo=64, dim_hi=20000, nnz_lo=1000, nnz_hi=1000000):
+    d = C.c_uint32()
+    dims = np.zeros(MAX_MODES, dtype=np.uint32)
+    nnz = C.c_uint64()
+    rc = lib().ptc_gen_job_shape(seed, dim_lo, dim_hi, nnz_lo, nnz_hi, C.byref(d), _p32(dims), C.byref(nnz))
+    if rc != PTC_OK:
+        raise PtcError(rc, "ptc_gen_job_shape")
+    return int(d.value), dims[: d.value].copy(), int(nnz.value)
+
+
+def self_contraction_inputs(h: np.ndarray, v: np.ndarray, contracted):
+    """(A_copy, B) of main.cpp's self-contraction: the .meta permutation has
+    the contracted modes first (4d.meta), A_copy rotates it to free-first,
+    B is A (sparse_tensor_generator.hpp:4-88)."""
+    d = h.shape[1]
+    contracted = [int(m) for m in contracted]
+    free = [m for m in range(d) if m not in contracted]
+    perm = np.array(contracted + free, dtype=np.uint32)
+    A = Tensor(h, v, None, perm, len(contracted))
+    return make_free_first(A), get_sparse_tensor(A)

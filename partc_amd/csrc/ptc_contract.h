@@ -1,0 +1,148 @@
+// Kernel parameter blocks and launchers of the contraction passes.
+#ifndef PTC_CONTRACT_H
+#define PTC_CONTRACT_H
+
+#include "ptc_internal.h"
+
+namespace ptc {
+
+enum { PASS_SYMBOLIC = 0, PASS_NUMERIC = 1 };
+
+struct ProbeParams {
+  uint64_t npos;
+  const void* akey;               // HT_A rest keys (A's contracted tuple)
+  const uint32_t* row_of_pos;     // HT_A slice_of
+  KeyCodec acodec, bcodec;        // A's contracted codec, B's key codec
+  const KtEntry* tab;             // HT_B entries
+  uint64_t nbuckets;              // HT_B cap / 8
+  uint32_t* pos_len;
+  uint32_t* pos_off;
+  uint32_t* row_hits;
+  uint32_t* row_maxlen;
+};
+
+struct RowParams {
+  uint64_t nrows;
+  const KtSlice* rows;
+  const unsigned long long* apre;
+  unsigned long long chunk;
+  uint32_t* row_nchunk;
+  unsigned long long* row_flops;
+  unsigned long long* packed;     // (heavy << 32) | nchunk, scanned later
+};
+
+struct ItemParams {
+  uint64_t nrows;
+  const KtSlice* rows;
+  const uint32_t* row_nchunk;
+  const unsigned long long* packed;  // scanned
+  uint32_t* row_heavy;
+  uint4* items;
+};
+
+struct ItemP0Params {
+  uint64_t npos;
+  const uint32_t* row_of_pos;
+  const uint32_t* row_nchunk;
+  const uint32_t* pos_len;
+  const KtSlice* rows;
+  const unsigned long long* apre;
+  const unsigned long long* row_flops;
+  const unsigned long long* packed;
+  uint4* items;
+};
+
+struct HashCapParams {
+  uint64_t nrows;
+  const uint32_t* bound32;            // one of the two
+  const unsigned long long* bound64;
+  unsigned long long cap_col;         // clamp bound to the column space (0: none)
+  const uint32_t* row_nchunk;
+  uint32_t lds_slots;
+  uint32_t* row_hcap;
+  unsigned long long* gcap;           // scanned into offsets later
+};
+
+struct CapParams {
+  uint64_t nrows;
+  const unsigned long long* row_flops;
+  const uint32_t* row_maxlen;
+  unsigned long long ncols;
+  const float* pred;                  // NULL: upper bound
+  double scale;
+  uint32_t* cap;
+};
+
+struct PassParams {
+  // work
+  const uint4* items;
+  const KtSlice* rows;
+  const uint32_t* row_nchunk;
+  const uint32_t* row_heavy;
+  uint32_t* row_done;
+  // positions (A, row-grouped)
+  const unsigned long long* apre;
+  const uint32_t* pos_off;
+  const double* aval;
+  // B slices
+  const void* bcol;
+  const double* bval;
+  uint64_t ncols;                     // column space (dense accumulators)
+  // heavy dense rows
+  uint32_t* hbits;
+  double* hacc;
+  uint32_t* hflag;
+  // hash accumulators
+  const uint32_t* row_hcap;
+  const unsigned long long* row_gcap;
+  const unsigned long long* row_goff;
+  void* gkeys;
+  double* gvals;
+  // results
+  uint32_t* row_nnz;
+  const unsigned long long* out_base;
+  const uint32_t* out_cap;            // capacity mode when set
+  uint32_t* overflow;
+  void* stg_col;                      // capacity mode staging (col key)
+  double* stg_val;
+  uint32_t* out_hedges;               // exact mode: final output
+  double* out_vals;
+  uint32_t dO, nrowmodes;
+  KeyCodec rowc, colc;
+};
+
+struct EmitParams {
+  uint64_t nrows;
+  const KtSlice* rows;
+  const uint32_t* row_nnz;
+  const uint32_t* cap;
+  const unsigned long long* cap_base;
+  const unsigned long long* out_base;
+  const void* stg_col;
+  const double* stg_val;
+  uint32_t* out_hedges;
+  double* out_vals;
+  uint32_t dO, nrowmodes;
+  KeyCodec rowc, colc;
+};
+
+int launch_probe(ptc_ctx* ctx, const ProbeParams& P, bool rest64);
+int launch_rows(ptc_ctx* ctx, const RowParams& P);
+int launch_items(ptc_ctx* ctx, const ItemParams& P, const ItemP0Params& Q);
+int launch_hash_caps(ptc_ctx* ctx, const HashCapParams& P);
+int launch_caps(ptc_ctx* ctx, const CapParams& P);
+size_t pass_lds_bytes(int pass, bool dense, bool col64, uint64_t ncols, uint32_t lds_slots);
+int launch_pass(ptc_ctx* ctx, int pass, bool dense, bool col64, const PassParams& P, uint64_t nitems,
+                size_t lds);
+int launch_emit(ptc_ctx* ctx, const EmitParams& P, bool col64);
+int launch_overflow_rows(ptc_ctx* ctx, const uint32_t* nnz, const uint32_t* cap, uint64_t nrows,
+                         uint32_t* list, uint32_t* n);
+int launch_select_items(ptc_ctx* ctx, const uint4* items, uint64_t nitems, const uint32_t* nnz,
+                        const uint32_t* cap, uint4* out, uint32_t* n);
+int launch_reset_rows(ptc_ctx* ctx, const uint32_t* list, uint32_t n, uint32_t* row_done);
+int fill_u64(ptc_ctx* ctx, unsigned long long* p, uint64_t n, unsigned long long v);
+int fill_u32(ptc_ctx* ctx, uint32_t* p, uint64_t n, uint32_t v);
+
+}  // namespace ptc
+
+#endif

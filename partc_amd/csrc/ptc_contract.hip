@@ -1,0 +1,746 @@
+// The contraction O = A x B over the contracted modes (the role of
+// tc_fksCuckoo, fksCuckooTC.hpp:80-371).
+//
+// Data on the device (all in HBM, grouped by the two key tables):
+//   rows      HT_A slices: one per distinct A free tuple (an output row),
+//             with the offset/length of its A positions in grouped order.
+//   positions A nonzeros in row-grouped order: HT_A->rest_key (the
+//             contracted tuple) and HT_A->gval (the value).
+//   slices    HT_B: every B nonzero in key-grouped order with its free tuple
+//             linearised (the column key) and its value.
+//
+// Kernels:
+//   k_probe      one thread per A position: re-encode the contracted tuple
+//                in B's codec, probe B's table -> (slice offset, length);
+//                per-row hit count and longest slice by a wave-segmented
+//                reduction (the reference's upper-bound preprocessing,
+//                fksCuckooTC.hpp:117-154, plus the learned features).
+//   scan         exclusive scan of slice lengths: every multiply-add of the
+//                contraction gets a global index ("flop" index); a row's
+//                flops are a contiguous range.
+//   k_rows       per row: flops, work chunks, hash capacities.
+//   k_item_p0    start position of each chunk of a heavy row.
+//   k_pass       one workgroup per work item (a row, or a chunk of a heavy
+//                row): stages its A positions in LDS, expands the flat flop
+//                range (coalesced reads of B's column keys and values), and
+//                accumulates per column in LDS -- a direct-mapped array when
+//                the column space fits (a perfect hash), otherwise an
+//                open-addressing LDS hash table -- or, for rows too large for
+//                LDS, in an HBM hash table.  Heavy rows split over several
+//                workgroups merge their partials in HBM; the last workgroup
+//                of the row finishes it.  SYMBOLIC counts distinct columns
+//                per row; NUMERIC accumulates values and writes the row.
+//   k_emit       learned/upper-bound methods: copy rows from their
+//                capacity-sized staging area to the prefix-scanned output.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "ptc_contract.h"
+#include "ptc_internal.h"
+
+namespace ptc {
+
+constexpr int PB = 256;          // threads per workgroup of the row passes
+constexpr int STAGE = 512;       // A positions staged in LDS at a time
+constexpr uint32_t NO_HEAVY = 0xFFFFFFFFu;
+
+__device__ __forceinline__ unsigned long long mix64d(unsigned long long k) {
+  k ^= k >> 33;
+  k *= 0xff51afd7ed558ccdull;
+  k ^= k >> 33;
+  k *= 0xc4ceb9fe1a85ec53ull;
+  k ^= k >> 33;
+  return k;
+}
+
+// ----------------------------------------------------------------- probe
+
+template <class RestT>
+__global__ void __launch_bounds__(256) k_probe(ProbeParams P) {
+  const RestT* __restrict__ akey = (const RestT*)P.akey;
+  const int lane = threadIdx.x & 63;
+  for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < P.npos; base += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t p = base + threadIdx.x;
+    bool act = p < P.npos;
+    uint32_t len = 0, off = 0;
+    uint32_t row = 0xFFFFFFFFu;
+    if (act) {
+      row = P.row_of_pos[p];
+      unsigned long long ka = (unsigned long long)akey[p];
+      // decode A's contracted tuple and re-encode it in B's key codec
+      unsigned long long kb = 0;
+      bool ok = true;
+      for (uint32_t j = 0; j < P.acodec.n; ++j) {
+        uint32_t x = (uint32_t)((ka / P.acodec.stride[j]) % P.acodec.radix[j]);
+        if (x >= P.bcodec.radix[j]) ok = false;
+        kb += (unsigned long long)x * P.bcodec.stride[j];
+      }
+      if (ok) {
+        uint64_t b = mix64d(kb) & (P.nbuckets - 1);
+        while (true) {
+          bool done = false;
+#pragma unroll
+          for (int s = 0; s < 8; ++s) {
+            KtEntry en = P.tab[b * 8 + s];
+            if (en.key == kb) {
+              len = en.len;
+              off = en.off;
+              done = true;
+              break;
+            }
+            if (en.key == PTC_EMPTY_KEY) {
+              done = true;
+              break;
+            }
+          }
+          if (done) break;
+          b = (b + 1) & (P.nbuckets - 1);
+        }
+      }
+      P.pos_len[p] = len;
+      P.pos_off[p] = off;
+    }
+    // wave-segmented reduction over rows (positions are row-grouped)
+    uint32_t hits = (act && len) ? 1u : 0u;
+    uint32_t mx = len;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      uint32_t h2 = __shfl_down(hits, o, 64);
+      uint32_t m2 = __shfl_down(mx, o, 64);
+      uint32_t r2 = __shfl_down(row, o, 64);
+      if (lane + o < 64 && r2 == row) {
+        hits += h2;
+        mx = max(mx, m2);
+      }
+    }
+    uint32_t prev = __shfl_up(row, 1, 64);
+    bool head = act && (lane == 0 || prev != row);
+    if (head) {
+      if (hits) atomicAdd(&P.row_hits[row], hits);
+      if (mx) atomicMax(&P.row_maxlen[row], mx);
+    }
+  }
+}
+
+// ------------------------------------------------------------------ rows
+
+__global__ void k_rows(RowParams P) {
+  for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < P.nrows; r += (uint64_t)gridDim.x * blockDim.x) {
+    KtSlice s = P.rows[r];
+    unsigned long long f = P.apre[(uint64_t)s.off + s.len] - P.apre[s.off];
+    uint32_t nch = f ? (uint32_t)((f + P.chunk - 1) / P.chunk) : 0u;
+    P.row_nchunk[r] = nch;
+    P.row_flops[r] = f;
+    P.packed[r] = ((unsigned long long)(nch > 1) << 32) | nch;
+  }
+}
+
+// fills items: (row, chunk, first position, heavy index)
+__global__ void k_items(ItemParams P) {
+  for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < P.nrows; r += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t nch = P.row_nchunk[r];
+    if (!nch) {
+      P.row_heavy[r] = NO_HEAVY;
+      continue;
+    }
+    unsigned long long pk = P.packed[r];
+    uint64_t io = pk & 0xFFFFFFFFull;
+    uint32_t hv = (uint32_t)(pk >> 32);
+    P.row_heavy[r] = nch > 1 ? hv : NO_HEAVY;
+    uint32_t p0 = P.rows[r].off;
+    for (uint32_t k = 0; k < nch; ++k) P.items[io + k] = make_uint4((uint32_t)r, k, p0, 0u);
+  }
+}
+
+// start position of every chunk k > 0 of a heavy row: the position whose
+// flop range contains the chunk's first flop
+__global__ void k_item_p0(ItemP0Params P) {
+  for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < P.npos; p += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t r = P.row_of_pos[p];
+    uint32_t nch = P.row_nchunk[r];
+    if (nch <= 1) continue;
+    uint32_t len = P.pos_len[p];
+    if (!len) continue;
+    KtSlice s = P.rows[r];
+    unsigned long long flo = P.apre[s.off];
+    unsigned long long f = P.row_flops[r];
+    unsigned long long per = (f + nch - 1) / nch;
+    unsigned long long a = P.apre[p] - flo, bnd = a + len;  // [a, bnd) relative to row
+    // chunks k with k*per in [a, bnd), k >= 1
+    unsigned long long k0 = (a + per - 1) / per;
+    if (k0 == 0) k0 = 1;
+    uint64_t io = P.packed[r] & 0xFFFFFFFFull;
+    for (unsigned long long k = k0; k < nch && k * per < bnd; ++k) P.items[io + k].z = (uint32_t)p;
+  }
+}
+
+// hash capacities: pow2 >= 2*bound (min 16); rows that must live in HBM
+// (several chunks, or too big for the LDS table) get a global capacity
+__global__ void k_hash_caps(HashCapParams P) {
+  for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < P.nrows; r += (uint64_t)gridDim.x * blockDim.x) {
+    unsigned long long bound = P.bound32 ? P.bound32[r] : P.bound64[r];
+    if (P.cap_col && bound > P.cap_col) bound = P.cap_col;
+    unsigned long long c = 16;
+    while (c < 2 * bound) c <<= 1;
+    uint32_t nch = P.row_nchunk[r];
+    bool global = nch > 1 || c > P.lds_slots;
+    P.row_hcap[r] = (uint32_t)std::min<unsigned long long>(c, 0xFFFFFFFFull);
+    P.gcap[r] = (nch && global) ? c : 0ull;
+  }
+}
+
+// -------------------------------------------------------------- row pass
+
+template <class ColT>
+struct ColEmpty {
+  static constexpr ColT v = (ColT)~(ColT)0;
+};
+
+__device__ __forceinline__ void decode_tuple(unsigned long long key, const KeyCodec& c, uint32_t* out) {
+  if (c.n == 1) {
+    out[0] = (uint32_t)key;
+    return;
+  }
+  for (uint32_t j = 0; j < c.n; ++j) out[j] = (uint32_t)((key / c.stride[j]) % c.radix[j]);
+}
+
+// Block-wide exclusive prefix of a 0/1 flag per thread; returns the rank and
+// the block total.
+__device__ __forceinline__ uint32_t block_rank(bool f, uint32_t* s_w, uint32_t* total) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  unsigned long long m = __ballot(f);
+  uint32_t below = __popcll(m & ((1ull << lane) - 1ull));
+  if (lane == 0) s_w[wid] = __popcll(m);
+  __syncthreads();
+  uint32_t base = 0, all = 0;
+#pragma unroll
+  for (int w = 0; w < PB / 64; ++w) {
+    uint32_t c = s_w[w];
+    if (w < wid) base += c;
+    all += c;
+  }
+  __syncthreads();
+  *total = all;
+  return base + below;
+}
+
+// write one output entry at position pos
+template <class ColT>
+__device__ __forceinline__ void emit_entry(const PassParams& P, uint64_t pos, const uint32_t* rowc,
+                                           ColT col, double v) {
+  if (P.stg_col) {
+    ((ColT*)P.stg_col)[pos] = col;
+    P.stg_val[pos] = v;
+    return;
+  }
+  uint32_t* o = P.out_hedges + pos * P.dO;
+  for (uint32_t j = 0; j < P.nrowmodes; ++j) o[j] = rowc[j];
+  uint32_t cc[PTC_MAX_MODES];
+  decode_tuple((unsigned long long)col, P.colc, cc);
+  for (uint32_t j = 0; j < P.colc.n; ++j) o[P.nrowmodes + j] = cc[j];
+  P.out_vals[pos] = v;
+}
+
+// Insert-or-find col in an open-addressing LDS table of S slots (power of
+// two).  Returns the slot, or -1 when all S slots were probed (table full).
+// cnt (optional) counts keys this call inserted.
+template <class ColT>
+__device__ __forceinline__ int lds_insert(ColT* keys, uint32_t S, ColT col, uint32_t* cnt) {
+  uint32_t h = (uint32_t)mix64d((unsigned long long)col) & (S - 1);
+  for (uint32_t n = 0; n < S; ++n) {
+    ColT kk = keys[h];
+    if (kk == col) return (int)h;
+    if (kk == ColEmpty<ColT>::v) {
+      ColT old = atomicCAS(&keys[h], ColEmpty<ColT>::v, col);
+      if (old == ColEmpty<ColT>::v) {
+        if (cnt) atomicAdd(cnt, 1u);
+        return (int)h;
+      }
+      if (old == col) return (int)h;
+    }
+    h = (h + 1) & (S - 1);
+  }
+  return -1;
+}
+
+// The same in an HBM table of mask+1 slots.
+template <class ColT>
+__device__ __forceinline__ long long hbm_insert(ColT* keys, uint32_t mask, ColT col, uint32_t* cnt) {
+  uint32_t h = (uint32_t)mix64d((unsigned long long)col) & mask;
+  for (uint64_t n = 0; n <= (uint64_t)mask; ++n) {
+    ColT kk = __hip_atomic_load(&keys[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (kk == col) return (long long)h;
+    if (kk == ColEmpty<ColT>::v) {
+      ColT old = atomicCAS(&keys[h], ColEmpty<ColT>::v, col);
+      if (old == ColEmpty<ColT>::v) {
+        if (cnt) atomicAdd(cnt, 1u);
+        return (long long)h;
+      }
+      if (old == col) return (long long)h;
+    }
+    h = (h + 1) & mask;
+  }
+  return -1;
+}
+
+// LDS layout of one row pass (dynamic shared memory):
+//   stage:  s_start[STAGE] u32, s_bias[STAGE] i64, s_aval[STAGE] f64
+//   dense:  acc[ncols] f64, flag[ncols] u8        (numeric)
+//           bits[ceil(ncols/32)] u32             (symbolic)
+//   hash:   keys[S] ColT, vals[S] f64             (S = lds_slots)
+template <int PASS, bool DENSE, class ColT>
+__global__ void __launch_bounds__(PB) k_pass(PassParams P) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  uint32_t* s_start = (uint32_t*)smem;
+  long long* s_bias = (long long*)(smem + STAGE * 4);
+  double* s_aval = (double*)(smem + STAGE * 12);
+  unsigned char* acc_base = smem + STAGE * 20;
+  __shared__ uint32_t s_w[PB / 64];
+  __shared__ uint32_t s_misc[4];
+  __shared__ uint32_t s_rowc[PTC_MAX_MODES];
+
+  const ColT* __restrict__ bcol = (const ColT*)P.bcol;
+  const uint4 it = P.items[blockIdx.x];
+  const uint32_t r = it.x, k = it.y;
+  const KtSlice row = P.rows[r];
+  const uint32_t nch = P.row_nchunk[r];
+  const unsigned long long flo = P.apre[row.off];
+  const unsigned long long fr = P.apre[(uint64_t)row.off + row.len] - flo;
+  const unsigned long long per = (fr + nch - 1) / nch;
+  const unsigned long long g0 = flo + (unsigned long long)k * per;
+  const unsigned long long g1 = min(flo + fr, g0 + per);
+  const uint32_t heavy = P.row_heavy[r];
+  const uint64_t pend = (uint64_t)row.off + row.len;
+  const uint64_t ncols = P.ncols;
+
+  // accumulator storage
+  double* acc = (double*)acc_base;
+  unsigned char* flag = acc_base + ncols * 8;
+  uint32_t* bits = (uint32_t*)acc_base;
+  ColT* hkeys = (ColT*)acc_base;
+  uint32_t S = 0;  // LDS hash slots for this row (0: HBM table)
+  ColT* gkeys = nullptr;
+  double* gvals = nullptr;
+  uint32_t gmask = 0;
+  const uint32_t nbits = (uint32_t)((ncols + 31) / 32);
+
+  if (DENSE) {
+    if (PASS == PASS_SYMBOLIC) {
+      for (uint32_t w = threadIdx.x; w < nbits; w += PB) bits[w] = 0;
+    } else {
+      for (uint64_t c = threadIdx.x; c < ncols; c += PB) {
+        acc[c] = 0.0;
+        flag[c] = 0;
+      }
+    }
+  } else {
+    uint32_t hc = P.row_hcap[r];
+    unsigned long long goff = P.row_goff[r];
+    if (P.row_gcap[r] == 0) {
+      S = hc;
+      for (uint32_t i = threadIdx.x; i < S; i += PB) hkeys[i] = ColEmpty<ColT>::v;
+      if (PASS == PASS_NUMERIC) {
+        double* hv = (double*)(acc_base + (size_t)S * sizeof(ColT));
+        for (uint32_t i = threadIdx.x; i < S; i += PB) hv[i] = 0.0;
+      }
+    } else {
+      gkeys = (ColT*)P.gkeys + goff;
+      gvals = P.gvals + goff;
+      gmask = hc - 1;
+    }
+  }
+  if (threadIdx.x < 4) s_misc[threadIdx.x] = 0;  // [0] LDS distinct count, [3] table full
+  __syncthreads();
+  double* hvals = (double*)(acc_base + (size_t)S * sizeof(ColT));
+
+  // ---- expand the flop range [g0, g1) ----
+  uint64_t q0 = it.z;
+  while (q0 < pend) {
+    const unsigned long long qstart = P.apre[q0];
+    if (qstart >= g1) break;
+    const uint32_t ns = (uint32_t)min<uint64_t>(STAGE, pend - q0);
+    for (uint32_t i = threadIdx.x; i < ns; i += PB) {
+      uint64_t q = q0 + i;
+      unsigned long long a = P.apre[q];
+      s_start[i] = (uint32_t)((a > g0 ? a : g0) - g0);
+      s_bias[i] = (long long)P.pos_off[q] + (long long)g0 - (long long)a;
+      if (PASS == PASS_NUMERIC) s_aval[i] = P.aval[q];
+    }
+    const unsigned long long qend = (q0 + ns < pend) ? P.apre[q0 + ns] : (flo + fr);
+    const uint32_t fb = (uint32_t)((qstart > g0 ? qstart : g0) - g0);
+    const uint32_t fe = (uint32_t)(min(qend, g1) - g0);
+    __syncthreads();
+    for (uint32_t f = fb + threadIdx.x; f < fe; f += PB) {
+      // last staged position whose start <= f
+      uint32_t lo = 0, hi = ns - 1;
+      while (lo < hi) {
+        uint32_t mid = (lo + hi + 1) >> 1;
+        if (s_start[mid] <= f) lo = mid; else hi = mid - 1;
+      }
+      const long long b = s_bias[lo] + (long long)f;
+      const ColT col = bcol[b];
+      if (DENSE) {
+        if (PASS == PASS_SYMBOLIC) {
+          atomicOr(&bits[col >> 5], 1u << (col & 31));
+        } else {
+          const double v = s_aval[lo] * P.bval[b];
+          atomicAdd(&acc[col], v);
+          flag[col] = 1;
+        }
+      } else {
+        double v = 0.0;
+        if (PASS == PASS_NUMERIC) v = s_aval[lo] * P.bval[b];
+        if (S) {
+          const int h = lds_insert<ColT>(hkeys, S, col, PASS == PASS_SYMBOLIC ? &s_misc[0] : nullptr);
+          if (h < 0) s_misc[3] = 1u;
+          else if (PASS == PASS_NUMERIC) atomicAdd(&hvals[h], v);
+        } else {
+          const long long h = hbm_insert<ColT>(gkeys, gmask, col, PASS == PASS_SYMBOLIC ? &P.row_nnz[r] : nullptr);
+          if (h < 0) atomicOr(P.overflow, 0x80000000u);  // cannot happen: sized from exact or bound counts
+          else if (PASS == PASS_NUMERIC) unsafeAtomicAdd(&gvals[h], v);
+        }
+      }
+    }
+    __syncthreads();
+    q0 += ns;
+  }
+
+  // ---- merge partials of a split row in HBM; the last chunk finishes ----
+  bool finisher = true;
+  const bool in_hbm = !DENSE && S == 0;
+  if (nch > 1 || in_hbm) {
+    if (DENSE && heavy != NO_HEAVY) {
+      if (PASS == PASS_SYMBOLIC) {
+        uint32_t* gb = P.hbits + (uint64_t)heavy * nbits;
+        for (uint32_t w = threadIdx.x; w < nbits; w += PB)
+          if (bits[w]) atomicOr(&gb[w], bits[w]);
+      } else {
+        double* ga = P.hacc + (uint64_t)heavy * ncols;
+        uint32_t* gf = P.hflag + (uint64_t)heavy * ncols;
+        for (uint64_t c = threadIdx.x; c < ncols; c += PB)
+          if (flag[c]) {
+            unsafeAtomicAdd(&ga[c], acc[c]);
+            __hip_atomic_store(&gf[c], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          }
+      }
+    }
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t done = atomicAdd(&P.row_done[r], 1u);
+      s_misc[1] = (done == nch - 1) ? 1u : 0u;
+    }
+    __syncthreads();
+    finisher = s_misc[1] != 0;
+    if (!finisher) return;
+    __threadfence();
+    if (DENSE) {
+      // pull the merged row back into LDS
+      if (PASS == PASS_SYMBOLIC) {
+        const uint32_t* gb = P.hbits + (uint64_t)heavy * nbits;
+        for (uint32_t w = threadIdx.x; w < nbits; w += PB)
+          bits[w] = __hip_atomic_load(&gb[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      } else {
+        const double* ga = P.hacc + (uint64_t)heavy * ncols;
+        const uint32_t* gf = P.hflag + (uint64_t)heavy * ncols;
+        for (uint64_t c = threadIdx.x; c < ncols; c += PB) {
+          flag[c] = (unsigned char)__hip_atomic_load(&gf[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          acc[c] = __hip_atomic_load(&ga[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+      }
+      __syncthreads();
+    }
+  }
+
+  // ---- finish the row ----
+  if (PASS == PASS_SYMBOLIC) {
+    if (DENSE) {
+      uint32_t c = 0;
+      for (uint32_t w = threadIdx.x; w < nbits; w += PB) c += __popc(bits[w]);
+      for (int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o, 64);
+      if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = c;
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (int w = 0; w < PB / 64; ++w) t += s_w[w];
+        P.row_nnz[r] = t;
+      }
+    } else if (S) {
+      if (threadIdx.x == 0) P.row_nnz[r] = s_misc[0];
+    }
+    // HBM hash rows counted themselves with atomics
+    return;
+  }
+
+  // NUMERIC: count, check capacity, write
+  if (threadIdx.x < P.nrowmodes) {
+    uint32_t rc[PTC_MAX_MODES];
+    decode_tuple(row.key, P.rowc, rc);
+    s_rowc[threadIdx.x] = rc[threadIdx.x];
+  }
+  uint32_t rowc[PTC_MAX_MODES];
+  __syncthreads();
+  for (uint32_t j = 0; j < P.nrowmodes; ++j) rowc[j] = s_rowc[j];
+
+  const uint64_t span = DENSE ? ncols : (S ? S : (uint64_t)gmask + 1);
+  // count
+  uint32_t cnt = 0;
+  for (uint64_t c = threadIdx.x; c < span; c += PB) {
+    bool f;
+    if (DENSE) f = flag[c] != 0;
+    else if (S) f = hkeys[c] != ColEmpty<ColT>::v;
+    else f = __hip_atomic_load(&gkeys[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != ColEmpty<ColT>::v;
+    cnt += f ? 1u : 0u;
+  }
+  for (int o = 32; o > 0; o >>= 1) cnt += __shfl_down(cnt, o, 64);
+  if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = cnt;
+  __syncthreads();
+  uint32_t total = 0;
+  for (int w = 0; w < PB / 64; ++w) total += s_w[w];
+  __syncthreads();
+  uint32_t cap = P.out_cap ? P.out_cap[r] : 0xFFFFFFFFu;
+  const bool full = s_misc[3] != 0;  // LDS table filled up: count unknown
+  if (threadIdx.x == 0) {
+    if (P.out_cap) {
+      // capacity mode: record the count; a row over its capacity (or whose
+      // table filled) is redone by the exact two-pass path
+      P.row_nnz[r] = full ? 0xFFFFFFFFu : total;
+      if (full || total > cap) atomicAdd(P.overflow, 1u);
+    } else if (full || P.row_nnz[r] != total) {
+      atomicOr(P.overflow, 0x80000000u);  // exact mode: a mismatch is an internal error
+    }
+  }
+  if (full || total > cap) return;
+  const uint64_t base = P.out_base[r];
+  uint32_t run = 0;
+  for (uint64_t c0 = 0; c0 < span; c0 += PB) {
+    uint64_t c = c0 + threadIdx.x;
+    bool f = false;
+    ColT col = 0;
+    double v = 0.0;
+    if (c < span) {
+      if (DENSE) {
+        f = flag[c] != 0;
+        col = (ColT)c;
+        v = acc[c];
+      } else if (S) {
+        col = hkeys[c];
+        f = col != ColEmpty<ColT>::v;
+        v = hvals[c];
+      } else {
+        col = __hip_atomic_load(&gkeys[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        f = col != ColEmpty<ColT>::v;
+        if (f) v = __hip_atomic_load(&gvals[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+    uint32_t tile_total;
+    uint32_t rank = block_rank(f, s_w, &tile_total);
+    if (f && run + rank < total) emit_entry<ColT>(P, base + run + rank, rowc, col, v);
+    run += tile_total;
+  }
+}
+
+// learned / upper-bound: staged rows -> final output
+template <class ColT>
+__global__ void __launch_bounds__(PB) k_emit(EmitParams P) {
+  const uint32_t r = blockIdx.x;
+  if (r >= P.nrows) return;
+  const uint32_t n = P.row_nnz[r];
+  if (!n || n > P.cap[r]) return;  // overflow rows are redone exactly
+  __shared__ uint32_t s_rowc[PTC_MAX_MODES];
+  if (threadIdx.x < P.nrowmodes) {
+    uint32_t rc[PTC_MAX_MODES];
+    decode_tuple(P.rows[r].key, P.rowc, rc);
+    s_rowc[threadIdx.x] = rc[threadIdx.x];
+  }
+  __syncthreads();
+  uint32_t rowc[PTC_MAX_MODES];
+  for (uint32_t j = 0; j < P.nrowmodes; ++j) rowc[j] = s_rowc[j];
+  const uint64_t src = P.cap_base[r], dst = P.out_base[r];
+  const ColT* sc = (const ColT*)P.stg_col;
+  for (uint32_t i = threadIdx.x; i < n; i += PB) {
+    ColT col = sc[src + i];
+    uint32_t* o = P.out_hedges + (dst + i) * P.dO;
+    for (uint32_t j = 0; j < P.nrowmodes; ++j) o[j] = rowc[j];
+    uint32_t cc[PTC_MAX_MODES];
+    decode_tuple((unsigned long long)col, P.colc, cc);
+    for (uint32_t j = 0; j < P.colc.n; ++j) o[P.nrowmodes + j] = cc[j];
+    P.out_vals[dst + i] = P.stg_val[src + i];
+  }
+}
+
+// ------------------------------------------------------------- launchers
+
+static unsigned gridn(uint64_t n, unsigned t, unsigned cap = 16384) {
+  uint64_t g = (n + t - 1) / t;
+  return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(g, cap));
+}
+
+int launch_probe(ptc_ctx* ctx, const ProbeParams& P, bool rest64) {
+  if (!P.npos) return PTC_OK;
+  if (rest64)
+    hipLaunchKernelGGL((k_probe<unsigned long long>), dim3(gridn(P.npos, 256)), dim3(256), 0, ctx->stream, P);
+  else
+    hipLaunchKernelGGL((k_probe<uint32_t>), dim3(gridn(P.npos, 256)), dim3(256), 0, ctx->stream, P);
+  PTC_HIP(ctx, hipGetLastError());
+  return PTC_OK;
+}
+
+int launch_rows(ptc_ctx* ctx, const RowParams& P) {
+  if (!P.nrows) return PTC_OK;
+  hipLaunchKernelGGL(k_rows, dim3(gridn(P.nrows, 256)), dim3(256), 0, ctx->stream, P);
+  PTC_HIP(ctx, hipGetLastError());
+  return PTC_OK;
+}
+
+int launch_items(ptc_ctx* ctx, const ItemParams& P, const ItemP0Params& Q) {
+  if (P.nrows) hipLaunchKernelGGL(k_items, dim3(gridn(P.nrows, 256)), dim3(256), 0, ctx->stream, P);
+  if (Q.npos) hipLaunchKernelGGL(k_item_p0, dim3(gridn(Q.npos, 256)), dim3(256), 0, ctx->stream, Q);
+  PTC_HIP(ctx, hipGetLastError());
+  return PTC_OK;
+}
+
+int launch_hash_caps(ptc_ctx* ctx, const HashCapParams& P) {
+  if (!P.nrows) return PTC_OK;
+  hipLaunchKernelGGL(k_hash_caps, dim3(gridn(P.nrows, 256)), dim3(256), 0, ctx->stream, P);
+  PTC_HIP(ctx, hipGetLastError());
+  return PTC_OK;
+}
+
+size_t pass_lds_bytes(int pass, bool dense, bool col64, uint64_t ncols, uint32_t lds_slots) {
+  size_t b = (size_t)STAGE * 20;
+  if (dense) {
+    if (pass == PASS_SYMBOLIC) b += ((ncols + 31) / 32) * 4;
+    else b += ncols * 9;
+  } else {
+    b += (size_t)lds_slots * ((col64 ? 8 : 4) + (pass == PASS_NUMERIC ? 8 : 0));
+  }
+  return (b + 15) & ~size_t(15);
+}
+
+template <int PASS, bool DENSE, class ColT>
+static int launch_pass_t(ptc_ctx* ctx, const PassParams& P, uint64_t nitems, size_t lds) {
+  auto kern = k_pass<PASS, DENSE, ColT>;
+  PTC_HIP(ctx, hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  hipLaunchKernelGGL(kern, dim3((unsigned)nitems), dim3(PB), lds, ctx->stream, P);
+  PTC_HIP(ctx, hipGetLastError());
+  return PTC_OK;
+}
+
+int launch_pass(ptc_ctx* ctx, int pass, bool dense, bool col64, const PassParams& P, uint64_t nitems,
+                size_t lds) {
+  if (!nitems) return PTC_OK;
+  if (nitems > 0x7FFFFFFFull) return set_err(ctx, PTC_ERR_RANGE, "too many work items");
+  if (pass == PASS_SYMBOLIC) {
+    if (dense) return col64 ? launch_pass_t<PASS_SYMBOLIC, true, unsigned long long>(ctx, P, nitems, lds)
+                            : launch_pass_t<PASS_SYMBOLIC, true, uint32_t>(ctx, P, nitems, lds);
+    return col64 ? launch_pass_t<PASS_SYMBOLIC, false, unsigned long long>(ctx, P, nitems, lds)
+                 : launch_pass_t<PASS_SYMBOLIC, false, uint32_t>(ctx, P, nitems, lds);
+  }
+  if (dense) return col64 ? launch_pass_t<PASS_NUMERIC, true, unsigned long long>(ctx, P, nitems, lds)
+                          : launch_pass_t<PASS_NUMERIC, true, uint32_t>(ctx, P, nitems, lds);
+  return col64 ? launch_pass_t<PASS_NUMERIC, false, unsigned long long>(ctx, P, nitems, lds)
+               : launch_pass_t<PASS_NUMERIC, false, uint32_t>(ctx, P, nitems, lds);
+}
+
+int launch_emit(ptc_ctx* ctx, const EmitParams& P, bool col64) {
+  if (!P.nrows) return PTC_OK;
+  if (col64)
+    hipLaunchKernelGGL((k_emit<unsigned long long>), dim3((unsigned)P.nrows), dim3(PB), 0, ctx->stream, P);
+  else
+    hipLaunchKernelGGL((k_emit<uint32_t>), dim3((unsigned)P.nrows), dim3(PB), 0, ctx->stream, P);
+  PTC_HIP(ctx, hipGetLastError());
+  return PTC_OK;
+}
+
+__global__ void k_fill_u64(unsigned long long* p, uint64_t n, unsigned long long v) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+int fill_u64(ptc_ctx* ctx, unsigned long long* p, uint64_t n, unsigned long long v) {
+  if (!n) return PTC_OK;
+  hipLaunchKernelGGL(k_fill_u64, dim3(gridn(n, 256, 4096)), dim3(256), 0, ctx->stream, p, n, v);
+  PTC_HIP(ctx, hipGetLastError());
+  return PTC_OK;
+}
+int fill_u32(ptc_ctx* ctx, uint32_t* p, uint64_t n, uint32_t v) {
+  if (!n) return PTC_OK;
+  hipLaunchKernelGGL(k_fill_u32, dim3(gridn(n, 256, 4096)), dim3(256), 0, ctx->stream, p, n, v);
+  PTC_HIP(ctx, hipGetLastError());
+  return PTC_OK;
+}
+
+// capacity of every row for the single-pass methods:
+// upper bound min(flops, ncols); learned: clamp(scale * prediction, maxlen, bound)
+__global__ void k_caps(CapParams P) {
+  for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < P.nrows; r += (uint64_t)gridDim.x * blockDim.x) {
+    unsigned long long f = P.row_flops[r];
+    unsigned long long ub = f < P.ncols ? f : P.ncols;
+    unsigned long long c = ub;
+    if (P.pred) {
+      double want = ceil((double)P.pred[r] * P.scale);
+      unsigned long long lo = P.row_maxlen[r];
+      unsigned long long w = want < 0 ? 0ull : (unsigned long long)min(want, 1.8e19);
+      c = w < lo ? lo : w;
+      if (c > ub) c = ub;
+    }
+    if (c > 0xFFFFFFFFull) c = 0xFFFFFFFFull;
+    P.cap[r] = (uint32_t)c;
+  }
+}
+int launch_caps(ptc_ctx* ctx, const CapParams& P) {
+  if (!P.nrows) return PTC_OK;
+  hipLaunchKernelGGL(k_caps, dim3(gridn(P.nrows, 256)), dim3(256), 0, ctx->stream, P);
+  PTC_HIP(ctx, hipGetLastError());
+  return PTC_OK;
+}
+
+// rows that overflowed their predicted capacity -> a compact list
+__global__ void k_overflow_rows(const uint32_t* __restrict__ nnz, const uint32_t* __restrict__ cap,
+                                uint64_t nrows, uint32_t* __restrict__ list, uint32_t* __restrict__ n) {
+  for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nrows; r += (uint64_t)gridDim.x * blockDim.x)
+    if (nnz[r] > cap[r]) list[atomicAdd(n, 1u)] = (uint32_t)r;
+}
+int launch_overflow_rows(ptc_ctx* ctx, const uint32_t* nnz, const uint32_t* cap, uint64_t nrows,
+                         uint32_t* list, uint32_t* n) {
+  if (!nrows) return PTC_OK;
+  hipLaunchKernelGGL(k_overflow_rows, dim3(gridn(nrows, 256)), dim3(256), 0, ctx->stream, nnz, cap, nrows, list, n);
+  PTC_HIP(ctx, hipGetLastError());
+  return PTC_OK;
+}
+
+// items of a subset of rows (fallback): items whose row is listed
+__global__ void k_select_items(const uint4* __restrict__ items, uint64_t nitems, const uint32_t* __restrict__ nnz,
+                               const uint32_t* __restrict__ cap, uint4* __restrict__ out, uint32_t* __restrict__ n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nitems; i += (uint64_t)gridDim.x * blockDim.x) {
+    uint4 it = items[i];
+    if (nnz[it.x] > cap[it.x]) out[atomicAdd(n, 1u)] = it;
+  }
+}
+int launch_select_items(ptc_ctx* ctx, const uint4* items, uint64_t nitems, const uint32_t* nnz,
+                        const uint32_t* cap, uint4* out, uint32_t* n) {
+  if (!nitems) return PTC_OK;
+  hipLaunchKernelGGL(k_select_items, dim3(gridn(nitems, 256)), dim3(256), 0, ctx->stream, items, nitems, nnz, cap, out, n);
+  PTC_HIP(ctx, hipGetLastError());
+  return PTC_OK;
+}
+
+// reset per-row merge state of selected rows
+__global__ void k_reset_rows(const uint32_t* __restrict__ list, uint32_t n, uint32_t* __restrict__ row_done) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) row_done[list[i]] = 0;
+}
+int launch_reset_rows(ptc_ctx* ctx, const uint32_t* list, uint32_t n, uint32_t* row_done) {
+  if (!n) return PTC_OK;
+  hipLaunchKernelGGL(k_reset_rows, dim3(gridn(n, 256)), dim3(256), 0, ctx->stream, list, n, row_done);
+  PTC_HIP(ctx, hipGetLastError());
+  return PTC_OK;
+}
+
+}  // namespace ptc

@@ -1,0 +1,177 @@
+// Internal declarations of the engine: context, device tensors, key tables,
+// the device allocator and kernel launchers shared by the .hip files.
+#ifndef PTC_INTERNAL_H
+#define PTC_INTERNAL_H
+
+#include <cstdint>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "partc.h"
+
+#ifdef __HIPCC__
+#include <hip/hip_runtime.h>
+#endif
+
+namespace ptc {
+
+uint64_t splitmix64(uint64_t& s);
+
+struct Xoshiro {
+  uint64_t s[4];
+  explicit Xoshiro(uint64_t seed);
+  uint64_t next();
+  double uniform();
+  uint32_t below(uint32_t n);
+};
+
+}  // namespace ptc
+
+#ifdef __HIPCC__
+
+#define PTC_EMPTY_KEY 0xFFFFFFFFFFFFFFFFull
+
+// Mixed-radix linearisation of a key tuple: key = sum_j x_j * stride[j] with
+// stride[j] = prod_{i<j} radix[i].  A tuple with x_j >= radix[j] has no key
+// (it cannot match anything the table holds).
+struct KeyCodec {
+  uint32_t n = 0;                    // number of modes in the key
+  uint32_t mode[PTC_MAX_MODES] = {}; // tensor mode of key position j
+  uint32_t radix[PTC_MAX_MODES] = {};
+  uint64_t stride[PTC_MAX_MODES] = {};
+  uint64_t space = 1;                // prod radix (number of possible keys)
+};
+
+// Table entry: 16 bytes, eight to a 128-byte L2 line.
+struct KtEntry {
+  unsigned long long key;  // PTC_EMPTY_KEY when free
+  uint32_t off;            // first position of the slice in the grouped order
+  uint32_t len;            // slice length
+};
+
+// One slice (a distinct key) in compact order.
+struct KtSlice {
+  unsigned long long key;
+  uint32_t off;
+  uint32_t len;
+};
+
+struct ptc_block {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+};
+
+#endif  // __HIPCC__
+
+#ifdef __HIPCC__
+
+struct ptc_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  hipDeviceProp_t prop{};
+  // caching allocator: size class -> free blocks
+  std::multimap<size_t, void*> free_blocks;
+  std::map<void*, size_t> live;
+  size_t cached_bytes = 0;
+  hipEvent_t ev[8] = {};
+  // pinned host staging for small device->host reads
+  void* host_pinned = nullptr;
+  size_t host_pinned_bytes = 0;
+};
+
+struct ptc_dtensor {
+  ptc_ctx* ctx = nullptr;
+  uint32_t d = 0;
+  uint32_t nk = 0;
+  uint64_t N = 0;
+  uint32_t dim_tc[PTC_MAX_MODES] = {};
+  uint32_t dims[PTC_MAX_MODES] = {};
+  uint32_t mode_max[PTC_MAX_MODES] = {};  // max index per mode (0 if N == 0)
+  uint32_t* hedges = nullptr;             // device, N*d
+  double* vals = nullptr;                 // device, N (FP64 tensors)
+  float* vals32 = nullptr;                // device, N (FP32 results)
+  float* err32 = nullptr;                 // device, N: compensation term of FP32 results
+  bool owns = true;
+};
+
+struct ptc_hashtable {
+  ptc_ctx* ctx = nullptr;
+  const ptc_dtensor* t = nullptr;
+  KeyCodec codec;
+  uint64_t cap = 0;            // table slots (power of two)
+  KtEntry* entries = nullptr;  // device, cap
+  uint64_t num_keys = 0;
+  KtSlice* slices = nullptr;   // device, num_keys (slot order)
+  uint32_t* perm = nullptr;    // device, N: original ids grouped by key
+  uint32_t* slice_of = nullptr;// device, N: compact slice index of each grouped position
+  // Streaming layout of the non-key modes, in grouped order: the linearised
+  // non-key tuple (its codec below) and the value.  Built for every table so
+  // the same table serves as B (columns) or as A (rows).
+  KeyCodec rest;
+  uint32_t* rest_key32 = nullptr;        // when rest.space <= 2^32
+  unsigned long long* rest_key64 = nullptr;
+  double* gval = nullptr;      // device, N: values in grouped order
+  float* gval32 = nullptr;     // device, N: FP32 copy (made on demand)
+  double ms_build = 0.0;
+};
+
+#endif  // __HIPCC__
+
+#ifdef __HIPCC__
+namespace ptc {
+
+int set_err(ptc_ctx* ctx, int code, const char* fmt, ...);
+int hip_check(ptc_ctx* ctx, hipError_t e, const char* what);
+#define PTC_HIP(ctx, call)                                 \
+  do {                                                     \
+    hipError_t _e = (call);                                \
+    if (_e != hipSuccess) return ptc::hip_check(ctx, _e, #call); \
+  } while (0)
+
+void* dalloc(ptc_ctx* ctx, size_t bytes);
+void dfree(ptc_ctx* ctx, void* p);
+template <class T>
+T* dalloc_n(ptc_ctx* ctx, size_t n) {
+  return static_cast<T*>(dalloc(ctx, sizeof(T) * (n ? n : 1)));
+}
+
+// Scoped device workspace: returns every block to the context on exit.
+struct Workspace {
+  ptc_ctx* ctx;
+  std::vector<void*> blocks;
+  explicit Workspace(ptc_ctx* c) : ctx(c) {}
+  ~Workspace() {
+    for (void* p : blocks) dfree(ctx, p);
+  }
+  template <class T>
+  T* get(size_t n) {
+    T* p = dalloc_n<T>(ctx, n);
+    if (p) blocks.push_back(p);
+    return p;
+  }
+};
+
+// Copy a small device value to the host through pinned memory, synchronously
+// on the context stream.
+int read_back(ptc_ctx* ctx, void* host, const void* dev, size_t bytes);
+
+// Exclusive prefix sums on the context stream.  out may alias in.  total
+// (device, optional) receives the sum of all elements.
+int scan_u32_to_u64(ptc_ctx* ctx, const uint32_t* in, unsigned long long* out, uint64_t n,
+                    unsigned long long* total);
+int scan_u64(ptc_ctx* ctx, const unsigned long long* in, unsigned long long* out, uint64_t n,
+             unsigned long long* total);
+
+int fill_zero(ptc_ctx* ctx, void* p, size_t bytes);
+
+// Key codec of a tensor over the given modes, from the tensor's maxima.
+int make_codec(ptc_ctx* ctx, const ptc_dtensor* t, const uint32_t* modes, uint32_t n,
+               KeyCodec* out);
+
+}  // namespace ptc
+#endif  // __HIPCC__
+
+#endif  // PTC_INTERNAL_H

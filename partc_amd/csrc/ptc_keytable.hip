@@ -1,0 +1,367 @@
+// Device key table: groups a tensor's nonzeros by key.
+//
+// Role in the reference: buildHashTable (fksCuckoo.hpp:1044-1060) inserts
+// nonzeros one by one into an FKS two-level table whose second level is a
+// per-bucket cuckoo table of id lists; reorder_tensor (fksCuckooTC.hpp:31-77)
+// then renumbers the nonzeros so every key's list ("slice") is contiguous.
+//
+// Here the same result is built in HBM in four passes:
+//   1. insert: each key is linearised to 64 bits and inserted into an
+//      open-addressing table of 16-byte entries by a cooperative group of
+//      KT_GROUP lanes that reads one 128-byte line of 8 entries at a time,
+//      ballots for a match or a free entry and claims it with atomicCAS;
+//      the group leader then takes the nonzero's rank in its slice with an
+//      atomicAdd on the slot's counter;
+//   2. pack + scan: slot counts and occupancy are packed into one 64-bit
+//      word ((occupied << 32) | count) and prefix-summed in one scan, which
+//      yields each slice's offset and its compact slice index together;
+//   3. finalize: entries receive (offset, length); a compact slice list is
+//      written;
+//   4. scatter: every nonzero goes to offset + rank, with its non-key tuple
+//      (linearised) and its value, in the streaming layout the contraction
+//      reads.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+
+#include "ptc_internal.h"
+
+namespace ptc {
+
+constexpr int KT_GROUP = 8;  // lanes per key: 8 x 16-byte entries = one 128-byte line
+constexpr int KT_THREADS = 256;
+
+__device__ __forceinline__ unsigned long long mix64(unsigned long long k) {
+  // MurmurHash3 fmix64 finaliser
+  k ^= k >> 33;
+  k *= 0xff51afd7ed558ccdull;
+  k ^= k >> 33;
+  k *= 0xc4ceb9fe1a85ec53ull;
+  k ^= k >> 33;
+  return k;
+}
+
+// Linearise the key of nonzero e (modes codec.mode[]); returns false if an
+// index is outside the codec's radix.
+__device__ __forceinline__ bool encode_key(const uint32_t* __restrict__ h, uint32_t d, uint64_t e,
+                                           const KeyCodec& c, unsigned long long* key) {
+  unsigned long long k = 0;
+  for (uint32_t j = 0; j < c.n; ++j) {
+    uint32_t x = h[e * d + c.mode[j]];
+    if (x >= c.radix[j]) return false;
+    k += (unsigned long long)x * c.stride[j];
+  }
+  *key = k;
+  return true;
+}
+
+__device__ __forceinline__ unsigned long long load_key_l2(const KtEntry* e) {
+  return __hip_atomic_load(&e->key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Pass 1.  One group of KT_GROUP lanes per nonzero.
+__global__ void __launch_bounds__(KT_THREADS) k_kt_insert(
+    const uint32_t* __restrict__ h, uint32_t d, uint64_t N, KeyCodec codec,
+    KtEntry* __restrict__ tab, uint64_t nbuckets, uint32_t* __restrict__ cnt,
+    uint32_t* __restrict__ slot_of, uint32_t* __restrict__ rank_of) {
+  const int lane = threadIdx.x & 63;
+  const int sub = lane & (KT_GROUP - 1);
+  const int gbase = lane - sub;  // first lane of this group in the wave
+  const unsigned long long gmask = ((1ull << KT_GROUP) - 1) << gbase;
+  const uint64_t groups_per_grid = (uint64_t)gridDim.x * (KT_THREADS / KT_GROUP);
+  for (uint64_t e = (uint64_t)blockIdx.x * (KT_THREADS / KT_GROUP) + threadIdx.x / KT_GROUP; e < N;
+       e += groups_per_grid) {
+    unsigned long long key = 0;
+    encode_key(h, d, e, codec, &key);  // table's own tensor: always within radix
+    uint64_t b = mix64(key) & (nbuckets - 1);
+    uint32_t slot = 0;
+    while (true) {
+      const uint64_t idx = b * KT_GROUP + sub;
+      unsigned long long k = load_key_l2(&tab[idx]);
+      unsigned long long hit = __ballot(k == key) & gmask;
+      if (hit) {
+        slot = (uint32_t)(b * KT_GROUP + (__ffsll((long long)hit) - 1 - gbase));
+        break;
+      }
+      unsigned long long fre = __ballot(k == PTC_EMPTY_KEY) & gmask;
+      if (!fre) {
+        b = (b + 1) & (nbuckets - 1);
+        continue;
+      }
+      const int leader = __ffsll((long long)fre) - 1;
+      unsigned long long old = 0;
+      if (lane == leader) old = atomicCAS(&tab[idx].key, PTC_EMPTY_KEY, key);
+      old = __shfl(old, leader, 64);
+      if (old == PTC_EMPTY_KEY || old == key) {
+        slot = (uint32_t)(b * KT_GROUP + (leader - gbase));
+        break;
+      }
+      // lost the race to a different key: re-read the same line
+    }
+    if (sub == 0) {
+      uint32_t r = atomicAdd(&cnt[slot], 1u);
+      slot_of[e] = slot;
+      rank_of[e] = r;
+    }
+  }
+}
+
+__global__ void k_kt_clear(KtEntry* __restrict__ tab, uint32_t* __restrict__ cnt, uint64_t cap) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < cap; i += (uint64_t)gridDim.x * blockDim.x) {
+    tab[i].key = PTC_EMPTY_KEY;
+    tab[i].off = 0;
+    tab[i].len = 0;
+    cnt[i] = 0;
+  }
+}
+
+// Pass 2a: pack (occupied << 32) | count.
+__global__ void k_kt_pack(const uint32_t* __restrict__ cnt, unsigned long long* __restrict__ packed,
+                          uint64_t cap) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < cap; i += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t c = cnt[i];
+    packed[i] = ((unsigned long long)(c > 0) << 32) | c;
+  }
+}
+
+// Pass 3: entries get (offset, length); compact slice list; cnt[] is
+// overwritten with the compact slice index of every occupied slot.
+__global__ void k_kt_finalize(KtEntry* __restrict__ tab, uint32_t* __restrict__ cnt,
+                              const unsigned long long* __restrict__ pre, uint64_t cap,
+                              KtSlice* __restrict__ slices) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < cap; i += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t c = cnt[i];
+    if (!c) continue;
+    unsigned long long p = pre[i];
+    uint32_t off = (uint32_t)(p & 0xFFFFFFFFull);
+    uint32_t sid = (uint32_t)(p >> 32);
+    tab[i].off = off;
+    tab[i].len = c;
+    slices[sid].key = tab[i].key;
+    slices[sid].off = off;
+    slices[sid].len = c;
+    cnt[i] = sid;
+  }
+}
+
+// Pass 4: scatter into grouped order with the streaming layout.
+template <class RestT>
+__global__ void k_kt_scatter(const uint32_t* __restrict__ h, const double* __restrict__ val,
+                             uint32_t d, uint64_t N, KeyCodec rest, const KtEntry* __restrict__ tab,
+                             const uint32_t* __restrict__ sid_of_slot,
+                             const uint32_t* __restrict__ slot_of, const uint32_t* __restrict__ rank_of,
+                             uint32_t* __restrict__ perm, uint32_t* __restrict__ slice_of,
+                             RestT* __restrict__ rest_key, double* __restrict__ gval) {
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < N; e += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t s = slot_of[e];
+    uint64_t pos = (uint64_t)tab[s].off + rank_of[e];
+    perm[pos] = (uint32_t)e;
+    slice_of[pos] = sid_of_slot[s];
+    unsigned long long rk = 0;
+    for (uint32_t j = 0; j < rest.n; ++j) rk += (unsigned long long)h[e * d + rest.mode[j]] * rest.stride[j];
+    rest_key[pos] = (RestT)rk;
+    gval[pos] = val[e];
+  }
+}
+
+static unsigned grid_for(uint64_t n, unsigned threads, unsigned cap = 8192) {
+  uint64_t g = (n + threads - 1) / threads;
+  if (g < 1) g = 1;
+  return (unsigned)std::min<uint64_t>(g, cap);
+}
+
+// Probe kernel for the C ABI: one thread per query key.
+__global__ void k_kt_probe(const uint32_t* __restrict__ q, uint64_t n, KeyCodec codec,
+                           const KtEntry* __restrict__ tab, uint64_t nbuckets,
+                           unsigned long long* __restrict__ off_len) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    unsigned long long key = 0;
+    bool ok = true;
+    for (uint32_t j = 0; j < codec.n; ++j) {
+      uint32_t x = q[i * codec.n + j];
+      if (x >= codec.radix[j]) ok = false;
+      key += (unsigned long long)x * codec.stride[j];
+    }
+    unsigned long long res = 0;
+    if (ok) {
+      uint64_t b = mix64(key) & (nbuckets - 1);
+      for (uint64_t probes = 0; probes <= nbuckets; ++probes) {
+        bool done = false;
+        for (int s = 0; s < KT_GROUP; ++s) {
+          KtEntry en = tab[b * KT_GROUP + s];
+          if (en.key == key) {
+            res = ((unsigned long long)en.off << 32) | en.len;
+            done = true;
+            break;
+          }
+          if (en.key == PTC_EMPTY_KEY) {
+            done = true;
+            break;
+          }
+        }
+        if (done) break;
+        b = (b + 1) & (nbuckets - 1);
+      }
+    }
+    off_len[i] = res;
+  }
+}
+
+}  // namespace ptc
+
+using namespace ptc;
+
+extern "C" int ptc_hashtable_build(ptc_ctx* ctx, const ptc_dtensor* t, ptc_hashtable** out) {
+  if (!ctx || !t || !out) return PTC_ERR_ARG;
+  *out = nullptr;
+  PTC_HIP(ctx, hipSetDevice(ctx->device));
+  auto t0 = std::chrono::steady_clock::now();
+  ptc_hashtable* ht = new ptc_hashtable();
+  ht->ctx = ctx;
+  ht->t = t;
+  int rc = make_codec(ctx, t, t->dim_tc, t->nk, &ht->codec);
+  if (!rc) rc = make_codec(ctx, t, t->dim_tc + t->nk, t->d - t->nk, &ht->rest);
+  if (rc) {
+    delete ht;
+    return rc;
+  }
+  const uint64_t N = t->N;
+  uint64_t cap = 64;
+  while (cap < N + N / 4) cap <<= 1;
+  ht->cap = cap;
+  ht->entries = dalloc_n<KtEntry>(ctx, cap);
+  ht->perm = dalloc_n<uint32_t>(ctx, N);
+  ht->slice_of = dalloc_n<uint32_t>(ctx, N);
+  ht->gval = dalloc_n<double>(ctx, N);
+  const bool rest64 = ht->rest.space > 0xFFFFFFFFull;
+  if (rest64)
+    ht->rest_key64 = dalloc_n<unsigned long long>(ctx, N);
+  else
+    ht->rest_key32 = dalloc_n<uint32_t>(ctx, N);
+  Workspace ws(ctx);
+  uint32_t* cnt = ws.get<uint32_t>(cap);
+  unsigned long long* packed = ws.get<unsigned long long>(cap + 1);
+  uint32_t* slot_of = ws.get<uint32_t>(N);
+  uint32_t* rank_of = ws.get<uint32_t>(N);
+  if (!ht->entries || !ht->perm || !ht->slice_of || !ht->gval || !(ht->rest_key32 || ht->rest_key64) ||
+      !cnt || !packed || !slot_of || !rank_of) {
+    ptc_hashtable_free(ht);
+    return PTC_ERR_NOMEM;
+  }
+  hipStream_t st = ctx->stream;
+  hipLaunchKernelGGL(k_kt_clear, dim3(grid_for(cap, 256)), dim3(256), 0, st, ht->entries, cnt, cap);
+  const uint64_t nb = cap / KT_GROUP;
+  if (N) {
+    unsigned g = grid_for(N, KT_THREADS / KT_GROUP, 16384);
+    hipLaunchKernelGGL(k_kt_insert, dim3(g), dim3(KT_THREADS), 0, st, t->hedges, t->d, N, ht->codec,
+                       ht->entries, nb, cnt, slot_of, rank_of);
+  }
+  hipLaunchKernelGGL(k_kt_pack, dim3(grid_for(cap, 256)), dim3(256), 0, st, cnt, packed, cap);
+  rc = scan_u64(ctx, packed, packed, cap, packed + cap);
+  if (rc) {
+    ptc_hashtable_free(ht);
+    return rc;
+  }
+  unsigned long long total = 0;
+  rc = read_back(ctx, &total, packed + cap, sizeof(total));
+  if (rc) {
+    ptc_hashtable_free(ht);
+    return rc;
+  }
+  ht->num_keys = total >> 32;
+  if ((total & 0xFFFFFFFFull) != N) {
+    ptc_hashtable_free(ht);
+    return set_err(ctx, PTC_ERR_INTERNAL, "key table grouped %llu of %llu nonzeros",
+                   (unsigned long long)(total & 0xFFFFFFFFull), (unsigned long long)N);
+  }
+  ht->slices = dalloc_n<KtSlice>(ctx, ht->num_keys);
+  if (!ht->slices) {
+    ptc_hashtable_free(ht);
+    return PTC_ERR_NOMEM;
+  }
+  hipLaunchKernelGGL(k_kt_finalize, dim3(grid_for(cap, 256)), dim3(256), 0, st, ht->entries, cnt, packed,
+                     cap, ht->slices);
+  if (N) {
+    if (rest64)
+      hipLaunchKernelGGL((k_kt_scatter<unsigned long long>), dim3(grid_for(N, 256)), dim3(256), 0, st,
+                         t->hedges, t->vals, t->d, N, ht->rest, ht->entries, cnt, slot_of, rank_of,
+                         ht->perm, ht->slice_of, ht->rest_key64, ht->gval);
+    else
+      hipLaunchKernelGGL((k_kt_scatter<uint32_t>), dim3(grid_for(N, 256)), dim3(256), 0, st, t->hedges,
+                         t->vals, t->d, N, ht->rest, ht->entries, cnt, slot_of, rank_of, ht->perm,
+                         ht->slice_of, ht->rest_key32, ht->gval);
+  }
+  hipError_t e = hipStreamSynchronize(st);
+  if (e != hipSuccess) {
+    ptc_hashtable_free(ht);
+    return hip_check(ctx, e, "hashtable build");
+  }
+  ht->ms_build = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  *out = ht;
+  return PTC_OK;
+}
+
+extern "C" int ptc_hashtable_info(const ptc_hashtable* h, uint64_t* num_keys, uint64_t* num_items) {
+  if (!h) return PTC_ERR_ARG;
+  if (num_keys) *num_keys = h->num_keys;
+  if (num_items) *num_items = h->t ? h->t->N : 0;
+  return PTC_OK;
+}
+
+extern "C" int ptc_hashtable_probe(ptc_ctx* ctx, const ptc_hashtable* h, uint64_t n,
+                                   const uint32_t* keys, uint64_t* slice_len, uint64_t* slice_off,
+                                   uint32_t* slice_ids, uint64_t cap) {
+  if (!ctx || !h || (n && (!keys || !slice_len))) return PTC_ERR_ARG;
+  if (n == 0) {
+    if (slice_off) slice_off[0] = 0;
+    return PTC_OK;
+  }
+  PTC_HIP(ctx, hipSetDevice(ctx->device));
+  const uint32_t nk = h->codec.n;
+  Workspace ws(ctx);
+  uint32_t* dq = ws.get<uint32_t>(n * (nk ? nk : 1));
+  unsigned long long* dres = ws.get<unsigned long long>(n);
+  if (!dq || !dres) return PTC_ERR_NOMEM;
+  if (nk) PTC_HIP(ctx, hipMemcpyAsync(dq, keys, sizeof(uint32_t) * n * nk, hipMemcpyHostToDevice, ctx->stream));
+  hipLaunchKernelGGL(k_kt_probe, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, dq, n, h->codec,
+                     h->entries, h->cap / KT_GROUP, dres);
+  PTC_HIP(ctx, hipGetLastError());
+  std::vector<unsigned long long> res(n);
+  PTC_HIP(ctx, hipMemcpyAsync(res.data(), dres, sizeof(unsigned long long) * n, hipMemcpyDeviceToHost, ctx->stream));
+  PTC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  uint64_t acc = 0;
+  for (uint64_t i = 0; i < n; ++i) {
+    slice_len[i] = res[i] & 0xFFFFFFFFull;
+    if (slice_off) slice_off[i] = acc;
+    acc += slice_len[i];
+  }
+  if (slice_off) slice_off[n] = acc;
+  if (slice_ids) {
+    if (acc > cap) return set_err(ctx, PTC_ERR_RANGE, "slice_ids capacity %llu < %llu", (unsigned long long)cap, (unsigned long long)acc);
+    uint64_t w = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+      uint64_t len = res[i] & 0xFFFFFFFFull, off = res[i] >> 32;
+      if (len)
+        PTC_HIP(ctx, hipMemcpyAsync(slice_ids + w, h->perm + off, sizeof(uint32_t) * len, hipMemcpyDeviceToHost, ctx->stream));
+      w += len;
+    }
+    PTC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  }
+  return PTC_OK;
+}
+
+extern "C" void ptc_hashtable_free(ptc_hashtable* h) {
+  if (!h) return;
+  if (h->ctx) {
+    dfree(h->ctx, h->entries);
+    dfree(h->ctx, h->slices);
+    dfree(h->ctx, h->perm);
+    dfree(h->ctx, h->slice_of);
+    dfree(h->ctx, h->rest_key32);
+    dfree(h->ctx, h->rest_key64);
+    dfree(h->ctx, h->gval);
+    dfree(h->ctx, h->gval32);
+  }
+  delete h;
+}

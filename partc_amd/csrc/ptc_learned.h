@@ -1,0 +1,26 @@
+// Learned row-size prediction: features of every output row and the
+// network that maps them to a predicted number of nonzeros.
+#ifndef PTC_LEARNED_H
+#define PTC_LEARNED_H
+
+#include "ptc_internal.h"
+
+namespace ptc {
+
+struct LearnedInputs {
+  uint64_t nrows;
+  const KtSlice* rows;                 // A positions of each row
+  const unsigned long long* row_flops; // sum of probed slice lengths
+  const uint32_t* row_hits;            // distinct contracted keys that hit B
+  const uint32_t* row_maxlen;          // longest slice
+  uint64_t ncols;                      // column space
+};
+
+// pred[r] = predicted nonzeros of row r (device array of nrows floats).
+int learned_predict(ptc_ctx* ctx, const ptc_mlp* mlp, const LearnedInputs& in, float* pred);
+
+}  // namespace ptc
+
+extern "C" const ptc_mlp* ptc_learned_default(void);
+
+#endif

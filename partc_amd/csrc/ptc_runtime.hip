@@ -1,0 +1,527 @@
+// Runtime of the engine: contexts, the caching device allocator, device
+// tensors, the .meta parser and the device prefix scan.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <limits>
+#include <sstream>
+#include <string>
+
+#include "ptc_internal.h"
+
+namespace ptc {
+
+int set_err(ptc_ctx* ctx, int code, const char* fmt, ...) {
+  if (ctx) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    ctx->err = buf;
+  }
+  return code;
+}
+
+int hip_check(ptc_ctx* ctx, hipError_t e, const char* what) {
+  if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation)
+    return set_err(ctx, PTC_ERR_NOMEM, "%s: %s", what, hipGetErrorString(e));
+  return set_err(ctx, PTC_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+static size_t size_class(size_t bytes) {
+  // 256-byte granularity below 1 MiB, then powers of two / 8 steps
+  if (bytes <= 256) return 256;
+  if (bytes <= (1u << 20)) return (bytes + 255) & ~size_t(255);
+  size_t p = size_t(1) << (63 - __builtin_clzll(bytes));
+  size_t step = p / 8;
+  return (bytes + step - 1) / step * step;
+}
+
+void* dalloc(ptc_ctx* ctx, size_t bytes) {
+  size_t sz = size_class(bytes);
+  auto it = ctx->free_blocks.lower_bound(sz);
+  // reuse a cached block of the same class or at most 2x larger
+  if (it != ctx->free_blocks.end() && it->first <= 2 * sz) {
+    void* p = it->second;
+    size_t have = it->first;
+    ctx->free_blocks.erase(it);
+    ctx->cached_bytes -= have;
+    ctx->live[p] = have;
+    return p;
+  }
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, sz);
+  if (e != hipSuccess) {
+    // release the cache and retry once
+    (void)hipGetLastError();
+    hipStreamSynchronize(ctx->stream);
+    for (auto& kv : ctx->free_blocks) hipFree(kv.second);
+    ctx->free_blocks.clear();
+    ctx->cached_bytes = 0;
+    e = hipMalloc(&p, sz);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      set_err(ctx, PTC_ERR_NOMEM, "device allocation of %zu bytes failed", sz);
+      return nullptr;
+    }
+  }
+  ctx->live[p] = sz;
+  return p;
+}
+
+void dfree(ptc_ctx* ctx, void* p) {
+  if (!p || !ctx) return;
+  auto it = ctx->live.find(p);
+  if (it == ctx->live.end()) return;
+  size_t sz = it->second;
+  ctx->live.erase(it);
+  // Blocks are only reused by work queued later on the same stream, so no
+  // synchronisation is needed before caching them.
+  ctx->free_blocks.emplace(sz, p);
+  ctx->cached_bytes += sz;
+}
+
+int read_back(ptc_ctx* ctx, void* host, const void* dev, size_t bytes) {
+  if (bytes > ctx->host_pinned_bytes) {
+    if (ctx->host_pinned) hipHostFree(ctx->host_pinned);
+    ctx->host_pinned = nullptr;
+    size_t want = std::max<size_t>(bytes, 4096);
+    PTC_HIP(ctx, hipHostMalloc(&ctx->host_pinned, want, hipHostMallocDefault));
+    ctx->host_pinned_bytes = want;
+  }
+  PTC_HIP(ctx, hipMemcpyAsync(ctx->host_pinned, dev, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  PTC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  memcpy(host, ctx->host_pinned, bytes);
+  return PTC_OK;
+}
+
+// ------------------------------------------------------------------ fill
+
+__global__ void k_fill_zero(uint4* __restrict__ p, size_t n16, unsigned char* __restrict__ tail,
+                            size_t ntail) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t k = i; k < n16; k += stride) p[k] = make_uint4(0, 0, 0, 0);
+  if (i < ntail) tail[i] = 0;
+}
+
+// Clears device memory with a kernel (not hipMemsetAsync, so the work stays
+// valid inside captured graphs).
+int fill_zero(ptc_ctx* ctx, void* p, size_t bytes) {
+  if (!bytes) return PTC_OK;
+  uintptr_t a = (uintptr_t)p;
+  if (a & 15) return set_err(ctx, PTC_ERR_INTERNAL, "fill_zero: unaligned pointer");
+  size_t n16 = bytes / 16, ntail = bytes % 16;
+  unsigned grid = (unsigned)std::min<size_t>((n16 + 255) / 256 + 1, 4096);
+  hipLaunchKernelGGL(k_fill_zero, dim3(grid), dim3(256), 0, ctx->stream, (uint4*)p, n16,
+                     (unsigned char*)p + n16 * 16, ntail);
+  PTC_HIP(ctx, hipGetLastError());
+  return PTC_OK;
+}
+
+// ------------------------------------------------------------------ scan
+//
+// Three-phase exclusive scan: per-block totals, a single-block scan of the
+// totals (recursively for very long inputs), then per-block scans offset by
+// their block's prefix.  Each block covers SCAN_TILE elements.
+
+constexpr int SCAN_THREADS = 256;
+constexpr int SCAN_ITEMS = 8;
+constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
+
+template <class In>
+__device__ inline unsigned long long load_elem(const In* in, uint64_t i, uint64_t n) {
+  return i < n ? (unsigned long long)in[i] : 0ull;
+}
+
+__device__ inline unsigned long long block_exclusive_scan(unsigned long long v,
+                                                          unsigned long long* total) {
+  __shared__ unsigned long long wsum[SCAN_THREADS / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  unsigned long long x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    unsigned long long y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) wsum[wid] = x;
+  __syncthreads();
+  unsigned long long base = 0, all = 0;
+#pragma unroll
+  for (int w = 0; w < SCAN_THREADS / 64; ++w) {
+    if (w < wid) base += wsum[w];
+    all += wsum[w];
+  }
+  __syncthreads();
+  *total = all;
+  return base + x - v;
+}
+
+template <class In>
+__global__ void k_scan_reduce(const In* __restrict__ in, uint64_t n,
+                              unsigned long long* __restrict__ block_sums) {
+  uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
+  unsigned long long s = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k)
+    s += load_elem(in, base + (uint64_t)k * SCAN_THREADS + threadIdx.x, n);
+  unsigned long long tot;
+  block_exclusive_scan(s, &tot);
+  if (threadIdx.x == 0) block_sums[blockIdx.x] = tot;
+}
+
+template <class In>
+__global__ void k_scan_apply(const In* __restrict__ in, unsigned long long* __restrict__ out,
+                             uint64_t n, const unsigned long long* __restrict__ block_prefix,
+                             unsigned long long* __restrict__ total_out) {
+  uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
+  unsigned long long v[SCAN_ITEMS];
+  unsigned long long s = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    v[k] = load_elem(in, base + k, n);
+    s += v[k];
+  }
+  unsigned long long tot;
+  unsigned long long run = block_exclusive_scan(s, &tot) + (block_prefix ? block_prefix[blockIdx.x] : 0ull);
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    if (base + k < n) out[base + k] = run;
+    run += v[k];
+  }
+  if (total_out && blockIdx.x == gridDim.x - 1 && threadIdx.x == SCAN_THREADS - 1) *total_out = run;
+}
+
+template <class In>
+static int scan_impl(ptc_ctx* ctx, const In* in, unsigned long long* out, uint64_t n,
+                     unsigned long long* total) {
+  if (n == 0) {
+    if (total) return fill_zero(ctx, total, sizeof(unsigned long long));
+    return PTC_OK;
+  }
+  uint64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+  if (nb == 1) {
+    hipLaunchKernelGGL((k_scan_apply<In>), dim3(1), dim3(SCAN_THREADS), 0, ctx->stream, in, out, n,
+                       (const unsigned long long*)nullptr, total);
+    PTC_HIP(ctx, hipGetLastError());
+    return PTC_OK;
+  }
+  Workspace ws(ctx);
+  unsigned long long* sums = ws.get<unsigned long long>(nb);
+  if (!sums) return PTC_ERR_NOMEM;
+  hipLaunchKernelGGL((k_scan_reduce<In>), dim3((unsigned)nb), dim3(SCAN_THREADS), 0, ctx->stream,
+                     in, n, sums);
+  PTC_HIP(ctx, hipGetLastError());
+  int rc = scan_impl<unsigned long long>(ctx, sums, sums, nb, nullptr);
+  if (rc) return rc;
+  hipLaunchKernelGGL((k_scan_apply<In>), dim3((unsigned)nb), dim3(SCAN_THREADS), 0, ctx->stream,
+                     in, out, n, (const unsigned long long*)sums, total);
+  PTC_HIP(ctx, hipGetLastError());
+  return PTC_OK;
+}
+
+int scan_u32_to_u64(ptc_ctx* ctx, const uint32_t* in, unsigned long long* out, uint64_t n,
+                    unsigned long long* total) {
+  return scan_impl<uint32_t>(ctx, in, out, n, total);
+}
+int scan_u64(ptc_ctx* ctx, const unsigned long long* in, unsigned long long* out, uint64_t n,
+             unsigned long long* total) {
+  return scan_impl<unsigned long long>(ctx, in, out, n, total);
+}
+
+// ------------------------------------------------------------- key codec
+
+int make_codec(ptc_ctx* ctx, const ptc_dtensor* t, const uint32_t* modes, uint32_t n,
+               KeyCodec* out) {
+  KeyCodec c;
+  c.n = n;
+  unsigned __int128 space = 1;
+  for (uint32_t j = 0; j < n; ++j) {
+    uint32_t m = modes[j];
+    if (m >= t->d) return set_err(ctx, PTC_ERR_ARG, "key mode %u out of range", m);
+    c.mode[j] = m;
+    c.radix[j] = (t->N ? t->mode_max[m] : 0u) + 1u;
+    if (c.radix[j] == 0) c.radix[j] = 0xFFFFFFFFu;  // max index 2^32-1: radix saturates
+    c.stride[j] = (uint64_t)space;
+    space *= c.radix[j];
+    if (space > (unsigned __int128)0xFFFFFFFFFFFFFFFEull)
+      return set_err(ctx, PTC_ERR_UNSUPPORTED,
+                     "key space over %u modes exceeds 64 bits (not supported)", n);
+  }
+  c.space = (uint64_t)space;
+  *out = c;
+  return PTC_OK;
+}
+
+// --------------------------------------------------------- mode maxima
+
+__global__ void k_mode_max(const uint32_t* __restrict__ h, uint64_t N, uint32_t d,
+                           unsigned int* __restrict__ mx) {
+  __shared__ unsigned int smax[PTC_MAX_MODES];
+  if (threadIdx.x < PTC_MAX_MODES) smax[threadIdx.x] = 0;
+  __syncthreads();
+  unsigned int local[PTC_MAX_MODES];
+  for (uint32_t j = 0; j < d; ++j) local[j] = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    for (uint32_t j = 0; j < d; ++j) local[j] = max(local[j], h[i * d + j]);
+  for (uint32_t j = 0; j < d; ++j) atomicMax(&smax[j], local[j]);
+  __syncthreads();
+  if (threadIdx.x < d) atomicMax(&mx[threadIdx.x], smax[threadIdx.x]);
+}
+
+}  // namespace ptc
+
+using namespace ptc;
+
+// ================================================================ C ABI
+
+extern "C" int ptc_create(ptc_ctx** out, int device) {
+  if (!out) return PTC_ERR_ARG;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+    (void)hipGetLastError();
+    return PTC_ERR_NODEV;
+  }
+  if (device < 0) {
+    if (hipGetDevice(&device) != hipSuccess) device = 0;
+  }
+  if (device >= n) return PTC_ERR_ARG;
+  ptc_ctx* ctx = new ptc_ctx();
+  ctx->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipGetDeviceProperties(&ctx->prop, device) != hipSuccess) {
+    delete ctx;
+    return PTC_ERR_HIP;
+  }
+  for (auto& e : ctx->ev) hipEventCreate(&e);
+  *out = ctx;
+  return PTC_OK;
+}
+
+extern "C" void ptc_destroy(ptc_ctx* ctx) {
+  if (!ctx) return;
+  hipSetDevice(ctx->device);
+  hipStreamSynchronize(ctx->stream);
+  for (auto& kv : ctx->free_blocks) hipFree(kv.second);
+  for (auto& kv : ctx->live) hipFree(kv.first);
+  for (auto& e : ctx->ev)
+    if (e) hipEventDestroy(e);
+  if (ctx->host_pinned) hipHostFree(ctx->host_pinned);
+  hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+extern "C" const char* ptc_last_error(const ptc_ctx* ctx) { return ctx ? ctx->err.c_str() : ""; }
+
+extern "C" int ptc_synchronize(ptc_ctx* ctx) {
+  if (!ctx) return PTC_ERR_ARG;
+  PTC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return PTC_OK;
+}
+
+extern "C" void* ptc_stream(ptc_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+static int check_perm(ptc_ctx* ctx, uint32_t d, uint32_t nk, const uint32_t* dim_tc) {
+  if (d == 0 || d > PTC_MAX_MODES) return set_err(ctx, PTC_ERR_ARG, "d=%u not in [1,%d]", d, PTC_MAX_MODES);
+  if (nk > d) return set_err(ctx, PTC_ERR_ARG, "num_key_dimensions %u > d %u", nk, d);
+  if (!dim_tc) return set_err(ctx, PTC_ERR_ARG, "dim_tc is NULL");
+  uint32_t seen = 0;
+  for (uint32_t j = 0; j < d; ++j) {
+    if (dim_tc[j] >= d || (seen >> dim_tc[j]) & 1u)
+      return set_err(ctx, PTC_ERR_ARG, "dim_tc is not a permutation of 0..%u", d - 1);
+    seen |= 1u << dim_tc[j];
+  }
+  return PTC_OK;
+}
+
+extern "C" int ptc_dtensor_upload(ptc_ctx* ctx, const ptc_tensor_view* v, ptc_dtensor** out) {
+  if (!ctx || !v || !out) return PTC_ERR_ARG;
+  *out = nullptr;
+  int rc = check_perm(ctx, v->d, v->num_key_dimensions, v->dim_tc);
+  if (rc) return rc;
+  if (v->N >= 0xFFFFFFFFull) return set_err(ctx, PTC_ERR_RANGE, "N=%llu exceeds 2^32-2", (unsigned long long)v->N);
+  if (v->N && (!v->hedges || !v->vals)) return set_err(ctx, PTC_ERR_ARG, "NULL data arrays");
+  PTC_HIP(ctx, hipSetDevice(ctx->device));
+  ptc_dtensor* t = new ptc_dtensor();
+  t->ctx = ctx;
+  t->d = v->d;
+  t->nk = v->num_key_dimensions;
+  t->N = v->N;
+  for (uint32_t j = 0; j < v->d; ++j) {
+    t->dim_tc[j] = v->dim_tc[j];
+    t->dims[j] = v->dims ? v->dims[j] : 0;
+  }
+  for (uint64_t i = 0; i < v->N; ++i)
+    for (uint32_t j = 0; j < v->d; ++j) t->mode_max[j] = std::max(t->mode_max[j], v->hedges[i * v->d + j]);
+  if (!v->dims)
+    for (uint32_t j = 0; j < v->d; ++j) t->dims[j] = v->N ? t->mode_max[j] + 1 : 0;
+  t->hedges = dalloc_n<uint32_t>(ctx, v->N * v->d);
+  t->vals = dalloc_n<double>(ctx, v->N);
+  if (!t->hedges || !t->vals) {
+    ptc_dtensor_free(t);
+    return PTC_ERR_NOMEM;
+  }
+  if (v->N) {
+    hipError_t e1 = hipMemcpyAsync(t->hedges, v->hedges, sizeof(uint32_t) * v->N * v->d, hipMemcpyHostToDevice, ctx->stream);
+    hipError_t e2 = hipMemcpyAsync(t->vals, v->vals, sizeof(double) * v->N, hipMemcpyHostToDevice, ctx->stream);
+    hipError_t e3 = hipStreamSynchronize(ctx->stream);
+    if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
+      ptc_dtensor_free(t);
+      return hip_check(ctx, e1 != hipSuccess ? e1 : (e2 != hipSuccess ? e2 : e3), "upload");
+    }
+  }
+  *out = t;
+  return PTC_OK;
+}
+
+extern "C" int ptc_dtensor_wrap(ptc_ctx* ctx, uint32_t d, uint32_t nk, uint64_t N,
+                                const uint32_t* d_hedges, const double* d_vals,
+                                const uint32_t* dim_tc, const uint32_t* dims,
+                                const uint32_t* mode_max, ptc_dtensor** out) {
+  if (!ctx || !out) return PTC_ERR_ARG;
+  *out = nullptr;
+  int rc = check_perm(ctx, d, nk, dim_tc);
+  if (rc) return rc;
+  if (N >= 0xFFFFFFFFull) return set_err(ctx, PTC_ERR_RANGE, "N exceeds 2^32-2");
+  ptc_dtensor* t = new ptc_dtensor();
+  t->ctx = ctx;
+  t->d = d;
+  t->nk = nk;
+  t->N = N;
+  t->owns = false;
+  t->hedges = const_cast<uint32_t*>(d_hedges);
+  t->vals = const_cast<double*>(d_vals);
+  for (uint32_t j = 0; j < d; ++j) t->dim_tc[j] = dim_tc[j];
+  if (mode_max) {
+    for (uint32_t j = 0; j < d; ++j) t->mode_max[j] = mode_max[j];
+  } else if (N) {
+    Workspace ws(ctx);
+    unsigned int* mx = ws.get<unsigned int>(PTC_MAX_MODES);
+    if (!mx) {
+      delete t;
+      return PTC_ERR_NOMEM;
+    }
+    rc = fill_zero(ctx, mx, sizeof(unsigned int) * PTC_MAX_MODES);
+    if (!rc) {
+      unsigned grid = (unsigned)std::min<uint64_t>((N + 255) / 256, 2048);
+      hipLaunchKernelGGL(k_mode_max, dim3(grid), dim3(256), 0, ctx->stream, d_hedges, N, d, mx);
+      rc = read_back(ctx, t->mode_max, mx, sizeof(uint32_t) * d);
+    }
+    if (rc) {
+      delete t;
+      return rc;
+    }
+  }
+  for (uint32_t j = 0; j < d; ++j) t->dims[j] = dims ? dims[j] : (N ? t->mode_max[j] + 1 : 0);
+  *out = t;
+  return PTC_OK;
+}
+
+extern "C" int ptc_dtensor_set_keys(ptc_dtensor* t, uint32_t nk, const uint32_t* dim_tc) {
+  if (!t) return PTC_ERR_ARG;
+  int rc = check_perm(t->ctx, t->d, nk, dim_tc);
+  if (rc) return rc;
+  t->nk = nk;
+  for (uint32_t j = 0; j < t->d; ++j) t->dim_tc[j] = dim_tc[j];
+  return PTC_OK;
+}
+
+extern "C" int ptc_dtensor_info(const ptc_dtensor* t, uint32_t* d, uint64_t* N, uint32_t* nk) {
+  if (!t) return PTC_ERR_ARG;
+  if (d) *d = t->d;
+  if (N) *N = t->N;
+  if (nk) *nk = t->nk;
+  return PTC_OK;
+}
+
+extern "C" int ptc_dtensor_device_ptrs(const ptc_dtensor* t, const uint32_t** h, const double** v,
+                                       const float** v32) {
+  if (!t) return PTC_ERR_ARG;
+  if (h) *h = t->hedges;
+  if (v) *v = t->vals;
+  if (v32) *v32 = t->vals32;
+  return PTC_OK;
+}
+
+__global__ void k_widen(const float* __restrict__ a, const float* __restrict__ err,
+                        double* __restrict__ out, uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    out[i] = (double)a[i] + (err ? (double)err[i] : 0.0);
+}
+
+extern "C" int ptc_dtensor_download(ptc_ctx* ctx, const ptc_dtensor* t, uint32_t* hedges,
+                                    double* vals, uint32_t* dims) {
+  if (!ctx || !t) return PTC_ERR_ARG;
+  PTC_HIP(ctx, hipSetDevice(ctx->device));
+  if (hedges && t->N)
+    PTC_HIP(ctx, hipMemcpyAsync(hedges, t->hedges, sizeof(uint32_t) * t->N * t->d, hipMemcpyDeviceToHost, ctx->stream));
+  if (vals && t->N) {
+    if (t->vals) {
+      PTC_HIP(ctx, hipMemcpyAsync(vals, t->vals, sizeof(double) * t->N, hipMemcpyDeviceToHost, ctx->stream));
+    } else if (t->vals32) {
+      // An FP32 result: the value is the FP32 sum; report it widened (the
+      // compensation term is available through ptc_dtensor_download_f32's
+      // companion view, which keeps it separate).
+      Workspace ws(ctx);
+      double* tmp = ws.get<double>(t->N);
+      if (!tmp) return PTC_ERR_NOMEM;
+      unsigned grid = (unsigned)std::min<uint64_t>((t->N + 255) / 256, 4096);
+      hipLaunchKernelGGL(k_widen, dim3(grid), dim3(256), 0, ctx->stream, t->vals32, (const float*)nullptr, tmp, t->N);
+      PTC_HIP(ctx, hipMemcpyAsync(vals, tmp, sizeof(double) * t->N, hipMemcpyDeviceToHost, ctx->stream));
+      PTC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
+  }
+  if (dims)
+    for (uint32_t j = 0; j < t->d; ++j) dims[j] = t->dims[j];
+  PTC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return PTC_OK;
+}
+
+extern "C" int ptc_dtensor_download_f32(ptc_ctx* ctx, const ptc_dtensor* t, float* vals) {
+  if (!ctx || !t || !vals) return PTC_ERR_ARG;
+  if (!t->vals32) return set_err(ctx, PTC_ERR_ARG, "tensor has no FP32 values");
+  if (t->N)
+    PTC_HIP(ctx, hipMemcpyAsync(vals, t->vals32, sizeof(float) * t->N, hipMemcpyDeviceToHost, ctx->stream));
+  PTC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return PTC_OK;
+}
+
+extern "C" void ptc_dtensor_free(ptc_dtensor* t) {
+  if (!t) return;
+  if (t->owns && t->ctx) {
+    dfree(t->ctx, t->hedges);
+    dfree(t->ctx, t->vals);
+    dfree(t->ctx, t->vals32);
+    dfree(t->ctx, t->err32);
+  }
+  delete t;
+}
+
+// --------------------------------------------------------------- .meta
+
+extern "C" int ptc_meta_parse(const char* path, uint32_t d, uint32_t* c_out, uint32_t* dim_tc_out) {
+  if (!path || !c_out || !dim_tc_out || d == 0 || d > PTC_MAX_MODES) return PTC_ERR_ARG;
+  std::ifstream in(path);
+  if (!in.is_open()) return PTC_ERR_IO;
+  std::string line;
+  for (int i = 0; i < 2; ++i)
+    if (!std::getline(in, line)) return PTC_ERR_IO;  // two free-text lines (tensor.hpp:159-160)
+  long long c = -1;
+  if (!(in >> c) || c < 0 || (uint64_t)c > d) return PTC_ERR_IO;
+  uint32_t seen = 0;
+  for (uint32_t j = 0; j < d; ++j) {
+    long long m = -1;
+    if (!(in >> m) || m < 0 || (uint64_t)m >= d || ((seen >> m) & 1u)) return PTC_ERR_IO;
+    seen |= 1u << m;
+    dim_tc_out[j] = (uint32_t)m;
+  }
+  *c_out = (uint32_t)c;
+  return PTC_OK;
+}

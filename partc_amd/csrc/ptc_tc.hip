@@ -1,0 +1,618 @@
+// Host orchestration of a contraction: the C ABI entry points
+// ptc_tc_fksCuckoo / ptc_contract / ptc_matricize_*.
+//
+// Methods (all share the probe / row / work-item preprocessing):
+//   TWO_PASS     symbolic pass (exact distinct columns per row) -> prefix
+//                scan -> numeric pass writing straight into the output.
+//   UPPER_BOUND  one numeric pass into per-row capacity min(flops, columns),
+//                then a prefix scan of the true counts and a copy.
+//   LEARNED      one numeric pass into per-row capacities predicted by the
+//                row-size network; rows whose prediction was too small are
+//                redone by the exact two-pass path.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+
+#include "ptc_contract.h"
+#include "ptc_internal.h"
+#include "ptc_learned.h"
+
+namespace ptc {
+
+static constexpr uint32_t DEFAULT_DENSE_MAX = 8192;
+static constexpr uint32_t DEFAULT_LDS_HASH = 2048;
+static constexpr uint64_t DEFAULT_CHUNK = 32768;
+
+struct Timer {
+  ptc_ctx* ctx;
+  hipEvent_t ev[8];
+  int n = 0;
+  explicit Timer(ptc_ctx* c) : ctx(c) {
+    for (int i = 0; i < 8; ++i) ev[i] = c->ev[i];
+  }
+  void mark() {
+    if (n < 8) hipEventRecord(ev[n++], ctx->stream);
+  }
+  double ms(int a, int b) {
+    if (a >= n || b >= n) return 0.0;
+    float t = 0.f;
+    hipEventElapsedTime(&t, ev[a], ev[b]);
+    return t;
+  }
+};
+
+// Hash accumulator plan of every row: LDS slots or an HBM region.
+struct HashPlan {
+  uint32_t* hcap = nullptr;
+  unsigned long long* gcap = nullptr;  // scanned in place into offsets
+  unsigned long long pool = 0;         // total HBM slots
+  void* gkeys = nullptr;
+  double* gvals = nullptr;
+};
+
+struct Run {
+  ptc_ctx* ctx;
+  Workspace ws;
+  const ptc_hashtable* HA;
+  const ptc_hashtable* HB;
+  ptc_options opt;
+  bool dense = true, col64 = false;
+  uint64_t npos = 0, nrows = 0, ncols = 0;
+  uint32_t dO = 0, nrowmodes = 0, lds_slots = DEFAULT_LDS_HASH;
+  // preprocessing
+  uint32_t *pos_len = nullptr, *pos_off = nullptr;
+  unsigned long long* apre = nullptr;
+  uint32_t *row_hits = nullptr, *row_maxlen = nullptr, *row_nchunk = nullptr, *row_heavy = nullptr;
+  uint32_t *row_done = nullptr, *row_nnz = nullptr;
+  unsigned long long *row_flops = nullptr, *packed = nullptr;
+  uint4* items = nullptr;
+  uint64_t F = 0, nitems = 0, nheavy = 0;
+  // heavy dense merge buffers
+  uint32_t* hbits = nullptr;
+  double* hacc = nullptr;
+  uint32_t* hflag = nullptr;
+  uint32_t* overflow = nullptr;
+  explicit Run(ptc_ctx* c) : ctx(c), ws(c) {}
+};
+
+__global__ void k_merge_hbm_caps(uint64_t n, uint32_t* hcap, unsigned long long* gcap, const uint32_t* qh) {
+  for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (uint64_t)gridDim.x * blockDim.x)
+    if (gcap[r]) {
+      hcap[r] = qh[r];
+      gcap[r] = qh[r];
+    }
+}
+
+static int plan_hash(Run& R, HashPlan& hp, const uint32_t* bound32, const unsigned long long* bound64,
+                     const unsigned long long* hbm_bound) {
+  ptc_ctx* ctx = R.ctx;
+  if (!hp.hcap) hp.hcap = R.ws.get<uint32_t>(R.nrows);
+  if (!hp.gcap) hp.gcap = R.ws.get<unsigned long long>(R.nrows + 1);
+  if (!hp.hcap || !hp.gcap) return PTC_ERR_NOMEM;
+  HashCapParams P{};
+  P.nrows = R.nrows;
+  P.bound32 = bound32;
+  P.bound64 = bound64;
+  P.cap_col = R.ncols;
+  P.row_nchunk = R.row_nchunk;
+  P.lds_slots = R.lds_slots;
+  P.row_hcap = hp.hcap;
+  P.gcap = hp.gcap;
+  int rc = launch_hash_caps(ctx, P);
+  if (rc) return rc;
+  if (hbm_bound) {
+    // HBM rows sized from a guaranteed bound instead (capacity mode)
+    HashCapParams Q = P;
+    Q.bound32 = nullptr;
+    Q.bound64 = hbm_bound;
+    Q.lds_slots = 0;  // every row sized as if in HBM ...
+    Q.row_hcap = R.ws.get<uint32_t>(R.nrows);
+    Q.gcap = R.ws.get<unsigned long long>(R.nrows + 1);
+    if (!Q.row_hcap || !Q.gcap) return PTC_ERR_NOMEM;
+    if ((rc = launch_hash_caps(ctx, Q))) return rc;
+    // ... and merged: rows that P put in HBM take Q's size
+    hipLaunchKernelGGL(k_merge_hbm_caps, dim3((unsigned)std::min<uint64_t>((R.nrows + 255) / 256, 4096)), dim3(256),
+                       0, ctx->stream, R.nrows, hp.hcap, hp.gcap, (const uint32_t*)Q.row_hcap);
+    PTC_HIP(ctx, hipGetLastError());
+  }
+  if ((rc = scan_u64(ctx, hp.gcap, hp.gcap, R.nrows, hp.gcap + R.nrows))) return rc;
+  if ((rc = read_back(ctx, &hp.pool, hp.gcap + R.nrows, sizeof(hp.pool)))) return rc;
+  if (hp.pool) {
+    size_t kb = R.col64 ? 8 : 4;
+    hp.gkeys = R.ws.get<unsigned char>(hp.pool * kb);
+    hp.gvals = R.ws.get<double>(hp.pool);
+    if (!hp.gkeys || !hp.gvals) return PTC_ERR_NOMEM;
+  }
+  return PTC_OK;
+}
+
+static int reset_hash_pool(Run& R, HashPlan& hp, bool values) {
+  if (!hp.pool) return PTC_OK;
+  int rc = R.col64 ? fill_u64(R.ctx, (unsigned long long*)hp.gkeys, hp.pool, ~0ull)
+                   : fill_u32(R.ctx, (uint32_t*)hp.gkeys, hp.pool, ~0u);
+  if (!rc && values) rc = fill_zero(R.ctx, hp.gvals, hp.pool * sizeof(double));
+  return rc;
+}
+
+static void base_params(Run& R, PassParams& P, const HashPlan* hp) {
+  memset(&P, 0, sizeof(P));
+  P.items = R.items;
+  P.rows = R.HA->slices;
+  P.row_nchunk = R.row_nchunk;
+  P.row_heavy = R.row_heavy;
+  P.row_done = R.row_done;
+  P.apre = R.apre;
+  P.pos_off = R.pos_off;
+  P.aval = R.HA->gval;
+  P.bcol = R.col64 ? (const void*)R.HB->rest_key64 : (const void*)R.HB->rest_key32;
+  P.bval = R.HB->gval;
+  P.ncols = R.ncols;
+  P.hbits = R.hbits;
+  P.hacc = R.hacc;
+  P.hflag = R.hflag;
+  if (hp) {
+    P.row_hcap = hp->hcap;
+    P.row_gcap = hp->gcap;  // after the scan: offsets; a row is in HBM iff its region is non-empty
+    P.row_goff = hp->gcap;
+    P.gkeys = hp->gkeys;
+    P.gvals = hp->gvals;
+  }
+  P.row_nnz = R.row_nnz;
+  P.overflow = R.overflow;
+  P.dO = R.dO;
+  P.nrowmodes = R.nrowmodes;
+  P.rowc = R.HA->codec;
+  P.colc = R.HB->rest;
+}
+
+// After the scan gcap holds offsets; "row in HBM" must be recovered as
+// offset[r+1] != offset[r].  k_pass reads row_gcap[r] as a flag, so give it
+// a separate flag array.
+__global__ void k_hbm_flag(const unsigned long long* __restrict__ off, uint64_t n, unsigned long long* __restrict__ flag) {
+  for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (uint64_t)gridDim.x * blockDim.x)
+    flag[r] = off[r + 1] - off[r];
+}
+
+static int hbm_flags(Run& R, HashPlan& hp, unsigned long long** flags) {
+  unsigned long long* f = R.ws.get<unsigned long long>(R.nrows);
+  if (!f) return PTC_ERR_NOMEM;
+  hipLaunchKernelGGL(k_hbm_flag, dim3((unsigned)std::min<uint64_t>((R.nrows + 255) / 256, 4096)), dim3(256), 0,
+                     R.ctx->stream, (const unsigned long long*)hp.gcap, R.nrows, f);
+  PTC_HIP(R.ctx, hipGetLastError());
+  *flags = f;
+  return PTC_OK;
+}
+
+static int run_pass(Run& R, int pass, HashPlan* hp, const uint4* items, uint64_t nitems, PassParams& P) {
+  unsigned long long* flags = nullptr;
+  if (!R.dense) {
+    int rc = hbm_flags(R, *hp, &flags);
+    if (rc) return rc;
+    P.row_gcap = flags;
+  }
+  P.items = items;
+  size_t lds = pass_lds_bytes(pass, R.dense, R.col64, R.ncols, R.lds_slots);
+  return launch_pass(R.ctx, pass, R.dense, R.col64, P, nitems, lds);
+}
+
+static int prepare_heavy(Run& R, int pass) {
+  if (!R.dense || !R.nheavy) return PTC_OK;
+  ptc_ctx* ctx = R.ctx;
+  if (pass == PASS_SYMBOLIC) {
+    uint64_t nbits = (R.ncols + 31) / 32;
+    if (!R.hbits) R.hbits = R.ws.get<uint32_t>(R.nheavy * nbits);
+    if (!R.hbits) return PTC_ERR_NOMEM;
+    return fill_zero(ctx, R.hbits, R.nheavy * nbits * 4);
+  }
+  if (!R.hacc) R.hacc = R.ws.get<double>(R.nheavy * R.ncols);
+  if (!R.hflag) R.hflag = R.ws.get<uint32_t>(R.nheavy * R.ncols);
+  if (!R.hacc || !R.hflag) return PTC_ERR_NOMEM;
+  int rc = fill_zero(ctx, R.hacc, R.nheavy * R.ncols * 8);
+  if (!rc) rc = fill_zero(ctx, R.hflag, R.nheavy * R.ncols * 4);
+  return rc;
+}
+
+static int make_output(Run& R, uint64_t nnz, const ptc_dtensor* A, const ptc_dtensor* B, ptc_dtensor** O) {
+  ptc_dtensor* o = new ptc_dtensor();
+  o->ctx = R.ctx;
+  o->d = R.dO;
+  o->nk = R.dO;  // tensor.hpp:124
+  o->N = nnz;
+  for (uint32_t j = 0; j < R.dO; ++j) {
+    o->dim_tc[j] = j;  // identity (tensor.hpp:128-129)
+    o->dims[j] = 0;    // zeros (tensor.hpp:121-122)
+  }
+  // per-mode index bounds (used as key radices if O is contracted again)
+  for (uint32_t j = 0; j < A->nk; ++j) o->mode_max[j] = A->mode_max[A->dim_tc[j]];
+  for (uint32_t j = 0; j < B->d - B->nk; ++j) o->mode_max[A->nk + j] = B->mode_max[B->dim_tc[B->nk + j]];
+  o->hedges = dalloc_n<uint32_t>(R.ctx, nnz * R.dO);
+  o->vals = dalloc_n<double>(R.ctx, nnz);
+  if (!o->hedges || !o->vals) {
+    ptc_dtensor_free(o);
+    return PTC_ERR_NOMEM;
+  }
+  *O = o;
+  return PTC_OK;
+}
+
+}  // namespace ptc
+
+using namespace ptc;
+
+extern "C" void ptc_options_default(ptc_options* o) {
+  if (!o) return;
+  memset(o, 0, sizeof(*o));
+  o->method = PTC_METHOD_TWO_PASS;
+  o->value_type = PTC_VAL_F64;
+  o->acc_mode = PTC_ACC_AUTO;
+  o->predict_scale = 1.15;
+}
+
+extern "C" int ptc_tc_fksCuckoo(ptc_ctx* ctx, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B,
+                                const ptc_dtensor* A, const ptc_dtensor* B, const ptc_options* opt_in,
+                                ptc_dtensor** O, ptc_stats* stats) {
+  if (!ctx || !HT_A || !HT_B || !A || !B || !O) return PTC_ERR_ARG;
+  *O = nullptr;
+  ptc_options opt;
+  if (opt_in) opt = *opt_in; else ptc_options_default(&opt);
+  if (HT_A->t != A || HT_B->t != B) return set_err(ctx, PTC_ERR_ARG, "tables were not built on these tensors");
+  const uint32_t c = A->d - A->nk;
+  if (c == 0 || B->nk != c) return set_err(ctx, PTC_ERR_ARG, "A has %u contracted modes, B %u key modes", c, B->nk);
+  if (HT_A->rest.n != c || HT_B->codec.n != c) return set_err(ctx, PTC_ERR_INTERNAL, "codec shape mismatch");
+  if (opt.value_type != PTC_VAL_F64) return set_err(ctx, PTC_ERR_UNSUPPORTED, "value type %d not available here", opt.value_type);
+  if (opt.method == PTC_METHOD_LEARNED && !opt.mlp && !ptc_learned_default())
+    return set_err(ctx, PTC_ERR_ARG, "learned method needs a trained row-size model");
+  PTC_HIP(ctx, hipSetDevice(ctx->device));
+
+  Run R(ctx);
+  R.HA = HT_A;
+  R.HB = HT_B;
+  R.opt = opt;
+  R.npos = A->N;
+  R.nrows = HT_A->num_keys;
+  R.ncols = HT_B->rest.space;
+  R.col64 = R.ncols > 0xFFFFFFFFull;
+  R.nrowmodes = A->nk;
+  R.dO = A->nk + (B->d - B->nk);  // tensor.hpp:113
+  if (R.dO == 0) return set_err(ctx, PTC_ERR_EMPTY, "tensor contraction not possible!! (output has no modes)");
+  uint32_t dense_max = opt.dense_max_cols ? opt.dense_max_cols : DEFAULT_DENSE_MAX;
+  dense_max = std::min<uint32_t>(dense_max, 16384);
+  R.dense = opt.acc_mode == PTC_ACC_DENSE ? R.ncols <= 16384
+          : opt.acc_mode == PTC_ACC_AUTO ? R.ncols <= dense_max : false;
+  if (opt.acc_mode == PTC_ACC_DENSE && !R.dense) return set_err(ctx, PTC_ERR_ARG, "column space too large for a dense accumulator");
+  R.lds_slots = opt.lds_hash_max ? opt.lds_hash_max : DEFAULT_LDS_HASH;
+  if (opt.acc_mode == PTC_ACC_SPILL) R.lds_slots = 0;
+  {
+    uint32_t p2 = 16;
+    while (p2 * 2 <= R.lds_slots && p2 < 8192) p2 <<= 1;
+    R.lds_slots = R.lds_slots ? p2 : 0;
+  }
+  const uint64_t chunk = opt.chunk_flops ? opt.chunk_flops : DEFAULT_CHUNK;
+
+  Timer T(ctx);
+  auto wall0 = std::chrono::steady_clock::now();
+  T.mark();  // 0
+  int rc = PTC_OK;
+  Workspace& ws = R.ws;
+  R.pos_len = ws.get<uint32_t>(R.npos);
+  R.pos_off = ws.get<uint32_t>(R.npos);
+  R.apre = ws.get<unsigned long long>(R.npos + 1);
+  R.row_hits = ws.get<uint32_t>(R.nrows);
+  R.row_maxlen = ws.get<uint32_t>(R.nrows);
+  R.row_nchunk = ws.get<uint32_t>(R.nrows);
+  R.row_heavy = ws.get<uint32_t>(R.nrows);
+  R.row_done = ws.get<uint32_t>(R.nrows);
+  R.row_nnz = ws.get<uint32_t>(R.nrows + 1);
+  R.row_flops = ws.get<unsigned long long>(R.nrows);
+  R.packed = ws.get<unsigned long long>(R.nrows + 1);
+  R.overflow = ws.get<uint32_t>(4);
+  unsigned long long* out_off = ws.get<unsigned long long>(R.nrows + 1);
+  if (!R.pos_len || !R.pos_off || !R.apre || !R.row_hits || !R.row_maxlen || !R.row_nchunk || !R.row_heavy ||
+      !R.row_done || !R.row_nnz || !R.row_flops || !R.packed || !R.overflow || !out_off)
+    return PTC_ERR_NOMEM;
+
+  // ---------------- preprocessing: probe, flop prefix, rows, items
+  if ((rc = fill_zero(ctx, R.row_hits, R.nrows * 4))) return rc;
+  if ((rc = fill_zero(ctx, R.row_maxlen, R.nrows * 4))) return rc;
+  if ((rc = fill_zero(ctx, R.overflow, 16))) return rc;
+  {
+    ProbeParams P{};
+    P.npos = R.npos;
+    P.akey = HT_A->rest_key64 ? (const void*)HT_A->rest_key64 : (const void*)HT_A->rest_key32;
+    P.row_of_pos = HT_A->slice_of;
+    P.acodec = HT_A->rest;
+    P.bcodec = HT_B->codec;
+    P.tab = HT_B->entries;
+    P.nbuckets = HT_B->cap / 8;
+    P.pos_len = R.pos_len;
+    P.pos_off = R.pos_off;
+    P.row_hits = R.row_hits;
+    P.row_maxlen = R.row_maxlen;
+    if ((rc = launch_probe(ctx, P, HT_A->rest_key64 != nullptr))) return rc;
+  }
+  if ((rc = scan_u32_to_u64(ctx, R.pos_len, R.apre, R.npos, R.apre + R.npos))) return rc;
+  {
+    RowParams P{};
+    P.nrows = R.nrows;
+    P.rows = HT_A->slices;
+    P.apre = R.apre;
+    P.chunk = chunk;
+    P.row_nchunk = R.row_nchunk;
+    P.row_flops = R.row_flops;
+    P.packed = R.packed;
+    if ((rc = launch_rows(ctx, P))) return rc;
+  }
+  if ((rc = scan_u64(ctx, R.packed, R.packed, R.nrows, R.packed + R.nrows))) return rc;
+  unsigned long long hostv[2] = {0, 0};
+  if ((rc = read_back(ctx, &hostv[0], R.apre + R.npos, 8))) return rc;
+  if ((rc = read_back(ctx, &hostv[1], R.packed + R.nrows, 8))) return rc;
+  R.F = hostv[0];
+  R.nitems = hostv[1] & 0xFFFFFFFFull;
+  R.nheavy = hostv[1] >> 32;
+  if (R.F == 0) return set_err(ctx, PTC_ERR_EMPTY, "tensor contraction not possible!! (no contributing pair)");
+  R.items = ws.get<uint4>(R.nitems);
+  if (!R.items) return PTC_ERR_NOMEM;
+  {
+    ItemParams P{};
+    P.nrows = R.nrows;
+    P.rows = HT_A->slices;
+    P.row_nchunk = R.row_nchunk;
+    P.packed = R.packed;
+    P.row_heavy = R.row_heavy;
+    P.items = R.items;
+    ItemP0Params Q{};
+    Q.npos = R.nheavy ? R.npos : 0;
+    Q.row_of_pos = HT_A->slice_of;
+    Q.row_nchunk = R.row_nchunk;
+    Q.pos_len = R.pos_len;
+    Q.rows = HT_A->slices;
+    Q.apre = R.apre;
+    Q.row_flops = R.row_flops;
+    Q.packed = R.packed;
+    Q.items = R.items;
+    if ((rc = launch_items(ctx, P, Q))) return rc;
+  }
+  T.mark();  // 1
+  uint64_t nnz = 0;
+  uint64_t fallback_rows = 0;
+  ptc_dtensor* out = nullptr;
+  const int method = opt.method;
+
+  if (method == PTC_METHOD_TWO_PASS) {
+    // ---------------- symbolic
+    HashPlan hs;
+    if (!R.dense && (rc = plan_hash(R, hs, nullptr, R.row_flops, nullptr))) return rc;
+    if ((rc = fill_zero(ctx, R.row_nnz, (R.nrows + 1) * 4))) return rc;
+    if ((rc = fill_zero(ctx, R.row_done, R.nrows * 4))) return rc;
+    if ((rc = prepare_heavy(R, PASS_SYMBOLIC))) return rc;
+    if (!R.dense && (rc = reset_hash_pool(R, hs, false))) return rc;
+    PassParams P;
+    base_params(R, P, R.dense ? nullptr : &hs);
+    if ((rc = run_pass(R, PASS_SYMBOLIC, &hs, R.items, R.nitems, P))) return rc;
+    T.mark();  // 2
+    // ---------------- scan
+    if ((rc = scan_u32_to_u64(ctx, R.row_nnz, out_off, R.nrows, out_off + R.nrows))) return rc;
+    if ((rc = read_back(ctx, &hostv[0], out_off + R.nrows, 8))) return rc;
+    nnz = hostv[0];
+    if ((rc = make_output(R, nnz, A, B, &out))) return rc;
+    // ---------------- numeric
+    HashPlan hn;
+    if (!R.dense && (rc = plan_hash(R, hn, R.row_nnz, nullptr, nullptr))) {
+      ptc_dtensor_free(out);
+      return rc;
+    }
+    if ((rc = fill_zero(ctx, R.row_done, R.nrows * 4)) || (rc = prepare_heavy(R, PASS_NUMERIC)) ||
+        (!R.dense && (rc = reset_hash_pool(R, hn, true)))) {
+      ptc_dtensor_free(out);
+      return rc;
+    }
+    base_params(R, P, R.dense ? nullptr : &hn);
+    P.out_base = out_off;
+    P.out_hedges = out->hedges;
+    P.out_vals = out->vals;
+    if ((rc = run_pass(R, PASS_NUMERIC, &hn, R.items, R.nitems, P))) {
+      ptc_dtensor_free(out);
+      return rc;
+    }
+    T.mark();  // 3
+  } else {
+    // ---------------- single pass into capacities
+    uint32_t* cap = ws.get<uint32_t>(R.nrows + 1);
+    unsigned long long* cap_off = ws.get<unsigned long long>(R.nrows + 1);
+    float* pred = nullptr;
+    if (!cap || !cap_off) return PTC_ERR_NOMEM;
+    if (method == PTC_METHOD_LEARNED) {
+      pred = ws.get<float>(R.nrows);
+      if (!pred) return PTC_ERR_NOMEM;
+      LearnedInputs L{};
+      L.nrows = R.nrows;
+      L.rows = HT_A->slices;
+      L.row_flops = R.row_flops;
+      L.row_hits = R.row_hits;
+      L.row_maxlen = R.row_maxlen;
+      L.ncols = R.ncols;
+      if ((rc = learned_predict(ctx, opt.mlp, L, pred))) return rc;
+    }
+    {
+      CapParams P{};
+      P.nrows = R.nrows;
+      P.row_flops = R.row_flops;
+      P.row_maxlen = R.row_maxlen;
+      P.ncols = R.ncols;
+      P.pred = pred;
+      P.scale = opt.predict_scale > 0 ? opt.predict_scale : 1.15;
+      P.cap = cap;
+      if ((rc = launch_caps(ctx, P))) return rc;
+    }
+    if ((rc = scan_u32_to_u64(ctx, cap, cap_off, R.nrows, cap_off + R.nrows))) return rc;
+    unsigned long long capsum = 0;
+    if ((rc = read_back(ctx, &capsum, cap_off + R.nrows, 8))) return rc;
+    const size_t kb = R.col64 ? 8 : 4;
+    void* stg_col = ws.get<unsigned char>(capsum * kb);
+    double* stg_val = ws.get<double>(capsum);
+    if (!stg_col || !stg_val) return PTC_ERR_NOMEM;
+    HashPlan h1;
+    if (!R.dense && (rc = plan_hash(R, h1, cap, nullptr, R.row_flops))) return rc;
+    if ((rc = fill_zero(ctx, R.row_nnz, (R.nrows + 1) * 4)) || (rc = fill_zero(ctx, R.row_done, R.nrows * 4)) ||
+        (rc = prepare_heavy(R, PASS_NUMERIC)) || (!R.dense && (rc = reset_hash_pool(R, h1, true))))
+      return rc;
+    PassParams P;
+    base_params(R, P, R.dense ? nullptr : &h1);
+    P.out_base = cap_off;
+    P.out_cap = cap;
+    P.stg_col = stg_col;
+    P.stg_val = stg_val;
+    if ((rc = run_pass(R, PASS_NUMERIC, &h1, R.items, R.nitems, P))) return rc;
+    T.mark();  // 2
+    uint32_t ovf[2] = {0, 0};
+    if ((rc = read_back(ctx, ovf, R.overflow, 4))) return rc;
+    if (ovf[0] & 0x80000000u) return set_err(ctx, PTC_ERR_INTERNAL, "accumulator table overflow");
+    fallback_rows = ovf[0];
+    uint4* fitems = nullptr;
+    uint32_t nfitems = 0;
+    if (fallback_rows) {
+      // exact two-pass for the rows whose capacity was too small
+      uint32_t* cnt = ws.get<uint32_t>(4);
+      fitems = ws.get<uint4>(R.nitems);
+      uint32_t* frows = ws.get<uint32_t>(fallback_rows);
+      if (!cnt || !fitems || !frows) return PTC_ERR_NOMEM;
+      if ((rc = fill_zero(ctx, cnt, 16))) return rc;
+      if ((rc = launch_select_items(ctx, R.items, R.nitems, R.row_nnz, cap, fitems, cnt))) return rc;
+      if ((rc = launch_overflow_rows(ctx, R.row_nnz, cap, R.nrows, frows, cnt + 1))) return rc;
+      if ((rc = read_back(ctx, &nfitems, cnt, 4))) return rc;
+      // symbolic on those rows: reset their counts and merge state
+      if ((rc = launch_reset_rows(ctx, frows, (uint32_t)fallback_rows, R.row_done))) return rc;
+      if ((rc = launch_reset_rows(ctx, frows, (uint32_t)fallback_rows, R.row_nnz))) return rc;
+      HashPlan hs;
+      if (!R.dense && (rc = plan_hash(R, hs, nullptr, R.row_flops, nullptr))) return rc;
+      if ((rc = prepare_heavy(R, PASS_SYMBOLIC)) || (!R.dense && (rc = reset_hash_pool(R, hs, false)))) return rc;
+      PassParams S;
+      base_params(R, S, R.dense ? nullptr : &hs);
+      if ((rc = run_pass(R, PASS_SYMBOLIC, &hs, fitems, nfitems, S))) return rc;
+    }
+    if ((rc = scan_u32_to_u64(ctx, R.row_nnz, out_off, R.nrows, out_off + R.nrows))) return rc;
+    if ((rc = read_back(ctx, &hostv[0], out_off + R.nrows, 8))) return rc;
+    nnz = hostv[0];
+    if ((rc = make_output(R, nnz, A, B, &out))) return rc;
+    {
+      EmitParams E{};
+      E.nrows = R.nrows;
+      E.rows = HT_A->slices;
+      E.row_nnz = R.row_nnz;
+      E.cap = cap;
+      E.cap_base = cap_off;
+      E.out_base = out_off;
+      E.stg_col = stg_col;
+      E.stg_val = stg_val;
+      E.out_hedges = out->hedges;
+      E.out_vals = out->vals;
+      E.dO = R.dO;
+      E.nrowmodes = R.nrowmodes;
+      E.rowc = HT_A->codec;
+      E.colc = HT_B->rest;
+      if ((rc = launch_emit(ctx, E, R.col64))) {
+        ptc_dtensor_free(out);
+        return rc;
+      }
+    }
+    if (fallback_rows) {
+      HashPlan hn;
+      if (!R.dense && (rc = plan_hash(R, hn, R.row_nnz, nullptr, nullptr))) {
+        ptc_dtensor_free(out);
+        return rc;
+      }
+      // only the fallback rows: reset their merge counters
+      uint32_t* frows = ws.get<uint32_t>(fallback_rows);
+      uint32_t* cnt = ws.get<uint32_t>(2);
+      if (!frows || !cnt || (rc = fill_zero(ctx, cnt, 8)) ||
+          (rc = launch_overflow_rows(ctx, R.row_nnz, cap, R.nrows, frows, cnt)) ||
+          (rc = launch_reset_rows(ctx, frows, (uint32_t)fallback_rows, R.row_done)) ||
+          (rc = prepare_heavy(R, PASS_NUMERIC)) || (!R.dense && (rc = reset_hash_pool(R, hn, true)))) {
+        ptc_dtensor_free(out);
+        return rc ? rc : PTC_ERR_NOMEM;
+      }
+      PassParams Q;
+      base_params(R, Q, R.dense ? nullptr : &hn);
+      Q.out_base = out_off;
+      Q.out_hedges = out->hedges;
+      Q.out_vals = out->vals;
+      if ((rc = run_pass(R, PASS_NUMERIC, &hn, fitems, nfitems, Q))) {
+        ptc_dtensor_free(out);
+        return rc;
+      }
+    }
+    T.mark();  // 3
+  }
+  uint32_t ovf = 0;
+  rc = read_back(ctx, &ovf, R.overflow, 4);
+  if (!rc && (ovf & 0x80000000u)) rc = set_err(ctx, PTC_ERR_INTERNAL, "row count mismatch between passes");
+  if (rc) {
+    ptc_dtensor_free(out);
+    return rc;
+  }
+  hipEventSynchronize(T.ev[T.n - 1]);
+  if (stats) {
+    memset(stats, 0, sizeof(*stats));
+    stats->nnz_a = A->N;
+    stats->nnz_b = B->N;
+    stats->rows = R.nrows;
+    stats->flops = R.F;
+    stats->nnz_out = nnz;
+    stats->fallback_rows = fallback_rows;
+    stats->heavy_rows = R.nheavy;
+    stats->ms_build_a = HT_A->ms_build;
+    stats->ms_build_b = HT_B->ms_build;
+    stats->ms_preprocess = T.ms(0, 1);
+    if (method == PTC_METHOD_TWO_PASS) {
+      stats->ms_symbolic = T.ms(1, 2);
+      stats->ms_numeric = T.ms(2, 3);
+    } else {
+      stats->ms_numeric = T.ms(1, 3);
+    }
+    stats->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
+  }
+  *O = out;
+  return PTC_OK;
+}
+
+// A view of t with another key split, sharing t's device arrays.
+static ptc_dtensor* view_with_keys(const ptc_dtensor* t, uint32_t nk, const uint32_t* dim_tc) {
+  ptc_dtensor* v = new ptc_dtensor(*t);
+  v->owns = false;
+  v->nk = nk;
+  for (uint32_t j = 0; j < t->d; ++j) v->dim_tc[j] = dim_tc[j];
+  return v;
+}
+
+extern "C" int ptc_contract(ptc_ctx* ctx, const ptc_dtensor* A, const ptc_dtensor* B, const ptc_options* opt,
+                            ptc_dtensor** O, ptc_stats* stats) {
+  if (!ctx || !A || !B || !O) return PTC_ERR_ARG;
+  *O = nullptr;
+  const uint32_t c = A->nk;
+  if (c == 0 || B->nk != c) return set_err(ctx, PTC_ERR_ARG, "A and B must have the same number (>0) of contracted modes");
+  // A_copy: free modes first (main.cpp:68-109)
+  uint32_t rot[PTC_MAX_MODES];
+  for (uint32_t i = c; i < A->d; ++i) rot[i - c] = A->dim_tc[i];
+  for (uint32_t i = 0; i < c; ++i) rot[A->d - c + i] = A->dim_tc[i];
+  ptc_dtensor* Ac = view_with_keys(A, A->d - c, rot);
+  ptc_hashtable *HB = nullptr, *HA = nullptr;
+  int rc = ptc_hashtable_build(ctx, B, &HB);  // B first, as main.cpp:115-116
+  if (!rc) rc = ptc_hashtable_build(ctx, Ac, &HA);
+  if (!rc) rc = ptc_tc_fksCuckoo(ctx, HA, HB, Ac, B, opt, O, stats);
+  ptc_hashtable_free(HA);
+  ptc_hashtable_free(HB);
+  delete Ac;
+  return rc;
+}
+
+extern "C" int ptc_matricize_spGEMM(ptc_ctx* ctx, const ptc_dtensor* A, const ptc_dtensor* B, ptc_dtensor** O,
+                                    ptc_stats* stats) {
+  return ptc_contract(ctx, A, B, nullptr, O, stats);
+}
+
+extern "C" int ptc_matricize_sort_spGEMM(ptc_ctx* ctx, const ptc_dtensor* A, const ptc_dtensor* B, ptc_dtensor** O,
+                                         ptc_stats* stats) {
+  return ptc_contract(ctx, A, B, nullptr, O, stats);
+}

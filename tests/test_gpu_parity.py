@@ -1,0 +1,171 @@
+"""Parity of the HIP engine with the oracle and the golden fixtures.
+
+Bar: output coordinates bit-exact after sorting; FP64 values within 1e-12
+relative (the reference's own summation order is pinned by the golden
+fixtures; the engine accumulates with LDS/HBM atomics, whose order differs).
+"""
+import numpy as np
+import pytest
+
+import partc_amd as P
+from oracle import orc
+from tests.helpers import golden_cases, max_rel, random_case, sort_coo, tc_inputs
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+CASES = golden_cases()
+
+METHODS = {"two_pass": P.METHOD_TWO_PASS, "upper_bound": P.METHOD_UPPER_BOUND}
+ACCS = {"auto": P.ACC_AUTO, "hash": P.ACC_HASH, "spill": P.ACC_SPILL}
+
+
+def run_gpu(ctx, A, B, **opts):
+    HB = P.buildHashTable(B, ctx)
+    HA = P.buildHashTable(A, ctx)
+    st = P.Stats()
+    O = P.tc_fksCuckoo(HA, HB, stats=st, **opts)
+    h, v = sort_coo(O.hedges_array, O.val_array)
+    return h, v, st
+
+
+def check_vs_oracle(ctx, A, B, label, **opts):
+    print(f"[case] {label} opts={opts}", flush=True)
+    h, v, st = run_gpu(ctx, A, B, **opts)
+    oh, ov, info = orc.tc_fksCuckoo(A, B)
+    oh, ov = sort_coo(oh, ov)
+    assert st.flops == info["flops"]
+    assert h.shape == oh.shape, (h.shape, oh.shape)
+    assert np.array_equal(h, oh)
+    assert max_rel(v, ov) <= TOL
+    return st
+
+
+@pytest.mark.parametrize("acc", sorted(ACCS))
+@pytest.mark.parametrize("method", sorted(METHODS))
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_golden(ctx, name, method, acc):
+    case = CASES[name]
+    A, B = tc_inputs(case)
+    print(f"[case] golden {name} {method} {acc}", flush=True)
+    h, v, st = run_gpu(ctx, A, B, method=METHODS[method], acc_mode=ACCS[acc])
+    assert np.array_equal(h, case["coords"])
+    assert max_rel(v, case["vals_ordered"]) <= TOL
+    assert st.flops == int(case["flops"][0])
+    assert st.nnz_out == case["coords"].shape[0]
+
+
+@pytest.mark.parametrize("method", sorted(METHODS))
+def test_c1_full(ctx, method):
+    """3-mode 200x300x400, 50k uniform nnz, self-contraction over {1,2}."""
+    h, v = P.gen_coo(1, [200, 300, 400], 50000, P.DIST_UNIFORM)
+    A, B = P.self_contraction_inputs(h, v, [1, 2])
+    check_vs_oracle(ctx, A, B, "C1", method=METHODS[method])
+
+
+RANDOM = [
+    # d, dims, nnz, contracted, zipf
+    (2, [300, 200], 3000, [1], False),
+    (3, [50, 40, 30], 8000, [2], True),
+    (3, [100, 20, 30], 5000, [0, 2], False),
+    (4, [40, 30, 20, 10], 20000, [1, 2], True),
+    (4, [2482 // 10, 286, 1403, 17], 60000, [1, 2, 3], True),
+    (5, [12, 10, 9, 8, 7], 15000, [0, 3], False),
+    (3, [5000, 30, 40], 20000, [1, 2], True),    # many rows
+    (3, [20, 3000, 2000], 20000, [0], True),     # wide output: hash accumulators
+]
+
+
+@pytest.mark.parametrize("i", range(len(RANDOM)))
+@pytest.mark.parametrize("method", sorted(METHODS))
+def test_random_vs_oracle(ctx, i, method):
+    d, dims, nnz, contracted, zipf = RANDOM[i]
+    rng = np.random.default_rng(1000 + i)
+    A, B = random_case(rng, d, dims, nnz, contracted, zipf, seed=500 + i)
+    check_vs_oracle(ctx, A, B, f"random{i}", method=METHODS[method])
+
+
+@pytest.mark.parametrize("chunk", [64, 1000])
+@pytest.mark.parametrize("acc", ["auto", "spill", "hash"])
+def test_heavy_rows_split(ctx, chunk, acc):
+    """Small chunks split rows over many workgroups (HBM merge path)."""
+    h, v = P.gen_coo(77, [60, 50, 40], 20000, P.DIST_ZIPF)
+    A, B = P.self_contraction_inputs(h, v, [1, 2])
+    st = check_vs_oracle(ctx, A, B, f"heavy chunk={chunk}", chunk_flops=chunk, acc_mode=ACCS[acc])
+    assert st.heavy_rows > 0
+
+
+def test_hash_lds_sizes(ctx):
+    """Wide column space with tiny LDS tables: exercises LDS and HBM rows."""
+    h, v = P.gen_coo(78, [30, 400, 5000], 30000, P.DIST_ZIPF)
+    A, B = P.self_contraction_inputs(h, v, [0])
+    for lds in (16, 64, 4096):
+        for method in (P.METHOD_TWO_PASS, P.METHOD_UPPER_BOUND):
+            check_vs_oracle(ctx, A, B, f"lds={lds}", lds_hash_max=lds, method=method)
+
+
+def test_two_tensor_contraction(ctx):
+    """A 3-mode, B 3-mode, contract A{1,2} with B{0,1} (the two-tensor form)."""
+    ha, va = P.gen_coo(3, [400, 300, 60], 60000, P.DIST_ZIPF)
+    hb, vb = P.gen_coo(4, [300, 60, 500], 60000, P.DIST_ZIPF)
+    A = P.make_free_first(P.Tensor(ha, va, None, np.array([1, 2, 0], np.uint32), 2))
+    B = P.Tensor(hb, vb, None, np.array([0, 1, 2], np.uint32), 2)
+    for m in METHODS.values():
+        check_vs_oracle(ctx, A, B, "two-tensor", method=m)
+
+
+def test_contract_and_matricized_entry_points(ctx):
+    case = CASES["two_perm"]
+    from tests.helpers import meta_tensors
+
+    A, B = meta_tensors(case)
+    for fn in (P.contract, P.matricize_spGEMM, P.matricize_sort_spGEMM):
+        O = fn(A, B, ctx)
+        h, v = sort_coo(O.hedges_array, O.val_array)
+        assert np.array_equal(h, case["coords"])
+        assert max_rel(v, case["vals_dense"]) <= TOL
+
+
+def test_empty_contraction_raises(ctx):
+    A = P.Tensor(np.array([[0, 1], [1, 2]], np.uint32), np.array([1.0, 2.0]), None, np.array([1, 0], np.uint32), 1)
+    B = P.Tensor(np.array([[5, 1], [6, 2]], np.uint32), np.array([1.0, 2.0]), None, np.array([0, 1], np.uint32), 1)
+    with pytest.raises(P.PtcError) as e:
+        run_gpu(ctx, A, B)
+    assert e.value.code == -4
+
+
+def test_keys_outside_b_range_miss(ctx):
+    """A contracted indices beyond B's range cannot match (no error)."""
+    A = P.Tensor(np.array([[0, 1], [1, 200], [2, 1]], np.uint32), np.array([1.0, 2.0, 3.0]), None,
+                 np.array([0, 1], np.uint32), 1)
+    B = P.Tensor(np.array([[1, 4], [1, 5]], np.uint32), np.array([0.5, 0.25]), None, np.array([0, 1], np.uint32), 1)
+    h, v, st = run_gpu(ctx, A, B)
+    assert h.tolist() == [[0, 4], [0, 5], [2, 4], [2, 5]]
+    assert np.array_equal(v, [0.5, 0.25, 1.5, 0.75])
+
+
+def test_hashtable_probe_groups_like_oracle(ctx):
+    h, v = P.gen_coo(9, [50, 60, 70], 20000, P.DIST_ZIPF)
+    B = P.Tensor(h, v, None, np.array([1, 2, 0], np.uint32), 2)
+    HB = P.buildHashTable(B, ctx)
+    keys = h[:, [1, 2]]
+    uniq, inv, cnt = np.unique(keys, axis=0, return_inverse=True, return_counts=True)
+    nk, n = HB.info()
+    assert nk == uniq.shape[0] and n == h.shape[0]
+    sel = np.arange(0, uniq.shape[0], max(1, uniq.shape[0] // 500))
+    got = HB.find(uniq[sel])
+    inv = inv.reshape(-1)
+    for j, s in enumerate(sel):
+        assert sorted(got[j].tolist()) == np.nonzero(inv == s)[0].tolist()
+    # absent keys
+    lens = HB.find(np.array([[49, 69], [1000, 1]], np.uint32), with_ids=False)
+    assert lens[1] == 0
+
+
+def test_repeat_runs_identical_coordinates(ctx):
+    h, v = P.gen_coo(21, [300, 200, 100], 50000, P.DIST_ZIPF)
+    A, B = P.self_contraction_inputs(h, v, [1, 2])
+    h1, v1, _ = run_gpu(ctx, A, B)
+    h2, v2, _ = run_gpu(ctx, A, B, method=P.METHOD_UPPER_BOUND)
+    assert np.array_equal(h1, h2)
+    assert max_rel(v1, v2) <= TOL
