@@ -1,0 +1,252 @@
+"""Benchmark of the hot path: the nips-shape 4-mode self-contraction.
+
+Workload (one GPU): a seeded 4-mode COO tensor, dims 2482 x 2862 x 14036 x 17,
+3,101,609 unique nonzeros with coordinates drawn per mode from Zipf(1.1) and
+values U(0,1] in FP64, contracted with itself over modes {1,2,3} into a sparse
+2482 x 2482 output.
+
+A step is one whole contraction from COO tensors resident in HBM to the COO
+output in HBM: the two key tables (B on its contracted modes, A on its free
+modes) and the contraction itself -- what main.cpp:115-118 times as HT_B
+insertion + HT_A insertion + tc_fksCuckoo.  The value is useful multiply-adds
+(F = sum_k |A_k| |B_k|) per second of step time.
+
+Printed: one JSON line (rank 0) with the contract's fields plus
+  methods      each method's step time and MAD/s (two-pass exact, upper-bound
+               single pass, learned single pass when a model is available);
+  roofline     the dominant kernel (the numeric row pass) against HBM peak:
+               algorithmic bytes per launch / HIP-event kernel duration;
+  cpu_baseline the oracle restatement of the reference (one thread, as the
+               reference's numeric loop runs) on the same tensor.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+DIMS = [2482, 2862, 14036, 17]
+NNZ = 3101609
+CONTRACTED = [1, 2, 3]
+SEED = 2
+WORKLOAD = ("nips-shape 4-mode self-contraction: dims 2482x2862x14036x17, 3101609 unique nnz, "
+            "Zipf(1.1) per mode, FP64 U(0,1], contract modes {1,2,3} -> 2482x2482")
+
+
+def algorithmic_bytes(nnz_a, d_a, c, d_b, F, nnz_c, d_o):
+    """Slice-streaming model of the contraction: read A once, one key probe
+    per A nonzero, stream one B-slice entry (free index + value) per
+    multiply-add, write C once."""
+    return (nnz_a * (4 * d_a + 8) + nnz_a * (4 * c + 8) + F * (4 * (d_b - c) + 8) + nnz_c * (4 * d_o + 8))
+
+
+def kernel_bytes(npos, F, d_b, c, nnz_c, d_o):
+    """Algorithmic bytes of one numeric row pass: per A position its flop
+    prefix (8), slice offset (4) and value (8); per multiply-add one B entry
+    (column key 4*(d_B-c) + value 8); per output entry its coordinates and
+    value."""
+    return npos * 20 + F * (4 * (d_b - c) + 8) + nnz_c * (4 * d_o + 8)
+
+
+def run_method(P, ctx, dA, dB, method, steps, warmup):
+    """Times `steps` whole contractions; returns per-step ms and stats."""
+    import ctypes as C
+
+    L = P.lib()
+    opts = P.options(method=method)
+    st = P.Stats()
+
+    def one():
+        hb, ha, out = C.c_void_p(), C.c_void_p(), C.c_void_p()
+        rc = L.ptc_hashtable_build(ctx.handle, dB._h, C.byref(hb))
+        if rc == 0:
+            rc = L.ptc_hashtable_build(ctx.handle, dA._h, C.byref(ha))
+        if rc == 0:
+            rc = L.ptc_tc_fksCuckoo(ctx.handle, ha, hb, dA._h, dB._h, C.byref(opts), C.byref(out), C.byref(st))
+        L.ptc_dtensor_free(out)
+        L.ptc_hashtable_free(ha)
+        L.ptc_hashtable_free(hb)
+        if rc != 0:
+            raise P.PtcError(rc, ctx.last_error())
+
+    for _ in range(warmup):
+        one()
+    ctx.synchronize()
+    kern_ms, kern_launch = 0.0, 0
+    t0 = time.perf_counter()
+    per = []
+    for _ in range(steps):
+        s0 = time.perf_counter()
+        one()
+        per.append(time.perf_counter() - s0)
+        kern_ms += st.ms_kernel_numeric
+        kern_launch += st.kernel_launches_numeric
+    ctx.synchronize()
+    total = time.perf_counter() - t0
+    return {
+        "ms_per_step": 1e3 * total / steps,
+        "ms_min": 1e3 * min(per),
+        "stats": st.as_dict(),
+        "kernel_ms_per_launch": kern_ms / max(kern_launch, 1),
+        "kernel_launches": kern_launch,
+    }
+
+
+def cpu_baseline(P, h, v):
+    """The oracle restatement of the reference on this host: the whole
+    contraction, one thread (the reference's numeric loop runs a team of
+    one, fksCuckooTC.hpp:171-173)."""
+    from oracle import orc
+
+    A, B = P.self_contraction_inputs(h, v, CONTRACTED)
+    t0 = time.perf_counter()
+    oh, ov, info = orc.tc_fksCuckoo(A, B)
+    wall = time.perf_counter() - t0
+    F = info["flops"]
+    return {
+        "value": F / wall,
+        "unit": "MAD/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": (f"whole workload (all {info['rows']} rows), single thread: HT_B {info['t_build_b']:.2f}s, "
+                   f"HT_A {info['t_build_a']:.2f}s, preprocessing {info['t_preprocess']:.2f}s, "
+                   f"contraction {info['t_contract']:.2f}s; nnz_out {oh.shape[0]}"),
+        "nnz_out": int(oh.shape[0]),
+        "seconds": wall,
+    }
+
+
+def load_traffic():
+    f = ROOT / "profiles" / "traffic_numeric_pass_nips.json"
+    if f.exists():
+        try:
+            return json.loads(f.read_text())
+        except Exception:
+            return None
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--methods", default="two_pass,upper_bound,learned")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the contraction shards by output "
+              "row; multi-GPU runs are not part of this build)", file=sys.stderr)
+        sys.exit(2)
+    rank = int(os.environ.get("RANK", "0"))
+
+    import partc_amd as P
+
+    h, v = P.gen_coo(SEED, DIMS, NNZ, P.DIST_ZIPF, 1.1)
+    A, B = P.self_contraction_inputs(h, v, CONTRACTED)
+    ctx = P.Context(0)
+    dA = P.DeviceTensor.upload(A, ctx)
+    dB = P.DeviceTensor.upload(B, ctx)
+
+    avail = {"two_pass": P.METHOD_TWO_PASS, "upper_bound": P.METHOD_UPPER_BOUND}
+    try:
+        if P.learned_available():
+            avail["learned"] = P.METHOD_LEARNED
+    except AttributeError:
+        pass
+    results = {}
+    for name in args.methods.split(","):
+        if name in avail:
+            results[name] = run_method(P, ctx, dA, dB, avail[name], args.steps, args.warmup)
+            print(f"# {name}: {results[name]['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
+
+    head = "learned" if "learned" in results else ("two_pass" if "two_pass" in results else next(iter(results)))
+    r = results[head]
+    st = r["stats"]
+    F = int(st["flops"])
+    nnz_c = int(st["nnz_out"])
+    d, c = 4, 3
+    d_o = (d - c) * 2
+    value = F / (r["ms_per_step"] * 1e-3)
+    alg = algorithmic_bytes(NNZ, d, c, d, F, nnz_c, d_o)
+    kb = kernel_bytes(NNZ, F, d, c, nnz_c, d_o)
+    kms = r["kernel_ms_per_launch"]
+    achieved = kb / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
+    traffic = load_traffic()
+    roof = {
+        "kernel": "k_pass<NUMERIC,dense> (numeric row pass)",
+        "bound": "hbm",
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": traffic.get("bytes_per_launch") if traffic else None,
+        "algorithmic_bytes_per_launch": int(kb),
+        "kernel_ms": round(kms, 4),
+    }
+    out = {
+        "metric": "useful multiply-adds/s, nips-shape 4-mode self-contraction (learned 1-pass vs exact 2-pass)",
+        "value": value,
+        "unit": "MAD/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded generator; no dataset)",
+        "config": {
+            "workload": WORKLOAD,
+            "method": head,
+            "flops": F,
+            "nnz_out": nnz_c,
+            "step": "build HT_B + build HT_A + tc_fksCuckoo, inputs resident in HBM",
+        },
+        "step_hbm": {
+            "algorithmic_bytes": int(alg),
+            "GBps": round(alg / (r["ms_per_step"] * 1e-3) / 1e9, 1),
+            "frac": round(alg / (r["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        },
+        "methods": {
+            k: {
+                "ms_per_step": round(x["ms_per_step"], 4),
+                "MADps": F / (x["ms_per_step"] * 1e-3),
+                "ms_build_b": round(x["stats"]["ms_build_b"], 4),
+                "ms_build_a": round(x["stats"]["ms_build_a"], 4),
+                "ms_preprocess": round(x["stats"]["ms_preprocess"], 4),
+                "ms_symbolic": round(x["stats"]["ms_symbolic"], 4),
+                "ms_numeric": round(x["stats"]["ms_numeric"], 4),
+                "ms_contract": round(x["stats"]["ms_total"], 4),
+                "kernel_numeric_ms": round(x["kernel_ms_per_launch"], 4),
+                "kernel_symbolic_ms": round(x["stats"]["ms_kernel_symbolic"], 4),
+                "fallback_rows": int(x["stats"]["fallback_rows"]),
+                "heavy_rows": int(x["stats"]["heavy_rows"]),
+            }
+            for k, x in results.items()
+        },
+        "roofline": roof,
+    }
+    if "two_pass" in results and head != "two_pass":
+        out["speedup_vs_two_pass"] = results["two_pass"]["ms_per_step"] / r["ms_per_step"]
+    if not args.no_cpu and rank == 0:
+        out["cpu_baseline"] = cpu_baseline(P, h, v)
+        cb = out["cpu_baseline"]
+        if cb["nnz_out"] != nnz_c:
+            out["cpu_baseline"]["note"] = f"nnz mismatch: cpu {cb['nnz_out']} gpu {nnz_c}"
+    if rank == 0:
+        print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

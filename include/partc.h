@@ -164,6 +164,10 @@ typedef struct {
   uint64_t heavy_rows;    /* rows split over several workgroups              */
   uint64_t spill_rows;    /* rows accumulated in HBM                          */
   double ms_build_b, ms_build_a, ms_preprocess, ms_symbolic, ms_numeric, ms_total;
+  /* HIP-event durations of the row-pass kernels alone (k_pass launches on
+   * the context stream), summed over their launches in this call */
+  double ms_kernel_symbolic, ms_kernel_numeric;
+  uint64_t kernel_launches_numeric;
 } ptc_stats;
 
 void ptc_options_default(ptc_options* opt);

@@ -97,6 +97,9 @@ class Stats(C.Structure):
         ("ms_symbolic", C.c_double),
         ("ms_numeric", C.c_double),
         ("ms_total", C.c_double),
+        ("ms_kernel_symbolic", C.c_double),
+        ("ms_kernel_numeric", C.c_double),
+        ("kernel_launches_numeric", C.c_uint64),
     ]
 
     def as_dict(self) -> dict:
@@ -501,3 +504,13 @@ def self_contraction_inputs(h: np.ndarray, v: np.ndarray, contracted):
     perm = np.array(contracted + free, dtype=np.uint32)
     A = Tensor(h, v, None, perm, len(contracted))
     return make_free_first(A), get_sparse_tensor(A)
+
+
+def learned_available() -> bool:
+    """True when a trained row-size model is loaded into the library."""
+    L = lib()
+    f = getattr(L, "ptc_learned_default", None)
+    if f is None:
+        return False
+    f.restype = C.c_void_p
+    return bool(f())

@@ -87,6 +87,7 @@ struct PassParams {
   // B slices
   const void* bcol;
   const double* bval;
+  uint64_t nb;                        // number of B nonzeros
   uint64_t ncols;                     // column space (dense accumulators)
   // heavy dense rows
   uint32_t* hbits;

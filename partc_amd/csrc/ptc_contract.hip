@@ -383,7 +383,15 @@ __global__ void __launch_bounds__(PB) k_pass(PassParams P) {
         if (s_start[mid] <= f) lo = mid; else hi = mid - 1;
       }
       const long long b = s_bias[lo] + (long long)f;
+      if ((unsigned long long)b >= P.nb) {  // cannot happen for consistent tables
+        atomicOr(P.overflow, 0x40000000u);
+        continue;
+      }
       const ColT col = bcol[b];
+      if (DENSE && (uint64_t)col >= ncols) {
+        atomicOr(P.overflow, 0x40000000u);
+        continue;
+      }
       if (DENSE) {
         if (PASS == PASS_SYMBOLIC) {
           atomicOr(&bits[col >> 5], 1u << (col & 31));
@@ -516,6 +524,8 @@ __global__ void __launch_bounds__(PB) k_pass(PassParams P) {
     }
   }
   if (full || total > cap) return;
+  // never write past the row's allocation (exact mode: the symbolic count)
+  const uint32_t limit = min(total, P.out_cap ? cap : P.row_nnz[r]);
   const uint64_t base = P.out_base[r];
   uint32_t run = 0;
   for (uint64_t c0 = 0; c0 < span; c0 += PB) {
@@ -540,7 +550,7 @@ __global__ void __launch_bounds__(PB) k_pass(PassParams P) {
     }
     uint32_t tile_total;
     uint32_t rank = block_rank(f, s_w, &tile_total);
-    if (f && run + rank < total) emit_entry<ColT>(P, base + run + rank, rowc, col, v);
+    if (f && run + rank < limit) emit_entry<ColT>(P, base + run + rank, rowc, col, v);
     run += tile_total;
   }
 }
@@ -627,8 +637,15 @@ template <int PASS, bool DENSE, class ColT>
 static int launch_pass_t(ptc_ctx* ctx, const PassParams& P, uint64_t nitems, size_t lds) {
   auto kern = k_pass<PASS, DENSE, ColT>;
   PTC_HIP(ctx, hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  const bool timed = ctx->nkev + 2 <= 16;
+  if (timed) hipEventRecord(ctx->kev[ctx->nkev], ctx->stream);
   hipLaunchKernelGGL(kern, dim3((unsigned)nitems), dim3(PB), lds, ctx->stream, P);
   PTC_HIP(ctx, hipGetLastError());
+  if (timed) {
+    hipEventRecord(ctx->kev[ctx->nkev + 1], ctx->stream);
+    ctx->kev_pass[ctx->nkev / 2] = PASS;
+    ctx->nkev += 2;
+  }
   return PTC_OK;
 }
 

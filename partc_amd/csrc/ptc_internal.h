@@ -77,6 +77,9 @@ struct ptc_ctx {
   std::map<void*, size_t> live;
   size_t cached_bytes = 0;
   hipEvent_t ev[8] = {};
+  hipEvent_t kev[16] = {};   // kernel-level timing events
+  int nkev = 0;
+  int kev_pass[8] = {};
   // pinned host staging for small device->host reads
   void* host_pinned = nullptr;
   size_t host_pinned_bytes = 0;

@@ -301,6 +301,7 @@ extern "C" int ptc_create(ptc_ctx** out, int device) {
     return PTC_ERR_HIP;
   }
   for (auto& e : ctx->ev) hipEventCreate(&e);
+  for (auto& e : ctx->kev) hipEventCreate(&e);
   *out = ctx;
   return PTC_OK;
 }
@@ -312,6 +313,8 @@ extern "C" void ptc_destroy(ptc_ctx* ctx) {
   for (auto& kv : ctx->free_blocks) hipFree(kv.second);
   for (auto& kv : ctx->live) hipFree(kv.first);
   for (auto& e : ctx->ev)
+    if (e) hipEventDestroy(e);
+  for (auto& e : ctx->kev)
     if (e) hipEventDestroy(e);
   if (ctx->host_pinned) hipHostFree(ctx->host_pinned);
   hipStreamDestroy(ctx->stream);

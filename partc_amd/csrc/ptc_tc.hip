@@ -85,6 +85,16 @@ __global__ void k_merge_hbm_caps(uint64_t n, uint32_t* hcap, unsigned long long*
     }
 }
 
+// Dynamic LDS a workgroup may use on this device.
+static uint64_t lds_limit(ptc_ctx* ctx) {
+  int optin = 0;
+  if (hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, ctx->device) != hipSuccess || optin <= 0) {
+    (void)hipGetLastError();
+    optin = (int)ctx->prop.sharedMemPerBlock;
+  }
+  return (uint64_t)optin > 4096 ? (uint64_t)optin - 1024 : 65536 - 1024;  // headroom for static LDS
+}
+
 static int plan_hash(Run& R, HashPlan& hp, const uint32_t* bound32, const unsigned long long* bound64,
                      const unsigned long long* hbm_bound) {
   ptc_ctx* ctx = R.ctx;
@@ -148,6 +158,7 @@ static void base_params(Run& R, PassParams& P, const HashPlan* hp) {
   P.aval = R.HA->gval;
   P.bcol = R.col64 ? (const void*)R.HB->rest_key64 : (const void*)R.HB->rest_key32;
   P.bval = R.HB->gval;
+  P.nb = R.HB->t->N;
   P.ncols = R.ncols;
   P.hbits = R.hbits;
   P.hacc = R.hacc;
@@ -277,9 +288,13 @@ extern "C" int ptc_tc_fksCuckoo(ptc_ctx* ctx, const ptc_hashtable* HT_A, const p
   R.nrowmodes = A->nk;
   R.dO = A->nk + (B->d - B->nk);  // tensor.hpp:113
   if (R.dO == 0) return set_err(ctx, PTC_ERR_EMPTY, "tensor contraction not possible!! (output has no modes)");
-  uint32_t dense_max = opt.dense_max_cols ? opt.dense_max_cols : DEFAULT_DENSE_MAX;
-  dense_max = std::min<uint32_t>(dense_max, 16384);
-  R.dense = opt.acc_mode == PTC_ACC_DENSE ? R.ncols <= 16384
+  // largest dense column space whose numeric pass fits the workgroup's LDS
+  const uint64_t lds_cap = lds_limit(ctx);
+  uint64_t dense_fit = 0;
+  while (dense_fit < 65536 && pass_lds_bytes(PASS_NUMERIC, true, false, dense_fit + 256, 0) <= lds_cap) dense_fit += 256;
+  uint64_t dense_max = opt.dense_max_cols ? opt.dense_max_cols : DEFAULT_DENSE_MAX;
+  dense_max = std::min<uint64_t>(dense_max, dense_fit);
+  R.dense = opt.acc_mode == PTC_ACC_DENSE ? R.ncols <= dense_fit
           : opt.acc_mode == PTC_ACC_AUTO ? R.ncols <= dense_max : false;
   if (opt.acc_mode == PTC_ACC_DENSE && !R.dense) return set_err(ctx, PTC_ERR_ARG, "column space too large for a dense accumulator");
   R.lds_slots = opt.lds_hash_max ? opt.lds_hash_max : DEFAULT_LDS_HASH;
@@ -288,10 +303,12 @@ extern "C" int ptc_tc_fksCuckoo(ptc_ctx* ctx, const ptc_hashtable* HT_A, const p
     uint32_t p2 = 16;
     while (p2 * 2 <= R.lds_slots && p2 < 8192) p2 <<= 1;
     R.lds_slots = R.lds_slots ? p2 : 0;
+    while (R.lds_slots > 16 && pass_lds_bytes(PASS_NUMERIC, false, R.col64, 0, R.lds_slots) > lds_cap) R.lds_slots >>= 1;
   }
   const uint64_t chunk = opt.chunk_flops ? opt.chunk_flops : DEFAULT_CHUNK;
 
   Timer T(ctx);
+  ctx->nkev = 0;
   auto wall0 = std::chrono::steady_clock::now();
   T.mark();  // 0
   int rc = PTC_OK;
@@ -548,6 +565,7 @@ extern "C" int ptc_tc_fksCuckoo(ptc_ctx* ctx, const ptc_hashtable* HT_A, const p
   uint32_t ovf = 0;
   rc = read_back(ctx, &ovf, R.overflow, 4);
   if (!rc && (ovf & 0x80000000u)) rc = set_err(ctx, PTC_ERR_INTERNAL, "row count mismatch between passes");
+  if (!rc && (ovf & 0x40000000u)) rc = set_err(ctx, PTC_ERR_INTERNAL, "slice index out of range");
   if (rc) {
     ptc_dtensor_free(out);
     return rc;
@@ -572,6 +590,16 @@ extern "C" int ptc_tc_fksCuckoo(ptc_ctx* ctx, const ptc_hashtable* HT_A, const p
       stats->ms_numeric = T.ms(1, 3);
     }
     stats->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
+    for (int i = 0; i + 1 < ctx->nkev; i += 2) {
+      float t = 0.f;
+      hipEventElapsedTime(&t, ctx->kev[i], ctx->kev[i + 1]);
+      if (ctx->kev_pass[i / 2] == PASS_NUMERIC) {
+        stats->ms_kernel_numeric += t;
+        stats->kernel_launches_numeric++;
+      } else {
+        stats->ms_kernel_symbolic += t;
+      }
+    }
   }
   *O = out;
   return PTC_OK;

@@ -97,7 +97,7 @@ def test_heavy_rows_split(ctx, chunk, acc):
 
 def test_hash_lds_sizes(ctx):
     """Wide column space with tiny LDS tables: exercises LDS and HBM rows."""
-    h, v = P.gen_coo(78, [30, 400, 5000], 30000, P.DIST_ZIPF)
+    h, v = P.gen_coo(78, [300, 400, 5000], 6000, P.DIST_ZIPF)
     A, B = P.self_contraction_inputs(h, v, [0])
     for lds in (16, 64, 4096):
         for method in (P.METHOD_TWO_PASS, P.METHOD_UPPER_BOUND):
