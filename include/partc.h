@@ -200,6 +200,44 @@ int ptc_matricize_spGEMM(ptc_ctx* ctx, const ptc_dtensor* A, const ptc_dtensor* 
 int ptc_matricize_sort_spGEMM(ptc_ctx* ctx, const ptc_dtensor* A, const ptc_dtensor* B,
                               ptc_dtensor** O, ptc_stats* stats);
 
+/* ------------------------------------------------ learned row-size model */
+
+/* A 3-layer perceptron (PTC_MLP_IN -> 64 -> 64 -> 1) that predicts each
+ * output row's nonzeros from features known after the key probes, so the
+ * LEARNED method can size accumulators and output rows without the symbolic
+ * pass.  No reference counterpart: the reference over-allocates every row to
+ * its flop bound (fksCuckooTC.hpp:156, :200-205).  Training and inference run
+ * on the GPU (hand-written kernels). */
+#define PTC_MLP_IN 6
+#define PTC_MLP_HIDDEN 64
+#define PTC_MLP_NPARAM (PTC_MLP_HIDDEN * PTC_MLP_IN + PTC_MLP_HIDDEN + PTC_MLP_HIDDEN * PTC_MLP_HIDDEN + \
+                        PTC_MLP_HIDDEN + PTC_MLP_HIDDEN + 1)
+int ptc_mlp_create(ptc_ctx* ctx, uint64_t seed, ptc_mlp** out);
+void ptc_mlp_free(ptc_mlp* m);
+int ptc_mlp_num_params(void);
+int ptc_mlp_get_params(const ptc_mlp* m, float* host);
+int ptc_mlp_set_params(ptc_mlp* m, const float* host);
+/* Loss (log pred - log nnz)^2 averaged over n samples and its gradient
+ * (host arrays: X n x PTC_MLP_IN, ub and y n). */
+int ptc_mlp_grad(ptc_ctx* ctx, ptc_mlp* m, const float* X, const float* ub, const float* y, uint64_t n,
+                 float* grad_out, float* loss_out);
+/* Adam over shuffled mini-batches; loss_out (optional) gets each epoch's mean loss. */
+int ptc_mlp_train(ptc_ctx* ctx, ptc_mlp* m, const float* X, const float* ub, const float* y, uint64_t n,
+                  uint32_t epochs, uint32_t batch, float lr, uint64_t seed, float* loss_out);
+int ptc_mlp_predict(ptc_ctx* ctx, const ptc_mlp* m, const float* X, const float* ub, uint64_t n,
+                    float* pred_out);
+int ptc_mlp_save(const ptc_mlp* m, const char* path);
+int ptc_mlp_load(ptc_ctx* ctx, const char* path, ptc_mlp** out);
+/* The model the LEARNED method uses when ptc_options.mlp is NULL. */
+void ptc_learned_set_default(const ptc_mlp* m);
+const ptc_mlp* ptc_learned_default(void);
+/* Training data of one contraction: per-row features X (rows x
+ * PTC_MLP_IN), upper bounds ub and exact nonzeros from the symbolic pass.
+ * *nrows_out is always set; PTC_ERR_RANGE if cap_rows is too small. */
+int ptc_row_profile(ptc_ctx* ctx, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B,
+                    const ptc_dtensor* A, const ptc_dtensor* B, float* X, float* ub, uint32_t* nnz,
+                    uint64_t cap_rows, uint64_t* nrows_out);
+
 /* ------------------------------------------------------ synthetic tensors */
 
 enum { PTC_DIST_UNIFORM = 0, PTC_DIST_ZIPF = 1 };

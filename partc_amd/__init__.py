@@ -507,10 +507,149 @@ def self_contraction_inputs(h: np.ndarray, v: np.ndarray, contracted):
 
 
 def learned_available() -> bool:
-    """True when a trained row-size model is loaded into the library."""
+    """True when a trained row-size model is loaded (or loadable) as the
+    LEARNED method's default."""
     L = lib()
-    f = getattr(L, "ptc_learned_default", None)
-    if f is None:
+    L.ptc_learned_default.restype = C.c_void_p
+    if L.ptc_learned_default():
+        return True
+    return load_default_model()
+
+
+# ------------------------------------------------------- learned row sizes
+
+MLP_IN = 6
+MLP_HIDDEN = 64
+MLP_NPARAM = MLP_HIDDEN * MLP_IN + MLP_HIDDEN + MLP_HIDDEN * MLP_HIDDEN + MLP_HIDDEN + MLP_HIDDEN + 1
+DEFAULT_MODEL = _PKG / "models" / "rowsize_mlp.bin"
+
+
+def _mlp_sigs(L):
+    vp, u32, u64, i32, f32 = C.c_void_p, C.c_uint32, C.c_uint64, C.c_int, C.c_float
+    PF = C.POINTER(C.c_float)
+    sig = {
+        "ptc_mlp_create": (i32, [vp, u64, C.POINTER(vp)]),
+        "ptc_mlp_free": (None, [vp]),
+        "ptc_mlp_num_params": (i32, []),
+        "ptc_mlp_get_params": (i32, [vp, PF]),
+        "ptc_mlp_set_params": (i32, [vp, PF]),
+        "ptc_mlp_grad": (i32, [vp, vp, PF, PF, PF, u64, PF, PF]),
+        "ptc_mlp_train": (i32, [vp, vp, PF, PF, PF, u64, u32, u32, f32, u64, PF]),
+        "ptc_mlp_predict": (i32, [vp, vp, PF, PF, u64, PF]),
+        "ptc_mlp_save": (i32, [vp, C.c_char_p]),
+        "ptc_mlp_load": (i32, [vp, C.c_char_p, C.POINTER(vp)]),
+        "ptc_learned_set_default": (None, [vp]),
+        "ptc_learned_default": (vp, []),
+        "ptc_row_profile": (i32, [vp, vp, vp, vp, vp, PF, PF, C.POINTER(C.c_uint32), u64, C.POINTER(C.c_uint64)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+
+
+def _pf(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class RowSizeModel:
+    """The row-size network (ptc_mlp): 6 -> 64 -> 64 -> 1, trained on the GPU."""
+
+    def __init__(self, ctx: Optional[Context] = None, seed: int = 1, handle=None):
+        self.ctx = ctx or default_context()
+        _mlp_sigs(lib())
+        if handle is not None:
+            self._h = handle
+        else:
+            self._h = C.c_void_p()
+            _check(lib().ptc_mlp_create(self.ctx.handle, seed, C.byref(self._h)), self.ctx)
+
+    @classmethod
+    def load(cls, path, ctx: Optional[Context] = None) -> "RowSizeModel":
+        ctx = ctx or default_context()
+        _mlp_sigs(lib())
+        h = C.c_void_p()
+        _check(lib().ptc_mlp_load(ctx.handle, str(path).encode(), C.byref(h)), ctx)
+        return cls(ctx, handle=h)
+
+    def save(self, path) -> None:
+        Path(path).parent.mkdir(parents=True, exist_ok=True)
+        _check(lib().ptc_mlp_save(self._h, str(path).encode()), self.ctx)
+
+    def params(self) -> np.ndarray:
+        w = np.zeros(MLP_NPARAM, dtype=np.float32)
+        _check(lib().ptc_mlp_get_params(self._h, _pf(w)), self.ctx)
+        return w
+
+    def set_params(self, w) -> None:
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        assert w.shape == (MLP_NPARAM,)
+        _check(lib().ptc_mlp_set_params(self._h, _pf(w)), self.ctx)
+
+    @staticmethod
+    def _arrs(X, ub, y=None):
+        X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, MLP_IN)
+        ub = np.ascontiguousarray(ub, dtype=np.float32).reshape(-1)
+        y = None if y is None else np.ascontiguousarray(y, dtype=np.float32).reshape(-1)
+        return X, ub, y
+
+    def grad(self, X, ub, y):
+        X, ub, y = self._arrs(X, ub, y)
+        g = np.zeros(MLP_NPARAM, dtype=np.float32)
+        loss = C.c_float()
+        _check(lib().ptc_mlp_grad(self.ctx.handle, self._h, _pf(X), _pf(ub), _pf(y), X.shape[0], _pf(g),
+                                  C.byref(loss)), self.ctx)
+        return g, float(loss.value)
+
+    def train(self, X, ub, y, epochs=20, batch=4096, lr=1e-3, seed=7):
+        X, ub, y = self._arrs(X, ub, y)
+        losses = np.zeros(max(epochs, 1), dtype=np.float32)
+        _check(lib().ptc_mlp_train(self.ctx.handle, self._h, _pf(X), _pf(ub), _pf(y), X.shape[0], epochs, batch, lr,
+                                   seed, _pf(losses)), self.ctx)
+        return losses[:epochs]
+
+    def predict(self, X, ub):
+        X, ub, _ = self._arrs(X, ub)
+        p = np.zeros(X.shape[0], dtype=np.float32)
+        _check(lib().ptc_mlp_predict(self.ctx.handle, self._h, _pf(X), _pf(ub), X.shape[0], _pf(p)), self.ctx)
+        return p
+
+    def set_default(self) -> None:
+        lib().ptc_learned_set_default(self._h)
+
+    def free(self) -> None:
+        if self._h:
+            lib().ptc_mlp_free(self._h)
+            self._h = C.c_void_p()
+
+
+def row_profile(HT_A: HashTable, HT_B: HashTable):
+    """(X, ub, nnz) per output row: learned features and the exact symbolic
+    counts (training data of the row-size network)."""
+    _mlp_sigs(lib())
+    ctx = HT_A.ctx
+    nrows = C.c_uint64()
+    k, _ = HT_A.info()
+    X = np.zeros((max(k, 1), MLP_IN), dtype=np.float32)
+    ub = np.zeros(max(k, 1), dtype=np.float32)
+    nnz = np.zeros(max(k, 1), dtype=np.uint32)
+    _check(lib().ptc_row_profile(ctx.handle, HT_A._h, HT_B._h, HT_A.tensor._h, HT_B.tensor._h, _pf(X), _pf(ub),
+                                 nnz.ctypes.data_as(C.POINTER(C.c_uint32)), X.shape[0], C.byref(nrows)), ctx)
+    n = int(nrows.value)
+    return X[:n], ub[:n], nnz[:n]
+
+
+_default_model = None
+
+
+def load_default_model(ctx: Optional[Context] = None) -> bool:
+    """Load partc_amd/models/rowsize_mlp.bin (if present) as the model of the
+    LEARNED method."""
+    global _default_model
+    if _default_model is not None:
+        return True
+    if not DEFAULT_MODEL.exists():
         return False
-    f.restype = C.c_void_p
-    return bool(f())
+    _default_model = RowSizeModel.load(DEFAULT_MODEL, ctx)
+    _default_model.set_default()
+    return True

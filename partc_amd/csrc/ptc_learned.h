@@ -18,6 +18,8 @@ struct LearnedInputs {
 
 // pred[r] = predicted nonzeros of row r (device array of nrows floats).
 int learned_predict(ptc_ctx* ctx, const ptc_mlp* mlp, const LearnedInputs& in, float* pred);
+// Feature rows X (nrows x PTC_MLP_IN) and upper bounds ub (device arrays).
+int learned_features(ptc_ctx* ctx, const LearnedInputs& in, float* X, float* ub);
 
 }  // namespace ptc
 

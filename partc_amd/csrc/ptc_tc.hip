@@ -74,6 +74,7 @@ struct Run {
   double* hacc = nullptr;
   uint32_t* hflag = nullptr;
   uint32_t* overflow = nullptr;
+  unsigned long long* out_off = nullptr;
   explicit Run(ptc_ctx* c) : ctx(c), ws(c) {}
 };
 
@@ -261,13 +262,11 @@ extern "C" void ptc_options_default(ptc_options* o) {
   o->predict_scale = 1.15;
 }
 
-extern "C" int ptc_tc_fksCuckoo(ptc_ctx* ctx, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B,
-                                const ptc_dtensor* A, const ptc_dtensor* B, const ptc_options* opt_in,
-                                ptc_dtensor** O, ptc_stats* stats) {
-  if (!ctx || !HT_A || !HT_B || !A || !B || !O) return PTC_ERR_ARG;
-  *O = nullptr;
-  ptc_options opt;
-  if (opt_in) opt = *opt_in; else ptc_options_default(&opt);
+// Validation, accumulator choice and the shared preprocessing: key probes,
+// flop prefix, per-row chunks, work items.
+static int prepare(Run& R, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B, const ptc_dtensor* A,
+                   const ptc_dtensor* B, const ptc_options& opt) {
+  ptc_ctx* ctx = R.ctx;
   if (HT_A->t != A || HT_B->t != B) return set_err(ctx, PTC_ERR_ARG, "tables were not built on these tensors");
   const uint32_t c = A->d - A->nk;
   if (c == 0 || B->nk != c) return set_err(ctx, PTC_ERR_ARG, "A has %u contracted modes, B %u key modes", c, B->nk);
@@ -277,7 +276,6 @@ extern "C" int ptc_tc_fksCuckoo(ptc_ctx* ctx, const ptc_hashtable* HT_A, const p
     return set_err(ctx, PTC_ERR_ARG, "learned method needs a trained row-size model");
   PTC_HIP(ctx, hipSetDevice(ctx->device));
 
-  Run R(ctx);
   R.HA = HT_A;
   R.HB = HT_B;
   R.opt = opt;
@@ -307,10 +305,6 @@ extern "C" int ptc_tc_fksCuckoo(ptc_ctx* ctx, const ptc_hashtable* HT_A, const p
   }
   const uint64_t chunk = opt.chunk_flops ? opt.chunk_flops : DEFAULT_CHUNK;
 
-  Timer T(ctx);
-  ctx->nkev = 0;
-  auto wall0 = std::chrono::steady_clock::now();
-  T.mark();  // 0
   int rc = PTC_OK;
   Workspace& ws = R.ws;
   R.pos_len = ws.get<uint32_t>(R.npos);
@@ -325,9 +319,9 @@ extern "C" int ptc_tc_fksCuckoo(ptc_ctx* ctx, const ptc_hashtable* HT_A, const p
   R.row_flops = ws.get<unsigned long long>(R.nrows);
   R.packed = ws.get<unsigned long long>(R.nrows + 1);
   R.overflow = ws.get<uint32_t>(4);
-  unsigned long long* out_off = ws.get<unsigned long long>(R.nrows + 1);
+  R.out_off = ws.get<unsigned long long>(R.nrows + 1);
   if (!R.pos_len || !R.pos_off || !R.apre || !R.row_hits || !R.row_maxlen || !R.row_nchunk || !R.row_heavy ||
-      !R.row_done || !R.row_nnz || !R.row_flops || !R.packed || !R.overflow || !out_off)
+      !R.row_done || !R.row_nnz || !R.row_flops || !R.packed || !R.overflow || !R.out_off)
     return PTC_ERR_NOMEM;
 
   // ---------------- preprocessing: probe, flop prefix, rows, items
@@ -391,6 +385,26 @@ extern "C" int ptc_tc_fksCuckoo(ptc_ctx* ctx, const ptc_hashtable* HT_A, const p
     Q.items = R.items;
     if ((rc = launch_items(ctx, P, Q))) return rc;
   }
+  return PTC_OK;
+}
+
+extern "C" int ptc_tc_fksCuckoo(ptc_ctx* ctx, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B,
+                                const ptc_dtensor* A, const ptc_dtensor* B, const ptc_options* opt_in,
+                                ptc_dtensor** O, ptc_stats* stats) {
+  if (!ctx || !HT_A || !HT_B || !A || !B || !O) return PTC_ERR_ARG;
+  *O = nullptr;
+  ptc_options opt;
+  if (opt_in) opt = *opt_in; else ptc_options_default(&opt);
+  Run R(ctx);
+  Timer T(ctx);
+  ctx->nkev = 0;
+  auto wall0 = std::chrono::steady_clock::now();
+  T.mark();  // 0
+  int rc = prepare(R, HT_A, HT_B, A, B, opt);
+  if (rc) return rc;
+  Workspace& ws = R.ws;
+  unsigned long long* out_off = R.out_off;
+  unsigned long long hostv[2] = {0, 0};
   T.mark();  // 1
   uint64_t nnz = 0;
   uint64_t fallback_rows = 0;
@@ -643,4 +657,43 @@ extern "C" int ptc_matricize_spGEMM(ptc_ctx* ctx, const ptc_dtensor* A, const pt
 extern "C" int ptc_matricize_sort_spGEMM(ptc_ctx* ctx, const ptc_dtensor* A, const ptc_dtensor* B, ptc_dtensor** O,
                                          ptc_stats* stats) {
   return ptc_contract(ctx, A, B, nullptr, O, stats);
+}
+
+// Per-row training data for the row-size network: the learned features and
+// the exact distinct-column counts of the symbolic pass.
+extern "C" int ptc_row_profile(ptc_ctx* ctx, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B,
+                               const ptc_dtensor* A, const ptc_dtensor* B, float* X, float* ub, uint32_t* nnz,
+                               uint64_t cap_rows, uint64_t* nrows_out) {
+  if (!ctx || !HT_A || !HT_B || !A || !B || !nrows_out) return PTC_ERR_ARG;
+  ptc_options opt;
+  ptc_options_default(&opt);
+  Run R(ctx);
+  int rc = prepare(R, HT_A, HT_B, A, B, opt);
+  if (rc) return rc;
+  *nrows_out = R.nrows;
+  if (R.nrows > cap_rows || !X || !ub || !nnz) return set_err(ctx, PTC_ERR_RANGE, "profile needs %llu rows", (unsigned long long)R.nrows);
+  HashPlan hs;
+  if (!R.dense && (rc = plan_hash(R, hs, nullptr, R.row_flops, nullptr))) return rc;
+  if ((rc = fill_zero(ctx, R.row_nnz, (R.nrows + 1) * 4)) || (rc = fill_zero(ctx, R.row_done, R.nrows * 4)) ||
+      (rc = prepare_heavy(R, PASS_SYMBOLIC)) || (!R.dense && (rc = reset_hash_pool(R, hs, false))))
+    return rc;
+  PassParams P;
+  base_params(R, P, R.dense ? nullptr : &hs);
+  if ((rc = run_pass(R, PASS_SYMBOLIC, &hs, R.items, R.nitems, P))) return rc;
+  float* dX = R.ws.get<float>(R.nrows * PTC_MLP_IN);
+  float* dub = R.ws.get<float>(R.nrows);
+  if (!dX || !dub) return PTC_ERR_NOMEM;
+  LearnedInputs L{};
+  L.nrows = R.nrows;
+  L.rows = HT_A->slices;
+  L.row_flops = R.row_flops;
+  L.row_hits = R.row_hits;
+  L.row_maxlen = R.row_maxlen;
+  L.ncols = R.ncols;
+  if ((rc = learned_features(ctx, L, dX, dub))) return rc;
+  PTC_HIP(ctx, hipMemcpyAsync(X, dX, sizeof(float) * R.nrows * PTC_MLP_IN, hipMemcpyDeviceToHost, ctx->stream));
+  PTC_HIP(ctx, hipMemcpyAsync(ub, dub, sizeof(float) * R.nrows, hipMemcpyDeviceToHost, ctx->stream));
+  PTC_HIP(ctx, hipMemcpyAsync(nnz, R.row_nnz, sizeof(uint32_t) * R.nrows, hipMemcpyDeviceToHost, ctx->stream));
+  PTC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return PTC_OK;
 }
