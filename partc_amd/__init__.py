@@ -20,6 +20,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import weakref
 from dataclasses import dataclass, field
 from pathlib import Path
 from typing import Optional
@@ -254,9 +255,15 @@ class Context:
 
     def __init__(self, device: int = -1):
         self._h = C.c_void_p()
+        # objects holding device memory of this context; freed before it
+        self._children = weakref.WeakSet()
         rc = lib().ptc_create(C.byref(self._h), device)
         if rc != PTC_OK:
             raise PtcError(rc, "ptc_create")
+
+    def adopt(self, obj):
+        self._children.add(obj)
+        return obj
 
     @property
     def handle(self):
@@ -274,6 +281,13 @@ class Context:
 
     def close(self) -> None:
         if self._h:
+            # tables before tensors (a table references its tensor)
+            kids = sorted(list(self._children), key=lambda o: 0 if isinstance(o, HashTable) else 1)
+            for obj in kids:
+                try:
+                    obj.free()
+                except Exception:
+                    pass
             lib().ptc_destroy(self._h)
             self._h = C.c_void_p()
 
@@ -301,6 +315,7 @@ class DeviceTensor:
         self.ctx = ctx
         self._h = handle
         self._keep = keep
+        ctx.adopt(self)
 
     @classmethod
     def upload(cls, t: Tensor, ctx: Optional[Context] = None) -> "DeviceTensor":
@@ -353,6 +368,7 @@ class HashTable:
         self.ctx = ctx
         self._h = handle
         self.tensor = tensor
+        ctx.adopt(self)
 
     def info(self):
         k, n = C.c_uint64(), C.c_uint64()
@@ -563,6 +579,7 @@ class RowSizeModel:
         else:
             self._h = C.c_void_p()
             _check(lib().ptc_mlp_create(self.ctx.handle, seed, C.byref(self._h)), self.ctx)
+        self.ctx.adopt(self)
 
     @classmethod
     def load(cls, path, ctx: Optional[Context] = None) -> "RowSizeModel":
@@ -621,6 +638,12 @@ class RowSizeModel:
         if self._h:
             lib().ptc_mlp_free(self._h)
             self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 def row_profile(HT_A: HashTable, HT_B: HashTable):
