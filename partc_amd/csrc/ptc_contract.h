@@ -29,6 +29,7 @@ struct RowParams {
   uint32_t* row_nchunk;
   unsigned long long* row_flops;
   unsigned long long* packed;     // (heavy << 32) | nchunk, scanned later
+  unsigned long long* slabs;      // nchunk of split rows, 0 otherwise (scanned later)
 };
 
 struct ItemParams {
@@ -82,6 +83,7 @@ struct PassParams {
   uint32_t* row_done;
   // positions (A, row-grouped)
   const unsigned long long* apre;
+  const uint32_t* pos_len;
   const uint32_t* pos_off;
   const double* aval;
   // B slices
@@ -89,10 +91,13 @@ struct PassParams {
   const double* bval;
   uint64_t nb;                        // number of B nonzeros
   uint64_t ncols;                     // column space (dense accumulators)
-  // heavy dense rows
+  // heavy dense rows: symbolic bitmaps (one per heavy row); numeric slabs
+  // (one partial row per chunk, first slab of row r at row_slab[r])
   uint32_t* hbits;
   double* hacc;
-  uint32_t* hflag;
+  unsigned char* hflag;
+  const unsigned long long* row_slab;
+  int use_flags;                      // 1: a flag byte per column marks touches
   // hash accumulators
   const uint32_t* row_hcap;
   const unsigned long long* row_gcap;

@@ -137,6 +137,7 @@ __global__ void k_rows(RowParams P) {
     P.row_nchunk[r] = nch;
     P.row_flops[r] = f;
     P.packed[r] = ((unsigned long long)(nch > 1) << 32) | nch;
+    P.slabs[r] = nch > 1 ? nch : 0ull;
   }
 }
 
@@ -288,10 +289,21 @@ __device__ __forceinline__ long long hbm_insert(ColT* keys, uint32_t mask, ColT 
   return -1;
 }
 
+// Bits of -0.0: the initial value of a dense accumulator entry.  When no
+// product can be +-0 (no zero or tiny input values), an entry still holding
+// exactly these bits after the pass was never touched, because x + (-0.0) = x
+// for every x and a sum of nonzero terms is never -0.0.  Otherwise a flag
+// byte per column records touches.
+#define PTC_NEG_ZERO_BITS 0x8000000000000000ull
+
+__device__ __forceinline__ bool touched(double v) {
+  return (unsigned long long)__double_as_longlong(v) != PTC_NEG_ZERO_BITS;
+}
+
 // LDS layout of one row pass (dynamic shared memory):
 //   stage:  s_start[STAGE] u32, s_bias[STAGE] i64, s_aval[STAGE] f64
-//   dense:  acc[ncols] f64, flag[ncols] u8        (numeric)
-//           bits[ceil(ncols/32)] u32             (symbolic)
+//   dense:  acc[ncols] f64 (+ flag[ncols] u8 when flags are needed)  (numeric)
+//           bits[ceil(ncols/32)] u32                                 (symbolic)
 //   hash:   keys[S] ColT, vals[S] f64             (S = lds_slots)
 template <int PASS, bool DENSE, class ColT>
 __global__ void __launch_bounds__(PB) k_pass(PassParams P) {
@@ -303,7 +315,9 @@ __global__ void __launch_bounds__(PB) k_pass(PassParams P) {
   __shared__ uint32_t s_w[PB / 64];
   __shared__ uint32_t s_misc[4];
   __shared__ uint32_t s_rowc[PTC_MAX_MODES];
+  __shared__ uint32_t s_ns;
 
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const ColT* __restrict__ bcol = (const ColT*)P.bcol;
   const uint4 it = P.items[blockIdx.x];
   const uint32_t r = it.x, k = it.y;
@@ -317,6 +331,7 @@ __global__ void __launch_bounds__(PB) k_pass(PassParams P) {
   const uint32_t heavy = P.row_heavy[r];
   const uint64_t pend = (uint64_t)row.off + row.len;
   const uint64_t ncols = P.ncols;
+  const bool flags = P.use_flags != 0;
 
   // accumulator storage
   double* acc = (double*)acc_base;
@@ -333,10 +348,10 @@ __global__ void __launch_bounds__(PB) k_pass(PassParams P) {
     if (PASS == PASS_SYMBOLIC) {
       for (uint32_t w = threadIdx.x; w < nbits; w += PB) bits[w] = 0;
     } else {
-      for (uint64_t c = threadIdx.x; c < ncols; c += PB) {
-        acc[c] = 0.0;
-        flag[c] = 0;
-      }
+      const double nz = __longlong_as_double((long long)PTC_NEG_ZERO_BITS);
+      for (uint64_t c = threadIdx.x; c < ncols; c += PB) acc[c] = nz;
+      if (flags)
+        for (uint64_t c = threadIdx.x; c < ncols; c += PB) flag[c] = 0;
     }
   } else {
     uint32_t hc = P.row_hcap[r];
@@ -363,63 +378,124 @@ __global__ void __launch_bounds__(PB) k_pass(PassParams P) {
   while (q0 < pend) {
     const unsigned long long qstart = P.apre[q0];
     if (qstart >= g1) break;
-    const uint32_t ns = (uint32_t)min<uint64_t>(STAGE, pend - q0);
-    for (uint32_t i = threadIdx.x; i < ns; i += PB) {
-      uint64_t q = q0 + i;
-      unsigned long long a = P.apre[q];
-      s_start[i] = (uint32_t)((a > g0 ? a : g0) - g0);
-      s_bias[i] = (long long)P.pos_off[q] + (long long)g0 - (long long)a;
-      if (PASS == PASS_NUMERIC) s_aval[i] = P.aval[q];
+    // stage STAGE raw positions, keeping (in order) those with flops in range
+    const uint64_t qlim = min<uint64_t>(pend, q0 + STAGE);
+    uint32_t keep[STAGE / PB];
+    uint32_t nkeep = 0;
+#pragma unroll
+    for (int j = 0; j < STAGE / PB; ++j) {
+      const uint64_t q = q0 + (uint64_t)threadIdx.x * (STAGE / PB) + j;
+      keep[j] = 0;
+      if (q < qlim) {
+        const uint32_t len = P.pos_len[q];
+        const unsigned long long a = P.apre[q];
+        if (len && a + len > g0 && a < g1) {
+          keep[j] = 1;
+          ++nkeep;
+        }
+      }
     }
-    const unsigned long long qend = (q0 + ns < pend) ? P.apre[q0 + ns] : (flo + fr);
+    // block-wide exclusive prefix of the per-thread kept counts
+    uint32_t incl = nkeep;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      uint32_t y = __shfl_up(incl, o, 64);
+      if (lane >= o) incl += y;
+    }
+    if (lane == 63) s_w[wid] = incl;
+    __syncthreads();
+    uint32_t wbase = 0, ns = 0;
+#pragma unroll
+    for (int w = 0; w < PB / 64; ++w) {
+      if (w < wid) wbase += s_w[w];
+      ns += s_w[w];
+    }
+    uint32_t dst = wbase + incl - nkeep;
+#pragma unroll
+    for (int j = 0; j < STAGE / PB; ++j) {
+      if (!keep[j]) continue;
+      const uint64_t q = q0 + (uint64_t)threadIdx.x * (STAGE / PB) + j;
+      const unsigned long long a = P.apre[q];
+      s_start[dst] = (uint32_t)((a > g0 ? a : g0) - g0);
+      s_bias[dst] = (long long)P.pos_off[q] + (long long)g0 - (long long)a;
+      if (PASS == PASS_NUMERIC) s_aval[dst] = P.aval[q];
+      ++dst;
+    }
+    const unsigned long long qend = (qlim < pend) ? P.apre[qlim] : (flo + fr);
     const uint32_t fb = (uint32_t)((qstart > g0 ? qstart : g0) - g0);
     const uint32_t fe = (uint32_t)(min(qend, g1) - g0);
     __syncthreads();
-    for (uint32_t f = fb + threadIdx.x; f < fe; f += PB) {
-      // last staged position whose start <= f
-      uint32_t lo = 0, hi = ns - 1;
-      while (lo < hi) {
-        uint32_t mid = (lo + hi + 1) >> 1;
-        if (s_start[mid] <= f) lo = mid; else hi = mid - 1;
-      }
-      const long long b = s_bias[lo] + (long long)f;
-      if ((unsigned long long)b >= P.nb) {  // cannot happen for consistent tables
-        atomicOr(P.overflow, 0x40000000u);
-        continue;
-      }
-      const ColT col = bcol[b];
-      if (DENSE && (uint64_t)col >= ncols) {
-        atomicOr(P.overflow, 0x40000000u);
-        continue;
-      }
-      if (DENSE) {
-        if (PASS == PASS_SYMBOLIC) {
-          atomicOr(&bits[col >> 5], 1u << (col & 31));
-        } else {
-          const double v = s_aval[lo] * P.bval[b];
-          atomicAdd(&acc[col], v);
-          flag[col] = 1;
+    if (ns) {
+      // each wave expands a contiguous, 64-aligned share of [fb, fe)
+      const uint32_t nf = fe - fb;
+      const uint32_t span = ((nf + 64 * (PB / 64) - 1) / (64 * (PB / 64))) * 64;
+      const uint32_t wb = fb + wid * span;
+      const uint32_t we = min(fe, wb + span);
+      // position of the wave's first flop (searched once, by every lane)
+      uint32_t qa = 0;
+      if (wb < we) {
+        uint32_t lo = 0, hi = ns - 1;
+        while (lo < hi) {
+          uint32_t mid = (lo + hi + 1) >> 1;
+          if (s_start[mid] <= wb) lo = mid; else hi = mid - 1;
         }
-      } else {
-        double v = 0.0;
-        if (PASS == PASS_NUMERIC) v = s_aval[lo] * P.bval[b];
-        if (S) {
-          const int h = lds_insert<ColT>(hkeys, S, col, PASS == PASS_SYMBOLIC ? &s_misc[0] : nullptr);
-          if (h < 0) s_misc[3] = 1u;
-          else if (PASS == PASS_NUMERIC) atomicAdd(&hvals[h], v);
-        } else {
-          const long long h = hbm_insert<ColT>(gkeys, gmask, col, PASS == PASS_SYMBOLIC ? &P.row_nnz[r] : nullptr);
-          if (h < 0) atomicOr(P.overflow, 0x80000000u);  // cannot happen: sized from exact or bound counts
-          else if (PASS == PASS_NUMERIC) unsafeAtomicAdd(&gvals[h], v);
+        qa = lo;
+      }
+      for (uint32_t f0 = wb; f0 < we; f0 += 64) {
+        // every kept position has at least one flop, so the wave's 64 flops
+        // lie in positions [qa, qa + 63]
+        if (qa + 1 < ns && s_start[qa + 1] <= f0) ++qa;
+        const uint32_t f = min(f0 + lane, we - 1);
+        uint32_t i = qa;
+        const uint32_t hi0 = min(qa + 63, ns - 1);
+        if (hi0 > qa && s_start[qa + 1] <= min(f0 + 63, we - 1)) {
+          uint32_t lo = qa, hi = hi0;
+          while (lo < hi) {
+            uint32_t mid = (lo + hi + 1) >> 1;
+            if (s_start[mid] <= f) lo = mid; else hi = mid - 1;
+          }
+          i = lo;
         }
+        const uint32_t iq = __shfl(i, 63, 64);
+        if (f0 + lane < we) {
+          const long long b = s_bias[i] + (long long)f;
+          if ((unsigned long long)b >= P.nb) {  // cannot happen for consistent tables
+            atomicOr(P.overflow, 0x40000000u);
+          } else {
+            const ColT col = bcol[b];
+            if (DENSE) {
+              if ((uint64_t)col >= ncols) {
+                atomicOr(P.overflow, 0x40000000u);
+              } else if (PASS == PASS_SYMBOLIC) {
+                atomicOr(&bits[col >> 5], 1u << (col & 31));
+              } else {
+                const double v = s_aval[i] * P.bval[b];
+                atomicAdd(&acc[col], v);
+                if (flags) flag[col] = 1;
+              }
+            } else {
+              double v = 0.0;
+              if (PASS == PASS_NUMERIC) v = s_aval[i] * P.bval[b];
+              if (S) {
+                const int h = lds_insert<ColT>(hkeys, S, col, PASS == PASS_SYMBOLIC ? &s_misc[0] : nullptr);
+                if (h < 0) s_misc[3] = 1u;
+                else if (PASS == PASS_NUMERIC) atomicAdd(&hvals[h], v);
+              } else {
+                const long long h = hbm_insert<ColT>(gkeys, gmask, col, PASS == PASS_SYMBOLIC ? &P.row_nnz[r] : nullptr);
+                if (h < 0) atomicOr(P.overflow, 0x80000000u);  // cannot happen: sized from exact or bound counts
+                else if (PASS == PASS_NUMERIC) unsafeAtomicAdd(&gvals[h], v);
+              }
+            }
+          }
+        }
+        qa = iq;
       }
     }
     __syncthreads();
-    q0 += ns;
+    q0 = qlim;
   }
 
-  // ---- merge partials of a split row in HBM; the last chunk finishes ----
-  bool finisher = true;
+  // ---- merge partials of a split row; the last chunk finishes ----
   const bool in_hbm = !DENSE && S == 0;
   if (nch > 1 || in_hbm) {
     if (DENSE && heavy != NO_HEAVY) {
@@ -428,13 +504,14 @@ __global__ void __launch_bounds__(PB) k_pass(PassParams P) {
         for (uint32_t w = threadIdx.x; w < nbits; w += PB)
           if (bits[w]) atomicOr(&gb[w], bits[w]);
       } else {
-        double* ga = P.hacc + (uint64_t)heavy * ncols;
-        uint32_t* gf = P.hflag + (uint64_t)heavy * ncols;
-        for (uint64_t c = threadIdx.x; c < ncols; c += PB)
-          if (flag[c]) {
-            unsafeAtomicAdd(&ga[c], acc[c]);
-            __hip_atomic_store(&gf[c], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          }
+        // this chunk's partial row goes to its own slab (plain stores)
+        const uint64_t slab = P.row_slab[r] + k;
+        double* ga = P.hacc + slab * ncols;
+        for (uint64_t c = threadIdx.x; c < ncols; c += PB) ga[c] = acc[c];
+        if (flags) {
+          unsigned char* gf = (unsigned char*)P.hflag + slab * ncols;
+          for (uint64_t c = threadIdx.x; c < ncols; c += PB) gf[c] = flag[c];
+        }
       }
     }
     __threadfence();
@@ -444,21 +521,26 @@ __global__ void __launch_bounds__(PB) k_pass(PassParams P) {
       s_misc[1] = (done == nch - 1) ? 1u : 0u;
     }
     __syncthreads();
-    finisher = s_misc[1] != 0;
-    if (!finisher) return;
+    if (!s_misc[1]) return;
     __threadfence();
-    if (DENSE) {
+    if (DENSE && heavy != NO_HEAVY) {
       // pull the merged row back into LDS
       if (PASS == PASS_SYMBOLIC) {
         const uint32_t* gb = P.hbits + (uint64_t)heavy * nbits;
         for (uint32_t w = threadIdx.x; w < nbits; w += PB)
           bits[w] = __hip_atomic_load(&gb[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       } else {
-        const double* ga = P.hacc + (uint64_t)heavy * ncols;
-        const uint32_t* gf = P.hflag + (uint64_t)heavy * ncols;
+        // sum the chunks' partials in chunk order (deterministic)
+        const uint64_t slab0 = P.row_slab[r];
         for (uint64_t c = threadIdx.x; c < ncols; c += PB) {
-          flag[c] = (unsigned char)__hip_atomic_load(&gf[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          acc[c] = __hip_atomic_load(&ga[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          double s = __longlong_as_double((long long)PTC_NEG_ZERO_BITS);
+          unsigned char fl = 0;
+          for (uint32_t kk = 0; kk < nch; ++kk) {
+            s += __hip_atomic_load(P.hacc + (slab0 + kk) * ncols + c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (flags) fl |= ((const volatile unsigned char*)P.hflag)[(slab0 + kk) * ncols + c];
+          }
+          acc[c] = s;
+          if (flags) flag[c] = fl;
         }
       }
       __syncthreads();
@@ -471,7 +553,7 @@ __global__ void __launch_bounds__(PB) k_pass(PassParams P) {
       uint32_t c = 0;
       for (uint32_t w = threadIdx.x; w < nbits; w += PB) c += __popc(bits[w]);
       for (int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o, 64);
-      if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = c;
+      if (lane == 0) s_w[wid] = c;
       __syncthreads();
       if (threadIdx.x == 0) {
         uint32_t t = 0;
@@ -500,13 +582,13 @@ __global__ void __launch_bounds__(PB) k_pass(PassParams P) {
   uint32_t cnt = 0;
   for (uint64_t c = threadIdx.x; c < span; c += PB) {
     bool f;
-    if (DENSE) f = flag[c] != 0;
+    if (DENSE) f = flags ? flag[c] != 0 : touched(acc[c]);
     else if (S) f = hkeys[c] != ColEmpty<ColT>::v;
     else f = __hip_atomic_load(&gkeys[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != ColEmpty<ColT>::v;
     cnt += f ? 1u : 0u;
   }
   for (int o = 32; o > 0; o >>= 1) cnt += __shfl_down(cnt, o, 64);
-  if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = cnt;
+  if (lane == 0) s_w[wid] = cnt;
   __syncthreads();
   uint32_t total = 0;
   for (int w = 0; w < PB / 64; ++w) total += s_w[w];
@@ -535,9 +617,9 @@ __global__ void __launch_bounds__(PB) k_pass(PassParams P) {
     double v = 0.0;
     if (c < span) {
       if (DENSE) {
-        f = flag[c] != 0;
-        col = (ColT)c;
         v = acc[c];
+        f = flags ? flag[c] != 0 : touched(v);
+        col = (ColT)c;
       } else if (S) {
         col = hkeys[c];
         f = col != ColEmpty<ColT>::v;
@@ -626,7 +708,7 @@ size_t pass_lds_bytes(int pass, bool dense, bool col64, uint64_t ncols, uint32_t
   size_t b = (size_t)STAGE * 20;
   if (dense) {
     if (pass == PASS_SYMBOLIC) b += ((ncols + 31) / 32) * 4;
-    else b += ncols * 9;
+    else b += ncols * 9;  // accumulator + optional flag byte
   } else {
     b += (size_t)lds_slots * ((col64 ? 8 : 4) + (pass == PASS_NUMERIC ? 8 : 0));
   }

@@ -98,6 +98,9 @@ struct ptc_dtensor {
   float* vals32 = nullptr;                // device, N (FP32 results)
   float* err32 = nullptr;                 // device, N: compensation term of FP32 results
   bool owns = true;
+  // every value finite with magnitude >= 2^-500, so no product of two values
+  // is +-0 (lets dense accumulators detect touched columns without flags)
+  bool vals_safe = false;
 };
 
 struct ptc_hashtable {

@@ -10,8 +10,9 @@
 //      open-addressing table of 16-byte entries by a cooperative group of
 //      KT_GROUP lanes that reads one 128-byte line of 8 entries at a time,
 //      ballots for a match or a free entry and claims it with atomicCAS;
-//      the group leader then takes the nonzero's rank in its slice with an
-//      atomicAdd on the slot's counter;
+//      a second kernel ranks every nonzero within its slot, aggregating a
+//      block's nonzeros per slot in LDS so that a hot key costs one global
+//      atomicAdd per block instead of one per nonzero;
 //   2. pack + scan: slot counts and occupancy are packed into one 64-bit
 //      word ((occupied << 32) | count) and prefix-summed in one scan, which
 //      yields each slice's offset and its compact slice index together;
@@ -63,8 +64,7 @@ __device__ __forceinline__ unsigned long long load_key_l2(const KtEntry* e) {
 // Pass 1.  One group of KT_GROUP lanes per nonzero.
 __global__ void __launch_bounds__(KT_THREADS) k_kt_insert(
     const uint32_t* __restrict__ h, uint32_t d, uint64_t N, KeyCodec codec,
-    KtEntry* __restrict__ tab, uint64_t nbuckets, uint32_t* __restrict__ cnt,
-    uint32_t* __restrict__ slot_of, uint32_t* __restrict__ rank_of) {
+    KtEntry* __restrict__ tab, uint64_t nbuckets, uint32_t* __restrict__ slot_of) {
   const int lane = threadIdx.x & 63;
   const int sub = lane & (KT_GROUP - 1);
   const int gbase = lane - sub;  // first lane of this group in the wave
@@ -99,11 +99,61 @@ __global__ void __launch_bounds__(KT_THREADS) k_kt_insert(
       }
       // lost the race to a different key: re-read the same line
     }
-    if (sub == 0) {
-      uint32_t r = atomicAdd(&cnt[slot], 1u);
-      slot_of[e] = slot;
-      rank_of[e] = r;
+    if (sub == 0) slot_of[e] = slot;
+  }
+}
+
+// Pass 1b: rank of every nonzero within its slot.  A block aggregates its
+// chunk of nonzeros by slot in an LDS table first, so each distinct slot of
+// the chunk costs one global atomicAdd (a Zipf-hot key would otherwise
+// serialise one global atomic per nonzero).
+constexpr int RK_THREADS = 256;
+constexpr int RK_ITEMS = 8;
+constexpr int RK_CHUNK = RK_THREADS * RK_ITEMS;
+constexpr int RK_SLOTS = 2 * RK_CHUNK;
+
+__global__ void __launch_bounds__(RK_THREADS) k_kt_rank(const uint32_t* __restrict__ slot_of, uint64_t N,
+                                                          uint32_t* __restrict__ cnt,
+                                                          uint32_t* __restrict__ rank_of) {
+  __shared__ uint32_t lkey[RK_SLOTS];
+  __shared__ uint32_t lcnt[RK_SLOTS];
+  for (int i = threadIdx.x; i < RK_SLOTS; i += RK_THREADS) {
+    lkey[i] = 0xFFFFFFFFu;
+    lcnt[i] = 0;
+  }
+  __syncthreads();
+  const uint64_t base = (uint64_t)blockIdx.x * RK_CHUNK;
+  uint32_t ls[RK_ITEMS], lr[RK_ITEMS];
+#pragma unroll
+  for (int k = 0; k < RK_ITEMS; ++k) {
+    const uint64_t e = base + (uint64_t)k * RK_THREADS + threadIdx.x;
+    ls[k] = 0xFFFFFFFFu;
+    if (e >= N) continue;
+    const uint32_t s = slot_of[e];
+    uint32_t h = (s * 2654435761u) & (RK_SLOTS - 1);
+    while (true) {
+      uint32_t cur = lkey[h];
+      if (cur == s) break;
+      if (cur == 0xFFFFFFFFu) {
+        uint32_t old = atomicCAS(&lkey[h], 0xFFFFFFFFu, s);
+        if (old == 0xFFFFFFFFu || old == s) break;
+      }
+      h = (h + 1) & (RK_SLOTS - 1);  // at most RK_CHUNK keys in 2*RK_CHUNK slots
     }
+    ls[k] = h;
+    lr[k] = atomicAdd(&lcnt[h], 1u);
+  }
+  __syncthreads();
+  // one global atomic per distinct slot of the chunk; lcnt becomes the base
+  for (int i = threadIdx.x; i < RK_SLOTS; i += RK_THREADS) {
+    const uint32_t s = lkey[i];
+    if (s != 0xFFFFFFFFu) lcnt[i] = atomicAdd(&cnt[s], lcnt[i]);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < RK_ITEMS; ++k) {
+    const uint64_t e = base + (uint64_t)k * RK_THREADS + threadIdx.x;
+    if (e < N) rank_of[e] = lcnt[ls[k]] + lr[k];
   }
 }
 
@@ -255,7 +305,9 @@ extern "C" int ptc_hashtable_build(ptc_ctx* ctx, const ptc_dtensor* t, ptc_hasht
   if (N) {
     unsigned g = grid_for(N, KT_THREADS / KT_GROUP, 16384);
     hipLaunchKernelGGL(k_kt_insert, dim3(g), dim3(KT_THREADS), 0, st, t->hedges, t->d, N, ht->codec,
-                       ht->entries, nb, cnt, slot_of, rank_of);
+                       ht->entries, nb, slot_of);
+    hipLaunchKernelGGL(k_kt_rank, dim3((unsigned)((N + RK_CHUNK - 1) / RK_CHUNK)), dim3(RK_THREADS), 0, st,
+                       (const uint32_t*)slot_of, N, cnt, rank_of);
   }
   hipLaunchKernelGGL(k_kt_pack, dim3(grid_for(cap, 256)), dim3(256), 0, st, cnt, packed, cap);
   rc = scan_u64(ctx, packed, packed, cap, packed + cap);

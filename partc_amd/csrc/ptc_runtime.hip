@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -363,6 +364,14 @@ extern "C" int ptc_dtensor_upload(ptc_ctx* ctx, const ptc_tensor_view* v, ptc_dt
   }
   for (uint64_t i = 0; i < v->N; ++i)
     for (uint32_t j = 0; j < v->d; ++j) t->mode_max[j] = std::max(t->mode_max[j], v->hedges[i * v->d + j]);
+  t->vals_safe = true;
+  for (uint64_t i = 0; i < v->N; ++i) {
+    const double a = std::fabs(v->vals[i]);
+    if (!(a >= 0x1p-500 && a <= 1.7976931348623157e308)) {
+      t->vals_safe = false;
+      break;
+    }
+  }
   if (!v->dims)
     for (uint32_t j = 0; j < v->d; ++j) t->dims[j] = v->N ? t->mode_max[j] + 1 : 0;
   t->hedges = dalloc_n<uint32_t>(ctx, v->N * v->d);

@@ -72,7 +72,10 @@ struct Run {
   // heavy dense merge buffers
   uint32_t* hbits = nullptr;
   double* hacc = nullptr;
-  uint32_t* hflag = nullptr;
+  unsigned char* hflag = nullptr;
+  unsigned long long* row_slab = nullptr;  // first partial-row slab of each split row
+  uint64_t nslabs = 0;
+  bool use_flags = true;
   uint32_t* overflow = nullptr;
   unsigned long long* out_off = nullptr;
   explicit Run(ptc_ctx* c) : ctx(c), ws(c) {}
@@ -155,6 +158,7 @@ static void base_params(Run& R, PassParams& P, const HashPlan* hp) {
   P.row_heavy = R.row_heavy;
   P.row_done = R.row_done;
   P.apre = R.apre;
+  P.pos_len = R.pos_len;
   P.pos_off = R.pos_off;
   P.aval = R.HA->gval;
   P.bcol = R.col64 ? (const void*)R.HB->rest_key64 : (const void*)R.HB->rest_key32;
@@ -164,6 +168,8 @@ static void base_params(Run& R, PassParams& P, const HashPlan* hp) {
   P.hbits = R.hbits;
   P.hacc = R.hacc;
   P.hflag = R.hflag;
+  P.row_slab = R.row_slab;
+  P.use_flags = R.use_flags ? 1 : 0;
   if (hp) {
     P.row_hcap = hp->hcap;
     P.row_gcap = hp->gcap;  // after the scan: offsets; a row is in HBM iff its region is non-empty
@@ -218,11 +224,12 @@ static int prepare_heavy(Run& R, int pass) {
     if (!R.hbits) return PTC_ERR_NOMEM;
     return fill_zero(ctx, R.hbits, R.nheavy * nbits * 4);
   }
-  if (!R.hacc) R.hacc = R.ws.get<double>(R.nheavy * R.ncols);
-  if (!R.hflag) R.hflag = R.ws.get<uint32_t>(R.nheavy * R.ncols);
-  if (!R.hacc || !R.hflag) return PTC_ERR_NOMEM;
-  int rc = fill_zero(ctx, R.hacc, R.nheavy * R.ncols * 8);
-  if (!rc) rc = fill_zero(ctx, R.hflag, R.nheavy * R.ncols * 4);
+  // one slab (partial dense row) per chunk of every split row; each chunk
+  // writes its whole slab, so no clearing is needed
+  if (!R.hacc) R.hacc = R.ws.get<double>(R.nslabs * R.ncols);
+  if (R.use_flags && !R.hflag) R.hflag = R.ws.get<unsigned char>(R.nslabs * R.ncols);
+  if (!R.hacc || (R.use_flags && !R.hflag)) return PTC_ERR_NOMEM;
+  int rc = PTC_OK;
   return rc;
 }
 
@@ -285,6 +292,7 @@ static int prepare(Run& R, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B,
   R.col64 = R.ncols > 0xFFFFFFFFull;
   R.nrowmodes = A->nk;
   R.dO = A->nk + (B->d - B->nk);  // tensor.hpp:113
+  R.use_flags = !(A->vals_safe && B->vals_safe);
   if (R.dO == 0) return set_err(ctx, PTC_ERR_EMPTY, "tensor contraction not possible!! (output has no modes)");
   // largest dense column space whose numeric pass fits the workgroup's LDS
   const uint64_t lds_cap = lds_limit(ctx);
@@ -318,10 +326,11 @@ static int prepare(Run& R, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B,
   R.row_nnz = ws.get<uint32_t>(R.nrows + 1);
   R.row_flops = ws.get<unsigned long long>(R.nrows);
   R.packed = ws.get<unsigned long long>(R.nrows + 1);
+  R.row_slab = ws.get<unsigned long long>(R.nrows + 1);
   R.overflow = ws.get<uint32_t>(4);
   R.out_off = ws.get<unsigned long long>(R.nrows + 1);
   if (!R.pos_len || !R.pos_off || !R.apre || !R.row_hits || !R.row_maxlen || !R.row_nchunk || !R.row_heavy ||
-      !R.row_done || !R.row_nnz || !R.row_flops || !R.packed || !R.overflow || !R.out_off)
+      !R.row_done || !R.row_nnz || !R.row_flops || !R.packed || !R.row_slab || !R.overflow || !R.out_off)
     return PTC_ERR_NOMEM;
 
   // ---------------- preprocessing: probe, flop prefix, rows, items
@@ -353,12 +362,16 @@ static int prepare(Run& R, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B,
     P.row_nchunk = R.row_nchunk;
     P.row_flops = R.row_flops;
     P.packed = R.packed;
+    P.slabs = R.row_slab;
     if ((rc = launch_rows(ctx, P))) return rc;
   }
   if ((rc = scan_u64(ctx, R.packed, R.packed, R.nrows, R.packed + R.nrows))) return rc;
-  unsigned long long hostv[2] = {0, 0};
+  if ((rc = scan_u64(ctx, R.row_slab, R.row_slab, R.nrows, R.row_slab + R.nrows))) return rc;
+  unsigned long long hostv[3] = {0, 0, 0};
   if ((rc = read_back(ctx, &hostv[0], R.apre + R.npos, 8))) return rc;
   if ((rc = read_back(ctx, &hostv[1], R.packed + R.nrows, 8))) return rc;
+  if ((rc = read_back(ctx, &hostv[2], R.row_slab + R.nrows, 8))) return rc;
+  R.nslabs = hostv[2];
   R.F = hostv[0];
   R.nitems = hostv[1] & 0xFFFFFFFFull;
   R.nheavy = hostv[1] >> 32;
