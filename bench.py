@@ -125,6 +125,25 @@ def cpu_baseline(P, h, v):
     }
 
 
+def copy_calibration(mb: int, reps: int = 5) -> None:
+    """A plain device-to-device copy of `mb` MiB (hipMemcpy), for scaling
+    the FETCH_SIZE / WRITE_SIZE counters of rocprofv3 --pmc runs."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    n = mb << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(n)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(n)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(n)) == 0
+    for _ in range(reps):
+        assert hip.hipMemcpy(b, a, C.c_size_t(n), 3) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_calibration_bytes": n, "reps": reps}))
+
+
 def load_traffic():
     f = ROOT / "profiles" / "traffic_numeric_pass_nips.json"
     if f.exists():
@@ -142,7 +161,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--methods", default="two_pass,upper_bound,learned")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        copy_calibration(args.calibrate_copy)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the contraction shards by output "
               "row; multi-GPU runs are not part of this build)", file=sys.stderr)

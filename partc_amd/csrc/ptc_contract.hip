@@ -230,9 +230,9 @@ __device__ __forceinline__ uint32_t block_rank(bool f, uint32_t* s_w, uint32_t* 
   return base + below;
 }
 
-// write one output entry at position pos
+// write one output entry at position pos (row coordinates from LDS)
 template <class ColT>
-__device__ __forceinline__ void emit_entry(const PassParams& P, uint64_t pos, const uint32_t* rowc,
+__device__ __forceinline__ void emit_entry(const PassParams& P, uint64_t pos, const uint32_t* s_rowc,
                                            ColT col, double v) {
   if (P.stg_col) {
     ((ColT*)P.stg_col)[pos] = col;
@@ -240,10 +240,16 @@ __device__ __forceinline__ void emit_entry(const PassParams& P, uint64_t pos, co
     return;
   }
   uint32_t* o = P.out_hedges + pos * P.dO;
-  for (uint32_t j = 0; j < P.nrowmodes; ++j) o[j] = rowc[j];
-  uint32_t cc[PTC_MAX_MODES];
-  decode_tuple((unsigned long long)col, P.colc, cc);
-  for (uint32_t j = 0; j < P.colc.n; ++j) o[P.nrowmodes + j] = cc[j];
+  for (uint32_t j = 0; j < P.nrowmodes; ++j) o[j] = s_rowc[j];
+  o += P.nrowmodes;
+  if (P.colc.n == 1) {
+    o[0] = (uint32_t)col;
+  } else {
+    // constant indices keep the codec in kernel-argument registers
+#pragma unroll
+    for (uint32_t j = 0; j < PTC_MAX_MODES; ++j)
+      if (j < P.colc.n) o[j] = (uint32_t)(((unsigned long long)col / P.colc.stride[j]) % P.colc.radix[j]);
+  }
   P.out_vals[pos] = v;
 }
 
@@ -441,54 +447,81 @@ __global__ void __launch_bounds__(PB) k_pass(PassParams P) {
         }
         qa = lo;
       }
-      for (uint32_t f0 = wb; f0 < we; f0 += 64) {
-        // every kept position has at least one flop, so the wave's 64 flops
-        // lie in positions [qa, qa + 63]
-        if (qa + 1 < ns && s_start[qa + 1] <= f0) ++qa;
-        const uint32_t f = min(f0 + lane, we - 1);
-        uint32_t i = qa;
-        const uint32_t hi0 = min(qa + 63, ns - 1);
-        if (hi0 > qa && s_start[qa + 1] <= min(f0 + 63, we - 1)) {
-          uint32_t lo = qa, hi = hi0;
-          while (lo < hi) {
-            uint32_t mid = (lo + hi + 1) >> 1;
-            if (s_start[mid] <= f) lo = mid; else hi = mid - 1;
+      // U groups of 64 flops per step: positions first (LDS only), then all
+      // global loads, then the accumulations -- 2U loads in flight per wave
+      constexpr int U = 4;
+      for (uint32_t f0 = wb; f0 < we; f0 += 64 * U) {
+        uint32_t ii[U];
+        long long bb[U];
+        bool act[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const uint32_t fu = f0 + 64 * u;
+          act[u] = false;
+          ii[u] = qa;
+          bb[u] = 0;
+          if (fu < we) {  // wave-uniform
+            // every kept position has at least one flop, so the wave's 64
+            // flops lie in positions [qa, qa + 63]
+            if (qa + 1 < ns && s_start[qa + 1] <= fu) ++qa;
+            const uint32_t f = min(fu + lane, we - 1);
+            uint32_t i = qa;
+            const uint32_t hi0 = min(qa + 63, ns - 1);
+            if (hi0 > qa && s_start[qa + 1] <= min(fu + 63, we - 1)) {
+              uint32_t lo = qa, hi = hi0;
+              while (lo < hi) {
+                uint32_t mid = (lo + hi + 1) >> 1;
+                if (s_start[mid] <= f) lo = mid; else hi = mid - 1;
+              }
+              i = lo;
+            }
+            ii[u] = i;
+            bb[u] = s_bias[i] + (long long)f;
+            act[u] = fu + lane < we;
+            if ((unsigned long long)bb[u] >= P.nb) {  // cannot happen for consistent tables
+              if (act[u]) atomicOr(P.overflow, 0x40000000u);
+              act[u] = false;
+            }
+            qa = __shfl(i, 63, 64);
           }
-          i = lo;
         }
-        const uint32_t iq = __shfl(i, 63, 64);
-        if (f0 + lane < we) {
-          const long long b = s_bias[i] + (long long)f;
-          if ((unsigned long long)b >= P.nb) {  // cannot happen for consistent tables
-            atomicOr(P.overflow, 0x40000000u);
-          } else {
-            const ColT col = bcol[b];
-            if (DENSE) {
-              if ((uint64_t)col >= ncols) {
-                atomicOr(P.overflow, 0x40000000u);
-              } else if (PASS == PASS_SYMBOLIC) {
-                atomicOr(&bits[col >> 5], 1u << (col & 31));
-              } else {
-                const double v = s_aval[i] * P.bval[b];
-                atomicAdd(&acc[col], v);
-                if (flags) flag[col] = 1;
-              }
+        ColT col[U];
+        double bv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          col[u] = 0;
+          bv[u] = 0.0;
+          if (act[u]) {
+            col[u] = bcol[bb[u]];
+            if (PASS == PASS_NUMERIC) bv[u] = P.bval[bb[u]];
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          if (!act[u]) continue;
+          const ColT c = col[u];
+          if (DENSE) {
+            if ((uint64_t)c >= ncols) {
+              atomicOr(P.overflow, 0x40000000u);
+            } else if (PASS == PASS_SYMBOLIC) {
+              atomicOr(&bits[c >> 5], 1u << (c & 31));
             } else {
-              double v = 0.0;
-              if (PASS == PASS_NUMERIC) v = s_aval[i] * P.bval[b];
-              if (S) {
-                const int h = lds_insert<ColT>(hkeys, S, col, PASS == PASS_SYMBOLIC ? &s_misc[0] : nullptr);
-                if (h < 0) s_misc[3] = 1u;
-                else if (PASS == PASS_NUMERIC) atomicAdd(&hvals[h], v);
-              } else {
-                const long long h = hbm_insert<ColT>(gkeys, gmask, col, PASS == PASS_SYMBOLIC ? &P.row_nnz[r] : nullptr);
-                if (h < 0) atomicOr(P.overflow, 0x80000000u);  // cannot happen: sized from exact or bound counts
-                else if (PASS == PASS_NUMERIC) unsafeAtomicAdd(&gvals[h], v);
-              }
+              atomicAdd(&acc[c], s_aval[ii[u]] * bv[u]);
+              if (flags) flag[c] = 1;
+            }
+          } else {
+            const double v = PASS == PASS_NUMERIC ? s_aval[ii[u]] * bv[u] : 0.0;
+            if (S) {
+              const int h = lds_insert<ColT>(hkeys, S, c, PASS == PASS_SYMBOLIC ? &s_misc[0] : nullptr);
+              if (h < 0) s_misc[3] = 1u;
+              else if (PASS == PASS_NUMERIC) atomicAdd(&hvals[h], v);
+            } else {
+              const long long h = hbm_insert<ColT>(gkeys, gmask, c, PASS == PASS_SYMBOLIC ? &P.row_nnz[r] : nullptr);
+              if (h < 0) atomicOr(P.overflow, 0x80000000u);  // cannot happen: sized from exact or bound counts
+              else if (PASS == PASS_NUMERIC) unsafeAtomicAdd(&gvals[h], v);
             }
           }
         }
-        qa = iq;
       }
     }
     __syncthreads();
@@ -568,14 +601,13 @@ __global__ void __launch_bounds__(PB) k_pass(PassParams P) {
   }
 
   // NUMERIC: count, check capacity, write
-  if (threadIdx.x < P.nrowmodes) {
-    uint32_t rc[PTC_MAX_MODES];
-    decode_tuple(row.key, P.rowc, rc);
-    s_rowc[threadIdx.x] = rc[threadIdx.x];
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (uint32_t j = 0; j < PTC_MAX_MODES; ++j)
+      if (j < P.nrowmodes)
+        s_rowc[j] = P.rowc.n == 1 ? (uint32_t)row.key : (uint32_t)((row.key / P.rowc.stride[j]) % P.rowc.radix[j]);
   }
-  uint32_t rowc[PTC_MAX_MODES];
   __syncthreads();
-  for (uint32_t j = 0; j < P.nrowmodes; ++j) rowc[j] = s_rowc[j];
 
   const uint64_t span = DENSE ? ncols : (S ? S : (uint64_t)gmask + 1);
   // count
@@ -632,7 +664,7 @@ __global__ void __launch_bounds__(PB) k_pass(PassParams P) {
     }
     uint32_t tile_total;
     uint32_t rank = block_rank(f, s_w, &tile_total);
-    if (f && run + rank < limit) emit_entry<ColT>(P, base + run + rank, rowc, col, v);
+    if (f && run + rank < limit) emit_entry<ColT>(P, base + run + rank, s_rowc, col, v);
     run += tile_total;
   }
 }
@@ -645,23 +677,28 @@ __global__ void __launch_bounds__(PB) k_emit(EmitParams P) {
   const uint32_t n = P.row_nnz[r];
   if (!n || n > P.cap[r]) return;  // overflow rows are redone exactly
   __shared__ uint32_t s_rowc[PTC_MAX_MODES];
-  if (threadIdx.x < P.nrowmodes) {
-    uint32_t rc[PTC_MAX_MODES];
-    decode_tuple(P.rows[r].key, P.rowc, rc);
-    s_rowc[threadIdx.x] = rc[threadIdx.x];
+  if (threadIdx.x == 0) {
+    const unsigned long long key = P.rows[r].key;
+#pragma unroll
+    for (uint32_t j = 0; j < PTC_MAX_MODES; ++j)
+      if (j < P.nrowmodes)
+        s_rowc[j] = P.rowc.n == 1 ? (uint32_t)key : (uint32_t)((key / P.rowc.stride[j]) % P.rowc.radix[j]);
   }
   __syncthreads();
-  uint32_t rowc[PTC_MAX_MODES];
-  for (uint32_t j = 0; j < P.nrowmodes; ++j) rowc[j] = s_rowc[j];
   const uint64_t src = P.cap_base[r], dst = P.out_base[r];
   const ColT* sc = (const ColT*)P.stg_col;
   for (uint32_t i = threadIdx.x; i < n; i += PB) {
-    ColT col = sc[src + i];
+    const ColT col = sc[src + i];
     uint32_t* o = P.out_hedges + (dst + i) * P.dO;
-    for (uint32_t j = 0; j < P.nrowmodes; ++j) o[j] = rowc[j];
-    uint32_t cc[PTC_MAX_MODES];
-    decode_tuple((unsigned long long)col, P.colc, cc);
-    for (uint32_t j = 0; j < P.colc.n; ++j) o[P.nrowmodes + j] = cc[j];
+    for (uint32_t j = 0; j < P.nrowmodes; ++j) o[j] = s_rowc[j];
+    o += P.nrowmodes;
+    if (P.colc.n == 1) {
+      o[0] = (uint32_t)col;
+    } else {
+#pragma unroll
+      for (uint32_t j = 0; j < PTC_MAX_MODES; ++j)
+        if (j < P.colc.n) o[j] = (uint32_t)(((unsigned long long)col / P.colc.stride[j]) % P.colc.radix[j]);
+    }
     P.out_vals[dst + i] = P.stg_val[src + i];
   }
 }
