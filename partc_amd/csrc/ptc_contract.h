@@ -137,7 +137,7 @@ int launch_rows(ptc_ctx* ctx, const RowParams& P);
 int launch_items(ptc_ctx* ctx, const ItemParams& P, const ItemP0Params& Q);
 int launch_hash_caps(ptc_ctx* ctx, const HashCapParams& P);
 int launch_caps(ptc_ctx* ctx, const CapParams& P);
-size_t pass_lds_bytes(int pass, bool dense, bool col64, uint64_t ncols, uint32_t lds_slots);
+size_t pass_lds_bytes(int pass, bool dense, bool col64, uint64_t ncols, uint32_t lds_slots, bool flags = true);
 int launch_pass(ptc_ctx* ctx, int pass, bool dense, bool col64, const PassParams& P, uint64_t nitems,
                 size_t lds);
 int launch_emit(ptc_ctx* ctx, const EmitParams& P, bool col64);

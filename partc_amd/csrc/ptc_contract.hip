@@ -45,8 +45,8 @@
 
 namespace ptc {
 
-constexpr int PB = 256;          // threads per workgroup of the row passes
-constexpr int STAGE = 512;       // A positions staged in LDS at a time
+constexpr int PB = 512;          // threads per workgroup of the row passes
+constexpr int STAGE = 1024;      // A positions staged in LDS at a time (2 per thread)
 constexpr uint32_t NO_HEAVY = 0xFFFFFFFFu;
 
 __device__ __forceinline__ unsigned long long mix64d(unsigned long long k) {
@@ -449,7 +449,7 @@ __global__ void __launch_bounds__(PB) k_pass(PassParams P) {
       }
       // U groups of 64 flops per step: positions first (LDS only), then all
       // global loads, then the accumulations -- 2U loads in flight per wave
-      constexpr int U = 4;
+      constexpr int U = 8;
       for (uint32_t f0 = wb; f0 < we; f0 += 64 * U) {
         uint32_t ii[U];
         long long bb[U];
@@ -741,11 +741,11 @@ int launch_hash_caps(ptc_ctx* ctx, const HashCapParams& P) {
   return PTC_OK;
 }
 
-size_t pass_lds_bytes(int pass, bool dense, bool col64, uint64_t ncols, uint32_t lds_slots) {
+size_t pass_lds_bytes(int pass, bool dense, bool col64, uint64_t ncols, uint32_t lds_slots, bool flags) {
   size_t b = (size_t)STAGE * 20;
   if (dense) {
     if (pass == PASS_SYMBOLIC) b += ((ncols + 31) / 32) * 4;
-    else b += ncols * 9;  // accumulator + optional flag byte
+    else b += ncols * (flags ? 9 : 8);  // accumulator (+ flag byte)
   } else {
     b += (size_t)lds_slots * ((col64 ? 8 : 4) + (pass == PASS_NUMERIC ? 8 : 0));
   }

@@ -175,10 +175,11 @@ __global__ void k_kt_pack(const uint32_t* __restrict__ cnt, unsigned long long* 
   }
 }
 
-// Pass 3: entries get (offset, length); compact slice list; cnt[] is
-// overwritten with the compact slice index of every occupied slot.
-__global__ void k_kt_finalize(KtEntry* __restrict__ tab, uint32_t* __restrict__ cnt,
-                              const unsigned long long* __restrict__ pre, uint64_t cap,
+// Pass 3: entries get (offset, length); compact slice list; pre[] is
+// overwritten with (offset | slice index << 32) of every occupied slot, the
+// one word the scatter reads per nonzero.
+__global__ void k_kt_finalize(KtEntry* __restrict__ tab, const uint32_t* __restrict__ cnt,
+                              unsigned long long* __restrict__ pre, uint64_t cap,
                               KtSlice* __restrict__ slices) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < cap; i += (uint64_t)gridDim.x * blockDim.x) {
     uint32_t c = cnt[i];
@@ -191,23 +192,21 @@ __global__ void k_kt_finalize(KtEntry* __restrict__ tab, uint32_t* __restrict__ 
     slices[sid].key = tab[i].key;
     slices[sid].off = off;
     slices[sid].len = c;
-    cnt[i] = sid;
   }
 }
 
 // Pass 4: scatter into grouped order with the streaming layout.
 template <class RestT>
 __global__ void k_kt_scatter(const uint32_t* __restrict__ h, const double* __restrict__ val,
-                             uint32_t d, uint64_t N, KeyCodec rest, const KtEntry* __restrict__ tab,
-                             const uint32_t* __restrict__ sid_of_slot,
+                             uint32_t d, uint64_t N, KeyCodec rest, const unsigned long long* __restrict__ slotinfo,
                              const uint32_t* __restrict__ slot_of, const uint32_t* __restrict__ rank_of,
                              uint32_t* __restrict__ perm, uint32_t* __restrict__ slice_of,
                              RestT* __restrict__ rest_key, double* __restrict__ gval) {
   for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < N; e += (uint64_t)gridDim.x * blockDim.x) {
-    uint32_t s = slot_of[e];
-    uint64_t pos = (uint64_t)tab[s].off + rank_of[e];
+    const unsigned long long si = slotinfo[slot_of[e]];
+    uint64_t pos = (si & 0xFFFFFFFFull) + rank_of[e];
     perm[pos] = (uint32_t)e;
-    slice_of[pos] = sid_of_slot[s];
+    slice_of[pos] = (uint32_t)(si >> 32);
     unsigned long long rk = 0;
     for (uint32_t j = 0; j < rest.n; ++j) rk += (unsigned long long)h[e * d + rest.mode[j]] * rest.stride[j];
     rest_key[pos] = (RestT)rk;
@@ -277,8 +276,10 @@ extern "C" int ptc_hashtable_build(ptc_ctx* ctx, const ptc_dtensor* t, ptc_hasht
     return rc;
   }
   const uint64_t N = t->N;
+  // distinct keys <= min(N, key space): size the table for 80% worst-case load
+  const uint64_t maxkeys = std::min<uint64_t>(N, ht->codec.space);
   uint64_t cap = 64;
-  while (cap < N + N / 4) cap <<= 1;
+  while (cap < maxkeys + maxkeys / 4) cap <<= 1;
   ht->cap = cap;
   ht->entries = dalloc_n<KtEntry>(ctx, cap);
   ht->perm = dalloc_n<uint32_t>(ctx, N);
@@ -337,11 +338,11 @@ extern "C" int ptc_hashtable_build(ptc_ctx* ctx, const ptc_dtensor* t, ptc_hasht
   if (N) {
     if (rest64)
       hipLaunchKernelGGL((k_kt_scatter<unsigned long long>), dim3(grid_for(N, 256)), dim3(256), 0, st,
-                         t->hedges, t->vals, t->d, N, ht->rest, ht->entries, cnt, slot_of, rank_of,
+                         t->hedges, t->vals, t->d, N, ht->rest, (const unsigned long long*)packed, slot_of, rank_of,
                          ht->perm, ht->slice_of, ht->rest_key64, ht->gval);
     else
       hipLaunchKernelGGL((k_kt_scatter<uint32_t>), dim3(grid_for(N, 256)), dim3(256), 0, st, t->hedges,
-                         t->vals, t->d, N, ht->rest, ht->entries, cnt, slot_of, rank_of, ht->perm,
+                         t->vals, t->d, N, ht->rest, (const unsigned long long*)packed, slot_of, rank_of, ht->perm,
                          ht->slice_of, ht->rest_key32, ht->gval);
   }
   hipError_t e = hipStreamSynchronize(st);

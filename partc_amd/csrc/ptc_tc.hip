@@ -211,7 +211,7 @@ static int run_pass(Run& R, int pass, HashPlan* hp, const uint4* items, uint64_t
     P.row_gcap = flags;
   }
   P.items = items;
-  size_t lds = pass_lds_bytes(pass, R.dense, R.col64, R.ncols, R.lds_slots);
+  size_t lds = pass_lds_bytes(pass, R.dense, R.col64, R.ncols, R.lds_slots, R.use_flags);
   return launch_pass(R.ctx, pass, R.dense, R.col64, P, nitems, lds);
 }
 
