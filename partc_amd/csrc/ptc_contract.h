@@ -38,6 +38,7 @@ struct ItemParams {
   const uint32_t* row_nchunk;
   const unsigned long long* packed;  // scanned
   uint32_t* row_heavy;
+  uint32_t* heavy_rows;              // heavy index -> row
   uint4* items;
 };
 
@@ -96,6 +97,7 @@ struct PassParams {
   uint32_t* hbits;
   double* hacc;
   unsigned char* hflag;
+  const uint32_t* merge_rows;         // merge launch: rows to finish (NULL: all rows)
   const unsigned long long* row_slab;
   int use_flags;                      // 1: a flag byte per column marks touches
   // hash accumulators
@@ -122,6 +124,7 @@ struct EmitParams {
   const KtSlice* rows;
   const uint32_t* row_nnz;
   const uint32_t* cap;
+  const uint32_t* row_fb;             // rows redone by the exact fallback (skipped)
   const unsigned long long* cap_base;
   const unsigned long long* out_base;
   const void* stg_col;
@@ -139,10 +142,10 @@ int launch_hash_caps(ptc_ctx* ctx, const HashCapParams& P);
 int launch_caps(ptc_ctx* ctx, const CapParams& P);
 size_t pass_lds_bytes(int pass, bool dense, bool col64, uint64_t ncols, uint32_t lds_slots, bool flags = true);
 int launch_pass(ptc_ctx* ctx, int pass, bool dense, bool col64, const PassParams& P, uint64_t nitems,
-                size_t lds);
+                size_t lds, bool merge = false);
 int launch_emit(ptc_ctx* ctx, const EmitParams& P, bool col64);
 int launch_overflow_rows(ptc_ctx* ctx, const uint32_t* nnz, const uint32_t* cap, uint64_t nrows,
-                         uint32_t* list, uint32_t* n);
+                         uint32_t* list, uint32_t* n, uint32_t* mark);
 int launch_select_items(ptc_ctx* ctx, const uint4* items, uint64_t nitems, const uint32_t* nnz,
                         const uint32_t* cap, uint4* out, uint32_t* n);
 int launch_reset_rows(ptc_ctx* ctx, const uint32_t* list, uint32_t n, uint32_t* row_done);

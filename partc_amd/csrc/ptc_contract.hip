@@ -45,8 +45,14 @@
 
 namespace ptc {
 
-constexpr int PB = 512;          // threads per workgroup of the row passes
-constexpr int STAGE = 1024;      // A positions staged in LDS at a time (2 per thread)
+#ifndef PTC_PASS_THREADS
+#define PTC_PASS_THREADS 256
+#endif
+#ifndef PTC_PASS_UNROLL
+#define PTC_PASS_UNROLL 4
+#endif
+constexpr int PB = PTC_PASS_THREADS;  // threads per workgroup of the row passes
+constexpr int STAGE = 2 * PB;         // A positions staged in LDS at a time (2 per thread)
 constexpr uint32_t NO_HEAVY = 0xFFFFFFFFu;
 
 __device__ __forceinline__ unsigned long long mix64d(unsigned long long k) {
@@ -153,6 +159,7 @@ __global__ void k_items(ItemParams P) {
     uint64_t io = pk & 0xFFFFFFFFull;
     uint32_t hv = (uint32_t)(pk >> 32);
     P.row_heavy[r] = nch > 1 ? hv : NO_HEAVY;
+    if (nch > 1) P.heavy_rows[hv] = (uint32_t)r;
     uint32_t p0 = P.rows[r].off;
     for (uint32_t k = 0; k < nch; ++k) P.items[io + k] = make_uint4((uint32_t)r, k, p0, 0u);
   }
@@ -311,7 +318,16 @@ __device__ __forceinline__ bool touched(double v) {
 //   dense:  acc[ncols] f64 (+ flag[ncols] u8 when flags are needed)  (numeric)
 //           bits[ceil(ncols/32)] u32                                 (symbolic)
 //   hash:   keys[S] ColT, vals[S] f64             (S = lds_slots)
-template <int PASS, bool DENSE, class ColT>
+// MERGE = false: one workgroup per work item (a row or a chunk of a split
+// row) expands its flops; a single-chunk row in LDS is finished at once, a
+// chunk of a split row stores its partial (dense accumulator or bitmap) to
+// its slab with plain stores, and a row accumulated in HBM is left there.
+// MERGE = true (a separate launch, so the kernel boundary orders the slab
+// stores before these loads -- no device-scope fence, which on gfx950 writes
+// back and invalidates the whole L2, is ever issued): one workgroup per split
+// or HBM row reduces the slabs in chunk order (deterministic) and finishes
+// the row.
+template <int PASS, bool DENSE, class ColT, bool MERGE>
 __global__ void __launch_bounds__(PB) k_pass(PassParams P) {
   extern __shared__ __align__(16) unsigned char smem[];
   uint32_t* s_start = (uint32_t*)smem;
@@ -325,10 +341,18 @@ __global__ void __launch_bounds__(PB) k_pass(PassParams P) {
 
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const ColT* __restrict__ bcol = (const ColT*)P.bcol;
-  const uint4 it = P.items[blockIdx.x];
+  uint4 it = make_uint4(0, 0, 0, 0);
+  if (MERGE) {
+    it.x = P.merge_rows ? P.merge_rows[blockIdx.x] : blockIdx.x;
+  } else {
+    it = P.items[blockIdx.x];
+  }
   const uint32_t r = it.x, k = it.y;
   const KtSlice row = P.rows[r];
   const uint32_t nch = P.row_nchunk[r];
+  if (MERGE && DENSE && nch <= 1) return;                               // not a split row
+  if (MERGE && !DENSE && (nch == 0 || P.row_gcap[r] == 0)) return;  // not an HBM row
+  if (MERGE && !DENSE && PASS == PASS_SYMBOLIC) return;  // HBM rows counted with atomics
   const unsigned long long flo = P.apre[row.off];
   const unsigned long long fr = P.apre[(uint64_t)row.off + row.len] - flo;
   const unsigned long long per = (fr + nch - 1) / nch;
@@ -350,7 +374,36 @@ __global__ void __launch_bounds__(PB) k_pass(PassParams P) {
   uint32_t gmask = 0;
   const uint32_t nbits = (uint32_t)((ncols + 31) / 32);
 
-  if (DENSE) {
+  if (DENSE && MERGE) {
+    // reduce the row's slabs in chunk order
+    const uint64_t slab0 = P.row_slab[r];
+    if (PASS == PASS_SYMBOLIC) {
+      for (uint32_t w = threadIdx.x; w < nbits; w += PB) {
+        uint32_t o = 0;
+        for (uint32_t kk = 0; kk < nch; ++kk) o |= P.hbits[(slab0 + kk) * nbits + w];
+        bits[w] = o;
+      }
+    } else {
+      for (uint64_t c = threadIdx.x; c < ncols; c += PB) {
+        double sum = __longlong_as_double((long long)PTC_NEG_ZERO_BITS);
+        unsigned char fl = 0;
+        uint32_t kk = 0;
+        for (; kk + 4 <= nch; kk += 4) {  // four independent loads in flight
+          const double* q = P.hacc + (slab0 + kk) * ncols + c;
+          const double x0 = q[0], x1 = q[ncols], x2 = q[2 * ncols], x3 = q[3 * ncols];
+          sum += x0;
+          sum += x1;
+          sum += x2;
+          sum += x3;
+        }
+        for (; kk < nch; ++kk) sum += P.hacc[(slab0 + kk) * ncols + c];
+        if (flags)
+          for (uint32_t j = 0; j < nch; ++j) fl |= P.hflag[(slab0 + j) * ncols + c];
+        acc[c] = sum;
+        if (flags) flag[c] = fl;
+      }
+    }
+  } else if (DENSE) {
     if (PASS == PASS_SYMBOLIC) {
       for (uint32_t w = threadIdx.x; w < nbits; w += PB) bits[w] = 0;
     } else {
@@ -380,7 +433,7 @@ __global__ void __launch_bounds__(PB) k_pass(PassParams P) {
   double* hvals = (double*)(acc_base + (size_t)S * sizeof(ColT));
 
   // ---- expand the flop range [g0, g1) ----
-  uint64_t q0 = it.z;
+  uint64_t q0 = MERGE ? pend : it.z;
   while (q0 < pend) {
     const unsigned long long qstart = P.apre[q0];
     if (qstart >= g1) break;
@@ -449,7 +502,7 @@ __global__ void __launch_bounds__(PB) k_pass(PassParams P) {
       }
       // U groups of 64 flops per step: positions first (LDS only), then all
       // global loads, then the accumulations -- 2U loads in flight per wave
-      constexpr int U = 8;
+      constexpr int U = PTC_PASS_UNROLL;
       for (uint32_t f0 = wb; f0 < we; f0 += 64 * U) {
         uint32_t ii[U];
         long long bb[U];
@@ -528,57 +581,26 @@ __global__ void __launch_bounds__(PB) k_pass(PassParams P) {
     q0 = qlim;
   }
 
-  // ---- merge partials of a split row; the last chunk finishes ----
-  const bool in_hbm = !DENSE && S == 0;
-  if (nch > 1 || in_hbm) {
-    if (DENSE && heavy != NO_HEAVY) {
+  // ---- a split row (or a row accumulated in HBM) is finished by the merge launch ----
+  if (!MERGE) {
+    if (DENSE && nch > 1) {
+      const uint64_t slab = P.row_slab[r] + k;
       if (PASS == PASS_SYMBOLIC) {
-        uint32_t* gb = P.hbits + (uint64_t)heavy * nbits;
-        for (uint32_t w = threadIdx.x; w < nbits; w += PB)
-          if (bits[w]) atomicOr(&gb[w], bits[w]);
+        uint32_t* gb = P.hbits + slab * nbits;
+        for (uint32_t w = threadIdx.x; w < nbits; w += PB) gb[w] = bits[w];
       } else {
-        // this chunk's partial row goes to its own slab (plain stores)
-        const uint64_t slab = P.row_slab[r] + k;
         double* ga = P.hacc + slab * ncols;
         for (uint64_t c = threadIdx.x; c < ncols; c += PB) ga[c] = acc[c];
         if (flags) {
-          unsigned char* gf = (unsigned char*)P.hflag + slab * ncols;
+          unsigned char* gf = P.hflag + slab * ncols;
           for (uint64_t c = threadIdx.x; c < ncols; c += PB) gf[c] = flag[c];
         }
       }
+      return;
     }
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      uint32_t done = atomicAdd(&P.row_done[r], 1u);
-      s_misc[1] = (done == nch - 1) ? 1u : 0u;
-    }
-    __syncthreads();
-    if (!s_misc[1]) return;
-    __threadfence();
-    if (DENSE && heavy != NO_HEAVY) {
-      // pull the merged row back into LDS
-      if (PASS == PASS_SYMBOLIC) {
-        const uint32_t* gb = P.hbits + (uint64_t)heavy * nbits;
-        for (uint32_t w = threadIdx.x; w < nbits; w += PB)
-          bits[w] = __hip_atomic_load(&gb[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      } else {
-        // sum the chunks' partials in chunk order (deterministic)
-        const uint64_t slab0 = P.row_slab[r];
-        for (uint64_t c = threadIdx.x; c < ncols; c += PB) {
-          double s = __longlong_as_double((long long)PTC_NEG_ZERO_BITS);
-          unsigned char fl = 0;
-          for (uint32_t kk = 0; kk < nch; ++kk) {
-            s += __hip_atomic_load(P.hacc + (slab0 + kk) * ncols + c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (flags) fl |= ((const volatile unsigned char*)P.hflag)[(slab0 + kk) * ncols + c];
-          }
-          acc[c] = s;
-          if (flags) flag[c] = fl;
-        }
-      }
-      __syncthreads();
-    }
+    if (!DENSE && S == 0) return;
   }
+  __syncthreads();
 
   // ---- finish the row ----
   if (PASS == PASS_SYMBOLIC) {
@@ -675,7 +697,7 @@ __global__ void __launch_bounds__(PB) k_emit(EmitParams P) {
   const uint32_t r = blockIdx.x;
   if (r >= P.nrows) return;
   const uint32_t n = P.row_nnz[r];
-  if (!n || n > P.cap[r]) return;  // overflow rows are redone exactly
+  if (!n || n > P.cap[r] || (P.row_fb && P.row_fb[r])) return;  // fallback rows are redone exactly
   __shared__ uint32_t s_rowc[PTC_MAX_MODES];
   if (threadIdx.x == 0) {
     const unsigned long long key = P.rows[r].key;
@@ -752,9 +774,9 @@ size_t pass_lds_bytes(int pass, bool dense, bool col64, uint64_t ncols, uint32_t
   return (b + 15) & ~size_t(15);
 }
 
-template <int PASS, bool DENSE, class ColT>
+template <int PASS, bool DENSE, class ColT, bool MERGE>
 static int launch_pass_t(ptc_ctx* ctx, const PassParams& P, uint64_t nitems, size_t lds) {
-  auto kern = k_pass<PASS, DENSE, ColT>;
+  auto kern = k_pass<PASS, DENSE, ColT, MERGE>;
   PTC_HIP(ctx, hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   const bool timed = ctx->nkev + 2 <= 16;
   if (timed) hipEventRecord(ctx->kev[ctx->nkev], ctx->stream);
@@ -768,20 +790,27 @@ static int launch_pass_t(ptc_ctx* ctx, const PassParams& P, uint64_t nitems, siz
   return PTC_OK;
 }
 
+template <bool MERGE>
+static int launch_pass_m(ptc_ctx* ctx, int pass, bool dense, bool col64, const PassParams& P, uint64_t nitems,
+                         size_t lds) {
+  if (pass == PASS_SYMBOLIC) {
+    if (dense) return col64 ? launch_pass_t<PASS_SYMBOLIC, true, unsigned long long, MERGE>(ctx, P, nitems, lds)
+                            : launch_pass_t<PASS_SYMBOLIC, true, uint32_t, MERGE>(ctx, P, nitems, lds);
+    return col64 ? launch_pass_t<PASS_SYMBOLIC, false, unsigned long long, MERGE>(ctx, P, nitems, lds)
+                 : launch_pass_t<PASS_SYMBOLIC, false, uint32_t, MERGE>(ctx, P, nitems, lds);
+  }
+  if (dense) return col64 ? launch_pass_t<PASS_NUMERIC, true, unsigned long long, MERGE>(ctx, P, nitems, lds)
+                          : launch_pass_t<PASS_NUMERIC, true, uint32_t, MERGE>(ctx, P, nitems, lds);
+  return col64 ? launch_pass_t<PASS_NUMERIC, false, unsigned long long, MERGE>(ctx, P, nitems, lds)
+               : launch_pass_t<PASS_NUMERIC, false, uint32_t, MERGE>(ctx, P, nitems, lds);
+}
+
 int launch_pass(ptc_ctx* ctx, int pass, bool dense, bool col64, const PassParams& P, uint64_t nitems,
-                size_t lds) {
+                size_t lds, bool merge) {
   if (!nitems) return PTC_OK;
   if (nitems > 0x7FFFFFFFull) return set_err(ctx, PTC_ERR_RANGE, "too many work items");
-  if (pass == PASS_SYMBOLIC) {
-    if (dense) return col64 ? launch_pass_t<PASS_SYMBOLIC, true, unsigned long long>(ctx, P, nitems, lds)
-                            : launch_pass_t<PASS_SYMBOLIC, true, uint32_t>(ctx, P, nitems, lds);
-    return col64 ? launch_pass_t<PASS_SYMBOLIC, false, unsigned long long>(ctx, P, nitems, lds)
-                 : launch_pass_t<PASS_SYMBOLIC, false, uint32_t>(ctx, P, nitems, lds);
-  }
-  if (dense) return col64 ? launch_pass_t<PASS_NUMERIC, true, unsigned long long>(ctx, P, nitems, lds)
-                          : launch_pass_t<PASS_NUMERIC, true, uint32_t>(ctx, P, nitems, lds);
-  return col64 ? launch_pass_t<PASS_NUMERIC, false, unsigned long long>(ctx, P, nitems, lds)
-               : launch_pass_t<PASS_NUMERIC, false, uint32_t>(ctx, P, nitems, lds);
+  return merge ? launch_pass_m<true>(ctx, pass, dense, col64, P, nitems, lds)
+               : launch_pass_m<false>(ctx, pass, dense, col64, P, nitems, lds);
 }
 
 int launch_emit(ptc_ctx* ctx, const EmitParams& P, bool col64) {
@@ -838,16 +867,21 @@ int launch_caps(ptc_ctx* ctx, const CapParams& P) {
   return PTC_OK;
 }
 
-// rows that overflowed their predicted capacity -> a compact list
+// rows that overflowed their predicted capacity -> a compact list and a
+// per-row mark
 __global__ void k_overflow_rows(const uint32_t* __restrict__ nnz, const uint32_t* __restrict__ cap,
-                                uint64_t nrows, uint32_t* __restrict__ list, uint32_t* __restrict__ n) {
-  for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nrows; r += (uint64_t)gridDim.x * blockDim.x)
-    if (nnz[r] > cap[r]) list[atomicAdd(n, 1u)] = (uint32_t)r;
+                                uint64_t nrows, uint32_t* __restrict__ list, uint32_t* __restrict__ n,
+                                uint32_t* __restrict__ mark) {
+  for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nrows; r += (uint64_t)gridDim.x * blockDim.x) {
+    const bool o = nnz[r] > cap[r];
+    mark[r] = o ? 1u : 0u;
+    if (o) list[atomicAdd(n, 1u)] = (uint32_t)r;
+  }
 }
 int launch_overflow_rows(ptc_ctx* ctx, const uint32_t* nnz, const uint32_t* cap, uint64_t nrows,
-                         uint32_t* list, uint32_t* n) {
+                         uint32_t* list, uint32_t* n, uint32_t* mark) {
   if (!nrows) return PTC_OK;
-  hipLaunchKernelGGL(k_overflow_rows, dim3(gridn(nrows, 256)), dim3(256), 0, ctx->stream, nnz, cap, nrows, list, n);
+  hipLaunchKernelGGL(k_overflow_rows, dim3(gridn(nrows, 256)), dim3(256), 0, ctx->stream, nnz, cap, nrows, list, n, mark);
   PTC_HIP(ctx, hipGetLastError());
   return PTC_OK;
 }

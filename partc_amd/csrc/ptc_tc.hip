@@ -74,6 +74,7 @@ struct Run {
   double* hacc = nullptr;
   unsigned char* hflag = nullptr;
   unsigned long long* row_slab = nullptr;  // first partial-row slab of each split row
+  uint32_t* heavy_rows = nullptr;          // split rows (heavy index -> row)
   uint64_t nslabs = 0;
   bool use_flags = true;
   uint32_t* overflow = nullptr;
@@ -203,7 +204,11 @@ static int hbm_flags(Run& R, HashPlan& hp, unsigned long long** flags) {
   return PTC_OK;
 }
 
-static int run_pass(Run& R, int pass, HashPlan* hp, const uint4* items, uint64_t nitems, PassParams& P) {
+// One row pass over `items`, then the merge launch that finishes split rows
+// (dense: the listed rows; hash: every row held in HBM).  merge_rows /
+// nmerge default to all split rows.
+static int run_pass(Run& R, int pass, HashPlan* hp, const uint4* items, uint64_t nitems, PassParams& P,
+                    const uint32_t* merge_rows = nullptr, uint64_t nmerge = ~0ull) {
   unsigned long long* flags = nullptr;
   if (!R.dense) {
     int rc = hbm_flags(R, *hp, &flags);
@@ -212,25 +217,37 @@ static int run_pass(Run& R, int pass, HashPlan* hp, const uint4* items, uint64_t
   }
   P.items = items;
   size_t lds = pass_lds_bytes(pass, R.dense, R.col64, R.ncols, R.lds_slots, R.use_flags);
-  return launch_pass(R.ctx, pass, R.dense, R.col64, P, nitems, lds);
+  int rc = launch_pass(R.ctx, pass, R.dense, R.col64, P, nitems, lds, false);
+  if (rc) return rc;
+  if (R.dense) {
+    if (!merge_rows) {
+      merge_rows = R.heavy_rows;
+      nmerge = R.nheavy;
+    }
+    P.merge_rows = merge_rows;
+    return launch_pass(R.ctx, pass, true, R.col64, P, nmerge, lds, true);
+  }
+  if (pass == PASS_NUMERIC && hp->pool) {
+    P.merge_rows = merge_rows;  // NULL: every row (rows not in HBM exit at once)
+    return launch_pass(R.ctx, pass, false, R.col64, P, merge_rows ? nmerge : R.nrows, lds, true);
+  }
+  return PTC_OK;
 }
 
+// Slabs of split rows: one partial per chunk (a bitmap in the symbolic pass,
+// a dense accumulator in the numeric pass); every chunk writes its whole
+// slab, so nothing is cleared.
 static int prepare_heavy(Run& R, int pass) {
   if (!R.dense || !R.nheavy) return PTC_OK;
-  ptc_ctx* ctx = R.ctx;
   if (pass == PASS_SYMBOLIC) {
-    uint64_t nbits = (R.ncols + 31) / 32;
-    if (!R.hbits) R.hbits = R.ws.get<uint32_t>(R.nheavy * nbits);
-    if (!R.hbits) return PTC_ERR_NOMEM;
-    return fill_zero(ctx, R.hbits, R.nheavy * nbits * 4);
+    const uint64_t nbits = (R.ncols + 31) / 32;
+    if (!R.hbits) R.hbits = R.ws.get<uint32_t>(R.nslabs * nbits);
+    return R.hbits ? PTC_OK : PTC_ERR_NOMEM;
   }
-  // one slab (partial dense row) per chunk of every split row; each chunk
-  // writes its whole slab, so no clearing is needed
   if (!R.hacc) R.hacc = R.ws.get<double>(R.nslabs * R.ncols);
   if (R.use_flags && !R.hflag) R.hflag = R.ws.get<unsigned char>(R.nslabs * R.ncols);
   if (!R.hacc || (R.use_flags && !R.hflag)) return PTC_ERR_NOMEM;
-  int rc = PTC_OK;
-  return rc;
+  return PTC_OK;
 }
 
 static int make_output(Run& R, uint64_t nnz, const ptc_dtensor* A, const ptc_dtensor* B, ptc_dtensor** O) {
@@ -311,7 +328,10 @@ static int prepare(Run& R, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B,
     R.lds_slots = R.lds_slots ? p2 : 0;
     while (R.lds_slots > 16 && pass_lds_bytes(PASS_NUMERIC, false, R.col64, 0, R.lds_slots) > lds_cap) R.lds_slots >>= 1;
   }
-  const uint64_t chunk = opt.chunk_flops ? opt.chunk_flops : DEFAULT_CHUNK;
+  // work per workgroup: large enough that a split row's slab (one dense row
+  // per chunk) stays a small fraction of the chunk's streamed bytes
+  const uint64_t chunk = opt.chunk_flops ? opt.chunk_flops
+                                         : (R.dense ? std::max<uint64_t>(DEFAULT_CHUNK, 16 * R.ncols) : DEFAULT_CHUNK);
 
   int rc = PTC_OK;
   Workspace& ws = R.ws;
@@ -377,7 +397,8 @@ static int prepare(Run& R, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B,
   R.nheavy = hostv[1] >> 32;
   if (R.F == 0) return set_err(ctx, PTC_ERR_EMPTY, "tensor contraction not possible!! (no contributing pair)");
   R.items = ws.get<uint4>(R.nitems);
-  if (!R.items) return PTC_ERR_NOMEM;
+  R.heavy_rows = ws.get<uint32_t>(R.nheavy);
+  if (!R.items || !R.heavy_rows) return PTC_ERR_NOMEM;
   {
     ItemParams P{};
     P.nrows = R.nrows;
@@ -385,6 +406,7 @@ static int prepare(Run& R, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B,
     P.row_nchunk = R.row_nchunk;
     P.packed = R.packed;
     P.row_heavy = R.row_heavy;
+    P.heavy_rows = R.heavy_rows;
     P.items = R.items;
     ItemP0Params Q{};
     Q.npos = R.nheavy ? R.npos : 0;
@@ -516,25 +538,27 @@ extern "C" int ptc_tc_fksCuckoo(ptc_ctx* ctx, const ptc_hashtable* HT_A, const p
     fallback_rows = ovf[0];
     uint4* fitems = nullptr;
     uint32_t nfitems = 0;
+    uint32_t* frows = nullptr;
+    uint32_t* row_fb = nullptr;
     if (fallback_rows) {
       // exact two-pass for the rows whose capacity was too small
       uint32_t* cnt = ws.get<uint32_t>(4);
       fitems = ws.get<uint4>(R.nitems);
-      uint32_t* frows = ws.get<uint32_t>(fallback_rows);
-      if (!cnt || !fitems || !frows) return PTC_ERR_NOMEM;
+      frows = ws.get<uint32_t>(fallback_rows);
+      row_fb = ws.get<uint32_t>(R.nrows);
+      if (!cnt || !fitems || !frows || !row_fb) return PTC_ERR_NOMEM;
       if ((rc = fill_zero(ctx, cnt, 16))) return rc;
       if ((rc = launch_select_items(ctx, R.items, R.nitems, R.row_nnz, cap, fitems, cnt))) return rc;
-      if ((rc = launch_overflow_rows(ctx, R.row_nnz, cap, R.nrows, frows, cnt + 1))) return rc;
+      if ((rc = launch_overflow_rows(ctx, R.row_nnz, cap, R.nrows, frows, cnt + 1, row_fb))) return rc;
       if ((rc = read_back(ctx, &nfitems, cnt, 4))) return rc;
-      // symbolic on those rows: reset their counts and merge state
-      if ((rc = launch_reset_rows(ctx, frows, (uint32_t)fallback_rows, R.row_done))) return rc;
+      // symbolic on those rows
       if ((rc = launch_reset_rows(ctx, frows, (uint32_t)fallback_rows, R.row_nnz))) return rc;
       HashPlan hs;
       if (!R.dense && (rc = plan_hash(R, hs, nullptr, R.row_flops, nullptr))) return rc;
       if ((rc = prepare_heavy(R, PASS_SYMBOLIC)) || (!R.dense && (rc = reset_hash_pool(R, hs, false)))) return rc;
       PassParams S;
       base_params(R, S, R.dense ? nullptr : &hs);
-      if ((rc = run_pass(R, PASS_SYMBOLIC, &hs, fitems, nfitems, S))) return rc;
+      if ((rc = run_pass(R, PASS_SYMBOLIC, &hs, fitems, nfitems, S, frows, fallback_rows))) return rc;
     }
     if ((rc = scan_u32_to_u64(ctx, R.row_nnz, out_off, R.nrows, out_off + R.nrows))) return rc;
     if ((rc = read_back(ctx, &hostv[0], out_off + R.nrows, 8))) return rc;
@@ -546,6 +570,7 @@ extern "C" int ptc_tc_fksCuckoo(ptc_ctx* ctx, const ptc_hashtable* HT_A, const p
       E.rows = HT_A->slices;
       E.row_nnz = R.row_nnz;
       E.cap = cap;
+      E.row_fb = row_fb;
       E.cap_base = cap_off;
       E.out_base = out_off;
       E.stg_col = stg_col;
@@ -563,26 +588,17 @@ extern "C" int ptc_tc_fksCuckoo(ptc_ctx* ctx, const ptc_hashtable* HT_A, const p
     }
     if (fallback_rows) {
       HashPlan hn;
-      if (!R.dense && (rc = plan_hash(R, hn, R.row_nnz, nullptr, nullptr))) {
+      if ((!R.dense && (rc = plan_hash(R, hn, R.row_nnz, nullptr, nullptr))) || (rc = prepare_heavy(R, PASS_NUMERIC)) ||
+          (!R.dense && (rc = reset_hash_pool(R, hn, true)))) {
         ptc_dtensor_free(out);
         return rc;
-      }
-      // only the fallback rows: reset their merge counters
-      uint32_t* frows = ws.get<uint32_t>(fallback_rows);
-      uint32_t* cnt = ws.get<uint32_t>(2);
-      if (!frows || !cnt || (rc = fill_zero(ctx, cnt, 8)) ||
-          (rc = launch_overflow_rows(ctx, R.row_nnz, cap, R.nrows, frows, cnt)) ||
-          (rc = launch_reset_rows(ctx, frows, (uint32_t)fallback_rows, R.row_done)) ||
-          (rc = prepare_heavy(R, PASS_NUMERIC)) || (!R.dense && (rc = reset_hash_pool(R, hn, true)))) {
-        ptc_dtensor_free(out);
-        return rc ? rc : PTC_ERR_NOMEM;
       }
       PassParams Q;
       base_params(R, Q, R.dense ? nullptr : &hn);
       Q.out_base = out_off;
       Q.out_hedges = out->hedges;
       Q.out_vals = out->vals;
-      if ((rc = run_pass(R, PASS_NUMERIC, &hn, fitems, nfitems, Q))) {
+      if ((rc = run_pass(R, PASS_NUMERIC, &hn, fitems, nfitems, Q, frows, fallback_rows))) {
         ptc_dtensor_free(out);
         return rc;
       }
