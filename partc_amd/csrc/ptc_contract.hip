@@ -46,7 +46,7 @@
 namespace ptc {
 
 #ifndef PTC_PASS_THREADS
-#define PTC_PASS_THREADS 256
+#define PTC_PASS_THREADS 512
 #endif
 #ifndef PTC_PASS_UNROLL
 #define PTC_PASS_UNROLL 4

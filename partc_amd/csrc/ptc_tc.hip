@@ -331,7 +331,7 @@ static int prepare(Run& R, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B,
   // work per workgroup: large enough that a split row's slab (one dense row
   // per chunk) stays a small fraction of the chunk's streamed bytes
   const uint64_t chunk = opt.chunk_flops ? opt.chunk_flops
-                                         : (R.dense ? std::max<uint64_t>(DEFAULT_CHUNK, 16 * R.ncols) : DEFAULT_CHUNK);
+                                         : (R.dense ? std::max<uint64_t>(DEFAULT_CHUNK, 8 * R.ncols) : DEFAULT_CHUNK);
 
   int rc = PTC_OK;
   Workspace& ws = R.ws;
