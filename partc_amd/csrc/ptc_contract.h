@@ -14,7 +14,8 @@ struct ProbeParams {
   const uint32_t* row_of_pos;     // HT_A slice_of
   KeyCodec acodec, bcodec;        // A's contracted codec, B's key codec
   const KtEntry* tab;             // HT_B entries
-  uint64_t nbuckets;              // HT_B cap / 8
+  uint64_t cap;                   // HT_B slots
+  int direct;                     // HT_B addressed by key
   uint32_t* pos_len;
   uint32_t* pos_off;
   uint32_t* row_hits;

@@ -86,26 +86,24 @@ __global__ void __launch_bounds__(256) k_probe(ProbeParams P) {
         if (x >= P.bcodec.radix[j]) ok = false;
         kb += (unsigned long long)x * P.bcodec.stride[j];
       }
-      if (ok) {
-        uint64_t b = mix64d(kb) & (P.nbuckets - 1);
+      if (ok && P.direct) {
+        if (kb < P.cap) {
+          const KtEntry en = P.tab[kb];
+          len = en.len;
+          off = en.off;
+        }
+      } else if (ok) {
+        // same probe order as the insertion: line mix64(k) mod (cap/8), onward
+        uint64_t i = (mix64d(kb) & (P.cap / 8 - 1)) * 8;
         while (true) {
-          bool done = false;
-#pragma unroll
-          for (int s = 0; s < 8; ++s) {
-            KtEntry en = P.tab[b * 8 + s];
-            if (en.key == kb) {
-              len = en.len;
-              off = en.off;
-              done = true;
-              break;
-            }
-            if (en.key == PTC_EMPTY_KEY) {
-              done = true;
-              break;
-            }
+          const KtEntry en = P.tab[i];
+          if (en.key == kb) {
+            len = en.len;
+            off = en.off;
+            break;
           }
-          if (done) break;
-          b = (b + 1) & (P.nbuckets - 1);
+          if (en.key == PTC_EMPTY_KEY) break;
+          i = (i + 1) & (P.cap - 1);
         }
       }
       P.pos_len[p] = len;

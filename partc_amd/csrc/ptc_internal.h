@@ -107,7 +107,8 @@ struct ptc_hashtable {
   ptc_ctx* ctx = nullptr;
   const ptc_dtensor* t = nullptr;
   KeyCodec codec;
-  uint64_t cap = 0;            // table slots (power of two)
+  uint64_t cap = 0;            // table slots (power of two; the key space when direct)
+  bool direct = false;         // slot = key (small key spaces)
   KtEntry* entries = nullptr;  // device, cap
   uint64_t num_keys = 0;
   KtSlice* slices = nullptr;   // device, num_keys (slot order)

@@ -30,7 +30,10 @@
 
 namespace ptc {
 
-constexpr int KT_GROUP = 8;  // lanes per key: 8 x 16-byte entries = one 128-byte line
+#ifndef PTC_KT_GROUP
+#define PTC_KT_GROUP 8
+#endif
+constexpr int KT_GROUP = PTC_KT_GROUP;  // lanes per key (8: 8 x 16-byte entries = one 128-byte line)
 constexpr int KT_THREADS = 256;
 
 __device__ __forceinline__ unsigned long long mix64(unsigned long long k) {
@@ -61,45 +64,76 @@ __device__ __forceinline__ unsigned long long load_key_l2(const KtEntry* e) {
   return __hip_atomic_load(&e->key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Pass 1.  One group of KT_GROUP lanes per nonzero.
+// Pass 1.  One group of G lanes per nonzero (G = 8: one 128-byte line of 8
+// entries per probe step).  Probe order for key k: the entries of line
+// mix64(k) mod (cap/8), then the following entries, wrapping at cap -- the
+// order every lookup scans, so a key always sits at the first entry that
+// was free on its path.
+template <int G>
 __global__ void __launch_bounds__(KT_THREADS) k_kt_insert(
     const uint32_t* __restrict__ h, uint32_t d, uint64_t N, KeyCodec codec,
-    KtEntry* __restrict__ tab, uint64_t nbuckets, uint32_t* __restrict__ slot_of) {
+    KtEntry* __restrict__ tab, uint64_t cap, uint32_t* __restrict__ slot_of) {
   const int lane = threadIdx.x & 63;
-  const int sub = lane & (KT_GROUP - 1);
+  const int sub = lane & (G - 1);
   const int gbase = lane - sub;  // first lane of this group in the wave
-  const unsigned long long gmask = ((1ull << KT_GROUP) - 1) << gbase;
-  const uint64_t groups_per_grid = (uint64_t)gridDim.x * (KT_THREADS / KT_GROUP);
-  for (uint64_t e = (uint64_t)blockIdx.x * (KT_THREADS / KT_GROUP) + threadIdx.x / KT_GROUP; e < N;
-       e += groups_per_grid) {
+  const unsigned long long gmask = (G == 64 ? ~0ull : ((1ull << G) - 1)) << gbase;
+  const uint64_t groups_per_grid = (uint64_t)gridDim.x * (KT_THREADS / G);
+  const uint64_t nlines = cap / 8;
+  for (uint64_t e = (uint64_t)blockIdx.x * (KT_THREADS / G) + threadIdx.x / G; e < N; e += groups_per_grid) {
     unsigned long long key = 0;
     encode_key(h, d, e, codec, &key);  // table's own tensor: always within radix
-    uint64_t b = mix64(key) & (nbuckets - 1);
+    uint64_t base = (mix64(key) & (nlines - 1)) * 8;
     uint32_t slot = 0;
     while (true) {
-      const uint64_t idx = b * KT_GROUP + sub;
+      const uint64_t idx = (base + sub) & (cap - 1);
       unsigned long long k = load_key_l2(&tab[idx]);
+      if (G == 1) {
+        if (k == key) {
+          slot = (uint32_t)idx;
+          break;
+        }
+        if (k == PTC_EMPTY_KEY) {
+          unsigned long long old = atomicCAS(&tab[idx].key, PTC_EMPTY_KEY, key);
+          if (old == PTC_EMPTY_KEY || old == key) {
+            slot = (uint32_t)idx;
+            break;
+          }
+          continue;  // lost the race: re-read the same entry
+        }
+        base = (base + 1) & (cap - 1);
+        continue;
+      }
       unsigned long long hit = __ballot(k == key) & gmask;
       if (hit) {
-        slot = (uint32_t)(b * KT_GROUP + (__ffsll((long long)hit) - 1 - gbase));
+        slot = (uint32_t)((base + (__ffsll((long long)hit) - 1 - gbase)) & (cap - 1));
         break;
       }
       unsigned long long fre = __ballot(k == PTC_EMPTY_KEY) & gmask;
       if (!fre) {
-        b = (b + 1) & (nbuckets - 1);
+        base = (base + G) & (cap - 1);
         continue;
       }
       const int leader = __ffsll((long long)fre) - 1;
       unsigned long long old = 0;
-      if (lane == leader) old = atomicCAS(&tab[idx].key, PTC_EMPTY_KEY, key);
+      if (lane == leader) old = atomicCAS(&tab[(base + (leader - gbase)) & (cap - 1)].key, PTC_EMPTY_KEY, key);
       old = __shfl(old, leader, 64);
       if (old == PTC_EMPTY_KEY || old == key) {
-        slot = (uint32_t)(b * KT_GROUP + (leader - gbase));
+        slot = (uint32_t)((base + (leader - gbase)) & (cap - 1));
         break;
       }
-      // lost the race to a different key: re-read the same line
+      // lost the race to a different key: re-read the same entries
     }
     if (sub == 0) slot_of[e] = slot;
+  }
+}
+
+// Direct addressing (a perfect hash) when the key space is small: slot = key.
+__global__ void k_kt_direct(const uint32_t* __restrict__ h, uint32_t d, uint64_t N, KeyCodec codec,
+                            uint32_t* __restrict__ slot_of) {
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < N; e += (uint64_t)gridDim.x * blockDim.x) {
+    unsigned long long key = 0;
+    encode_key(h, d, e, codec, &key);
+    slot_of[e] = (uint32_t)key;
   }
 }
 
@@ -180,13 +214,14 @@ __global__ void k_kt_pack(const uint32_t* __restrict__ cnt, unsigned long long* 
 // one word the scatter reads per nonzero.
 __global__ void k_kt_finalize(KtEntry* __restrict__ tab, const uint32_t* __restrict__ cnt,
                               unsigned long long* __restrict__ pre, uint64_t cap,
-                              KtSlice* __restrict__ slices) {
+                              KtSlice* __restrict__ slices, int direct) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < cap; i += (uint64_t)gridDim.x * blockDim.x) {
     uint32_t c = cnt[i];
     if (!c) continue;
     unsigned long long p = pre[i];
     uint32_t off = (uint32_t)(p & 0xFFFFFFFFull);
     uint32_t sid = (uint32_t)(p >> 32);
+    if (direct) tab[i].key = i;
     tab[i].off = off;
     tab[i].len = c;
     slices[sid].key = tab[i].key;
@@ -195,18 +230,26 @@ __global__ void k_kt_finalize(KtEntry* __restrict__ tab, const uint32_t* __restr
   }
 }
 
-// Pass 4: scatter into grouped order with the streaming layout.
-template <class RestT>
-__global__ void k_kt_scatter(const uint32_t* __restrict__ h, const double* __restrict__ val,
-                             uint32_t d, uint64_t N, KeyCodec rest, const unsigned long long* __restrict__ slotinfo,
+// Pass 4a: scatter ids into grouped order (two 4-byte random stores).
+__global__ void k_kt_scatter(uint64_t N, const unsigned long long* __restrict__ slotinfo,
                              const uint32_t* __restrict__ slot_of, const uint32_t* __restrict__ rank_of,
-                             uint32_t* __restrict__ perm, uint32_t* __restrict__ slice_of,
-                             RestT* __restrict__ rest_key, double* __restrict__ gval) {
+                             uint32_t* __restrict__ perm, uint32_t* __restrict__ slice_of) {
   for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < N; e += (uint64_t)gridDim.x * blockDim.x) {
     const unsigned long long si = slotinfo[slot_of[e]];
-    uint64_t pos = (si & 0xFFFFFFFFull) + rank_of[e];
+    const uint64_t pos = (si & 0xFFFFFFFFull) + rank_of[e];
     perm[pos] = (uint32_t)e;
     slice_of[pos] = (uint32_t)(si >> 32);
+  }
+}
+
+// Pass 4b: the streaming layout in grouped order, written coalesced: the
+// linearised non-key tuple and the value of every grouped position.
+template <class RestT>
+__global__ void k_kt_gather(const uint32_t* __restrict__ h, const double* __restrict__ val, uint32_t d,
+                            uint64_t N, KeyCodec rest, const uint32_t* __restrict__ perm,
+                            RestT* __restrict__ rest_key, double* __restrict__ gval) {
+  for (uint64_t pos = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; pos < N; pos += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t e = perm[pos];
     unsigned long long rk = 0;
     for (uint32_t j = 0; j < rest.n; ++j) rk += (unsigned long long)h[e * d + rest.mode[j]] * rest.stride[j];
     rest_key[pos] = (RestT)rk;
@@ -222,7 +265,7 @@ static unsigned grid_for(uint64_t n, unsigned threads, unsigned cap = 8192) {
 
 // Probe kernel for the C ABI: one thread per query key.
 __global__ void k_kt_probe(const uint32_t* __restrict__ q, uint64_t n, KeyCodec codec,
-                           const KtEntry* __restrict__ tab, uint64_t nbuckets,
+                           const KtEntry* __restrict__ tab, uint64_t cap, int direct,
                            unsigned long long* __restrict__ off_len) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
     unsigned long long key = 0;
@@ -233,24 +276,21 @@ __global__ void k_kt_probe(const uint32_t* __restrict__ q, uint64_t n, KeyCodec 
       key += (unsigned long long)x * codec.stride[j];
     }
     unsigned long long res = 0;
-    if (ok) {
-      uint64_t b = mix64(key) & (nbuckets - 1);
-      for (uint64_t probes = 0; probes <= nbuckets; ++probes) {
-        bool done = false;
-        for (int s = 0; s < KT_GROUP; ++s) {
-          KtEntry en = tab[b * KT_GROUP + s];
-          if (en.key == key) {
-            res = ((unsigned long long)en.off << 32) | en.len;
-            done = true;
-            break;
-          }
-          if (en.key == PTC_EMPTY_KEY) {
-            done = true;
-            break;
-          }
+    if (ok && direct) {
+      if (key < cap) {
+        KtEntry en = tab[key];
+        if (en.len) res = ((unsigned long long)en.off << 32) | en.len;
+      }
+    } else if (ok) {
+      uint64_t i = (mix64(key) & (cap / 8 - 1)) * 8;
+      for (uint64_t probes = 0; probes < cap; ++probes) {
+        KtEntry en = tab[i];
+        if (en.key == key) {
+          res = ((unsigned long long)en.off << 32) | en.len;
+          break;
         }
-        if (done) break;
-        b = (b + 1) & (nbuckets - 1);
+        if (en.key == PTC_EMPTY_KEY) break;
+        i = (i + 1) & (cap - 1);
       }
     }
     off_len[i] = res;
@@ -276,10 +316,13 @@ extern "C" int ptc_hashtable_build(ptc_ctx* ctx, const ptc_dtensor* t, ptc_hasht
     return rc;
   }
   const uint64_t N = t->N;
-  // distinct keys <= min(N, key space): size the table for 80% worst-case load
+  // A small key space is addressed directly (slot = key); otherwise distinct
+  // keys <= min(N, key space) and the table is sized for 80% worst-case load.
+  ht->direct = ht->codec.space <= std::max<uint64_t>(2 * N, 1u << 16) && ht->codec.space < (1ull << 31);
   const uint64_t maxkeys = std::min<uint64_t>(N, ht->codec.space);
   uint64_t cap = 64;
   while (cap < maxkeys + maxkeys / 4) cap <<= 1;
+  if (ht->direct) cap = std::max<uint64_t>(ht->codec.space, 1);
   ht->cap = cap;
   ht->entries = dalloc_n<KtEntry>(ctx, cap);
   ht->perm = dalloc_n<uint32_t>(ctx, N);
@@ -302,11 +345,15 @@ extern "C" int ptc_hashtable_build(ptc_ctx* ctx, const ptc_dtensor* t, ptc_hasht
   }
   hipStream_t st = ctx->stream;
   hipLaunchKernelGGL(k_kt_clear, dim3(grid_for(cap, 256)), dim3(256), 0, st, ht->entries, cnt, cap);
-  const uint64_t nb = cap / KT_GROUP;
-  if (N) {
+  if (N && ht->direct) {
+    hipLaunchKernelGGL(k_kt_direct, dim3(grid_for(N, 256)), dim3(256), 0, st, t->hedges, t->d, N, ht->codec,
+                       slot_of);
+  } else if (N) {
     unsigned g = grid_for(N, KT_THREADS / KT_GROUP, 16384);
-    hipLaunchKernelGGL(k_kt_insert, dim3(g), dim3(KT_THREADS), 0, st, t->hedges, t->d, N, ht->codec,
-                       ht->entries, nb, slot_of);
+    hipLaunchKernelGGL(k_kt_insert<KT_GROUP>, dim3(g), dim3(KT_THREADS), 0, st, t->hedges, t->d, N, ht->codec,
+                       ht->entries, cap, slot_of);
+  }
+  if (N) {
     hipLaunchKernelGGL(k_kt_rank, dim3((unsigned)((N + RK_CHUNK - 1) / RK_CHUNK)), dim3(RK_THREADS), 0, st,
                        (const uint32_t*)slot_of, N, cnt, rank_of);
   }
@@ -334,16 +381,16 @@ extern "C" int ptc_hashtable_build(ptc_ctx* ctx, const ptc_dtensor* t, ptc_hasht
     return PTC_ERR_NOMEM;
   }
   hipLaunchKernelGGL(k_kt_finalize, dim3(grid_for(cap, 256)), dim3(256), 0, st, ht->entries, cnt, packed,
-                     cap, ht->slices);
+                     cap, ht->slices, ht->direct ? 1 : 0);
   if (N) {
+    hipLaunchKernelGGL(k_kt_scatter, dim3(grid_for(N, 256)), dim3(256), 0, st, N, (const unsigned long long*)packed,
+                       (const uint32_t*)slot_of, (const uint32_t*)rank_of, ht->perm, ht->slice_of);
     if (rest64)
-      hipLaunchKernelGGL((k_kt_scatter<unsigned long long>), dim3(grid_for(N, 256)), dim3(256), 0, st,
-                         t->hedges, t->vals, t->d, N, ht->rest, (const unsigned long long*)packed, slot_of, rank_of,
-                         ht->perm, ht->slice_of, ht->rest_key64, ht->gval);
+      hipLaunchKernelGGL((k_kt_gather<unsigned long long>), dim3(grid_for(N, 256)), dim3(256), 0, st, t->hedges,
+                         t->vals, t->d, N, ht->rest, (const uint32_t*)ht->perm, ht->rest_key64, ht->gval);
     else
-      hipLaunchKernelGGL((k_kt_scatter<uint32_t>), dim3(grid_for(N, 256)), dim3(256), 0, st, t->hedges,
-                         t->vals, t->d, N, ht->rest, (const unsigned long long*)packed, slot_of, rank_of, ht->perm,
-                         ht->slice_of, ht->rest_key32, ht->gval);
+      hipLaunchKernelGGL((k_kt_gather<uint32_t>), dim3(grid_for(N, 256)), dim3(256), 0, st, t->hedges, t->vals,
+                         t->d, N, ht->rest, (const uint32_t*)ht->perm, ht->rest_key32, ht->gval);
   }
   hipError_t e = hipStreamSynchronize(st);
   if (e != hipSuccess) {
@@ -378,7 +425,7 @@ extern "C" int ptc_hashtable_probe(ptc_ctx* ctx, const ptc_hashtable* h, uint64_
   if (!dq || !dres) return PTC_ERR_NOMEM;
   if (nk) PTC_HIP(ctx, hipMemcpyAsync(dq, keys, sizeof(uint32_t) * n * nk, hipMemcpyHostToDevice, ctx->stream));
   hipLaunchKernelGGL(k_kt_probe, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, dq, n, h->codec,
-                     h->entries, h->cap / KT_GROUP, dres);
+                     h->entries, h->cap, h->direct ? 1 : 0, dres);
   PTC_HIP(ctx, hipGetLastError());
   std::vector<unsigned long long> res(n);
   PTC_HIP(ctx, hipMemcpyAsync(res.data(), dres, sizeof(unsigned long long) * n, hipMemcpyDeviceToHost, ctx->stream));

@@ -365,7 +365,8 @@ static int prepare(Run& R, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B,
     P.acodec = HT_A->rest;
     P.bcodec = HT_B->codec;
     P.tab = HT_B->entries;
-    P.nbuckets = HT_B->cap / 8;
+    P.cap = HT_B->cap;
+    P.direct = HT_B->direct ? 1 : 0;
     P.pos_len = R.pos_len;
     P.pos_off = R.pos_off;
     P.row_hits = R.row_hits;
