@@ -13,6 +13,7 @@ struct ProbeParams {
   const void* akey;               // HT_A rest keys (A's contracted tuple)
   const uint32_t* row_of_pos;     // HT_A slice_of
   KeyCodec acodec, bcodec;        // A's contracted codec, B's key codec
+  int same_codec;                 // the two codecs linearise identically
   const KtEntry* tab;             // HT_B entries
   uint64_t cap;                   // HT_B slots
   int direct;                     // HT_B addressed by key

@@ -77,14 +77,18 @@ __global__ void __launch_bounds__(256) k_probe(ProbeParams P) {
     uint32_t row = 0xFFFFFFFFu;
     if (act) {
       row = P.row_of_pos[p];
-      unsigned long long ka = (unsigned long long)akey[p];
-      // decode A's contracted tuple and re-encode it in B's key codec
-      unsigned long long kb = 0;
+      const unsigned long long ka = (unsigned long long)akey[p];
+      // A's contracted tuple in B's key codec (the same key when both
+      // codecs agree, as in a self-contraction)
+      unsigned long long kb = ka;
       bool ok = true;
-      for (uint32_t j = 0; j < P.acodec.n; ++j) {
-        uint32_t x = (uint32_t)((ka / P.acodec.stride[j]) % P.acodec.radix[j]);
-        if (x >= P.bcodec.radix[j]) ok = false;
-        kb += (unsigned long long)x * P.bcodec.stride[j];
+      if (!P.same_codec) {
+        kb = 0;
+        for (uint32_t j = 0; j < P.acodec.n; ++j) {
+          const uint32_t x = (uint32_t)((ka / P.acodec.stride[j]) % P.acodec.radix[j]);
+          if (x >= P.bcodec.radix[j]) ok = false;
+          kb += (unsigned long long)x * P.bcodec.stride[j];
+        }
       }
       if (ok && P.direct) {
         if (kb < P.cap) {
@@ -93,17 +97,26 @@ __global__ void __launch_bounds__(256) k_probe(ProbeParams P) {
           off = en.off;
         }
       } else if (ok) {
-        // same probe order as the insertion: line mix64(k) mod (cap/8), onward
+        // same probe order as the insertion: line mix64(k) mod (cap/8), then
+        // onward; a whole 8-entry line is loaded at once
         uint64_t i = (mix64d(kb) & (P.cap / 8 - 1)) * 8;
-        while (true) {
-          const KtEntry en = P.tab[i];
-          if (en.key == kb) {
-            len = en.len;
-            off = en.off;
-            break;
+        bool done = false;
+        while (!done) {
+          KtEntry en[8];
+#pragma unroll
+          for (int s = 0; s < 8; ++s) en[s] = P.tab[(i + s) & (P.cap - 1)];
+#pragma unroll
+          for (int s = 0; s < 8; ++s) {
+            if (done) continue;
+            if (en[s].key == kb) {
+              len = en[s].len;
+              off = en[s].off;
+              done = true;
+            } else if (en[s].key == PTC_EMPTY_KEY) {
+              done = true;
+            }
           }
-          if (en.key == PTC_EMPTY_KEY) break;
-          i = (i + 1) & (P.cap - 1);
+          i = (i + 8) & (P.cap - 1);
         }
       }
       P.pos_len[p] = len;

@@ -137,10 +137,12 @@ __global__ void k_kt_direct(const uint32_t* __restrict__ h, uint32_t d, uint64_t
   }
 }
 
-// Pass 1b: rank of every nonzero within its slot.  A block aggregates its
-// chunk of nonzeros by slot in an LDS table first, so each distinct slot of
-// the chunk costs one global atomicAdd (a Zipf-hot key would otherwise
-// serialise one global atomic per nonzero).
+// Pass 1b: rank of every nonzero within its slot.  Each wave sorts its 64
+// slot ids with a bitonic network over lanes, so equal slots form runs; a
+// run's head then finds the slot in the block's LDS table and takes the
+// run's ranks with one LDS atomicAdd (a Zipf-hot key would otherwise hit one
+// LDS counter from most lanes).  Finally each distinct slot of the block
+// costs one global atomicAdd.
 constexpr int RK_THREADS = 256;
 constexpr int RK_ITEMS = 8;
 constexpr int RK_CHUNK = RK_THREADS * RK_ITEMS;
@@ -151,6 +153,9 @@ __global__ void __launch_bounds__(RK_THREADS) k_kt_rank(const uint32_t* __restri
                                                           uint32_t* __restrict__ rank_of) {
   __shared__ uint32_t lkey[RK_SLOTS];
   __shared__ uint32_t lcnt[RK_SLOTS];
+  __shared__ uint32_t xh[RK_THREADS];  // per original lane: LDS slot of its key
+  __shared__ uint32_t xr[RK_THREADS];  // per original lane: rank within the block
+  const int lane = threadIdx.x & 63, wbase = threadIdx.x - lane;
   for (int i = threadIdx.x; i < RK_SLOTS; i += RK_THREADS) {
     lkey[i] = 0xFFFFFFFFu;
     lcnt[i] = 0;
@@ -158,24 +163,54 @@ __global__ void __launch_bounds__(RK_THREADS) k_kt_rank(const uint32_t* __restri
   __syncthreads();
   const uint64_t base = (uint64_t)blockIdx.x * RK_CHUNK;
   uint32_t ls[RK_ITEMS], lr[RK_ITEMS];
-#pragma unroll
   for (int k = 0; k < RK_ITEMS; ++k) {
     const uint64_t e = base + (uint64_t)k * RK_THREADS + threadIdx.x;
-    ls[k] = 0xFFFFFFFFu;
-    if (e >= N) continue;
-    const uint32_t s = slot_of[e];
-    uint32_t h = (s * 2654435761u) & (RK_SLOTS - 1);
-    while (true) {
-      uint32_t cur = lkey[h];
-      if (cur == s) break;
-      if (cur == 0xFFFFFFFFu) {
-        uint32_t old = atomicCAS(&lkey[h], 0xFFFFFFFFu, s);
-        if (old == 0xFFFFFFFFu || old == s) break;
+    uint32_t key = e < N ? slot_of[e] : 0xFFFFFFFFu;
+    uint32_t pl = (uint32_t)lane;
+    // bitonic sort of (key, lane) across the wave, ascending
+#pragma unroll
+    for (int kk = 2; kk <= 64; kk <<= 1) {
+#pragma unroll
+      for (int j = kk >> 1; j > 0; j >>= 1) {
+        const uint32_t ok = __shfl_xor(key, j, 64), op = __shfl_xor(pl, j, 64);
+        const bool up = (lane & kk) == 0;
+        const bool lower = (lane & j) == 0;
+        const bool other_less = ok < key || (ok == key && op < pl);
+        if (lower == up ? other_less : !other_less) {
+          key = ok;
+          pl = op;
+        }
       }
-      h = (h + 1) & (RK_SLOTS - 1);  // at most RK_CHUNK keys in 2*RK_CHUNK slots
     }
-    ls[k] = h;
-    lr[k] = atomicAdd(&lcnt[h], 1u);
+    const uint32_t prevk = __shfl_up(key, 1, 64);
+    const bool head = lane == 0 || prevk != key;
+    const unsigned long long hm = __ballot(head);
+    const unsigned long long upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+    const int start = 63 - __clzll((long long)(hm & upto));
+    const unsigned long long above = hm & ~upto;
+    const int next = above ? (__ffsll((long long)above) - 1) : 64;
+    uint32_t h = 0, rbase = 0;
+    if (head && key != 0xFFFFFFFFu) {
+      h = (key * 2654435761u) & (RK_SLOTS - 1);
+      while (true) {  // at most RK_CHUNK keys in 2*RK_CHUNK slots
+        const uint32_t cur = lkey[h];
+        if (cur == key) break;
+        if (cur == 0xFFFFFFFFu) {
+          const uint32_t old = atomicCAS(&lkey[h], 0xFFFFFFFFu, key);
+          if (old == 0xFFFFFFFFu || old == key) break;
+        }
+        h = (h + 1) & (RK_SLOTS - 1);
+      }
+      rbase = atomicAdd(&lcnt[h], (uint32_t)(next - start));
+    }
+    h = __shfl(h, start, 64);
+    rbase = __shfl(rbase, start, 64);
+    xh[wbase + pl] = h;
+    xr[wbase + pl] = rbase + (uint32_t)(lane - start);
+    __syncthreads();
+    ls[k] = xh[threadIdx.x];
+    lr[k] = xr[threadIdx.x];
+    __syncthreads();
   }
   __syncthreads();
   // one global atomic per distinct slot of the chunk; lcnt becomes the base

@@ -364,6 +364,9 @@ static int prepare(Run& R, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B,
     P.row_of_pos = HT_A->slice_of;
     P.acodec = HT_A->rest;
     P.bcodec = HT_B->codec;
+    P.same_codec = 1;
+    for (uint32_t j = 0; j < P.acodec.n; ++j)
+      if (P.acodec.radix[j] != P.bcodec.radix[j]) P.same_codec = 0;
     P.tab = HT_B->entries;
     P.cap = HT_B->cap;
     P.direct = HT_B->direct ? 1 : 0;
@@ -389,9 +392,15 @@ static int prepare(Run& R, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B,
   if ((rc = scan_u64(ctx, R.packed, R.packed, R.nrows, R.packed + R.nrows))) return rc;
   if ((rc = scan_u64(ctx, R.row_slab, R.row_slab, R.nrows, R.row_slab + R.nrows))) return rc;
   unsigned long long hostv[3] = {0, 0, 0};
-  if ((rc = read_back(ctx, &hostv[0], R.apre + R.npos, 8))) return rc;
-  if ((rc = read_back(ctx, &hostv[1], R.packed + R.nrows, 8))) return rc;
-  if ((rc = read_back(ctx, &hostv[2], R.row_slab + R.nrows, 8))) return rc;
+  {
+    // one host round trip for the three totals
+    unsigned long long* tot = ws.get<unsigned long long>(3);
+    if (!tot) return PTC_ERR_NOMEM;
+    PTC_HIP(ctx, hipMemcpyAsync(tot, R.apre + R.npos, 8, hipMemcpyDeviceToDevice, ctx->stream));
+    PTC_HIP(ctx, hipMemcpyAsync(tot + 1, R.packed + R.nrows, 8, hipMemcpyDeviceToDevice, ctx->stream));
+    PTC_HIP(ctx, hipMemcpyAsync(tot + 2, R.row_slab + R.nrows, 8, hipMemcpyDeviceToDevice, ctx->stream));
+    if ((rc = read_back(ctx, hostv, tot, 24))) return rc;
+  }
   R.nslabs = hostv[2];
   R.F = hostv[0];
   R.nitems = hostv[1] & 0xFFFFFFFFull;
