@@ -69,14 +69,9 @@ __device__ __forceinline__ unsigned long long mix64d(unsigned long long k) {
 template <class RestT>
 __global__ void __launch_bounds__(256) k_probe(ProbeParams P) {
   const RestT* __restrict__ akey = (const RestT*)P.akey;
-  const int lane = threadIdx.x & 63;
-  for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < P.npos; base += (uint64_t)gridDim.x * blockDim.x) {
-    uint64_t p = base + threadIdx.x;
-    bool act = p < P.npos;
+  for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < P.npos; p += (uint64_t)gridDim.x * blockDim.x) {
     uint32_t len = 0, off = 0;
-    uint32_t row = 0xFFFFFFFFu;
-    if (act) {
-      row = P.row_of_pos[p];
+    {
       const unsigned long long ka = (unsigned long long)akey[p];
       // A's contracted tuple in B's key codec (the same key when both
       // codecs agree, as in a self-contraction)
@@ -122,26 +117,49 @@ __global__ void __launch_bounds__(256) k_probe(ProbeParams P) {
       P.pos_len[p] = len;
       P.pos_off[p] = off;
     }
-    // wave-segmented reduction over rows (positions are row-grouped)
-    uint32_t hits = (act && len) ? 1u : 0u;
-    uint32_t mx = len;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-      uint32_t h2 = __shfl_down(hits, o, 64);
-      uint32_t m2 = __shfl_down(mx, o, 64);
-      uint32_t r2 = __shfl_down(row, o, 64);
-      if (lane + o < 64 && r2 == row) {
-        hits += h2;
-        mx = max(mx, m2);
-      }
-    }
-    uint32_t prev = __shfl_up(row, 1, 64);
-    bool head = act && (lane == 0 || prev != row);
-    if (head) {
-      if (hits) atomicAdd(&P.row_hits[row], hits);
-      if (mx) atomicMax(&P.row_maxlen[row], mx);
-    }
   }
+}
+
+// Per-row features for the learned method (A positions are row-grouped, so a
+// row is a contiguous range): contracted keys that hit B and the longest
+// slice.  One workgroup per row; no atomics.
+__global__ void __launch_bounds__(256) k_row_features(const KtSlice* __restrict__ rows, uint64_t nrows,
+                                                      const uint32_t* __restrict__ pos_len,
+                                                      uint32_t* __restrict__ row_hits,
+                                                      uint32_t* __restrict__ row_maxlen) {
+  __shared__ uint32_t sh[4], sm[4];
+  for (uint64_t r = blockIdx.x; r < nrows; r += gridDim.x) {
+    const KtSlice s = rows[r];
+    uint32_t h = 0, m = 0;
+    for (uint64_t p = s.off + threadIdx.x; p < (uint64_t)s.off + s.len; p += blockDim.x) {
+      const uint32_t l = pos_len[p];
+      h += l ? 1u : 0u;
+      m = max(m, l);
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      h += __shfl_down(h, o, 64);
+      m = max(m, (uint32_t)__shfl_down(m, o, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+      sh[threadIdx.x >> 6] = h;
+      sm[threadIdx.x >> 6] = m;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      row_hits[r] = sh[0] + sh[1] + sh[2] + sh[3];
+      row_maxlen[r] = max(max(sm[0], sm[1]), max(sm[2], sm[3]));
+    }
+    __syncthreads();
+  }
+}
+
+int launch_row_features(ptc_ctx* ctx, const KtSlice* rows, uint64_t nrows, const uint32_t* pos_len,
+                        uint32_t* row_hits, uint32_t* row_maxlen) {
+  if (!nrows) return PTC_OK;
+  hipLaunchKernelGGL(k_row_features, dim3((unsigned)std::min<uint64_t>(nrows, 65535)), dim3(256), 0, ctx->stream,
+                     rows, nrows, pos_len, row_hits, row_maxlen);
+  PTC_HIP(ctx, hipGetLastError());
+  return PTC_OK;
 }
 
 // ------------------------------------------------------------------ rows

@@ -226,6 +226,84 @@ __global__ void __launch_bounds__(RK_THREADS) k_kt_rank(const uint32_t* __restri
   }
 }
 
+// Direct tables with a small key space: a stable-offset counting sort.  Each
+// block counts its chunk per slot in LDS (wave-sorted runs, one LDS atomic
+// per run) and writes its counts into a slot-major matrix hist[slot][block];
+// one flat exclusive scan of that matrix then gives every (slot, block) its
+// first output position -- no global atomics, however hot a key is.
+constexpr uint32_t RD_MAX = 8192;
+
+__global__ void __launch_bounds__(RK_THREADS) k_kt_rank_direct(const uint32_t* __restrict__ slot_of, uint64_t N,
+                                                                 uint32_t space, uint64_t nblocks,
+                                                                 uint32_t* __restrict__ hist,
+                                                                 uint32_t* __restrict__ rank_of) {
+  __shared__ uint32_t lcnt[RD_MAX];
+  __shared__ uint32_t xr[RK_THREADS];
+  const int lane = threadIdx.x & 63, wbase = threadIdx.x - lane;
+  for (uint32_t i = threadIdx.x; i < space; i += RK_THREADS) lcnt[i] = 0;
+  __syncthreads();
+  const uint64_t base = (uint64_t)blockIdx.x * RK_CHUNK;
+  uint32_t lr[RK_ITEMS];
+  for (int k = 0; k < RK_ITEMS; ++k) {
+    const uint64_t e = base + (uint64_t)k * RK_THREADS + threadIdx.x;
+    uint32_t key = e < N ? slot_of[e] : 0xFFFFFFFFu;
+    uint32_t pl = (uint32_t)lane;
+#pragma unroll
+    for (int kk = 2; kk <= 64; kk <<= 1) {
+#pragma unroll
+      for (int j = kk >> 1; j > 0; j >>= 1) {
+        const uint32_t ok = __shfl_xor(key, j, 64), op = __shfl_xor(pl, j, 64);
+        const bool up = (lane & kk) == 0;
+        const bool lower = (lane & j) == 0;
+        const bool other_less = ok < key || (ok == key && op < pl);
+        if (lower == up ? other_less : !other_less) {
+          key = ok;
+          pl = op;
+        }
+      }
+    }
+    const uint32_t prevk = __shfl_up(key, 1, 64);
+    const bool head = lane == 0 || prevk != key;
+    const unsigned long long hm = __ballot(head);
+    const unsigned long long upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+    const int start = 63 - __clzll((long long)(hm & upto));
+    const unsigned long long above = hm & ~upto;
+    const int next = above ? (__ffsll((long long)above) - 1) : 64;
+    uint32_t rbase = 0;
+    if (head && key != 0xFFFFFFFFu) rbase = atomicAdd(&lcnt[key], (uint32_t)(next - start));
+    rbase = __shfl(rbase, start, 64);
+    xr[wbase + pl] = rbase + (uint32_t)(lane - start);
+    __syncthreads();
+    lr[k] = xr[threadIdx.x];
+    __syncthreads();
+  }
+  for (uint32_t i = threadIdx.x; i < space; i += RK_THREADS) hist[(uint64_t)i * nblocks + blockIdx.x] = lcnt[i];
+#pragma unroll
+  for (int k = 0; k < RK_ITEMS; ++k) {
+    const uint64_t e = base + (uint64_t)k * RK_THREADS + threadIdx.x;
+    if (e < N) rank_of[e] = lr[k];
+  }
+}
+
+__global__ void k_kt_direct_cnt(const unsigned long long* __restrict__ hpre, uint32_t space, uint64_t nblocks,
+                                uint32_t* __restrict__ cnt) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < space; i += (uint64_t)gridDim.x * blockDim.x)
+    cnt[i] = (uint32_t)(hpre[(i + 1) * nblocks] - hpre[i * nblocks]);
+}
+
+// Pass 4a for direct tables: position = hist prefix of (slot, block) + rank.
+__global__ void k_kt_scatter_direct(uint64_t N, const unsigned long long* __restrict__ slotinfo,
+                                    const unsigned long long* __restrict__ hpre, uint64_t nblocks,
+                                    const uint32_t* __restrict__ slot_of, const uint32_t* __restrict__ rank_of,
+                                    uint32_t* __restrict__ perm, uint32_t* __restrict__ slice_of) {
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < N; e += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t s = slot_of[e];
+    const uint64_t pos = hpre[(uint64_t)s * nblocks + e / RK_CHUNK] + rank_of[e];
+    perm[pos] = (uint32_t)e;
+    slice_of[pos] = (uint32_t)(slotinfo[s] >> 32);
+  }
+}
+
 __global__ void k_kt_clear(KtEntry* __restrict__ tab, uint32_t* __restrict__ cnt, uint64_t cap) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < cap; i += (uint64_t)gridDim.x * blockDim.x) {
     tab[i].key = PTC_EMPTY_KEY;
@@ -380,6 +458,9 @@ extern "C" int ptc_hashtable_build(ptc_ctx* ctx, const ptc_dtensor* t, ptc_hasht
   }
   hipStream_t st = ctx->stream;
   hipLaunchKernelGGL(k_kt_clear, dim3(grid_for(cap, 256)), dim3(256), 0, st, ht->entries, cnt, cap);
+  const uint64_t nblk = (N + RK_CHUNK - 1) / RK_CHUNK;
+  const bool counting = ht->direct && cap <= RD_MAX && cap * nblk <= (64ull << 20);
+  unsigned long long* hpre = nullptr;
   if (N && ht->direct) {
     hipLaunchKernelGGL(k_kt_direct, dim3(grid_for(N, 256)), dim3(256), 0, st, t->hedges, t->d, N, ht->codec,
                        slot_of);
@@ -388,9 +469,24 @@ extern "C" int ptc_hashtable_build(ptc_ctx* ctx, const ptc_dtensor* t, ptc_hasht
     hipLaunchKernelGGL(k_kt_insert<KT_GROUP>, dim3(g), dim3(KT_THREADS), 0, st, t->hedges, t->d, N, ht->codec,
                        ht->entries, cap, slot_of);
   }
-  if (N) {
-    hipLaunchKernelGGL(k_kt_rank, dim3((unsigned)((N + RK_CHUNK - 1) / RK_CHUNK)), dim3(RK_THREADS), 0, st,
-                       (const uint32_t*)slot_of, N, cnt, rank_of);
+  if (N && counting) {
+    uint32_t* hist = ws.get<uint32_t>(cap * nblk);
+    hpre = ws.get<unsigned long long>(cap * nblk + 1);
+    if (!hist || !hpre) {
+      ptc_hashtable_free(ht);
+      return PTC_ERR_NOMEM;
+    }
+    hipLaunchKernelGGL(k_kt_rank_direct, dim3((unsigned)nblk), dim3(RK_THREADS), 0, st, (const uint32_t*)slot_of, N,
+                       (uint32_t)cap, nblk, hist, rank_of);
+    if ((rc = scan_u32_to_u64(ctx, hist, hpre, cap * nblk, hpre + cap * nblk))) {
+      ptc_hashtable_free(ht);
+      return rc;
+    }
+    hipLaunchKernelGGL(k_kt_direct_cnt, dim3(grid_for(cap, 256)), dim3(256), 0, st, (const unsigned long long*)hpre,
+                       (uint32_t)cap, nblk, cnt);
+  } else if (N) {
+    hipLaunchKernelGGL(k_kt_rank, dim3((unsigned)nblk), dim3(RK_THREADS), 0, st, (const uint32_t*)slot_of, N, cnt,
+                       rank_of);
   }
   hipLaunchKernelGGL(k_kt_pack, dim3(grid_for(cap, 256)), dim3(256), 0, st, cnt, packed, cap);
   rc = scan_u64(ctx, packed, packed, cap, packed + cap);
@@ -418,8 +514,13 @@ extern "C" int ptc_hashtable_build(ptc_ctx* ctx, const ptc_dtensor* t, ptc_hasht
   hipLaunchKernelGGL(k_kt_finalize, dim3(grid_for(cap, 256)), dim3(256), 0, st, ht->entries, cnt, packed,
                      cap, ht->slices, ht->direct ? 1 : 0);
   if (N) {
-    hipLaunchKernelGGL(k_kt_scatter, dim3(grid_for(N, 256)), dim3(256), 0, st, N, (const unsigned long long*)packed,
-                       (const uint32_t*)slot_of, (const uint32_t*)rank_of, ht->perm, ht->slice_of);
+    if (counting)
+      hipLaunchKernelGGL(k_kt_scatter_direct, dim3(grid_for(N, 256)), dim3(256), 0, st, N,
+                         (const unsigned long long*)packed, (const unsigned long long*)hpre, nblk,
+                         (const uint32_t*)slot_of, (const uint32_t*)rank_of, ht->perm, ht->slice_of);
+    else
+      hipLaunchKernelGGL(k_kt_scatter, dim3(grid_for(N, 256)), dim3(256), 0, st, N, (const unsigned long long*)packed,
+                         (const uint32_t*)slot_of, (const uint32_t*)rank_of, ht->perm, ht->slice_of);
     if (rest64)
       hipLaunchKernelGGL((k_kt_gather<unsigned long long>), dim3(grid_for(N, 256)), dim3(256), 0, st, t->hedges,
                          t->vals, t->d, N, ht->rest, (const uint32_t*)ht->perm, ht->rest_key64, ht->gval);

@@ -79,6 +79,7 @@ struct Run {
   bool use_flags = true;
   uint32_t* overflow = nullptr;
   unsigned long long* out_off = nullptr;
+  bool want_features = false;  // per-row hit counts and longest slices (learned features)
   explicit Run(ptc_ctx* c) : ctx(c), ws(c) {}
 };
 
@@ -354,8 +355,6 @@ static int prepare(Run& R, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B,
     return PTC_ERR_NOMEM;
 
   // ---------------- preprocessing: probe, flop prefix, rows, items
-  if ((rc = fill_zero(ctx, R.row_hits, R.nrows * 4))) return rc;
-  if ((rc = fill_zero(ctx, R.row_maxlen, R.nrows * 4))) return rc;
   if ((rc = fill_zero(ctx, R.overflow, 16))) return rc;
   {
     ProbeParams P{};
@@ -376,6 +375,8 @@ static int prepare(Run& R, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B,
     P.row_maxlen = R.row_maxlen;
     if ((rc = launch_probe(ctx, P, HT_A->rest_key64 != nullptr))) return rc;
   }
+  if (opt.method == PTC_METHOD_LEARNED || R.want_features)
+    if ((rc = launch_row_features(ctx, HT_A->slices, R.nrows, R.pos_len, R.row_hits, R.row_maxlen))) return rc;
   if ((rc = scan_u32_to_u64(ctx, R.pos_len, R.apre, R.npos, R.apre + R.npos))) return rc;
   {
     RowParams P{};
@@ -707,6 +708,7 @@ extern "C" int ptc_row_profile(ptc_ctx* ctx, const ptc_hashtable* HT_A, const pt
   ptc_options opt;
   ptc_options_default(&opt);
   Run R(ctx);
+  R.want_features = true;
   int rc = prepare(R, HT_A, HT_B, A, B, opt);
   if (rc) return rc;
   *nrows_out = R.nrows;
