@@ -30,8 +30,10 @@
 
 namespace ptc {
 
+// Lanes per key of the cooperative insertion; 1 selects the fused
+// per-lane insert + wave-aggregated rank kernel.
 #ifndef PTC_KT_GROUP
-#define PTC_KT_GROUP 8
+#define PTC_KT_GROUP 1
 #endif
 constexpr int KT_GROUP = PTC_KT_GROUP;  // lanes per key (8: 8 x 16-byte entries = one 128-byte line)
 constexpr int KT_THREADS = 256;
@@ -124,6 +126,70 @@ __global__ void __launch_bounds__(KT_THREADS) k_kt_insert(
       // lost the race to a different key: re-read the same entries
     }
     if (sub == 0) slot_of[e] = slot;
+  }
+}
+
+// Pass 1 (fused, the default for hashed tables): every lane inserts its own
+// key with atomicCAS along the probe order above; then the wavefront sorts
+// its 64 slots with a bitonic network over lanes, and each run of equal
+// slots takes its ranks with one global atomicAdd on the slot's counter, so
+// a key repeated across the wave costs one atomic.
+__global__ void __launch_bounds__(KT_THREADS) k_kt_insert_rank(
+    const uint32_t* __restrict__ h, uint32_t d, uint64_t N, KeyCodec codec, KtEntry* __restrict__ tab,
+    uint64_t cap, uint32_t* __restrict__ cnt, uint32_t* __restrict__ slot_of, uint32_t* __restrict__ rank_of) {
+  __shared__ uint32_t xr[KT_THREADS];
+  const int lane = threadIdx.x & 63, wbase = threadIdx.x - lane;
+  const uint64_t nlines = cap / 8;
+  for (uint64_t e0 = (uint64_t)blockIdx.x * KT_THREADS; e0 < N; e0 += (uint64_t)gridDim.x * KT_THREADS) {
+    const uint64_t e = e0 + threadIdx.x;
+    uint32_t slot = 0xFFFFFFFFu;
+    if (e < N) {
+      unsigned long long key = 0;
+      encode_key(h, d, e, codec, &key);  // table's own tensor: always within radix
+      uint64_t i = (mix64(key) & (nlines - 1)) * 8;
+      while (true) {
+        const unsigned long long k = load_key_l2(&tab[i]);
+        if (k == key) break;
+        if (k == PTC_EMPTY_KEY) {
+          const unsigned long long old = atomicCAS(&tab[i].key, PTC_EMPTY_KEY, key);
+          if (old == PTC_EMPTY_KEY || old == key) break;
+          continue;  // lost the race to another key: re-read this entry
+        }
+        i = (i + 1) & (cap - 1);
+      }
+      slot = (uint32_t)i;
+      slot_of[e] = slot;
+    }
+    // rank: runs of equal slots across the wave
+    uint32_t key = slot, pl = (uint32_t)lane;
+#pragma unroll
+    for (int kk = 2; kk <= 64; kk <<= 1) {
+#pragma unroll
+      for (int j = kk >> 1; j > 0; j >>= 1) {
+        const uint32_t ok = __shfl_xor(key, j, 64), op = __shfl_xor(pl, j, 64);
+        const bool up = (lane & kk) == 0;
+        const bool lower = (lane & j) == 0;
+        const bool other_less = ok < key || (ok == key && op < pl);
+        if (lower == up ? other_less : !other_less) {
+          key = ok;
+          pl = op;
+        }
+      }
+    }
+    const uint32_t prevk = __shfl_up(key, 1, 64);
+    const bool head = lane == 0 || prevk != key;
+    const unsigned long long hm = __ballot(head);
+    const unsigned long long upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+    const int start = 63 - __clzll((long long)(hm & upto));
+    const unsigned long long above = hm & ~upto;
+    const int next = above ? (__ffsll((long long)above) - 1) : 64;
+    uint32_t rbase = 0;
+    if (head && key != 0xFFFFFFFFu) rbase = atomicAdd(&cnt[key], (uint32_t)(next - start));
+    rbase = __shfl(rbase, start, 64);
+    xr[wbase + pl] = rbase + (uint32_t)(lane - start);
+    __syncthreads();
+    if (e < N) rank_of[e] = xr[threadIdx.x];
+    __syncthreads();
   }
 }
 
@@ -464,6 +530,9 @@ extern "C" int ptc_hashtable_build(ptc_ctx* ctx, const ptc_dtensor* t, ptc_hasht
   if (N && ht->direct) {
     hipLaunchKernelGGL(k_kt_direct, dim3(grid_for(N, 256)), dim3(256), 0, st, t->hedges, t->d, N, ht->codec,
                        slot_of);
+  } else if (N && KT_GROUP == 1) {
+    hipLaunchKernelGGL(k_kt_insert_rank, dim3(grid_for(N, KT_THREADS, 16384)), dim3(KT_THREADS), 0, st, t->hedges,
+                       t->d, N, ht->codec, ht->entries, cap, cnt, slot_of, rank_of);
   } else if (N) {
     unsigned g = grid_for(N, KT_THREADS / KT_GROUP, 16384);
     hipLaunchKernelGGL(k_kt_insert<KT_GROUP>, dim3(g), dim3(KT_THREADS), 0, st, t->hedges, t->d, N, ht->codec,
@@ -484,7 +553,7 @@ extern "C" int ptc_hashtable_build(ptc_ctx* ctx, const ptc_dtensor* t, ptc_hasht
     }
     hipLaunchKernelGGL(k_kt_direct_cnt, dim3(grid_for(cap, 256)), dim3(256), 0, st, (const unsigned long long*)hpre,
                        (uint32_t)cap, nblk, cnt);
-  } else if (N) {
+  } else if (N && (ht->direct || KT_GROUP > 1)) {
     hipLaunchKernelGGL(k_kt_rank, dim3((unsigned)nblk), dim3(RK_THREADS), 0, st, (const uint32_t*)slot_of, N, cnt,
                        rank_of);
   }
