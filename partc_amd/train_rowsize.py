@@ -88,14 +88,13 @@ def collect(P, ctx, split, pool):
     specs = family_specs(split)
     Xs, us, ys = [], [], []
     t0 = time.time()
-    if True:
-        for i, item in enumerate(pool.imap(make_inputs, specs)):
-            X, ub, y = profile(P, ctx, item)
-            Xs.append(X)
-            us.append(ub)
-            ys.append(y)
-            if (i + 1) % 16 == 0:
-                print(f"[{split}] {i + 1}/{len(specs)} contractions profiled ({time.time() - t0:.1f}s)", flush=True)
+    for i, item in enumerate(pool.imap(make_inputs, specs)):
+        X, ub, y = profile(P, ctx, item)
+        Xs.append(X)
+        us.append(ub)
+        ys.append(y)
+        if (i + 1) % 8 == 0:
+            print(f"[{split}] {i + 1}/{len(specs)} contractions profiled ({time.time() - t0:.1f}s)", flush=True)
     return np.concatenate(Xs), np.concatenate(us), np.concatenate(ys)
 
 
