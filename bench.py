@@ -207,7 +207,7 @@ def main():
     achieved = kb / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
     traffic = load_traffic()
     roof = {
-        "kernel": "k_pass<NUMERIC,dense> (numeric row pass)",
+        "kernel": "numeric row pass: k_pass<NUMERIC,dense> row launch + its merge launch",
         "bound": "hbm",
         "achieved": round(achieved, 1),
         "peak": HBM_PEAK_GBS,

@@ -139,7 +139,7 @@ struct EmitParams {
 
 int launch_probe(ptc_ctx* ctx, const ProbeParams& P, bool rest64);
 int launch_rows(ptc_ctx* ctx, const RowParams& P);
-int launch_row_features(ptc_ctx* ctx, const KtSlice* rows, uint64_t nrows, const uint32_t* pos_len,
+int launch_row_features(ptc_ctx* ctx, const uint32_t* row_of_pos, uint64_t npos, const uint32_t* pos_len,
                         uint32_t* row_hits, uint32_t* row_maxlen);
 int launch_items(ptc_ctx* ctx, const ItemParams& P, const ItemP0Params& Q);
 int launch_hash_caps(ptc_ctx* ctx, const HashCapParams& P);

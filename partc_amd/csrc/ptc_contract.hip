@@ -120,44 +120,58 @@ __global__ void __launch_bounds__(256) k_probe(ProbeParams P) {
   }
 }
 
-// Per-row features for the learned method (A positions are row-grouped, so a
-// row is a contiguous range): contracted keys that hit B and the longest
-// slice.  One workgroup per row; no atomics.
-__global__ void __launch_bounds__(256) k_row_features(const KtSlice* __restrict__ rows, uint64_t nrows,
+// Per-row features for the learned method: contracted keys that hit B and
+// the longest slice.  A positions are row-grouped, so each thread reduces 16
+// consecutive positions, then a wave-segmented reduction leaves one atomic
+// per (wave, row) -- about npos/1024 atomics on the hottest row.
+constexpr int RF_ITEMS = 16;
+__global__ void __launch_bounds__(256) k_row_features(const uint32_t* __restrict__ row_of_pos, uint64_t npos,
                                                       const uint32_t* __restrict__ pos_len,
                                                       uint32_t* __restrict__ row_hits,
                                                       uint32_t* __restrict__ row_maxlen) {
-  __shared__ uint32_t sh[4], sm[4];
-  for (uint64_t r = blockIdx.x; r < nrows; r += gridDim.x) {
-    const KtSlice s = rows[r];
-    uint32_t h = 0, m = 0;
-    for (uint64_t p = s.off + threadIdx.x; p < (uint64_t)s.off + s.len; p += blockDim.x) {
-      const uint32_t l = pos_len[p];
-      h += l ? 1u : 0u;
-      m = max(m, l);
+  const int lane = threadIdx.x & 63;
+  const uint64_t p0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * RF_ITEMS;
+  uint32_t row = 0xFFFFFFFFu, h = 0, m = 0;
+  for (int k = 0; k < RF_ITEMS; ++k) {
+    const uint64_t p = p0 + k;
+    if (p >= npos) break;
+    const uint32_t r = row_of_pos[p], l = pos_len[p];
+    if (r != row) {
+      if (row != 0xFFFFFFFFu) {  // flush a finished run (rare: rows are long)
+        if (h) atomicAdd(&row_hits[row], h);
+        if (m) atomicMax(&row_maxlen[row], m);
+      }
+      row = r;
+      h = 0;
+      m = 0;
     }
-    for (int o = 32; o > 0; o >>= 1) {
-      h += __shfl_down(h, o, 64);
-      m = max(m, (uint32_t)__shfl_down(m, o, 64));
+    h += l ? 1u : 0u;
+    m = max(m, l);
+  }
+  // wave-segmented reduction of each thread's last run
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t h2 = __shfl_down(h, o, 64);
+    const uint32_t m2 = __shfl_down(m, o, 64);
+    const uint32_t r2 = __shfl_down(row, o, 64);
+    if (lane + o < 64 && r2 == row) {
+      h += h2;
+      m = max(m, m2);
     }
-    if ((threadIdx.x & 63) == 0) {
-      sh[threadIdx.x >> 6] = h;
-      sm[threadIdx.x >> 6] = m;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      row_hits[r] = sh[0] + sh[1] + sh[2] + sh[3];
-      row_maxlen[r] = max(max(sm[0], sm[1]), max(sm[2], sm[3]));
-    }
-    __syncthreads();
+  }
+  const uint32_t prev = __shfl_up(row, 1, 64);
+  if (row != 0xFFFFFFFFu && (lane == 0 || prev != row)) {
+    if (h) atomicAdd(&row_hits[row], h);
+    if (m) atomicMax(&row_maxlen[row], m);
   }
 }
 
-int launch_row_features(ptc_ctx* ctx, const KtSlice* rows, uint64_t nrows, const uint32_t* pos_len,
+int launch_row_features(ptc_ctx* ctx, const uint32_t* row_of_pos, uint64_t npos, const uint32_t* pos_len,
                         uint32_t* row_hits, uint32_t* row_maxlen) {
-  if (!nrows) return PTC_OK;
-  hipLaunchKernelGGL(k_row_features, dim3((unsigned)std::min<uint64_t>(nrows, 65535)), dim3(256), 0, ctx->stream,
-                     rows, nrows, pos_len, row_hits, row_maxlen);
+  if (!npos) return PTC_OK;
+  const uint64_t per_block = 256ull * RF_ITEMS;
+  hipLaunchKernelGGL(k_row_features, dim3((unsigned)((npos + per_block - 1) / per_block)), dim3(256), 0, ctx->stream,
+                     row_of_pos, npos, pos_len, row_hits, row_maxlen);
   PTC_HIP(ctx, hipGetLastError());
   return PTC_OK;
 }
@@ -813,7 +827,7 @@ static int launch_pass_t(ptc_ctx* ctx, const PassParams& P, uint64_t nitems, siz
   PTC_HIP(ctx, hipGetLastError());
   if (timed) {
     hipEventRecord(ctx->kev[ctx->nkev + 1], ctx->stream);
-    ctx->kev_pass[ctx->nkev / 2] = PASS;
+    ctx->kev_pass[ctx->nkev / 2] = PASS + (MERGE ? 2 : 0);
     ctx->nkev += 2;
   }
   return PTC_OK;

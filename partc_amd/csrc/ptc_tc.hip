@@ -375,8 +375,10 @@ static int prepare(Run& R, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B,
     P.row_maxlen = R.row_maxlen;
     if ((rc = launch_probe(ctx, P, HT_A->rest_key64 != nullptr))) return rc;
   }
-  if (opt.method == PTC_METHOD_LEARNED || R.want_features)
-    if ((rc = launch_row_features(ctx, HT_A->slices, R.nrows, R.pos_len, R.row_hits, R.row_maxlen))) return rc;
+  if (opt.method == PTC_METHOD_LEARNED || R.want_features) {
+    if ((rc = fill_zero(ctx, R.row_hits, R.nrows * 4)) || (rc = fill_zero(ctx, R.row_maxlen, R.nrows * 4))) return rc;
+    if ((rc = launch_row_features(ctx, HT_A->slice_of, R.npos, R.pos_len, R.row_hits, R.row_maxlen))) return rc;
+  }
   if ((rc = scan_u32_to_u64(ctx, R.pos_len, R.apre, R.npos, R.apre + R.npos))) return rc;
   {
     RowParams P{};
@@ -644,12 +646,14 @@ extern "C" int ptc_tc_fksCuckoo(ptc_ctx* ctx, const ptc_hashtable* HT_A, const p
       stats->ms_numeric = T.ms(1, 3);
     }
     stats->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
+    // pass time = its row-pass launch + its merge launch
     for (int i = 0; i + 1 < ctx->nkev; i += 2) {
       float t = 0.f;
       hipEventElapsedTime(&t, ctx->kev[i], ctx->kev[i + 1]);
-      if (ctx->kev_pass[i / 2] == PASS_NUMERIC) {
+      const int kind = ctx->kev_pass[i / 2];
+      if (kind % 2 == PASS_NUMERIC) {
         stats->ms_kernel_numeric += t;
-        stats->kernel_launches_numeric++;
+        if (kind == PASS_NUMERIC) stats->kernel_launches_numeric++;
       } else {
         stats->ms_kernel_symbolic += t;
       }
