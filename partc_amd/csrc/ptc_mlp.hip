@@ -91,17 +91,41 @@ __global__ void k_mlp_features(LearnedInputs in, float* __restrict__ X, float* _
   }
 }
 
+// Inference: one wavefront per row, lane i = hidden unit i.  The second
+// layer's weights are staged transposed (W2T[k][i]) so the 64 lanes read
+// consecutive LDS words; h1 is broadcast lane by lane with shuffles.
 __global__ void __launch_bounds__(256) k_mlp_predict(const float* __restrict__ wg, const float* __restrict__ X,
                                                      const float* __restrict__ ub, uint64_t n,
                                                      float* __restrict__ pred) {
-  __shared__ float w[NPARAM];
-  for (int i = threadIdx.x; i < NPARAM; i += blockDim.x) w[i] = wg[i];
+  static_assert(MLP_H == 64, "one lane per hidden unit");
+  __shared__ float w2t[MLP_H * MLP_H];
+  __shared__ float w1[MLP_H * MLP_IN], b1[MLP_H], b2[MLP_H], w3[MLP_H];
+  for (int i = threadIdx.x; i < MLP_H * MLP_H; i += blockDim.x) {
+    const int r = i / MLP_H, c = i % MLP_H;  // W2[r][c]
+    w2t[c * MLP_H + r] = wg[OFF_W2 + i];
+  }
+  for (int i = threadIdx.x; i < MLP_H * MLP_IN; i += blockDim.x) w1[i] = wg[OFF_W1 + i];
+  for (int i = threadIdx.x; i < MLP_H; i += blockDim.x) {
+    b1[i] = wg[OFF_B1 + i];
+    b2[i] = wg[OFF_B2 + i];
+    w3[i] = wg[OFF_W3 + i];
+  }
   __syncthreads();
-  for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (uint64_t)gridDim.x * blockDim.x) {
-    float x[MLP_IN], h1[MLP_H], h2[MLP_H];
-    for (int k = 0; k < MLP_IN; ++k) x[k] = X[r * MLP_IN + k];
-    float z = mlp_fwd(w, x, h1, h2);
-    pred[r] = ub[r] * sigmoidf_(z);
+  const float b3 = wg[OFF_B3];
+  const int lane = threadIdx.x & 63;
+  const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x / 64);
+  for (uint64_t r = (uint64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64; r < n; r += waves) {
+    float a = b1[lane];
+#pragma unroll
+    for (int k = 0; k < MLP_IN; ++k) a += w1[lane * MLP_IN + k] * X[r * MLP_IN + k];
+    const float h1 = a > 0.f ? a : 0.f;
+    float s = b2[lane];
+#pragma unroll 16
+    for (int k = 0; k < MLP_H; ++k) s += w2t[k * MLP_H + lane] * __shfl(h1, k, 64);
+    const float h2 = s > 0.f ? s : 0.f;
+    float z = w3[lane] * h2;
+    for (int o = 32; o > 0; o >>= 1) z += __shfl_down(z, o, 64);
+    if (lane == 0) pred[r] = ub[r] * sigmoidf_(z + b3);
   }
 }
 
