@@ -27,8 +27,9 @@
 //                the column space fits (a perfect hash), otherwise an
 //                open-addressing LDS hash table -- or, for rows too large for
 //                LDS, in an HBM hash table.  Heavy rows split over several
-//                workgroups merge their partials in HBM; the last workgroup
-//                of the row finishes it.  SYMBOLIC counts distinct columns
+//                workgroups write partial slabs to HBM; a second launch of
+//                the same kernel (MERGE) reduces a row's slabs in chunk
+//                order and finishes the row.  SYMBOLIC counts distinct columns
 //                per row; NUMERIC accumulates values and writes the row.
 //   k_emit       learned/upper-bound methods: copy rows from their
 //                capacity-sized staging area to the prefix-scanned output.

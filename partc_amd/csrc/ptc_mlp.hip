@@ -11,7 +11,7 @@
 //
 // Kernels (all FP32, one thread per sample or per weight):
 //   k_mlp_features  row arrays -> feature rows (on the device, per contraction)
-//   k_mlp_predict   weights staged in LDS, one thread per row
+//   k_mlp_predict   weights staged in LDS, one wavefront per row
 //   k_mlp_forward   training forward: stores H1, H2 and dL/dz
 //   k_mlp_backward  per-sample dH2, dH1
 //   k_mlp_wgrad     per-weight reduction over the batch (dW = dH^T H)
@@ -90,6 +90,10 @@ __global__ void k_mlp_features(LearnedInputs in, float* __restrict__ X, float* _
     ub[r] = (float)u;
   }
 }
+
+// Rows per 256-thread inference block: four wavefronts, two rows each, so a
+// few thousand rows still spread over every CU.
+constexpr int PREDICT_ROWS_PER_BLOCK = 8;
 
 // Inference: one wavefront per row, lane i = hidden unit i.  The second
 // layer's weights are staged transposed (W2T[k][i]) so the 64 lanes read
@@ -244,7 +248,7 @@ int learned_predict(ptc_ctx* ctx, const ptc_mlp* mlp, const LearnedInputs& in, f
   float* ub = ws.get<float>(in.nrows);
   if (!X || !ub) return PTC_ERR_NOMEM;
   hipLaunchKernelGGL(k_mlp_features, dim3(gridn(in.nrows, 256)), dim3(256), 0, ctx->stream, in, X, ub);
-  hipLaunchKernelGGL(k_mlp_predict, dim3(gridn(in.nrows, 256, 2048)), dim3(256), 0, ctx->stream,
+  hipLaunchKernelGGL(k_mlp_predict, dim3(gridn(in.nrows, PREDICT_ROWS_PER_BLOCK, 2048)), dim3(256), 0, ctx->stream,
                      (const float*)mlp->w, (const float*)X, (const float*)ub, in.nrows, pred);
   PTC_HIP(ctx, hipGetLastError());
   return PTC_OK;
@@ -449,7 +453,7 @@ extern "C" int ptc_mlp_predict(ptc_ctx* ctx, const ptc_mlp* m, const float* X, c
   if (rc) return rc;
   float* dp = ws.get<float>(n);
   if (!dp) return PTC_ERR_NOMEM;
-  hipLaunchKernelGGL(k_mlp_predict, dim3(gridn(n, 256, 2048)), dim3(256), 0, ctx->stream, (const float*)m->w,
+  hipLaunchKernelGGL(k_mlp_predict, dim3(gridn(n, PREDICT_ROWS_PER_BLOCK, 2048)), dim3(256), 0, ctx->stream, (const float*)m->w,
                      (const float*)dX, (const float*)du, n, dp);
   PTC_HIP(ctx, hipGetLastError());
   PTC_HIP(ctx, hipMemcpyAsync(pred_out, dp, sizeof(float) * n, hipMemcpyDeviceToHost, ctx->stream));
