@@ -57,12 +57,12 @@ def kernel_bytes(npos, F, d_b, c, nnz_c, d_o):
     return npos * 20 + F * (4 * (d_b - c) + 8) + nnz_c * (4 * d_o + 8)
 
 
-def run_method(P, ctx, dA, dB, method, steps, warmup):
+def run_method(P, ctx, dA, dB, method, steps, warmup, value_type=0):
     """Times `steps` whole contractions; returns per-step ms and stats."""
     import ctypes as C
 
     L = P.lib()
-    opts = P.options(method=method)
+    opts = P.options(method=method, value_type=value_type)
     st = P.Stats()
 
     def one():
@@ -101,6 +101,22 @@ def run_method(P, ctx, dA, dB, method, steps, warmup):
     }
 
 
+def f32_errors(P, ctx, dA, dB, method, value_type, ref):
+    """One untimed FP32 contraction; max relative error of its values (hi =
+    the sum rounded to FP32) against the FP64 oracle's output `ref`."""
+    HB, HA = P.buildHashTable(dB, ctx), P.buildHashTable(dA, ctx)
+    O = P.tc_fksCuckoo(HA, HB, download=False, method=method, value_type=value_type)
+    h = O.download().hedges_array
+    hi = O.download_f32().astype(np.float64)
+    o = np.lexsort(h.T[::-1])
+    h, hi = h[o], hi[o]
+    rh, rv = ref
+    if not np.array_equal(h, rh):
+        return {"coords_equal": False}
+    rel = np.abs(hi - rv) / np.abs(rv)
+    return {"coords_equal": True, "max_rel_err": float(rel.max()), "mean_rel_err": float(rel.mean())}
+
+
 def cpu_baseline(P, h, v):
     """The oracle restatement of the reference on this host: the whole
     contraction, one thread (the reference's numeric loop runs a team of
@@ -122,7 +138,12 @@ def cpu_baseline(P, h, v):
                    f"contraction {info['t_contract']:.2f}s; nnz_out {oh.shape[0]}"),
         "nnz_out": int(oh.shape[0]),
         "seconds": wall,
-    }
+    }, _sorted(oh, ov)
+
+
+def _sorted(h, v):
+    o = np.lexsort(h.T[::-1])
+    return h[o], v[o]
 
 
 def copy_calibration(mb: int, reps: int = 5) -> None:
@@ -161,6 +182,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--methods", default="two_pass,upper_bound,learned")
+    ap.add_argument("--no-f32", action="store_true", help="skip the FP32 compensated leg")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -263,11 +285,33 @@ def main():
     }
     if "two_pass" in results and head != "two_pass":
         out["speedup_vs_two_pass"] = results["two_pass"]["ms_per_step"] / r["ms_per_step"]
+    ref = None
     if not args.no_cpu and rank == 0:
-        out["cpu_baseline"] = cpu_baseline(P, h, v)
+        out["cpu_baseline"], ref = cpu_baseline(P, h, v)
         cb = out["cpu_baseline"]
         if cb["nnz_out"] != nnz_c:
             out["cpu_baseline"]["note"] = f"nnz mismatch: cpu {cb['nnz_out']} gpu {nnz_c}"
+    if not args.no_f32:
+        # the same tensors with values rounded to FP32: two-sum compensated
+        # FP32 accumulation, exact products accumulated in FP64, and the plain
+        # FP32 path for comparison
+        fA = P.DeviceTensor.upload_f32(A, ctx)
+        fB = P.DeviceTensor.upload_f32(B, ctx)
+        f32 = {}
+        for name in args.methods.split(","):
+            if name not in avail:
+                continue
+            for vt_name, vt in (("twosum", P.VAL_F32_TWOSUM), ("f64acc", P.VAL_F32_F64ACC),
+                                ("plain", P.VAL_F32_PLAIN)):
+                x = run_method(P, ctx, fA, fB, avail[name], args.steps, args.warmup, value_type=vt)
+                rec = {"ms_per_step": round(x["ms_per_step"], 4), "MADps": F / (x["ms_per_step"] * 1e-3),
+                       "kernel_numeric_ms": round(x["kernel_ms_per_launch"], 4),
+                       "fallback_rows": int(x["stats"]["fallback_rows"])}
+                if ref is not None:
+                    rec.update(f32_errors(P, ctx, fA, fB, avail[name], vt, ref))
+                f32[f"{name}_{vt_name}"] = rec
+                print(f"# f32 {name} {vt_name}: {rec['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
+        out["f32"] = f32
     if rank == 0:
         print(json.dumps(out), flush=True)
 

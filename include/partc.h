@@ -75,6 +75,12 @@ void* ptc_stream(ptc_ctx* ctx);
 /* Copy a host tensor to the device (the reference keeps every tensor in host
  * malloc memory; main.cpp:59-116).  Per-mode maxima are computed here. */
 int ptc_dtensor_upload(ptc_ctx* ctx, const ptc_tensor_view* t, ptc_dtensor** out);
+/* The same for a tensor with FP32 values (t->vals is ignored; vals holds N
+ * floats).  Key tables built on it stream 4-byte values, and contracting two
+ * FP32 tensors needs an FP32 value_type (PTC_VAL_F32_*).  No
+ * reference counterpart: the reference's Tensor holds doubles only
+ * (tensor.hpp:25). */
+int ptc_dtensor_upload_f32(ptc_ctx* ctx, const ptc_tensor_view* t, const float* vals, ptc_dtensor** out);
 /* Wrap device arrays that the caller owns (not freed by ptc_dtensor_free).
  * mode_max may be NULL: the maxima are then reduced on the device. */
 int ptc_dtensor_wrap(ptc_ctx* ctx, uint32_t d, uint32_t num_key_dimensions, uint64_t N,
@@ -84,14 +90,17 @@ int ptc_dtensor_wrap(ptc_ctx* ctx, uint32_t d, uint32_t num_key_dimensions, uint
  * main.cpp:68-109 does with a full copy to build A_copy). */
 int ptc_dtensor_set_keys(ptc_dtensor* t, uint32_t num_key_dimensions, const uint32_t* dim_tc);
 int ptc_dtensor_info(const ptc_dtensor* t, uint32_t* d, uint64_t* N, uint32_t* num_key_dimensions);
-/* Device pointers of a tensor (hedges, vals; vals32 for FP32 outputs). */
+/* Device pointers of a tensor (hedges; vals for FP64, vals32 for FP32). */
 int ptc_dtensor_device_ptrs(const ptc_dtensor* t, const uint32_t** hedges, const double** vals,
                             const float** vals32);
-/* Copy a device tensor back.  hedges: N*d uint32, vals: N float64 (for an
- * FP32 result the values are widened).  dims may be NULL. */
+/* Copy a device tensor back.  hedges: N*d uint32, vals: N float64 (FP32
+ * values are widened exactly).  dims may be NULL. */
 int ptc_dtensor_download(ptc_ctx* ctx, const ptc_dtensor* t, uint32_t* hedges, double* vals,
                          uint32_t* dims);
-int ptc_dtensor_download_f32(ptc_ctx* ctx, const ptc_dtensor* t, float* vals);
+/* FP32 values of a tensor.  For an FP32 contraction result, lo (may be NULL)
+ * receives the compensation: hi = vals is the sum rounded to FP32 and
+ * hi + lo carries it to about twice FP32 precision. */
+int ptc_dtensor_download_f32(ptc_ctx* ctx, const ptc_dtensor* t, float* vals, float* lo);
 void ptc_dtensor_free(ptc_dtensor* t);
 
 /* ------------------------------------------------------------ .meta spec */
@@ -133,7 +142,15 @@ typedef enum {
 
 typedef enum {
   PTC_VAL_F64 = 0,        /* default and oracle path                          */
-  PTC_VAL_F32_TWOSUM = 1  /* FP32 values, TwoSum-compensated accumulation    */
+  PTC_VAL_F32_TWOSUM = 1, /* FP32 values; products and sums compensated by
+                             TwoProduct (fma) + TwoSum, error ~ FP32 rounding
+                             of the exact result whatever the row length    */
+  PTC_VAL_F32_PLAIN = 2,  /* FP32 values, plain FP32 atomic accumulation
+                             (the uncompensated baseline)                    */
+  PTC_VAL_F32_F64ACC = 3  /* FP32 values, exact FP32 x FP32 products summed in
+                             FP64 accumulators (gfx950's FP64 LDS atomic adds
+                             run several times faster than its FP32 ones);
+                             the same (hi, lo) FP32 pair out                 */
 } ptc_value_type;
 
 typedef enum {
@@ -152,7 +169,7 @@ typedef struct {
   uint64_t chunk_flops;   /* target multiply-adds per workgroup (0 = default) */
   double predict_scale;   /* learned path: capacity = scale * prediction (0 = default 1.15) */
   const ptc_mlp* mlp;     /* learned path: predictor (NULL = built-in)       */
-  int deterministic;      /* 1: one wavefront per row in A order (bit-stable values) */
+  int deterministic;      /* reserved, must be 0 (PTC_ERR_UNSUPPORTED otherwise) */
 } ptc_options;
 
 typedef struct {

@@ -45,7 +45,7 @@ ERRORS = {
 }
 
 METHOD_TWO_PASS, METHOD_LEARNED, METHOD_UPPER_BOUND = 0, 1, 2
-VAL_F64, VAL_F32_TWOSUM = 0, 1
+VAL_F64, VAL_F32_TWOSUM, VAL_F32_PLAIN, VAL_F32_F64ACC = 0, 1, 2, 3
 ACC_AUTO, ACC_DENSE, ACC_HASH, ACC_SPILL = 0, 1, 2, 3
 DIST_UNIFORM, DIST_ZIPF = 0, 1
 
@@ -127,12 +127,13 @@ def lib() -> C.CDLL:
         "ptc_synchronize": (i32, [vp]),
         "ptc_stream": (vp, [vp]),
         "ptc_dtensor_upload": (i32, [vp, C.POINTER(TensorView), C.POINTER(vp)]),
+        "ptc_dtensor_upload_f32": (i32, [vp, C.POINTER(TensorView), C.POINTER(C.c_float), C.POINTER(vp)]),
         "ptc_dtensor_wrap": (i32, [vp, u32, u32, u64, vp, vp, P32, P32, P32, C.POINTER(vp)]),
         "ptc_dtensor_set_keys": (i32, [vp, u32, P32]),
         "ptc_dtensor_info": (i32, [vp, P32, P64, P32]),
         "ptc_dtensor_device_ptrs": (i32, [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp)]),
         "ptc_dtensor_download": (i32, [vp, vp, P32, PD, P32]),
-        "ptc_dtensor_download_f32": (i32, [vp, vp, C.POINTER(C.c_float)]),
+        "ptc_dtensor_download_f32": (i32, [vp, vp, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
         "ptc_dtensor_free": (None, [vp]),
         "ptc_meta_parse": (i32, [C.c_char_p, u32, P32, P32]),
         "ptc_hashtable_build": (i32, [vp, vp, C.POINTER(vp)]),
@@ -325,6 +326,17 @@ class DeviceTensor:
         _check(lib().ptc_dtensor_upload(ctx.handle, C.byref(v), C.byref(h)), ctx)
         return cls(ctx, h)
 
+    @classmethod
+    def upload_f32(cls, t: Tensor, ctx: Optional[Context] = None) -> "DeviceTensor":
+        """Upload with the values rounded to FP32 (nearest even)."""
+        ctx = ctx or default_context()
+        h = C.c_void_p()
+        v = t.view()
+        v32 = np.ascontiguousarray(t.val_array, dtype=np.float32)
+        _check(lib().ptc_dtensor_upload_f32(ctx.handle, C.byref(v), v32.ctypes.data_as(C.POINTER(C.c_float)),
+                                            C.byref(h)), ctx)
+        return cls(ctx, h, keep=v32)
+
     def info(self):
         d, n, nk = C.c_uint32(), C.c_uint64(), C.c_uint32()
         _check(lib().ptc_dtensor_info(self._h, C.byref(d), C.byref(n), C.byref(nk)))
@@ -342,12 +354,16 @@ class DeviceTensor:
         _check(lib().ptc_dtensor_download(self.ctx.handle, self._h, _p32(h), _pd(v), _p32(dims)), self.ctx)
         return Tensor(h, v, dims[:d].copy(), np.arange(d, dtype=np.uint32), nk)
 
-    def download_f32(self) -> np.ndarray:
+    def download_f32(self, with_lo: bool = False):
+        """FP32 values; with_lo=True also returns the lo part of an FP32
+        contraction result (hi + lo ~ the sum to twice FP32 precision)."""
         d, n, nk = self.info()
         v = np.empty(n, dtype=np.float32)
-        _check(lib().ptc_dtensor_download_f32(self.ctx.handle, self._h, v.ctypes.data_as(C.POINTER(C.c_float))),
-               self.ctx)
-        return v
+        lo = np.empty(n, dtype=np.float32) if with_lo else None
+        pf = C.POINTER(C.c_float)
+        _check(lib().ptc_dtensor_download_f32(self.ctx.handle, self._h, v.ctypes.data_as(pf),
+                                              lo.ctypes.data_as(pf) if with_lo else None), self.ctx)
+        return (v, lo) if with_lo else v
 
     def free(self) -> None:
         if self._h:

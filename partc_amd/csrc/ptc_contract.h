@@ -89,9 +89,11 @@ struct PassParams {
   const uint32_t* pos_len;
   const uint32_t* pos_off;
   const double* aval;
+  const float* aval32;                // FP32 path: A values in grouped order
   // B slices
   const void* bcol;
   const double* bval;
+  const float* bval32;                // FP32 path: B values in grouped order
   uint64_t nb;                        // number of B nonzeros
   uint64_t ncols;                     // column space (dense accumulators)
   // heavy dense rows: symbolic bitmaps (one per heavy row); numeric slabs
@@ -117,6 +119,10 @@ struct PassParams {
   double* stg_val;
   uint32_t* out_hedges;               // exact mode: final output
   double* out_vals;
+  float* out_vals32;                  // FP32 path: hi = fl(sum) and lo = sum - hi
+  float* out_err32;
+  int f32;                            // FP32 values in, (hi, lo) FP32 pairs out
+  int val;                            // ptc_value_type: accumulation mode of the numeric pass
   uint32_t dO, nrowmodes;
   KeyCodec rowc, colc;
 };
@@ -133,6 +139,9 @@ struct EmitParams {
   const double* stg_val;
   uint32_t* out_hedges;
   double* out_vals;
+  float* out_vals32;
+  float* out_err32;
+  int f32;
   uint32_t dO, nrowmodes;
   KeyCodec rowc, colc;
 };

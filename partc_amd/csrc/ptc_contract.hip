@@ -281,13 +281,15 @@ __device__ __forceinline__ uint32_t block_rank(bool f, uint32_t* s_w, uint32_t* 
   return base + below;
 }
 
-// write one output entry at position pos (row coordinates from LDS)
-template <class ColT>
+// write one output entry at position pos (row coordinates from LDS); an
+// FP32 result is the pair (hi, lo) with hi = fl(sum), lo = sum - hi
+template <class ColT, bool F32>
 __device__ __forceinline__ void emit_entry(const PassParams& P, uint64_t pos, const uint32_t* s_rowc,
-                                           ColT col, double v) {
+                                           ColT col, double v, float hi, float lo) {
   if (P.stg_col) {
     ((ColT*)P.stg_col)[pos] = col;
-    P.stg_val[pos] = v;
+    if (F32) ((float2*)P.stg_val)[pos] = make_float2(hi, lo);
+    else P.stg_val[pos] = v;
     return;
   }
   uint32_t* o = P.out_hedges + pos * P.dO;
@@ -301,7 +303,12 @@ __device__ __forceinline__ void emit_entry(const PassParams& P, uint64_t pos, co
     for (uint32_t j = 0; j < PTC_MAX_MODES; ++j)
       if (j < P.colc.n) o[j] = (uint32_t)(((unsigned long long)col / P.colc.stride[j]) % P.colc.radix[j]);
   }
-  P.out_vals[pos] = v;
+  if (F32) {
+    P.out_vals32[pos] = hi;
+    P.out_err32[pos] = lo;
+  } else {
+    P.out_vals[pos] = v;
+  }
 }
 
 // Insert-or-find col in an open-addressing LDS table of S slots (power of
@@ -356,6 +363,57 @@ __device__ __forceinline__ long long hbm_insert(ColT* keys, uint32_t mask, ColT 
 __device__ __forceinline__ bool touched(double v) {
   return (unsigned long long)__double_as_longlong(v) != PTC_NEG_ZERO_BITS;
 }
+__device__ __forceinline__ bool touched32(float v) { return __float_as_uint(v) != 0x80000000u; }
+
+// FP32 compensated accumulation (error-free transformations):
+//   TwoProduct  p = fl(a*b), ep = fma(a, b, -p)          (a*b = p + ep exactly)
+//   TwoSum      s' = fl(o + p), e = (o - (s' - b')) + (p - b'), b' = s' - o
+//                                                        (o + p = s' + e exactly)
+// The sum lives in s and is updated by a hardware atomic add that returns
+// the old value o; because the add rounds to nearest like the local o + p,
+// the error e of that very update is recomputed locally and added, with ep,
+// into the compensation word c.  The row's value is s + c: the products are
+// summed as in twice the working precision, whatever the order of the
+// atomics (Ogita, Rump and Oishi, "Accurate sum and dot product", SIAM J.
+// Sci. Comput. 26(6), 2005, Algorithm Dot2, with the additions made atomic).
+__device__ __forceinline__ float two_sum_err(float o, float p, float sn) {
+  const float bp = sn - o;
+  return (o - (sn - bp)) + (p - bp);
+}
+template <bool COMP>
+__device__ __forceinline__ void acc_f32_lds(float* s, float* c, float a, float b) {
+  const float p = a * b;
+  if (COMP) {
+    const float o = atomicAdd(s, p);
+    const float e = two_sum_err(o, p, o + p) + fmaf(a, b, -p);
+    if (e != 0.f) atomicAdd(c, e);
+  } else {
+    atomicAdd(s, p);
+  }
+}
+template <bool COMP>
+__device__ __forceinline__ void acc_f32_hbm(float* s, float* c, float a, float b) {
+  const float p = a * b;
+  if (COMP) {
+    const float o = unsafeAtomicAdd(s, p);
+    const float e = two_sum_err(o, p, o + p) + fmaf(a, b, -p);
+    if (e != 0.f) unsafeAtomicAdd(c, e);
+  } else {
+    unsafeAtomicAdd(s, p);
+  }
+}
+// (s, c) -> (hi, lo): hi = fl(s + c), lo = the rounding error of that add
+template <bool COMP>
+__device__ __forceinline__ void f32_finish(float s, float c, float* hi, float* lo) {
+  if (!COMP) {
+    *hi = s;
+    *lo = 0.f;
+    return;
+  }
+  const float h = s + c;
+  *hi = h;
+  *lo = two_sum_err(s, c, h);
+}
 
 // LDS layout of one row pass (dynamic shared memory):
 //   stage:  s_start[STAGE] u32, s_bias[STAGE] i64, s_aval[STAGE] f64
@@ -371,12 +429,20 @@ __device__ __forceinline__ bool touched(double v) {
 // back and invalidates the whole L2, is ever issued): one workgroup per split
 // or HBM row reduces the slabs in chunk order (deterministic) and finishes
 // the row.
-template <int PASS, bool DENSE, class ColT, bool MERGE>
+// VAL (ptc_value_type): 0 FP64 values; 1 FP32 values, FP32 accumulators
+// with TwoProduct + TwoSum compensation; 2 FP32 values, plain FP32
+// accumulators; 3 FP32 values, exact products accumulated in FP64.
+template <int PASS, bool DENSE, class ColT, bool MERGE, int VAL>
 __global__ void __launch_bounds__(PB) k_pass(PassParams P) {
+  constexpr bool F32 = VAL == 1 || VAL == 2;  // FP32 (sum, compensation) accumulators
+  constexpr bool F32IN = VAL != 0;            // FP32 values in, (hi, lo) FP32 pairs out
+  constexpr bool COMP = VAL == 1;
+  static_assert(!F32IN || PASS == PASS_NUMERIC, "FP32 values only matter to the numeric pass");
   extern __shared__ __align__(16) unsigned char smem[];
   uint32_t* s_start = (uint32_t*)smem;
   long long* s_bias = (long long*)(smem + STAGE * 4);
   double* s_aval = (double*)(smem + STAGE * 12);
+  float* s_aval32 = (float*)s_aval;  // FP32: the first half of the same region
   unsigned char* acc_base = smem + STAGE * 20;
   __shared__ uint32_t s_w[PB / 64];
   __shared__ uint32_t s_misc[4];
@@ -409,6 +475,8 @@ __global__ void __launch_bounds__(PB) k_pass(PassParams P) {
 
   // accumulator storage
   double* acc = (double*)acc_base;
+  float* acc_s = (float*)acc_base;  // FP32: sums [ncols], then compensations [ncols]
+  float* acc_c = acc_s + ncols;
   unsigned char* flag = acc_base + ncols * 8;
   uint32_t* bits = (uint32_t*)acc_base;
   ColT* hkeys = (ColT*)acc_base;
@@ -426,6 +494,25 @@ __global__ void __launch_bounds__(PB) k_pass(PassParams P) {
         uint32_t o = 0;
         for (uint32_t kk = 0; kk < nch; ++kk) o |= P.hbits[(slab0 + kk) * nbits + w];
         bits[w] = o;
+      }
+    } else if (F32) {
+      // slabs hold (sum, compensation) pairs; combine them in chunk order
+      const float2* h2 = (const float2*)P.hacc;
+      for (uint64_t c = threadIdx.x; c < ncols; c += PB) {
+        float S = -0.0f, Cc = 0.f;
+        unsigned char fl = 0;
+#pragma unroll 4
+        for (uint32_t kk = 0; kk < nch; ++kk) {
+          const float2 x = h2[(slab0 + kk) * ncols + c];
+          const float sn = S + x.x;
+          if (COMP) Cc += x.y + two_sum_err(S, x.x, sn);
+          S = sn;
+        }
+        if (flags)
+          for (uint32_t j = 0; j < nch; ++j) fl |= P.hflag[(slab0 + j) * ncols + c];
+        acc_s[c] = S;
+        acc_c[c] = Cc;
+        if (flags) flag[c] = fl;
       }
     } else {
       for (uint64_t c = threadIdx.x; c < ncols; c += PB) {
@@ -450,6 +537,13 @@ __global__ void __launch_bounds__(PB) k_pass(PassParams P) {
   } else if (DENSE) {
     if (PASS == PASS_SYMBOLIC) {
       for (uint32_t w = threadIdx.x; w < nbits; w += PB) bits[w] = 0;
+    } else if (F32) {
+      for (uint64_t c = threadIdx.x; c < ncols; c += PB) {
+        acc_s[c] = -0.0f;
+        acc_c[c] = 0.f;
+      }
+      if (flags)
+        for (uint64_t c = threadIdx.x; c < ncols; c += PB) flag[c] = 0;
     } else {
       const double nz = __longlong_as_double((long long)PTC_NEG_ZERO_BITS);
       for (uint64_t c = threadIdx.x; c < ncols; c += PB) acc[c] = nz;
@@ -475,6 +569,8 @@ __global__ void __launch_bounds__(PB) k_pass(PassParams P) {
   if (threadIdx.x < 4) s_misc[threadIdx.x] = 0;  // [0] LDS distinct count, [3] table full
   __syncthreads();
   double* hvals = (double*)(acc_base + (size_t)S * sizeof(ColT));
+  float* hs32 = (float*)hvals;  // FP32: sums [S], then compensations [S]
+  float* hc32 = hs32 + S;
 
   // ---- expand the flop range [g0, g1) ----
   uint64_t q0 = MERGE ? pend : it.z;
@@ -521,7 +617,10 @@ __global__ void __launch_bounds__(PB) k_pass(PassParams P) {
       const unsigned long long a = P.apre[q];
       s_start[dst] = (uint32_t)((a > g0 ? a : g0) - g0);
       s_bias[dst] = (long long)P.pos_off[q] + (long long)g0 - (long long)a;
-      if (PASS == PASS_NUMERIC) s_aval[dst] = P.aval[q];
+      if (PASS == PASS_NUMERIC) {
+        if (F32IN) s_aval32[dst] = P.aval32[q];
+        else s_aval[dst] = P.aval[q];
+      }
       ++dst;
     }
     const unsigned long long qend = (qlim < pend) ? P.apre[qlim] : (flo + fr);
@@ -584,14 +683,54 @@ __global__ void __launch_bounds__(PB) k_pass(PassParams P) {
         }
         ColT col[U];
         double bv[U];
+        float bv32[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
           col[u] = 0;
           bv[u] = 0.0;
+          bv32[u] = 0.f;
           if (act[u]) {
             col[u] = bcol[bb[u]];
-            if (PASS == PASS_NUMERIC) bv[u] = P.bval[bb[u]];
+            if (PASS == PASS_NUMERIC) {
+              if (F32IN) bv32[u] = P.bval32[bb[u]];
+              else bv[u] = P.bval[bb[u]];
+            }
           }
+        }
+        if constexpr (F32 && DENSE) {
+          // all U sum updates first (a returning atomic each when
+          // compensating), then the U compensation updates: one wait on LDS
+          // returns covers the whole group
+          float o[U], pr[U], ep[U];
+          bool ok[U];
+#pragma unroll
+          for (int u = 0; u < U; ++u) {
+            ok[u] = act[u] && (uint64_t)col[u] < ncols;
+            if (act[u] && !ok[u]) atomicOr(P.overflow, 0x40000000u);
+            o[u] = 0.f;
+            pr[u] = 0.f;
+            ep[u] = 0.f;
+            if (ok[u]) {
+              const float a = s_aval32[ii[u]], b = bv32[u];
+              pr[u] = a * b;
+              if (COMP) {
+                ep[u] = fmaf(a, b, -pr[u]);
+                o[u] = atomicAdd(&acc_s[col[u]], pr[u]);
+              } else {
+                atomicAdd(&acc_s[col[u]], pr[u]);
+              }
+              if (flags) flag[col[u]] = 1;
+            }
+          }
+          if (COMP) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+              if (!ok[u]) continue;
+              const float e = two_sum_err(o[u], pr[u], o[u] + pr[u]) + ep[u];
+              if (e != 0.f) atomicAdd(&acc_c[col[u]], e);
+            }
+          }
+          continue;
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -603,19 +742,32 @@ __global__ void __launch_bounds__(PB) k_pass(PassParams P) {
             } else if (PASS == PASS_SYMBOLIC) {
               atomicOr(&bits[c >> 5], 1u << (c & 31));
             } else {
-              atomicAdd(&acc[c], s_aval[ii[u]] * bv[u]);
+              if (F32) acc_f32_lds<COMP>(&acc_s[c], &acc_c[c], s_aval32[ii[u]], bv32[u]);
+              else if (F32IN) atomicAdd(&acc[c], (double)s_aval32[ii[u]] * (double)bv32[u]);
+              else atomicAdd(&acc[c], s_aval[ii[u]] * bv[u]);
               if (flags) flag[c] = 1;
             }
           } else {
-            const double v = PASS == PASS_NUMERIC ? s_aval[ii[u]] * bv[u] : 0.0;
+            const double v = (PASS != PASS_NUMERIC || F32) ? 0.0
+                           : F32IN ? (double)s_aval32[ii[u]] * (double)bv32[u] : s_aval[ii[u]] * bv[u];
             if (S) {
               const int h = lds_insert<ColT>(hkeys, S, c, PASS == PASS_SYMBOLIC ? &s_misc[0] : nullptr);
               if (h < 0) s_misc[3] = 1u;
-              else if (PASS == PASS_NUMERIC) atomicAdd(&hvals[h], v);
+              else if (PASS == PASS_NUMERIC) {
+                if (F32) acc_f32_lds<COMP>(&hs32[h], &hc32[h], s_aval32[ii[u]], bv32[u]);
+                else atomicAdd(&hvals[h], v);
+              }
             } else {
               const long long h = hbm_insert<ColT>(gkeys, gmask, c, PASS == PASS_SYMBOLIC ? &P.row_nnz[r] : nullptr);
               if (h < 0) atomicOr(P.overflow, 0x80000000u);  // cannot happen: sized from exact or bound counts
-              else if (PASS == PASS_NUMERIC) unsafeAtomicAdd(&gvals[h], v);
+              else if (PASS == PASS_NUMERIC) {
+                if (F32) {
+                  float* g2 = (float*)gvals;  // (sum, compensation) pairs
+                  acc_f32_hbm<COMP>(&g2[2 * h], &g2[2 * h + 1], s_aval32[ii[u]], bv32[u]);
+                } else {
+                  unsafeAtomicAdd(&gvals[h], v);
+                }
+              }
             }
           }
         }
@@ -632,6 +784,13 @@ __global__ void __launch_bounds__(PB) k_pass(PassParams P) {
       if (PASS == PASS_SYMBOLIC) {
         uint32_t* gb = P.hbits + slab * nbits;
         for (uint32_t w = threadIdx.x; w < nbits; w += PB) gb[w] = bits[w];
+      } else if (F32) {
+        float2* ga = (float2*)P.hacc + slab * ncols;
+        for (uint64_t c = threadIdx.x; c < ncols; c += PB) ga[c] = make_float2(acc_s[c], acc_c[c]);
+        if (flags) {
+          unsigned char* gf = P.hflag + slab * ncols;
+          for (uint64_t c = threadIdx.x; c < ncols; c += PB) gf[c] = flag[c];
+        }
       } else {
         double* ga = P.hacc + slab * ncols;
         for (uint64_t c = threadIdx.x; c < ncols; c += PB) ga[c] = acc[c];
@@ -680,7 +839,7 @@ __global__ void __launch_bounds__(PB) k_pass(PassParams P) {
   uint32_t cnt = 0;
   for (uint64_t c = threadIdx.x; c < span; c += PB) {
     bool f;
-    if (DENSE) f = flags ? flag[c] != 0 : touched(acc[c]);
+    if (DENSE) f = flags ? flag[c] != 0 : (F32 ? touched32(acc_s[c]) : touched(acc[c]));
     else if (S) f = hkeys[c] != ColEmpty<ColT>::v;
     else f = __hip_atomic_load(&gkeys[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != ColEmpty<ColT>::v;
     cnt += f ? 1u : 0u;
@@ -713,30 +872,57 @@ __global__ void __launch_bounds__(PB) k_pass(PassParams P) {
     bool f = false;
     ColT col = 0;
     double v = 0.0;
+    float fs = 0.f, fc = 0.f;
     if (c < span) {
       if (DENSE) {
-        v = acc[c];
-        f = flags ? flag[c] != 0 : touched(v);
+        if (F32) {
+          fs = acc_s[c];
+          fc = acc_c[c];
+          f = flags ? flag[c] != 0 : touched32(fs);
+        } else {
+          v = acc[c];
+          f = flags ? flag[c] != 0 : touched(v);
+        }
         col = (ColT)c;
       } else if (S) {
         col = hkeys[c];
         f = col != ColEmpty<ColT>::v;
-        v = hvals[c];
+        if (F32) {
+          fs = hs32[c];
+          fc = hc32[c];
+        } else {
+          v = hvals[c];
+        }
       } else {
         col = __hip_atomic_load(&gkeys[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         f = col != ColEmpty<ColT>::v;
-        if (f) v = __hip_atomic_load(&gvals[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (f) {
+          if (F32) {
+            float* g2 = (float*)gvals;
+            fs = __hip_atomic_load(&g2[2 * c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            fc = __hip_atomic_load(&g2[2 * c + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          } else {
+            v = __hip_atomic_load(&gvals[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          }
+        }
       }
+    }
+    float hi = 0.f, lo = 0.f;
+    if (F32) {
+      f32_finish<COMP>(fs, fc, &hi, &lo);
+    } else if (F32IN) {
+      hi = (float)v;  // the FP64 sum as an FP32 pair
+      lo = (float)(v - (double)hi);
     }
     uint32_t tile_total;
     uint32_t rank = block_rank(f, s_w, &tile_total);
-    if (f && run + rank < limit) emit_entry<ColT>(P, base + run + rank, s_rowc, col, v);
+    if (f && run + rank < limit) emit_entry<ColT, F32IN>(P, base + run + rank, s_rowc, col, v, hi, lo);
     run += tile_total;
   }
 }
 
 // learned / upper-bound: staged rows -> final output
-template <class ColT>
+template <class ColT, bool F32>
 __global__ void __launch_bounds__(PB) k_emit(EmitParams P) {
   const uint32_t r = blockIdx.x;
   if (r >= P.nrows) return;
@@ -765,7 +951,13 @@ __global__ void __launch_bounds__(PB) k_emit(EmitParams P) {
       for (uint32_t j = 0; j < PTC_MAX_MODES; ++j)
         if (j < P.colc.n) o[j] = (uint32_t)(((unsigned long long)col / P.colc.stride[j]) % P.colc.radix[j]);
     }
-    P.out_vals[dst + i] = P.stg_val[src + i];
+    if (F32) {
+      const float2 x = ((const float2*)P.stg_val)[src + i];
+      P.out_vals32[dst + i] = x.x;
+      P.out_err32[dst + i] = x.y;
+    } else {
+      P.out_vals[dst + i] = P.stg_val[src + i];
+    }
   }
 }
 
@@ -818,9 +1010,9 @@ size_t pass_lds_bytes(int pass, bool dense, bool col64, uint64_t ncols, uint32_t
   return (b + 15) & ~size_t(15);
 }
 
-template <int PASS, bool DENSE, class ColT, bool MERGE>
+template <int PASS, bool DENSE, class ColT, bool MERGE, int VAL = 0>
 static int launch_pass_t(ptc_ctx* ctx, const PassParams& P, uint64_t nitems, size_t lds) {
-  auto kern = k_pass<PASS, DENSE, ColT, MERGE>;
+  auto kern = k_pass<PASS, DENSE, ColT, MERGE, VAL>;
   PTC_HIP(ctx, hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   const bool timed = ctx->nkev + 2 <= 16;
   if (timed) hipEventRecord(ctx->kev[ctx->nkev], ctx->stream);
@@ -834,6 +1026,14 @@ static int launch_pass_t(ptc_ctx* ctx, const PassParams& P, uint64_t nitems, siz
   return PTC_OK;
 }
 
+template <bool MERGE, int VAL>
+static int launch_numeric_v(ptc_ctx* ctx, bool dense, bool col64, const PassParams& P, uint64_t nitems, size_t lds) {
+  if (dense) return col64 ? launch_pass_t<PASS_NUMERIC, true, unsigned long long, MERGE, VAL>(ctx, P, nitems, lds)
+                          : launch_pass_t<PASS_NUMERIC, true, uint32_t, MERGE, VAL>(ctx, P, nitems, lds);
+  return col64 ? launch_pass_t<PASS_NUMERIC, false, unsigned long long, MERGE, VAL>(ctx, P, nitems, lds)
+               : launch_pass_t<PASS_NUMERIC, false, uint32_t, MERGE, VAL>(ctx, P, nitems, lds);
+}
+
 template <bool MERGE>
 static int launch_pass_m(ptc_ctx* ctx, int pass, bool dense, bool col64, const PassParams& P, uint64_t nitems,
                          size_t lds) {
@@ -842,6 +1042,12 @@ static int launch_pass_m(ptc_ctx* ctx, int pass, bool dense, bool col64, const P
                             : launch_pass_t<PASS_SYMBOLIC, true, uint32_t, MERGE>(ctx, P, nitems, lds);
     return col64 ? launch_pass_t<PASS_SYMBOLIC, false, unsigned long long, MERGE>(ctx, P, nitems, lds)
                  : launch_pass_t<PASS_SYMBOLIC, false, uint32_t, MERGE>(ctx, P, nitems, lds);
+  }
+  switch (P.val) {
+    case 1: return launch_numeric_v<MERGE, 1>(ctx, dense, col64, P, nitems, lds);
+    case 2: return launch_numeric_v<MERGE, 2>(ctx, dense, col64, P, nitems, lds);
+    case 3: return launch_numeric_v<MERGE, 3>(ctx, dense, col64, P, nitems, lds);
+    default: break;
   }
   if (dense) return col64 ? launch_pass_t<PASS_NUMERIC, true, unsigned long long, MERGE>(ctx, P, nitems, lds)
                           : launch_pass_t<PASS_NUMERIC, true, uint32_t, MERGE>(ctx, P, nitems, lds);
@@ -859,10 +1065,16 @@ int launch_pass(ptc_ctx* ctx, int pass, bool dense, bool col64, const PassParams
 
 int launch_emit(ptc_ctx* ctx, const EmitParams& P, bool col64) {
   if (!P.nrows) return PTC_OK;
-  if (col64)
-    hipLaunchKernelGGL((k_emit<unsigned long long>), dim3((unsigned)P.nrows), dim3(PB), 0, ctx->stream, P);
-  else
-    hipLaunchKernelGGL((k_emit<uint32_t>), dim3((unsigned)P.nrows), dim3(PB), 0, ctx->stream, P);
+  if (P.f32) {
+    if (col64)
+      hipLaunchKernelGGL((k_emit<unsigned long long, true>), dim3((unsigned)P.nrows), dim3(PB), 0, ctx->stream, P);
+    else
+      hipLaunchKernelGGL((k_emit<uint32_t, true>), dim3((unsigned)P.nrows), dim3(PB), 0, ctx->stream, P);
+  } else if (col64) {
+    hipLaunchKernelGGL((k_emit<unsigned long long, false>), dim3((unsigned)P.nrows), dim3(PB), 0, ctx->stream, P);
+  } else {
+    hipLaunchKernelGGL((k_emit<uint32_t, false>), dim3((unsigned)P.nrows), dim3(PB), 0, ctx->stream, P);
+  }
   PTC_HIP(ctx, hipGetLastError());
   return PTC_OK;
 }

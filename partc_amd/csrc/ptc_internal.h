@@ -95,8 +95,8 @@ struct ptc_dtensor {
   uint32_t mode_max[PTC_MAX_MODES] = {};  // max index per mode (0 if N == 0)
   uint32_t* hedges = nullptr;             // device, N*d
   double* vals = nullptr;                 // device, N (FP64 tensors)
-  float* vals32 = nullptr;                // device, N (FP32 results)
-  float* err32 = nullptr;                 // device, N: compensation term of FP32 results
+  float* vals32 = nullptr;                // device, N: FP32 values (vals is then NULL)
+  float* err32 = nullptr;                 // device, N: FP32 results only, lo = sum - vals32
   bool owns = true;
   // every value finite with magnitude >= 2^-500, so no product of two values
   // is +-0 (lets dense accumulators detect touched columns without flags)
@@ -121,7 +121,7 @@ struct ptc_hashtable {
   uint32_t* rest_key32 = nullptr;        // when rest.space <= 2^32
   unsigned long long* rest_key64 = nullptr;
   double* gval = nullptr;      // device, N: values in grouped order
-  float* gval32 = nullptr;     // device, N: FP32 copy (made on demand)
+  float* gval32 = nullptr;     // device, N: values in grouped order, FP32 tensors (gval is then NULL)
   double ms_build = 0.0;
 };
 

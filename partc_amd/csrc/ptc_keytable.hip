@@ -423,10 +423,10 @@ __global__ void k_kt_scatter(uint64_t N, const unsigned long long* __restrict__ 
 
 // Pass 4b: the streaming layout in grouped order, written coalesced: the
 // linearised non-key tuple and the value of every grouped position.
-template <class RestT>
-__global__ void k_kt_gather(const uint32_t* __restrict__ h, const double* __restrict__ val, uint32_t d,
+template <class RestT, class ValT>
+__global__ void k_kt_gather(const uint32_t* __restrict__ h, const ValT* __restrict__ val, uint32_t d,
                             uint64_t N, KeyCodec rest, const uint32_t* __restrict__ perm,
-                            RestT* __restrict__ rest_key, double* __restrict__ gval) {
+                            RestT* __restrict__ rest_key, ValT* __restrict__ gval) {
   for (uint64_t pos = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; pos < N; pos += (uint64_t)gridDim.x * blockDim.x) {
     const uint64_t e = perm[pos];
     unsigned long long rk = 0;
@@ -506,7 +506,11 @@ extern "C" int ptc_hashtable_build(ptc_ctx* ctx, const ptc_dtensor* t, ptc_hasht
   ht->entries = dalloc_n<KtEntry>(ctx, cap);
   ht->perm = dalloc_n<uint32_t>(ctx, N);
   ht->slice_of = dalloc_n<uint32_t>(ctx, N);
-  ht->gval = dalloc_n<double>(ctx, N);
+  const bool f32 = t->vals == nullptr && t->vals32 != nullptr;  // an FP32 tensor
+  if (f32)
+    ht->gval32 = dalloc_n<float>(ctx, N);
+  else
+    ht->gval = dalloc_n<double>(ctx, N);
   const bool rest64 = ht->rest.space > 0xFFFFFFFFull;
   if (rest64)
     ht->rest_key64 = dalloc_n<unsigned long long>(ctx, N);
@@ -517,7 +521,7 @@ extern "C" int ptc_hashtable_build(ptc_ctx* ctx, const ptc_dtensor* t, ptc_hasht
   unsigned long long* packed = ws.get<unsigned long long>(cap + 1);
   uint32_t* slot_of = ws.get<uint32_t>(N);
   uint32_t* rank_of = ws.get<uint32_t>(N);
-  if (!ht->entries || !ht->perm || !ht->slice_of || !ht->gval || !(ht->rest_key32 || ht->rest_key64) ||
+  if (!ht->entries || !ht->perm || !ht->slice_of || !(ht->gval || ht->gval32) || !(ht->rest_key32 || ht->rest_key64) ||
       !cnt || !packed || !slot_of || !rank_of) {
     ptc_hashtable_free(ht);
     return PTC_ERR_NOMEM;
@@ -590,12 +594,22 @@ extern "C" int ptc_hashtable_build(ptc_ctx* ctx, const ptc_dtensor* t, ptc_hasht
     else
       hipLaunchKernelGGL(k_kt_scatter, dim3(grid_for(N, 256)), dim3(256), 0, st, N, (const unsigned long long*)packed,
                          (const uint32_t*)slot_of, (const uint32_t*)rank_of, ht->perm, ht->slice_of);
-    if (rest64)
-      hipLaunchKernelGGL((k_kt_gather<unsigned long long>), dim3(grid_for(N, 256)), dim3(256), 0, st, t->hedges,
-                         t->vals, t->d, N, ht->rest, (const uint32_t*)ht->perm, ht->rest_key64, ht->gval);
-    else
-      hipLaunchKernelGGL((k_kt_gather<uint32_t>), dim3(grid_for(N, 256)), dim3(256), 0, st, t->hedges, t->vals,
-                         t->d, N, ht->rest, (const uint32_t*)ht->perm, ht->rest_key32, ht->gval);
+    const unsigned g = grid_for(N, 256);
+    const uint32_t* pm = (const uint32_t*)ht->perm;
+    if (f32) {
+      if (rest64)
+        hipLaunchKernelGGL((k_kt_gather<unsigned long long, float>), dim3(g), dim3(256), 0, st, t->hedges,
+                           (const float*)t->vals32, t->d, N, ht->rest, pm, ht->rest_key64, ht->gval32);
+      else
+        hipLaunchKernelGGL((k_kt_gather<uint32_t, float>), dim3(g), dim3(256), 0, st, t->hedges,
+                           (const float*)t->vals32, t->d, N, ht->rest, pm, ht->rest_key32, ht->gval32);
+    } else if (rest64) {
+      hipLaunchKernelGGL((k_kt_gather<unsigned long long, double>), dim3(g), dim3(256), 0, st, t->hedges,
+                         (const double*)t->vals, t->d, N, ht->rest, pm, ht->rest_key64, ht->gval);
+    } else {
+      hipLaunchKernelGGL((k_kt_gather<uint32_t, double>), dim3(g), dim3(256), 0, st, t->hedges,
+                         (const double*)t->vals, t->d, N, ht->rest, pm, ht->rest_key32, ht->gval);
+    }
   }
   hipError_t e = hipStreamSynchronize(st);
   if (e != hipSuccess) {

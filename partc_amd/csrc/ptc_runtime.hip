@@ -345,13 +345,17 @@ static int check_perm(ptc_ctx* ctx, uint32_t d, uint32_t nk, const uint32_t* dim
   return PTC_OK;
 }
 
-extern "C" int ptc_dtensor_upload(ptc_ctx* ctx, const ptc_tensor_view* v, ptc_dtensor** out) {
+// Upload of an FP64 (v64) or FP32 (v32) tensor.  vals_safe: every |value| is
+// far enough from 0 and from overflow that no product is 0, subnormal or
+// infinite (then untouched dense accumulator entries can be recognised by
+// their -0.0 initial value).
+static int upload(ptc_ctx* ctx, const ptc_tensor_view* v, const double* v64, const float* v32, ptc_dtensor** out) {
   if (!ctx || !v || !out) return PTC_ERR_ARG;
   *out = nullptr;
   int rc = check_perm(ctx, v->d, v->num_key_dimensions, v->dim_tc);
   if (rc) return rc;
   if (v->N >= 0xFFFFFFFFull) return set_err(ctx, PTC_ERR_RANGE, "N=%llu exceeds 2^32-2", (unsigned long long)v->N);
-  if (v->N && (!v->hedges || !v->vals)) return set_err(ctx, PTC_ERR_ARG, "NULL data arrays");
+  if (v->N && (!v->hedges || !(v64 || v32))) return set_err(ctx, PTC_ERR_ARG, "NULL data arrays");
   PTC_HIP(ctx, hipSetDevice(ctx->device));
   ptc_dtensor* t = new ptc_dtensor();
   t->ctx = ctx;
@@ -366,8 +370,9 @@ extern "C" int ptc_dtensor_upload(ptc_ctx* ctx, const ptc_tensor_view* v, ptc_dt
     for (uint32_t j = 0; j < v->d; ++j) t->mode_max[j] = std::max(t->mode_max[j], v->hedges[i * v->d + j]);
   t->vals_safe = true;
   for (uint64_t i = 0; i < v->N; ++i) {
-    const double a = std::fabs(v->vals[i]);
-    if (!(a >= 0x1p-500 && a <= 1.7976931348623157e308)) {
+    const double a = v64 ? std::fabs(v64[i]) : std::fabs((double)v32[i]);
+    const bool ok = v64 ? (a >= 0x1p-500 && a <= 0x1p500) : (a >= 0x1p-60 && a <= 0x1p60);
+    if (!ok) {
       t->vals_safe = false;
       break;
     }
@@ -375,14 +380,18 @@ extern "C" int ptc_dtensor_upload(ptc_ctx* ctx, const ptc_tensor_view* v, ptc_dt
   if (!v->dims)
     for (uint32_t j = 0; j < v->d; ++j) t->dims[j] = v->N ? t->mode_max[j] + 1 : 0;
   t->hedges = dalloc_n<uint32_t>(ctx, v->N * v->d);
-  t->vals = dalloc_n<double>(ctx, v->N);
-  if (!t->hedges || !t->vals) {
+  if (v64)
+    t->vals = dalloc_n<double>(ctx, v->N);
+  else
+    t->vals32 = dalloc_n<float>(ctx, v->N);
+  if (!t->hedges || !(t->vals || t->vals32)) {
     ptc_dtensor_free(t);
     return PTC_ERR_NOMEM;
   }
   if (v->N) {
     hipError_t e1 = hipMemcpyAsync(t->hedges, v->hedges, sizeof(uint32_t) * v->N * v->d, hipMemcpyHostToDevice, ctx->stream);
-    hipError_t e2 = hipMemcpyAsync(t->vals, v->vals, sizeof(double) * v->N, hipMemcpyHostToDevice, ctx->stream);
+    hipError_t e2 = v64 ? hipMemcpyAsync(t->vals, v64, sizeof(double) * v->N, hipMemcpyHostToDevice, ctx->stream)
+                        : hipMemcpyAsync(t->vals32, v32, sizeof(float) * v->N, hipMemcpyHostToDevice, ctx->stream);
     hipError_t e3 = hipStreamSynchronize(ctx->stream);
     if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
       ptc_dtensor_free(t);
@@ -391,6 +400,15 @@ extern "C" int ptc_dtensor_upload(ptc_ctx* ctx, const ptc_tensor_view* v, ptc_dt
   }
   *out = t;
   return PTC_OK;
+}
+
+extern "C" int ptc_dtensor_upload(ptc_ctx* ctx, const ptc_tensor_view* v, ptc_dtensor** out) {
+  return upload(ctx, v, v ? v->vals : nullptr, nullptr, out);
+}
+
+extern "C" int ptc_dtensor_upload_f32(ptc_ctx* ctx, const ptc_tensor_view* v, const float* vals, ptc_dtensor** out) {
+  if (v && v->N && !vals) return ctx ? set_err(ctx, PTC_ERR_ARG, "NULL FP32 values") : PTC_ERR_ARG;
+  return upload(ctx, v, nullptr, vals, out);
 }
 
 extern "C" int ptc_dtensor_wrap(ptc_ctx* ctx, uint32_t d, uint32_t nk, uint64_t N,
@@ -478,9 +496,8 @@ extern "C" int ptc_dtensor_download(ptc_ctx* ctx, const ptc_dtensor* t, uint32_t
     if (t->vals) {
       PTC_HIP(ctx, hipMemcpyAsync(vals, t->vals, sizeof(double) * t->N, hipMemcpyDeviceToHost, ctx->stream));
     } else if (t->vals32) {
-      // An FP32 result: the value is the FP32 sum; report it widened (the
-      // compensation term is available through ptc_dtensor_download_f32's
-      // companion view, which keeps it separate).
+      // FP32 values, widened exactly (the lo part of an FP32 result comes
+      // from ptc_dtensor_download_f32)
       Workspace ws(ctx);
       double* tmp = ws.get<double>(t->N);
       if (!tmp) return PTC_ERR_NOMEM;
@@ -496,11 +513,14 @@ extern "C" int ptc_dtensor_download(ptc_ctx* ctx, const ptc_dtensor* t, uint32_t
   return PTC_OK;
 }
 
-extern "C" int ptc_dtensor_download_f32(ptc_ctx* ctx, const ptc_dtensor* t, float* vals) {
+extern "C" int ptc_dtensor_download_f32(ptc_ctx* ctx, const ptc_dtensor* t, float* vals, float* lo) {
   if (!ctx || !t || !vals) return PTC_ERR_ARG;
   if (!t->vals32) return set_err(ctx, PTC_ERR_ARG, "tensor has no FP32 values");
+  if (lo && !t->err32) return set_err(ctx, PTC_ERR_ARG, "tensor is not an FP32 contraction result (no lo part)");
   if (t->N)
     PTC_HIP(ctx, hipMemcpyAsync(vals, t->vals32, sizeof(float) * t->N, hipMemcpyDeviceToHost, ctx->stream));
+  if (t->N && lo)
+    PTC_HIP(ctx, hipMemcpyAsync(lo, t->err32, sizeof(float) * t->N, hipMemcpyDeviceToHost, ctx->stream));
   PTC_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return PTC_OK;
 }

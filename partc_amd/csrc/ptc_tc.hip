@@ -80,6 +80,8 @@ struct Run {
   uint32_t* overflow = nullptr;
   unsigned long long* out_off = nullptr;
   bool want_features = false;  // per-row hit counts and longest slices (learned features)
+  bool f32 = false;            // FP32 values (both tables carry gval32)
+  int val = PTC_VAL_F64;       // ptc_value_type of the numeric pass
   explicit Run(ptc_ctx* c) : ctx(c), ws(c) {}
 };
 
@@ -163,8 +165,12 @@ static void base_params(Run& R, PassParams& P, const HashPlan* hp) {
   P.pos_len = R.pos_len;
   P.pos_off = R.pos_off;
   P.aval = R.HA->gval;
+  P.aval32 = R.HA->gval32;
   P.bcol = R.col64 ? (const void*)R.HB->rest_key64 : (const void*)R.HB->rest_key32;
   P.bval = R.HB->gval;
+  P.bval32 = R.HB->gval32;
+  P.f32 = R.f32 ? 1 : 0;
+  P.val = R.val;
   P.nb = R.HB->t->N;
   P.ncols = R.ncols;
   P.hbits = R.hbits;
@@ -265,8 +271,13 @@ static int make_output(Run& R, uint64_t nnz, const ptc_dtensor* A, const ptc_dte
   for (uint32_t j = 0; j < A->nk; ++j) o->mode_max[j] = A->mode_max[A->dim_tc[j]];
   for (uint32_t j = 0; j < B->d - B->nk; ++j) o->mode_max[A->nk + j] = B->mode_max[B->dim_tc[B->nk + j]];
   o->hedges = dalloc_n<uint32_t>(R.ctx, nnz * R.dO);
-  o->vals = dalloc_n<double>(R.ctx, nnz);
-  if (!o->hedges || !o->vals) {
+  if (R.f32) {
+    o->vals32 = dalloc_n<float>(R.ctx, nnz);
+    o->err32 = dalloc_n<float>(R.ctx, nnz);
+  } else {
+    o->vals = dalloc_n<double>(R.ctx, nnz);
+  }
+  if (!o->hedges || !(o->vals || (o->vals32 && o->err32))) {
     ptc_dtensor_free(o);
     return PTC_ERR_NOMEM;
   }
@@ -296,7 +307,19 @@ static int prepare(Run& R, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B,
   const uint32_t c = A->d - A->nk;
   if (c == 0 || B->nk != c) return set_err(ctx, PTC_ERR_ARG, "A has %u contracted modes, B %u key modes", c, B->nk);
   if (HT_A->rest.n != c || HT_B->codec.n != c) return set_err(ctx, PTC_ERR_INTERNAL, "codec shape mismatch");
-  if (opt.value_type != PTC_VAL_F64) return set_err(ctx, PTC_ERR_UNSUPPORTED, "value type %d not available here", opt.value_type);
+  if (opt.deterministic) return set_err(ctx, PTC_ERR_UNSUPPORTED, "the deterministic (reference-order) mode is not available");
+  {
+    const bool a32 = HT_A->gval32 != nullptr, b32 = HT_B->gval32 != nullptr;
+    if (a32 != b32) return set_err(ctx, PTC_ERR_ARG, "A and B must both hold FP64 or both FP32 values");
+    if (opt.value_type < PTC_VAL_F64 || opt.value_type > PTC_VAL_F32_F64ACC)
+      return set_err(ctx, PTC_ERR_ARG, "unknown value type %d", opt.value_type);
+    if (opt.value_type == PTC_VAL_F64 && a32)
+      return set_err(ctx, PTC_ERR_ARG, "FP32 tensors need an FP32 value type (PTC_VAL_F32_*)");
+    if (opt.value_type != PTC_VAL_F64 && !a32)
+      return set_err(ctx, PTC_ERR_ARG, "FP32 value types need FP32 tensors (ptc_dtensor_upload_f32)");
+    R.f32 = a32;
+    R.val = opt.value_type;
+  }
   if (opt.method == PTC_METHOD_LEARNED && !opt.mlp && !ptc_learned_default())
     return set_err(ctx, PTC_ERR_ARG, "learned method needs a trained row-size model");
   PTC_HIP(ctx, hipSetDevice(ctx->device));
@@ -491,6 +514,8 @@ extern "C" int ptc_tc_fksCuckoo(ptc_ctx* ctx, const ptc_hashtable* HT_A, const p
     P.out_base = out_off;
     P.out_hedges = out->hedges;
     P.out_vals = out->vals;
+    P.out_vals32 = out->vals32;
+    P.out_err32 = out->err32;
     if ((rc = run_pass(R, PASS_NUMERIC, &hn, R.items, R.nitems, P))) {
       ptc_dtensor_free(out);
       return rc;
@@ -590,6 +615,9 @@ extern "C" int ptc_tc_fksCuckoo(ptc_ctx* ctx, const ptc_hashtable* HT_A, const p
       E.stg_val = stg_val;
       E.out_hedges = out->hedges;
       E.out_vals = out->vals;
+      E.out_vals32 = out->vals32;
+      E.out_err32 = out->err32;
+      E.f32 = R.f32 ? 1 : 0;
       E.dO = R.dO;
       E.nrowmodes = R.nrowmodes;
       E.rowc = HT_A->codec;
@@ -611,6 +639,8 @@ extern "C" int ptc_tc_fksCuckoo(ptc_ctx* ctx, const ptc_hashtable* HT_A, const p
       Q.out_base = out_off;
       Q.out_hedges = out->hedges;
       Q.out_vals = out->vals;
+      Q.out_vals32 = out->vals32;
+      Q.out_err32 = out->err32;
       if ((rc = run_pass(R, PASS_NUMERIC, &hn, fitems, nfitems, Q, frows, fallback_rows))) {
         ptc_dtensor_free(out);
         return rc;
