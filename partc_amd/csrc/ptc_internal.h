@@ -114,6 +114,7 @@ struct ptc_hashtable {
   KtSlice* slices = nullptr;   // device, num_keys (slot order)
   uint32_t* perm = nullptr;    // device, N: original ids grouped by key
   uint32_t* slice_of = nullptr;// device, N: compact slice index of each grouped position
+                               // (hashed tables: built on first use, hashtable_slice_of)
   // Streaming layout of the non-key modes, in grouped order: the linearised
   // non-key tuple (its codec below) and the value.  Built for every table so
   // the same table serves as B (columns) or as A (rows).
@@ -131,6 +132,7 @@ struct ptc_hashtable {
 namespace ptc {
 
 int set_err(ptc_ctx* ctx, int code, const char* fmt, ...);
+int hashtable_slice_of(ptc_ctx* ctx, const ptc_hashtable* h, const uint32_t** out);
 int hip_check(ptc_ctx* ctx, hipError_t e, const char* what);
 #define PTC_HIP(ctx, call)                                 \
   do {                                                     \

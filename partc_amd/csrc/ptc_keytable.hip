@@ -6,19 +6,20 @@
 // then renumbers the nonzeros so every key's list ("slice") is contiguous.
 //
 // Here the same result is built in HBM in four passes:
-//   1. insert: each key is linearised to 64 bits and inserted into an
-//      open-addressing table of 16-byte entries by a cooperative group of
-//      KT_GROUP lanes that reads one 128-byte line of 8 entries at a time,
-//      ballots for a match or a free entry and claims it with atomicCAS;
-//      a second kernel ranks every nonzero within its slot, aggregating a
-//      block's nonzeros per slot in LDS so that a hot key costs one global
-//      atomicAdd per block instead of one per nonzero;
+//   1. insert + rank: each key is linearised to 64 bits; every lane inserts
+//      its own key with atomicCAS into an open-addressing table of 16-byte
+//      entries (8 to a 128-byte line, probing from line mix64(key)); the
+//      wavefront then sorts its 64 slots (bitonic network over lanes) so a
+//      run of equal keys takes its ranks with one atomicAdd on the entry's
+//      length.  Small key spaces are addressed directly (slot = key) and
+//      ranked by a stable counting sort with per-block histograms, with no
+//      global atomics at all however hot a key is;
 //   2. pack + scan: slot counts and occupancy are packed into one 64-bit
 //      word ((occupied << 32) | count) and prefix-summed in one scan, which
 //      yields each slice's offset and its compact slice index together;
 //   3. finalize: entries receive (offset, length); a compact slice list is
 //      written;
-//   4. scatter: every nonzero goes to offset + rank, with its non-key tuple
+//   4. place: every nonzero goes to offset + rank, with its non-key tuple
 //      (linearised) and its value, in the streaming layout the contraction
 //      reads.
 #include <hip/hip_runtime.h>
@@ -30,12 +31,6 @@
 
 namespace ptc {
 
-// Lanes per key of the cooperative insertion; 1 selects the fused
-// per-lane insert + wave-aggregated rank kernel.
-#ifndef PTC_KT_GROUP
-#define PTC_KT_GROUP 1
-#endif
-constexpr int KT_GROUP = PTC_KT_GROUP;  // lanes per key (8: 8 x 16-byte entries = one 128-byte line)
 constexpr int KT_THREADS = 256;
 
 __device__ __forceinline__ unsigned long long mix64(unsigned long long k) {
@@ -66,77 +61,27 @@ __device__ __forceinline__ unsigned long long load_key_l2(const KtEntry* e) {
   return __hip_atomic_load(&e->key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Pass 1.  One group of G lanes per nonzero (G = 8: one 128-byte line of 8
-// entries per probe step).  Probe order for key k: the entries of line
-// mix64(k) mod (cap/8), then the following entries, wrapping at cap -- the
-// order every lookup scans, so a key always sits at the first entry that
-// was free on its path.
-template <int G>
-__global__ void __launch_bounds__(KT_THREADS) k_kt_insert(
-    const uint32_t* __restrict__ h, uint32_t d, uint64_t N, KeyCodec codec,
-    KtEntry* __restrict__ tab, uint64_t cap, uint32_t* __restrict__ slot_of) {
-  const int lane = threadIdx.x & 63;
-  const int sub = lane & (G - 1);
-  const int gbase = lane - sub;  // first lane of this group in the wave
-  const unsigned long long gmask = (G == 64 ? ~0ull : ((1ull << G) - 1)) << gbase;
-  const uint64_t groups_per_grid = (uint64_t)gridDim.x * (KT_THREADS / G);
-  const uint64_t nlines = cap / 8;
-  for (uint64_t e = (uint64_t)blockIdx.x * (KT_THREADS / G) + threadIdx.x / G; e < N; e += groups_per_grid) {
-    unsigned long long key = 0;
-    encode_key(h, d, e, codec, &key);  // table's own tensor: always within radix
-    uint64_t base = (mix64(key) & (nlines - 1)) * 8;
-    uint32_t slot = 0;
-    while (true) {
-      const uint64_t idx = (base + sub) & (cap - 1);
-      unsigned long long k = load_key_l2(&tab[idx]);
-      if (G == 1) {
-        if (k == key) {
-          slot = (uint32_t)idx;
-          break;
-        }
-        if (k == PTC_EMPTY_KEY) {
-          unsigned long long old = atomicCAS(&tab[idx].key, PTC_EMPTY_KEY, key);
-          if (old == PTC_EMPTY_KEY || old == key) {
-            slot = (uint32_t)idx;
-            break;
-          }
-          continue;  // lost the race: re-read the same entry
-        }
-        base = (base + 1) & (cap - 1);
-        continue;
-      }
-      unsigned long long hit = __ballot(k == key) & gmask;
-      if (hit) {
-        slot = (uint32_t)((base + (__ffsll((long long)hit) - 1 - gbase)) & (cap - 1));
-        break;
-      }
-      unsigned long long fre = __ballot(k == PTC_EMPTY_KEY) & gmask;
-      if (!fre) {
-        base = (base + G) & (cap - 1);
-        continue;
-      }
-      const int leader = __ffsll((long long)fre) - 1;
-      unsigned long long old = 0;
-      if (lane == leader) old = atomicCAS(&tab[(base + (leader - gbase)) & (cap - 1)].key, PTC_EMPTY_KEY, key);
-      old = __shfl(old, leader, 64);
-      if (old == PTC_EMPTY_KEY || old == key) {
-        slot = (uint32_t)((base + (leader - gbase)) & (cap - 1));
-        break;
-      }
-      // lost the race to a different key: re-read the same entries
-    }
-    if (sub == 0) slot_of[e] = slot;
-  }
-}
+// One nonzero's streaming payload (linearised non-key tuple, value), kept
+// in input order so the grouped layout is gathered with one read per
+// position.
+template <class RestT, class ValT>
+struct KtPay {
+  RestT k;
+  ValT v;
+};
 
 // Pass 1 (fused, the default for hashed tables): every lane inserts its own
 // key with atomicCAS along the probe order above; then the wavefront sorts
 // its 64 slots with a bitonic network over lanes, and each run of equal
-// slots takes its ranks with one global atomicAdd on the slot's counter, so
-// a key repeated across the wave costs one atomic.
+// slots takes its ranks with one atomicAdd on the length field of the
+// slot's own entry (the line the insert just touched), so a key repeated
+// across the wave costs one atomic.  The nonzero's payload is written in
+// input order on the way.
+template <class RestT, class ValT>
 __global__ void __launch_bounds__(KT_THREADS) k_kt_insert_rank(
     const uint32_t* __restrict__ h, uint32_t d, uint64_t N, KeyCodec codec, KtEntry* __restrict__ tab,
-    uint64_t cap, uint32_t* __restrict__ cnt, uint32_t* __restrict__ slot_of, uint32_t* __restrict__ rank_of) {
+    uint64_t cap, uint32_t* __restrict__ slot_of, uint32_t* __restrict__ rank_of, KeyCodec rest,
+    const ValT* __restrict__ val, KtPay<RestT, ValT>* __restrict__ pay) {
   __shared__ uint32_t xr[KT_THREADS];
   const int lane = threadIdx.x & 63, wbase = threadIdx.x - lane;
   const uint64_t nlines = cap / 8;
@@ -159,6 +104,12 @@ __global__ void __launch_bounds__(KT_THREADS) k_kt_insert_rank(
       }
       slot = (uint32_t)i;
       slot_of[e] = slot;
+      unsigned long long rk = 0;
+      for (uint32_t j = 0; j < rest.n; ++j) rk += (unsigned long long)h[e * d + rest.mode[j]] * rest.stride[j];
+      KtPay<RestT, ValT> pl;
+      pl.k = (RestT)rk;
+      pl.v = val[e];
+      pay[e] = pl;
     }
     // rank: runs of equal slots across the wave
     uint32_t key = slot, pl = (uint32_t)lane;
@@ -184,7 +135,7 @@ __global__ void __launch_bounds__(KT_THREADS) k_kt_insert_rank(
     const unsigned long long above = hm & ~upto;
     const int next = above ? (__ffsll((long long)above) - 1) : 64;
     uint32_t rbase = 0;
-    if (head && key != 0xFFFFFFFFu) rbase = atomicAdd(&cnt[key], (uint32_t)(next - start));
+    if (head && key != 0xFFFFFFFFu) rbase = atomicAdd(&tab[key].len, (uint32_t)(next - start));
     rbase = __shfl(rbase, start, 64);
     xr[wbase + pl] = rbase + (uint32_t)(lane - start);
     __syncthreads();
@@ -375,15 +326,17 @@ __global__ void k_kt_clear(KtEntry* __restrict__ tab, uint32_t* __restrict__ cnt
     tab[i].key = PTC_EMPTY_KEY;
     tab[i].off = 0;
     tab[i].len = 0;
-    cnt[i] = 0;
+    if (cnt) cnt[i] = 0;
   }
 }
 
 // Pass 2a: pack (occupied << 32) | count.
-__global__ void k_kt_pack(const uint32_t* __restrict__ cnt, unsigned long long* __restrict__ packed,
-                          uint64_t cap) {
+// Counts come from cnt[] (direct tables) or from the entries' length fields
+// (hashed tables, cnt == NULL).
+__global__ void k_kt_pack(const uint32_t* __restrict__ cnt, const KtEntry* __restrict__ tab,
+                          unsigned long long* __restrict__ packed, uint64_t cap) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < cap; i += (uint64_t)gridDim.x * blockDim.x) {
-    uint32_t c = cnt[i];
+    uint32_t c = cnt ? cnt[i] : tab[i].len;
     packed[i] = ((unsigned long long)(c > 0) << 32) | c;
   }
 }
@@ -395,7 +348,7 @@ __global__ void k_kt_finalize(KtEntry* __restrict__ tab, const uint32_t* __restr
                               unsigned long long* __restrict__ pre, uint64_t cap,
                               KtSlice* __restrict__ slices, int direct) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < cap; i += (uint64_t)gridDim.x * blockDim.x) {
-    uint32_t c = cnt[i];
+    uint32_t c = cnt ? cnt[i] : tab[i].len;
     if (!c) continue;
     unsigned long long p = pre[i];
     uint32_t off = (uint32_t)(p & 0xFFFFFFFFull);
@@ -409,7 +362,43 @@ __global__ void k_kt_finalize(KtEntry* __restrict__ tab, const uint32_t* __restr
   }
 }
 
-// Pass 4a: scatter ids into grouped order (two 4-byte random stores).
+// Pass 4a for hashed tables: ids into grouped order (one 4-byte random
+// store; the slice index of each position is derived later, only for a
+// table that serves as A).
+__global__ void k_kt_scatter_perm(uint64_t N, const unsigned long long* __restrict__ slotinfo,
+                                  const uint32_t* __restrict__ slot_of, const uint32_t* __restrict__ rank_of,
+                                  uint32_t* __restrict__ perm) {
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < N; e += (uint64_t)gridDim.x * blockDim.x)
+    perm[(slotinfo[slot_of[e]] & 0xFFFFFFFFull) + rank_of[e]] = (uint32_t)e;
+}
+
+// Pass 4b for hashed tables: the streaming layout in grouped order from the
+// input-order payload (one random 8- or 16-byte read per position).
+template <class RestT, class ValT>
+__global__ void k_kt_gather_pay(uint64_t N, const uint32_t* __restrict__ perm, const KtPay<RestT, ValT>* __restrict__ pay,
+                                RestT* __restrict__ rest_key, ValT* __restrict__ gval) {
+  for (uint64_t pos = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; pos < N; pos += (uint64_t)gridDim.x * blockDim.x) {
+    const KtPay<RestT, ValT> pl = pay[perm[pos]];
+    rest_key[pos] = pl.k;
+    gval[pos] = pl.v;
+  }
+}
+
+// Slice index of every grouped position: binary search of the slice
+// offsets (ascending with the slice index).
+__global__ void k_kt_slice_of(uint64_t N, const KtSlice* __restrict__ slices, uint64_t nslices,
+                              uint32_t* __restrict__ slice_of) {
+  for (uint64_t pos = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; pos < N; pos += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t lo = 0, hi = nslices - 1;
+    while (lo < hi) {
+      const uint64_t mid = (lo + hi + 1) >> 1;
+      if (slices[mid].off <= pos) lo = mid; else hi = mid - 1;
+    }
+    slice_of[pos] = (uint32_t)lo;
+  }
+}
+
+// Pass 4a (unfused): scatter ids into grouped order (two 4-byte random stores).
 __global__ void k_kt_scatter(uint64_t N, const unsigned long long* __restrict__ slotinfo,
                              const uint32_t* __restrict__ slot_of, const uint32_t* __restrict__ rank_of,
                              uint32_t* __restrict__ perm, uint32_t* __restrict__ slice_of) {
@@ -421,8 +410,38 @@ __global__ void k_kt_scatter(uint64_t N, const unsigned long long* __restrict__ 
   }
 }
 
-// Pass 4b: the streaming layout in grouped order, written coalesced: the
-// linearised non-key tuple and the value of every grouped position.
+// Pass 4 (fused, the default): every nonzero e, read in input order (its
+// coordinates and value are sequential reads), goes straight to its grouped
+// position pos = slice offset + rank: id, slice id, linearised non-key tuple
+// and value are stored there.  The stores are scattered, but the four
+// destination arrays stay resident in the last-level cache while they fill,
+// so no HBM line is read per nonzero -- unlike gathering in grouped order,
+// which reads a random line of the coordinates and one of the values for
+// every position.  COUNTING: direct tables ordered by the (slot, block)
+// prefix of the counting sort.
+template <class RestT, class ValT, bool COUNTING>
+__global__ void k_kt_place(uint64_t N, const uint32_t* __restrict__ h, uint32_t d, KeyCodec rest,
+                           const ValT* __restrict__ val, const unsigned long long* __restrict__ slotinfo,
+                           const unsigned long long* __restrict__ hpre, uint64_t nblocks,
+                           const uint32_t* __restrict__ slot_of, const uint32_t* __restrict__ rank_of,
+                           uint32_t* __restrict__ perm, uint32_t* __restrict__ slice_of,
+                           RestT* __restrict__ rest_key, ValT* __restrict__ gval) {
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < N; e += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t s = slot_of[e];
+    const unsigned long long si = slotinfo[s];
+    const uint64_t pos = (COUNTING ? hpre[(uint64_t)s * nblocks + e / RK_CHUNK] : (si & 0xFFFFFFFFull)) + rank_of[e];
+    unsigned long long rk = 0;
+    for (uint32_t j = 0; j < rest.n; ++j) rk += (unsigned long long)h[e * d + rest.mode[j]] * rest.stride[j];
+    perm[pos] = (uint32_t)e;
+    slice_of[pos] = (uint32_t)(si >> 32);
+    rest_key[pos] = (RestT)rk;
+    gval[pos] = val[e];
+  }
+}
+
+// Pass 4b (unfused): the streaming layout in grouped order, written
+// coalesced: the linearised non-key tuple and the value of every grouped
+// position.
 template <class RestT, class ValT>
 __global__ void k_kt_gather(const uint32_t* __restrict__ h, const ValT* __restrict__ val, uint32_t d,
                             uint64_t N, KeyCodec rest, const uint32_t* __restrict__ perm,
@@ -505,7 +524,6 @@ extern "C" int ptc_hashtable_build(ptc_ctx* ctx, const ptc_dtensor* t, ptc_hasht
   ht->cap = cap;
   ht->entries = dalloc_n<KtEntry>(ctx, cap);
   ht->perm = dalloc_n<uint32_t>(ctx, N);
-  ht->slice_of = dalloc_n<uint32_t>(ctx, N);
   const bool f32 = t->vals == nullptr && t->vals32 != nullptr;  // an FP32 tensor
   if (f32)
     ht->gval32 = dalloc_n<float>(ctx, N);
@@ -516,52 +534,71 @@ extern "C" int ptc_hashtable_build(ptc_ctx* ctx, const ptc_dtensor* t, ptc_hasht
     ht->rest_key64 = dalloc_n<unsigned long long>(ctx, N);
   else
     ht->rest_key32 = dalloc_n<uint32_t>(ctx, N);
+  const uint64_t nblk = (N + RK_CHUNK - 1) / RK_CHUNK;
+  // Three ways to group:
+  //  hashed    insert + wave-aggregated rank (counts in the entries), ids
+  //            scattered, streaming layout gathered from the input-order
+  //            payload; slice indices only when needed (a table used as A);
+  //  counting  direct table with a small key space: stable counting sort,
+  //            every nonzero placed at once (k_kt_place);
+  //  direct    other direct tables: block-aggregated ranks, ids and slice
+  //            indices scattered, streaming layout gathered.
+  const bool hashed = !ht->direct;
+  const bool counting = ht->direct && cap <= RD_MAX && cap * nblk <= (64ull << 20);
+  if (!hashed) ht->slice_of = dalloc_n<uint32_t>(ctx, N);
   Workspace ws(ctx);
-  uint32_t* cnt = ws.get<uint32_t>(cap);
+  uint32_t* cnt = hashed ? nullptr : ws.get<uint32_t>(cap);
   unsigned long long* packed = ws.get<unsigned long long>(cap + 1);
   uint32_t* slot_of = ws.get<uint32_t>(N);
   uint32_t* rank_of = ws.get<uint32_t>(N);
-  if (!ht->entries || !ht->perm || !ht->slice_of || !(ht->gval || ht->gval32) || !(ht->rest_key32 || ht->rest_key64) ||
-      !cnt || !packed || !slot_of || !rank_of) {
+  const size_t pay_bytes = rest64 || !f32 ? 16 : 8;
+  void* pay = hashed ? (void*)ws.get<unsigned char>(N * pay_bytes) : nullptr;
+  if (!ht->entries || !ht->perm || (!hashed && !ht->slice_of) || !(ht->gval || ht->gval32) ||
+      !(ht->rest_key32 || ht->rest_key64) || (!hashed && !cnt) || !packed || !slot_of || !rank_of || (hashed && N && !pay)) {
     ptc_hashtable_free(ht);
     return PTC_ERR_NOMEM;
   }
   hipStream_t st = ctx->stream;
   hipLaunchKernelGGL(k_kt_clear, dim3(grid_for(cap, 256)), dim3(256), 0, st, ht->entries, cnt, cap);
-  const uint64_t nblk = (N + RK_CHUNK - 1) / RK_CHUNK;
-  const bool counting = ht->direct && cap <= RD_MAX && cap * nblk <= (64ull << 20);
   unsigned long long* hpre = nullptr;
-  if (N && ht->direct) {
+  if (N && hashed) {
+    const unsigned g = grid_for(N, KT_THREADS, 16384);
+#define PTC_INSERT(RT, VT, VALS)                                                                                  \
+  hipLaunchKernelGGL((k_kt_insert_rank<RT, VT>), dim3(g), dim3(KT_THREADS), 0, st, t->hedges, t->d, N, ht->codec, \
+                     ht->entries, cap, slot_of, rank_of, ht->rest, (const VT*)(VALS), (KtPay<RT, VT>*)pay)
+    if (rest64) {
+      if (f32) PTC_INSERT(unsigned long long, float, t->vals32);
+      else PTC_INSERT(unsigned long long, double, t->vals);
+    } else {
+      if (f32) PTC_INSERT(uint32_t, float, t->vals32);
+      else PTC_INSERT(uint32_t, double, t->vals);
+    }
+#undef PTC_INSERT
+  } else if (N) {
     hipLaunchKernelGGL(k_kt_direct, dim3(grid_for(N, 256)), dim3(256), 0, st, t->hedges, t->d, N, ht->codec,
                        slot_of);
-  } else if (N && KT_GROUP == 1) {
-    hipLaunchKernelGGL(k_kt_insert_rank, dim3(grid_for(N, KT_THREADS, 16384)), dim3(KT_THREADS), 0, st, t->hedges,
-                       t->d, N, ht->codec, ht->entries, cap, cnt, slot_of, rank_of);
-  } else if (N) {
-    unsigned g = grid_for(N, KT_THREADS / KT_GROUP, 16384);
-    hipLaunchKernelGGL(k_kt_insert<KT_GROUP>, dim3(g), dim3(KT_THREADS), 0, st, t->hedges, t->d, N, ht->codec,
-                       ht->entries, cap, slot_of);
-  }
-  if (N && counting) {
-    uint32_t* hist = ws.get<uint32_t>(cap * nblk);
-    hpre = ws.get<unsigned long long>(cap * nblk + 1);
-    if (!hist || !hpre) {
-      ptc_hashtable_free(ht);
-      return PTC_ERR_NOMEM;
+    if (counting) {
+      uint32_t* hist = ws.get<uint32_t>(cap * nblk);
+      hpre = ws.get<unsigned long long>(cap * nblk + 1);
+      if (!hist || !hpre) {
+        ptc_hashtable_free(ht);
+        return PTC_ERR_NOMEM;
+      }
+      hipLaunchKernelGGL(k_kt_rank_direct, dim3((unsigned)nblk), dim3(RK_THREADS), 0, st, (const uint32_t*)slot_of, N,
+                         (uint32_t)cap, nblk, hist, rank_of);
+      if ((rc = scan_u32_to_u64(ctx, hist, hpre, cap * nblk, hpre + cap * nblk))) {
+        ptc_hashtable_free(ht);
+        return rc;
+      }
+      hipLaunchKernelGGL(k_kt_direct_cnt, dim3(grid_for(cap, 256)), dim3(256), 0, st, (const unsigned long long*)hpre,
+                         (uint32_t)cap, nblk, cnt);
+    } else {
+      hipLaunchKernelGGL(k_kt_rank, dim3((unsigned)nblk), dim3(RK_THREADS), 0, st, (const uint32_t*)slot_of, N, cnt,
+                         rank_of);
     }
-    hipLaunchKernelGGL(k_kt_rank_direct, dim3((unsigned)nblk), dim3(RK_THREADS), 0, st, (const uint32_t*)slot_of, N,
-                       (uint32_t)cap, nblk, hist, rank_of);
-    if ((rc = scan_u32_to_u64(ctx, hist, hpre, cap * nblk, hpre + cap * nblk))) {
-      ptc_hashtable_free(ht);
-      return rc;
-    }
-    hipLaunchKernelGGL(k_kt_direct_cnt, dim3(grid_for(cap, 256)), dim3(256), 0, st, (const unsigned long long*)hpre,
-                       (uint32_t)cap, nblk, cnt);
-  } else if (N && (ht->direct || KT_GROUP > 1)) {
-    hipLaunchKernelGGL(k_kt_rank, dim3((unsigned)nblk), dim3(RK_THREADS), 0, st, (const uint32_t*)slot_of, N, cnt,
-                       rank_of);
   }
-  hipLaunchKernelGGL(k_kt_pack, dim3(grid_for(cap, 256)), dim3(256), 0, st, cnt, packed, cap);
+  hipLaunchKernelGGL(k_kt_pack, dim3(grid_for(cap, 256)), dim3(256), 0, st, (const uint32_t*)cnt,
+                     (const KtEntry*)ht->entries, packed, cap);
   rc = scan_u64(ctx, packed, packed, cap, packed + cap);
   if (rc) {
     ptc_hashtable_free(ht);
@@ -584,32 +621,50 @@ extern "C" int ptc_hashtable_build(ptc_ctx* ctx, const ptc_dtensor* t, ptc_hasht
     ptc_hashtable_free(ht);
     return PTC_ERR_NOMEM;
   }
-  hipLaunchKernelGGL(k_kt_finalize, dim3(grid_for(cap, 256)), dim3(256), 0, st, ht->entries, cnt, packed,
-                     cap, ht->slices, ht->direct ? 1 : 0);
-  if (N) {
-    if (counting)
-      hipLaunchKernelGGL(k_kt_scatter_direct, dim3(grid_for(N, 256)), dim3(256), 0, st, N,
-                         (const unsigned long long*)packed, (const unsigned long long*)hpre, nblk,
-                         (const uint32_t*)slot_of, (const uint32_t*)rank_of, ht->perm, ht->slice_of);
-    else
-      hipLaunchKernelGGL(k_kt_scatter, dim3(grid_for(N, 256)), dim3(256), 0, st, N, (const unsigned long long*)packed,
-                         (const uint32_t*)slot_of, (const uint32_t*)rank_of, ht->perm, ht->slice_of);
-    const unsigned g = grid_for(N, 256);
-    const uint32_t* pm = (const uint32_t*)ht->perm;
-    if (f32) {
-      if (rest64)
-        hipLaunchKernelGGL((k_kt_gather<unsigned long long, float>), dim3(g), dim3(256), 0, st, t->hedges,
-                           (const float*)t->vals32, t->d, N, ht->rest, pm, ht->rest_key64, ht->gval32);
-      else
-        hipLaunchKernelGGL((k_kt_gather<uint32_t, float>), dim3(g), dim3(256), 0, st, t->hedges,
-                           (const float*)t->vals32, t->d, N, ht->rest, pm, ht->rest_key32, ht->gval32);
-    } else if (rest64) {
-      hipLaunchKernelGGL((k_kt_gather<unsigned long long, double>), dim3(g), dim3(256), 0, st, t->hedges,
-                         (const double*)t->vals, t->d, N, ht->rest, pm, ht->rest_key64, ht->gval);
+  hipLaunchKernelGGL(k_kt_finalize, dim3(grid_for(cap, 256)), dim3(256), 0, st, ht->entries, (const uint32_t*)cnt,
+                     packed, cap, ht->slices, ht->direct ? 1 : 0);
+  const unsigned g = grid_for(N, 256, 16384);
+  const unsigned long long* sinfo = (const unsigned long long*)packed;
+  const uint32_t *so = slot_of, *ro = rank_of;
+  if (N && hashed) {
+    hipLaunchKernelGGL(k_kt_scatter_perm, dim3(g), dim3(256), 0, st, N, sinfo, so, ro, ht->perm);
+#define PTC_GATHER(RT, VT, RK, GV) \
+  hipLaunchKernelGGL((k_kt_gather_pay<RT, VT>), dim3(g), dim3(256), 0, st, N, (const uint32_t*)ht->perm, \
+                     (const KtPay<RT, VT>*)pay, RK, GV)
+    if (rest64) {
+      if (f32) PTC_GATHER(unsigned long long, float, ht->rest_key64, ht->gval32);
+      else PTC_GATHER(unsigned long long, double, ht->rest_key64, ht->gval);
     } else {
-      hipLaunchKernelGGL((k_kt_gather<uint32_t, double>), dim3(g), dim3(256), 0, st, t->hedges,
-                         (const double*)t->vals, t->d, N, ht->rest, pm, ht->rest_key32, ht->gval);
+      if (f32) PTC_GATHER(uint32_t, float, ht->rest_key32, ht->gval32);
+      else PTC_GATHER(uint32_t, double, ht->rest_key32, ht->gval);
     }
+#undef PTC_GATHER
+  } else if (N && counting) {
+    const unsigned long long* hp = (const unsigned long long*)hpre;
+#define PTC_PLACE(RT, VT, RK, GV, VALS)                                                                        \
+  hipLaunchKernelGGL((k_kt_place<RT, VT, true>), dim3(g), dim3(256), 0, st, N, (const uint32_t*)t->hedges, t->d, \
+                     ht->rest, (const VT*)(VALS), sinfo, hp, nblk, so, ro, ht->perm, ht->slice_of, RK, GV)
+    if (rest64) {
+      if (f32) PTC_PLACE(unsigned long long, float, ht->rest_key64, ht->gval32, t->vals32);
+      else PTC_PLACE(unsigned long long, double, ht->rest_key64, ht->gval, t->vals);
+    } else {
+      if (f32) PTC_PLACE(uint32_t, float, ht->rest_key32, ht->gval32, t->vals32);
+      else PTC_PLACE(uint32_t, double, ht->rest_key32, ht->gval, t->vals);
+    }
+#undef PTC_PLACE
+  } else if (N) {
+    hipLaunchKernelGGL(k_kt_scatter, dim3(g), dim3(256), 0, st, N, sinfo, so, ro, ht->perm, ht->slice_of);
+#define PTC_GATHER(RT, VT, RK, GV, VALS)                                                                       \
+  hipLaunchKernelGGL((k_kt_gather<RT, VT>), dim3(g), dim3(256), 0, st, t->hedges, (const VT*)(VALS), t->d, N, \
+                     ht->rest, (const uint32_t*)ht->perm, RK, GV)
+    if (rest64) {
+      if (f32) PTC_GATHER(unsigned long long, float, ht->rest_key64, ht->gval32, t->vals32);
+      else PTC_GATHER(unsigned long long, double, ht->rest_key64, ht->gval, t->vals);
+    } else {
+      if (f32) PTC_GATHER(uint32_t, float, ht->rest_key32, ht->gval32, t->vals32);
+      else PTC_GATHER(uint32_t, double, ht->rest_key32, ht->gval, t->vals);
+    }
+#undef PTC_GATHER
   }
   hipError_t e = hipStreamSynchronize(st);
   if (e != hipSuccess) {
@@ -620,6 +675,26 @@ extern "C" int ptc_hashtable_build(ptc_ctx* ctx, const ptc_dtensor* t, ptc_hasht
   *out = ht;
   return PTC_OK;
 }
+
+namespace ptc {
+// Slice index of every grouped position, built on first use (hashed tables
+// skip it at build time: only a table that serves as A needs it).
+int hashtable_slice_of(ptc_ctx* ctx, const ptc_hashtable* hc, const uint32_t** out) {
+  ptc_hashtable* h = const_cast<ptc_hashtable*>(hc);
+  if (!h->slice_of) {
+    const uint64_t N = h->t->N;
+    h->slice_of = dalloc_n<uint32_t>(ctx, std::max<uint64_t>(N, 1));
+    if (!h->slice_of) return PTC_ERR_NOMEM;
+    if (N && h->num_keys) {
+      hipLaunchKernelGGL(k_kt_slice_of, dim3(grid_for(N, 256, 16384)), dim3(256), 0, ctx->stream, N,
+                         (const KtSlice*)h->slices, h->num_keys, h->slice_of);
+      PTC_HIP(ctx, hipGetLastError());
+    }
+  }
+  *out = h->slice_of;
+  return PTC_OK;
+}
+}  // namespace ptc
 
 extern "C" int ptc_hashtable_info(const ptc_hashtable* h, uint64_t* num_keys, uint64_t* num_items) {
   if (!h) return PTC_ERR_ARG;

@@ -358,6 +358,8 @@ static int prepare(Run& R, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B,
                                          : (R.dense ? std::max<uint64_t>(DEFAULT_CHUNK, 8 * R.ncols) : DEFAULT_CHUNK);
 
   int rc = PTC_OK;
+  const uint32_t* row_of_pos = nullptr;  // HT_A's slice index of every A position
+  if ((rc = hashtable_slice_of(ctx, HT_A, &row_of_pos))) return rc;
   Workspace& ws = R.ws;
   R.pos_len = ws.get<uint32_t>(R.npos);
   R.pos_off = ws.get<uint32_t>(R.npos);
@@ -383,7 +385,7 @@ static int prepare(Run& R, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B,
     ProbeParams P{};
     P.npos = R.npos;
     P.akey = HT_A->rest_key64 ? (const void*)HT_A->rest_key64 : (const void*)HT_A->rest_key32;
-    P.row_of_pos = HT_A->slice_of;
+    P.row_of_pos = row_of_pos;
     P.acodec = HT_A->rest;
     P.bcodec = HT_B->codec;
     P.same_codec = 1;
@@ -400,7 +402,7 @@ static int prepare(Run& R, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B,
   }
   if (opt.method == PTC_METHOD_LEARNED || R.want_features) {
     if ((rc = fill_zero(ctx, R.row_hits, R.nrows * 4)) || (rc = fill_zero(ctx, R.row_maxlen, R.nrows * 4))) return rc;
-    if ((rc = launch_row_features(ctx, HT_A->slice_of, R.npos, R.pos_len, R.row_hits, R.row_maxlen))) return rc;
+    if ((rc = launch_row_features(ctx, row_of_pos, R.npos, R.pos_len, R.row_hits, R.row_maxlen))) return rc;
   }
   if ((rc = scan_u32_to_u64(ctx, R.pos_len, R.apre, R.npos, R.apre + R.npos))) return rc;
   {
@@ -446,7 +448,7 @@ static int prepare(Run& R, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B,
     P.items = R.items;
     ItemP0Params Q{};
     Q.npos = R.nheavy ? R.npos : 0;
-    Q.row_of_pos = HT_A->slice_of;
+    Q.row_of_pos = row_of_pos;
     Q.row_nchunk = R.row_nchunk;
     Q.pos_len = R.pos_len;
     Q.rows = HT_A->slices;
