@@ -24,7 +24,8 @@ struct ProbeParams {
 };
 
 struct RowParams {
-  uint64_t nrows;
+  uint64_t nrows;                 // rows allocated (a bound when live is set)
+  const unsigned long long* live; // HT_A's device total: rows = *live >> 32 (NULL: nrows)
   const KtSlice* rows;
   const unsigned long long* apre;
   unsigned long long chunk;

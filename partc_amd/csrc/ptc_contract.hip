@@ -180,7 +180,15 @@ int launch_row_features(ptc_ctx* ctx, const uint32_t* row_of_pos, uint64_t npos,
 // ------------------------------------------------------------------ rows
 
 __global__ void k_rows(RowParams P) {
+  const uint64_t live = P.live ? (*P.live >> 32) : P.nrows;
   for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < P.nrows; r += (uint64_t)gridDim.x * blockDim.x) {
+    if (r >= live) {  // allocated beyond the table's rows: no work, zero sums
+      P.row_nchunk[r] = 0;
+      P.row_flops[r] = 0;
+      P.packed[r] = 0;
+      P.slabs[r] = 0;
+      continue;
+    }
     KtSlice s = P.rows[r];
     unsigned long long f = P.apre[(uint64_t)s.off + s.len] - P.apre[s.off];
     uint32_t nch = f ? (uint32_t)((f + P.chunk - 1) / P.chunk) : 0u;
@@ -1013,7 +1021,11 @@ size_t pass_lds_bytes(int pass, bool dense, bool col64, uint64_t ncols, uint32_t
 template <int PASS, bool DENSE, class ColT, bool MERGE, int VAL = 0>
 static int launch_pass_t(ptc_ctx* ctx, const PassParams& P, uint64_t nitems, size_t lds) {
   auto kern = k_pass<PASS, DENSE, ColT, MERGE, VAL>;
-  PTC_HIP(ctx, hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  static size_t lds_set = 0;  // largest dynamic LDS this kernel was opened to
+  if (lds > lds_set) {
+    PTC_HIP(ctx, hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    lds_set = lds;
+  }
   const bool timed = ctx->nkev + 2 <= 16;
   if (timed) hipEventRecord(ctx->kev[ctx->nkev], ctx->stream);
   hipLaunchKernelGGL(kern, dim3((unsigned)nitems), dim3(PB), lds, ctx->stream, P);

@@ -110,8 +110,14 @@ struct ptc_hashtable {
   uint64_t cap = 0;            // table slots (power of two; the key space when direct)
   bool direct = false;         // slot = key (small key spaces)
   KtEntry* entries = nullptr;  // device, cap
+  // Distinct keys.  A build does not wait for the device: the count stays in
+  // d_total ((keys << 32) | grouped nonzeros) until hashtable_num_keys reads
+  // it back once.
   uint64_t num_keys = 0;
-  KtSlice* slices = nullptr;   // device, num_keys (slot order)
+  bool keys_known = false;
+  unsigned long long* d_total = nullptr;
+  uint64_t slices_cap = 0;     // slices allocated: min(N, cap) >= num_keys
+  KtSlice* slices = nullptr;   // device, num_keys used (slot order)
   uint32_t* perm = nullptr;    // device, N: original ids grouped by key
   uint32_t* slice_of = nullptr;// device, N: compact slice index of each grouped position
                                // (hashed tables: built on first use, hashtable_slice_of)
@@ -123,7 +129,8 @@ struct ptc_hashtable {
   unsigned long long* rest_key64 = nullptr;
   double* gval = nullptr;      // device, N: values in grouped order
   float* gval32 = nullptr;     // device, N: values in grouped order, FP32 tensors (gval is then NULL)
-  double ms_build = 0.0;
+  hipEvent_t ev_start = nullptr, ev_end = nullptr;  // bracket the build on the stream
+  double ms_build = -1.0;      // from the events, on first request (hashtable_ms_build)
 };
 
 #endif  // __HIPCC__
@@ -133,6 +140,8 @@ namespace ptc {
 
 int set_err(ptc_ctx* ctx, int code, const char* fmt, ...);
 int hashtable_slice_of(ptc_ctx* ctx, const ptc_hashtable* h, const uint32_t** out);
+int hashtable_num_keys(ptc_ctx* ctx, const ptc_hashtable* h, uint64_t* out);
+double hashtable_ms_build(const ptc_hashtable* h);
 int hip_check(ptc_ctx* ctx, hipError_t e, const char* what);
 #define PTC_HIP(ctx, call)                                 \
   do {                                                     \

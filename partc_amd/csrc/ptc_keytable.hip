@@ -386,8 +386,9 @@ __global__ void k_kt_gather_pay(uint64_t N, const uint32_t* __restrict__ perm, c
 
 // Slice index of every grouped position: binary search of the slice
 // offsets (ascending with the slice index).
-__global__ void k_kt_slice_of(uint64_t N, const KtSlice* __restrict__ slices, uint64_t nslices,
-                              uint32_t* __restrict__ slice_of) {
+__global__ void k_kt_slice_of(uint64_t N, const KtSlice* __restrict__ slices,
+                              const unsigned long long* __restrict__ total, uint32_t* __restrict__ slice_of) {
+  const uint64_t nslices = *total >> 32;  // >= 1 when N > 0
   for (uint64_t pos = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; pos < N; pos += (uint64_t)gridDim.x * blockDim.x) {
     uint64_t lo = 0, hi = nslices - 1;
     while (lo < hi) {
@@ -503,14 +504,18 @@ extern "C" int ptc_hashtable_build(ptc_ctx* ctx, const ptc_dtensor* t, ptc_hasht
   if (!ctx || !t || !out) return PTC_ERR_ARG;
   *out = nullptr;
   PTC_HIP(ctx, hipSetDevice(ctx->device));
-  auto t0 = std::chrono::steady_clock::now();
   ptc_hashtable* ht = new ptc_hashtable();
   ht->ctx = ctx;
   ht->t = t;
+  if (hipEventCreate(&ht->ev_start) != hipSuccess || hipEventCreate(&ht->ev_end) != hipSuccess) {
+    ptc_hashtable_free(ht);
+    return hip_check(ctx, hipGetLastError(), "event");
+  }
+  hipEventRecord(ht->ev_start, ctx->stream);
   int rc = make_codec(ctx, t, t->dim_tc, t->nk, &ht->codec);
   if (!rc) rc = make_codec(ctx, t, t->dim_tc + t->nk, t->d - t->nk, &ht->rest);
   if (rc) {
-    delete ht;
+    ptc_hashtable_free(ht);
     return rc;
   }
   const uint64_t N = t->N;
@@ -599,27 +604,18 @@ extern "C" int ptc_hashtable_build(ptc_ctx* ctx, const ptc_dtensor* t, ptc_hasht
   }
   hipLaunchKernelGGL(k_kt_pack, dim3(grid_for(cap, 256)), dim3(256), 0, st, (const uint32_t*)cnt,
                      (const KtEntry*)ht->entries, packed, cap);
-  rc = scan_u64(ctx, packed, packed, cap, packed + cap);
-  if (rc) {
-    ptc_hashtable_free(ht);
-    return rc;
-  }
-  unsigned long long total = 0;
-  rc = read_back(ctx, &total, packed + cap, sizeof(total));
-  if (rc) {
-    ptc_hashtable_free(ht);
-    return rc;
-  }
-  ht->num_keys = total >> 32;
-  if ((total & 0xFFFFFFFFull) != N) {
-    ptc_hashtable_free(ht);
-    return set_err(ctx, PTC_ERR_INTERNAL, "key table grouped %llu of %llu nonzeros",
-                   (unsigned long long)(total & 0xFFFFFFFFull), (unsigned long long)N);
-  }
-  ht->slices = dalloc_n<KtSlice>(ctx, ht->num_keys);
-  if (!ht->slices) {
+  // no read-back: slices are sized by the bound min(N, cap) on distinct keys
+  ht->d_total = dalloc_n<unsigned long long>(ctx, 1);
+  ht->slices_cap = std::max<uint64_t>(1, std::min<uint64_t>(N, cap));
+  ht->slices = dalloc_n<KtSlice>(ctx, ht->slices_cap);
+  if (!ht->d_total || !ht->slices) {
     ptc_hashtable_free(ht);
     return PTC_ERR_NOMEM;
+  }
+  rc = scan_u64(ctx, packed, packed, cap, ht->d_total);
+  if (rc) {
+    ptc_hashtable_free(ht);
+    return rc;
   }
   hipLaunchKernelGGL(k_kt_finalize, dim3(grid_for(cap, 256)), dim3(256), 0, st, ht->entries, (const uint32_t*)cnt,
                      packed, cap, ht->slices, ht->direct ? 1 : 0);
@@ -666,12 +662,12 @@ extern "C" int ptc_hashtable_build(ptc_ctx* ctx, const ptc_dtensor* t, ptc_hasht
     }
 #undef PTC_GATHER
   }
-  hipError_t e = hipStreamSynchronize(st);
+  hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     ptc_hashtable_free(ht);
     return hip_check(ctx, e, "hashtable build");
   }
-  ht->ms_build = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  hipEventRecord(ht->ev_end, st);
   *out = ht;
   return PTC_OK;
 }
@@ -685,20 +681,51 @@ int hashtable_slice_of(ptc_ctx* ctx, const ptc_hashtable* hc, const uint32_t** o
     const uint64_t N = h->t->N;
     h->slice_of = dalloc_n<uint32_t>(ctx, std::max<uint64_t>(N, 1));
     if (!h->slice_of) return PTC_ERR_NOMEM;
-    if (N && h->num_keys) {
+    if (N) {
       hipLaunchKernelGGL(k_kt_slice_of, dim3(grid_for(N, 256, 16384)), dim3(256), 0, ctx->stream, N,
-                         (const KtSlice*)h->slices, h->num_keys, h->slice_of);
+                         (const KtSlice*)h->slices, (const unsigned long long*)h->d_total, h->slice_of);
       PTC_HIP(ctx, hipGetLastError());
     }
   }
   *out = h->slice_of;
   return PTC_OK;
 }
+
+int hashtable_num_keys(ptc_ctx* ctx, const ptc_hashtable* hc, uint64_t* out) {
+  ptc_hashtable* h = const_cast<ptc_hashtable*>(hc);
+  if (!h->keys_known) {
+    unsigned long long total = 0;
+    int rc = read_back(ctx, &total, h->d_total, sizeof(total));
+    if (rc) return rc;
+    if ((total & 0xFFFFFFFFull) != h->t->N)
+      return set_err(ctx, PTC_ERR_INTERNAL, "key table grouped %llu of %llu nonzeros",
+                     (unsigned long long)(total & 0xFFFFFFFFull), (unsigned long long)h->t->N);
+    h->num_keys = total >> 32;
+    h->keys_known = true;
+  }
+  *out = h->num_keys;
+  return PTC_OK;
+}
+
+double hashtable_ms_build(const ptc_hashtable* hc) {
+  ptc_hashtable* h = const_cast<ptc_hashtable*>(hc);
+  if (h->ms_build < 0 && h->ev_start && h->ev_end) {
+    float ms = 0.f;
+    if (hipEventSynchronize(h->ev_end) == hipSuccess && hipEventElapsedTime(&ms, h->ev_start, h->ev_end) == hipSuccess)
+      h->ms_build = ms;
+    else
+      (void)hipGetLastError();
+  }
+  return h->ms_build;
+}
 }  // namespace ptc
 
 extern "C" int ptc_hashtable_info(const ptc_hashtable* h, uint64_t* num_keys, uint64_t* num_items) {
   if (!h) return PTC_ERR_ARG;
-  if (num_keys) *num_keys = h->num_keys;
+  if (num_keys) {
+    int rc = hashtable_num_keys(h->ctx, h, num_keys);
+    if (rc) return rc;
+  }
   if (num_items) *num_items = h->t ? h->t->N : 0;
   return PTC_OK;
 }
@@ -756,6 +783,9 @@ extern "C" void ptc_hashtable_free(ptc_hashtable* h) {
     dfree(h->ctx, h->rest_key64);
     dfree(h->ctx, h->gval);
     dfree(h->ctx, h->gval32);
+    dfree(h->ctx, h->d_total);
   }
+  if (h->ev_start) hipEventDestroy(h->ev_start);
+  if (h->ev_end) hipEventDestroy(h->ev_end);
   delete h;
 }

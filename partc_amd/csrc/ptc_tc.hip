@@ -24,6 +24,7 @@ namespace ptc {
 static constexpr uint32_t DEFAULT_DENSE_MAX = 8192;
 static constexpr uint32_t DEFAULT_LDS_HASH = 2048;
 static constexpr uint64_t DEFAULT_CHUNK = 32768;
+static constexpr unsigned long long STAGING_BOUND = 1ull << 28;  // entries (4 GB of column keys + FP64 values)
 
 struct Timer {
   ptc_ctx* ctx;
@@ -328,7 +329,9 @@ static int prepare(Run& R, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B,
   R.HB = HT_B;
   R.opt = opt;
   R.npos = A->N;
-  R.nrows = HT_A->num_keys;
+  // rows: HT_A's key count when known, else its bound; the exact count is
+  // read back with the preprocessing totals below
+  R.nrows = HT_A->keys_known ? HT_A->num_keys : HT_A->slices_cap;
   R.ncols = HT_B->rest.space;
   R.col64 = R.ncols > 0xFFFFFFFFull;
   R.nrowmodes = A->nk;
@@ -408,6 +411,7 @@ static int prepare(Run& R, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B,
   {
     RowParams P{};
     P.nrows = R.nrows;
+    P.live = HT_A->keys_known ? nullptr : HT_A->d_total;
     P.rows = HT_A->slices;
     P.apre = R.apre;
     P.chunk = chunk;
@@ -419,16 +423,26 @@ static int prepare(Run& R, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B,
   }
   if ((rc = scan_u64(ctx, R.packed, R.packed, R.nrows, R.packed + R.nrows))) return rc;
   if ((rc = scan_u64(ctx, R.row_slab, R.row_slab, R.nrows, R.row_slab + R.nrows))) return rc;
-  unsigned long long hostv[3] = {0, 0, 0};
+  unsigned long long hostv[4] = {0, 0, 0, 0};
   {
-    // one host round trip for the three totals
-    unsigned long long* tot = ws.get<unsigned long long>(3);
+    // one host round trip for the totals and HT_A's key count
+    unsigned long long* tot = ws.get<unsigned long long>(4);
     if (!tot) return PTC_ERR_NOMEM;
     PTC_HIP(ctx, hipMemcpyAsync(tot, R.apre + R.npos, 8, hipMemcpyDeviceToDevice, ctx->stream));
     PTC_HIP(ctx, hipMemcpyAsync(tot + 1, R.packed + R.nrows, 8, hipMemcpyDeviceToDevice, ctx->stream));
     PTC_HIP(ctx, hipMemcpyAsync(tot + 2, R.row_slab + R.nrows, 8, hipMemcpyDeviceToDevice, ctx->stream));
-    if ((rc = read_back(ctx, hostv, tot, 24))) return rc;
+    PTC_HIP(ctx, hipMemcpyAsync(tot + 3, HT_A->d_total, 8, hipMemcpyDeviceToDevice, ctx->stream));
+    if ((rc = read_back(ctx, hostv, tot, 32))) return rc;
   }
+  if (!HT_A->keys_known) {
+    if ((hostv[3] & 0xFFFFFFFFull) != A->N)
+      return set_err(ctx, PTC_ERR_INTERNAL, "key table grouped %llu of %llu nonzeros",
+                     (unsigned long long)(hostv[3] & 0xFFFFFFFFull), (unsigned long long)A->N);
+    ptc_hashtable* ha = const_cast<ptc_hashtable*>(HT_A);
+    ha->num_keys = hostv[3] >> 32;
+    ha->keys_known = true;
+  }
+  R.nrows = HT_A->num_keys;
   R.nslabs = hostv[2];
   R.F = hostv[0];
   R.nitems = hostv[1] & 0xFFFFFFFFull;
@@ -553,8 +567,12 @@ extern "C" int ptc_tc_fksCuckoo(ptc_ctx* ctx, const ptc_hashtable* HT_A, const p
       if ((rc = launch_caps(ctx, P))) return rc;
     }
     if ((rc = scan_u32_to_u64(ctx, cap, cap_off, R.nrows, cap_off + R.nrows))) return rc;
-    unsigned long long capsum = 0;
-    if ((rc = read_back(ctx, &capsum, cap_off + R.nrows, 8))) return rc;
+    // Every capacity is at most min(row flops, columns), so the staging area
+    // is sized by min(F, rows x columns) without waiting for the sum; only a
+    // bound above STAGING_BOUND entries is replaced by the exact sum.
+    unsigned long long capsum = R.F;
+    if (R.ncols && R.nrows <= ~0ull / R.ncols) capsum = std::min<unsigned long long>(capsum, R.nrows * R.ncols);
+    if (capsum > STAGING_BOUND && (rc = read_back(ctx, &capsum, cap_off + R.nrows, 8))) return rc;
     const size_t kb = R.col64 ? 8 : 4;
     void* stg_col = ws.get<unsigned char>(capsum * kb);
     double* stg_val = ws.get<double>(capsum);
@@ -572,10 +590,22 @@ extern "C" int ptc_tc_fksCuckoo(ptc_ctx* ctx, const ptc_hashtable* HT_A, const p
     P.stg_val = stg_val;
     if ((rc = run_pass(R, PASS_NUMERIC, &h1, R.items, R.nitems, P))) return rc;
     T.mark();  // 2
-    uint32_t ovf[2] = {0, 0};
-    if ((rc = read_back(ctx, ovf, R.overflow, 4))) return rc;
-    if (ovf[0] & 0x80000000u) return set_err(ctx, PTC_ERR_INTERNAL, "accumulator table overflow");
-    fallback_rows = ovf[0];
+    // In the common case no row overflowed its capacity and the scan of the
+    // counts is final: read the overflow word and the total together.
+    if ((rc = scan_u32_to_u64(ctx, R.row_nnz, out_off, R.nrows, out_off + R.nrows))) return rc;
+    unsigned long long rb[2] = {0, 0};
+    {
+      unsigned long long* pair = ws.get<unsigned long long>(2);
+      if (!pair) return PTC_ERR_NOMEM;
+      PTC_HIP(ctx, hipMemcpyAsync(pair, R.overflow, 4, hipMemcpyDeviceToDevice, ctx->stream));
+      PTC_HIP(ctx, hipMemcpyAsync(pair + 1, out_off + R.nrows, 8, hipMemcpyDeviceToDevice, ctx->stream));
+      if ((rc = read_back(ctx, rb, pair, 16))) return rc;
+    }
+    const uint32_t ovf0 = (uint32_t)rb[0];
+    if (ovf0 & 0x80000000u) return set_err(ctx, PTC_ERR_INTERNAL, "accumulator table overflow");
+    if (ovf0 & 0x40000000u) return set_err(ctx, PTC_ERR_INTERNAL, "slice index out of range");
+    fallback_rows = ovf0;
+    nnz = rb[1];
     uint4* fitems = nullptr;
     uint32_t nfitems = 0;
     uint32_t* frows = nullptr;
@@ -599,10 +629,10 @@ extern "C" int ptc_tc_fksCuckoo(ptc_ctx* ctx, const ptc_hashtable* HT_A, const p
       PassParams S;
       base_params(R, S, R.dense ? nullptr : &hs);
       if ((rc = run_pass(R, PASS_SYMBOLIC, &hs, fitems, nfitems, S, frows, fallback_rows))) return rc;
+      if ((rc = scan_u32_to_u64(ctx, R.row_nnz, out_off, R.nrows, out_off + R.nrows))) return rc;
+      if ((rc = read_back(ctx, &hostv[0], out_off + R.nrows, 8))) return rc;
+      nnz = hostv[0];
     }
-    if ((rc = scan_u32_to_u64(ctx, R.row_nnz, out_off, R.nrows, out_off + R.nrows))) return rc;
-    if ((rc = read_back(ctx, &hostv[0], out_off + R.nrows, 8))) return rc;
-    nnz = hostv[0];
     if ((rc = make_output(R, nnz, A, B, &out))) return rc;
     {
       EmitParams E{};
@@ -650,16 +680,20 @@ extern "C" int ptc_tc_fksCuckoo(ptc_ctx* ctx, const ptc_hashtable* HT_A, const p
     }
     T.mark();  // 3
   }
-  uint32_t ovf = 0;
-  rc = read_back(ctx, &ovf, R.overflow, 4);
-  if (!rc && (ovf & 0x80000000u)) rc = set_err(ctx, PTC_ERR_INTERNAL, "row count mismatch between passes");
-  if (!rc && (ovf & 0x40000000u)) rc = set_err(ctx, PTC_ERR_INTERNAL, "slice index out of range");
-  if (rc) {
-    ptc_dtensor_free(out);
-    return rc;
+  // The consistency flags of the last numeric pass: in the single-pass
+  // methods without fallback rows they were read with the counts already.
+  if (method == PTC_METHOD_TWO_PASS || fallback_rows) {
+    uint32_t ovf = 0;
+    rc = read_back(ctx, &ovf, R.overflow, 4);
+    if (!rc && (ovf & 0x80000000u)) rc = set_err(ctx, PTC_ERR_INTERNAL, "row count mismatch between passes");
+    if (!rc && (ovf & 0x40000000u)) rc = set_err(ctx, PTC_ERR_INTERNAL, "slice index out of range");
+    if (rc) {
+      ptc_dtensor_free(out);
+      return rc;
+    }
   }
-  hipEventSynchronize(T.ev[T.n - 1]);
   if (stats) {
+    hipEventSynchronize(T.ev[T.n - 1]);
     memset(stats, 0, sizeof(*stats));
     stats->nnz_a = A->N;
     stats->nnz_b = B->N;
@@ -668,8 +702,8 @@ extern "C" int ptc_tc_fksCuckoo(ptc_ctx* ctx, const ptc_hashtable* HT_A, const p
     stats->nnz_out = nnz;
     stats->fallback_rows = fallback_rows;
     stats->heavy_rows = R.nheavy;
-    stats->ms_build_a = HT_A->ms_build;
-    stats->ms_build_b = HT_B->ms_build;
+    stats->ms_build_a = hashtable_ms_build(HT_A);
+    stats->ms_build_b = hashtable_ms_build(HT_B);
     stats->ms_preprocess = T.ms(0, 1);
     if (method == PTC_METHOD_TWO_PASS) {
       stats->ms_symbolic = T.ms(1, 2);
