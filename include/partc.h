@@ -101,6 +101,11 @@ int ptc_dtensor_download(ptc_ctx* ctx, const ptc_dtensor* t, uint32_t* hedges, d
  * receives the compensation: hi = vals is the sum rounded to FP32 and
  * hi + lo carries it to about twice FP32 precision. */
 int ptc_dtensor_download_f32(ptc_ctx* ctx, const ptc_dtensor* t, float* vals, float* lo);
+/* Overwrite the FP64 values of a device tensor in place (same coordinates):
+ * the input of a batch plan's replay (ptc_batch_run).  Values must keep the
+ * tensor's class for the single-contraction entry points: nonzero and far
+ * from overflow if they were at upload. */
+int ptc_dtensor_set_values(ptc_ctx* ctx, ptc_dtensor* t, const double* vals);
 void ptc_dtensor_free(ptc_dtensor* t);
 
 /* ------------------------------------------------------------ .meta spec */
@@ -216,6 +221,45 @@ int ptc_matricize_spGEMM(ptc_ctx* ctx, const ptc_dtensor* A, const ptc_dtensor* 
                          ptc_dtensor** O, ptc_stats* stats);
 int ptc_matricize_sort_spGEMM(ptc_ctx* ctx, const ptc_dtensor* A, const ptc_dtensor* B,
                               ptc_dtensor** O, ptc_stats* stats);
+
+/* --------------------------------------------------- batched contractions */
+
+/* Many independent contractions of uneven sizes, each with ptc_contract's
+ * meaning (A and B contracted-first, the exact two-pass method, FP64).  No
+ * reference counterpart: main.cpp:59-118 runs one contraction per process.
+ *
+ * ptc_batch_plan builds every job's key tables, runs the preprocessing and
+ * symbolic passes of all jobs (one host round trip per step for the whole
+ * batch; each pass one persistent launch per work bin) and allocates the
+ * outputs.  The numeric phase is captured once as a hipGraph: re-reading the
+ * values of A and B from their device arrays, then one persistent numeric
+ * launch per bin.  ptc_batch_run launches it; calling it again after the
+ * values of A or B changed (same coordinates) recomputes every output.  A
+ * plan keeps its tables, work arrays and outputs until ptc_batch_free; the
+ * tensors must outlive it. */
+typedef struct ptc_batch ptc_batch;
+typedef struct {
+  uint32_t jobs;
+  uint32_t bins;          /* numeric launch bins (kernel variant x LDS class) */
+  uint32_t queue_items;   /* work items over all jobs                         */
+  uint32_t runs;          /* numeric graph launches so far                    */
+  uint64_t flops, nnz_out, graph_nodes;
+  double ms_build;        /* all key tables (device time)                     */
+  double ms_symbolic;     /* preprocessing + symbolic passes (device time)    */
+  double ms_plan;         /* ptc_batch_plan, wall                             */
+  double ms_numeric;      /* last numeric graph launch (device time)          */
+  double ms_total;        /* ptc_contract_batched, wall                       */
+} ptc_batch_stats;
+
+int ptc_batch_plan(ptc_ctx* ctx, int n, const ptc_dtensor* const* A, const ptc_dtensor* const* B,
+                   const ptc_options* opt, ptc_batch** out, ptc_batch_stats* stats);
+int ptc_batch_run(ptc_batch* b, ptc_batch_stats* stats);
+/* Job j's output, owned by the plan. */
+int ptc_batch_output(const ptc_batch* b, int j, const ptc_dtensor** O);
+void ptc_batch_free(ptc_batch* b);
+/* Plan, run once and hand the outputs over (free each with ptc_dtensor_free). */
+int ptc_contract_batched(ptc_ctx* ctx, int n, const ptc_dtensor* const* A, const ptc_dtensor* const* B,
+                         const ptc_options* opt, ptc_dtensor** O, ptc_batch_stats* stats);
 
 /* ------------------------------------------------ learned row-size model */
 

@@ -82,6 +82,26 @@ class Options(C.Structure):
     ]
 
 
+class BatchStats(C.Structure):
+    _fields_ = [
+        ("jobs", C.c_uint32),
+        ("bins", C.c_uint32),
+        ("queue_items", C.c_uint32),
+        ("runs", C.c_uint32),
+        ("flops", C.c_uint64),
+        ("nnz_out", C.c_uint64),
+        ("graph_nodes", C.c_uint64),
+        ("ms_build", C.c_double),
+        ("ms_symbolic", C.c_double),
+        ("ms_plan", C.c_double),
+        ("ms_numeric", C.c_double),
+        ("ms_total", C.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class Stats(C.Structure):
     _fields_ = [
         ("nnz_a", C.c_uint64),
@@ -146,6 +166,14 @@ def lib() -> C.CDLL:
         "ptc_matricize_spGEMM": (i32, [vp, vp, vp, C.POINTER(vp), C.POINTER(Stats)]),
         "ptc_matricize_sort_spGEMM": (i32, [vp, vp, vp, C.POINTER(vp), C.POINTER(Stats)]),
         "ptc_gen_coo": (i32, [u64, u32, P32, u64, i32, C.c_double, P32, PD]),
+        "ptc_dtensor_set_values": (i32, [vp, vp, PD]),
+        "ptc_batch_plan": (i32, [vp, i32, C.POINTER(vp), C.POINTER(vp), C.POINTER(Options), C.POINTER(vp),
+                                 C.POINTER(BatchStats)]),
+        "ptc_batch_run": (i32, [vp, C.POINTER(BatchStats)]),
+        "ptc_batch_output": (i32, [vp, i32, C.POINTER(vp)]),
+        "ptc_batch_free": (None, [vp]),
+        "ptc_contract_batched": (i32, [vp, i32, C.POINTER(vp), C.POINTER(vp), C.POINTER(Options), C.POINTER(vp),
+                                       C.POINTER(BatchStats)]),
         "ptc_gen_job_shape": (i32, [u64, u32, u32, u64, u64, P32, P32, P64]),
     }
     for name, (res, args) in sig.items():
@@ -282,8 +310,8 @@ class Context:
 
     def close(self) -> None:
         if self._h:
-            # tables before tensors (a table references its tensor)
-            kids = sorted(list(self._children), key=lambda o: 0 if isinstance(o, HashTable) else 1)
+            # tables and batch plans before tensors (they reference tensors)
+            kids = sorted(list(self._children), key=lambda o: 0 if isinstance(o, (HashTable, BatchPlan)) else 1)
             for obj in kids:
                 try:
                     obj.free()
@@ -353,6 +381,11 @@ class DeviceTensor:
         dims = np.zeros(MAX_MODES, dtype=np.uint32)
         _check(lib().ptc_dtensor_download(self.ctx.handle, self._h, _p32(h), _pd(v), _p32(dims)), self.ctx)
         return Tensor(h, v, dims[:d].copy(), np.arange(d, dtype=np.uint32), nk)
+
+    def set_values(self, vals) -> None:
+        """Overwrite the FP64 values in place (same coordinates)."""
+        v = np.ascontiguousarray(vals, dtype=np.float64)
+        _check(lib().ptc_dtensor_set_values(self.ctx.handle, self._h, _pd(v)), self.ctx)
 
     def download_f32(self, with_lo: bool = False):
         """FP32 values; with_lo=True also returns the lo part of an FP32
@@ -480,6 +513,77 @@ def contract(A, B, ctx: Optional[Context] = None, *, download: bool = True, stat
     _check(lib().ptc_contract(ctx.handle, dA._h, dB._h, C.byref(o), C.byref(out), C.byref(st)), ctx)
     dO = DeviceTensor(ctx, out)
     return dO.download() if download else dO
+
+
+def _handles(ts, ctx):
+    devs = [_as_device(t, ctx) for t in ts]
+    arr = (C.c_void_p * len(devs))(*[d._h.value for d in devs])
+    return devs, arr
+
+
+class BatchPlan:
+    """A batch of contractions planned on the device (ptc_batch_plan): run()
+    launches the captured numeric graph; after changing input values in
+    place (DeviceTensor.set_values) run() recomputes every output."""
+
+    def __init__(self, As, Bs, ctx: Optional[Context] = None, **opts):
+        if len(As) != len(Bs) or not As:
+            raise ValueError("need equally many A and B tensors")
+        self.ctx = ctx or default_context()
+        self._A, a = _handles(As, self.ctx)
+        self._B, b = _handles(Bs, self.ctx)
+        o = options(**opts)
+        self.stats = BatchStats()
+        h = C.c_void_p()
+        _check(lib().ptc_batch_plan(self.ctx.handle, len(As), a, b, C.byref(o), C.byref(h), C.byref(self.stats)),
+               self.ctx)
+        self._h = h
+        self.n = len(As)
+        self.ctx.adopt(self)
+
+    @property
+    def inputs(self):
+        return self._A, self._B
+
+    def run(self) -> BatchStats:
+        _check(lib().ptc_batch_run(self._h, C.byref(self.stats)), self.ctx)
+        return self.stats
+
+    def output(self, j: int) -> Tensor:
+        h = C.c_void_p()
+        _check(lib().ptc_batch_output(self._h, j, C.byref(h)), self.ctx)
+        t = DeviceTensor(self.ctx, h)
+        try:
+            return t.download()
+        finally:
+            t._h = C.c_void_p()  # owned by the plan
+
+    def free(self) -> None:
+        if self._h:
+            lib().ptc_batch_free(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def contract_batched(As, Bs, ctx: Optional[Context] = None, *, download: bool = True,
+                     stats: Optional[BatchStats] = None, **opts):
+    """Every (A[j], B[j]) contracted as contract() would, as one batch
+    (ptc_contract_batched)."""
+    ctx = ctx or default_context()
+    keep_a, a = _handles(As, ctx)  # the device tensors must outlive the call
+    keep_b, b = _handles(Bs, ctx)
+    o = options(**opts)
+    outs = (C.c_void_p * len(As))()
+    st = stats if stats is not None else BatchStats()
+    _check(lib().ptc_contract_batched(ctx.handle, len(As), a, b, C.byref(o), outs, C.byref(st)), ctx)
+    del keep_a, keep_b
+    res = [DeviceTensor(ctx, C.c_void_p(x)) for x in outs]
+    return [r.download() for r in res] if download else res
 
 
 def matricize_spGEMM(A, B, ctx: Optional[Context] = None) -> Tensor:

@@ -1,0 +1,532 @@
+// Batched contractions: many independent contractions of uneven sizes.
+//
+// Each job is a ptc_contract (.meta-style tensors, A_copy made free-first,
+// HT_B then HT_A, exact two-pass method), but the batch is organised so that
+// the device is not driven job by job:
+//
+//   plan   all key tables are enqueued without waiting; every step that ends
+//          in a device -> host read (preprocessing totals, HBM accumulator
+//          pools, output sizes) is launched for every job first and read back
+//          for the whole batch in one round trip; the symbolic passes of all
+//          jobs run as persistent launches over one work queue per bin.
+//   run    the numeric phase -- refresh of the grouped values from the input
+//          tensors, accumulator resets, one persistent numeric launch per bin
+//          and its merge launch -- is captured once in a hipGraph and
+//          launched as a whole.  Replaying it recomputes every output for new
+//          values on the same sparsity pattern.
+//
+// Bins: jobs are grouped by accumulator kind (dense LDS columns / LDS or HBM
+// hash), column key width and LDS footprint class, so one launch's dynamic
+// LDS fits all its jobs without sizing small jobs for the largest.  Within a
+// bin the queue lists the jobs with the heaviest work items first.
+//
+// No reference counterpart: main.cpp:59-118 runs one contraction per
+// process.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <tuple>
+#include <vector>
+
+#include "ptc_contract.h"
+#include "ptc_internal.h"
+#include "ptc_run.h"
+
+namespace ptc {
+
+// One table's value refresh: gval[pos] = vals[perm[pos]].
+struct RefreshSeg {
+  const uint32_t* perm;
+  const double* vals;
+  double* gval;
+  uint64_t n;
+};
+
+// One fill: n 32-bit words (or 64-bit when wide) set to v.
+struct FillSeg {
+  void* p;
+  uint64_t n;
+  unsigned long long v;
+  int wide;
+};
+
+__global__ void k_refresh_batch(const RefreshSeg* __restrict__ segs, const unsigned long long* __restrict__ start,
+                                int nseg, unsigned long long total) {
+  for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (unsigned long long)gridDim.x * blockDim.x) {
+    int lo = 0, hi = nseg - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (start[mid] <= i) lo = mid; else hi = mid - 1;
+    }
+    const RefreshSeg s = segs[lo];
+    const uint64_t pos = i - start[lo];
+    s.gval[pos] = s.vals[s.perm[pos]];
+  }
+}
+
+// one workgroup per segment; grid-stride inside it
+__global__ void k_fill_batch(const FillSeg* __restrict__ segs, int nseg) {
+  for (int k = blockIdx.x; k < nseg; k += gridDim.x) {
+    const FillSeg s = segs[k];
+    if (s.wide) {
+      unsigned long long* p = (unsigned long long*)s.p;
+      for (uint64_t i = threadIdx.x; i < s.n; i += blockDim.x) p[i] = s.v;
+    } else {
+      uint32_t* p = (uint32_t*)s.p;
+      for (uint64_t i = threadIdx.x; i < s.n; i += blockDim.x) p[i] = (uint32_t)s.v;
+    }
+  }
+}
+
+__global__ void k_gather_words(const uint32_t* const* __restrict__ src, int n, uint32_t* __restrict__ dst) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) dst[i] = *src[i];
+}
+
+// A launch group: one kernel variant and LDS size over several jobs.
+struct Bin {
+  bool dense = true, col64 = false;
+  size_t lds = 0;
+  std::vector<int> jobs;             // batch job indices
+  PassParams* d_params = nullptr;    // device, one per job of the bin
+  uint2* d_queue = nullptr;          // main work
+  uint32_t nq = 0;
+  uint2* d_mqueue = nullptr;         // merge work
+  uint32_t nmq = 0;
+  uint32_t* d_head = nullptr;        // two counters: main, merge
+  unsigned grid = 0, mgrid = 0;      // device-filling grids (main, merge)
+};
+
+// LDS footprint classes: a bin's launch opens the class size, so jobs of a
+// bin differ in footprint by at most the class step.
+static size_t lds_class(size_t b) {
+  static const size_t cls[] = {16384, 24576, 32768, 49152, 65536, 98304, 131072, 163840};
+  for (size_t c : cls)
+    if (b <= c) return c;
+  return b;
+}
+
+}  // namespace ptc
+
+using namespace ptc;
+
+struct ptc_batch {
+  ptc_ctx* ctx = nullptr;
+  int n = 0;
+  ptc_options opt{};
+  std::vector<const ptc_dtensor*> A, B;
+  std::vector<ptc_dtensor*> Ac;
+  std::vector<ptc_hashtable*> HA, HB;
+  std::vector<std::unique_ptr<Run>> runs;
+  std::vector<HashPlan> hn;
+  std::vector<ptc_dtensor*> out;
+  std::vector<Bin> bins;
+  Workspace* ws = nullptr;           // batch-level device arrays
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipEvent_t ev[8] = {};
+  uint32_t** d_ovf_src = nullptr;
+  uint32_t* d_ovf = nullptr;
+  std::vector<std::vector<unsigned char>> host_keep;  // sources of asynchronous uploads
+  ptc_batch_stats st{};
+};
+
+namespace {
+
+// Device copy of v; the host bytes are kept by the batch until it is freed,
+// so the asynchronous copy never reads freed memory.
+template <class T>
+T* upload_vec(ptc_batch* b, const std::vector<T>& v) {
+  if (v.empty()) return nullptr;
+  T* d = b->ws->get<T>(v.size());
+  if (!d) return nullptr;
+  b->host_keep.emplace_back((const unsigned char*)v.data(), (const unsigned char*)v.data() + sizeof(T) * v.size());
+  if (hipMemcpyAsync(d, b->host_keep.back().data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, b->ctx->stream) !=
+      hipSuccess)
+    return nullptr;
+  return d;
+}
+
+// Host round trip of one 64-bit word per job (dst[j] = *src[j] on the device).
+int read_words64(ptc_batch* b, const std::vector<const unsigned long long*>& src, std::vector<unsigned long long>& out) {
+  ptc_ctx* ctx = b->ctx;
+  const int n = (int)src.size();
+  out.assign(n, 0);
+  if (!n) return PTC_OK;
+  unsigned long long* dst = b->ws->get<unsigned long long>(n);
+  if (!dst) return PTC_ERR_NOMEM;
+  for (int j = 0; j < n; ++j)
+    if (src[j]) PTC_HIP(ctx, hipMemcpyAsync(dst + j, src[j], 8, hipMemcpyDeviceToDevice, ctx->stream));
+  int rc = read_back(ctx, out.data(), dst, sizeof(unsigned long long) * n);
+  for (int j = 0; j < n; ++j)
+    if (!src[j]) out[j] = 0;
+  return rc;
+}
+
+void stats_mark(ptc_batch* b, int i) { hipEventRecord(b->ev[i], b->ctx->stream); }
+double stats_ms(ptc_batch* b, int i, int j) {
+  float t = 0.f;
+  if (hipEventElapsedTime(&t, b->ev[i], b->ev[j]) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0.0;
+  }
+  return t;
+}
+
+// Bins of the jobs for one pass and their queues; params[j] must be filled.
+int make_bins(ptc_batch* b, int pass, const std::vector<PassParams>& params, const std::vector<HashPlan*>& hp,
+              std::vector<Bin>& bins) {
+  std::map<std::tuple<bool, bool, size_t>, int> key;
+  bins.clear();
+  for (int j = 0; j < b->n; ++j) {
+    Run& R = *b->runs[j];
+    const size_t need = pass_lds_bytes(pass, R.dense, R.col64, R.ncols, R.lds_slots, R.use_flags);
+    const size_t lds = std::max(need, std::min<size_t>(lds_class(need), lds_limit(b->ctx)));
+    auto k = std::make_tuple(R.dense, R.col64, lds);
+    auto it = key.find(k);
+    if (it == key.end()) {
+      it = key.emplace(k, (int)bins.size()).first;
+      Bin nb;
+      nb.dense = R.dense;
+      nb.col64 = R.col64;
+      nb.lds = lds;
+      bins.push_back(nb);
+    }
+    bins[it->second].jobs.push_back(j);
+  }
+  for (Bin& bn : bins) {
+    // heaviest work items first: jobs by mean flops per item
+    std::stable_sort(bn.jobs.begin(), bn.jobs.end(), [&](int x, int y) {
+      const Run &a = *b->runs[x], &c = *b->runs[y];
+      return (double)a.F / std::max<uint64_t>(a.nitems, 1) > (double)c.F / std::max<uint64_t>(c.nitems, 1);
+    });
+    std::vector<PassParams> bp;
+    std::vector<uint2> q, mq;
+    for (size_t s = 0; s < bn.jobs.size(); ++s) {
+      const int j = bn.jobs[s];
+      Run& R = *b->runs[j];
+      bp.push_back(params[j]);
+      for (uint64_t i = 0; i < R.nitems; ++i) q.push_back(make_uint2((uint32_t)s, (uint32_t)i));
+      if (bn.dense) {
+        for (uint64_t h = 0; h < R.nheavy; ++h) mq.push_back(make_uint2((uint32_t)s, (uint32_t)h));
+      } else if (pass == PASS_NUMERIC && hp[j] && hp[j]->pool) {
+        for (uint64_t r = 0; r < R.nrows; ++r) mq.push_back(make_uint2((uint32_t)s, (uint32_t)r));
+      }
+    }
+    if (q.size() > 0xFFFFFFFFull || mq.size() > 0xFFFFFFFFull) return set_err(b->ctx, PTC_ERR_RANGE, "batch queue too long");
+    bn.nq = (uint32_t)q.size();
+    bn.nmq = (uint32_t)mq.size();
+    bn.d_params = upload_vec(b, bp);
+    bn.d_queue = upload_vec(b, q);
+    bn.d_mqueue = upload_vec(b, mq);
+    bn.d_head = b->ws->get<uint32_t>(2);
+    if (!bn.d_params || (bn.nq && !bn.d_queue) || (bn.nmq && !bn.d_mqueue) || !bn.d_head) return PTC_ERR_NOMEM;
+    int rc = pass_batch_grid(b->ctx, pass, bn.dense, bn.col64, 0, false, bn.lds, &bn.grid);
+    if (!rc) rc = pass_batch_grid(b->ctx, pass, bn.dense, bn.col64, 0, true, bn.lds, &bn.mgrid);
+    if (rc) return rc;
+  }
+  return PTC_OK;
+}
+
+int launch_bins(ptc_batch* b, int pass, std::vector<Bin>& bins, std::vector<FillSeg>* fills) {
+  ptc_ctx* ctx = b->ctx;
+  int rc = PTC_OK;
+  for (Bin& bn : bins) {
+    if ((rc = launch_pass_batch(ctx, pass, bn.dense, bn.col64, 0, false, bn.d_params, bn.d_queue, bn.nq, bn.d_head,
+                                bn.lds, bn.grid)))
+      return rc;
+    if ((rc = launch_pass_batch(ctx, pass, bn.dense, bn.col64, 0, true, bn.d_params, bn.d_mqueue, bn.nmq,
+                                bn.d_head + 1, bn.lds, bn.mgrid)))
+      return rc;
+  }
+  (void)fills;
+  return PTC_OK;
+}
+
+// Zeroes the bins' queue heads (before a pass).
+int reset_heads(ptc_batch* b, std::vector<Bin>& bins) {
+  std::vector<FillSeg> f;
+  for (Bin& bn : bins) f.push_back(FillSeg{bn.d_head, 2, 0ull, 0});
+  if (f.empty()) return PTC_OK;
+  FillSeg* d = upload_vec(b, f);
+  if (!d) return PTC_ERR_NOMEM;
+  hipLaunchKernelGGL(k_fill_batch, dim3((unsigned)f.size()), dim3(256), 0, b->ctx->stream, (const FillSeg*)d,
+                     (int)f.size());
+  PTC_HIP(b->ctx, hipGetLastError());
+  return PTC_OK;
+}
+
+int plan(ptc_batch* b) {
+  ptc_ctx* ctx = b->ctx;
+  const int n = b->n;
+  int rc = PTC_OK;
+  stats_mark(b, 0);
+  // ---- key tables: B then A_copy of every job (main.cpp:115-116), no waits
+  for (int j = 0; j < n; ++j) {
+    const ptc_dtensor* A = b->A[j];
+    const uint32_t c = A->nk;
+    if (c == 0 || b->B[j]->nk != c)
+      return set_err(ctx, PTC_ERR_ARG, "job %d: A and B must have the same number (>0) of contracted modes", j);
+    uint32_t rot[PTC_MAX_MODES];
+    for (uint32_t i = c; i < A->d; ++i) rot[i - c] = A->dim_tc[i];
+    for (uint32_t i = 0; i < c; ++i) rot[A->d - c + i] = A->dim_tc[i];
+    b->Ac[j] = view_with_keys(A, A->d - c, rot);
+    if ((rc = ptc_hashtable_build(ctx, b->B[j], &b->HB[j]))) return rc;
+    if ((rc = ptc_hashtable_build(ctx, b->Ac[j], &b->HA[j]))) return rc;
+  }
+  stats_mark(b, 1);
+  // ---- preprocessing: one read-back for all jobs
+  unsigned long long* tot = b->ws->get<unsigned long long>(4 * (size_t)n);
+  if (!tot) return PTC_ERR_NOMEM;
+  for (int j = 0; j < n; ++j) {
+    b->runs[j].reset(new Run(ctx));
+    if ((rc = prepare_launch(*b->runs[j], b->HA[j], b->HB[j], b->Ac[j], b->B[j], b->opt, tot + 4 * j)))
+      return set_err(ctx, rc, "job %d: %s", j, ctx->err.c_str());
+  }
+  std::vector<unsigned long long> hv(4 * (size_t)n);
+  if ((rc = read_back(ctx, hv.data(), tot, sizeof(unsigned long long) * 4 * n))) return rc;
+  for (int j = 0; j < n; ++j) {
+    Run& R = *b->runs[j];
+    R.use_flags = true;  // replays may bring any values
+    if ((rc = prepare_finish(R, &hv[4 * (size_t)j]))) return set_err(ctx, rc, "job %d: %s", j, ctx->err.c_str());
+    b->st.flops += R.F;
+    b->st.queue_items += (uint32_t)R.nitems;
+  }
+  // ---- symbolic: HBM pools of hash jobs in one read-back, then one
+  // persistent launch per bin
+  std::vector<HashPlan> hs(n);
+  std::vector<const unsigned long long*> src(n, nullptr);
+  for (int j = 0; j < n; ++j) {
+    Run& R = *b->runs[j];
+    if (!R.dense) {
+      if ((rc = plan_hash_launch(R, hs[j], nullptr, R.row_flops, nullptr))) return rc;
+      src[j] = hs[j].gcap + R.nrows;
+    }
+  }
+  std::vector<unsigned long long> pools;
+  if ((rc = read_words64(b, src, pools))) return rc;
+  std::vector<PassParams> params(n);
+  std::vector<HashPlan*> hptr(n, nullptr);
+  for (int j = 0; j < n; ++j) {
+    Run& R = *b->runs[j];
+    if (!R.dense) {
+      if ((rc = plan_hash_finish(R, hs[j], pools[j])) || (rc = reset_hash_pool(R, hs[j], false))) return rc;
+      hptr[j] = &hs[j];
+    }
+    if ((rc = fill_zero(ctx, R.row_nnz, (R.nrows + 1) * 4)) || (rc = prepare_heavy(R, PASS_SYMBOLIC))) return rc;
+    base_params(R, params[j], R.dense ? nullptr : &hs[j]);
+    if (!R.dense) {
+      unsigned long long* flags = nullptr;
+      if ((rc = hbm_flags(R, hs[j], &flags))) return rc;
+      params[j].row_gcap = flags;
+    }
+    params[j].merge_rows = R.dense ? R.heavy_rows : nullptr;
+  }
+  std::vector<Bin> sbins;
+  if ((rc = make_bins(b, PASS_SYMBOLIC, params, hptr, sbins)) || (rc = reset_heads(b, sbins))) return rc;
+  if ((rc = launch_bins(b, PASS_SYMBOLIC, sbins, nullptr))) return rc;
+  stats_mark(b, 2);
+  // ---- output sizes and the symbolic pass's consistency flags: one read-back
+  std::vector<const unsigned long long*> fsrc(n);
+  for (int j = 0; j < n; ++j) {
+    Run& R = *b->runs[j];
+    if ((rc = scan_u32_to_u64(ctx, R.row_nnz, R.out_off, R.nrows, R.out_off + R.nrows))) return rc;
+    src[j] = R.out_off + R.nrows;
+    fsrc[j] = (const unsigned long long*)R.overflow;  // the first two flag words
+  }
+  std::vector<unsigned long long> nnz, sflags;
+  if ((rc = read_words64(b, src, nnz)) || (rc = read_words64(b, fsrc, sflags))) return rc;
+  for (int j = 0; j < n; ++j)
+    if ((uint32_t)sflags[j] & 0xC0000000u)
+      return set_err(ctx, PTC_ERR_INTERNAL, "job %d: symbolic pass consistency flags %08x", j, (uint32_t)sflags[j]);
+  for (int j = 0; j < n; ++j) {
+    if ((rc = make_output(*b->runs[j], nnz[j], b->Ac[j], b->B[j], &b->out[j]))) return rc;
+    b->st.nnz_out += nnz[j];
+  }
+  // ---- numeric plan: HBM pools from the exact counts, one read-back
+  for (int j = 0; j < n; ++j) {
+    Run& R = *b->runs[j];
+    src[j] = nullptr;
+    if (!R.dense) {
+      if ((rc = plan_hash_launch(R, b->hn[j], R.row_nnz, nullptr, nullptr))) return rc;
+      src[j] = b->hn[j].gcap + R.nrows;
+    }
+  }
+  if ((rc = read_words64(b, src, pools))) return rc;
+  std::vector<RefreshSeg> rseg;
+  std::vector<unsigned long long> rstart;
+  unsigned long long rtotal = 0;
+  std::vector<FillSeg> fills;
+  for (int j = 0; j < n; ++j) {
+    Run& R = *b->runs[j];
+    hptr[j] = nullptr;
+    if (!R.dense) {
+      if ((rc = plan_hash_finish(R, b->hn[j], pools[j]))) return rc;
+      hptr[j] = &b->hn[j];
+      if (b->hn[j].pool) {
+        fills.push_back(FillSeg{b->hn[j].gkeys, b->hn[j].pool, R.col64 ? ~0ull : 0xFFFFFFFFull, R.col64 ? 1 : 0});
+        fills.push_back(FillSeg{b->hn[j].gvals, b->hn[j].pool, 0ull, 1});
+      }
+    }
+    if ((rc = prepare_heavy(R, PASS_NUMERIC))) return rc;
+    base_params(R, params[j], R.dense ? nullptr : &b->hn[j]);
+    if (!R.dense) {
+      unsigned long long* flags = nullptr;
+      if ((rc = hbm_flags(R, b->hn[j], &flags))) return rc;
+      params[j].row_gcap = flags;
+    }
+    params[j].merge_rows = R.dense ? R.heavy_rows : nullptr;
+    params[j].out_base = R.out_off;
+    params[j].out_hedges = b->out[j]->hedges;
+    params[j].out_vals = b->out[j]->vals;
+    fills.push_back(FillSeg{R.overflow, 1, 0ull, 0});
+    // grouped values of both tables, re-read from the input tensors
+    const ptc_hashtable* tabs[2] = {b->HA[j], b->HB[j]};
+    for (const ptc_hashtable* t : tabs) {
+      rseg.push_back(RefreshSeg{t->perm, t->t->vals, t->gval, t->t->N});
+      rstart.push_back(rtotal);
+      rtotal += t->t->N;
+    }
+  }
+  if ((rc = make_bins(b, PASS_NUMERIC, params, hptr, b->bins))) return rc;
+  for (Bin& bn : b->bins) fills.push_back(FillSeg{bn.d_head, 2, 0ull, 0});
+  RefreshSeg* d_rseg = upload_vec(b, rseg);
+  unsigned long long* d_rstart = upload_vec(b, rstart);
+  FillSeg* d_fill = upload_vec(b, fills);
+  std::vector<const uint32_t*> ovf_src(n);
+  for (int j = 0; j < n; ++j) ovf_src[j] = b->runs[j]->overflow;
+  b->d_ovf_src = (uint32_t**)upload_vec(b, ovf_src);
+  b->d_ovf = b->ws->get<uint32_t>(n);
+  if (!d_rseg || !d_rstart || !d_fill || !b->d_ovf_src || !b->d_ovf) return PTC_ERR_NOMEM;
+  b->st.bins = (uint32_t)b->bins.size();
+  // ---- capture the numeric phase (no allocation, read-back or memset inside)
+  PTC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  stats_mark(b, 3);
+  PTC_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  const unsigned rgrid = (unsigned)std::min<unsigned long long>((rtotal + 255) / 256, 16384);
+  hipLaunchKernelGGL(k_refresh_batch, dim3(std::max(1u, rgrid)), dim3(256), 0, ctx->stream, (const RefreshSeg*)d_rseg,
+                     (const unsigned long long*)d_rstart, (int)rseg.size(), rtotal);
+  hipLaunchKernelGGL(k_fill_batch, dim3((unsigned)std::min<size_t>(fills.size(), 4096)), dim3(256), 0, ctx->stream,
+                     (const FillSeg*)d_fill, (int)fills.size());
+  rc = launch_bins(b, PASS_NUMERIC, b->bins, nullptr);
+  hipError_t ce = hipStreamEndCapture(ctx->stream, &b->graph);
+  if (rc) return rc;
+  if (ce != hipSuccess) return hip_check(ctx, ce, "graph capture");
+  PTC_HIP(ctx, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  size_t nodes = 0;
+  if (hipGraphGetNodes(b->graph, nullptr, &nodes) == hipSuccess) b->st.graph_nodes = nodes;
+  b->st.ms_build = stats_ms(b, 0, 1);
+  b->st.ms_symbolic = stats_ms(b, 1, 2);
+  return PTC_OK;
+}
+
+}  // namespace
+
+extern "C" int ptc_batch_plan(ptc_ctx* ctx, int n, const ptc_dtensor* const* A, const ptc_dtensor* const* B,
+                              const ptc_options* opt, ptc_batch** out, ptc_batch_stats* st) {
+  if (!ctx || n <= 0 || !A || !B || !out) return PTC_ERR_ARG;
+  *out = nullptr;
+  for (int j = 0; j < n; ++j)
+    if (!A[j] || !B[j]) return set_err(ctx, PTC_ERR_ARG, "job %d: NULL tensor", j);
+  PTC_HIP(ctx, hipSetDevice(ctx->device));
+  auto w0 = std::chrono::steady_clock::now();
+  ptc_batch* b = new ptc_batch();
+  b->ctx = ctx;
+  b->n = n;
+  if (opt) b->opt = *opt; else ptc_options_default(&b->opt);
+  b->opt.method = PTC_METHOD_TWO_PASS;
+  if (b->opt.value_type != PTC_VAL_F64) {
+    delete b;
+    return set_err(ctx, PTC_ERR_UNSUPPORTED, "batched contractions compute in FP64");
+  }
+  b->A.assign(A, A + n);
+  b->B.assign(B, B + n);
+  b->Ac.assign(n, nullptr);
+  b->HA.assign(n, nullptr);
+  b->HB.assign(n, nullptr);
+  b->runs.resize(n);
+  b->hn.resize(n);
+  b->out.assign(n, nullptr);
+  b->ws = new Workspace(ctx);
+  b->st.jobs = (uint32_t)n;
+  for (auto& e : b->ev) hipEventCreate(&e);
+  int rc = plan(b);
+  if (rc) {
+    (void)hipGetLastError();
+    ptc_batch_free(b);
+    return rc;
+  }
+  b->st.ms_plan = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
+  if (st) *st = b->st;
+  *out = b;
+  return PTC_OK;
+}
+
+extern "C" int ptc_batch_run(ptc_batch* b, ptc_batch_stats* st) {
+  if (!b || !b->exec) return PTC_ERR_ARG;
+  ptc_ctx* ctx = b->ctx;
+  PTC_HIP(ctx, hipSetDevice(ctx->device));
+  stats_mark(b, 4);
+  PTC_HIP(ctx, hipGraphLaunch(b->exec, ctx->stream));
+  stats_mark(b, 5);
+  // consistency flags of every job, one read-back
+  hipLaunchKernelGGL(k_gather_words, dim3((unsigned)std::min(1024, (b->n + 255) / 256)), dim3(256), 0, ctx->stream,
+                     (const uint32_t* const*)b->d_ovf_src, b->n, b->d_ovf);
+  std::vector<uint32_t> ovf(b->n);
+  int rc = read_back(ctx, ovf.data(), b->d_ovf, sizeof(uint32_t) * b->n);
+  if (rc) return rc;
+  for (int j = 0; j < b->n; ++j) {
+    if (ovf[j] & 0x80000000u) return set_err(ctx, PTC_ERR_INTERNAL, "job %d: row count mismatch between passes", j);
+    if (ovf[j] & 0x40000000u) return set_err(ctx, PTC_ERR_INTERNAL, "job %d: slice index out of range", j);
+  }
+  b->st.ms_numeric = stats_ms(b, 4, 5);
+  b->st.runs++;
+  if (st) *st = b->st;
+  return PTC_OK;
+}
+
+extern "C" int ptc_batch_output(const ptc_batch* b, int j, const ptc_dtensor** O) {
+  if (!b || !O || j < 0 || j >= b->n) return PTC_ERR_ARG;
+  *O = b->out[j];
+  return PTC_OK;
+}
+
+extern "C" void ptc_batch_free(ptc_batch* b) {
+  if (!b) return;
+  if (b->ctx) hipStreamSynchronize(b->ctx->stream);
+  if (b->exec) hipGraphExecDestroy(b->exec);
+  if (b->graph) hipGraphDestroy(b->graph);
+  for (auto* o : b->out) ptc_dtensor_free(o);
+  b->runs.clear();
+  for (auto* h : b->HA) ptc_hashtable_free(h);
+  for (auto* h : b->HB) ptc_hashtable_free(h);
+  for (auto* v : b->Ac) delete v;
+  delete b->ws;
+  for (auto& e : b->ev)
+    if (e) hipEventDestroy(e);
+  delete b;
+}
+
+extern "C" int ptc_contract_batched(ptc_ctx* ctx, int n, const ptc_dtensor* const* A, const ptc_dtensor* const* B,
+                                    const ptc_options* opt, ptc_dtensor** O, ptc_batch_stats* st) {
+  if (!O) return PTC_ERR_ARG;
+  auto w0 = std::chrono::steady_clock::now();
+  ptc_batch* b = nullptr;
+  int rc = ptc_batch_plan(ctx, n, A, B, opt, &b, nullptr);
+  if (rc) return rc;
+  rc = ptc_batch_run(b, nullptr);
+  if (!rc) {
+    for (int j = 0; j < n; ++j) {
+      O[j] = b->out[j];  // ownership passes to the caller
+      b->out[j] = nullptr;
+    }
+    b->st.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
+    if (st) *st = b->st;
+  }
+  ptc_batch_free(b);
+  return rc;
+}

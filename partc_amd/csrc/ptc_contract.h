@@ -158,6 +158,14 @@ size_t pass_lds_bytes(int pass, bool dense, bool col64, uint64_t ncols, uint32_t
 int launch_pass(ptc_ctx* ctx, int pass, bool dense, bool col64, const PassParams& P, uint64_t nitems,
                 size_t lds, bool merge = false);
 int launch_emit(ptc_ctx* ctx, const EmitParams& P, bool col64);
+// Persistent row pass over the queued work of several contractions (jobs:
+// device array of their PassParams; queue: (job, work index) pairs; head: a
+// zeroed device counter).  grid 0 = as many workgroups as fit on the device.
+int launch_pass_batch(ptc_ctx* ctx, int pass, bool dense, bool col64, int val, bool merge, const PassParams* jobs,
+                      const uint2* queue, uint32_t nq, uint32_t* head, size_t lds, unsigned grid = 0);
+// The same kernel's device-filling grid at this LDS size (opens the LDS
+// attribute too: call before a graph capture, then launch with the grid).
+int pass_batch_grid(ptc_ctx* ctx, int pass, bool dense, bool col64, int val, bool merge, size_t lds, unsigned* grid);
 int launch_overflow_rows(ptc_ctx* ctx, const uint32_t* nnz, const uint32_t* cap, uint64_t nrows,
                          uint32_t* list, uint32_t* n, uint32_t* mark);
 int launch_select_items(ptc_ctx* ctx, const uint4* items, uint64_t nitems, const uint32_t* nnz,

@@ -440,8 +440,10 @@ __device__ __forceinline__ void f32_finish(float s, float c, float* hi, float* l
 // VAL (ptc_value_type): 0 FP64 values; 1 FP32 values, FP32 accumulators
 // with TwoProduct + TwoSum compensation; 2 FP32 values, plain FP32
 // accumulators; 3 FP32 values, exact products accumulated in FP64.
+// One work item of a row pass: widx indexes P.items (MERGE: P.merge_rows,
+// or the row itself).  Every early return is uniform over the workgroup.
 template <int PASS, bool DENSE, class ColT, bool MERGE, int VAL>
-__global__ void __launch_bounds__(PB) k_pass(PassParams P) {
+__device__ __forceinline__ void pass_item(const PassParams& P, uint32_t widx) {
   constexpr bool F32 = VAL == 1 || VAL == 2;  // FP32 (sum, compensation) accumulators
   constexpr bool F32IN = VAL != 0;            // FP32 values in, (hi, lo) FP32 pairs out
   constexpr bool COMP = VAL == 1;
@@ -461,9 +463,9 @@ __global__ void __launch_bounds__(PB) k_pass(PassParams P) {
   const ColT* __restrict__ bcol = (const ColT*)P.bcol;
   uint4 it = make_uint4(0, 0, 0, 0);
   if (MERGE) {
-    it.x = P.merge_rows ? P.merge_rows[blockIdx.x] : blockIdx.x;
+    it.x = P.merge_rows ? P.merge_rows[widx] : widx;
   } else {
-    it = P.items[blockIdx.x];
+    it = P.items[widx];
   }
   const uint32_t r = it.x, k = it.y;
   const KtSlice row = P.rows[r];
@@ -929,6 +931,30 @@ __global__ void __launch_bounds__(PB) k_pass(PassParams P) {
   }
 }
 
+template <int PASS, bool DENSE, class ColT, bool MERGE, int VAL>
+__global__ void __launch_bounds__(PB) k_pass(PassParams P) {
+  pass_item<PASS, DENSE, ColT, MERGE, VAL>(P, blockIdx.x);
+}
+
+// Persistent row pass over the work of many contractions: each workgroup
+// takes (job, work index) pairs from the queue until it is exhausted.  The
+// queue lists the heaviest work first, so the tail of the launch is short
+// items whatever the mix of job sizes.
+template <int PASS, bool DENSE, class ColT, bool MERGE, int VAL>
+__global__ void __launch_bounds__(PB) k_pass_batch(const PassParams* __restrict__ jobs, const uint2* __restrict__ queue,
+                                                   uint32_t nq, uint32_t* __restrict__ head) {
+  __shared__ uint32_t s_q;
+  for (;;) {
+    __syncthreads();  // the previous item is done with LDS
+    if (threadIdx.x == 0) s_q = atomicAdd(head, 1u);
+    __syncthreads();
+    const uint32_t q = s_q;
+    if (q >= nq) return;
+    const uint2 w = queue[q];
+    pass_item<PASS, DENSE, ColT, MERGE, VAL>(jobs[w.x], w.y);
+  }
+}
+
 // learned / upper-bound: staged rows -> final output
 template <class ColT, bool F32>
 __global__ void __launch_bounds__(PB) k_emit(EmitParams P) {
@@ -1073,6 +1099,58 @@ int launch_pass(ptc_ctx* ctx, int pass, bool dense, bool col64, const PassParams
   if (nitems > 0x7FFFFFFFull) return set_err(ctx, PTC_ERR_RANGE, "too many work items");
   return merge ? launch_pass_m<true>(ctx, pass, dense, col64, P, nitems, lds)
                : launch_pass_m<false>(ctx, pass, dense, col64, P, nitems, lds);
+}
+
+// head == nullptr: only open the LDS attribute and report the grid.
+template <int PASS, bool DENSE, class ColT, bool MERGE, int VAL>
+static int launch_batch_t(ptc_ctx* ctx, const PassParams* jobs, const uint2* queue, uint32_t nq, uint32_t* head,
+                          size_t lds, unsigned grid, unsigned* grid_out = nullptr) {
+  auto kern = k_pass_batch<PASS, DENSE, ColT, MERGE, VAL>;
+  static size_t lds_set = 0;
+  if (lds > lds_set) {
+    PTC_HIP(ctx, hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    lds_set = lds;
+  }
+  if (!grid) {
+    int per_cu = 0;
+    PTC_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, PB, lds));
+    grid = (unsigned)std::max(1, per_cu) * (unsigned)std::max(1, ctx->prop.multiProcessorCount);
+  }
+  if (grid_out) *grid_out = grid;
+  if (!head) return PTC_OK;
+  grid = std::min<unsigned>(grid, nq);
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(PB), lds, ctx->stream, jobs, queue, nq, head);
+  PTC_HIP(ctx, hipGetLastError());
+  return PTC_OK;
+}
+
+template <bool MERGE, int VAL>
+static int launch_batch_v(ptc_ctx* ctx, int pass, bool dense, bool col64, const PassParams* jobs, const uint2* queue,
+                          uint32_t nq, uint32_t* head, size_t lds, unsigned grid, unsigned* gout) {
+#define PTC_LB(P_, D_, C_, V_) launch_batch_t<P_, D_, C_, MERGE, V_>(ctx, jobs, queue, nq, head, lds, grid, gout)
+  if (pass == PASS_SYMBOLIC) {
+    if (dense) return col64 ? PTC_LB(PASS_SYMBOLIC, true, unsigned long long, 0) : PTC_LB(PASS_SYMBOLIC, true, uint32_t, 0);
+    return col64 ? PTC_LB(PASS_SYMBOLIC, false, unsigned long long, 0) : PTC_LB(PASS_SYMBOLIC, false, uint32_t, 0);
+  }
+  if (dense) return col64 ? PTC_LB(PASS_NUMERIC, true, unsigned long long, VAL) : PTC_LB(PASS_NUMERIC, true, uint32_t, VAL);
+  return col64 ? PTC_LB(PASS_NUMERIC, false, unsigned long long, VAL) : PTC_LB(PASS_NUMERIC, false, uint32_t, VAL);
+#undef PTC_LB
+}
+
+int launch_pass_batch(ptc_ctx* ctx, int pass, bool dense, bool col64, int val, bool merge, const PassParams* jobs,
+                      const uint2* queue, uint32_t nq, uint32_t* head, size_t lds, unsigned grid) {
+  if (!nq) return PTC_OK;
+  if (pass == PASS_SYMBOLIC) val = 0;
+  if (val != 0) return set_err(ctx, PTC_ERR_UNSUPPORTED, "batched contractions compute in FP64");
+  return merge ? launch_batch_v<true, 0>(ctx, pass, dense, col64, jobs, queue, nq, head, lds, grid, nullptr)
+               : launch_batch_v<false, 0>(ctx, pass, dense, col64, jobs, queue, nq, head, lds, grid, nullptr);
+}
+
+int pass_batch_grid(ptc_ctx* ctx, int pass, bool dense, bool col64, int val, bool merge, size_t lds, unsigned* grid) {
+  if (pass == PASS_SYMBOLIC) val = 0;
+  if (val != 0) return set_err(ctx, PTC_ERR_UNSUPPORTED, "batched contractions compute in FP64");
+  return merge ? launch_batch_v<true, 0>(ctx, pass, dense, col64, nullptr, nullptr, 0, nullptr, lds, 0, grid)
+               : launch_batch_v<false, 0>(ctx, pass, dense, col64, nullptr, nullptr, 0, nullptr, lds, 0, grid);
 }
 
 int launch_emit(ptc_ctx* ctx, const EmitParams& P, bool col64) {

@@ -525,6 +525,16 @@ extern "C" int ptc_dtensor_download_f32(ptc_ctx* ctx, const ptc_dtensor* t, floa
   return PTC_OK;
 }
 
+extern "C" int ptc_dtensor_set_values(ptc_ctx* ctx, ptc_dtensor* t, const double* vals) {
+  if (!ctx || !t || (t->N && !vals)) return PTC_ERR_ARG;
+  if (!t->vals) return set_err(ctx, PTC_ERR_ARG, "tensor has no FP64 values");
+  if (t->N) {
+    PTC_HIP(ctx, hipMemcpyAsync(t->vals, vals, sizeof(double) * t->N, hipMemcpyHostToDevice, ctx->stream));
+    PTC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  }
+  return PTC_OK;
+}
+
 extern "C" void ptc_dtensor_free(ptc_dtensor* t) {
   if (!t) return;
   if (t->owns && t->ctx) {

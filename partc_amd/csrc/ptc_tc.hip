@@ -18,12 +18,10 @@
 #include "ptc_contract.h"
 #include "ptc_internal.h"
 #include "ptc_learned.h"
+#include "ptc_run.h"
 
 namespace ptc {
 
-static constexpr uint32_t DEFAULT_DENSE_MAX = 8192;
-static constexpr uint32_t DEFAULT_LDS_HASH = 2048;
-static constexpr uint64_t DEFAULT_CHUNK = 32768;
 static constexpr unsigned long long STAGING_BOUND = 1ull << 28;  // entries (4 GB of column keys + FP64 values)
 
 struct Timer {
@@ -44,48 +42,6 @@ struct Timer {
   }
 };
 
-// Hash accumulator plan of every row: LDS slots or an HBM region.
-struct HashPlan {
-  uint32_t* hcap = nullptr;
-  unsigned long long* gcap = nullptr;  // scanned in place into offsets
-  unsigned long long pool = 0;         // total HBM slots
-  void* gkeys = nullptr;
-  double* gvals = nullptr;
-};
-
-struct Run {
-  ptc_ctx* ctx;
-  Workspace ws;
-  const ptc_hashtable* HA;
-  const ptc_hashtable* HB;
-  ptc_options opt;
-  bool dense = true, col64 = false;
-  uint64_t npos = 0, nrows = 0, ncols = 0;
-  uint32_t dO = 0, nrowmodes = 0, lds_slots = DEFAULT_LDS_HASH;
-  // preprocessing
-  uint32_t *pos_len = nullptr, *pos_off = nullptr;
-  unsigned long long* apre = nullptr;
-  uint32_t *row_hits = nullptr, *row_maxlen = nullptr, *row_nchunk = nullptr, *row_heavy = nullptr;
-  uint32_t *row_done = nullptr, *row_nnz = nullptr;
-  unsigned long long *row_flops = nullptr, *packed = nullptr;
-  uint4* items = nullptr;
-  uint64_t F = 0, nitems = 0, nheavy = 0;
-  // heavy dense merge buffers
-  uint32_t* hbits = nullptr;
-  double* hacc = nullptr;
-  unsigned char* hflag = nullptr;
-  unsigned long long* row_slab = nullptr;  // first partial-row slab of each split row
-  uint32_t* heavy_rows = nullptr;          // split rows (heavy index -> row)
-  uint64_t nslabs = 0;
-  bool use_flags = true;
-  uint32_t* overflow = nullptr;
-  unsigned long long* out_off = nullptr;
-  bool want_features = false;  // per-row hit counts and longest slices (learned features)
-  bool f32 = false;            // FP32 values (both tables carry gval32)
-  int val = PTC_VAL_F64;       // ptc_value_type of the numeric pass
-  explicit Run(ptc_ctx* c) : ctx(c), ws(c) {}
-};
-
 __global__ void k_merge_hbm_caps(uint64_t n, uint32_t* hcap, unsigned long long* gcap, const uint32_t* qh) {
   for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (uint64_t)gridDim.x * blockDim.x)
     if (gcap[r]) {
@@ -95,7 +51,7 @@ __global__ void k_merge_hbm_caps(uint64_t n, uint32_t* hcap, unsigned long long*
 }
 
 // Dynamic LDS a workgroup may use on this device.
-static uint64_t lds_limit(ptc_ctx* ctx) {
+uint64_t lds_limit(ptc_ctx* ctx) {
   int optin = 0;
   if (hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, ctx->device) != hipSuccess || optin <= 0) {
     (void)hipGetLastError();
@@ -104,7 +60,7 @@ static uint64_t lds_limit(ptc_ctx* ctx) {
   return (uint64_t)optin > 4096 ? (uint64_t)optin - 1024 : 65536 - 1024;  // headroom for static LDS
 }
 
-static int plan_hash(Run& R, HashPlan& hp, const uint32_t* bound32, const unsigned long long* bound64,
+int plan_hash_launch(Run& R, HashPlan& hp, const uint32_t* bound32, const unsigned long long* bound64,
                      const unsigned long long* hbm_bound) {
   ptc_ctx* ctx = R.ctx;
   if (!hp.hcap) hp.hcap = R.ws.get<uint32_t>(R.nrows);
@@ -136,8 +92,11 @@ static int plan_hash(Run& R, HashPlan& hp, const uint32_t* bound32, const unsign
                        0, ctx->stream, R.nrows, hp.hcap, hp.gcap, (const uint32_t*)Q.row_hcap);
     PTC_HIP(ctx, hipGetLastError());
   }
-  if ((rc = scan_u64(ctx, hp.gcap, hp.gcap, R.nrows, hp.gcap + R.nrows))) return rc;
-  if ((rc = read_back(ctx, &hp.pool, hp.gcap + R.nrows, sizeof(hp.pool)))) return rc;
+  return scan_u64(ctx, hp.gcap, hp.gcap, R.nrows, hp.gcap + R.nrows);
+}
+
+int plan_hash_finish(Run& R, HashPlan& hp, unsigned long long pool) {
+  hp.pool = pool;
   if (hp.pool) {
     size_t kb = R.col64 ? 8 : 4;
     hp.gkeys = R.ws.get<unsigned char>(hp.pool * kb);
@@ -147,7 +106,16 @@ static int plan_hash(Run& R, HashPlan& hp, const uint32_t* bound32, const unsign
   return PTC_OK;
 }
 
-static int reset_hash_pool(Run& R, HashPlan& hp, bool values) {
+int plan_hash(Run& R, HashPlan& hp, const uint32_t* bound32, const unsigned long long* bound64,
+              const unsigned long long* hbm_bound) {
+  int rc = plan_hash_launch(R, hp, bound32, bound64, hbm_bound);
+  unsigned long long pool = 0;
+  if (!rc) rc = read_back(R.ctx, &pool, hp.gcap + R.nrows, sizeof(pool));
+  if (!rc) rc = plan_hash_finish(R, hp, pool);
+  return rc;
+}
+
+int reset_hash_pool(Run& R, HashPlan& hp, bool values) {
   if (!hp.pool) return PTC_OK;
   int rc = R.col64 ? fill_u64(R.ctx, (unsigned long long*)hp.gkeys, hp.pool, ~0ull)
                    : fill_u32(R.ctx, (uint32_t*)hp.gkeys, hp.pool, ~0u);
@@ -155,7 +123,7 @@ static int reset_hash_pool(Run& R, HashPlan& hp, bool values) {
   return rc;
 }
 
-static void base_params(Run& R, PassParams& P, const HashPlan* hp) {
+void base_params(Run& R, PassParams& P, const HashPlan* hp) {
   memset(&P, 0, sizeof(P));
   P.items = R.items;
   P.rows = R.HA->slices;
@@ -202,7 +170,7 @@ __global__ void k_hbm_flag(const unsigned long long* __restrict__ off, uint64_t 
     flag[r] = off[r + 1] - off[r];
 }
 
-static int hbm_flags(Run& R, HashPlan& hp, unsigned long long** flags) {
+int hbm_flags(Run& R, HashPlan& hp, unsigned long long** flags) {
   unsigned long long* f = R.ws.get<unsigned long long>(R.nrows);
   if (!f) return PTC_ERR_NOMEM;
   hipLaunchKernelGGL(k_hbm_flag, dim3((unsigned)std::min<uint64_t>((R.nrows + 255) / 256, 4096)), dim3(256), 0,
@@ -245,7 +213,7 @@ static int run_pass(Run& R, int pass, HashPlan* hp, const uint4* items, uint64_t
 // Slabs of split rows: one partial per chunk (a bitmap in the symbolic pass,
 // a dense accumulator in the numeric pass); every chunk writes its whole
 // slab, so nothing is cleared.
-static int prepare_heavy(Run& R, int pass) {
+int prepare_heavy(Run& R, int pass) {
   if (!R.dense || !R.nheavy) return PTC_OK;
   if (pass == PASS_SYMBOLIC) {
     const uint64_t nbits = (R.ncols + 31) / 32;
@@ -258,7 +226,7 @@ static int prepare_heavy(Run& R, int pass) {
   return PTC_OK;
 }
 
-static int make_output(Run& R, uint64_t nnz, const ptc_dtensor* A, const ptc_dtensor* B, ptc_dtensor** O) {
+int make_output(Run& R, uint64_t nnz, const ptc_dtensor* A, const ptc_dtensor* B, ptc_dtensor** O) {
   ptc_dtensor* o = new ptc_dtensor();
   o->ctx = R.ctx;
   o->d = R.dO;
@@ -299,10 +267,12 @@ extern "C" void ptc_options_default(ptc_options* o) {
   o->predict_scale = 1.15;
 }
 
+namespace ptc {
+
 // Validation, accumulator choice and the shared preprocessing: key probes,
 // flop prefix, per-row chunks, work items.
-static int prepare(Run& R, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B, const ptc_dtensor* A,
-                   const ptc_dtensor* B, const ptc_options& opt) {
+int prepare_launch(Run& R, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B, const ptc_dtensor* A,
+                   const ptc_dtensor* B, const ptc_options& opt, unsigned long long* tot) {
   ptc_ctx* ctx = R.ctx;
   if (HT_A->t != A || HT_B->t != B) return set_err(ctx, PTC_ERR_ARG, "tables were not built on these tensors");
   const uint32_t c = A->d - A->nk;
@@ -363,6 +333,7 @@ static int prepare(Run& R, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B,
   int rc = PTC_OK;
   const uint32_t* row_of_pos = nullptr;  // HT_A's slice index of every A position
   if ((rc = hashtable_slice_of(ctx, HT_A, &row_of_pos))) return rc;
+  R.row_of_pos = row_of_pos;
   Workspace& ws = R.ws;
   R.pos_len = ws.get<uint32_t>(R.npos);
   R.pos_off = ws.get<uint32_t>(R.npos);
@@ -423,17 +394,20 @@ static int prepare(Run& R, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B,
   }
   if ((rc = scan_u64(ctx, R.packed, R.packed, R.nrows, R.packed + R.nrows))) return rc;
   if ((rc = scan_u64(ctx, R.row_slab, R.row_slab, R.nrows, R.row_slab + R.nrows))) return rc;
-  unsigned long long hostv[4] = {0, 0, 0, 0};
-  {
-    // one host round trip for the totals and HT_A's key count
-    unsigned long long* tot = ws.get<unsigned long long>(4);
-    if (!tot) return PTC_ERR_NOMEM;
-    PTC_HIP(ctx, hipMemcpyAsync(tot, R.apre + R.npos, 8, hipMemcpyDeviceToDevice, ctx->stream));
-    PTC_HIP(ctx, hipMemcpyAsync(tot + 1, R.packed + R.nrows, 8, hipMemcpyDeviceToDevice, ctx->stream));
-    PTC_HIP(ctx, hipMemcpyAsync(tot + 2, R.row_slab + R.nrows, 8, hipMemcpyDeviceToDevice, ctx->stream));
-    PTC_HIP(ctx, hipMemcpyAsync(tot + 3, HT_A->d_total, 8, hipMemcpyDeviceToDevice, ctx->stream));
-    if ((rc = read_back(ctx, hostv, tot, 32))) return rc;
-  }
+  // the totals and HT_A's key count, for one host round trip
+  PTC_HIP(ctx, hipMemcpyAsync(tot, R.apre + R.npos, 8, hipMemcpyDeviceToDevice, ctx->stream));
+  PTC_HIP(ctx, hipMemcpyAsync(tot + 1, R.packed + R.nrows, 8, hipMemcpyDeviceToDevice, ctx->stream));
+  PTC_HIP(ctx, hipMemcpyAsync(tot + 2, R.row_slab + R.nrows, 8, hipMemcpyDeviceToDevice, ctx->stream));
+  PTC_HIP(ctx, hipMemcpyAsync(tot + 3, HT_A->d_total, 8, hipMemcpyDeviceToDevice, ctx->stream));
+  return PTC_OK;
+}
+
+int prepare_finish(Run& R, const unsigned long long* hostv) {
+  ptc_ctx* ctx = R.ctx;
+  const ptc_hashtable* HT_A = R.HA;
+  const ptc_dtensor* A = HT_A->t;
+  Workspace& ws = R.ws;
+  int rc = PTC_OK;
   if (!HT_A->keys_known) {
     if ((hostv[3] & 0xFFFFFFFFull) != A->N)
       return set_err(ctx, PTC_ERR_INTERNAL, "key table grouped %llu of %llu nonzeros",
@@ -462,7 +436,7 @@ static int prepare(Run& R, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B,
     P.items = R.items;
     ItemP0Params Q{};
     Q.npos = R.nheavy ? R.npos : 0;
-    Q.row_of_pos = row_of_pos;
+    Q.row_of_pos = R.row_of_pos;
     Q.row_nchunk = R.row_nchunk;
     Q.pos_len = R.pos_len;
     Q.rows = HT_A->slices;
@@ -474,6 +448,19 @@ static int prepare(Run& R, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B,
   }
   return PTC_OK;
 }
+
+int prepare(Run& R, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B, const ptc_dtensor* A,
+            const ptc_dtensor* B, const ptc_options& opt) {
+  unsigned long long* tot = R.ws.get<unsigned long long>(4);
+  if (!tot) return PTC_ERR_NOMEM;
+  int rc = prepare_launch(R, HT_A, HT_B, A, B, opt, tot);
+  unsigned long long hostv[4] = {0, 0, 0, 0};
+  if (!rc) rc = read_back(R.ctx, hostv, tot, 32);
+  if (!rc) rc = prepare_finish(R, hostv);
+  return rc;
+}
+
+}  // namespace ptc
 
 extern "C" int ptc_tc_fksCuckoo(ptc_ctx* ctx, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B,
                                 const ptc_dtensor* A, const ptc_dtensor* B, const ptc_options* opt_in,
@@ -730,13 +717,15 @@ extern "C" int ptc_tc_fksCuckoo(ptc_ctx* ctx, const ptc_hashtable* HT_A, const p
 }
 
 // A view of t with another key split, sharing t's device arrays.
-static ptc_dtensor* view_with_keys(const ptc_dtensor* t, uint32_t nk, const uint32_t* dim_tc) {
+namespace ptc {
+ptc_dtensor* view_with_keys(const ptc_dtensor* t, uint32_t nk, const uint32_t* dim_tc) {
   ptc_dtensor* v = new ptc_dtensor(*t);
   v->owns = false;
   v->nk = nk;
   for (uint32_t j = 0; j < t->d; ++j) v->dim_tc[j] = dim_tc[j];
   return v;
 }
+}  // namespace ptc
 
 extern "C" int ptc_contract(ptc_ctx* ctx, const ptc_dtensor* A, const ptc_dtensor* B, const ptc_options* opt,
                             ptc_dtensor** O, ptc_stats* stats) {
