@@ -69,19 +69,30 @@ __global__ void k_refresh_batch(const RefreshSeg* __restrict__ segs, const unsig
   }
 }
 
-// one workgroup per segment; grid-stride inside it
-__global__ void k_fill_batch(const FillSeg* __restrict__ segs, int nseg) {
-  for (int k = blockIdx.x; k < nseg; k += gridDim.x) {
-    const FillSeg s = segs[k];
-    if (s.wide) {
-      unsigned long long* p = (unsigned long long*)s.p;
-      for (uint64_t i = threadIdx.x; i < s.n; i += blockDim.x) p[i] = s.v;
-    } else {
-      uint32_t* p = (uint32_t*)s.p;
-      for (uint64_t i = threadIdx.x; i < s.n; i += blockDim.x) p[i] = (uint32_t)s.v;
+// all segments as one flat index space (start[k] = first index of segment k)
+__global__ void k_fill_batch(const FillSeg* __restrict__ segs, const unsigned long long* __restrict__ start, int nseg,
+                             unsigned long long total) {
+  for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (unsigned long long)gridDim.x * blockDim.x) {
+    int lo = 0, hi = nseg - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (start[mid] <= i) lo = mid; else hi = mid - 1;
     }
+    const FillSeg s = segs[lo];
+    const uint64_t k = i - start[lo];
+    if (s.wide) ((unsigned long long*)s.p)[k] = s.v;
+    else ((uint32_t*)s.p)[k] = (uint32_t)s.v;
   }
 }
+
+// Device fill of a list of segments: the launch (capturable) and its arrays.
+struct FillPlan {
+  FillSeg* segs = nullptr;
+  unsigned long long* start = nullptr;
+  int nseg = 0;
+  unsigned long long total = 0;
+};
 
 __global__ void k_gather_words(const uint32_t* const* __restrict__ src, int n, uint32_t* __restrict__ dst) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) dst[i] = *src[i];
@@ -247,17 +258,36 @@ int launch_bins(ptc_batch* b, int pass, std::vector<Bin>& bins, std::vector<Fill
   return PTC_OK;
 }
 
+int make_fill(ptc_batch* b, const std::vector<FillSeg>& f, FillPlan* fp) {
+  std::vector<unsigned long long> st;
+  fp->total = 0;
+  for (const FillSeg& s : f) {
+    st.push_back(fp->total);
+    fp->total += s.n;
+  }
+  fp->nseg = (int)f.size();
+  if (f.empty()) return PTC_OK;
+  fp->segs = upload_vec(b, f);
+  fp->start = upload_vec(b, st);
+  return fp->segs && fp->start ? PTC_OK : PTC_ERR_NOMEM;
+}
+
+int launch_fill(ptc_batch* b, const FillPlan& fp) {
+  if (!fp.nseg || !fp.total) return PTC_OK;
+  const unsigned grid = (unsigned)std::min<unsigned long long>((fp.total + 255) / 256, 16384);
+  hipLaunchKernelGGL(k_fill_batch, dim3(grid), dim3(256), 0, b->ctx->stream, (const FillSeg*)fp.segs,
+                     (const unsigned long long*)fp.start, fp.nseg, fp.total);
+  PTC_HIP(b->ctx, hipGetLastError());
+  return PTC_OK;
+}
+
 // Zeroes the bins' queue heads (before a pass).
 int reset_heads(ptc_batch* b, std::vector<Bin>& bins) {
   std::vector<FillSeg> f;
   for (Bin& bn : bins) f.push_back(FillSeg{bn.d_head, 2, 0ull, 0});
-  if (f.empty()) return PTC_OK;
-  FillSeg* d = upload_vec(b, f);
-  if (!d) return PTC_ERR_NOMEM;
-  hipLaunchKernelGGL(k_fill_batch, dim3((unsigned)f.size()), dim3(256), 0, b->ctx->stream, (const FillSeg*)d,
-                     (int)f.size());
-  PTC_HIP(b->ctx, hipGetLastError());
-  return PTC_OK;
+  FillPlan fp;
+  int rc = make_fill(b, f, &fp);
+  return rc ? rc : launch_fill(b, fp);
 }
 
 int plan(ptc_batch* b) {
@@ -396,12 +426,13 @@ int plan(ptc_batch* b) {
   for (Bin& bn : b->bins) fills.push_back(FillSeg{bn.d_head, 2, 0ull, 0});
   RefreshSeg* d_rseg = upload_vec(b, rseg);
   unsigned long long* d_rstart = upload_vec(b, rstart);
-  FillSeg* d_fill = upload_vec(b, fills);
+  FillPlan fplan;
+  if ((rc = make_fill(b, fills, &fplan))) return rc;
   std::vector<const uint32_t*> ovf_src(n);
   for (int j = 0; j < n; ++j) ovf_src[j] = b->runs[j]->overflow;
   b->d_ovf_src = (uint32_t**)upload_vec(b, ovf_src);
   b->d_ovf = b->ws->get<uint32_t>(n);
-  if (!d_rseg || !d_rstart || !d_fill || !b->d_ovf_src || !b->d_ovf) return PTC_ERR_NOMEM;
+  if (!d_rseg || !d_rstart || !b->d_ovf_src || !b->d_ovf) return PTC_ERR_NOMEM;
   b->st.bins = (uint32_t)b->bins.size();
   // ---- capture the numeric phase (no allocation, read-back or memset inside)
   PTC_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -410,9 +441,8 @@ int plan(ptc_batch* b) {
   const unsigned rgrid = (unsigned)std::min<unsigned long long>((rtotal + 255) / 256, 16384);
   hipLaunchKernelGGL(k_refresh_batch, dim3(std::max(1u, rgrid)), dim3(256), 0, ctx->stream, (const RefreshSeg*)d_rseg,
                      (const unsigned long long*)d_rstart, (int)rseg.size(), rtotal);
-  hipLaunchKernelGGL(k_fill_batch, dim3((unsigned)std::min<size_t>(fills.size(), 4096)), dim3(256), 0, ctx->stream,
-                     (const FillSeg*)d_fill, (int)fills.size());
-  rc = launch_bins(b, PASS_NUMERIC, b->bins, nullptr);
+  rc = launch_fill(b, fplan);
+  if (!rc) rc = launch_bins(b, PASS_NUMERIC, b->bins, nullptr);
   hipError_t ce = hipStreamEndCapture(ctx->stream, &b->graph);
   if (rc) return rc;
   if (ce != hipSuccess) return hip_check(ctx, ce, "graph capture");

@@ -78,55 +78,74 @@ struct CapParams {
   uint32_t* cap;
 };
 
-struct PassParams {
-  // work
-  const uint4* items;
-  const KtSlice* rows;
-  const uint32_t* row_nchunk;
-  const uint32_t* row_heavy;
-  uint32_t* row_done;
-  // positions (A, row-grouped)
-  const unsigned long long* apre;
-  const uint32_t* pos_len;
-  const uint32_t* pos_off;
-  const double* aval;
-  const float* aval32;                // FP32 path: A values in grouped order
-  // B slices
-  const void* bcol;
-  const double* bval;
-  const float* bval32;                // FP32 path: B values in grouped order
-  uint64_t nb;                        // number of B nonzeros
-  uint64_t ncols;                     // column space (dense accumulators)
-  // heavy dense rows: symbolic bitmaps (one per heavy row); numeric slabs
-  // (one partial row per chunk, first slab of row r at row_slab[r])
-  uint32_t* hbits;
-  double* hacc;
-  unsigned char* hflag;
-  const uint32_t* merge_rows;         // merge launch: rows to finish (NULL: all rows)
-  const unsigned long long* row_slab;
-  int use_flags;                      // 1: a flag byte per column marks touches
-  // hash accumulators
-  const uint32_t* row_hcap;
-  const unsigned long long* row_gcap;
-  const unsigned long long* row_goff;
-  void* gkeys;
-  double* gvals;
-  // results
-  uint32_t* row_nnz;
-  const unsigned long long* out_base;
-  const uint32_t* out_cap;            // capacity mode when set
-  uint32_t* overflow;
-  void* stg_col;                      // capacity mode staging (col key)
-  double* stg_val;
-  uint32_t* out_hedges;               // exact mode: final output
-  double* out_vals;
-  float* out_vals32;                  // FP32 path: hi = fl(sum) and lo = sum - hi
-  float* out_err32;
-  int f32;                            // FP32 values in, (hi, lo) FP32 pairs out
-  int val;                            // ptc_value_type: accumulation mode of the numeric pass
-  uint32_t dO, nrowmodes;
+// Parameters of a row pass.  The field list is written once and expanded
+// twice: PassParams, filled by host code, and PassParamsG, which in device
+// code is the same layout with every pointer typed as a global-memory
+// pointer.  A kernel that reads its parameters from memory (the persistent
+// batch pass) uses PassParamsG, so its loads and atomics are global
+// instructions rather than flat ones, which would also count against the LDS
+// wait counter and serialise global loads with LDS traffic.
+#define PTC_PASS_FIELDS(PTR)                                                                  \
+  /* work */                                                                                  \
+  PTR(const uint4) items;                                                                     \
+  PTR(const KtSlice) rows;                                                                    \
+  PTR(const uint32_t) row_nchunk;                                                             \
+  PTR(const uint32_t) row_heavy;                                                              \
+  /* positions (A, row-grouped) */                                                            \
+  PTR(const unsigned long long) apre;                                                         \
+  PTR(const uint32_t) pos_len;                                                                \
+  PTR(const uint32_t) pos_off;                                                                \
+  PTR(const double) aval;                                                                     \
+  PTR(const float) aval32; /* FP32 path: A values in grouped order */                         \
+  /* B slices */                                                                              \
+  PTR(const void) bcol;                                                                       \
+  PTR(const double) bval;                                                                     \
+  PTR(const float) bval32; /* FP32 path: B values in grouped order */                         \
+  uint64_t nb;             /* number of B nonzeros */                                         \
+  uint64_t ncols;          /* column space (dense accumulators) */                            \
+  /* heavy dense rows: symbolic bitmaps; numeric slabs (one partial row per chunk, */         \
+  /* first slab of row r at row_slab[r]) */                                                   \
+  PTR(uint32_t) hbits;                                                                        \
+  PTR(double) hacc;                                                                           \
+  PTR(unsigned char) hflag;                                                                   \
+  PTR(const uint32_t) merge_rows; /* merge launch: rows to finish (NULL: all rows) */         \
+  PTR(const unsigned long long) row_slab;                                                     \
+  int use_flags; /* 1: a flag byte per column marks touches */                                \
+  /* hash accumulators */                                                                     \
+  PTR(const uint32_t) row_hcap;                                                               \
+  PTR(const unsigned long long) row_gcap;                                                     \
+  PTR(const unsigned long long) row_goff;                                                     \
+  PTR(void) gkeys;                                                                            \
+  PTR(double) gvals;                                                                          \
+  /* results */                                                                               \
+  PTR(uint32_t) row_nnz;                                                                      \
+  PTR(const unsigned long long) out_base;                                                     \
+  PTR(const uint32_t) out_cap; /* capacity mode when set */                                   \
+  PTR(uint32_t) overflow;                                                                     \
+  PTR(void) stg_col; /* capacity mode staging (col key) */                                    \
+  PTR(double) stg_val;                                                                        \
+  PTR(uint32_t) out_hedges; /* exact mode: final output */                                    \
+  PTR(double) out_vals;                                                                       \
+  PTR(float) out_vals32; /* FP32 path: hi = fl(sum) and lo = sum - hi */                      \
+  PTR(float) out_err32;                                                                       \
+  int f32; /* FP32 values in, (hi, lo) FP32 pairs out */                                      \
+  int val; /* ptc_value_type: accumulation mode of the numeric pass */                        \
+  uint32_t dO, nrowmodes;                                                                     \
   KeyCodec rowc, colc;
+
+#define PTC_PTR(T) T*
+struct PassParams {
+  PTC_PASS_FIELDS(PTC_PTR)
 };
+#if defined(__HIP_DEVICE_COMPILE__)
+#define PTC_GPTR(T) __attribute__((address_space(1))) T*
+#else
+#define PTC_GPTR(T) T*  // host code never fills the device view
+#endif
+struct PassParamsG {
+  PTC_PASS_FIELDS(PTC_GPTR)
+};
+static_assert(sizeof(PassParamsG) == sizeof(PassParams), "device view of PassParams");
 
 struct EmitParams {
   uint64_t nrows;
@@ -170,7 +189,7 @@ int launch_overflow_rows(ptc_ctx* ctx, const uint32_t* nnz, const uint32_t* cap,
                          uint32_t* list, uint32_t* n, uint32_t* mark);
 int launch_select_items(ptc_ctx* ctx, const uint4* items, uint64_t nitems, const uint32_t* nnz,
                         const uint32_t* cap, uint4* out, uint32_t* n);
-int launch_reset_rows(ptc_ctx* ctx, const uint32_t* list, uint32_t n, uint32_t* row_done);
+int launch_reset_rows(ptc_ctx* ctx, const uint32_t* list, uint32_t n, uint32_t* counts);
 int fill_u64(ptc_ctx* ctx, unsigned long long* p, uint64_t n, unsigned long long v);
 int fill_u32(ptc_ctx* ctx, uint32_t* p, uint64_t n, uint32_t v);
 

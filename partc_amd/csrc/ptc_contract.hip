@@ -291,8 +291,8 @@ __device__ __forceinline__ uint32_t block_rank(bool f, uint32_t* s_w, uint32_t* 
 
 // write one output entry at position pos (row coordinates from LDS); an
 // FP32 result is the pair (hi, lo) with hi = fl(sum), lo = sum - hi
-template <class ColT, bool F32>
-__device__ __forceinline__ void emit_entry(const PassParams& P, uint64_t pos, const uint32_t* s_rowc,
+template <class ColT, bool F32, class PP>
+__device__ __forceinline__ void emit_entry(const PP& P, uint64_t pos, const uint32_t* s_rowc,
                                            ColT col, double v, float hi, float lo) {
   if (P.stg_col) {
     ((ColT*)P.stg_col)[pos] = col;
@@ -442,8 +442,8 @@ __device__ __forceinline__ void f32_finish(float s, float c, float* hi, float* l
 // accumulators; 3 FP32 values, exact products accumulated in FP64.
 // One work item of a row pass: widx indexes P.items (MERGE: P.merge_rows,
 // or the row itself).  Every early return is uniform over the workgroup.
-template <int PASS, bool DENSE, class ColT, bool MERGE, int VAL>
-__device__ __forceinline__ void pass_item(const PassParams& P, uint32_t widx) {
+template <int PASS, bool DENSE, class ColT, bool MERGE, int VAL, class PP>
+__device__ __forceinline__ void pass_item(const PP& P, uint32_t widx) {
   constexpr bool F32 = VAL == 1 || VAL == 2;  // FP32 (sum, compensation) accumulators
   constexpr bool F32IN = VAL != 0;            // FP32 values in, (hi, lo) FP32 pairs out
   constexpr bool COMP = VAL == 1;
@@ -951,7 +951,10 @@ __global__ void __launch_bounds__(PB) k_pass_batch(const PassParams* __restrict_
     const uint32_t q = s_q;
     if (q >= nq) return;
     const uint2 w = queue[q];
-    pass_item<PASS, DENSE, ColT, MERGE, VAL>(jobs[w.x], w.y);
+    // a private copy: the job's parameters are loaded once per item (into
+    // scalar registers), not re-read after every atomic of the item
+    const PassParamsG P = reinterpret_cast<const PassParamsG*>(jobs)[w.x];
+    pass_item<PASS, DENSE, ColT, MERGE, VAL>(P, w.y);
   }
 }
 
@@ -1249,12 +1252,12 @@ int launch_select_items(ptc_ctx* ctx, const uint4* items, uint64_t nitems, const
 }
 
 // reset per-row merge state of selected rows
-__global__ void k_reset_rows(const uint32_t* __restrict__ list, uint32_t n, uint32_t* __restrict__ row_done) {
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) row_done[list[i]] = 0;
+__global__ void k_reset_rows(const uint32_t* __restrict__ list, uint32_t n, uint32_t* __restrict__ counts) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) counts[list[i]] = 0;
 }
-int launch_reset_rows(ptc_ctx* ctx, const uint32_t* list, uint32_t n, uint32_t* row_done) {
+int launch_reset_rows(ptc_ctx* ctx, const uint32_t* list, uint32_t n, uint32_t* counts) {
   if (!n) return PTC_OK;
-  hipLaunchKernelGGL(k_reset_rows, dim3(gridn(n, 256)), dim3(256), 0, ctx->stream, list, n, row_done);
+  hipLaunchKernelGGL(k_reset_rows, dim3(gridn(n, 256)), dim3(256), 0, ctx->stream, list, n, counts);
   PTC_HIP(ctx, hipGetLastError());
   return PTC_OK;
 }

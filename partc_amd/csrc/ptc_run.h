@@ -37,7 +37,7 @@ struct Run {
   uint32_t *pos_len = nullptr, *pos_off = nullptr;
   unsigned long long* apre = nullptr;
   uint32_t *row_hits = nullptr, *row_maxlen = nullptr, *row_nchunk = nullptr, *row_heavy = nullptr;
-  uint32_t *row_done = nullptr, *row_nnz = nullptr;
+  uint32_t* row_nnz = nullptr;
   unsigned long long *row_flops = nullptr, *packed = nullptr;
   uint4* items = nullptr;
   uint64_t F = 0, nitems = 0, nheavy = 0;

@@ -129,7 +129,6 @@ void base_params(Run& R, PassParams& P, const HashPlan* hp) {
   P.rows = R.HA->slices;
   P.row_nchunk = R.row_nchunk;
   P.row_heavy = R.row_heavy;
-  P.row_done = R.row_done;
   P.apre = R.apre;
   P.pos_len = R.pos_len;
   P.pos_off = R.pos_off;
@@ -342,7 +341,6 @@ int prepare_launch(Run& R, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B,
   R.row_maxlen = ws.get<uint32_t>(R.nrows);
   R.row_nchunk = ws.get<uint32_t>(R.nrows);
   R.row_heavy = ws.get<uint32_t>(R.nrows);
-  R.row_done = ws.get<uint32_t>(R.nrows);
   R.row_nnz = ws.get<uint32_t>(R.nrows + 1);
   R.row_flops = ws.get<unsigned long long>(R.nrows);
   R.packed = ws.get<unsigned long long>(R.nrows + 1);
@@ -350,7 +348,7 @@ int prepare_launch(Run& R, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B,
   R.overflow = ws.get<uint32_t>(4);
   R.out_off = ws.get<unsigned long long>(R.nrows + 1);
   if (!R.pos_len || !R.pos_off || !R.apre || !R.row_hits || !R.row_maxlen || !R.row_nchunk || !R.row_heavy ||
-      !R.row_done || !R.row_nnz || !R.row_flops || !R.packed || !R.row_slab || !R.overflow || !R.out_off)
+      !R.row_nnz || !R.row_flops || !R.packed || !R.row_slab || !R.overflow || !R.out_off)
     return PTC_ERR_NOMEM;
 
   // ---------------- preprocessing: probe, flop prefix, rows, items
@@ -490,7 +488,6 @@ extern "C" int ptc_tc_fksCuckoo(ptc_ctx* ctx, const ptc_hashtable* HT_A, const p
     HashPlan hs;
     if (!R.dense && (rc = plan_hash(R, hs, nullptr, R.row_flops, nullptr))) return rc;
     if ((rc = fill_zero(ctx, R.row_nnz, (R.nrows + 1) * 4))) return rc;
-    if ((rc = fill_zero(ctx, R.row_done, R.nrows * 4))) return rc;
     if ((rc = prepare_heavy(R, PASS_SYMBOLIC))) return rc;
     if (!R.dense && (rc = reset_hash_pool(R, hs, false))) return rc;
     PassParams P;
@@ -508,7 +505,7 @@ extern "C" int ptc_tc_fksCuckoo(ptc_ctx* ctx, const ptc_hashtable* HT_A, const p
       ptc_dtensor_free(out);
       return rc;
     }
-    if ((rc = fill_zero(ctx, R.row_done, R.nrows * 4)) || (rc = prepare_heavy(R, PASS_NUMERIC)) ||
+    if ((rc = prepare_heavy(R, PASS_NUMERIC)) ||
         (!R.dense && (rc = reset_hash_pool(R, hn, true)))) {
       ptc_dtensor_free(out);
       return rc;
@@ -566,7 +563,7 @@ extern "C" int ptc_tc_fksCuckoo(ptc_ctx* ctx, const ptc_hashtable* HT_A, const p
     if (!stg_col || !stg_val) return PTC_ERR_NOMEM;
     HashPlan h1;
     if (!R.dense && (rc = plan_hash(R, h1, cap, nullptr, R.row_flops))) return rc;
-    if ((rc = fill_zero(ctx, R.row_nnz, (R.nrows + 1) * 4)) || (rc = fill_zero(ctx, R.row_done, R.nrows * 4)) ||
+    if ((rc = fill_zero(ctx, R.row_nnz, (R.nrows + 1) * 4)) ||
         (rc = prepare_heavy(R, PASS_NUMERIC)) || (!R.dense && (rc = reset_hash_pool(R, h1, true))))
       return rc;
     PassParams P;
@@ -774,7 +771,7 @@ extern "C" int ptc_row_profile(ptc_ctx* ctx, const ptc_hashtable* HT_A, const pt
   if (R.nrows > cap_rows || !X || !ub || !nnz) return set_err(ctx, PTC_ERR_RANGE, "profile needs %llu rows", (unsigned long long)R.nrows);
   HashPlan hs;
   if (!R.dense && (rc = plan_hash(R, hs, nullptr, R.row_flops, nullptr))) return rc;
-  if ((rc = fill_zero(ctx, R.row_nnz, (R.nrows + 1) * 4)) || (rc = fill_zero(ctx, R.row_done, R.nrows * 4)) ||
+  if ((rc = fill_zero(ctx, R.row_nnz, (R.nrows + 1) * 4)) ||
       (rc = prepare_heavy(R, PASS_SYMBOLIC)) || (!R.dense && (rc = reset_hash_pool(R, hs, false))))
     return rc;
   PassParams P;
