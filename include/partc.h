@@ -245,7 +245,8 @@ typedef struct {
   uint32_t runs;          /* numeric graph launches so far                    */
   uint64_t flops, nnz_out, graph_nodes;
   double ms_build;        /* all key tables (device time)                     */
-  double ms_symbolic;     /* preprocessing + symbolic passes (device time)    */
+  double ms_symbolic;     /* preprocessing + symbolic phase (device timeline) */
+  double ms_symbolic_pass;/* the symbolic persistent launches alone           */
   double ms_plan;         /* ptc_batch_plan, wall                             */
   double ms_numeric;      /* last numeric graph launch (device time)          */
   double ms_total;        /* ptc_contract_batched, wall                       */
@@ -257,6 +258,25 @@ int ptc_batch_run(ptc_batch* b, ptc_batch_stats* stats);
 /* Job j's output, owned by the plan. */
 int ptc_batch_output(const ptc_batch* b, int j, const ptc_dtensor** O);
 void ptc_batch_free(ptc_batch* b);
+/* Autotuner of a batch's row passes: coordinate descent over the dense LDS
+ * column limit (LDS vs hash accumulators), the LDS hash size (LDS vs HBM
+ * accumulators) and the multiply-adds per work item, each trial timed on the
+ * device (symbolic phase + numeric graph) on the batch's own key tables.
+ * best receives base with the chosen knobs. */
+#define PTC_TUNE_MAX_TRIALS 24
+typedef struct {
+  uint32_t trials;
+  uint32_t dense_max_cols, lds_hash_max;
+  uint64_t chunk_flops;
+  double default_ms, best_ms;  /* symbolic launches + numeric graph, device  */
+  double ms_wall;              /* the whole search                          */
+  /* every trial: its knobs and its time */
+  uint32_t trial_dense[PTC_TUNE_MAX_TRIALS], trial_hash[PTC_TUNE_MAX_TRIALS];
+  uint64_t trial_chunk[PTC_TUNE_MAX_TRIALS];
+  double trial_ms[PTC_TUNE_MAX_TRIALS];
+} ptc_tune_report;
+int ptc_batch_autotune(ptc_ctx* ctx, int n, const ptc_dtensor* const* A, const ptc_dtensor* const* B,
+                       const ptc_options* base, ptc_options* best, ptc_tune_report* report);
 /* Plan, run once and hand the outputs over (free each with ptc_dtensor_free). */
 int ptc_contract_batched(ptc_ctx* ctx, int n, const ptc_dtensor* const* A, const ptc_dtensor* const* B,
                          const ptc_options* opt, ptc_dtensor** O, ptc_batch_stats* stats);

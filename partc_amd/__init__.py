@@ -93,6 +93,7 @@ class BatchStats(C.Structure):
         ("graph_nodes", C.c_uint64),
         ("ms_build", C.c_double),
         ("ms_symbolic", C.c_double),
+        ("ms_symbolic_pass", C.c_double),
         ("ms_plan", C.c_double),
         ("ms_numeric", C.c_double),
         ("ms_total", C.c_double),
@@ -100,6 +101,29 @@ class BatchStats(C.Structure):
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class TuneReport(C.Structure):
+    _fields_ = [
+        ("trials", C.c_uint32),
+        ("dense_max_cols", C.c_uint32),
+        ("lds_hash_max", C.c_uint32),
+        ("chunk_flops", C.c_uint64),
+        ("default_ms", C.c_double),
+        ("best_ms", C.c_double),
+        ("ms_wall", C.c_double),
+        ("trial_dense", C.c_uint32 * 24),
+        ("trial_hash", C.c_uint32 * 24),
+        ("trial_chunk", C.c_uint64 * 24),
+        ("trial_ms", C.c_double * 24),
+    ]
+
+    def as_dict(self) -> dict:
+        d = {k: getattr(self, k) for k, _ in self._fields_ if not k.startswith("trial_")}
+        n = min(self.trials, 24)
+        d["trials_list"] = [(self.trial_dense[i], self.trial_hash[i], self.trial_chunk[i], round(self.trial_ms[i], 3))
+                            for i in range(n)]
+        return d
 
 
 class Stats(C.Structure):
@@ -172,6 +196,8 @@ def lib() -> C.CDLL:
         "ptc_batch_run": (i32, [vp, C.POINTER(BatchStats)]),
         "ptc_batch_output": (i32, [vp, i32, C.POINTER(vp)]),
         "ptc_batch_free": (None, [vp]),
+        "ptc_batch_autotune": (i32, [vp, i32, C.POINTER(vp), C.POINTER(vp), C.POINTER(Options), C.POINTER(Options),
+                                     C.POINTER(TuneReport)]),
         "ptc_contract_batched": (i32, [vp, i32, C.POINTER(vp), C.POINTER(vp), C.POINTER(Options), C.POINTER(vp),
                                        C.POINTER(BatchStats)]),
         "ptc_gen_job_shape": (i32, [u64, u32, u32, u64, u64, P32, P32, P64]),
@@ -568,6 +594,22 @@ class BatchPlan:
             self.free()
         except Exception:
             pass
+
+
+def autotune_batch(As, Bs, ctx: Optional[Context] = None, **base_opts):
+    """ptc_batch_autotune: returns (tuned options as keyword arguments, the
+    tuner's report)."""
+    ctx = ctx or default_context()
+    keep_a, a = _handles(As, ctx)
+    keep_b, b = _handles(Bs, ctx)
+    base = options(**base_opts)
+    best = Options()
+    rep = TuneReport()
+    _check(lib().ptc_batch_autotune(ctx.handle, len(As), a, b, C.byref(base), C.byref(best), C.byref(rep)), ctx)
+    del keep_a, keep_b
+    tuned = dict(base_opts, dense_max_cols=best.dense_max_cols, lds_hash_max=best.lds_hash_max,
+                 chunk_flops=best.chunk_flops)
+    return tuned, rep.as_dict()
 
 
 def contract_batched(As, Bs, ctx: Optional[Context] = None, *, download: bool = True,

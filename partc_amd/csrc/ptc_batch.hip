@@ -290,7 +290,8 @@ int reset_heads(ptc_batch* b, std::vector<Bin>& bins) {
   return rc ? rc : launch_fill(b, fp);
 }
 
-int plan(ptc_batch* b) {
+// Key tables of every job (independent of the tuning knobs).
+int plan_tables(ptc_batch* b) {
   ptc_ctx* ctx = b->ctx;
   const int n = b->n;
   int rc = PTC_OK;
@@ -309,6 +310,15 @@ int plan(ptc_batch* b) {
     if ((rc = ptc_hashtable_build(ctx, b->Ac[j], &b->HA[j]))) return rc;
   }
   stats_mark(b, 1);
+  return PTC_OK;
+}
+
+// Everything after the tables: preprocessing, symbolic passes, outputs and
+// the captured numeric graph, under b->opt.
+int plan_rest(ptc_batch* b) {
+  ptc_ctx* ctx = b->ctx;
+  const int n = b->n;
+  int rc = PTC_OK;
   // ---- preprocessing: one read-back for all jobs
   unsigned long long* tot = b->ws->get<unsigned long long>(4 * (size_t)n);
   if (!tot) return PTC_ERR_NOMEM;
@@ -358,7 +368,9 @@ int plan(ptc_batch* b) {
   }
   std::vector<Bin> sbins;
   if ((rc = make_bins(b, PASS_SYMBOLIC, params, hptr, sbins)) || (rc = reset_heads(b, sbins))) return rc;
+  stats_mark(b, 6);
   if ((rc = launch_bins(b, PASS_SYMBOLIC, sbins, nullptr))) return rc;
+  stats_mark(b, 7);
   stats_mark(b, 2);
   // ---- output sizes and the symbolic pass's consistency flags: one read-back
   std::vector<const unsigned long long*> fsrc(n);
@@ -451,7 +463,33 @@ int plan(ptc_batch* b) {
   if (hipGraphGetNodes(b->graph, nullptr, &nodes) == hipSuccess) b->st.graph_nodes = nodes;
   b->st.ms_build = stats_ms(b, 0, 1);
   b->st.ms_symbolic = stats_ms(b, 1, 2);
+  b->st.ms_symbolic_pass = stats_ms(b, 6, 7);
   return PTC_OK;
+}
+
+// Drops what plan_rest made, keeping the key tables.
+void clear_rest(ptc_batch* b) {
+  hipStreamSynchronize(b->ctx->stream);
+  if (b->exec) hipGraphExecDestroy(b->exec);
+  if (b->graph) hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+  for (auto*& o : b->out) {
+    ptc_dtensor_free(o);
+    o = nullptr;
+  }
+  for (auto& r : b->runs) r.reset();
+  b->hn.assign(b->n, HashPlan());
+  b->bins.clear();
+  delete b->ws;
+  b->ws = new Workspace(b->ctx);
+  b->host_keep.clear();
+  b->d_ovf_src = nullptr;
+  b->d_ovf = nullptr;
+  const double ms_build = b->st.ms_build;
+  b->st = ptc_batch_stats{};
+  b->st.jobs = (uint32_t)b->n;
+  b->st.ms_build = ms_build;
 }
 
 }  // namespace
@@ -484,7 +522,8 @@ extern "C" int ptc_batch_plan(ptc_ctx* ctx, int n, const ptc_dtensor* const* A, 
   b->ws = new Workspace(ctx);
   b->st.jobs = (uint32_t)n;
   for (auto& e : b->ev) hipEventCreate(&e);
-  int rc = plan(b);
+  int rc = plan_tables(b);
+  if (!rc) rc = plan_rest(b);
   if (rc) {
     (void)hipGetLastError();
     ptc_batch_free(b);
@@ -539,6 +578,86 @@ extern "C" void ptc_batch_free(ptc_batch* b) {
   for (auto& e : b->ev)
     if (e) hipEventDestroy(e);
   delete b;
+}
+
+// Coordinate descent over the knobs that shape a batch's row passes:
+// dense LDS column limit (LDS vs hash accumulators), LDS hash slots (LDS vs
+// HBM accumulators) and work per item (chunk).  Each trial re-plans the
+// batch on the same key tables and times its symbolic phase and one numeric
+// graph launch on the device.
+extern "C" int ptc_batch_autotune(ptc_ctx* ctx, int n, const ptc_dtensor* const* A, const ptc_dtensor* const* B,
+                                  const ptc_options* base, ptc_options* best, ptc_tune_report* report) {
+  if (!ctx || n <= 0 || !A || !B || !best) return PTC_ERR_ARG;
+  auto w0 = std::chrono::steady_clock::now();
+  ptc_options cur;
+  if (base) cur = *base; else ptc_options_default(&cur);
+  if (!cur.dense_max_cols) cur.dense_max_cols = DEFAULT_DENSE_MAX;
+  if (!cur.lds_hash_max) cur.lds_hash_max = DEFAULT_LDS_HASH;
+  if (!cur.chunk_flops) cur.chunk_flops = DEFAULT_CHUNK;
+  ptc_batch* b = nullptr;
+  // the tables, once, through a plan under the starting options
+  int rc = ptc_batch_plan(ctx, n, A, B, &cur, &b, nullptr);
+  if (rc) return rc;
+  ptc_tune_report rep{};
+  auto trial = [&](const ptc_options& o, double* ms) -> int {
+    clear_rest(b);
+    b->opt = o;
+    b->opt.method = PTC_METHOD_TWO_PASS;
+    int r = plan_rest(b);
+    if (!r) r = ptc_batch_run(b, nullptr);  // warm
+    if (!r) r = ptc_batch_run(b, nullptr);
+    if (r) return r;
+    *ms = b->st.ms_symbolic_pass + b->st.ms_numeric;
+    if (rep.trials < PTC_TUNE_MAX_TRIALS) {
+      rep.trial_dense[rep.trials] = o.dense_max_cols;
+      rep.trial_hash[rep.trials] = o.lds_hash_max;
+      rep.trial_chunk[rep.trials] = o.chunk_flops;
+      rep.trial_ms[rep.trials] = *ms;
+    }
+    rep.trials++;
+    return PTC_OK;
+  };
+  double best_ms = 0;
+  if ((rc = trial(cur, &best_ms))) {
+    ptc_batch_free(b);
+    return rc;
+  }
+  rep.default_ms = best_ms;
+  const uint32_t dense_c[] = {2048, 4096, 8192, 16384};
+  const uint32_t hash_c[] = {512, 1024, 2048, 4096, 8192};
+  const uint64_t chunk_c[] = {8192, 16384, 32768, 65536, 131072};
+  for (int knob = 0; knob < 3 && !rc; ++knob) {
+    const int nc = knob == 0 ? 4 : 5;
+    for (int k = 0; k < nc && !rc; ++k) {
+      ptc_options o = cur;
+      if (knob == 0) {
+        if (dense_c[k] == cur.dense_max_cols) continue;
+        o.dense_max_cols = dense_c[k];
+      } else if (knob == 1) {
+        if (hash_c[k] == cur.lds_hash_max) continue;
+        o.lds_hash_max = hash_c[k];
+      } else {
+        if (chunk_c[k] == cur.chunk_flops) continue;
+        o.chunk_flops = chunk_c[k];
+      }
+      double ms = 0;
+      rc = trial(o, &ms);
+      if (!rc && ms < best_ms) {
+        best_ms = ms;
+        cur = o;
+      }
+    }
+  }
+  ptc_batch_free(b);
+  if (rc) return rc;
+  *best = cur;
+  rep.best_ms = best_ms;
+  rep.dense_max_cols = cur.dense_max_cols;
+  rep.lds_hash_max = cur.lds_hash_max;
+  rep.chunk_flops = cur.chunk_flops;
+  rep.ms_wall = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
+  if (report) *report = rep;
+  return PTC_OK;
 }
 
 extern "C" int ptc_contract_batched(ptc_ctx* ctx, int n, const ptc_dtensor* const* A, const ptc_dtensor* const* B,

@@ -107,6 +107,18 @@ def test_batch_graph_replays_new_values(ctx, jobs):
     plan.free()
 
 
+def test_autotuned_batch_matches_oracle(ctx, jobs):
+    """The tuner's chosen thresholds change how work is binned and
+    accumulated, never the result."""
+    print("[case] batch autotune", flush=True)
+    tuned, rep = P.autotune_batch(jobs, jobs, ctx)
+    print(f"  tuned {tuned} report {rep}")
+    assert rep["trials"] >= 5 and rep["best_ms"] <= rep["default_ms"]
+    outs = P.contract_batched(jobs, jobs, ctx, **tuned)
+    for j, (T, O) in enumerate(zip(jobs, outs)):
+        check(O, oracle_of(T), f"tuned job {j}")
+
+
 def test_batch_errors(ctx, jobs):
     with pytest.raises(P.PtcError) as ei:
         P.contract_batched(jobs[:2], jobs[:2], ctx, value_type=P.VAL_F32_TWOSUM)
