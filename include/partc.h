@@ -175,6 +175,9 @@ typedef struct {
   double predict_scale;   /* learned path: capacity = scale * prediction (0 = default 1.15) */
   const ptc_mlp* mlp;     /* learned path: predictor (NULL = built-in)       */
   int deterministic;      /* reserved, must be 0 (PTC_ERR_UNSUPPORTED otherwise) */
+  uint32_t block_threads; /* threads per row-pass workgroup, a multiple of 64 up to
+                             1024 (0 = 1024 when the accumulator's LDS allows
+                             fewer than two 512-thread workgroups per CU)     */
 } ptc_options;
 
 typedef struct {

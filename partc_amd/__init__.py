@@ -79,6 +79,7 @@ class Options(C.Structure):
         ("predict_scale", C.c_double),
         ("mlp", C.c_void_p),
         ("deterministic", C.c_int),
+        ("block_threads", C.c_uint32),
     ]
 
 
@@ -497,7 +498,7 @@ def buildHashTable(T, ctx: Optional[Context] = None) -> HashTable:
 
 def options(method: int = METHOD_TWO_PASS, value_type: int = VAL_F64, acc_mode: int = ACC_AUTO,
             dense_max_cols: int = 0, lds_hash_max: int = 0, chunk_flops: int = 0, predict_scale: float = 0.0,
-            mlp=None, deterministic: int = 0) -> Options:
+            mlp=None, deterministic: int = 0, block_threads: int = 0) -> Options:
     o = Options()
     lib().ptc_options_default(C.byref(o))
     o.method = method
@@ -510,6 +511,7 @@ def options(method: int = METHOD_TWO_PASS, value_type: int = VAL_F64, acc_mode: 
         o.predict_scale = predict_scale
     o.mlp = mlp._h if mlp is not None else None
     o.deterministic = deterministic
+    o.block_threads = block_threads
     return o
 
 

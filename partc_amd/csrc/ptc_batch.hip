@@ -102,6 +102,7 @@ __global__ void k_gather_words(const uint32_t* const* __restrict__ src, int n, u
 struct Bin {
   bool dense = true, col64 = false;
   size_t lds = 0;
+  uint32_t threads = PB;
   std::vector<int> jobs;             // batch job indices
   PassParams* d_params = nullptr;    // device, one per job of the bin
   uint2* d_queue = nullptr;          // main work
@@ -191,13 +192,13 @@ double stats_ms(ptc_batch* b, int i, int j) {
 // Bins of the jobs for one pass and their queues; params[j] must be filled.
 int make_bins(ptc_batch* b, int pass, const std::vector<PassParams>& params, const std::vector<HashPlan*>& hp,
               std::vector<Bin>& bins) {
-  std::map<std::tuple<bool, bool, size_t>, int> key;
+  std::map<std::tuple<bool, bool, size_t, uint32_t>, int> key;
   bins.clear();
   for (int j = 0; j < b->n; ++j) {
     Run& R = *b->runs[j];
-    const size_t need = pass_lds_bytes(pass, R.dense, R.col64, R.ncols, R.lds_slots, R.use_flags);
+    const size_t need = pass_lds_bytes(pass, R.dense, R.col64, R.ncols, R.lds_slots, R.use_flags, R.threads);
     const size_t lds = std::max(need, std::min<size_t>(lds_class(need), lds_limit(b->ctx)));
-    auto k = std::make_tuple(R.dense, R.col64, lds);
+    auto k = std::make_tuple(R.dense, R.col64, lds, R.threads);
     auto it = key.find(k);
     if (it == key.end()) {
       it = key.emplace(k, (int)bins.size()).first;
@@ -205,6 +206,7 @@ int make_bins(ptc_batch* b, int pass, const std::vector<PassParams>& params, con
       nb.dense = R.dense;
       nb.col64 = R.col64;
       nb.lds = lds;
+      nb.threads = R.threads;
       bins.push_back(nb);
     }
     bins[it->second].jobs.push_back(j);
@@ -236,8 +238,8 @@ int make_bins(ptc_batch* b, int pass, const std::vector<PassParams>& params, con
     bn.d_mqueue = upload_vec(b, mq);
     bn.d_head = b->ws->get<uint32_t>(2);
     if (!bn.d_params || (bn.nq && !bn.d_queue) || (bn.nmq && !bn.d_mqueue) || !bn.d_head) return PTC_ERR_NOMEM;
-    int rc = pass_batch_grid(b->ctx, pass, bn.dense, bn.col64, 0, false, bn.lds, &bn.grid);
-    if (!rc) rc = pass_batch_grid(b->ctx, pass, bn.dense, bn.col64, 0, true, bn.lds, &bn.mgrid);
+    int rc = pass_batch_grid(b->ctx, pass, bn.dense, bn.col64, 0, false, bn.lds, &bn.grid, bn.threads);
+    if (!rc) rc = pass_batch_grid(b->ctx, pass, bn.dense, bn.col64, 0, true, bn.lds, &bn.mgrid, bn.threads);
     if (rc) return rc;
   }
   return PTC_OK;
@@ -248,10 +250,10 @@ int launch_bins(ptc_batch* b, int pass, std::vector<Bin>& bins, std::vector<Fill
   int rc = PTC_OK;
   for (Bin& bn : bins) {
     if ((rc = launch_pass_batch(ctx, pass, bn.dense, bn.col64, 0, false, bn.d_params, bn.d_queue, bn.nq, bn.d_head,
-                                bn.lds, bn.grid)))
+                                bn.lds, bn.grid, bn.threads)))
       return rc;
     if ((rc = launch_pass_batch(ctx, pass, bn.dense, bn.col64, 0, true, bn.d_params, bn.d_mqueue, bn.nmq,
-                                bn.d_head + 1, bn.lds, bn.mgrid)))
+                                bn.d_head + 1, bn.lds, bn.mgrid, bn.threads)))
       return rc;
   }
   (void)fills;
@@ -593,7 +595,8 @@ extern "C" int ptc_batch_autotune(ptc_ctx* ctx, int n, const ptc_dtensor* const*
   if (base) cur = *base; else ptc_options_default(&cur);
   if (!cur.dense_max_cols) cur.dense_max_cols = DEFAULT_DENSE_MAX;
   if (!cur.lds_hash_max) cur.lds_hash_max = DEFAULT_LDS_HASH;
-  if (!cur.chunk_flops) cur.chunk_flops = DEFAULT_CHUNK;
+  // chunk_flops 0 stays "automatic" (a rule of the accumulator kind and the
+  // column space) and is one of the candidates
   ptc_batch* b = nullptr;
   // the tables, once, through a plan under the starting options
   int rc = ptc_batch_plan(ctx, n, A, B, &cur, &b, nullptr);
@@ -625,7 +628,7 @@ extern "C" int ptc_batch_autotune(ptc_ctx* ctx, int n, const ptc_dtensor* const*
   rep.default_ms = best_ms;
   const uint32_t dense_c[] = {2048, 4096, 8192, 16384};
   const uint32_t hash_c[] = {512, 1024, 2048, 4096, 8192};
-  const uint64_t chunk_c[] = {8192, 16384, 32768, 65536, 131072};
+  const uint64_t chunk_c[] = {0, 16384, 32768, 65536, 131072};
   for (int knob = 0; knob < 3 && !rc; ++knob) {
     const int nc = knob == 0 ? 4 : 5;
     for (int k = 0; k < nc && !rc; ++k) {

@@ -4,9 +4,15 @@
 
 #include "ptc_internal.h"
 
+#ifndef PTC_PASS_THREADS
+#define PTC_PASS_THREADS 512
+#endif
+
 namespace ptc {
 
 enum { PASS_SYMBOLIC = 0, PASS_NUMERIC = 1 };
+constexpr int PB = PTC_PASS_THREADS;  // default threads per workgroup of the row passes
+constexpr int PB_MAX = 1024;          // largest row-pass workgroup (its launch bound)
 
 struct ProbeParams {
   uint64_t npos;
@@ -173,18 +179,21 @@ int launch_row_features(ptc_ctx* ctx, const uint32_t* row_of_pos, uint64_t npos,
 int launch_items(ptc_ctx* ctx, const ItemParams& P, const ItemP0Params& Q);
 int launch_hash_caps(ptc_ctx* ctx, const HashCapParams& P);
 int launch_caps(ptc_ctx* ctx, const CapParams& P);
-size_t pass_lds_bytes(int pass, bool dense, bool col64, uint64_t ncols, uint32_t lds_slots, bool flags = true);
+size_t pass_lds_bytes(int pass, bool dense, bool col64, uint64_t ncols, uint32_t lds_slots, bool flags = true,
+                      uint32_t threads = PB);
 int launch_pass(ptc_ctx* ctx, int pass, bool dense, bool col64, const PassParams& P, uint64_t nitems,
-                size_t lds, bool merge = false);
+                size_t lds, bool merge = false, uint32_t threads = PB);
 int launch_emit(ptc_ctx* ctx, const EmitParams& P, bool col64);
 // Persistent row pass over the queued work of several contractions (jobs:
 // device array of their PassParams; queue: (job, work index) pairs; head: a
 // zeroed device counter).  grid 0 = as many workgroups as fit on the device.
 int launch_pass_batch(ptc_ctx* ctx, int pass, bool dense, bool col64, int val, bool merge, const PassParams* jobs,
-                      const uint2* queue, uint32_t nq, uint32_t* head, size_t lds, unsigned grid = 0);
+                      const uint2* queue, uint32_t nq, uint32_t* head, size_t lds, unsigned grid = 0,
+                      uint32_t threads = PB);
 // The same kernel's device-filling grid at this LDS size (opens the LDS
 // attribute too: call before a graph capture, then launch with the grid).
-int pass_batch_grid(ptc_ctx* ctx, int pass, bool dense, bool col64, int val, bool merge, size_t lds, unsigned* grid);
+int pass_batch_grid(ptc_ctx* ctx, int pass, bool dense, bool col64, int val, bool merge, size_t lds, unsigned* grid,
+                    uint32_t threads = PB);
 int launch_overflow_rows(ptc_ctx* ctx, const uint32_t* nnz, const uint32_t* cap, uint64_t nrows,
                          uint32_t* list, uint32_t* n, uint32_t* mark);
 int launch_select_items(ptc_ctx* ctx, const uint4* items, uint64_t nitems, const uint32_t* nnz,

@@ -46,14 +46,10 @@
 
 namespace ptc {
 
-#ifndef PTC_PASS_THREADS
-#define PTC_PASS_THREADS 512
-#endif
 #ifndef PTC_PASS_UNROLL
 #define PTC_PASS_UNROLL 4
 #endif
-constexpr int PB = PTC_PASS_THREADS;  // threads per workgroup of the row passes
-constexpr int STAGE = 2 * PB;         // A positions staged in LDS at a time (2 per thread)
+// A positions staged in LDS at a time: 2 per thread of the launch
 constexpr uint32_t NO_HEAVY = 0xFFFFFFFFu;
 
 __device__ __forceinline__ unsigned long long mix64d(unsigned long long k) {
@@ -278,8 +274,8 @@ __device__ __forceinline__ uint32_t block_rank(bool f, uint32_t* s_w, uint32_t* 
   if (lane == 0) s_w[wid] = __popcll(m);
   __syncthreads();
   uint32_t base = 0, all = 0;
-#pragma unroll
-  for (int w = 0; w < PB / 64; ++w) {
+  const int nw = (int)(blockDim.x >> 6);
+  for (int w = 0; w < nw; ++w) {
     uint32_t c = s_w[w];
     if (w < wid) base += c;
     all += c;
@@ -448,13 +444,16 @@ __device__ __forceinline__ void pass_item(const PP& P, uint32_t widx) {
   constexpr bool F32IN = VAL != 0;            // FP32 values in, (hi, lo) FP32 pairs out
   constexpr bool COMP = VAL == 1;
   static_assert(!F32IN || PASS == PASS_NUMERIC, "FP32 values only matter to the numeric pass");
+  const uint32_t NT = blockDim.x;  // a multiple of 64, at most PB_MAX
+  const uint32_t STAGE = 2 * NT;
+  const int NW = (int)(NT >> 6);
   extern __shared__ __align__(16) unsigned char smem[];
   uint32_t* s_start = (uint32_t*)smem;
   long long* s_bias = (long long*)(smem + STAGE * 4);
   double* s_aval = (double*)(smem + STAGE * 12);
   float* s_aval32 = (float*)s_aval;  // FP32: the first half of the same region
   unsigned char* acc_base = smem + STAGE * 20;
-  __shared__ uint32_t s_w[PB / 64];
+  __shared__ uint32_t s_w[PB_MAX / 64];
   __shared__ uint32_t s_misc[4];
   __shared__ uint32_t s_rowc[PTC_MAX_MODES];
   __shared__ uint32_t s_ns;
@@ -500,7 +499,7 @@ __device__ __forceinline__ void pass_item(const PP& P, uint32_t widx) {
     // reduce the row's slabs in chunk order
     const uint64_t slab0 = P.row_slab[r];
     if (PASS == PASS_SYMBOLIC) {
-      for (uint32_t w = threadIdx.x; w < nbits; w += PB) {
+      for (uint32_t w = threadIdx.x; w < nbits; w += NT) {
         uint32_t o = 0;
         for (uint32_t kk = 0; kk < nch; ++kk) o |= P.hbits[(slab0 + kk) * nbits + w];
         bits[w] = o;
@@ -508,7 +507,7 @@ __device__ __forceinline__ void pass_item(const PP& P, uint32_t widx) {
     } else if (F32) {
       // slabs hold (sum, compensation) pairs; combine them in chunk order
       const float2* h2 = (const float2*)P.hacc;
-      for (uint64_t c = threadIdx.x; c < ncols; c += PB) {
+      for (uint64_t c = threadIdx.x; c < ncols; c += NT) {
         float S = -0.0f, Cc = 0.f;
         unsigned char fl = 0;
 #pragma unroll 4
@@ -525,7 +524,7 @@ __device__ __forceinline__ void pass_item(const PP& P, uint32_t widx) {
         if (flags) flag[c] = fl;
       }
     } else {
-      for (uint64_t c = threadIdx.x; c < ncols; c += PB) {
+      for (uint64_t c = threadIdx.x; c < ncols; c += NT) {
         double sum = __longlong_as_double((long long)PTC_NEG_ZERO_BITS);
         unsigned char fl = 0;
         uint32_t kk = 0;
@@ -546,29 +545,29 @@ __device__ __forceinline__ void pass_item(const PP& P, uint32_t widx) {
     }
   } else if (DENSE) {
     if (PASS == PASS_SYMBOLIC) {
-      for (uint32_t w = threadIdx.x; w < nbits; w += PB) bits[w] = 0;
+      for (uint32_t w = threadIdx.x; w < nbits; w += NT) bits[w] = 0;
     } else if (F32) {
-      for (uint64_t c = threadIdx.x; c < ncols; c += PB) {
+      for (uint64_t c = threadIdx.x; c < ncols; c += NT) {
         acc_s[c] = -0.0f;
         acc_c[c] = 0.f;
       }
       if (flags)
-        for (uint64_t c = threadIdx.x; c < ncols; c += PB) flag[c] = 0;
+        for (uint64_t c = threadIdx.x; c < ncols; c += NT) flag[c] = 0;
     } else {
       const double nz = __longlong_as_double((long long)PTC_NEG_ZERO_BITS);
-      for (uint64_t c = threadIdx.x; c < ncols; c += PB) acc[c] = nz;
+      for (uint64_t c = threadIdx.x; c < ncols; c += NT) acc[c] = nz;
       if (flags)
-        for (uint64_t c = threadIdx.x; c < ncols; c += PB) flag[c] = 0;
+        for (uint64_t c = threadIdx.x; c < ncols; c += NT) flag[c] = 0;
     }
   } else {
     uint32_t hc = P.row_hcap[r];
     unsigned long long goff = P.row_goff[r];
     if (P.row_gcap[r] == 0) {
       S = hc;
-      for (uint32_t i = threadIdx.x; i < S; i += PB) hkeys[i] = ColEmpty<ColT>::v;
+      for (uint32_t i = threadIdx.x; i < S; i += NT) hkeys[i] = ColEmpty<ColT>::v;
       if (PASS == PASS_NUMERIC) {
         double* hv = (double*)(acc_base + (size_t)S * sizeof(ColT));
-        for (uint32_t i = threadIdx.x; i < S; i += PB) hv[i] = 0.0;
+        for (uint32_t i = threadIdx.x; i < S; i += NT) hv[i] = 0.0;
       }
     } else {
       gkeys = (ColT*)P.gkeys + goff;
@@ -589,11 +588,11 @@ __device__ __forceinline__ void pass_item(const PP& P, uint32_t widx) {
     if (qstart >= g1) break;
     // stage STAGE raw positions, keeping (in order) those with flops in range
     const uint64_t qlim = min<uint64_t>(pend, q0 + STAGE);
-    uint32_t keep[STAGE / PB];
+    uint32_t keep[2];
     uint32_t nkeep = 0;
 #pragma unroll
-    for (int j = 0; j < STAGE / PB; ++j) {
-      const uint64_t q = q0 + (uint64_t)threadIdx.x * (STAGE / PB) + j;
+    for (int j = 0; j < 2; ++j) {
+      const uint64_t q = q0 + (uint64_t)threadIdx.x * 2 + j;
       keep[j] = 0;
       if (q < qlim) {
         const uint32_t len = P.pos_len[q];
@@ -614,16 +613,15 @@ __device__ __forceinline__ void pass_item(const PP& P, uint32_t widx) {
     if (lane == 63) s_w[wid] = incl;
     __syncthreads();
     uint32_t wbase = 0, ns = 0;
-#pragma unroll
-    for (int w = 0; w < PB / 64; ++w) {
+    for (int w = 0; w < NW; ++w) {
       if (w < wid) wbase += s_w[w];
       ns += s_w[w];
     }
     uint32_t dst = wbase + incl - nkeep;
 #pragma unroll
-    for (int j = 0; j < STAGE / PB; ++j) {
+    for (int j = 0; j < 2; ++j) {
       if (!keep[j]) continue;
-      const uint64_t q = q0 + (uint64_t)threadIdx.x * (STAGE / PB) + j;
+      const uint64_t q = q0 + (uint64_t)threadIdx.x * 2 + j;
       const unsigned long long a = P.apre[q];
       s_start[dst] = (uint32_t)((a > g0 ? a : g0) - g0);
       s_bias[dst] = (long long)P.pos_off[q] + (long long)g0 - (long long)a;
@@ -640,7 +638,7 @@ __device__ __forceinline__ void pass_item(const PP& P, uint32_t widx) {
     if (ns) {
       // each wave expands a contiguous, 64-aligned share of [fb, fe)
       const uint32_t nf = fe - fb;
-      const uint32_t span = ((nf + 64 * (PB / 64) - 1) / (64 * (PB / 64))) * 64;
+      const uint32_t span = ((nf + NT - 1) / NT) * 64;
       const uint32_t wb = fb + wid * span;
       const uint32_t we = min(fe, wb + span);
       // position of the wave's first flop (searched once, by every lane)
@@ -793,20 +791,20 @@ __device__ __forceinline__ void pass_item(const PP& P, uint32_t widx) {
       const uint64_t slab = P.row_slab[r] + k;
       if (PASS == PASS_SYMBOLIC) {
         uint32_t* gb = P.hbits + slab * nbits;
-        for (uint32_t w = threadIdx.x; w < nbits; w += PB) gb[w] = bits[w];
+        for (uint32_t w = threadIdx.x; w < nbits; w += NT) gb[w] = bits[w];
       } else if (F32) {
         float2* ga = (float2*)P.hacc + slab * ncols;
-        for (uint64_t c = threadIdx.x; c < ncols; c += PB) ga[c] = make_float2(acc_s[c], acc_c[c]);
+        for (uint64_t c = threadIdx.x; c < ncols; c += NT) ga[c] = make_float2(acc_s[c], acc_c[c]);
         if (flags) {
           unsigned char* gf = P.hflag + slab * ncols;
-          for (uint64_t c = threadIdx.x; c < ncols; c += PB) gf[c] = flag[c];
+          for (uint64_t c = threadIdx.x; c < ncols; c += NT) gf[c] = flag[c];
         }
       } else {
         double* ga = P.hacc + slab * ncols;
-        for (uint64_t c = threadIdx.x; c < ncols; c += PB) ga[c] = acc[c];
+        for (uint64_t c = threadIdx.x; c < ncols; c += NT) ga[c] = acc[c];
         if (flags) {
           unsigned char* gf = P.hflag + slab * ncols;
-          for (uint64_t c = threadIdx.x; c < ncols; c += PB) gf[c] = flag[c];
+          for (uint64_t c = threadIdx.x; c < ncols; c += NT) gf[c] = flag[c];
         }
       }
       return;
@@ -819,13 +817,13 @@ __device__ __forceinline__ void pass_item(const PP& P, uint32_t widx) {
   if (PASS == PASS_SYMBOLIC) {
     if (DENSE) {
       uint32_t c = 0;
-      for (uint32_t w = threadIdx.x; w < nbits; w += PB) c += __popc(bits[w]);
+      for (uint32_t w = threadIdx.x; w < nbits; w += NT) c += __popc(bits[w]);
       for (int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o, 64);
       if (lane == 0) s_w[wid] = c;
       __syncthreads();
       if (threadIdx.x == 0) {
         uint32_t t = 0;
-        for (int w = 0; w < PB / 64; ++w) t += s_w[w];
+        for (int w = 0; w < NW; ++w) t += s_w[w];
         P.row_nnz[r] = t;
       }
     } else if (S) {
@@ -847,7 +845,7 @@ __device__ __forceinline__ void pass_item(const PP& P, uint32_t widx) {
   const uint64_t span = DENSE ? ncols : (S ? S : (uint64_t)gmask + 1);
   // count
   uint32_t cnt = 0;
-  for (uint64_t c = threadIdx.x; c < span; c += PB) {
+  for (uint64_t c = threadIdx.x; c < span; c += NT) {
     bool f;
     if (DENSE) f = flags ? flag[c] != 0 : (F32 ? touched32(acc_s[c]) : touched(acc[c]));
     else if (S) f = hkeys[c] != ColEmpty<ColT>::v;
@@ -858,7 +856,7 @@ __device__ __forceinline__ void pass_item(const PP& P, uint32_t widx) {
   if (lane == 0) s_w[wid] = cnt;
   __syncthreads();
   uint32_t total = 0;
-  for (int w = 0; w < PB / 64; ++w) total += s_w[w];
+  for (int w = 0; w < NW; ++w) total += s_w[w];
   __syncthreads();
   uint32_t cap = P.out_cap ? P.out_cap[r] : 0xFFFFFFFFu;
   const bool full = s_misc[3] != 0;  // LDS table filled up: count unknown
@@ -877,7 +875,7 @@ __device__ __forceinline__ void pass_item(const PP& P, uint32_t widx) {
   const uint32_t limit = min(total, P.out_cap ? cap : P.row_nnz[r]);
   const uint64_t base = P.out_base[r];
   uint32_t run = 0;
-  for (uint64_t c0 = 0; c0 < span; c0 += PB) {
+  for (uint64_t c0 = 0; c0 < span; c0 += NT) {
     uint64_t c = c0 + threadIdx.x;
     bool f = false;
     ColT col = 0;
@@ -932,7 +930,7 @@ __device__ __forceinline__ void pass_item(const PP& P, uint32_t widx) {
 }
 
 template <int PASS, bool DENSE, class ColT, bool MERGE, int VAL>
-__global__ void __launch_bounds__(PB) k_pass(PassParams P) {
+__global__ void __launch_bounds__(PB_MAX) k_pass(PassParams P) {
   pass_item<PASS, DENSE, ColT, MERGE, VAL>(P, blockIdx.x);
 }
 
@@ -941,7 +939,7 @@ __global__ void __launch_bounds__(PB) k_pass(PassParams P) {
 // queue lists the heaviest work first, so the tail of the launch is short
 // items whatever the mix of job sizes.
 template <int PASS, bool DENSE, class ColT, bool MERGE, int VAL>
-__global__ void __launch_bounds__(PB) k_pass_batch(const PassParams* __restrict__ jobs, const uint2* __restrict__ queue,
+__global__ void __launch_bounds__(PB_MAX) k_pass_batch(const PassParams* __restrict__ jobs, const uint2* __restrict__ queue,
                                                    uint32_t nq, uint32_t* __restrict__ head) {
   __shared__ uint32_t s_q;
   for (;;) {
@@ -1036,8 +1034,9 @@ int launch_hash_caps(ptc_ctx* ctx, const HashCapParams& P) {
   return PTC_OK;
 }
 
-size_t pass_lds_bytes(int pass, bool dense, bool col64, uint64_t ncols, uint32_t lds_slots, bool flags) {
-  size_t b = (size_t)STAGE * 20;
+size_t pass_lds_bytes(int pass, bool dense, bool col64, uint64_t ncols, uint32_t lds_slots, bool flags,
+                      uint32_t threads) {
+  size_t b = (size_t)2 * threads * 20;  // staging: 2 positions per thread
   if (dense) {
     if (pass == PASS_SYMBOLIC) b += ((ncols + 31) / 32) * 4;
     else b += ncols * (flags ? 9 : 8);  // accumulator (+ flag byte)
@@ -1048,7 +1047,7 @@ size_t pass_lds_bytes(int pass, bool dense, bool col64, uint64_t ncols, uint32_t
 }
 
 template <int PASS, bool DENSE, class ColT, bool MERGE, int VAL = 0>
-static int launch_pass_t(ptc_ctx* ctx, const PassParams& P, uint64_t nitems, size_t lds) {
+static int launch_pass_t(ptc_ctx* ctx, const PassParams& P, uint64_t nitems, size_t lds, uint32_t threads) {
   auto kern = k_pass<PASS, DENSE, ColT, MERGE, VAL>;
   static size_t lds_set = 0;  // largest dynamic LDS this kernel was opened to
   if (lds > lds_set) {
@@ -1057,7 +1056,7 @@ static int launch_pass_t(ptc_ctx* ctx, const PassParams& P, uint64_t nitems, siz
   }
   const bool timed = ctx->nkev + 2 <= 16;
   if (timed) hipEventRecord(ctx->kev[ctx->nkev], ctx->stream);
-  hipLaunchKernelGGL(kern, dim3((unsigned)nitems), dim3(PB), lds, ctx->stream, P);
+  hipLaunchKernelGGL(kern, dim3((unsigned)nitems), dim3(threads), lds, ctx->stream, P);
   PTC_HIP(ctx, hipGetLastError());
   if (timed) {
     hipEventRecord(ctx->kev[ctx->nkev + 1], ctx->stream);
@@ -1068,46 +1067,48 @@ static int launch_pass_t(ptc_ctx* ctx, const PassParams& P, uint64_t nitems, siz
 }
 
 template <bool MERGE, int VAL>
-static int launch_numeric_v(ptc_ctx* ctx, bool dense, bool col64, const PassParams& P, uint64_t nitems, size_t lds) {
-  if (dense) return col64 ? launch_pass_t<PASS_NUMERIC, true, unsigned long long, MERGE, VAL>(ctx, P, nitems, lds)
-                          : launch_pass_t<PASS_NUMERIC, true, uint32_t, MERGE, VAL>(ctx, P, nitems, lds);
-  return col64 ? launch_pass_t<PASS_NUMERIC, false, unsigned long long, MERGE, VAL>(ctx, P, nitems, lds)
-               : launch_pass_t<PASS_NUMERIC, false, uint32_t, MERGE, VAL>(ctx, P, nitems, lds);
+static int launch_numeric_v(ptc_ctx* ctx, bool dense, bool col64, const PassParams& P, uint64_t nitems, size_t lds,
+                            uint32_t threads) {
+  if (dense) return col64 ? launch_pass_t<PASS_NUMERIC, true, unsigned long long, MERGE, VAL>(ctx, P, nitems, lds, threads)
+                          : launch_pass_t<PASS_NUMERIC, true, uint32_t, MERGE, VAL>(ctx, P, nitems, lds, threads);
+  return col64 ? launch_pass_t<PASS_NUMERIC, false, unsigned long long, MERGE, VAL>(ctx, P, nitems, lds, threads)
+               : launch_pass_t<PASS_NUMERIC, false, uint32_t, MERGE, VAL>(ctx, P, nitems, lds, threads);
 }
 
 template <bool MERGE>
 static int launch_pass_m(ptc_ctx* ctx, int pass, bool dense, bool col64, const PassParams& P, uint64_t nitems,
-                         size_t lds) {
+                         size_t lds, uint32_t threads) {
   if (pass == PASS_SYMBOLIC) {
-    if (dense) return col64 ? launch_pass_t<PASS_SYMBOLIC, true, unsigned long long, MERGE>(ctx, P, nitems, lds)
-                            : launch_pass_t<PASS_SYMBOLIC, true, uint32_t, MERGE>(ctx, P, nitems, lds);
-    return col64 ? launch_pass_t<PASS_SYMBOLIC, false, unsigned long long, MERGE>(ctx, P, nitems, lds)
-                 : launch_pass_t<PASS_SYMBOLIC, false, uint32_t, MERGE>(ctx, P, nitems, lds);
+    if (dense) return col64 ? launch_pass_t<PASS_SYMBOLIC, true, unsigned long long, MERGE>(ctx, P, nitems, lds, threads)
+                            : launch_pass_t<PASS_SYMBOLIC, true, uint32_t, MERGE>(ctx, P, nitems, lds, threads);
+    return col64 ? launch_pass_t<PASS_SYMBOLIC, false, unsigned long long, MERGE>(ctx, P, nitems, lds, threads)
+                 : launch_pass_t<PASS_SYMBOLIC, false, uint32_t, MERGE>(ctx, P, nitems, lds, threads);
   }
   switch (P.val) {
-    case 1: return launch_numeric_v<MERGE, 1>(ctx, dense, col64, P, nitems, lds);
-    case 2: return launch_numeric_v<MERGE, 2>(ctx, dense, col64, P, nitems, lds);
-    case 3: return launch_numeric_v<MERGE, 3>(ctx, dense, col64, P, nitems, lds);
+    case 1: return launch_numeric_v<MERGE, 1>(ctx, dense, col64, P, nitems, lds, threads);
+    case 2: return launch_numeric_v<MERGE, 2>(ctx, dense, col64, P, nitems, lds, threads);
+    case 3: return launch_numeric_v<MERGE, 3>(ctx, dense, col64, P, nitems, lds, threads);
     default: break;
   }
-  if (dense) return col64 ? launch_pass_t<PASS_NUMERIC, true, unsigned long long, MERGE>(ctx, P, nitems, lds)
-                          : launch_pass_t<PASS_NUMERIC, true, uint32_t, MERGE>(ctx, P, nitems, lds);
-  return col64 ? launch_pass_t<PASS_NUMERIC, false, unsigned long long, MERGE>(ctx, P, nitems, lds)
-               : launch_pass_t<PASS_NUMERIC, false, uint32_t, MERGE>(ctx, P, nitems, lds);
+  if (dense) return col64 ? launch_pass_t<PASS_NUMERIC, true, unsigned long long, MERGE>(ctx, P, nitems, lds, threads)
+                          : launch_pass_t<PASS_NUMERIC, true, uint32_t, MERGE>(ctx, P, nitems, lds, threads);
+  return col64 ? launch_pass_t<PASS_NUMERIC, false, unsigned long long, MERGE>(ctx, P, nitems, lds, threads)
+               : launch_pass_t<PASS_NUMERIC, false, uint32_t, MERGE>(ctx, P, nitems, lds, threads);
 }
 
 int launch_pass(ptc_ctx* ctx, int pass, bool dense, bool col64, const PassParams& P, uint64_t nitems,
-                size_t lds, bool merge) {
+                size_t lds, bool merge, uint32_t threads) {
   if (!nitems) return PTC_OK;
   if (nitems > 0x7FFFFFFFull) return set_err(ctx, PTC_ERR_RANGE, "too many work items");
-  return merge ? launch_pass_m<true>(ctx, pass, dense, col64, P, nitems, lds)
-               : launch_pass_m<false>(ctx, pass, dense, col64, P, nitems, lds);
+  if (threads < 64 || threads > PB_MAX || threads % 64) return set_err(ctx, PTC_ERR_ARG, "row pass threads %u", threads);
+  return merge ? launch_pass_m<true>(ctx, pass, dense, col64, P, nitems, lds, threads)
+               : launch_pass_m<false>(ctx, pass, dense, col64, P, nitems, lds, threads);
 }
 
 // head == nullptr: only open the LDS attribute and report the grid.
 template <int PASS, bool DENSE, class ColT, bool MERGE, int VAL>
 static int launch_batch_t(ptc_ctx* ctx, const PassParams* jobs, const uint2* queue, uint32_t nq, uint32_t* head,
-                          size_t lds, unsigned grid, unsigned* grid_out = nullptr) {
+                          size_t lds, unsigned grid, unsigned* grid_out, uint32_t threads) {
   auto kern = k_pass_batch<PASS, DENSE, ColT, MERGE, VAL>;
   static size_t lds_set = 0;
   if (lds > lds_set) {
@@ -1116,21 +1117,21 @@ static int launch_batch_t(ptc_ctx* ctx, const PassParams* jobs, const uint2* que
   }
   if (!grid) {
     int per_cu = 0;
-    PTC_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, PB, lds));
+    PTC_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, threads, lds));
     grid = (unsigned)std::max(1, per_cu) * (unsigned)std::max(1, ctx->prop.multiProcessorCount);
   }
   if (grid_out) *grid_out = grid;
   if (!head) return PTC_OK;
   grid = std::min<unsigned>(grid, nq);
-  hipLaunchKernelGGL(kern, dim3(grid), dim3(PB), lds, ctx->stream, jobs, queue, nq, head);
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(threads), lds, ctx->stream, jobs, queue, nq, head);
   PTC_HIP(ctx, hipGetLastError());
   return PTC_OK;
 }
 
 template <bool MERGE, int VAL>
 static int launch_batch_v(ptc_ctx* ctx, int pass, bool dense, bool col64, const PassParams* jobs, const uint2* queue,
-                          uint32_t nq, uint32_t* head, size_t lds, unsigned grid, unsigned* gout) {
-#define PTC_LB(P_, D_, C_, V_) launch_batch_t<P_, D_, C_, MERGE, V_>(ctx, jobs, queue, nq, head, lds, grid, gout)
+                          uint32_t nq, uint32_t* head, size_t lds, unsigned grid, unsigned* gout, uint32_t threads) {
+#define PTC_LB(P_, D_, C_, V_) launch_batch_t<P_, D_, C_, MERGE, V_>(ctx, jobs, queue, nq, head, lds, grid, gout, threads)
   if (pass == PASS_SYMBOLIC) {
     if (dense) return col64 ? PTC_LB(PASS_SYMBOLIC, true, unsigned long long, 0) : PTC_LB(PASS_SYMBOLIC, true, uint32_t, 0);
     return col64 ? PTC_LB(PASS_SYMBOLIC, false, unsigned long long, 0) : PTC_LB(PASS_SYMBOLIC, false, uint32_t, 0);
@@ -1141,19 +1142,20 @@ static int launch_batch_v(ptc_ctx* ctx, int pass, bool dense, bool col64, const 
 }
 
 int launch_pass_batch(ptc_ctx* ctx, int pass, bool dense, bool col64, int val, bool merge, const PassParams* jobs,
-                      const uint2* queue, uint32_t nq, uint32_t* head, size_t lds, unsigned grid) {
+                      const uint2* queue, uint32_t nq, uint32_t* head, size_t lds, unsigned grid, uint32_t threads) {
   if (!nq) return PTC_OK;
   if (pass == PASS_SYMBOLIC) val = 0;
   if (val != 0) return set_err(ctx, PTC_ERR_UNSUPPORTED, "batched contractions compute in FP64");
-  return merge ? launch_batch_v<true, 0>(ctx, pass, dense, col64, jobs, queue, nq, head, lds, grid, nullptr)
-               : launch_batch_v<false, 0>(ctx, pass, dense, col64, jobs, queue, nq, head, lds, grid, nullptr);
+  return merge ? launch_batch_v<true, 0>(ctx, pass, dense, col64, jobs, queue, nq, head, lds, grid, nullptr, threads)
+               : launch_batch_v<false, 0>(ctx, pass, dense, col64, jobs, queue, nq, head, lds, grid, nullptr, threads);
 }
 
-int pass_batch_grid(ptc_ctx* ctx, int pass, bool dense, bool col64, int val, bool merge, size_t lds, unsigned* grid) {
+int pass_batch_grid(ptc_ctx* ctx, int pass, bool dense, bool col64, int val, bool merge, size_t lds, unsigned* grid,
+                    uint32_t threads) {
   if (pass == PASS_SYMBOLIC) val = 0;
   if (val != 0) return set_err(ctx, PTC_ERR_UNSUPPORTED, "batched contractions compute in FP64");
-  return merge ? launch_batch_v<true, 0>(ctx, pass, dense, col64, nullptr, nullptr, 0, nullptr, lds, 0, grid)
-               : launch_batch_v<false, 0>(ctx, pass, dense, col64, nullptr, nullptr, 0, nullptr, lds, 0, grid);
+  return merge ? launch_batch_v<true, 0>(ctx, pass, dense, col64, nullptr, nullptr, 0, nullptr, lds, 0, grid, threads)
+               : launch_batch_v<false, 0>(ctx, pass, dense, col64, nullptr, nullptr, 0, nullptr, lds, 0, grid, threads);
 }
 
 int launch_emit(ptc_ctx* ctx, const EmitParams& P, bool col64) {

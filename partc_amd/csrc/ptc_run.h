@@ -13,7 +13,8 @@ namespace ptc {
 
 constexpr uint32_t DEFAULT_DENSE_MAX = 8192;
 constexpr uint32_t DEFAULT_LDS_HASH = 2048;
-constexpr uint64_t DEFAULT_CHUNK = 32768;
+constexpr uint64_t DEFAULT_CHUNK = 32768;         // multiply-adds per work item, hash accumulators
+constexpr uint64_t DEFAULT_DENSE_CHUNK = 65536;   // floor for dense accumulators
 
 // Hash accumulator plan of every row: LDS slots or an HBM region.
 struct HashPlan {
@@ -53,6 +54,7 @@ struct Run {
   unsigned long long* out_off = nullptr;
   bool want_features = false;  // per-row hit counts and longest slices (learned features)
   const uint32_t* row_of_pos = nullptr;  // HT_A's slice index of every A position
+  uint32_t threads = PB;       // threads per row-pass workgroup
   bool f32 = false;            // FP32 values (both tables carry gval32)
   int val = PTC_VAL_F64;       // ptc_value_type of the numeric pass
   explicit Run(ptc_ctx* c) : ctx(c), ws(c) {}

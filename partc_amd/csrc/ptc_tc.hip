@@ -191,8 +191,8 @@ static int run_pass(Run& R, int pass, HashPlan* hp, const uint4* items, uint64_t
     P.row_gcap = flags;
   }
   P.items = items;
-  size_t lds = pass_lds_bytes(pass, R.dense, R.col64, R.ncols, R.lds_slots, R.use_flags);
-  int rc = launch_pass(R.ctx, pass, R.dense, R.col64, P, nitems, lds, false);
+  size_t lds = pass_lds_bytes(pass, R.dense, R.col64, R.ncols, R.lds_slots, R.use_flags, R.threads);
+  int rc = launch_pass(R.ctx, pass, R.dense, R.col64, P, nitems, lds, false, R.threads);
   if (rc) return rc;
   if (R.dense) {
     if (!merge_rows) {
@@ -200,11 +200,11 @@ static int run_pass(Run& R, int pass, HashPlan* hp, const uint4* items, uint64_t
       nmerge = R.nheavy;
     }
     P.merge_rows = merge_rows;
-    return launch_pass(R.ctx, pass, true, R.col64, P, nmerge, lds, true);
+    return launch_pass(R.ctx, pass, true, R.col64, P, nmerge, lds, true, R.threads);
   }
   if (pass == PASS_NUMERIC && hp->pool) {
     P.merge_rows = merge_rows;  // NULL: every row (rows not in HBM exit at once)
-    return launch_pass(R.ctx, pass, false, R.col64, P, merge_rows ? nmerge : R.nrows, lds, true);
+    return launch_pass(R.ctx, pass, false, R.col64, P, merge_rows ? nmerge : R.nrows, lds, true, R.threads);
   }
   return PTC_OK;
 }
@@ -324,10 +324,26 @@ int prepare_launch(Run& R, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B,
     R.lds_slots = R.lds_slots ? p2 : 0;
     while (R.lds_slots > 16 && pass_lds_bytes(PASS_NUMERIC, false, R.col64, 0, R.lds_slots) > lds_cap) R.lds_slots >>= 1;
   }
-  // work per workgroup: large enough that a split row's slab (one dense row
-  // per chunk) stays a small fraction of the chunk's streamed bytes
+  // Threads per row-pass workgroup.  The accumulator's LDS is paid once per
+  // workgroup, so when a 512-thread workgroup's footprint leaves room for
+  // fewer than two per CU, 1024 threads keep twice the waves in flight on
+  // the same LDS.
+  {
+    const size_t at512 = pass_lds_bytes(PASS_NUMERIC, R.dense, R.col64, R.ncols, R.lds_slots, R.use_flags, 512);
+    uint32_t t = opt.block_threads ? opt.block_threads : (2 * at512 > lds_cap ? 1024u : 512u);
+    if (t < 64 || t > (uint32_t)PB_MAX || t % 64) return set_err(ctx, PTC_ERR_ARG, "block_threads %u", t);
+    while (t > 512 && pass_lds_bytes(PASS_NUMERIC, R.dense, R.col64, R.ncols, R.lds_slots, R.use_flags, t) > lds_cap)
+      t /= 2;
+    R.threads = t;
+  }
+  // Work per workgroup.  Dense: a split row writes one dense slab (8 bytes a
+  // column) per chunk, so chunks of at least 24 x columns multiply-adds
+  // (12 bytes streamed each) keep the slabs within a few percent of the
+  // streamed bytes; 65536 is the floor (measured on the nips-shape and
+  // two-tensor workloads).  Hash rows keep smaller chunks: their HBM
+  // accumulators are shared by all chunks of a row.
   const uint64_t chunk = opt.chunk_flops ? opt.chunk_flops
-                                         : (R.dense ? std::max<uint64_t>(DEFAULT_CHUNK, 8 * R.ncols) : DEFAULT_CHUNK);
+                                         : (R.dense ? std::max<uint64_t>(DEFAULT_DENSE_CHUNK, 24 * R.ncols) : DEFAULT_CHUNK);
 
   int rc = PTC_OK;
   const uint32_t* row_of_pos = nullptr;  // HT_A's slice index of every A position
