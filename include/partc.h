@@ -108,6 +108,30 @@ int ptc_dtensor_download_f32(ptc_ctx* ctx, const ptc_dtensor* t, float* vals, fl
 int ptc_dtensor_set_values(ptc_ctx* ctx, ptc_dtensor* t, const double* vals);
 void ptc_dtensor_free(ptc_dtensor* t);
 
+/* ------------------------------------------------------------ file readers */
+
+/* A COO tensor in host memory allocated by the library (free with
+ * ptc_host_tensor_free).  Coordinates 0-based, nnz x d row-major. */
+typedef struct {
+  uint32_t d;
+  uint64_t N;
+  uint32_t* hedges;
+  double* vals;
+  uint32_t dims[PTC_MAX_MODES];
+} ptc_host_tensor;
+
+/* FROSTT .tns: d 1-based coordinates and a value per line, d fixed by the
+ * first data line; dims = largest label per mode.  Replaces readTensor
+ * (mtxTnsIO.cpp:31-63 over pigo::Tensor, pigo.hpp:4363-4456). */
+int ptc_read_tns(const char* path, ptc_host_tensor* out);
+/* Matrix Market coordinate file: d = 2, dims = the header's rows and
+ * columns, entries must number the header's nnz.  Replaces readMatrix
+ * (mtxTnsIO.cpp:5-29 over pigo::COO, pigo.hpp:3149-3203). */
+int ptc_read_mtx(const char* path, ptc_host_tensor* out);
+void ptc_host_tensor_free(ptc_host_tensor* t);
+/* Write N nonzeros as .tns (1-based, values to 17 significant digits). */
+int ptc_write_tns(const char* path, uint32_t d, uint64_t N, const uint32_t* hedges, const double* vals);
+
 /* ------------------------------------------------------------ .meta spec */
 
 /* Parse a .meta file: two free-text lines, then c, then d mode ids whose

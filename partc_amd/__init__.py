@@ -127,6 +127,16 @@ class TuneReport(C.Structure):
         return d
 
 
+class HostTensor(C.Structure):
+    _fields_ = [
+        ("d", C.c_uint32),
+        ("N", C.c_uint64),
+        ("hedges", C.POINTER(C.c_uint32)),
+        ("vals", C.POINTER(C.c_double)),
+        ("dims", C.c_uint32 * 8),
+    ]
+
+
 class Stats(C.Structure):
     _fields_ = [
         ("nnz_a", C.c_uint64),
@@ -192,6 +202,10 @@ def lib() -> C.CDLL:
         "ptc_matricize_sort_spGEMM": (i32, [vp, vp, vp, C.POINTER(vp), C.POINTER(Stats)]),
         "ptc_gen_coo": (i32, [u64, u32, P32, u64, i32, C.c_double, P32, PD]),
         "ptc_dtensor_set_values": (i32, [vp, vp, PD]),
+        "ptc_read_tns": (i32, [C.c_char_p, C.POINTER(HostTensor)]),
+        "ptc_read_mtx": (i32, [C.c_char_p, C.POINTER(HostTensor)]),
+        "ptc_host_tensor_free": (None, [C.POINTER(HostTensor)]),
+        "ptc_write_tns": (i32, [C.c_char_p, u32, u64, P32, PD]),
         "ptc_batch_plan": (i32, [vp, i32, C.POINTER(vp), C.POINTER(vp), C.POINTER(Options), C.POINTER(vp),
                                  C.POINTER(BatchStats)]),
         "ptc_batch_run": (i32, [vp, C.POINTER(BatchStats)]),
@@ -270,6 +284,12 @@ class Tensor:
         self.num_key_dimensions = int(c.value)
         self.dim_tc = perm[: self.d].copy()
 
+    def write_tns(self, path) -> None:
+        """FROSTT .tns text, 1-based coordinates (ptc_write_tns)."""
+        rc = lib().ptc_write_tns(str(path).encode(), self.d, self.N, _p32(self.hedges_array), _pd(self.val_array))
+        if rc != PTC_OK:
+            raise PtcError(rc, f"cannot write {path}")
+
     def view(self) -> TensorView:
         return TensorView(self.d, int(self.num_key_dimensions), self.N, _p32(self.hedges_array),
                           _p32(self.dimensions_array), _pd(self.val_array), _p32(self.dim_tc))
@@ -282,6 +302,32 @@ class Tensor:
         order = np.lexsort(self.hedges_array.T[::-1])
         return Tensor(self.hedges_array[order], self.val_array[order], self.dimensions_array, self.dim_tc,
                       self.num_key_dimensions)
+
+
+def _read_host(fn, path) -> Tensor:
+    t = HostTensor()
+    rc = fn(str(path).encode(), C.byref(t))
+    if rc != PTC_OK:
+        raise PtcError(rc, f"cannot read {path}")
+    try:
+        n, d = int(t.N), int(t.d)
+        h = np.ctypeslib.as_array(t.hedges, shape=(n * d,)).reshape(n, d).copy() if n else np.zeros((0, d), np.uint32)
+        v = np.ctypeslib.as_array(t.vals, shape=(n,)).copy() if n else np.zeros(0)
+        dims = np.array(t.dims[:d], dtype=np.uint32)
+    finally:
+        lib().ptc_host_tensor_free(C.byref(t))
+    return Tensor(h, v, dims)
+
+
+def read_tns(path) -> Tensor:
+    """A FROSTT .tns file (readTensor, mtxTnsIO.cpp:31-63): 0-based
+    coordinates, dims = largest label per mode."""
+    return _read_host(lib().ptc_read_tns, path)
+
+
+def read_mtx(path) -> Tensor:
+    """A Matrix Market coordinate file (readMatrix, mtxTnsIO.cpp:5-29)."""
+    return _read_host(lib().ptc_read_mtx, path)
 
 
 def make_free_first(A: Tensor) -> Tensor:

@@ -80,6 +80,23 @@ def build_lib(verbose: bool = True) -> Path:
     return LIB
 
 
+CLI_SRC = PKG / "cli" / "partc_tc.cpp"
+CLI = PKG / "partc-tc"
+
+
+def build_cli(verbose: bool = True) -> Path:
+    """The command-line driver (the reference's main.cpp interface), a host
+    program over the C ABI, linked against libpartc.so."""
+    cmd = ["g++", "-O2", "-std=c++17", f"-I{ROOT / 'include'}", str(CLI_SRC), f"-L{PKG}", "-lpartc",
+           "-Wl,-rpath,$ORIGIN", "-o", str(CLI)]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"cli build failed\n{' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
+    if verbose:
+        print(f"built {CLI}")
+    return CLI
+
+
 def build_oracle(verbose: bool = True) -> None:
     """The checker: oracle/liboracle.so and, where the reference tree is
     present, oracle/_ref/libcxsparse.so."""
@@ -92,4 +109,5 @@ def build_oracle(verbose: bool = True) -> None:
 
 if __name__ == "__main__":
     build_lib()
+    build_cli()
     build_oracle()
