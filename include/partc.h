@@ -160,6 +160,15 @@ int ptc_hashtable_probe(ptc_ctx* ctx, const ptc_hashtable* h, uint64_t n, const 
                         uint64_t* slice_len, uint64_t* slice_off, uint32_t* slice_ids,
                         uint64_t cap);
 void ptc_hashtable_free(ptc_hashtable* h);
+/* The same table with a chosen grouping: PTC_GROUP_HASH (atomicCAS insert
+ * into an open-addressing table, ranks by wave-aggregated atomics),
+ * PTC_GROUP_SORT (stable LSD radix sort of the keys, slices from runs of
+ * equal keys, the probe index over distinct keys only; key spaces within 32
+ * bits) or PTC_GROUP_AUTO.  Sorting is the sort-based matricization of the
+ * reference (matricizedspGEMM_sorting.hpp:194-754, which exits before
+ * finishing) applied to the grouping step. */
+enum { PTC_GROUP_AUTO = 0, PTC_GROUP_HASH = 1, PTC_GROUP_SORT = 2 };
+int ptc_hashtable_build_ex(ptc_ctx* ctx, const ptc_dtensor* t, int grouping, ptc_hashtable** out);
 
 /* ------------------------------------------------------------ contraction */
 
@@ -243,7 +252,8 @@ int ptc_contract(ptc_ctx* ctx, const ptc_dtensor* A, const ptc_dtensor* B,
 
 /* matricize_spGEMM (matricizedspGEMM.hpp:172) and matricize_sort_spGEMM
  * (matricizedspGEMM_sorting.hpp:194; the reference exits before computing
- * it, :754): both tensors contracted-first; same result as ptc_contract. */
+ * it, :754): both tensors contracted-first; same result as ptc_contract.
+ * The sorting variant groups both operands by radix sort (PTC_GROUP_SORT). */
 int ptc_matricize_spGEMM(ptc_ctx* ctx, const ptc_dtensor* A, const ptc_dtensor* B,
                          ptc_dtensor** O, ptc_stats* stats);
 int ptc_matricize_sort_spGEMM(ptc_ctx* ctx, const ptc_dtensor* A, const ptc_dtensor* B,

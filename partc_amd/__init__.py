@@ -18,6 +18,7 @@ no CPU fallback.  A missing library or device raises.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
 import weakref
@@ -48,6 +49,7 @@ METHOD_TWO_PASS, METHOD_LEARNED, METHOD_UPPER_BOUND = 0, 1, 2
 VAL_F64, VAL_F32_TWOSUM, VAL_F32_PLAIN, VAL_F32_F64ACC = 0, 1, 2, 3
 ACC_AUTO, ACC_DENSE, ACC_HASH, ACC_SPILL = 0, 1, 2, 3
 DIST_UNIFORM, DIST_ZIPF = 0, 1
+GROUP_AUTO, GROUP_HASH, GROUP_SORT = 0, 1, 2
 
 
 class PtcError(RuntimeError):
@@ -192,6 +194,7 @@ def lib() -> C.CDLL:
         "ptc_dtensor_free": (None, [vp]),
         "ptc_meta_parse": (i32, [C.c_char_p, u32, P32, P32]),
         "ptc_hashtable_build": (i32, [vp, vp, C.POINTER(vp)]),
+        "ptc_hashtable_build_ex": (i32, [vp, vp, i32, C.POINTER(vp)]),
         "ptc_hashtable_info": (i32, [vp, P64, P64]),
         "ptc_hashtable_probe": (i32, [vp, vp, u64, P32, P64, P64, P32, u64]),
         "ptc_hashtable_free": (None, [vp]),
@@ -352,6 +355,22 @@ def _check(rc: int, ctx: Optional["Context"] = None) -> None:
         raise PtcError(rc, msg)
 
 
+_live_contexts: "weakref.WeakSet[Context]" = weakref.WeakSet()
+
+
+def _close_live_contexts() -> None:
+    """At interpreter exit, before the HIP runtime's own exit handlers: free
+    every context's device objects and streams while the runtime is whole."""
+    for c in list(_live_contexts):
+        try:
+            c.close()
+        except Exception:
+            pass
+
+
+atexit.register(_close_live_contexts)
+
+
 class Context:
     """One device and one HIP stream (ptc_create)."""
 
@@ -362,6 +381,7 @@ class Context:
         rc = lib().ptc_create(C.byref(self._h), device)
         if rc != PTC_OK:
             raise PtcError(rc, "ptc_create")
+        _live_contexts.add(self)
 
     def adopt(self, obj):
         self._children.add(obj)
@@ -532,13 +552,14 @@ def _as_device(t, ctx: Context) -> DeviceTensor:
     return t if isinstance(t, DeviceTensor) else DeviceTensor.upload(t, ctx)
 
 
-def buildHashTable(T, ctx: Optional[Context] = None) -> HashTable:
+def buildHashTable(T, ctx: Optional[Context] = None, grouping: int = GROUP_AUTO) -> HashTable:
     """Group T's nonzeros by its key modes (T.dim_tc[:num_key_dimensions])
-    on the device (fksCuckoo.hpp:1044-1060 with reorder_tensor)."""
+    on the device (fksCuckoo.hpp:1044-1060 with reorder_tensor); grouping
+    GROUP_HASH or GROUP_SORT picks the method (ptc_hashtable_build_ex)."""
     ctx = ctx or default_context()
     dT = _as_device(T, ctx)
     h = C.c_void_p()
-    _check(lib().ptc_hashtable_build(ctx.handle, dT._h, C.byref(h)), ctx)
+    _check(lib().ptc_hashtable_build_ex(ctx.handle, dT._h, grouping, C.byref(h)), ctx)
     return HashTable(ctx, h, dT)
 
 

@@ -32,6 +32,12 @@
 namespace ptc {
 
 constexpr int KT_THREADS = 256;
+// Automatic grouping sorts key spaces of at most this many bits (three
+// 8-bit radix passes); wider ones hash, where one insert per nonzero beats a
+// fourth pass over keys and ids (measured on MI355X: 30-bit keys of the
+// nips-shape B table hash in 0.52 ms, sort in 0.68 ms; the 20-bit two-tensor
+// B table sorts in 0.56 ms, hashes in 0.98 ms).
+constexpr int SORT_MAX_BITS = 24;
 
 __device__ __forceinline__ unsigned long long mix64(unsigned long long k) {
   // MurmurHash3 fmix64 finaliser
@@ -456,6 +462,161 @@ __global__ void k_kt_gather(const uint32_t* __restrict__ h, const ValT* __restri
   }
 }
 
+
+// ------------------------------------------------------------ sort grouping
+//
+// The same table built by sorting instead of hashing: a stable LSD radix sort
+// of (32-bit key, id) pairs with 8-bit digits, then slices from the runs of
+// equal keys, then the probe index over the distinct keys only.  No global
+// atomics: digit counts are per tile (4096 nonzeros) in LDS, offsets come
+// from one scan of the digit-major tile histograms, and ranks inside a tile
+// come from matching equal digits across the lanes of a wavefront.
+
+constexpr int RS_THREADS = 256;
+constexpr int RS_ITEMS = 16;
+constexpr int RS_TILE = RS_THREADS * RS_ITEMS;
+constexpr int RS_WAVES = RS_THREADS / 64;
+
+// Lanes of the wavefront holding the same 8-bit digit (0 for invalid lanes).
+__device__ __forceinline__ unsigned long long match_digit(uint32_t dg, bool valid) {
+  unsigned long long m = __ballot(valid);
+#pragma unroll
+  for (int b = 0; b < 8; ++b) {
+    const bool bit = ((dg >> b) & 1u) != 0;
+    const unsigned long long bb = __ballot(bit);
+    m &= bit ? bb : ~bb;
+  }
+  return valid ? m : 0ull;
+}
+
+// keys, ids = 0..N-1, and the input-order payload (non-key tuple, value)
+template <class RestT, class ValT>
+__global__ void k_rs_keys(const uint32_t* __restrict__ h, uint32_t d, uint64_t N, KeyCodec codec, KeyCodec rest,
+                          const ValT* __restrict__ val, uint32_t* __restrict__ keys, uint32_t* __restrict__ ids,
+                          KtPay<RestT, ValT>* __restrict__ pay) {
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < N; e += (uint64_t)gridDim.x * blockDim.x) {
+    unsigned long long key = 0;
+    encode_key(h, d, e, codec, &key);  // the table's own tensor: always within radix
+    keys[e] = (uint32_t)key;
+    ids[e] = (uint32_t)e;
+    unsigned long long rk = 0;
+    for (uint32_t j = 0; j < rest.n; ++j) rk += (unsigned long long)h[e * d + rest.mode[j]] * rest.stride[j];
+    KtPay<RestT, ValT> pl;
+    pl.k = (RestT)rk;
+    pl.v = val[e];
+    pay[e] = pl;
+  }
+}
+
+// digit counts of one tile -> hist[digit * ntiles + tile]
+__global__ void __launch_bounds__(RS_THREADS) k_rs_hist(const uint32_t* __restrict__ keys, uint64_t N, int shift,
+                                                         uint64_t ntiles, uint32_t* __restrict__ hist) {
+  __shared__ uint32_t lh[256];
+  const int lane = threadIdx.x & 63;
+  lh[threadIdx.x] = 0;
+  __syncthreads();
+  const uint64_t base = (uint64_t)blockIdx.x * RS_TILE;
+  for (int r = 0; r < RS_ITEMS; ++r) {
+    const uint64_t i = base + (uint64_t)r * RS_THREADS + threadIdx.x;
+    const bool valid = i < N;
+    const uint32_t dg = valid ? (keys[i] >> shift) & 255u : 0u;
+    const unsigned long long m = match_digit(dg, valid);
+    if (valid && __ffsll((long long)m) - 1 == lane) atomicAdd(&lh[dg], (uint32_t)__popcll(m));
+  }
+  __syncthreads();
+  hist[(uint64_t)threadIdx.x * ntiles + blockIdx.x] = lh[threadIdx.x];
+}
+
+// Stable scatter of one tile: round r handles elements base + r*256 + t, in
+// input order; an element's position is its (digit, tile) offset + the
+// count of its digit in earlier rounds + in earlier waves of this round +
+// its rank among the equal-digit lanes of its wave.
+__global__ void __launch_bounds__(RS_THREADS) k_rs_scatter(const uint32_t* __restrict__ kin,
+                                                            const uint32_t* __restrict__ iin, uint64_t N, int shift,
+                                                            uint64_t ntiles, const unsigned long long* __restrict__ hs,
+                                                            uint32_t* __restrict__ kout, uint32_t* __restrict__ iout) {
+  __shared__ unsigned long long next[256];
+  __shared__ uint32_t wcnt[RS_WAVES][256];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  next[threadIdx.x] = hs[(uint64_t)threadIdx.x * ntiles + blockIdx.x];
+  for (int ww = 0; ww < RS_WAVES; ++ww) wcnt[ww][threadIdx.x] = 0;
+  __syncthreads();
+  const uint64_t base = (uint64_t)blockIdx.x * RS_TILE;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  for (int r = 0; r < RS_ITEMS; ++r) {
+    const uint64_t i = base + (uint64_t)r * RS_THREADS + threadIdx.x;
+    const bool valid = i < N;
+    const uint32_t k = valid ? kin[i] : 0u, id = valid ? iin[i] : 0u;
+    const uint32_t dg = (k >> shift) & 255u;
+    const unsigned long long m = match_digit(dg, valid);
+    if (valid && __ffsll((long long)m) - 1 == lane) wcnt[w][dg] = (uint32_t)__popcll(m);
+    __syncthreads();
+    if (valid) {
+      unsigned long long pos = next[dg] + (unsigned long long)__popcll(m & below);
+      for (int ww = 0; ww < w; ++ww) pos += wcnt[ww][dg];
+      kout[pos] = k;
+      iout[pos] = id;
+    }
+    __syncthreads();
+    uint32_t tot = 0;
+    for (int ww = 0; ww < RS_WAVES; ++ww) {
+      tot += wcnt[ww][threadIdx.x];
+      wcnt[ww][threadIdx.x] = 0;
+    }
+    next[threadIdx.x] += tot;
+    __syncthreads();
+  }
+}
+
+__global__ void k_rs_heads(const uint32_t* __restrict__ K, uint64_t N, uint32_t* __restrict__ head) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (uint64_t)gridDim.x * blockDim.x)
+    head[i] = (i == 0 || K[i] != K[i - 1]) ? 1u : 0u;
+}
+
+// slice of every sorted position; heads write (key, off), tails write the
+// end into len (fixed to a length by k_rs_index)
+__global__ void k_rs_slices(const uint32_t* __restrict__ K, uint64_t N, const uint32_t* __restrict__ head,
+                            const unsigned long long* __restrict__ sx, KtSlice* __restrict__ slices,
+                            uint32_t* __restrict__ slice_of) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t sl = sx[i] + head[i] - 1;
+    slice_of[i] = (uint32_t)sl;
+    if (head[i]) {
+      slices[sl].key = K[i];
+      slices[sl].off = (uint32_t)i;
+    }
+    if (i + 1 == N || head[i + 1]) slices[sl].len = (uint32_t)(i + 1);
+  }
+}
+
+// Slice lengths, then the probe index: direct tables write entry[key];
+// hashed tables insert every distinct key once (no duplicates to resolve).
+__global__ void k_rs_index(KtSlice* __restrict__ slices, const unsigned long long* __restrict__ nkeys, KtEntry* __restrict__ tab,
+                           uint64_t cap, int direct, unsigned long long* __restrict__ total, uint64_t N) {
+  const uint64_t nk = *nkeys;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *total = (nk << 32) | N;
+  const uint64_t nlines = cap / 8;
+  for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < nk; s += (uint64_t)gridDim.x * blockDim.x) {
+    KtSlice sl = slices[s];
+    sl.len = sl.len - sl.off;
+    slices[s].len = sl.len;
+    uint64_t i;
+    if (direct) {
+      i = sl.key;
+      tab[i].key = sl.key;
+    } else {
+      i = (mix64(sl.key) & (nlines - 1)) * 8;
+      while (true) {
+        const unsigned long long old = atomicCAS(&tab[i].key, PTC_EMPTY_KEY, sl.key);
+        if (old == PTC_EMPTY_KEY) break;  // keys are distinct: a claimed entry is never this key's
+        i = (i + 1) & (cap - 1);
+      }
+    }
+    tab[i].off = sl.off;
+    tab[i].len = sl.len;
+  }
+}
+
 static unsigned grid_for(uint64_t n, unsigned threads, unsigned cap = 8192) {
   uint64_t g = (n + threads - 1) / threads;
   if (g < 1) g = 1;
@@ -500,9 +661,90 @@ __global__ void k_kt_probe(const uint32_t* __restrict__ q, uint64_t n, KeyCodec 
 
 using namespace ptc;
 
+namespace {
+
+// Sort grouping of a table whose arrays (entries, perm, gval/gval32,
+// rest_key32/64) are allocated; N > 0 and the key space fits 32 bits.
+int build_sorted(ptc_ctx* ctx, ptc_hashtable* ht, const ptc_dtensor* t, bool f32, bool rest64) {
+  const uint64_t N = t->N;
+  hipStream_t st = ctx->stream;
+  Workspace ws(ctx);
+  const uint64_t ntiles = (N + RS_TILE - 1) / RS_TILE;
+  uint32_t *k0 = ws.get<uint32_t>(N), *i0 = ws.get<uint32_t>(N), *k1 = ws.get<uint32_t>(N), *i1 = ws.get<uint32_t>(N);
+  uint32_t* hist = ws.get<uint32_t>(256 * ntiles);
+  unsigned long long* hs = ws.get<unsigned long long>(256 * ntiles + 1);
+  uint32_t* head = ws.get<uint32_t>(N);
+  unsigned long long* sx = ws.get<unsigned long long>(N + 1);
+  const size_t pay_bytes = rest64 || !f32 ? 16 : 8;
+  void* pay = ws.get<unsigned char>(N * pay_bytes);
+  ht->slice_of = dalloc_n<uint32_t>(ctx, N);
+  ht->d_total = dalloc_n<unsigned long long>(ctx, 1);
+  ht->slices_cap = std::max<uint64_t>(1, std::min<uint64_t>(N, ht->cap));
+  ht->slices = dalloc_n<KtSlice>(ctx, ht->slices_cap);
+  if (!k0 || !i0 || !k1 || !i1 || !hist || !hs || !head || !sx || !pay || !ht->slice_of || !ht->d_total || !ht->slices)
+    return PTC_ERR_NOMEM;
+  const unsigned g = grid_for(N, 256, 16384);
+#define PTC_RSKEYS(RT, VT, VALS)                                                                             \
+  hipLaunchKernelGGL((k_rs_keys<RT, VT>), dim3(g), dim3(256), 0, st, t->hedges, t->d, N, ht->codec, ht->rest, \
+                     (const VT*)(VALS), k0, i0, (KtPay<RT, VT>*)pay)
+  if (rest64) {
+    if (f32) PTC_RSKEYS(unsigned long long, float, t->vals32);
+    else PTC_RSKEYS(unsigned long long, double, t->vals);
+  } else {
+    if (f32) PTC_RSKEYS(uint32_t, float, t->vals32);
+    else PTC_RSKEYS(uint32_t, double, t->vals);
+  }
+#undef PTC_RSKEYS
+  // LSD passes over the significant bits of the key space
+  int bits = 1;
+  while (bits < 32 && (1ull << bits) < ht->codec.space) ++bits;
+  for (int shift = 0; shift < bits; shift += 8) {
+    hipLaunchKernelGGL(k_rs_hist, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, st, (const uint32_t*)k0, N, shift,
+                       ntiles, hist);
+    int rc = scan_u32_to_u64(ctx, hist, hs, 256 * ntiles, hs + 256 * ntiles);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_rs_scatter, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, st, (const uint32_t*)k0,
+                       (const uint32_t*)i0, N, shift, ntiles, (const unsigned long long*)hs, k1, i1);
+    std::swap(k0, k1);
+    std::swap(i0, i1);
+  }
+  // slices from runs of equal keys
+  hipLaunchKernelGGL(k_rs_heads, dim3(g), dim3(256), 0, st, (const uint32_t*)k0, N, head);
+  int rc = scan_u32_to_u64(ctx, head, sx, N, sx + N);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_rs_slices, dim3(g), dim3(256), 0, st, (const uint32_t*)k0, N, (const uint32_t*)head,
+                     (const unsigned long long*)sx, ht->slices, ht->slice_of);
+  hipLaunchKernelGGL(k_kt_clear, dim3(grid_for(ht->cap, 256)), dim3(256), 0, st, ht->entries, (uint32_t*)nullptr,
+                     ht->cap);
+  hipLaunchKernelGGL(k_rs_index, dim3(grid_for(ht->slices_cap, 256, 16384)), dim3(256), 0, st, ht->slices,
+                     (const unsigned long long*)(sx + N), ht->entries, ht->cap, ht->direct ? 1 : 0, ht->d_total, N);
+  // ids in grouped order, then the streaming layout from the payload
+  PTC_HIP(ctx, hipMemcpyAsync(ht->perm, i0, sizeof(uint32_t) * N, hipMemcpyDeviceToDevice, st));
+#define PTC_GATHER(RT, VT, RK, GV)                                                                         \
+  hipLaunchKernelGGL((k_kt_gather_pay<RT, VT>), dim3(g), dim3(256), 0, st, N, (const uint32_t*)ht->perm, \
+                     (const KtPay<RT, VT>*)pay, RK, GV)
+  if (rest64) {
+    if (f32) PTC_GATHER(unsigned long long, float, ht->rest_key64, ht->gval32);
+    else PTC_GATHER(unsigned long long, double, ht->rest_key64, ht->gval);
+  } else {
+    if (f32) PTC_GATHER(uint32_t, float, ht->rest_key32, ht->gval32);
+    else PTC_GATHER(uint32_t, double, ht->rest_key32, ht->gval);
+  }
+#undef PTC_GATHER
+  PTC_HIP(ctx, hipGetLastError());
+  return PTC_OK;
+}
+
+}  // namespace
+
 extern "C" int ptc_hashtable_build(ptc_ctx* ctx, const ptc_dtensor* t, ptc_hashtable** out) {
+  return ptc_hashtable_build_ex(ctx, t, PTC_GROUP_AUTO, out);
+}
+
+extern "C" int ptc_hashtable_build_ex(ptc_ctx* ctx, const ptc_dtensor* t, int grouping, ptc_hashtable** out) {
   if (!ctx || !t || !out) return PTC_ERR_ARG;
   *out = nullptr;
+  if (grouping < PTC_GROUP_AUTO || grouping > PTC_GROUP_SORT) return set_err(ctx, PTC_ERR_ARG, "grouping %d", grouping);
   PTC_HIP(ctx, hipSetDevice(ctx->device));
   ptc_hashtable* ht = new ptc_hashtable();
   ht->ctx = ctx;
@@ -540,6 +782,29 @@ extern "C" int ptc_hashtable_build(ptc_ctx* ctx, const ptc_dtensor* t, ptc_hasht
   else
     ht->rest_key32 = dalloc_n<uint32_t>(ctx, N);
   const uint64_t nblk = (N + RK_CHUNK - 1) / RK_CHUNK;
+  // sort grouping (32-bit key spaces): requested, or the default for hashed
+  // key spaces
+  const bool sortable = N > 0 && ht->codec.space <= 0xFFFFFFFFull;
+  int key_bits = 1;
+  while (key_bits < 64 && (1ull << key_bits) < ht->codec.space) ++key_bits;
+  if (grouping == PTC_GROUP_SORT && !sortable) {
+    ptc_hashtable_free(ht);
+    return set_err(ctx, PTC_ERR_UNSUPPORTED, "sort grouping needs a nonempty tensor and a key space within 32 bits");
+  }
+  if (sortable && (grouping == PTC_GROUP_SORT || (grouping == PTC_GROUP_AUTO && key_bits <= SORT_MAX_BITS))) {
+    if (!ht->entries || !ht->perm || !(ht->gval || ht->gval32) || !(ht->rest_key32 || ht->rest_key64)) {
+      ptc_hashtable_free(ht);
+      return PTC_ERR_NOMEM;
+    }
+    rc = build_sorted(ctx, ht, t, f32, rest64);
+    if (rc) {
+      ptc_hashtable_free(ht);
+      return rc;
+    }
+    hipEventRecord(ht->ev_end, ctx->stream);
+    *out = ht;
+    return PTC_OK;
+  }
   // Three ways to group:
   //  hashed    insert + wave-aggregated rank (counts in the entries), ids
   //            scattered, streaming layout gathered from the input-order

@@ -740,8 +740,17 @@ ptc_dtensor* view_with_keys(const ptc_dtensor* t, uint32_t nk, const uint32_t* d
 }
 }  // namespace ptc
 
+static int contract_grouped(ptc_ctx* ctx, const ptc_dtensor* A, const ptc_dtensor* B, const ptc_options* opt,
+                            ptc_dtensor** O, ptc_stats* stats, int grouping);
+
 extern "C" int ptc_contract(ptc_ctx* ctx, const ptc_dtensor* A, const ptc_dtensor* B, const ptc_options* opt,
                             ptc_dtensor** O, ptc_stats* stats) {
+  return contract_grouped(ctx, A, B, opt, O, stats, PTC_GROUP_AUTO);
+}
+
+// The driver sequence of main.cpp:68-118 with a chosen table grouping.
+static int contract_grouped(ptc_ctx* ctx, const ptc_dtensor* A, const ptc_dtensor* B, const ptc_options* opt,
+                            ptc_dtensor** O, ptc_stats* stats, int grouping) {
   if (!ctx || !A || !B || !O) return PTC_ERR_ARG;
   *O = nullptr;
   const uint32_t c = A->nk;
@@ -752,8 +761,8 @@ extern "C" int ptc_contract(ptc_ctx* ctx, const ptc_dtensor* A, const ptc_dtenso
   for (uint32_t i = 0; i < c; ++i) rot[A->d - c + i] = A->dim_tc[i];
   ptc_dtensor* Ac = view_with_keys(A, A->d - c, rot);
   ptc_hashtable *HB = nullptr, *HA = nullptr;
-  int rc = ptc_hashtable_build(ctx, B, &HB);  // B first, as main.cpp:115-116
-  if (!rc) rc = ptc_hashtable_build(ctx, Ac, &HA);
+  int rc = ptc_hashtable_build_ex(ctx, B, grouping, &HB);  // B first, as main.cpp:115-116
+  if (!rc) rc = ptc_hashtable_build_ex(ctx, Ac, grouping, &HA);
   if (!rc) rc = ptc_tc_fksCuckoo(ctx, HA, HB, Ac, B, opt, O, stats);
   ptc_hashtable_free(HA);
   ptc_hashtable_free(HB);
@@ -766,9 +775,14 @@ extern "C" int ptc_matricize_spGEMM(ptc_ctx* ctx, const ptc_dtensor* A, const pt
   return ptc_contract(ctx, A, B, nullptr, O, stats);
 }
 
+// The sorting matricization groups both operands by sorting their keys
+// (hashing where a key space exceeds 32 bits).
 extern "C" int ptc_matricize_sort_spGEMM(ptc_ctx* ctx, const ptc_dtensor* A, const ptc_dtensor* B, ptc_dtensor** O,
                                          ptc_stats* stats) {
-  return ptc_contract(ctx, A, B, nullptr, O, stats);
+  if (!ctx || !A || !B || !O) return PTC_ERR_ARG;
+  int rc = contract_grouped(ctx, A, B, nullptr, O, stats, PTC_GROUP_SORT);
+  if (rc == PTC_ERR_UNSUPPORTED) rc = contract_grouped(ctx, A, B, nullptr, O, stats, PTC_GROUP_HASH);
+  return rc;
 }
 
 // Per-row training data for the row-size network: the learned features and
