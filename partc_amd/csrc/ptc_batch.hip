@@ -326,6 +326,7 @@ int plan_rest(ptc_batch* b) {
   if (!tot) return PTC_ERR_NOMEM;
   for (int j = 0; j < n; ++j) {
     b->runs[j].reset(new Run(ctx));
+    b->runs[j]->force_flags = true;  // replays may bring any values
     if ((rc = prepare_launch(*b->runs[j], b->HA[j], b->HB[j], b->Ac[j], b->B[j], b->opt, tot + 4 * j)))
       return set_err(ctx, rc, "job %d: %s", j, ctx->err.c_str());
   }
@@ -333,7 +334,6 @@ int plan_rest(ptc_batch* b) {
   if ((rc = read_back(ctx, hv.data(), tot, sizeof(unsigned long long) * 4 * n))) return rc;
   for (int j = 0; j < n; ++j) {
     Run& R = *b->runs[j];
-    R.use_flags = true;  // replays may bring any values
     if ((rc = prepare_finish(R, &hv[4 * (size_t)j]))) return set_err(ctx, rc, "job %d: %s", j, ctx->err.c_str());
     b->st.flops += R.F;
     b->st.queue_items += (uint32_t)R.nitems;

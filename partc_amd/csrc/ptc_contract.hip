@@ -1051,7 +1051,9 @@ static int launch_pass_t(ptc_ctx* ctx, const PassParams& P, uint64_t nitems, siz
   auto kern = k_pass<PASS, DENSE, ColT, MERGE, VAL>;
   static size_t lds_set = 0;  // largest dynamic LDS this kernel was opened to
   if (lds > lds_set) {
-    PTC_HIP(ctx, hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    if (hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
+      return set_err(ctx, PTC_ERR_HIP, "opening %zu bytes of LDS for the row pass (%u threads): %s", lds, threads,
+                     hipGetErrorString(hipGetLastError()));
     lds_set = lds;
   }
   const bool timed = ctx->nkev + 2 <= 16;
@@ -1112,7 +1114,9 @@ static int launch_batch_t(ptc_ctx* ctx, const PassParams* jobs, const uint2* que
   auto kern = k_pass_batch<PASS, DENSE, ColT, MERGE, VAL>;
   static size_t lds_set = 0;
   if (lds > lds_set) {
-    PTC_HIP(ctx, hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    if (hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
+      return set_err(ctx, PTC_ERR_HIP, "opening %zu bytes of LDS for the batched row pass (%u threads): %s", lds, threads,
+                     hipGetErrorString(hipGetLastError()));
     lds_set = lds;
   }
   if (!grid) {

@@ -305,7 +305,7 @@ int prepare_launch(Run& R, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B,
   R.col64 = R.ncols > 0xFFFFFFFFull;
   R.nrowmodes = A->nk;
   R.dO = A->nk + (B->d - B->nk);  // tensor.hpp:113
-  R.use_flags = !(A->vals_safe && B->vals_safe);
+  R.use_flags = R.force_flags || !(A->vals_safe && B->vals_safe);
   if (R.dO == 0) return set_err(ctx, PTC_ERR_EMPTY, "tensor contraction not possible!! (output has no modes)");
   // largest dense column space whose numeric pass fits the workgroup's LDS
   const uint64_t lds_cap = lds_limit(ctx);

@@ -129,3 +129,20 @@ def test_batch_errors(ctx, jobs):
     with pytest.raises(P.PtcError) as ei:
         P.contract_batched([A], [B], ctx)
     assert ei.value.code == -1
+
+
+def test_batch_wide_dense_accumulators(ctx):
+    """A dense job whose accumulator and flag bytes fill most of a CU's LDS
+    (15000 columns with the dense limit raised to 16384): the batch keeps
+    flags for every job, so its workgroup size has to be chosen with them
+    counted."""
+    print("[case] batch wide dense", flush=True)
+    h, v = P.gen_coo(44, [15000, 30, 40], 20000, P.DIST_ZIPF)
+    wide = meta_self(h, v, [1, 2])
+    h, v = P.gen_coo(45, [50, 40, 30], 8000, P.DIST_ZIPF)
+    small = meta_self(h, v, [1, 2])
+    batch = [wide, small]
+    st = P.BatchStats()
+    outs = P.contract_batched(batch, batch, ctx, stats=st, dense_max_cols=16384)
+    for j, (T, O) in enumerate(zip(batch, outs)):
+        check(O, oracle_of(T), f"job {j}")
