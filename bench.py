@@ -392,6 +392,92 @@ def two_tensor_bench(P, args):
     ctx.close()
 
 
+def chain_bench(P, args):
+    """Two-step contraction chain: the nips-shape self-contraction
+    O(i, l) = sum A(i, j, k, m) A(l, j, k, m), then O2(i, n) = sum_l O(i, l)
+    C(l, n) with C a 2482 x 2000 sparse matrix (seed 5, 500000 nnz, Zipf(1.1)).
+    The second step needs O grouped by its row mode i.  "keyed" hands O over
+    with that table formed from the output's row grouping
+    (ptc_tc_fksCuckoo_keyed); "rebuild" builds it from O's COO, as a chain of
+    reference calls would (buildHashTable per contraction).  A step is the
+    whole chain from device-resident inputs: both steps' tables and
+    contractions."""
+    import ctypes as C
+
+    L = P.lib()
+    ctx = P.Context(0)
+    h, v = P.gen_coo(SEED, DIMS, NNZ, P.DIST_ZIPF, 1.1)
+    A, B = P.self_contraction_inputs(h, v, CONTRACTED)
+    hc, vc = P.gen_coo(5, [2482, 2000], 500000, P.DIST_ZIPF, 1.1)
+    Cm = P.Tensor(hc, vc, None, np.array([0, 1], np.uint32), 1)
+    dA, dB, dC = (P.DeviceTensor.upload(T, ctx) for T in (A, B, Cm))
+    method = P.METHOD_LEARNED if P.learned_available() else P.METHOD_TWO_PASS
+    opts = P.options(method=method)
+    st1, st2 = P.Stats(), P.Stats()
+    perm = np.array([0, 1], np.uint32)
+    pp = P._p32(perm)
+
+    def step(keyed):
+        hb, ha, hcm, o, ho, o2 = (C.c_void_p() for _ in range(6))
+        fr = C.c_int(0)
+        rc = L.ptc_hashtable_build(ctx.handle, dB._h, C.byref(hb))
+        rc = rc or L.ptc_hashtable_build(ctx.handle, dA._h, C.byref(ha))
+        if keyed:
+            rc = rc or L.ptc_tc_fksCuckoo_keyed(ctx.handle, ha, hb, dA._h, dB._h, C.byref(opts), 1, pp, C.byref(o),
+                                                C.byref(ho), C.byref(fr), C.byref(st1))
+            rc = rc or (0 if fr.value == 1 else -7)
+        else:
+            rc = rc or L.ptc_tc_fksCuckoo(ctx.handle, ha, hb, dA._h, dB._h, C.byref(opts), C.byref(o), C.byref(st1))
+            rc = rc or L.ptc_dtensor_set_keys(o, 1, pp)
+            rc = rc or L.ptc_hashtable_build(ctx.handle, o, C.byref(ho))
+        rc = rc or L.ptc_hashtable_build(ctx.handle, dC._h, C.byref(hcm))
+        rc = rc or L.ptc_tc_fksCuckoo(ctx.handle, ho, hcm, o, dC._h, C.byref(opts), C.byref(o2), C.byref(st2))
+        for t in (hb, ha, ho, hcm):
+            L.ptc_hashtable_free(t)
+        L.ptc_dtensor_free(o2)
+        L.ptc_dtensor_free(o)
+        if rc != 0:
+            raise P.PtcError(rc, ctx.last_error())
+        return st2.ms_build_a
+
+    res = {}
+    for name, keyed in (("rebuild", False), ("keyed", True)):
+        for _ in range(args.warmup):
+            step(keyed)
+        ctx.synchronize()
+        tab = []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            tab.append(step(keyed))
+        ctx.synchronize()
+        ms = 1e3 * (time.perf_counter() - t0) / args.steps
+        res[name] = {"ms_per_step": round(ms, 4), "ms_table_O": round(sum(tab) / len(tab), 4)}
+        print(f"# {name}: {ms:.3f} ms/step, O table {res[name]['ms_table_O']:.3f} ms", file=sys.stderr, flush=True)
+    F = int(st1.flops) + int(st2.flops)
+    out = {
+        "metric": "useful multiply-adds/s, two-step contraction chain (nips-shape self-contraction, then x matrix)",
+        "value": F / (res["keyed"]["ms_per_step"] * 1e-3),
+        "unit": "MAD/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": res["keyed"]["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded generator; no dataset)",
+        "config": {"workload": WORKLOAD + "; then O x C, C 2482x2000, 500000 nnz, Zipf(1.1), seed 5",
+                   "method": "learned" if method == P.METHOD_LEARNED else "two_pass",
+                   "flops": F, "nnz_O": int(st1.nnz_out), "nnz_out": int(st2.nnz_out),
+                   "step": "tables + contraction of both steps, inputs resident in HBM"},
+        "chain": res,
+        "speedup_keyed_vs_rebuild": res["rebuild"]["ms_per_step"] / res["keyed"]["ms_per_step"],
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
 def load_traffic():
     f = ROOT / "profiles" / "traffic_numeric_pass_nips.json"
     if f.exists():
@@ -410,9 +496,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--methods", default="two_pass,upper_bound,learned")
     ap.add_argument("--no-f32", action="store_true", help="skip the FP32 compensated leg")
-    ap.add_argument("--config", default="nips", choices=["nips", "batch", "twotensor"],
+    ap.add_argument("--config", default="nips", choices=["nips", "batch", "twotensor", "chain"],
                     help="nips: the headline self-contraction (default); batch: 256 uneven contractions; "
-                         "twotensor: the 5M-nnz two-tensor contraction")
+                         "twotensor: the 5M-nnz two-tensor contraction; chain: two contractions in a row")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -432,6 +518,9 @@ def main():
         return
     if args.config == "twotensor":
         two_tensor_bench(P, args)
+        return
+    if args.config == "chain":
+        chain_bench(P, args)
         return
 
     h, v = P.gen_coo(SEED, DIMS, NNZ, P.DIST_ZIPF, 1.1)

@@ -226,6 +226,7 @@ typedef struct {
    * the context stream), summed over their launches in this call */
   double ms_kernel_symbolic, ms_kernel_numeric;
   uint64_t kernel_launches_numeric;
+  double ms_build_out;    /* ptc_tc_fksCuckoo_keyed: device time of the output's key table */
 } ptc_stats;
 
 void ptc_options_default(ptc_options* opt);
@@ -241,6 +242,23 @@ void ptc_options_default(ptc_options* opt);
 int ptc_tc_fksCuckoo(ptc_ctx* ctx, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B,
                      const ptc_dtensor* A, const ptc_dtensor* B, const ptc_options* opt,
                      ptc_dtensor** O, ptc_stats* stats);
+
+/* One step of a contraction chain: ptc_tc_fksCuckoo, with the output handed
+ * over ready for the next contraction.  O's key split is set to
+ * (nk_next, dim_tc_next) -- dim_tc_next a permutation of O's modes, the
+ * first nk_next the next contraction's key -- and *HT_O is O's key table,
+ * the reorder_tensor layout (fksCuckooTC.hpp:31-77) that ptc_hashtable_build
+ * would give.  When the key modes are exactly O's row modes (A's free modes,
+ * in any order) the table is formed from the output's own row grouping with
+ * no build (*from_rows = 1): the slices are the output rows, in output
+ * order, and the table shares O's values (free the table before O).  Other
+ * key splits build the table (*from_rows = 0).  stats->ms_build_out is the
+ * table's device time.  No counterpart in the reference, which rebuilds a
+ * table per contraction (main.cpp:115-116). */
+int ptc_tc_fksCuckoo_keyed(ptc_ctx* ctx, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B,
+                           const ptc_dtensor* A, const ptc_dtensor* B, const ptc_options* opt,
+                           uint32_t nk_next, const uint32_t* dim_tc_next, ptc_dtensor** O,
+                           ptc_hashtable** HT_O, int* from_rows, ptc_stats* stats);
 
 /* Convenience: both tensors carry .meta-style dim_tc (contracted modes
  * first, num_key_dimensions = c, as main.cpp:65 leaves A).  Builds both

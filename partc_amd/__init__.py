@@ -158,6 +158,7 @@ class Stats(C.Structure):
         ("ms_kernel_symbolic", C.c_double),
         ("ms_kernel_numeric", C.c_double),
         ("kernel_launches_numeric", C.c_uint64),
+        ("ms_build_out", C.c_double),
     ]
 
     def as_dict(self) -> dict:
@@ -200,6 +201,8 @@ def lib() -> C.CDLL:
         "ptc_hashtable_free": (None, [vp]),
         "ptc_options_default": (None, [C.POINTER(Options)]),
         "ptc_tc_fksCuckoo": (i32, [vp, vp, vp, vp, vp, C.POINTER(Options), C.POINTER(vp), C.POINTER(Stats)]),
+        "ptc_tc_fksCuckoo_keyed": (i32, [vp, vp, vp, vp, vp, C.POINTER(Options), u32, P32, C.POINTER(vp),
+                                         C.POINTER(vp), C.POINTER(i32), C.POINTER(Stats)]),
         "ptc_contract": (i32, [vp, vp, vp, C.POINTER(Options), C.POINTER(vp), C.POINTER(Stats)]),
         "ptc_matricize_spGEMM": (i32, [vp, vp, vp, C.POINTER(vp), C.POINTER(Stats)]),
         "ptc_matricize_sort_spGEMM": (i32, [vp, vp, vp, C.POINTER(vp), C.POINTER(Stats)]),
@@ -594,6 +597,26 @@ def tc_fksCuckoo(HT_A: HashTable, HT_B: HashTable, A=None, B=None, *, download: 
                                   C.byref(out), C.byref(st)), ctx)
     dO = DeviceTensor(ctx, out)
     return dO.download() if download else dO
+
+
+def tc_fksCuckoo_keyed(HT_A: HashTable, HT_B: HashTable, num_key_dimensions: int, dim_tc, *,
+                       stats: Optional[Stats] = None, **opts):
+    """One step of a contraction chain (ptc_tc_fksCuckoo_keyed): O = A x B
+    as tc_fksCuckoo computes it, with O's key split set to
+    (num_key_dimensions, dim_tc) and O's key table for the next contraction.
+    Returns (DeviceTensor O, HashTable of O, from_rows): from_rows is True
+    when the table came from the output's row grouping (key modes = O's row
+    modes) rather than a build."""
+    ctx = HT_A.ctx
+    o = options(**opts)
+    perm = np.ascontiguousarray(dim_tc, dtype=np.uint32)
+    out, tab, fr = C.c_void_p(), C.c_void_p(), C.c_int(0)
+    st = stats if stats is not None else Stats()
+    _check(lib().ptc_tc_fksCuckoo_keyed(ctx.handle, HT_A._h, HT_B._h, HT_A.tensor._h, HT_B.tensor._h, C.byref(o),
+                                        num_key_dimensions, _p32(perm), C.byref(out), C.byref(tab), C.byref(fr),
+                                        C.byref(st)), ctx)
+    dO = DeviceTensor(ctx, out)
+    return dO, HashTable(ctx, tab, dO), bool(fr.value)
 
 
 def contract(A, B, ctx: Optional[Context] = None, *, download: bool = True, stats: Optional[Stats] = None,

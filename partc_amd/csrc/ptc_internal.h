@@ -129,6 +129,7 @@ struct ptc_hashtable {
   unsigned long long* rest_key64 = nullptr;
   double* gval = nullptr;      // device, N: values in grouped order
   float* gval32 = nullptr;     // device, N: values in grouped order, FP32 tensors (gval is then NULL)
+  bool shares_vals = false;    // gval/gval32 are the tensor's own values (grouped order = tensor order)
   hipEvent_t ev_start = nullptr, ev_end = nullptr;  // bracket the build on the stream
   double ms_build = -1.0;      // from the events, on first request (hashtable_ms_build)
 };
@@ -142,6 +143,12 @@ int set_err(ptc_ctx* ctx, int code, const char* fmt, ...);
 int hashtable_slice_of(ptc_ctx* ctx, const ptc_hashtable* h, const uint32_t** out);
 int hashtable_num_keys(ptc_ctx* ctx, const ptc_hashtable* h, uint64_t* out);
 double hashtable_ms_build(const ptc_hashtable* h);
+// Key table of a contraction's output O keyed on output row modes only
+// (O->dim_tc[0..nk) a permutation of 0..rowc.n-1), formed from the output's
+// row grouping: row i of the contraction (key rows[i].key in rowc) holds
+// O's positions [off[i], off[i] + cnt[i]).  The table shares O's values.
+int hashtable_from_rows(ptc_ctx* ctx, const ptc_dtensor* O, const KeyCodec& rowc, const KtSlice* rows,
+                        uint64_t nrows, const uint32_t* cnt, const unsigned long long* off, ptc_hashtable** out);
 int hip_check(ptc_ctx* ctx, hipError_t e, const char* what);
 #define PTC_HIP(ctx, call)                                 \
   do {                                                     \

@@ -983,6 +983,135 @@ double hashtable_ms_build(const ptc_hashtable* hc) {
   }
   return h->ms_build;
 }
+
+// ------------------------------------------------- tables from output rows
+//
+// A contraction writes its output grouped by row: row i (A's free key
+// rows[i].key) occupies [off[i], off[i] + cnt[i]).  When the next
+// contraction keys the output on exactly those modes, the row grouping is
+// the reorder_tensor layout (fksCuckooTC.hpp:31-77) already: the slices are
+// the nonempty rows, the grouped order is the output order (perm is the
+// identity) and the values are the output's own.  Only the keys are
+// re-encoded into the output's codec, the non-key tuples linearised, and the
+// probe index written.
+
+__global__ void k_row_flags(uint64_t nrows, const uint32_t* __restrict__ cnt, uint32_t* __restrict__ flag) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nrows; i += (uint64_t)gridDim.x * blockDim.x)
+    flag[i] = cnt[i] ? 1u : 0u;
+}
+
+// Row i -> slice sx[i] (nonempty rows only).  Key position q of the table's
+// codec is output mode keyc.mode[q], which is position keyc.mode[q] of the
+// row codec.  len holds the end of the range, as k_rs_index expects.
+__global__ void k_rows_to_slices(uint64_t nrows, const KtSlice* __restrict__ rows, const uint32_t* __restrict__ cnt,
+                                 const unsigned long long* __restrict__ off, const unsigned long long* __restrict__ sx,
+                                 KeyCodec rowc, KeyCodec keyc, KtSlice* __restrict__ slices) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nrows; i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t n = cnt[i];
+    if (!n) continue;
+    const unsigned long long k = rows[i].key;
+    unsigned long long nk = 0;
+    for (uint32_t q = 0; q < keyc.n; ++q) {
+      const uint32_t j = keyc.mode[q];
+      const unsigned long long x = rowc.n == 1 ? k : (k / rowc.stride[j]) % rowc.radix[j];
+      nk += x * keyc.stride[q];
+    }
+    KtSlice sl;
+    sl.key = nk;
+    sl.off = (uint32_t)off[i];
+    sl.len = (uint32_t)(off[i] + n);
+    slices[sx[i]] = sl;
+  }
+}
+
+template <class RestT>
+__global__ void k_rest_keys(const uint32_t* __restrict__ h, uint32_t d, uint64_t N, KeyCodec rest,
+                            RestT* __restrict__ out) {
+  for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < N; p += (uint64_t)gridDim.x * blockDim.x) {
+    unsigned long long k = 0;
+    for (uint32_t j = 0; j < rest.n; ++j) k += (unsigned long long)h[p * d + rest.mode[j]] * rest.stride[j];
+    out[p] = (RestT)k;
+  }
+}
+
+__global__ void k_iota(uint64_t N, uint32_t* __restrict__ out) {
+  for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < N; p += (uint64_t)gridDim.x * blockDim.x)
+    out[p] = (uint32_t)p;
+}
+
+int hashtable_from_rows(ptc_ctx* ctx, const ptc_dtensor* O, const KeyCodec& rowc, const KtSlice* rows,
+                        uint64_t nrows, const uint32_t* cnt, const unsigned long long* off, ptc_hashtable** out) {
+  *out = nullptr;
+  const uint64_t N = O->N;
+  if (N == 0 || N > 0xFFFFFFFFull || nrows == 0) return set_err(ctx, PTC_ERR_UNSUPPORTED, "output size for a row table");
+  for (uint32_t q = 0; q < O->nk; ++q)
+    if (O->dim_tc[q] >= rowc.n) return set_err(ctx, PTC_ERR_ARG, "key mode %u is not a row mode", O->dim_tc[q]);
+  ptc_hashtable* ht = new ptc_hashtable();
+  ht->ctx = ctx;
+  ht->t = O;
+  ht->shares_vals = true;
+  if (hipEventCreate(&ht->ev_start) != hipSuccess || hipEventCreate(&ht->ev_end) != hipSuccess) {
+    ptc_hashtable_free(ht);
+    return hip_check(ctx, hipGetLastError(), "event");
+  }
+  hipStream_t st = ctx->stream;
+  hipEventRecord(ht->ev_start, st);
+  int rc = make_codec(ctx, O, O->dim_tc, O->nk, &ht->codec);
+  if (!rc) rc = make_codec(ctx, O, O->dim_tc + O->nk, O->d - O->nk, &ht->rest);
+  if (rc) {
+    ptc_hashtable_free(ht);
+    return rc;
+  }
+  // the index: the same direct/hashed rule as a build, sized for the rows
+  ht->direct = ht->codec.space <= std::max<uint64_t>(2 * N, 1u << 16) && ht->codec.space < (1ull << 31);
+  const uint64_t maxkeys = std::min<uint64_t>(nrows, ht->codec.space);
+  uint64_t cap = 64;
+  while (cap < maxkeys + maxkeys / 4) cap <<= 1;
+  if (ht->direct) cap = std::max<uint64_t>(ht->codec.space, 1);
+  ht->cap = cap;
+  ht->entries = dalloc_n<KtEntry>(ctx, cap);
+  ht->perm = dalloc_n<uint32_t>(ctx, N);
+  ht->slices_cap = nrows;
+  ht->slices = dalloc_n<KtSlice>(ctx, nrows);
+  ht->d_total = dalloc_n<unsigned long long>(ctx, 1);
+  const bool rest64 = ht->rest.space > 0xFFFFFFFFull;
+  if (rest64) ht->rest_key64 = dalloc_n<unsigned long long>(ctx, N);
+  else ht->rest_key32 = dalloc_n<uint32_t>(ctx, N);
+  ht->gval = O->vals;
+  ht->gval32 = O->vals32;
+  Workspace ws(ctx);
+  uint32_t* flag = ws.get<uint32_t>(nrows);
+  unsigned long long* sx = ws.get<unsigned long long>(nrows + 1);
+  if (!ht->entries || !ht->perm || !ht->slices || !ht->d_total || !(ht->rest_key32 || ht->rest_key64) || !flag || !sx) {
+    ptc_hashtable_free(ht);
+    return PTC_ERR_NOMEM;
+  }
+  const unsigned gr = grid_for(nrows, 256, 16384), gn = grid_for(N, 256, 16384);
+  hipLaunchKernelGGL(k_row_flags, dim3(gr), dim3(256), 0, st, nrows, cnt, flag);
+  if ((rc = scan_u32_to_u64(ctx, flag, sx, nrows, sx + nrows))) {
+    ptc_hashtable_free(ht);
+    return rc;
+  }
+  hipLaunchKernelGGL(k_rows_to_slices, dim3(gr), dim3(256), 0, st, nrows, rows, cnt, off,
+                     (const unsigned long long*)sx, rowc, ht->codec, ht->slices);
+  hipLaunchKernelGGL(k_kt_clear, dim3(grid_for(cap, 256)), dim3(256), 0, st, ht->entries, (uint32_t*)nullptr, cap);
+  hipLaunchKernelGGL(k_rs_index, dim3(gr), dim3(256), 0, st, ht->slices, (const unsigned long long*)(sx + nrows),
+                     ht->entries, cap, ht->direct ? 1 : 0, ht->d_total, N);
+  hipLaunchKernelGGL(k_iota, dim3(gn), dim3(256), 0, st, N, ht->perm);
+  if (rest64)
+    hipLaunchKernelGGL((k_rest_keys<unsigned long long>), dim3(gn), dim3(256), 0, st, (const uint32_t*)O->hedges, O->d, N,
+                       ht->rest, ht->rest_key64);
+  else
+    hipLaunchKernelGGL((k_rest_keys<uint32_t>), dim3(gn), dim3(256), 0, st, (const uint32_t*)O->hedges, O->d, N,
+                       ht->rest, ht->rest_key32);
+  hipEventRecord(ht->ev_end, st);
+  if (hipGetLastError() != hipSuccess) {
+    ptc_hashtable_free(ht);
+    return set_err(ctx, PTC_ERR_HIP, "row table kernels");
+  }
+  *out = ht;
+  return PTC_OK;
+}
 }  // namespace ptc
 
 extern "C" int ptc_hashtable_info(const ptc_hashtable* h, uint64_t* num_keys, uint64_t* num_items) {
@@ -1046,8 +1175,10 @@ extern "C" void ptc_hashtable_free(ptc_hashtable* h) {
     dfree(h->ctx, h->slice_of);
     dfree(h->ctx, h->rest_key32);
     dfree(h->ctx, h->rest_key64);
-    dfree(h->ctx, h->gval);
-    dfree(h->ctx, h->gval32);
+    if (!h->shares_vals) {
+      dfree(h->ctx, h->gval);
+      dfree(h->ctx, h->gval32);
+    }
     dfree(h->ctx, h->d_total);
   }
   if (h->ev_start) hipEventDestroy(h->ev_start);

@@ -476,9 +476,55 @@ int prepare(Run& R, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B, const 
 
 }  // namespace ptc
 
+namespace {
+
+// The key split requested for the output of a contraction chain's step.
+struct KeyReq {
+  uint32_t nk;
+  uint32_t dim_tc[PTC_MAX_MODES];
+  ptc_hashtable** table;
+  int* from_rows;
+};
+
+int tc_impl(ptc_ctx* ctx, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B, const ptc_dtensor* A,
+            const ptc_dtensor* B, const ptc_options* opt_in, ptc_dtensor** O, ptc_stats* stats, const KeyReq* kr);
+
+}  // namespace
+
 extern "C" int ptc_tc_fksCuckoo(ptc_ctx* ctx, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B,
                                 const ptc_dtensor* A, const ptc_dtensor* B, const ptc_options* opt_in,
                                 ptc_dtensor** O, ptc_stats* stats) {
+  return tc_impl(ctx, HT_A, HT_B, A, B, opt_in, O, stats, nullptr);
+}
+
+extern "C" int ptc_tc_fksCuckoo_keyed(ptc_ctx* ctx, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B,
+                                      const ptc_dtensor* A, const ptc_dtensor* B, const ptc_options* opt,
+                                      uint32_t nk_next, const uint32_t* dim_tc_next, ptc_dtensor** O,
+                                      ptc_hashtable** HT_O, int* from_rows, ptc_stats* stats) {
+  if (!ctx || !A || !B || !O || !HT_O || !dim_tc_next) return PTC_ERR_ARG;
+  *O = nullptr;
+  *HT_O = nullptr;
+  if (from_rows) *from_rows = 0;
+  const uint32_t dO = A->nk + (B->d - B->nk);  // tensor.hpp:113
+  if (nk_next == 0 || nk_next > dO) return set_err(ctx, PTC_ERR_ARG, "%u key modes for a %u-mode output", nk_next, dO);
+  KeyReq kr{};
+  kr.nk = nk_next;
+  kr.table = HT_O;
+  kr.from_rows = from_rows;
+  uint32_t seen = 0;
+  for (uint32_t j = 0; j < dO; ++j) {
+    const uint32_t m = dim_tc_next[j];
+    if (m >= dO || (seen >> m) & 1u) return set_err(ctx, PTC_ERR_ARG, "dim_tc is not a permutation of the %u output modes", dO);
+    seen |= 1u << m;
+    kr.dim_tc[j] = m;
+  }
+  return tc_impl(ctx, HT_A, HT_B, A, B, opt, O, stats, &kr);
+}
+
+namespace {
+
+int tc_impl(ptc_ctx* ctx, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B, const ptc_dtensor* A,
+            const ptc_dtensor* B, const ptc_options* opt_in, ptc_dtensor** O, ptc_stats* stats, const KeyReq* kr) {
   if (!ctx || !HT_A || !HT_B || !A || !B || !O) return PTC_ERR_ARG;
   *O = nullptr;
   ptc_options opt;
@@ -692,9 +738,28 @@ extern "C" int ptc_tc_fksCuckoo(ptc_ctx* ctx, const ptc_hashtable* HT_A, const p
       return rc;
     }
   }
+  // The output's key table for the next contraction: from the row grouping
+  // when the key modes are exactly the row modes, else built.
+  ptc_hashtable* hto = nullptr;
+  bool rows_key = false;
+  if (kr) {
+    out->nk = kr->nk;
+    for (uint32_t j = 0; j < out->d; ++j) out->dim_tc[j] = kr->dim_tc[j];
+    rows_key = kr->nk == R.nrowmodes && nnz <= 0xFFFFFFFFull;
+    for (uint32_t q = 0; q < kr->nk; ++q) rows_key = rows_key && kr->dim_tc[q] < R.nrowmodes;
+    rc = rows_key ? hashtable_from_rows(ctx, out, HT_A->codec, HT_A->slices, R.nrows, R.row_nnz, out_off, &hto)
+                  : ptc_hashtable_build_ex(ctx, out, PTC_GROUP_AUTO, &hto);
+    if (rc) {
+      ptc_dtensor_free(out);
+      return rc;
+    }
+    *kr->table = hto;
+    if (kr->from_rows) *kr->from_rows = rows_key ? 1 : 0;
+  }
   if (stats) {
     hipEventSynchronize(T.ev[T.n - 1]);
     memset(stats, 0, sizeof(*stats));
+    if (hto) stats->ms_build_out = hashtable_ms_build(hto);
     stats->nnz_a = A->N;
     stats->nnz_b = B->N;
     stats->rows = R.nrows;
@@ -728,6 +793,8 @@ extern "C" int ptc_tc_fksCuckoo(ptc_ctx* ctx, const ptc_hashtable* HT_A, const p
   *O = out;
   return PTC_OK;
 }
+
+}  // namespace
 
 // A view of t with another key split, sharing t's device arrays.
 namespace ptc {
