@@ -161,13 +161,16 @@ int ptc_hashtable_probe(ptc_ctx* ctx, const ptc_hashtable* h, uint64_t n, const 
                         uint64_t cap);
 void ptc_hashtable_free(ptc_hashtable* h);
 /* The same table with a chosen grouping: PTC_GROUP_HASH (atomicCAS insert
- * into an open-addressing table, ranks by wave-aggregated atomics),
+ * into one open-addressing table, ranks by wave-aggregated atomics),
  * PTC_GROUP_SORT (stable LSD radix sort of the keys, slices from runs of
  * equal keys, the probe index over distinct keys only; key spaces within 32
- * bits) or PTC_GROUP_AUTO.  Sorting is the sort-based matricization of the
- * reference (matricizedspGEMM_sorting.hpp:194-754, which exits before
+ * bits), PTC_GROUP_PARTITION (nonzeros partitioned into buckets by key hash,
+ * each bucket grouped in an LDS table, the probe index over distinct keys
+ * only; fewer than 2^32 nonzeros) or PTC_GROUP_AUTO (sorting up to 24-bit
+ * key spaces, partitioning above).  Sorting is the sort-based matricization
+ * of the reference (matricizedspGEMM_sorting.hpp:194-754, which exits before
  * finishing) applied to the grouping step. */
-enum { PTC_GROUP_AUTO = 0, PTC_GROUP_HASH = 1, PTC_GROUP_SORT = 2 };
+enum { PTC_GROUP_AUTO = 0, PTC_GROUP_HASH = 1, PTC_GROUP_SORT = 2, PTC_GROUP_PARTITION = 3 };
 int ptc_hashtable_build_ex(ptc_ctx* ctx, const ptc_dtensor* t, int grouping, ptc_hashtable** out);
 
 /* ------------------------------------------------------------ contraction */

@@ -49,7 +49,7 @@ METHOD_TWO_PASS, METHOD_LEARNED, METHOD_UPPER_BOUND = 0, 1, 2
 VAL_F64, VAL_F32_TWOSUM, VAL_F32_PLAIN, VAL_F32_F64ACC = 0, 1, 2, 3
 ACC_AUTO, ACC_DENSE, ACC_HASH, ACC_SPILL = 0, 1, 2, 3
 DIST_UNIFORM, DIST_ZIPF = 0, 1
-GROUP_AUTO, GROUP_HASH, GROUP_SORT = 0, 1, 2
+GROUP_AUTO, GROUP_HASH, GROUP_SORT, GROUP_PARTITION = 0, 1, 2, 3
 
 
 class PtcError(RuntimeError):
@@ -558,7 +558,8 @@ def _as_device(t, ctx: Context) -> DeviceTensor:
 def buildHashTable(T, ctx: Optional[Context] = None, grouping: int = GROUP_AUTO) -> HashTable:
     """Group T's nonzeros by its key modes (T.dim_tc[:num_key_dimensions])
     on the device (fksCuckoo.hpp:1044-1060 with reorder_tensor); grouping
-    GROUP_HASH or GROUP_SORT picks the method (ptc_hashtable_build_ex)."""
+    GROUP_HASH, GROUP_SORT or GROUP_PARTITION picks the method
+    (ptc_hashtable_build_ex)."""
     ctx = ctx or default_context()
     dT = _as_device(T, ctx)
     h = C.c_void_p()

@@ -89,9 +89,9 @@ __global__ void __launch_bounds__(256) k_probe(ProbeParams P) {
           off = en.off;
         }
       } else if (ok) {
-        // same probe order as the insertion: line mix64(k) mod (cap/8), then
+        // same probe order as the insertion: the home line of mix64(k), then
         // onward; a whole 8-entry line is loaded at once
-        uint64_t i = (mix64d(kb) & (P.cap / 8 - 1)) * 8;
+        uint64_t i = kt_home(mix64d(kb), P.cap);
         bool done = false;
         while (!done) {
           KtEntry en[8];

@@ -58,6 +58,15 @@ struct KtSlice {
   uint32_t len;
 };
 
+// First slot of a key's probe sequence in a hashed table of cap slots (a
+// power of two >= 64, eight slots to a 128-byte line): the line numbered by
+// the top bits of the key's hash.  Keys whose hashes share their top bits
+// start in one contiguous region of the table, so a build that partitions
+// keys by those bits writes each partition's region on its own.
+__device__ __forceinline__ uint64_t kt_home(unsigned long long hash, uint64_t cap) {
+  return (hash >> (64 - __builtin_ctzll(cap >> 3))) << 3;
+}
+
 struct ptc_block {
   void* ptr = nullptr;
   size_t bytes = 0;

@@ -90,14 +90,13 @@ __global__ void __launch_bounds__(KT_THREADS) k_kt_insert_rank(
     const ValT* __restrict__ val, KtPay<RestT, ValT>* __restrict__ pay) {
   __shared__ uint32_t xr[KT_THREADS];
   const int lane = threadIdx.x & 63, wbase = threadIdx.x - lane;
-  const uint64_t nlines = cap / 8;
   for (uint64_t e0 = (uint64_t)blockIdx.x * KT_THREADS; e0 < N; e0 += (uint64_t)gridDim.x * KT_THREADS) {
     const uint64_t e = e0 + threadIdx.x;
     uint32_t slot = 0xFFFFFFFFu;
     if (e < N) {
       unsigned long long key = 0;
       encode_key(h, d, e, codec, &key);  // table's own tensor: always within radix
-      uint64_t i = (mix64(key) & (nlines - 1)) * 8;
+      uint64_t i = kt_home(mix64(key), cap);
       while (true) {
         const unsigned long long k = load_key_l2(&tab[i]);
         if (k == key) break;
@@ -595,7 +594,6 @@ __global__ void k_rs_index(KtSlice* __restrict__ slices, const unsigned long lon
                            uint64_t cap, int direct, unsigned long long* __restrict__ total, uint64_t N) {
   const uint64_t nk = *nkeys;
   if (blockIdx.x == 0 && threadIdx.x == 0) *total = (nk << 32) | N;
-  const uint64_t nlines = cap / 8;
   for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < nk; s += (uint64_t)gridDim.x * blockDim.x) {
     KtSlice sl = slices[s];
     sl.len = sl.len - sl.off;
@@ -605,7 +603,7 @@ __global__ void k_rs_index(KtSlice* __restrict__ slices, const unsigned long lon
       i = sl.key;
       tab[i].key = sl.key;
     } else {
-      i = (mix64(sl.key) & (nlines - 1)) * 8;
+      i = kt_home(mix64(sl.key), cap);
       while (true) {
         const unsigned long long old = atomicCAS(&tab[i].key, PTC_EMPTY_KEY, sl.key);
         if (old == PTC_EMPTY_KEY) break;  // keys are distinct: a claimed entry is never this key's
@@ -642,7 +640,7 @@ __global__ void k_kt_probe(const uint32_t* __restrict__ q, uint64_t n, KeyCodec 
         if (en.len) res = ((unsigned long long)en.off << 32) | en.len;
       }
     } else if (ok) {
-      uint64_t i = (mix64(key) & (cap / 8 - 1)) * 8;
+      uint64_t i = kt_home(mix64(key), cap);
       for (uint64_t probes = 0; probes < cap; ++probes) {
         KtEntry en = tab[i];
         if (en.key == key) {
@@ -654,6 +652,472 @@ __global__ void k_kt_probe(const uint32_t* __restrict__ q, uint64_t n, KeyCodec 
       }
     }
     off_len[i] = res;
+  }
+}
+
+// ------------------------------------------------------ partitioned hashing
+//
+// Wide keys group in two steps that keep every random access on chip:
+//   1. partition: nonzeros are scattered into buckets by the top bits of
+//      mix64(key).  Per-tile bucket counts and one scan give every (bucket,
+//      tile) pair its output range; each tile is counting-sorted by bucket in
+//      LDS first, so its nonzeros leave in runs of consecutive addresses,
+//      carrying the key, the id and the payload;
+//   2. group: one workgroup per bucket counts the bucket's keys in an LDS
+//      open-addressing table, turns the counts into offsets with a block
+//      scan, and places every nonzero at bucket start + its key's offset + a
+//      running count.  A bucket larger than the LDS table runs the same
+//      steps on its own region of a global table (bounded: distinct keys <=
+//      the bucket's size = the region's slots).  The probe index is written
+//      in the same kernel: a key's home line (kt_home) is numbered by the top
+//      bits of its hash, so a bucket's keys start in the bucket's own region
+//      of the index, which the bucket's workgroup clears and fills alone; a
+//      key whose probe runs past its region's end is left to a last pass.
+// A bucket's slices come out in offset order, so all slices are sorted by
+// offset; a final pass moves them to their compact positions.
+
+constexpr int PT_THREADS = 512;
+constexpr int PT_GROUP_THREADS = 1024;     // bucket workgroups: two per CU fill it
+constexpr uint32_t PT_TAB = 4096;          // LDS table slots per bucket workgroup
+constexpr uint32_t PT_TILE = 4096;         // nonzeros per partition tile (16-bit indices in LDS)
+constexpr uint32_t PT_MAX_BUCKETS = 4096;
+constexpr uint64_t PT_BUCKET_ITEMS = 3072; // target mean bucket size
+// Beyond this many nonzeros the mean bucket outgrows the LDS table: one
+// global table (PTC_GROUP_HASH) is used instead.
+constexpr uint64_t PT_MAX_N = (uint64_t)PT_MAX_BUCKETS * PT_BUCKET_ITEMS;
+
+__device__ __forceinline__ uint32_t pt_bucket(unsigned long long key, int bbits) {
+  return bbits ? (uint32_t)(mix64(key) >> (64 - bbits)) : 0u;
+}
+
+// Exclusive scan, in place, of a[0..n) in LDS (n <= 8 * blockDim.x).
+__device__ void pt_block_scan(uint32_t* a, uint32_t n, uint32_t* s_wave) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const uint32_t per = (n + blockDim.x - 1) / blockDim.x, s0 = threadIdx.x * per;
+  uint32_t v[8], sum = 0;
+#pragma unroll
+  for (uint32_t j = 0; j < 8; ++j) {
+    v[j] = j < per && s0 + j < n ? a[s0 + j] : 0u;
+    sum += v[j];
+  }
+  uint32_t inc = sum;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t x = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += x;
+  }
+  if (lane == 63) s_wave[wid] = inc;
+  __syncthreads();
+  uint32_t pre = inc - sum;
+  for (int w = 0; w < wid; ++w) pre += s_wave[w];
+#pragma unroll
+  for (uint32_t j = 0; j < 8; ++j)
+    if (j < per && s0 + j < n) {
+      a[s0 + j] = pre;
+      pre += v[j];
+    }
+  __syncthreads();
+}
+
+// bucket counts of one tile -> hist[bucket * ntiles + tile]
+__global__ void __launch_bounds__(PT_THREADS) k_pt_hist(const uint32_t* __restrict__ h, uint32_t d, uint64_t N,
+                                                        KeyCodec codec, int bbits, uint32_t nb, uint64_t ntiles,
+                                                        uint32_t* __restrict__ hist) {
+  __shared__ uint32_t lh[PT_MAX_BUCKETS];
+  for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) lh[b] = 0;
+  __syncthreads();
+  const uint64_t base = (uint64_t)blockIdx.x * PT_TILE, end = min(N, base + PT_TILE);
+  for (uint64_t e = base + threadIdx.x; e < end; e += blockDim.x) {
+    unsigned long long key = 0;
+    encode_key(h, d, e, codec, &key);  // the table's own tensor: always within radix
+    atomicAdd(&lh[pt_bucket(key, bbits)], 1u);
+  }
+  __syncthreads();
+  for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) hist[(uint64_t)b * ntiles + blockIdx.x] = lh[b];
+}
+
+// One tile into bucket order.  A rank pass reads the tile once (coalesced),
+// keeps every nonzero's key and non-key tuple in LDS and counting-sorts the
+// tile by bucket; the write pass then emits the nonzeros in bucket order,
+// so each bucket's run goes to consecutive addresses (key, id, payload).
+// Only the values are gathered, from the tile's few KB in L2.  Tiles are
+// dealt to XCDs in contiguous ranges (workgroup i runs on XCD i mod 8), so
+// the runs that adjacent tiles write side by side meet in one L2.
+// Dynamic LDS: keys (8 B), tuples (RestT) and slot order (2 B) per nonzero
+// of the tile, then two words per bucket.
+__device__ __forceinline__ uint64_t pt_tile_of(uint32_t wg, uint64_t ntiles) {
+  const uint64_t per = (ntiles + 7) / 8;
+  return (uint64_t)(wg % 8) * per + wg / 8;
+}
+
+template <class RestT, class ValT>
+__global__ void __launch_bounds__(PT_THREADS) k_pt_scatter(const uint32_t* __restrict__ h, uint32_t d, uint64_t N,
+                                                           KeyCodec codec, KeyCodec rest, const ValT* __restrict__ val,
+                                                           int bbits, uint32_t nb, uint64_t ntiles,
+                                                           const uint32_t* __restrict__ hist,
+                                                           const unsigned long long* __restrict__ off,
+                                                           unsigned long long* __restrict__ pkey,
+                                                           uint32_t* __restrict__ pid,
+                                                           KtPay<RestT, ValT>* __restrict__ ppay) {
+  extern __shared__ unsigned long long pt_smem[];
+  unsigned long long* lkey = pt_smem;                       // PT_TILE
+  RestT* lrk = reinterpret_cast<RestT*>(lkey + PT_TILE);    // PT_TILE
+  uint16_t* ls = reinterpret_cast<uint16_t*>(lrk + PT_TILE); // PT_TILE: tile index of every bucket-ordered slot
+  uint32_t* lcur = reinterpret_cast<uint32_t*>(ls + PT_TILE); // nb: bucket start in the tile, then its cursor
+  uint32_t* gdst = lcur + nb;                               // nb: output position of slot 0 of a bucket's run
+  __shared__ uint32_t s_wave[PT_THREADS / 64];
+  const uint64_t t = pt_tile_of(blockIdx.x, ntiles);
+  if (t >= ntiles) return;
+  const uint64_t base = t * PT_TILE;
+  const uint32_t n = (uint32_t)min<uint64_t>(PT_TILE, N - base);
+  for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) {
+    lcur[b] = hist[(uint64_t)b * ntiles + t];
+    gdst[b] = (uint32_t)off[(uint64_t)b * ntiles + t];  // positions < 2^32
+  }
+  __syncthreads();
+  pt_block_scan(lcur, nb, s_wave);
+  for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) gdst[b] -= lcur[b];  // modulo 2^32: slot j -> gdst + j
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+    const uint64_t e = base + i;
+    unsigned long long key = 0, rk = 0;
+    encode_key(h, d, e, codec, &key);
+    for (uint32_t q = 0; q < rest.n; ++q) rk += (unsigned long long)h[e * d + rest.mode[q]] * rest.stride[q];
+    lkey[i] = key;
+    lrk[i] = (RestT)rk;
+    ls[atomicAdd(&lcur[pt_bucket(key, bbits)], 1u)] = (uint16_t)i;
+  }
+  __syncthreads();
+  for (uint32_t j = threadIdx.x; j < n; j += blockDim.x) {
+    const uint32_t i = ls[j];
+    const unsigned long long key = lkey[i];
+    const uint32_t p = gdst[pt_bucket(key, bbits)] + j;
+    pkey[p] = key;
+    pid[p] = (uint32_t)(base + i);
+    KtPay<RestT, ValT> pl;
+    pl.k = lrk[i];
+    pl.v = val[base + i];
+    ppay[p] = pl;
+  }
+}
+
+// Table state of one bucket: LDS arrays, or the bucket's region of a global
+// table.  Either is touched by the bucket's workgroup alone, so every
+// access is a workgroup-scope atomic: global ones are performed in the L2
+// of the workgroup's own XCD, and __syncthreads orders them (no device-wide
+// fence, which on gfx950 writes the L2 back).
+__device__ __forceinline__ unsigned long long pt_ld64(const unsigned long long* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ uint32_t pt_ld32(const uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ unsigned long long pt_cas64(unsigned long long* p, unsigned long long cmp,
+                                                       unsigned long long v) {
+  __hip_atomic_compare_exchange_strong(p, &cmp, v, __ATOMIC_RELAXED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  return cmp;  // the value found
+}
+__device__ __forceinline__ uint32_t pt_add32(uint32_t* p, uint32_t v) {
+  return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void pt_st64(unsigned long long* p, unsigned long long v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void pt_st32(uint32_t* p, uint32_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Slot of key in a table of cap slots (linear probing from mix64(key) mod
+// cap); inserts the key when absent.  The global tables hold at most cap
+// distinct keys (cap = the bucket's size).  The LDS table counts its keys in
+// *ndist and refuses a new key (PT_FULL) once PT_LDS_KEYS are in: with at
+// most one insert per thread in flight past that check, at most
+// PT_LDS_KEYS + PT_GROUP_THREADS - 1 < PT_TAB keys ever enter, so every
+// probe ends.
+constexpr uint32_t PT_LDS_KEYS = PT_TAB - PT_GROUP_THREADS - 1;
+constexpr uint32_t PT_FULL = 0xFFFFFFFFu;
+
+template <bool GLOBAL>
+__device__ __forceinline__ uint32_t pt_slot(unsigned long long* tk, uint32_t cap, unsigned long long key,
+                                            uint32_t* ndist = nullptr) {
+  uint32_t s = GLOBAL ? (uint32_t)(mix64(key) % cap) : (uint32_t)mix64(key) & (PT_TAB - 1);
+  while (true) {
+    const unsigned long long k = pt_ld64(&tk[s]);
+    if (k == key) return s;
+    if (k == PTC_EMPTY_KEY) {
+      if (ndist && pt_ld32(ndist) >= PT_LDS_KEYS) return PT_FULL;
+      const unsigned long long old = pt_cas64(&tk[s], PTC_EMPTY_KEY, key);
+      if (old == PTC_EMPTY_KEY && ndist) pt_add32(ndist, 1u);
+      if (old == PTC_EMPTY_KEY || old == key) return s;
+      continue;  // another key took the slot: re-read it
+    }
+    s = GLOBAL ? (s + 1 == cap ? 0 : s + 1) : (s + 1) & (PT_TAB - 1);
+  }
+}
+
+// The probe index of a hashed table, written per bucket region
+// [r0, r1) (region mode).
+struct PtIndex {
+  KtEntry* tab;
+  uint64_t cap;
+  uint64_t r0, r1;
+  int region;          // 1: this kernel writes the index
+  uint32_t* map;       // LDS image of the region (slot -> local slice), or NULL
+  uint32_t* ovf_cnt;   // keys whose probe ran past their region
+  uint32_t* ovf_list;  // their slice records (indices into tmps)
+};
+
+__device__ __forceinline__ uint32_t pt_cas32(uint32_t* p, uint32_t cmp, uint32_t v) {
+  __hip_atomic_compare_exchange_strong(p, &cmp, v, __ATOMIC_RELAXED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  return cmp;
+}
+
+// Group the n nonzeros of one bucket (positions [bs, bs + n) of the
+// partitioned arrays) with a table of cap >= n slots.
+// Returns false (before writing anything outside LDS) when the LDS table
+// fills up: the bucket is then grouped again in its global region.
+template <bool GLOBAL, class RestT, class ValT>
+__device__ bool pt_group_bucket(unsigned long long* tk, uint32_t* tc, uint32_t cap, uint64_t bs, uint32_t n,
+                                const unsigned long long* __restrict__ pkey, const uint32_t* __restrict__ pid,
+                                const KtPay<RestT, ValT>* __restrict__ ppay, KtSlice* __restrict__ tmps,
+                                uint32_t* __restrict__ nkeys_out, uint32_t* __restrict__ perm,
+                                RestT* __restrict__ rest_out, ValT* __restrict__ gval, const PtIndex& X,
+                                unsigned long long* s_carry, unsigned long long* s_wave, uint32_t* s_ndist) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  for (uint32_t s = threadIdx.x; s < cap; s += blockDim.x) {
+    pt_st64(&tk[s], PTC_EMPTY_KEY);
+    pt_st32(&tc[s], 0u);
+  }
+  if (X.region && X.map) {
+    for (uint32_t s = threadIdx.x; s < (uint32_t)(X.r1 - X.r0); s += blockDim.x) X.map[s] = 0xFFFFFFFFu;
+  } else if (X.region) {
+    for (uint64_t s = X.r0 + threadIdx.x; s < X.r1; s += blockDim.x) {
+      pt_st64(&X.tab[s].key, PTC_EMPTY_KEY);
+      X.tab[s].off = 0;
+      X.tab[s].len = 0;
+    }
+  }
+  if (threadIdx.x == 0) {
+    *s_carry = 0;
+    *s_ndist = 0;
+  }
+  __syncthreads();
+  // counts per key (four keys per thread loaded at a time)
+  constexpr int U = 4;
+  for (uint32_t i0 = threadIdx.x; i0 < n; i0 += U * blockDim.x) {
+    unsigned long long k[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t i = i0 + u * blockDim.x;
+      k[u] = i < n ? pkey[bs + i] : 0ull;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (i0 + u * blockDim.x < n) {
+        const uint32_t s = pt_slot<GLOBAL>(tk, cap, k[u], GLOBAL ? nullptr : s_ndist);
+        if (s != PT_FULL) pt_add32(&tc[s], 1u);
+      }
+  }
+  __syncthreads();
+  if (!GLOBAL && pt_ld32(s_ndist) >= PT_LDS_KEYS) return false;  // the same value in every thread
+  // counts -> offsets in slot order; (slices << 32 | nonzeros) scanned
+  // together, 8 consecutive slots per thread, a chunk of 8 * blockDim slots
+  // at a time
+  const uint32_t chunk = 8u * blockDim.x;
+  for (uint32_t c0 = 0; c0 < cap; c0 += chunk) {
+    const uint32_t s0 = c0 + 8u * threadIdx.x;
+    unsigned long long v[8], sum = 0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const uint32_t s = s0 + j;
+      v[j] = 0;
+      if (s < cap) {
+        const uint32_t c = pt_ld32(&tc[s]);
+        if (c) v[j] = (1ull << 32) | c;
+      }
+      sum += v[j];
+    }
+    unsigned long long inc = sum;  // inclusive scan over the wave
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const unsigned long long x = __shfl_up(inc, o, 64);
+      if (lane >= o) inc += x;
+    }
+    if (lane == 63) s_wave[wid] = inc;
+    __syncthreads();
+    unsigned long long pre = *s_carry;
+    for (int w = 0; w < wid; ++w) pre += s_wave[w];
+    pre += inc - sum;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const uint32_t s = s0 + j;
+      if (v[j]) {
+        KtSlice sl;
+        sl.key = pt_ld64(&tk[s]);
+        sl.off = (uint32_t)(bs + (uint32_t)pre);
+        sl.len = (uint32_t)v[j];
+        const uint64_t ti = bs + (pre >> 32);
+        tmps[ti] = sl;
+        pt_st32(&tc[s], (uint32_t)pre);  // the key's offset: the placing cursor
+        if (X.region && X.map) {
+          // the region's image in LDS; written out whole below
+          const uint32_t R = (uint32_t)(X.r1 - X.r0), lsi = (uint32_t)(pre >> 32);
+          uint32_t e = (uint32_t)(kt_home(mix64(sl.key), X.cap) - X.r0);  // < R: its top bits are the bucket's
+          while (e < R && pt_cas32(&X.map[e], 0xFFFFFFFFu, lsi) != 0xFFFFFFFFu) ++e;
+          if (e >= R) X.ovf_list[atomicAdd(X.ovf_cnt, 1u)] = (uint32_t)ti;  // the last pass inserts it
+        } else if (X.region) {
+          uint64_t e = kt_home(mix64(sl.key), X.cap);  // inside [r0, r1): its top bits are the bucket's
+          while (true) {
+            if (e >= X.r1) {  // ran past the region: the last pass inserts it
+              X.ovf_list[atomicAdd(X.ovf_cnt, 1u)] = (uint32_t)ti;
+              break;
+            }
+            if (pt_cas64(&X.tab[e].key, PTC_EMPTY_KEY, sl.key) == PTC_EMPTY_KEY) {
+              X.tab[e].off = sl.off;
+              X.tab[e].len = sl.len;
+              break;
+            }
+            ++e;
+          }
+        }
+      }
+      pre += v[j];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      unsigned long long t = *s_carry;
+      for (int w = 0; w < nw; ++w) t += s_wave[w];
+      *s_carry = t;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *nkeys_out = (uint32_t)(*s_carry >> 32);
+  if (X.region && X.map) {  // the region of the probe index, written in order
+    for (uint32_t s = threadIdx.x; s < (uint32_t)(X.r1 - X.r0); s += blockDim.x) {
+      const uint32_t j = X.map[s];
+      KtEntry en;
+      en.key = PTC_EMPTY_KEY;
+      en.off = 0;
+      en.len = 0;
+      if (j != 0xFFFFFFFFu) {
+        const KtSlice sl = tmps[bs + j];
+        en.key = sl.key;
+        en.off = sl.off;
+        en.len = sl.len;
+      }
+      X.tab[X.r0 + s] = en;
+    }
+  }
+  // place: every nonzero at its key's cursor (loads of four at a time in
+  // flight)
+  for (uint32_t i0 = threadIdx.x; i0 < n; i0 += U * blockDim.x) {
+    unsigned long long k[U];
+    uint32_t id[U];
+    KtPay<RestT, ValT> pl[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t i = i0 + u * blockDim.x;
+      const uint64_t q = bs + (i < n ? i : 0u);
+      k[u] = pkey[q];
+      id[u] = pid[q];
+      pl[u] = ppay[q];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (i0 + u * blockDim.x >= n) continue;
+      const uint32_t s = pt_slot<GLOBAL>(tk, cap, k[u]);  // present: a read-only probe
+      const uint64_t p = bs + pt_add32(&tc[s], 1u);
+      perm[p] = id[u];
+      rest_out[p] = pl[u].k;
+      gval[p] = pl[u].v;
+    }
+  }
+  return true;
+}
+
+template <class RestT, class ValT>
+__global__ void __launch_bounds__(PT_GROUP_THREADS) k_pt_group(
+    const unsigned long long* __restrict__ off, uint64_t ntiles, uint32_t nb, uint64_t N,
+    const unsigned long long* __restrict__ pkey, const uint32_t* __restrict__ pid,
+    const KtPay<RestT, ValT>* __restrict__ ppay, unsigned long long* __restrict__ gkey, uint32_t* __restrict__ gcnt,
+    KtSlice* __restrict__ tmps, uint32_t* __restrict__ bkeys, uint32_t* __restrict__ perm,
+    RestT* __restrict__ rest_out, ValT* __restrict__ gval, KtEntry* __restrict__ tab, uint64_t cap, int region,
+    uint32_t* __restrict__ ovf_cnt, uint32_t* __restrict__ ovf_list) {
+  __shared__ unsigned long long lk[PT_TAB];
+  __shared__ uint32_t lc[PT_TAB];
+  __shared__ uint32_t lmap[PT_TAB];
+  __shared__ unsigned long long s_wave[PT_GROUP_THREADS / 64];
+  __shared__ unsigned long long s_carry;
+  __shared__ uint32_t s_ndist;
+  const uint32_t b = blockIdx.x;
+  const uint64_t bs = off[(uint64_t)b * ntiles];
+  const uint64_t be = b + 1 < nb ? off[(uint64_t)(b + 1) * ntiles] : N;
+  const uint32_t n = (uint32_t)(be - bs);
+  PtIndex X;
+  X.tab = tab;
+  X.cap = cap;
+  X.region = region;
+  X.r0 = region ? (cap / nb) * b : 0;
+  X.r1 = region ? X.r0 + cap / nb : 0;
+  X.map = region && cap / nb <= PT_TAB ? lmap : nullptr;
+  X.ovf_cnt = ovf_cnt;
+  X.ovf_list = ovf_list;
+  // the LDS table while the bucket's distinct keys fit, else the region
+  if (!pt_group_bucket<false>(lk, lc, PT_TAB, bs, n, pkey, pid, ppay, tmps, &bkeys[b], perm, rest_out, gval, X,
+                              &s_carry, s_wave, &s_ndist)) {
+    __syncthreads();
+    pt_group_bucket<true>(gkey + bs, gcnt + bs, n, bs, n, pkey, pid, ppay, tmps, &bkeys[b], perm, rest_out, gval, X,
+                          &s_carry, s_wave, &s_ndist);
+  }
+}
+
+// Slices to their compact positions; without region mode, the probe index
+// as well (direct tables: entry[key]; hashed: one insert per distinct key).
+__global__ void k_pt_slices(const unsigned long long* __restrict__ off, uint64_t ntiles, uint32_t nb,
+                            const uint32_t* __restrict__ bkeys, const unsigned long long* __restrict__ sbase,
+                            const KtSlice* __restrict__ tmps, KtSlice* __restrict__ slices, KtEntry* __restrict__ tab,
+                            uint64_t cap, int index, int direct, unsigned long long* __restrict__ total, uint64_t N) {
+  const uint32_t b = blockIdx.x;
+  if (b == 0 && threadIdx.x == 0) *total = (sbase[nb] << 32) | N;
+  const uint64_t bs = off[(uint64_t)b * ntiles], base = sbase[b];
+  const uint32_t nk = bkeys[b];
+  for (uint32_t j = threadIdx.x; j < nk; j += blockDim.x) {
+    const KtSlice sl = tmps[bs + j];
+    slices[base + j] = sl;
+    if (!index) continue;
+    if (direct) {
+      tab[sl.key].key = sl.key;
+      tab[sl.key].off = sl.off;
+      tab[sl.key].len = sl.len;
+      continue;
+    }
+    uint64_t i = kt_home(mix64(sl.key), cap);
+    while (true) {
+      const unsigned long long old = atomicCAS(&tab[i].key, PTC_EMPTY_KEY, sl.key);
+      if (old == PTC_EMPTY_KEY) break;  // keys are distinct: a claimed entry is never this key's
+      i = (i + 1) & (cap - 1);
+    }
+    tab[i].off = sl.off;
+    tab[i].len = sl.len;
+  }
+}
+
+// Keys whose probe ran past their region: inserted once every region is
+// written, onward from their home line (the table keeps free slots: cap >
+// distinct keys).
+__global__ void k_pt_overflow(const uint32_t* __restrict__ cnt, const uint32_t* __restrict__ list,
+                              const KtSlice* __restrict__ tmps, KtEntry* __restrict__ tab, uint64_t cap) {
+  const uint32_t n = *cnt;
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    const KtSlice sl = tmps[list[k]];
+    uint64_t i = kt_home(mix64(sl.key), cap);
+    while (true) {
+      const unsigned long long old = atomicCAS(&tab[i].key, PTC_EMPTY_KEY, sl.key);
+      if (old == PTC_EMPTY_KEY) break;
+      i = (i + 1) & (cap - 1);
+    }
+    tab[i].off = sl.off;
+    tab[i].len = sl.len;
   }
 }
 
@@ -735,6 +1199,85 @@ int build_sorted(ptc_ctx* ctx, ptc_hashtable* ht, const ptc_dtensor* t, bool f32
   return PTC_OK;
 }
 
+// Partitioned hashing (wide keys): partition by hash bucket, group each
+// bucket in LDS, compact the slices and write the probe index.
+int build_partitioned(ptc_ctx* ctx, ptc_hashtable* ht, const ptc_dtensor* t, bool f32, bool rest64) {
+  const uint64_t N = t->N;
+  hipStream_t st = ctx->stream;
+  uint32_t nb = 1;
+  int bbits = 0;
+  while (nb < PT_MAX_BUCKETS && (uint64_t)nb * PT_BUCKET_ITEMS < N) {
+    nb <<= 1;
+    ++bbits;
+  }
+  const uint64_t ntiles = (N + PT_TILE - 1) / PT_TILE;
+  // region mode: every bucket owns cap / nb slots of the probe index
+  const bool region = !ht->direct && ht->cap / 8 >= nb;
+  Workspace ws(ctx);
+  const size_t pay_bytes = rest64 || !f32 ? 16 : 8;
+  uint32_t* hist = ws.get<uint32_t>((uint64_t)nb * ntiles);
+  unsigned long long* off = ws.get<unsigned long long>((uint64_t)nb * ntiles + 1);
+  unsigned long long* pkey = ws.get<unsigned long long>(N);
+  uint32_t* pid = ws.get<uint32_t>(N);
+  void* ppay = ws.get<unsigned char>(N * pay_bytes);
+  unsigned long long* gkey = ws.get<unsigned long long>(N);
+  uint32_t* gcnt = ws.get<uint32_t>(N);
+  KtSlice* tmps = ws.get<KtSlice>(N);
+  uint32_t* bkeys = ws.get<uint32_t>(nb);
+  unsigned long long* sbase = ws.get<unsigned long long>(nb + 1);
+  uint32_t* ovf = ws.get<uint32_t>(N + 1);  // count, then the list
+  ht->d_total = dalloc_n<unsigned long long>(ctx, 1);
+  ht->slices_cap = std::max<uint64_t>(1, std::min<uint64_t>(N, ht->cap));
+  ht->slices = dalloc_n<KtSlice>(ctx, ht->slices_cap);
+  if (!hist || !off || !pkey || !pid || !ppay || !gkey || !gcnt || !tmps || !bkeys || !sbase || !ovf ||
+      !ht->d_total || !ht->slices)
+    return PTC_ERR_NOMEM;
+  int rc = fill_zero(ctx, ovf, 4);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_pt_hist, dim3((unsigned)ntiles), dim3(PT_THREADS), 0, st, t->hedges, t->d, N, ht->codec, bbits,
+                     nb, ntiles, hist);
+  if ((rc = scan_u32_to_u64(ctx, hist, off, (uint64_t)nb * ntiles, off + (uint64_t)nb * ntiles))) return rc;
+  if (!region)
+    hipLaunchKernelGGL(k_kt_clear, dim3(grid_for(ht->cap, 256)), dim3(256), 0, st, ht->entries, (uint32_t*)nullptr,
+                       ht->cap);
+#define PTC_PT(RT, VT, VALS, RK, GV)                                                                                 \
+  do {                                                                                                               \
+    const size_t lds = (size_t)PT_TILE * (8 + sizeof(RT) + 2) + (size_t)nb * 8;                                   \
+    static size_t lds_set = 0;                                                                                       \
+    if (lds > 65536 && lds > lds_set) {                                                                              \
+      PTC_HIP(ctx, hipFuncSetAttribute((const void*)k_pt_scatter<RT, VT>,                                           \
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));                      \
+      lds_set = lds;                                                                                                 \
+    }                                                                                                                \
+    hipLaunchKernelGGL((k_pt_scatter<RT, VT>), dim3((unsigned)(8 * ((ntiles + 7) / 8))), dim3(PT_THREADS), lds, st,  \
+                       t->hedges, t->d, N,                                                                          \
+                       ht->codec, ht->rest, (const VT*)(VALS), bbits, nb, ntiles, (const uint32_t*)hist,            \
+                       (const unsigned long long*)off, pkey, pid, (KtPay<RT, VT>*)ppay);                            \
+    hipLaunchKernelGGL((k_pt_group<RT, VT>), dim3(nb), dim3(PT_GROUP_THREADS), 0, st,                              \
+                       (const unsigned long long*)off,                                                              \
+                       ntiles, nb, N, (const unsigned long long*)pkey, (const uint32_t*)pid,                        \
+                       (const KtPay<RT, VT>*)ppay, gkey, gcnt, tmps, bkeys, ht->perm, RK, GV, ht->entries, ht->cap, \
+                       region ? 1 : 0, ovf, ovf + 1);                                                               \
+  } while (0)
+  if (rest64) {
+    if (f32) PTC_PT(unsigned long long, float, t->vals32, ht->rest_key64, ht->gval32);
+    else PTC_PT(unsigned long long, double, t->vals, ht->rest_key64, ht->gval);
+  } else {
+    if (f32) PTC_PT(uint32_t, float, t->vals32, ht->rest_key32, ht->gval32);
+    else PTC_PT(uint32_t, double, t->vals, ht->rest_key32, ht->gval);
+  }
+#undef PTC_PT
+  if ((rc = scan_u32_to_u64(ctx, bkeys, sbase, nb, sbase + nb))) return rc;
+  hipLaunchKernelGGL(k_pt_slices, dim3(nb), dim3(256), 0, st, (const unsigned long long*)off, ntiles, nb,
+                     (const uint32_t*)bkeys, (const unsigned long long*)sbase, (const KtSlice*)tmps, ht->slices,
+                     ht->entries, ht->cap, region ? 0 : 1, ht->direct ? 1 : 0, ht->d_total, N);
+  if (region)
+    hipLaunchKernelGGL(k_pt_overflow, dim3(64), dim3(256), 0, st, (const uint32_t*)ovf, (const uint32_t*)(ovf + 1),
+                       (const KtSlice*)tmps, ht->entries, ht->cap);
+  PTC_HIP(ctx, hipGetLastError());
+  return PTC_OK;
+}
+
 }  // namespace
 
 extern "C" int ptc_hashtable_build(ptc_ctx* ctx, const ptc_dtensor* t, ptc_hashtable** out) {
@@ -744,7 +1287,7 @@ extern "C" int ptc_hashtable_build(ptc_ctx* ctx, const ptc_dtensor* t, ptc_hasht
 extern "C" int ptc_hashtable_build_ex(ptc_ctx* ctx, const ptc_dtensor* t, int grouping, ptc_hashtable** out) {
   if (!ctx || !t || !out) return PTC_ERR_ARG;
   *out = nullptr;
-  if (grouping < PTC_GROUP_AUTO || grouping > PTC_GROUP_SORT) return set_err(ctx, PTC_ERR_ARG, "grouping %d", grouping);
+  if (grouping < PTC_GROUP_AUTO || grouping > PTC_GROUP_PARTITION) return set_err(ctx, PTC_ERR_ARG, "grouping %d", grouping);
   PTC_HIP(ctx, hipSetDevice(ctx->device));
   ptc_hashtable* ht = new ptc_hashtable();
   ht->ctx = ctx;
@@ -797,6 +1340,26 @@ extern "C" int ptc_hashtable_build_ex(ptc_ctx* ctx, const ptc_dtensor* t, int gr
       return PTC_ERR_NOMEM;
     }
     rc = build_sorted(ctx, ht, t, f32, rest64);
+    if (rc) {
+      ptc_hashtable_free(ht);
+      return rc;
+    }
+    hipEventRecord(ht->ev_end, ctx->stream);
+    *out = ht;
+    return PTC_OK;
+  }
+  // partitioned hashing: requested, or the default for wide hashed key spaces
+  const bool partitionable = N > 0 && N <= PT_MAX_N;
+  if (grouping == PTC_GROUP_PARTITION && !partitionable) {
+    ptc_hashtable_free(ht);
+    return set_err(ctx, PTC_ERR_UNSUPPORTED, "partitioned grouping needs 1 to %llu nonzeros", (unsigned long long)PT_MAX_N);
+  }
+  if (partitionable && (grouping == PTC_GROUP_PARTITION || (grouping == PTC_GROUP_AUTO && !ht->direct))) {
+    if (!ht->entries || !ht->perm || !(ht->gval || ht->gval32) || !(ht->rest_key32 || ht->rest_key64)) {
+      ptc_hashtable_free(ht);
+      return PTC_ERR_NOMEM;
+    }
+    rc = build_partitioned(ctx, ht, t, f32, rest64);
     if (rc) {
       ptc_hashtable_free(ht);
       return rc;
