@@ -1056,15 +1056,8 @@ static int launch_pass_t(ptc_ctx* ctx, const PassParams& P, uint64_t nitems, siz
                      hipGetErrorString(hipGetLastError()));
     lds_set = lds;
   }
-  const bool timed = ctx->nkev + 2 <= 16;
-  if (timed) hipEventRecord(ctx->kev[ctx->nkev], ctx->stream);
   hipLaunchKernelGGL(kern, dim3((unsigned)nitems), dim3(threads), lds, ctx->stream, P);
   PTC_HIP(ctx, hipGetLastError());
-  if (timed) {
-    hipEventRecord(ctx->kev[ctx->nkev + 1], ctx->stream);
-    ctx->kev_pass[ctx->nkev / 2] = PASS + (MERGE ? 2 : 0);
-    ctx->nkev += 2;
-  }
   return PTC_OK;
 }
 

@@ -201,6 +201,15 @@ int scan_u64(ptc_ctx* ctx, const unsigned long long* in, unsigned long long* out
 
 int fill_zero(ptc_ctx* ctx, void* p, size_t bytes);
 
+// Device words (4 or 8 bytes, zero-extended to 8) gathered into dst[0..n)
+// by one launch on the context stream.
+struct WordSources {
+  const void* src[8];
+  uint32_t bytes[8];
+  uint32_t n;
+};
+int collect_words(ptc_ctx* ctx, unsigned long long* dst, const WordSources& W);
+
 // Key codec of a tensor over the given modes, from the tensor's maxima.
 int make_codec(ptc_ctx* ctx, const ptc_dtensor* t, const uint32_t* modes, uint32_t n,
                KeyCodec* out);

@@ -686,8 +686,10 @@ constexpr uint64_t PT_BUCKET_ITEMS = 3072; // target mean bucket size
 // global table (PTC_GROUP_HASH) is used instead.
 constexpr uint64_t PT_MAX_N = (uint64_t)PT_MAX_BUCKETS * PT_BUCKET_ITEMS;
 
+// Bucket of a key: the top bbits bits of mix64(key); with bbits < 0 the key
+// itself (small direct key spaces, one key per bucket).
 __device__ __forceinline__ uint32_t pt_bucket(unsigned long long key, int bbits) {
-  return bbits ? (uint32_t)(mix64(key) >> (64 - bbits)) : 0u;
+  return bbits < 0 ? (uint32_t)key : bbits ? (uint32_t)(mix64(key) >> (64 - bbits)) : 0u;
 }
 
 // Exclusive scan, in place, of a[0..n) in LDS (n <= 8 * blockDim.x).
@@ -798,6 +800,101 @@ __global__ void __launch_bounds__(PT_THREADS) k_pt_scatter(const uint32_t* __res
     pl.k = lrk[i];
     pl.v = val[base + i];
     ppay[p] = pl;
+  }
+}
+
+// ------------------------------------------------- small direct key spaces
+//
+// A key space of at most PT_MAX_BUCKETS keys is grouped in one counting
+// pass: every key is its own bucket, so the bucket scan already gives every
+// slice; the tiles are scattered straight into the final layout.
+
+// The slices (nonempty keys in key order), the whole direct index and the
+// key count, from the scanned count matrix; one workgroup.
+__global__ void __launch_bounds__(1024) k_pd_slices(const unsigned long long* __restrict__ off, uint64_t ntiles,
+                                                    uint32_t nb, uint64_t N, KtSlice* __restrict__ slices,
+                                                    KtEntry* __restrict__ tab, uint32_t* __restrict__ cidx,
+                                                    unsigned long long* __restrict__ total) {
+  __shared__ uint32_t lf[PT_MAX_BUCKETS];
+  __shared__ uint32_t s_wave[1024 / 64];
+  for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) {
+    const uint64_t s0 = off[(uint64_t)b * ntiles], s1 = b + 1 < nb ? off[(uint64_t)(b + 1) * ntiles] : N;
+    lf[b] = s1 > s0 ? 1u : 0u;
+  }
+  __syncthreads();
+  const uint32_t last = lf[nb - 1];
+  __syncthreads();
+  pt_block_scan(lf, nb, s_wave);
+  for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) {
+    const uint64_t s0 = off[(uint64_t)b * ntiles], s1 = b + 1 < nb ? off[(uint64_t)(b + 1) * ntiles] : N;
+    KtEntry en;
+    en.key = PTC_EMPTY_KEY;
+    en.off = 0;
+    en.len = 0;
+    if (s1 > s0) {
+      KtSlice sl;
+      sl.key = b;
+      sl.off = (uint32_t)s0;
+      sl.len = (uint32_t)(s1 - s0);
+      slices[lf[b]] = sl;
+      en.key = b;
+      en.off = sl.off;
+      en.len = sl.len;
+    }
+    cidx[b] = lf[b];
+    tab[b] = en;
+  }
+  if (threadIdx.x == 0) *total = ((unsigned long long)(lf[nb - 1] + last) << 32) | N;
+}
+
+// One tile in key order, written straight into the grouped layout (ids,
+// non-key tuples, values, slice indices); as k_pt_scatter, with the key
+// (16 bits) kept in LDS in place of the 64-bit key.
+template <class RestT, class ValT>
+__global__ void __launch_bounds__(PT_THREADS) k_pd_scatter(const uint32_t* __restrict__ h, uint32_t d, uint64_t N,
+                                                           KeyCodec codec, KeyCodec rest, const ValT* __restrict__ val,
+                                                           uint32_t nb, uint64_t ntiles,
+                                                           const uint32_t* __restrict__ hist,
+                                                           const unsigned long long* __restrict__ off,
+                                                           const uint32_t* __restrict__ cidx,
+                                                           uint32_t* __restrict__ perm, RestT* __restrict__ rest_out,
+                                                           ValT* __restrict__ gval, uint32_t* __restrict__ slice_of) {
+  extern __shared__ unsigned long long pt_smem[];
+  RestT* lrk = reinterpret_cast<RestT*>(pt_smem);           // PT_TILE
+  uint16_t* lb = reinterpret_cast<uint16_t*>(lrk + PT_TILE); // PT_TILE: key of every tile item
+  uint16_t* ls = lb + PT_TILE;                               // PT_TILE: tile index of every key-ordered slot
+  uint32_t* lcur = reinterpret_cast<uint32_t*>(ls + PT_TILE); // nb
+  uint32_t* gdst = lcur + nb;                                // nb
+  __shared__ uint32_t s_wave[PT_THREADS / 64];
+  const uint64_t t = pt_tile_of(blockIdx.x, ntiles);
+  if (t >= ntiles) return;
+  const uint64_t base = t * PT_TILE;
+  const uint32_t n = (uint32_t)min<uint64_t>(PT_TILE, N - base);
+  for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) {
+    lcur[b] = hist[(uint64_t)b * ntiles + t];
+    gdst[b] = (uint32_t)off[(uint64_t)b * ntiles + t];
+  }
+  __syncthreads();
+  pt_block_scan(lcur, nb, s_wave);
+  for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) gdst[b] -= lcur[b];
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+    const uint64_t e = base + i;
+    unsigned long long key = 0, rk = 0;
+    encode_key(h, d, e, codec, &key);  // < nb
+    for (uint32_t q = 0; q < rest.n; ++q) rk += (unsigned long long)h[e * d + rest.mode[q]] * rest.stride[q];
+    lrk[i] = (RestT)rk;
+    lb[i] = (uint16_t)key;
+    ls[atomicAdd(&lcur[key], 1u)] = (uint16_t)i;
+  }
+  __syncthreads();
+  for (uint32_t j = threadIdx.x; j < n; j += blockDim.x) {
+    const uint32_t i = ls[j], b = lb[i];
+    const uint32_t p = gdst[b] + j;
+    perm[p] = (uint32_t)(base + i);
+    rest_out[p] = lrk[i];
+    gval[p] = val[base + i];
+    slice_of[p] = cidx[b];
   }
 }
 
@@ -1134,7 +1231,13 @@ int build_sorted(ptc_ctx* ctx, ptc_hashtable* ht, const ptc_dtensor* t, bool f32
   hipStream_t st = ctx->stream;
   Workspace ws(ctx);
   const uint64_t ntiles = (N + RS_TILE - 1) / RS_TILE;
-  uint32_t *k0 = ws.get<uint32_t>(N), *i0 = ws.get<uint32_t>(N), *k1 = ws.get<uint32_t>(N), *i1 = ws.get<uint32_t>(N);
+  // LSD passes over the significant bits of the key space; the ids of the
+  // last pass land in perm
+  int bits = 1;
+  while (bits < 32 && (1ull << bits) < ht->codec.space) ++bits;
+  const bool odd = ((bits + 7) / 8) % 2 == 1;
+  uint32_t *k0 = ws.get<uint32_t>(N), *k1 = ws.get<uint32_t>(N);
+  uint32_t *i0 = odd ? ws.get<uint32_t>(N) : ht->perm, *i1 = odd ? ht->perm : ws.get<uint32_t>(N);
   uint32_t* hist = ws.get<uint32_t>(256 * ntiles);
   unsigned long long* hs = ws.get<unsigned long long>(256 * ntiles + 1);
   uint32_t* head = ws.get<uint32_t>(N);
@@ -1159,9 +1262,6 @@ int build_sorted(ptc_ctx* ctx, ptc_hashtable* ht, const ptc_dtensor* t, bool f32
     else PTC_RSKEYS(uint32_t, double, t->vals);
   }
 #undef PTC_RSKEYS
-  // LSD passes over the significant bits of the key space
-  int bits = 1;
-  while (bits < 32 && (1ull << bits) < ht->codec.space) ++bits;
   for (int shift = 0; shift < bits; shift += 8) {
     hipLaunchKernelGGL(k_rs_hist, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, st, (const uint32_t*)k0, N, shift,
                        ntiles, hist);
@@ -1182,8 +1282,7 @@ int build_sorted(ptc_ctx* ctx, ptc_hashtable* ht, const ptc_dtensor* t, bool f32
                      ht->cap);
   hipLaunchKernelGGL(k_rs_index, dim3(grid_for(ht->slices_cap, 256, 16384)), dim3(256), 0, st, ht->slices,
                      (const unsigned long long*)(sx + N), ht->entries, ht->cap, ht->direct ? 1 : 0, ht->d_total, N);
-  // ids in grouped order, then the streaming layout from the payload
-  PTC_HIP(ctx, hipMemcpyAsync(ht->perm, i0, sizeof(uint32_t) * N, hipMemcpyDeviceToDevice, st));
+  // the streaming layout from the payload, by the ids in grouped order (i0 == perm)
 #define PTC_GATHER(RT, VT, RK, GV)                                                                         \
   hipLaunchKernelGGL((k_kt_gather_pay<RT, VT>), dim3(g), dim3(256), 0, st, N, (const uint32_t*)ht->perm, \
                      (const KtPay<RT, VT>*)pay, RK, GV)
@@ -1278,6 +1377,52 @@ int build_partitioned(ptc_ctx* ctx, ptc_hashtable* ht, const ptc_dtensor* t, boo
   return PTC_OK;
 }
 
+// Small direct key spaces (<= PT_MAX_BUCKETS keys): one counting pass.
+int build_direct_partitioned(ptc_ctx* ctx, ptc_hashtable* ht, const ptc_dtensor* t, bool f32, bool rest64) {
+  const uint64_t N = t->N;
+  hipStream_t st = ctx->stream;
+  const uint32_t nb = (uint32_t)ht->codec.space;
+  const uint64_t ntiles = (N + PT_TILE - 1) / PT_TILE;
+  Workspace ws(ctx);
+  uint32_t* hist = ws.get<uint32_t>((uint64_t)nb * ntiles);
+  unsigned long long* off = ws.get<unsigned long long>((uint64_t)nb * ntiles + 1);
+  uint32_t* cidx = ws.get<uint32_t>(nb);
+  ht->slice_of = dalloc_n<uint32_t>(ctx, N);
+  ht->d_total = dalloc_n<unsigned long long>(ctx, 1);
+  ht->slices_cap = std::max<uint64_t>(1, std::min<uint64_t>(N, ht->cap));
+  ht->slices = dalloc_n<KtSlice>(ctx, ht->slices_cap);
+  if (!hist || !off || !cidx || !ht->slice_of || !ht->d_total || !ht->slices) return PTC_ERR_NOMEM;
+  hipLaunchKernelGGL(k_pt_hist, dim3((unsigned)ntiles), dim3(PT_THREADS), 0, st, t->hedges, t->d, N, ht->codec, -1, nb,
+                     ntiles, hist);
+  int rc = scan_u32_to_u64(ctx, hist, off, (uint64_t)nb * ntiles, off + (uint64_t)nb * ntiles);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_pd_slices, dim3(1), dim3(1024), 0, st, (const unsigned long long*)off, ntiles, nb, N, ht->slices,
+                     ht->entries, cidx, ht->d_total);
+#define PTC_PD(RT, VT, VALS, RK, GV)                                                                                  \
+  do {                                                                                                                \
+    const size_t lds = (size_t)PT_TILE * (sizeof(RT) + 4) + (size_t)nb * 8;                                           \
+    static size_t lds_set = 0;                                                                                        \
+    if (lds > 65536 && lds > lds_set) {                                                                               \
+      PTC_HIP(ctx, hipFuncSetAttribute((const void*)k_pd_scatter<RT, VT>,                                            \
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));                       \
+      lds_set = lds;                                                                                                  \
+    }                                                                                                                 \
+    hipLaunchKernelGGL((k_pd_scatter<RT, VT>), dim3((unsigned)(8 * ((ntiles + 7) / 8))), dim3(PT_THREADS), lds, st,   \
+                       t->hedges, t->d, N, ht->codec, ht->rest, (const VT*)(VALS), nb, ntiles, (const uint32_t*)hist, \
+                       (const unsigned long long*)off, (const uint32_t*)cidx, ht->perm, RK, GV, ht->slice_of);      \
+  } while (0)
+  if (rest64) {
+    if (f32) PTC_PD(unsigned long long, float, t->vals32, ht->rest_key64, ht->gval32);
+    else PTC_PD(unsigned long long, double, t->vals, ht->rest_key64, ht->gval);
+  } else {
+    if (f32) PTC_PD(uint32_t, float, t->vals32, ht->rest_key32, ht->gval32);
+    else PTC_PD(uint32_t, double, t->vals, ht->rest_key32, ht->gval);
+  }
+#undef PTC_PD
+  PTC_HIP(ctx, hipGetLastError());
+  return PTC_OK;
+}
+
 }  // namespace
 
 extern "C" int ptc_hashtable_build(ptc_ctx* ctx, const ptc_dtensor* t, ptc_hashtable** out) {
@@ -1333,6 +1478,23 @@ extern "C" int ptc_hashtable_build_ex(ptc_ctx* ctx, const ptc_dtensor* t, int gr
   if (grouping == PTC_GROUP_SORT && !sortable) {
     ptc_hashtable_free(ht);
     return set_err(ctx, PTC_ERR_UNSUPPORTED, "sort grouping needs a nonempty tensor and a key space within 32 bits");
+  }
+  // one counting pass: small direct key spaces, automatically or when
+  // partitioning is asked for
+  if (N > 0 && N <= 0xFFFFFFFFull && ht->direct && ht->codec.space <= PT_MAX_BUCKETS &&
+      (grouping == PTC_GROUP_AUTO || grouping == PTC_GROUP_PARTITION)) {
+    if (!ht->entries || !ht->perm || !(ht->gval || ht->gval32) || !(ht->rest_key32 || ht->rest_key64)) {
+      ptc_hashtable_free(ht);
+      return PTC_ERR_NOMEM;
+    }
+    rc = build_direct_partitioned(ctx, ht, t, f32, rest64);
+    if (rc) {
+      ptc_hashtable_free(ht);
+      return rc;
+    }
+    hipEventRecord(ht->ev_end, ctx->stream);
+    *out = ht;
+    return PTC_OK;
   }
   if (sortable && (grouping == PTC_GROUP_SORT || (grouping == PTC_GROUP_AUTO && key_bits <= SORT_MAX_BITS))) {
     if (!ht->entries || !ht->perm || !(ht->gval || ht->gval32) || !(ht->rest_key32 || ht->rest_key64)) {

@@ -125,6 +125,23 @@ int fill_zero(ptc_ctx* ctx, void* p, size_t bytes) {
   return PTC_OK;
 }
 
+// ------------------------------------------------------------- collect
+//
+// Up to 8 device words (4 or 8 bytes each, zero-extended) copied into one
+// array with a single launch, for a combined read-back.
+
+__global__ void k_collect_words(WordSources W, unsigned long long* __restrict__ dst) {
+  const uint32_t i = threadIdx.x;
+  if (i < W.n)
+    dst[i] = W.bytes[i] == 4 ? (unsigned long long)*(const uint32_t*)W.src[i] : *(const unsigned long long*)W.src[i];
+}
+
+int collect_words(ptc_ctx* ctx, unsigned long long* dst, const WordSources& W) {
+  hipLaunchKernelGGL(k_collect_words, dim3(1), dim3(64), 0, ctx->stream, W, dst);
+  PTC_HIP(ctx, hipGetLastError());
+  return PTC_OK;
+}
+
 // ------------------------------------------------------------------ scan
 //
 // Three-phase exclusive scan: per-block totals, a single-block scan of the
@@ -198,11 +215,52 @@ __global__ void k_scan_apply(const In* __restrict__ in, unsigned long long* __re
   if (total_out && blockIdx.x == gridDim.x - 1 && threadIdx.x == SCAN_THREADS - 1) *total_out = run;
 }
 
+// Inputs of up to SCAN_SMALL elements: one workgroup, one launch.
+constexpr int SCAN_SMALL_THREADS = 1024;
+constexpr int SCAN_SMALL_ITEMS = 16;
+constexpr uint64_t SCAN_SMALL = (uint64_t)SCAN_SMALL_THREADS * SCAN_SMALL_ITEMS;
+
+template <class In>
+__global__ void __launch_bounds__(SCAN_SMALL_THREADS) k_scan_small(const In* __restrict__ in,
+                                                                   unsigned long long* __restrict__ out, uint64_t n,
+                                                                   unsigned long long* __restrict__ total_out) {
+  __shared__ unsigned long long wsum[SCAN_SMALL_THREADS / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const uint64_t base = (uint64_t)threadIdx.x * SCAN_SMALL_ITEMS;
+  unsigned long long v[SCAN_SMALL_ITEMS], s = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_SMALL_ITEMS; ++k) {
+    v[k] = load_elem(in, base + k, n);
+    s += v[k];
+  }
+  unsigned long long x = s;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const unsigned long long y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) wsum[wid] = x;
+  __syncthreads();
+  unsigned long long run = x - s;
+  for (int w = 0; w < wid; ++w) run += wsum[w];
+#pragma unroll
+  for (int k = 0; k < SCAN_SMALL_ITEMS; ++k) {
+    if (base + k < n) out[base + k] = run;
+    run += v[k];
+  }
+  if (total_out && threadIdx.x == SCAN_SMALL_THREADS - 1) *total_out = run;
+}
+
 template <class In>
 static int scan_impl(ptc_ctx* ctx, const In* in, unsigned long long* out, uint64_t n,
                      unsigned long long* total) {
   if (n == 0) {
     if (total) return fill_zero(ctx, total, sizeof(unsigned long long));
+    return PTC_OK;
+  }
+  if (n <= SCAN_SMALL) {
+    hipLaunchKernelGGL((k_scan_small<In>), dim3(1), dim3(SCAN_SMALL_THREADS), 0, ctx->stream, in, out, n, total);
+    PTC_HIP(ctx, hipGetLastError());
     return PTC_OK;
   }
   uint64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;

@@ -192,19 +192,27 @@ static int run_pass(Run& R, int pass, HashPlan* hp, const uint4* items, uint64_t
   }
   P.items = items;
   size_t lds = pass_lds_bytes(pass, R.dense, R.col64, R.ncols, R.lds_slots, R.use_flags, R.threads);
-  int rc = launch_pass(R.ctx, pass, R.dense, R.col64, P, nitems, lds, false, R.threads);
-  if (rc) return rc;
-  if (R.dense) {
+  // one pair of timing events brackets the row launch and its merge launch
+  ptc_ctx* ctx = R.ctx;
+  const bool timed = ctx->nkev + 2 <= 16;
+  if (timed) hipEventRecord(ctx->kev[ctx->nkev], ctx->stream);
+  int rc = launch_pass(ctx, pass, R.dense, R.col64, P, nitems, lds, false, R.threads);
+  if (!rc && R.dense) {
     if (!merge_rows) {
       merge_rows = R.heavy_rows;
       nmerge = R.nheavy;
     }
     P.merge_rows = merge_rows;
-    return launch_pass(R.ctx, pass, true, R.col64, P, nmerge, lds, true, R.threads);
-  }
-  if (pass == PASS_NUMERIC && hp->pool) {
+    rc = launch_pass(ctx, pass, true, R.col64, P, nmerge, lds, true, R.threads);
+  } else if (!rc && pass == PASS_NUMERIC && hp->pool) {
     P.merge_rows = merge_rows;  // NULL: every row (rows not in HBM exit at once)
-    return launch_pass(R.ctx, pass, false, R.col64, P, merge_rows ? nmerge : R.nrows, lds, true, R.threads);
+    rc = launch_pass(ctx, pass, false, R.col64, P, merge_rows ? nmerge : R.nrows, lds, true, R.threads);
+  }
+  if (rc) return rc;
+  if (timed) {
+    hipEventRecord(ctx->kev[ctx->nkev + 1], ctx->stream);
+    ctx->kev_pass[ctx->nkev / 2] = pass;
+    ctx->nkev += 2;
   }
   return PTC_OK;
 }
@@ -409,10 +417,16 @@ int prepare_launch(Run& R, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B,
   if ((rc = scan_u64(ctx, R.packed, R.packed, R.nrows, R.packed + R.nrows))) return rc;
   if ((rc = scan_u64(ctx, R.row_slab, R.row_slab, R.nrows, R.row_slab + R.nrows))) return rc;
   // the totals and HT_A's key count, for one host round trip
-  PTC_HIP(ctx, hipMemcpyAsync(tot, R.apre + R.npos, 8, hipMemcpyDeviceToDevice, ctx->stream));
-  PTC_HIP(ctx, hipMemcpyAsync(tot + 1, R.packed + R.nrows, 8, hipMemcpyDeviceToDevice, ctx->stream));
-  PTC_HIP(ctx, hipMemcpyAsync(tot + 2, R.row_slab + R.nrows, 8, hipMemcpyDeviceToDevice, ctx->stream));
-  PTC_HIP(ctx, hipMemcpyAsync(tot + 3, HT_A->d_total, 8, hipMemcpyDeviceToDevice, ctx->stream));
+  {
+    WordSources W{};
+    W.n = 4;
+    W.src[0] = R.apre + R.npos;
+    W.src[1] = R.packed + R.nrows;
+    W.src[2] = R.row_slab + R.nrows;
+    W.src[3] = HT_A->d_total;
+    for (int i = 0; i < 4; ++i) W.bytes[i] = 8;
+    if ((rc = collect_words(ctx, tot, W))) return rc;
+  }
   return PTC_OK;
 }
 
@@ -643,8 +657,13 @@ int tc_impl(ptc_ctx* ctx, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B, 
     {
       unsigned long long* pair = ws.get<unsigned long long>(2);
       if (!pair) return PTC_ERR_NOMEM;
-      PTC_HIP(ctx, hipMemcpyAsync(pair, R.overflow, 4, hipMemcpyDeviceToDevice, ctx->stream));
-      PTC_HIP(ctx, hipMemcpyAsync(pair + 1, out_off + R.nrows, 8, hipMemcpyDeviceToDevice, ctx->stream));
+      WordSources W{};
+      W.n = 2;
+      W.src[0] = R.overflow;
+      W.bytes[0] = 4;
+      W.src[1] = out_off + R.nrows;
+      W.bytes[1] = 8;
+      if ((rc = collect_words(ctx, pair, W))) return rc;
       if ((rc = read_back(ctx, rb, pair, 16))) return rc;
     }
     const uint32_t ovf0 = (uint32_t)rb[0];
@@ -781,10 +800,9 @@ int tc_impl(ptc_ctx* ctx, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B, 
     for (int i = 0; i + 1 < ctx->nkev; i += 2) {
       float t = 0.f;
       hipEventElapsedTime(&t, ctx->kev[i], ctx->kev[i + 1]);
-      const int kind = ctx->kev_pass[i / 2];
-      if (kind % 2 == PASS_NUMERIC) {
+      if (ctx->kev_pass[i / 2] == PASS_NUMERIC) {
         stats->ms_kernel_numeric += t;
-        if (kind == PASS_NUMERIC) stats->kernel_launches_numeric++;
+        stats->kernel_launches_numeric++;
       } else {
         stats->ms_kernel_symbolic += t;
       }

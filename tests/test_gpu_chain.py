@@ -4,8 +4,7 @@ against the oracle run on the oracle's own intermediate (coordinates
 bit-exact after sorting, values within 1e-11 relative: the intermediate's
 values already differ from the oracle's by up to 1e-12 relative and the next
 contraction sums them); the table formed from the output's rows is checked
-against a table built on the same output (same keys, same id lists in the
-same order)."""
+against a table built on the same output (same keys, same ids per key)."""
 import numpy as np
 import pytest
 
@@ -39,7 +38,8 @@ def check(O, ref):
 
 def same_table(ctx, HT, dO, key_modes):
     """HT (from the output's rows) against a table built on the same output:
-    the same number of keys and, for every key, the same ids in order."""
+    the same number of keys and, for every key, the same ids (a built table
+    may order a slice's ids differently; the row table keeps output order)."""
     built = P.buildHashTable(dO, ctx)
     assert HT.info() == built.info()
     Oh = dO.download().hedges_array
@@ -48,7 +48,8 @@ def same_table(ctx, HT, dO, key_modes):
     a = HT.find(keys)
     b = built.find(keys)
     for x, y in zip(a, b):
-        assert np.array_equal(x, y)
+        assert np.array_equal(x, np.sort(x))  # output order within a row
+        assert np.array_equal(x, np.sort(y))
     built.free()
 
 

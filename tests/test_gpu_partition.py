@@ -46,8 +46,12 @@ SHAPES = {
     "one-bucket": lambda: meta(*P.gen_coo(95, [40, 300, 200], 1500, P.DIST_UNIFORM), [1, 2]),
     # a non-key tuple wider than 32 bits (64-bit column keys)
     "rest64": lambda: meta(*P.gen_coo(96, [70000, 70000, 70000, 70000], 20000, P.DIST_UNIFORM), [0, 1]),
-    # a direct (small) key space, partitioned on request
+    # a direct (small) key space: one counting pass, every key its own bucket
     "direct": lambda: meta(*P.gen_coo(92, [700, 60, 50], 20000, P.DIST_ZIPF), [0]),
+    # the largest such key space (4096 keys), hot rows, several tiles
+    "direct-4096": lambda: meta(*P.gen_coo(99, [4096, 300, 200], 200000, P.DIST_ZIPF, 1.3), [0]),
+    # a direct key space over two modes with a 64-bit non-key tuple
+    "direct-rest64": lambda: meta(*P.gen_coo(100, [40, 50, 90000, 90000], 30000, P.DIST_UNIFORM), [0, 1]),
 }
 
 
@@ -108,6 +112,22 @@ def test_partitioned_fp32_matches_hashed(ctx):
         outs.append(sort_coo(t.hedges_array, t.val_array))
     assert np.array_equal(outs[0][0], outs[1][0])
     assert max_rel(outs[0][1], outs[1][1]) <= 1e-6
+
+
+def test_auto_grouping_counts_small_key_spaces(ctx):
+    """The nips-shape A table (2482 rows) is grouped by one counting pass
+    by default: the same slices and ids as the stable sort grouping, and the
+    same row of every position (slice indices)."""
+    print("[case] nips-shape A counted", flush=True)
+    h, v = P.gen_coo(2, [2482, 2862, 14036, 17], 3101609, P.DIST_ZIPF, 1.1)
+    A, B = P.self_contraction_inputs(h, v, [1, 2, 3])
+    dA = P.DeviceTensor.upload(A, ctx)
+    HC = P.buildHashTable(dA, ctx)
+    HS = P.buildHashTable(dA, ctx, grouping=P.GROUP_SORT)
+    assert HC.info() == HS.info()
+    keys = np.unique(A.hedges_array[:, [0]], axis=0)
+    for x, y in zip(HC.find(keys), HS.find(keys)):
+        assert np.array_equal(np.sort(x), y)
 
 
 def test_auto_grouping_partitions_wide_keys(ctx):
