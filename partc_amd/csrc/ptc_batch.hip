@@ -298,7 +298,9 @@ int plan_tables(ptc_batch* b) {
   const int n = b->n;
   int rc = PTC_OK;
   stats_mark(b, 0);
-  // ---- key tables: B then A_copy of every job (main.cpp:115-116), no waits
+  // ---- key tables: B and A_copy of every job (main.cpp:115-116), all
+  // built together, no waits
+  std::vector<const ptc_dtensor*> ts(2 * (size_t)n);
   for (int j = 0; j < n; ++j) {
     const ptc_dtensor* A = b->A[j];
     const uint32_t c = A->nk;
@@ -308,8 +310,14 @@ int plan_tables(ptc_batch* b) {
     for (uint32_t i = c; i < A->d; ++i) rot[i - c] = A->dim_tc[i];
     for (uint32_t i = 0; i < c; ++i) rot[A->d - c + i] = A->dim_tc[i];
     b->Ac[j] = view_with_keys(A, A->d - c, rot);
-    if ((rc = ptc_hashtable_build(ctx, b->B[j], &b->HB[j]))) return rc;
-    if ((rc = ptc_hashtable_build(ctx, b->Ac[j], &b->HA[j]))) return rc;
+    ts[2 * j] = b->B[j];
+    ts[2 * j + 1] = b->Ac[j];
+  }
+  std::vector<ptc_hashtable*> hs(2 * (size_t)n, nullptr);
+  if ((rc = hashtable_build_many(ctx, 2 * n, ts.data(), hs.data()))) return rc;
+  for (int j = 0; j < n; ++j) {
+    b->HB[j] = hs[2 * j];
+    b->HA[j] = hs[2 * j + 1];
   }
   stats_mark(b, 1);
   return PTC_OK;

@@ -150,6 +150,9 @@ namespace ptc {
 
 int set_err(ptc_ctx* ctx, int code, const char* fmt, ...);
 int hashtable_slice_of(ptc_ctx* ctx, const ptc_hashtable* h, const uint32_t** out);
+// Key tables of n tensors built together (one launch per step for all of
+// them); the same slices as ptc_hashtable_build gives each.
+int hashtable_build_many(ptc_ctx* ctx, int n, const ptc_dtensor* const* ts, ptc_hashtable** out);
 int hashtable_num_keys(ptc_ctx* ctx, const ptc_hashtable* h, uint64_t* out);
 double hashtable_ms_build(const ptc_hashtable* h);
 // Key table of a contraction's output O keyed on output row modes only

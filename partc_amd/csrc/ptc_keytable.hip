@@ -721,21 +721,27 @@ __device__ void pt_block_scan(uint32_t* a, uint32_t n, uint32_t* s_wave) {
   __syncthreads();
 }
 
-// bucket counts of one tile -> hist[bucket * ntiles + tile]
-__global__ void __launch_bounds__(PT_THREADS) k_pt_hist(const uint32_t* __restrict__ h, uint32_t d, uint64_t N,
-                                                        KeyCodec codec, int bbits, uint32_t nb, uint64_t ntiles,
-                                                        uint32_t* __restrict__ hist) {
+// bucket counts of tile t -> hist[bucket * ntiles + t]
+__device__ __forceinline__ void pt_hist_tile(const uint32_t* __restrict__ h, uint32_t d, uint64_t N,
+                                             const KeyCodec& codec, int bbits, uint32_t nb, uint64_t ntiles,
+                                             uint32_t* __restrict__ hist, uint64_t t) {
   __shared__ uint32_t lh[PT_MAX_BUCKETS];
   for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) lh[b] = 0;
   __syncthreads();
-  const uint64_t base = (uint64_t)blockIdx.x * PT_TILE, end = min(N, base + PT_TILE);
+  const uint64_t base = t * PT_TILE, end = min(N, base + PT_TILE);
   for (uint64_t e = base + threadIdx.x; e < end; e += blockDim.x) {
     unsigned long long key = 0;
     encode_key(h, d, e, codec, &key);  // the table's own tensor: always within radix
     atomicAdd(&lh[pt_bucket(key, bbits)], 1u);
   }
   __syncthreads();
-  for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) hist[(uint64_t)b * ntiles + blockIdx.x] = lh[b];
+  for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) hist[(uint64_t)b * ntiles + t] = lh[b];
+}
+
+__global__ void __launch_bounds__(PT_THREADS) k_pt_hist(const uint32_t* __restrict__ h, uint32_t d, uint64_t N,
+                                                        KeyCodec codec, int bbits, uint32_t nb, uint64_t ntiles,
+                                                        uint32_t* __restrict__ hist) {
+  pt_hist_tile(h, d, N, codec, bbits, nb, ntiles, hist, blockIdx.x);
 }
 
 // One tile into bucket order.  A rank pass reads the tile once (coalesced),
@@ -753,14 +759,13 @@ __device__ __forceinline__ uint64_t pt_tile_of(uint32_t wg, uint64_t ntiles) {
 }
 
 template <class RestT, class ValT>
-__global__ void __launch_bounds__(PT_THREADS) k_pt_scatter(const uint32_t* __restrict__ h, uint32_t d, uint64_t N,
-                                                           KeyCodec codec, KeyCodec rest, const ValT* __restrict__ val,
-                                                           int bbits, uint32_t nb, uint64_t ntiles,
-                                                           const uint32_t* __restrict__ hist,
-                                                           const unsigned long long* __restrict__ off,
-                                                           unsigned long long* __restrict__ pkey,
-                                                           uint32_t* __restrict__ pid,
-                                                           KtPay<RestT, ValT>* __restrict__ ppay) {
+__device__ __forceinline__ void pt_scatter_tile(const uint32_t* __restrict__ h, uint32_t d, uint64_t N,
+                                                const KeyCodec& codec, const KeyCodec& rest,
+                                                const ValT* __restrict__ val, int bbits, uint32_t nb, uint64_t ntiles,
+                                                const uint32_t* __restrict__ hist,
+                                                const unsigned long long* __restrict__ off,
+                                                unsigned long long* __restrict__ pkey, uint32_t* __restrict__ pid,
+                                                KtPay<RestT, ValT>* __restrict__ ppay, uint64_t t) {
   extern __shared__ unsigned long long pt_smem[];
   unsigned long long* lkey = pt_smem;                       // PT_TILE
   RestT* lrk = reinterpret_cast<RestT*>(lkey + PT_TILE);    // PT_TILE
@@ -768,8 +773,6 @@ __global__ void __launch_bounds__(PT_THREADS) k_pt_scatter(const uint32_t* __res
   uint32_t* lcur = reinterpret_cast<uint32_t*>(ls + PT_TILE); // nb: bucket start in the tile, then its cursor
   uint32_t* gdst = lcur + nb;                               // nb: output position of slot 0 of a bucket's run
   __shared__ uint32_t s_wave[PT_THREADS / 64];
-  const uint64_t t = pt_tile_of(blockIdx.x, ntiles);
-  if (t >= ntiles) return;
   const uint64_t base = t * PT_TILE;
   const uint32_t n = (uint32_t)min<uint64_t>(PT_TILE, N - base);
   for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) {
@@ -803,6 +806,20 @@ __global__ void __launch_bounds__(PT_THREADS) k_pt_scatter(const uint32_t* __res
   }
 }
 
+template <class RestT, class ValT>
+__global__ void __launch_bounds__(PT_THREADS) k_pt_scatter(const uint32_t* __restrict__ h, uint32_t d, uint64_t N,
+                                                           KeyCodec codec, KeyCodec rest, const ValT* __restrict__ val,
+                                                           int bbits, uint32_t nb, uint64_t ntiles,
+                                                           const uint32_t* __restrict__ hist,
+                                                           const unsigned long long* __restrict__ off,
+                                                           unsigned long long* __restrict__ pkey,
+                                                           uint32_t* __restrict__ pid,
+                                                           KtPay<RestT, ValT>* __restrict__ ppay) {
+  const uint64_t t = pt_tile_of(blockIdx.x, ntiles);
+  if (t >= ntiles) return;
+  pt_scatter_tile<RestT, ValT>(h, d, N, codec, rest, val, bbits, nb, ntiles, hist, off, pkey, pid, ppay, t);
+}
+
 // ------------------------------------------------- small direct key spaces
 //
 // A key space of at most PT_MAX_BUCKETS keys is grouped in one counting
@@ -810,15 +827,17 @@ __global__ void __launch_bounds__(PT_THREADS) k_pt_scatter(const uint32_t* __res
 // slice; the tiles are scattered straight into the final layout.
 
 // The slices (nonempty keys in key order), the whole direct index and the
-// key count, from the scanned count matrix; one workgroup.
-__global__ void __launch_bounds__(1024) k_pd_slices(const unsigned long long* __restrict__ off, uint64_t ntiles,
-                                                    uint32_t nb, uint64_t N, KtSlice* __restrict__ slices,
-                                                    KtEntry* __restrict__ tab, uint32_t* __restrict__ cidx,
-                                                    unsigned long long* __restrict__ total) {
+// key count, from the scanned count matrix; one workgroup.  Positions in
+// off start at obase (the table's first position in a set of tables).
+__device__ __forceinline__ void pd_slices_one(const unsigned long long* __restrict__ off, uint64_t ntiles,
+                                              uint32_t nb, uint64_t N, uint64_t obase, KtSlice* __restrict__ slices,
+                                              KtEntry* __restrict__ tab, uint32_t* __restrict__ cidx,
+                                              unsigned long long* __restrict__ total) {
   __shared__ uint32_t lf[PT_MAX_BUCKETS];
   __shared__ uint32_t s_wave[1024 / 64];
   for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) {
-    const uint64_t s0 = off[(uint64_t)b * ntiles], s1 = b + 1 < nb ? off[(uint64_t)(b + 1) * ntiles] : N;
+    const uint64_t s0 = off[(uint64_t)b * ntiles] - obase;
+    const uint64_t s1 = b + 1 < nb ? off[(uint64_t)(b + 1) * ntiles] - obase : N;
     lf[b] = s1 > s0 ? 1u : 0u;
   }
   __syncthreads();
@@ -826,7 +845,8 @@ __global__ void __launch_bounds__(1024) k_pd_slices(const unsigned long long* __
   __syncthreads();
   pt_block_scan(lf, nb, s_wave);
   for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) {
-    const uint64_t s0 = off[(uint64_t)b * ntiles], s1 = b + 1 < nb ? off[(uint64_t)(b + 1) * ntiles] : N;
+    const uint64_t s0 = off[(uint64_t)b * ntiles] - obase;
+    const uint64_t s1 = b + 1 < nb ? off[(uint64_t)(b + 1) * ntiles] - obase : N;
     KtEntry en;
     en.key = PTC_EMPTY_KEY;
     en.off = 0;
@@ -847,18 +867,25 @@ __global__ void __launch_bounds__(1024) k_pd_slices(const unsigned long long* __
   if (threadIdx.x == 0) *total = ((unsigned long long)(lf[nb - 1] + last) << 32) | N;
 }
 
+__global__ void __launch_bounds__(1024) k_pd_slices(const unsigned long long* __restrict__ off, uint64_t ntiles,
+                                                    uint32_t nb, uint64_t N, KtSlice* __restrict__ slices,
+                                                    KtEntry* __restrict__ tab, uint32_t* __restrict__ cidx,
+                                                    unsigned long long* __restrict__ total) {
+  pd_slices_one(off, ntiles, nb, N, 0, slices, tab, cidx, total);
+}
+
 // One tile in key order, written straight into the grouped layout (ids,
 // non-key tuples, values, slice indices); as k_pt_scatter, with the key
 // (16 bits) kept in LDS in place of the 64-bit key.
 template <class RestT, class ValT>
-__global__ void __launch_bounds__(PT_THREADS) k_pd_scatter(const uint32_t* __restrict__ h, uint32_t d, uint64_t N,
-                                                           KeyCodec codec, KeyCodec rest, const ValT* __restrict__ val,
-                                                           uint32_t nb, uint64_t ntiles,
-                                                           const uint32_t* __restrict__ hist,
-                                                           const unsigned long long* __restrict__ off,
-                                                           const uint32_t* __restrict__ cidx,
-                                                           uint32_t* __restrict__ perm, RestT* __restrict__ rest_out,
-                                                           ValT* __restrict__ gval, uint32_t* __restrict__ slice_of) {
+__device__ __forceinline__ void pd_scatter_tile(const uint32_t* __restrict__ h, uint32_t d, uint64_t N,
+                                                const KeyCodec& codec, const KeyCodec& rest,
+                                                const ValT* __restrict__ val, uint32_t nb, uint64_t ntiles,
+                                                const uint32_t* __restrict__ hist,
+                                                const unsigned long long* __restrict__ off, uint64_t obase,
+                                                const uint32_t* __restrict__ cidx, uint32_t* __restrict__ perm,
+                                                RestT* __restrict__ rest_out, ValT* __restrict__ gval,
+                                                uint32_t* __restrict__ slice_of, uint64_t t) {
   extern __shared__ unsigned long long pt_smem[];
   RestT* lrk = reinterpret_cast<RestT*>(pt_smem);           // PT_TILE
   uint16_t* lb = reinterpret_cast<uint16_t*>(lrk + PT_TILE); // PT_TILE: key of every tile item
@@ -866,13 +893,11 @@ __global__ void __launch_bounds__(PT_THREADS) k_pd_scatter(const uint32_t* __res
   uint32_t* lcur = reinterpret_cast<uint32_t*>(ls + PT_TILE); // nb
   uint32_t* gdst = lcur + nb;                                // nb
   __shared__ uint32_t s_wave[PT_THREADS / 64];
-  const uint64_t t = pt_tile_of(blockIdx.x, ntiles);
-  if (t >= ntiles) return;
   const uint64_t base = t * PT_TILE;
   const uint32_t n = (uint32_t)min<uint64_t>(PT_TILE, N - base);
   for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) {
     lcur[b] = hist[(uint64_t)b * ntiles + t];
-    gdst[b] = (uint32_t)off[(uint64_t)b * ntiles + t];
+    gdst[b] = (uint32_t)(off[(uint64_t)b * ntiles + t] - obase);
   }
   __syncthreads();
   pt_block_scan(lcur, nb, s_wave);
@@ -896,6 +921,21 @@ __global__ void __launch_bounds__(PT_THREADS) k_pd_scatter(const uint32_t* __res
     gval[p] = val[base + i];
     slice_of[p] = cidx[b];
   }
+}
+
+template <class RestT, class ValT>
+__global__ void __launch_bounds__(PT_THREADS) k_pd_scatter(const uint32_t* __restrict__ h, uint32_t d, uint64_t N,
+                                                           KeyCodec codec, KeyCodec rest, const ValT* __restrict__ val,
+                                                           uint32_t nb, uint64_t ntiles,
+                                                           const uint32_t* __restrict__ hist,
+                                                           const unsigned long long* __restrict__ off,
+                                                           const uint32_t* __restrict__ cidx,
+                                                           uint32_t* __restrict__ perm, RestT* __restrict__ rest_out,
+                                                           ValT* __restrict__ gval, uint32_t* __restrict__ slice_of) {
+  const uint64_t t = pt_tile_of(blockIdx.x, ntiles);
+  if (t >= ntiles) return;
+  pd_scatter_tile<RestT, ValT>(h, d, N, codec, rest, val, nb, ntiles, hist, off, 0, cidx, perm, rest_out, gval,
+                               slice_of, t);
 }
 
 // Table state of one bucket: LDS arrays, or the bucket's region of a global
@@ -972,9 +1012,12 @@ __device__ __forceinline__ uint32_t pt_cas32(uint32_t* p, uint32_t cmp, uint32_t
 // Group the n nonzeros of one bucket (positions [bs, bs + n) of the
 // partitioned arrays) with a table of cap >= n slots.
 // Returns false (before writing anything outside LDS) when the LDS table
-// fills up: the bucket is then grouped again in its global region.
+// fills up: the bucket is then grouped again in its global region.  The
+// grouped layout and the slice offsets are positions less obase (the
+// table's first position when several tables share the partitioned arrays).
 template <bool GLOBAL, class RestT, class ValT>
 __device__ bool pt_group_bucket(unsigned long long* tk, uint32_t* tc, uint32_t cap, uint64_t bs, uint32_t n,
+                                uint64_t obase,
                                 const unsigned long long* __restrict__ pkey, const uint32_t* __restrict__ pid,
                                 const KtPay<RestT, ValT>* __restrict__ ppay, KtSlice* __restrict__ tmps,
                                 uint32_t* __restrict__ nkeys_out, uint32_t* __restrict__ perm,
@@ -1051,7 +1094,7 @@ __device__ bool pt_group_bucket(unsigned long long* tk, uint32_t* tc, uint32_t c
       if (v[j]) {
         KtSlice sl;
         sl.key = pt_ld64(&tk[s]);
-        sl.off = (uint32_t)(bs + (uint32_t)pre);
+        sl.off = (uint32_t)(bs - obase + (uint32_t)pre);
         sl.len = (uint32_t)v[j];
         const uint64_t ti = bs + (pre >> 32);
         tmps[ti] = sl;
@@ -1123,7 +1166,7 @@ __device__ bool pt_group_bucket(unsigned long long* tk, uint32_t* tc, uint32_t c
     for (int u = 0; u < U; ++u) {
       if (i0 + u * blockDim.x >= n) continue;
       const uint32_t s = pt_slot<GLOBAL>(tk, cap, k[u]);  // present: a read-only probe
-      const uint64_t p = bs + pt_add32(&tc[s], 1u);
+      const uint64_t p = bs - obase + pt_add32(&tc[s], 1u);
       perm[p] = id[u];
       rest_out[p] = pl[u].k;
       gval[p] = pl[u].v;
@@ -1132,23 +1175,28 @@ __device__ bool pt_group_bucket(unsigned long long* tk, uint32_t* tc, uint32_t c
   return true;
 }
 
+// Group bucket b of a table whose partitioned nonzeros start at obase and
+// end at oend (positions in the shared partitioned arrays).
 template <class RestT, class ValT>
-__global__ void __launch_bounds__(PT_GROUP_THREADS) k_pt_group(
-    const unsigned long long* __restrict__ off, uint64_t ntiles, uint32_t nb, uint64_t N,
-    const unsigned long long* __restrict__ pkey, const uint32_t* __restrict__ pid,
-    const KtPay<RestT, ValT>* __restrict__ ppay, unsigned long long* __restrict__ gkey, uint32_t* __restrict__ gcnt,
-    KtSlice* __restrict__ tmps, uint32_t* __restrict__ bkeys, uint32_t* __restrict__ perm,
-    RestT* __restrict__ rest_out, ValT* __restrict__ gval, KtEntry* __restrict__ tab, uint64_t cap, int region,
-    uint32_t* __restrict__ ovf_cnt, uint32_t* __restrict__ ovf_list) {
+__device__ __forceinline__ void pt_group_one(uint32_t b, const unsigned long long* __restrict__ off, uint64_t ntiles,
+                                             uint32_t nb, uint64_t obase, uint64_t oend,
+                                             const unsigned long long* __restrict__ pkey,
+                                             const uint32_t* __restrict__ pid,
+                                             const KtPay<RestT, ValT>* __restrict__ ppay,
+                                             unsigned long long* __restrict__ gkey, uint32_t* __restrict__ gcnt,
+                                             KtSlice* __restrict__ tmps, uint32_t* __restrict__ nkeys_out,
+                                             uint32_t* __restrict__ perm, RestT* __restrict__ rest_out,
+                                             ValT* __restrict__ gval, KtEntry* __restrict__ tab, uint64_t cap,
+                                             int region, uint32_t* __restrict__ ovf_cnt,
+                                             uint32_t* __restrict__ ovf_list) {
   __shared__ unsigned long long lk[PT_TAB];
   __shared__ uint32_t lc[PT_TAB];
   __shared__ uint32_t lmap[PT_TAB];
   __shared__ unsigned long long s_wave[PT_GROUP_THREADS / 64];
   __shared__ unsigned long long s_carry;
   __shared__ uint32_t s_ndist;
-  const uint32_t b = blockIdx.x;
   const uint64_t bs = off[(uint64_t)b * ntiles];
-  const uint64_t be = b + 1 < nb ? off[(uint64_t)(b + 1) * ntiles] : N;
+  const uint64_t be = b + 1 < nb ? off[(uint64_t)(b + 1) * ntiles] : oend;
   const uint32_t n = (uint32_t)(be - bs);
   PtIndex X;
   X.tab = tab;
@@ -1160,23 +1208,38 @@ __global__ void __launch_bounds__(PT_GROUP_THREADS) k_pt_group(
   X.ovf_cnt = ovf_cnt;
   X.ovf_list = ovf_list;
   // the LDS table while the bucket's distinct keys fit, else the region
-  if (!pt_group_bucket<false>(lk, lc, PT_TAB, bs, n, pkey, pid, ppay, tmps, &bkeys[b], perm, rest_out, gval, X,
-                              &s_carry, s_wave, &s_ndist)) {
+  if (!pt_group_bucket<false>(lk, lc, PT_TAB, bs, n, obase, pkey, pid, ppay, tmps, nkeys_out, perm, rest_out, gval,
+                              X, &s_carry, s_wave, &s_ndist)) {
     __syncthreads();
-    pt_group_bucket<true>(gkey + bs, gcnt + bs, n, bs, n, pkey, pid, ppay, tmps, &bkeys[b], perm, rest_out, gval, X,
-                          &s_carry, s_wave, &s_ndist);
+    pt_group_bucket<true>(gkey + bs, gcnt + bs, n, bs, n, obase, pkey, pid, ppay, tmps, nkeys_out, perm, rest_out,
+                          gval, X, &s_carry, s_wave, &s_ndist);
   }
 }
 
-// Slices to their compact positions; without region mode, the probe index
-// as well (direct tables: entry[key]; hashed: one insert per distinct key).
-__global__ void k_pt_slices(const unsigned long long* __restrict__ off, uint64_t ntiles, uint32_t nb,
-                            const uint32_t* __restrict__ bkeys, const unsigned long long* __restrict__ sbase,
-                            const KtSlice* __restrict__ tmps, KtSlice* __restrict__ slices, KtEntry* __restrict__ tab,
-                            uint64_t cap, int index, int direct, unsigned long long* __restrict__ total, uint64_t N) {
-  const uint32_t b = blockIdx.x;
-  if (b == 0 && threadIdx.x == 0) *total = (sbase[nb] << 32) | N;
-  const uint64_t bs = off[(uint64_t)b * ntiles], base = sbase[b];
+template <class RestT, class ValT>
+__global__ void __launch_bounds__(PT_GROUP_THREADS) k_pt_group(
+    const unsigned long long* __restrict__ off, uint64_t ntiles, uint32_t nb, uint64_t N,
+    const unsigned long long* __restrict__ pkey, const uint32_t* __restrict__ pid,
+    const KtPay<RestT, ValT>* __restrict__ ppay, unsigned long long* __restrict__ gkey, uint32_t* __restrict__ gcnt,
+    KtSlice* __restrict__ tmps, uint32_t* __restrict__ bkeys, uint32_t* __restrict__ perm,
+    RestT* __restrict__ rest_out, ValT* __restrict__ gval, KtEntry* __restrict__ tab, uint64_t cap, int region,
+    uint32_t* __restrict__ ovf_cnt, uint32_t* __restrict__ ovf_list) {
+  pt_group_one<RestT, ValT>(blockIdx.x, off, ntiles, nb, 0, N, pkey, pid, ppay, gkey, gcnt, tmps, &bkeys[blockIdx.x],
+                            perm, rest_out, gval, tab, cap, region, ovf_cnt, ovf_list);
+}
+
+// Slices of bucket b to their compact positions (sbase: the scanned key
+// counts of the table's buckets, from its first); without region mode, the
+// probe index as well (direct tables: entry[key]; hashed: one insert per
+// distinct key).
+__device__ __forceinline__ void pt_slices_one(uint32_t b, const unsigned long long* __restrict__ off,
+                                              uint64_t ntiles, uint32_t nb, const uint32_t* __restrict__ bkeys,
+                                              const unsigned long long* __restrict__ sbase,
+                                              const KtSlice* __restrict__ tmps, KtSlice* __restrict__ slices,
+                                              KtEntry* __restrict__ tab, uint64_t cap, int index, int direct,
+                                              unsigned long long* __restrict__ total, uint64_t N) {
+  if (b == 0 && threadIdx.x == 0) *total = ((sbase[nb] - sbase[0]) << 32) | N;
+  const uint64_t bs = off[(uint64_t)b * ntiles], base = sbase[b] - sbase[0];
   const uint32_t nk = bkeys[b];
   for (uint32_t j = threadIdx.x; j < nk; j += blockDim.x) {
     const KtSlice sl = tmps[bs + j];
@@ -1199,6 +1262,13 @@ __global__ void k_pt_slices(const unsigned long long* __restrict__ off, uint64_t
   }
 }
 
+__global__ void k_pt_slices(const unsigned long long* __restrict__ off, uint64_t ntiles, uint32_t nb,
+                            const uint32_t* __restrict__ bkeys, const unsigned long long* __restrict__ sbase,
+                            const KtSlice* __restrict__ tmps, KtSlice* __restrict__ slices, KtEntry* __restrict__ tab,
+                            uint64_t cap, int index, int direct, unsigned long long* __restrict__ total, uint64_t N) {
+  pt_slices_one(blockIdx.x, off, ntiles, nb, bkeys, sbase, tmps, slices, tab, cap, index, direct, total, N);
+}
+
 // Keys whose probe ran past their region: inserted once every region is
 // written, onward from their home line (the table keeps free slots: cap >
 // distinct keys).
@@ -1215,6 +1285,162 @@ __global__ void k_pt_overflow(const uint32_t* __restrict__ cnt, const uint32_t* 
     }
     tab[i].off = sl.off;
     tab[i].len = sl.len;
+  }
+}
+
+// ---------------------------------------------------------- many tables
+//
+// A set of FP64 tables built together (a batch plan): the same counting
+// and partitioned steps, with one launch per step for all tables.  Every
+// table's count matrix, tiles, positions and buckets are laid end to end;
+// a workgroup finds its table by a binary search over the bases.  The
+// descriptors are read through global-address-space pointers so the
+// kernels' loads and stores stay global (not flat) instructions.
+
+#define PTC_MT_FIELDS(PTR)                                                                      \
+  PTR(const uint32_t) hedges;                                                                   \
+  PTR(const double) vals;                                                                       \
+  PTR(uint32_t) perm;                                                                           \
+  PTR(uint32_t) rest32;                                                                         \
+  PTR(unsigned long long) rest64;                                                               \
+  PTR(double) gval;                                                                             \
+  PTR(KtEntry) entries;                                                                         \
+  PTR(KtSlice) slices;                                                                          \
+  PTR(uint32_t) slice_of;                                                                       \
+  PTR(unsigned long long) total;                                                                \
+  PTR(uint32_t) cidx;                                                                           \
+  KeyCodec codec;                                                                               \
+  KeyCodec rest;                                                                                \
+  uint64_t N, cap, ntiles, tile_base, cnt_base, item_base, bucket_base;                         \
+  uint32_t d, nb;                                                                               \
+  int bbits, counting, direct, region, wide;
+
+#define PTC_MT_PTR(T) T*
+struct MtTable {
+  PTC_MT_FIELDS(PTC_MT_PTR)
+};
+#ifdef __HIP_DEVICE_COMPILE__
+#define PTC_MT_GPTR(T) __attribute__((address_space(1))) T*
+#else
+#define PTC_MT_GPTR(T) T*
+#endif
+struct MtTableG {
+  PTC_MT_FIELDS(PTC_MT_GPTR)
+};
+static_assert(sizeof(MtTableG) == sizeof(MtTable), "device view of MtTable");
+
+// the last table whose base (tile, bucket or position) is <= g
+enum { MT_TILE = 0, MT_BUCKET = 1, MT_ITEM = 2 };
+template <int WHICH>
+__device__ __forceinline__ int mt_find(const MtTableG* __restrict__ T, int nt, uint64_t g) {
+  int lo = 0, hi = nt - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    const uint64_t b = WHICH == MT_TILE ? T[mid].tile_base : WHICH == MT_BUCKET ? T[mid].bucket_base : T[mid].item_base;
+    if (b <= g) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ void __launch_bounds__(PT_THREADS) k_mt_hist(const MtTableG* __restrict__ tabs, int nt,
+                                                        uint32_t* __restrict__ hist) {
+  const MtTableG& T = tabs[mt_find<MT_TILE>(tabs, nt, blockIdx.x)];
+  const KeyCodec codec = T.codec;
+  pt_hist_tile(T.hedges, T.d, T.N, codec, T.bbits, T.nb, T.ntiles, hist + T.cnt_base, blockIdx.x - T.tile_base);
+}
+
+// Direct tables grouped by partitioning write only their present keys:
+// the rest of their index is cleared first.
+__global__ void k_mt_clear(const MtTableG* __restrict__ tabs, int nt) {
+  const MtTableG& T = tabs[blockIdx.y];
+  if (T.counting || !T.direct) return;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < T.cap; i += (uint64_t)gridDim.x * blockDim.x) {
+    KtEntry en;
+    en.key = PTC_EMPTY_KEY;
+    en.off = 0;
+    en.len = 0;
+    T.entries[i] = en;
+  }
+}
+
+__global__ void __launch_bounds__(1024) k_mt_pd_slices(const MtTableG* __restrict__ tabs, int nt,
+                                                       const unsigned long long* __restrict__ off) {
+  const MtTableG& T = tabs[blockIdx.x];
+  if (!T.counting) return;
+  pd_slices_one(off + T.cnt_base, T.ntiles, T.nb, T.N, T.item_base, T.slices, T.entries, T.cidx, T.total);
+}
+
+__global__ void __launch_bounds__(PT_THREADS) k_mt_scatter(const MtTableG* __restrict__ tabs, int nt,
+                                                           uint64_t gtiles, const uint32_t* __restrict__ hist,
+                                                           const unsigned long long* __restrict__ off,
+                                                           unsigned long long* __restrict__ pkey,
+                                                           uint32_t* __restrict__ pid, void* __restrict__ ppay) {
+  const uint64_t g = pt_tile_of(blockIdx.x, gtiles);
+  if (g >= gtiles) return;
+  const MtTableG& T = tabs[mt_find<MT_TILE>(tabs, nt, g)];
+  const uint64_t lt = g - T.tile_base;
+  const KeyCodec codec = T.codec, rest = T.rest;
+  const uint32_t* hs = hist + T.cnt_base;
+  const unsigned long long* os = off + T.cnt_base;
+  if (T.counting) {
+    if (T.wide)
+      pd_scatter_tile<unsigned long long, double>(T.hedges, T.d, T.N, codec, rest, T.vals, T.nb, T.ntiles, hs, os,
+                                                  T.item_base, T.cidx, T.perm, T.rest64, T.gval, T.slice_of, lt);
+    else
+      pd_scatter_tile<uint32_t, double>(T.hedges, T.d, T.N, codec, rest, T.vals, T.nb, T.ntiles, hs, os, T.item_base,
+                                        T.cidx, T.perm, T.rest32, T.gval, T.slice_of, lt);
+  } else {
+    if (T.wide)
+      pt_scatter_tile<unsigned long long, double>(T.hedges, T.d, T.N, codec, rest, T.vals, T.bbits, T.nb, T.ntiles, hs,
+                                                  os, pkey, pid, (KtPay<unsigned long long, double>*)ppay, lt);
+    else
+      pt_scatter_tile<uint32_t, double>(T.hedges, T.d, T.N, codec, rest, T.vals, T.bbits, T.nb, T.ntiles, hs, os, pkey,
+                                        pid, (KtPay<uint32_t, double>*)ppay, lt);
+  }
+}
+
+__global__ void __launch_bounds__(PT_GROUP_THREADS) k_mt_group(
+    const MtTableG* __restrict__ tabs, int nt, const unsigned long long* __restrict__ off,
+    const unsigned long long* __restrict__ pkey, const uint32_t* __restrict__ pid, const void* __restrict__ ppay,
+    unsigned long long* __restrict__ gkey, uint32_t* __restrict__ gcnt, KtSlice* __restrict__ tmps,
+    uint32_t* __restrict__ bkeys, uint32_t* __restrict__ ovf_cnt, uint32_t* __restrict__ ovf_list) {
+  const MtTableG& T = tabs[mt_find<MT_BUCKET>(tabs, nt, blockIdx.x)];
+  const uint32_t b = (uint32_t)(blockIdx.x - T.bucket_base);
+  const unsigned long long* os = off + T.cnt_base;
+  if (T.wide)
+    pt_group_one<unsigned long long, double>(b, os, T.ntiles, T.nb, T.item_base, T.item_base + T.N, pkey, pid,
+                                             (const KtPay<unsigned long long, double>*)ppay, gkey, gcnt, tmps,
+                                             &bkeys[blockIdx.x], T.perm, T.rest64, T.gval, T.entries, T.cap, T.region,
+                                             ovf_cnt, ovf_list);
+  else
+    pt_group_one<uint32_t, double>(b, os, T.ntiles, T.nb, T.item_base, T.item_base + T.N, pkey, pid,
+                                   (const KtPay<uint32_t, double>*)ppay, gkey, gcnt, tmps, &bkeys[blockIdx.x], T.perm,
+                                   T.rest32, T.gval, T.entries, T.cap, T.region, ovf_cnt, ovf_list);
+}
+
+__global__ void k_mt_slices(const MtTableG* __restrict__ tabs, int nt, const unsigned long long* __restrict__ off,
+                            const uint32_t* __restrict__ bkeys, const unsigned long long* __restrict__ sbase,
+                            const KtSlice* __restrict__ tmps) {
+  const MtTableG& T = tabs[mt_find<MT_BUCKET>(tabs, nt, blockIdx.x)];
+  pt_slices_one((uint32_t)(blockIdx.x - T.bucket_base), off + T.cnt_base, T.ntiles, T.nb, bkeys + T.bucket_base,
+                sbase + T.bucket_base, tmps, T.slices, T.entries, T.cap, T.region ? 0 : 1, T.direct, T.total, T.N);
+}
+
+__global__ void k_mt_overflow(const MtTableG* __restrict__ tabs, int nt, const uint32_t* __restrict__ cnt,
+                              const uint32_t* __restrict__ list, const KtSlice* __restrict__ tmps) {
+  const uint32_t n = *cnt;
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    const MtTableG& T = tabs[mt_find<MT_ITEM>(tabs, nt, list[k])];
+    const KtSlice sl = tmps[list[k]];
+    const uint64_t cap = T.cap;
+    uint64_t i = kt_home(mix64(sl.key), cap);
+    while (true) {
+      const unsigned long long old = atomicCAS(&T.entries[i].key, PTC_EMPTY_KEY, sl.key);
+      if (old == PTC_EMPTY_KEY) break;
+      i = (i + 1) & (cap - 1);
+    }
+    T.entries[i].off = sl.off;
+    T.entries[i].len = sl.len;
   }
 }
 
@@ -1663,6 +1889,158 @@ extern "C" int ptc_hashtable_build_ex(ptc_ctx* ctx, const ptc_dtensor* t, int gr
 }
 
 namespace ptc {
+// Key tables of n tensors at once (batch plans).  FP64 tensors with 1 to
+// PT_MAX_N nonzeros are grouped together -- one counting pass for direct key
+// spaces up to PT_MAX_BUCKETS keys, partitioned hashing for the rest -- with
+// one launch per step; any other tensor is built on its own.  The tables
+// hold the same slices as ptc_hashtable_build's.
+int hashtable_build_many(ptc_ctx* ctx, int n, const ptc_dtensor* const* ts, ptc_hashtable** out) {
+  for (int i = 0; i < n; ++i) out[i] = nullptr;
+  PTC_HIP(ctx, hipSetDevice(ctx->device));
+  std::vector<MtTable> T;
+  uint64_t tiles = 0, cnts = 0, items = 0, buckets = 0, keyspace = 0;
+  size_t lds = 0;
+  int rc = PTC_OK;
+  auto fail = [&](int code) {
+    for (int i = 0; i < n; ++i) {
+      ptc_hashtable_free(out[i]);
+      out[i] = nullptr;
+    }
+    return code;
+  };
+  for (int i = 0; i < n; ++i) {
+    const ptc_dtensor* t = ts[i];
+    if (!t->vals || t->N == 0 || t->N > PT_MAX_N) {
+      if ((rc = ptc_hashtable_build(ctx, t, &out[i]))) return fail(rc);
+      continue;
+    }
+    ptc_hashtable* ht = new ptc_hashtable();
+    out[i] = ht;
+    ht->ctx = ctx;
+    ht->t = t;
+    if ((rc = make_codec(ctx, t, t->dim_tc, t->nk, &ht->codec)) ||
+        (rc = make_codec(ctx, t, t->dim_tc + t->nk, t->d - t->nk, &ht->rest)))
+      return fail(rc);
+    const uint64_t N = t->N;
+    ht->direct = ht->codec.space <= std::max<uint64_t>(2 * N, 1u << 16) && ht->codec.space < (1ull << 31);
+    const uint64_t maxkeys = std::min<uint64_t>(N, ht->codec.space);
+    uint64_t cap = 64;
+    while (cap < maxkeys + maxkeys / 4) cap <<= 1;
+    if (ht->direct) cap = std::max<uint64_t>(ht->codec.space, 1);
+    ht->cap = cap;
+    const bool wide = ht->rest.space > 0xFFFFFFFFull;
+    const bool counting = ht->direct && ht->codec.space <= PT_MAX_BUCKETS;
+    ht->entries = dalloc_n<KtEntry>(ctx, cap);
+    ht->perm = dalloc_n<uint32_t>(ctx, N);
+    ht->gval = dalloc_n<double>(ctx, N);
+    if (wide) ht->rest_key64 = dalloc_n<unsigned long long>(ctx, N);
+    else ht->rest_key32 = dalloc_n<uint32_t>(ctx, N);
+    ht->d_total = dalloc_n<unsigned long long>(ctx, 1);
+    ht->slices_cap = std::max<uint64_t>(1, std::min<uint64_t>(N, cap));
+    ht->slices = dalloc_n<KtSlice>(ctx, ht->slices_cap);
+    if (counting) ht->slice_of = dalloc_n<uint32_t>(ctx, N);
+    if (!ht->entries || !ht->perm || !ht->gval || !(ht->rest_key32 || ht->rest_key64) || !ht->d_total ||
+        !ht->slices || (counting && !ht->slice_of))
+      return fail(PTC_ERR_NOMEM);
+    MtTable m{};
+    m.hedges = t->hedges;
+    m.vals = t->vals;
+    m.perm = ht->perm;
+    m.rest32 = ht->rest_key32;
+    m.rest64 = ht->rest_key64;
+    m.gval = ht->gval;
+    m.entries = ht->entries;
+    m.slices = ht->slices;
+    m.slice_of = ht->slice_of;
+    m.total = ht->d_total;
+    m.codec = ht->codec;
+    m.rest = ht->rest;
+    m.N = N;
+    m.cap = cap;
+    m.d = t->d;
+    m.counting = counting ? 1 : 0;
+    m.direct = ht->direct ? 1 : 0;
+    m.wide = wide ? 1 : 0;
+    if (counting) {
+      m.nb = (uint32_t)ht->codec.space;
+      m.bbits = -1;
+    } else {
+      m.nb = 1;
+      m.bbits = 0;
+      while (m.nb < PT_MAX_BUCKETS && (uint64_t)m.nb * PT_BUCKET_ITEMS < N) {
+        m.nb <<= 1;
+        ++m.bbits;
+      }
+    }
+    m.region = !counting && !ht->direct && cap / 8 >= m.nb ? 1 : 0;
+    m.ntiles = (N + PT_TILE - 1) / PT_TILE;
+    m.tile_base = tiles;
+    m.cnt_base = cnts;
+    m.item_base = items;
+    m.bucket_base = buckets;
+    m.cidx = reinterpret_cast<uint32_t*>(keyspace);  // an offset until the shared array exists
+    tiles += m.ntiles;
+    cnts += (uint64_t)m.nb * m.ntiles;
+    items += N;
+    if (counting) keyspace += m.nb;
+    else buckets += m.nb;
+    const size_t rs = wide ? 8 : 4;
+    lds = std::max(lds, counting ? (size_t)PT_TILE * (rs + 4) + (size_t)m.nb * 8
+                                 : (size_t)PT_TILE * (8 + rs + 2) + (size_t)m.nb * 8);
+    T.push_back(m);
+  }
+  if (T.empty()) return PTC_OK;
+  if (items > 0xFFFFFFFFull)
+    return fail(set_err(ctx, PTC_ERR_RANGE, "%llu nonzeros in one table set", (unsigned long long)items));
+  hipStream_t st = ctx->stream;
+  Workspace ws(ctx);
+  uint32_t* hist = ws.get<uint32_t>(cnts);
+  unsigned long long* off = ws.get<unsigned long long>(cnts + 1);
+  unsigned long long* pkey = ws.get<unsigned long long>(items);
+  uint32_t* pid = ws.get<uint32_t>(items);
+  void* ppay = ws.get<unsigned char>(items * 16);
+  unsigned long long* gkey = ws.get<unsigned long long>(items);
+  uint32_t* gcnt = ws.get<uint32_t>(items);
+  KtSlice* tmps = ws.get<KtSlice>(items);
+  uint32_t* bkeys = ws.get<uint32_t>(buckets + 1);
+  unsigned long long* sbase = ws.get<unsigned long long>(buckets + 1);
+  uint32_t* ovf = ws.get<uint32_t>(items + 1);
+  uint32_t* cidx = ws.get<uint32_t>(keyspace + 1);
+  MtTable* dT = ws.get<MtTable>(T.size());
+  if (!hist || !off || !pkey || !pid || !ppay || !gkey || !gcnt || !tmps || !bkeys || !sbase || !ovf || !cidx || !dT)
+    return fail(PTC_ERR_NOMEM);
+  for (MtTable& m : T) m.cidx = cidx + reinterpret_cast<uintptr_t>(m.cidx);
+  // a synchronous copy: the host vector is gone when the kernels run
+  PTC_HIP(ctx, hipMemcpy(dT, T.data(), sizeof(MtTable) * T.size(), hipMemcpyHostToDevice));
+  const int nt = (int)T.size();
+  const MtTableG* dG = reinterpret_cast<const MtTableG*>(dT);
+  if ((rc = fill_zero(ctx, ovf, 4))) return fail(rc);
+  hipLaunchKernelGGL(k_mt_hist, dim3((unsigned)tiles), dim3(PT_THREADS), 0, st, dG, nt, hist);
+  if ((rc = scan_u32_to_u64(ctx, hist, off, cnts, off + cnts))) return fail(rc);
+  hipLaunchKernelGGL(k_mt_clear, dim3(16, (unsigned)nt), dim3(256), 0, st, dG, nt);
+  hipLaunchKernelGGL(k_mt_pd_slices, dim3((unsigned)nt), dim3(1024), 0, st, dG, nt, (const unsigned long long*)off);
+  static size_t lds_set = 0;
+  if (lds > 65536 && lds > lds_set) {
+    PTC_HIP(ctx, hipFuncSetAttribute((const void*)k_mt_scatter, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    lds_set = lds;
+  }
+  hipLaunchKernelGGL(k_mt_scatter, dim3((unsigned)(8 * ((tiles + 7) / 8))), dim3(PT_THREADS), lds, st, dG, nt, tiles,
+                     (const uint32_t*)hist, (const unsigned long long*)off, pkey, pid, ppay);
+  if (buckets) {
+    hipLaunchKernelGGL(k_mt_group, dim3((unsigned)buckets), dim3(PT_GROUP_THREADS), 0, st, dG, nt,
+                       (const unsigned long long*)off, (const unsigned long long*)pkey, (const uint32_t*)pid,
+                       (const void*)ppay, gkey, gcnt, tmps, bkeys, ovf, ovf + 1);
+    if ((rc = scan_u32_to_u64(ctx, bkeys, sbase, buckets, sbase + buckets))) return fail(rc);
+    hipLaunchKernelGGL(k_mt_slices, dim3((unsigned)buckets), dim3(256), 0, st, dG, nt, (const unsigned long long*)off,
+                       (const uint32_t*)bkeys, (const unsigned long long*)sbase, (const KtSlice*)tmps);
+    hipLaunchKernelGGL(k_mt_overflow, dim3(64), dim3(256), 0, st, dG, nt, (const uint32_t*)ovf,
+                       (const uint32_t*)(ovf + 1), (const KtSlice*)tmps);
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(hip_check(ctx, e, "batched key tables"));
+  return PTC_OK;
+}
+
 // Slice index of every grouped position, built on first use (hashed tables
 // skip it at build time: only a table that serves as A needs it).
 int hashtable_slice_of(ptc_ctx* ctx, const ptc_hashtable* hc, const uint32_t** out) {
