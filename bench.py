@@ -478,6 +478,37 @@ def chain_bench(P, args):
     ctx.close()
 
 
+def stream_copy_gbps(mb: int = 1024, reps: int = 10) -> float:
+    """Achievable HBM bandwidth on this GPU: a device-to-device copy of `mb`
+    MiB (read + write), timed with HIP events; the best of `reps`."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    n = mb << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    e0, e1 = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(a), C.c_size_t(n)) or hip.hipMalloc(C.byref(b), C.c_size_t(n)):
+        return 0.0
+    hip.hipEventCreate(C.byref(e0))
+    hip.hipEventCreate(C.byref(e1))
+    hip.hipMemset(a, 1, C.c_size_t(n))
+    best = 0.0
+    for _ in range(reps):
+        hip.hipEventRecord(e0, None)
+        hip.hipMemcpyAsync(b, a, C.c_size_t(n), 3, None)
+        hip.hipEventRecord(e1, None)
+        hip.hipEventSynchronize(e1)
+        ms = C.c_float(0)
+        hip.hipEventElapsedTime(C.byref(ms), e0, e1)
+        if ms.value > 0:
+            best = max(best, 2 * n / (ms.value * 1e-3) / 1e9)
+    hip.hipEventDestroy(e0)
+    hip.hipEventDestroy(e1)
+    hip.hipFree(a)
+    hip.hipFree(b)
+    return best
+
+
 def load_traffic():
     f = ROOT / "profiles" / "traffic_numeric_pass_nips.json"
     if f.exists():
@@ -565,6 +596,10 @@ def main():
         "algorithmic_bytes_per_launch": int(kb),
         "kernel_ms": round(kms, 4),
     }
+    copy_gbps = stream_copy_gbps()
+    if copy_gbps > 0:  # what a plain device copy reaches on this GPU
+        roof["achievable_copy"] = round(copy_gbps, 1)
+        roof["frac_of_achievable"] = round(achieved / copy_gbps, 4)
     out = {
         "metric": "useful multiply-adds/s, nips-shape 4-mode self-contraction (learned 1-pass vs exact 2-pass)",
         "value": value,
