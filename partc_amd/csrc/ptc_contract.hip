@@ -35,6 +35,8 @@
 //                capacity-sized staging area to the prefix-scanned output.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -459,7 +461,9 @@ __device__ __forceinline__ void pass_item(const PP& P, uint32_t widx) {
   __shared__ uint32_t s_ns;
 
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  const ColT* __restrict__ bcol = (const ColT*)P.bcol;
+  // dense rows stream 16-bit column keys (their column space is below 2^16)
+  using BCol = typename std::conditional<DENSE, uint16_t, ColT>::type;
+  const BCol* __restrict__ bcol = (const BCol*)P.bcol;
   uint4 it = make_uint4(0, 0, 0, 0);
   if (MERGE) {
     it.x = P.merge_rows ? P.merge_rows[widx] : widx;

@@ -136,6 +136,7 @@ struct ptc_hashtable {
   KeyCodec rest;
   uint32_t* rest_key32 = nullptr;        // when rest.space <= 2^32
   unsigned long long* rest_key64 = nullptr;
+  uint16_t* rest_key16 = nullptr;        // 16-bit copy (rest.space <= 2^16), made on first use
   double* gval = nullptr;      // device, N: values in grouped order
   float* gval32 = nullptr;     // device, N: values in grouped order, FP32 tensors (gval is then NULL)
   bool shares_vals = false;    // gval/gval32 are the tensor's own values (grouped order = tensor order)
@@ -150,6 +151,9 @@ namespace ptc {
 
 int set_err(ptc_ctx* ctx, int code, const char* fmt, ...);
 int hashtable_slice_of(ptc_ctx* ctx, const ptc_hashtable* h, const uint32_t** out);
+// The non-key keys of a table as 16 bits (rest.space <= 2^16), made on
+// first use: what dense row passes stream.
+int hashtable_col16(ptc_ctx* ctx, const ptc_hashtable* h, const uint16_t** out);
 // Key tables of n tensors built together (one launch per step for all of
 // them); the same slices as ptc_hashtable_build gives each.
 int hashtable_build_many(ptc_ctx* ctx, int n, const ptc_dtensor* const* ts, ptc_hashtable** out);

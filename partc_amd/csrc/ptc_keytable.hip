@@ -2041,6 +2041,29 @@ int hashtable_build_many(ptc_ctx* ctx, int n, const ptc_dtensor* const* ts, ptc_
   return PTC_OK;
 }
 
+__global__ void k_narrow16(uint64_t N, const uint32_t* __restrict__ in, uint16_t* __restrict__ out) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (uint64_t)gridDim.x * blockDim.x)
+    out[i] = (uint16_t)in[i];
+}
+
+int hashtable_col16(ptc_ctx* ctx, const ptc_hashtable* hc, const uint16_t** out) {
+  ptc_hashtable* h = const_cast<ptc_hashtable*>(hc);
+  if (!h->rest_key16) {
+    if (!h->rest_key32 || h->rest.space > 65536)
+      return set_err(ctx, PTC_ERR_INTERNAL, "16-bit keys of a %llu-key space", (unsigned long long)h->rest.space);
+    const uint64_t N = h->t->N;
+    h->rest_key16 = dalloc_n<uint16_t>(ctx, std::max<uint64_t>(N, 1));
+    if (!h->rest_key16) return PTC_ERR_NOMEM;
+    if (N) {
+      hipLaunchKernelGGL(k_narrow16, dim3(grid_for(N, 256, 16384)), dim3(256), 0, ctx->stream, N,
+                         (const uint32_t*)h->rest_key32, h->rest_key16);
+      PTC_HIP(ctx, hipGetLastError());
+    }
+  }
+  *out = h->rest_key16;
+  return PTC_OK;
+}
+
 // Slice index of every grouped position, built on first use (hashed tables
 // skip it at build time: only a table that serves as A needs it).
 int hashtable_slice_of(ptc_ctx* ctx, const ptc_hashtable* hc, const uint32_t** out) {
@@ -2278,6 +2301,7 @@ extern "C" void ptc_hashtable_free(ptc_hashtable* h) {
     dfree(h->ctx, h->slice_of);
     dfree(h->ctx, h->rest_key32);
     dfree(h->ctx, h->rest_key64);
+    dfree(h->ctx, h->rest_key16);
     if (!h->shares_vals) {
       dfree(h->ctx, h->gval);
       dfree(h->ctx, h->gval32);

@@ -56,6 +56,7 @@ struct Run {
   const uint32_t* row_of_pos = nullptr;  // HT_A's slice index of every A position
   uint32_t threads = PB;       // threads per row-pass workgroup
   bool force_flags = false;    // touch flags whatever the values (set before prepare)
+  const uint16_t* bcol16 = nullptr;  // dense rows: B's column keys as 16 bits
   bool f32 = false;            // FP32 values (both tables carry gval32)
   int val = PTC_VAL_F64;       // ptc_value_type of the numeric pass
   explicit Run(ptc_ctx* c) : ctx(c), ws(c) {}

@@ -134,7 +134,7 @@ void base_params(Run& R, PassParams& P, const HashPlan* hp) {
   P.pos_off = R.pos_off;
   P.aval = R.HA->gval;
   P.aval32 = R.HA->gval32;
-  P.bcol = R.col64 ? (const void*)R.HB->rest_key64 : (const void*)R.HB->rest_key32;
+  P.bcol = R.dense ? (const void*)R.bcol16 : R.col64 ? (const void*)R.HB->rest_key64 : (const void*)R.HB->rest_key32;
   P.bval = R.HB->gval;
   P.bval32 = R.HB->gval32;
   P.f32 = R.f32 ? 1 : 0;
@@ -354,6 +354,7 @@ int prepare_launch(Run& R, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B,
                                          : (R.dense ? std::max<uint64_t>(DEFAULT_DENSE_CHUNK, 24 * R.ncols) : DEFAULT_CHUNK);
 
   int rc = PTC_OK;
+  if (R.dense && (rc = hashtable_col16(ctx, HT_B, &R.bcol16))) return rc;
   const uint32_t* row_of_pos = nullptr;  // HT_A's slice index of every A position
   if ((rc = hashtable_slice_of(ctx, HT_A, &row_of_pos))) return rc;
   R.row_of_pos = row_of_pos;
