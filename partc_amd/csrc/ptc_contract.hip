@@ -92,15 +92,16 @@ __global__ void __launch_bounds__(256) k_probe(ProbeParams P) {
         }
       } else if (ok) {
         // same probe order as the insertion: the home line of mix64(k), then
-        // onward; a whole 8-entry line is loaded at once
+        // onward; half a line (four entries, 64 bytes) is loaded at a time
+        // -- the key is nearly always in the first half
         uint64_t i = kt_home(mix64d(kb), P.cap);
         bool done = false;
         while (!done) {
-          KtEntry en[8];
+          KtEntry en[4];
 #pragma unroll
-          for (int s = 0; s < 8; ++s) en[s] = P.tab[(i + s) & (P.cap - 1)];
+          for (int s = 0; s < 4; ++s) en[s] = P.tab[(i + s) & (P.cap - 1)];
 #pragma unroll
-          for (int s = 0; s < 8; ++s) {
+          for (int s = 0; s < 4; ++s) {
             if (done) continue;
             if (en[s].key == kb) {
               len = en[s].len;
@@ -110,7 +111,7 @@ __global__ void __launch_bounds__(256) k_probe(ProbeParams P) {
               done = true;
             }
           }
-          i = (i + 8) & (P.cap - 1);
+          i = (i + 4) & (P.cap - 1);
         }
       }
       P.pos_len[p] = len;
