@@ -422,8 +422,17 @@ __device__ __forceinline__ void f32_finish(float s, float c, float* hi, float* l
   *lo = two_sum_err(s, c, h);
 }
 
+// Orders one wave's LDS accesses around an exchange between its lanes (no
+// hardware wait: a wave's LDS accesses already complete in order).
+__device__ __forceinline__ void wave_exchange_fence() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // LDS layout of one row pass (dynamic shared memory):
-//   stage:  s_start[STAGE] u32, s_bias[STAGE] i64, s_aval[STAGE] f64
+//   stage:  s_start[STAGE] u32, s_bias[STAGE] u32, s_aval[STAGE] f64,
+//           s_bnd[64] u8 per wave (boundary flags of a group of 64 flops)
 //   dense:  acc[ncols] f64 (+ flag[ncols] u8 when flags are needed)  (numeric)
 //           bits[ceil(ncols/32)] u32                                 (symbolic)
 //   hash:   keys[S] ColT, vals[S] f64             (S = lds_slots)
@@ -452,10 +461,12 @@ __device__ __forceinline__ void pass_item(const PP& P, uint32_t widx) {
   const int NW = (int)(NT >> 6);
   extern __shared__ __align__(16) unsigned char smem[];
   uint32_t* s_start = (uint32_t*)smem;
-  long long* s_bias = (long long*)(smem + STAGE * 4);
-  double* s_aval = (double*)(smem + STAGE * 12);
+  // B position of a staged A position's first flop, less the flop index,
+  // modulo 2^32: adding any flop index of the position gives its B position
+  uint32_t* s_bias = (uint32_t*)(smem + STAGE * 4);
+  double* s_aval = (double*)(smem + STAGE * 8);
   float* s_aval32 = (float*)s_aval;  // FP32: the first half of the same region
-  unsigned char* acc_base = smem + STAGE * 20;
+  unsigned char* acc_base = smem + STAGE * 16 + NT;
   __shared__ uint32_t s_w[PB_MAX / 64];
   __shared__ uint32_t s_misc[4];
   __shared__ uint32_t s_rowc[PTC_MAX_MODES];
@@ -629,7 +640,7 @@ __device__ __forceinline__ void pass_item(const PP& P, uint32_t widx) {
       const uint64_t q = q0 + (uint64_t)threadIdx.x * 2 + j;
       const unsigned long long a = P.apre[q];
       s_start[dst] = (uint32_t)((a > g0 ? a : g0) - g0);
-      s_bias[dst] = (long long)P.pos_off[q] + (long long)g0 - (long long)a;
+      s_bias[dst] = (uint32_t)((unsigned long long)P.pos_off[q] + g0 - a);
       if (PASS == PASS_NUMERIC) {
         if (F32IN) s_aval32[dst] = P.aval32[q];
         else s_aval[dst] = P.aval[q];
@@ -659,9 +670,10 @@ __device__ __forceinline__ void pass_item(const PP& P, uint32_t widx) {
       // U groups of 64 flops per step: positions first (LDS only), then all
       // global loads, then the accumulations -- 2U loads in flight per wave
       constexpr int U = PTC_PASS_UNROLL;
+      unsigned char* s_bnd = smem + STAGE * 16 + wid * 64;  // this wave's boundary flags
       for (uint32_t f0 = wb; f0 < we; f0 += 64 * U) {
         uint32_t ii[U];
-        long long bb[U];
+        uint32_t bb[U];
         bool act[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -671,23 +683,32 @@ __device__ __forceinline__ void pass_item(const PP& P, uint32_t widx) {
           bb[u] = 0;
           if (fu < we) {  // wave-uniform
             // every kept position has at least one flop, so the wave's 64
-            // flops lie in positions [qa, qa + 63]
+            // flops lie in positions [qa, qa + 63], and after this step
+            // s_start[qa] <= fu < s_start[qa + 1]
             if (qa + 1 < ns && s_start[qa + 1] <= fu) ++qa;
             const uint32_t f = min(fu + lane, we - 1);
             uint32_t i = qa;
-            const uint32_t hi0 = min(qa + 63, ns - 1);
-            if (hi0 > qa && s_start[qa + 1] <= min(fu + 63, we - 1)) {
-              uint32_t lo = qa, hi = hi0;
-              while (lo < hi) {
-                uint32_t mid = (lo + hi + 1) >> 1;
-                if (s_start[mid] <= f) lo = mid; else hi = mid - 1;
-              }
-              i = lo;
+            if (qa + 1 < ns && s_start[qa + 1] <= min(fu + 63, we - 1)) {
+              // the group crosses positions: lane k flags the flop where
+              // position qa + 1 + k starts (a byte per flop of the group),
+              // and a lane's position is qa + the flags at or before its flop
+              const uint32_t bi = qa + 1 + lane;
+              const uint32_t dk = bi < ns ? s_start[bi] - fu : 64u;
+              // the lanes of the wave exchange the flags through LDS: the
+              // wave's LDS accesses complete in order, and the wavefront-scope
+              // fences keep the compiler from moving a flag access across the
+              // exchange or answering a lane's load from its own store
+              wave_exchange_fence();
+              s_bnd[lane] = 0;
+              if (dk < 64) s_bnd[dk] = 1;
+              wave_exchange_fence();
+              const unsigned long long m = __ballot(s_bnd[lane] != 0);
+              i = qa + (uint32_t)__popcll(m & (~0ull >> (63 - lane)));
             }
             ii[u] = i;
-            bb[u] = s_bias[i] + (long long)f;
+            bb[u] = s_bias[i] + f;
             act[u] = fu + lane < we;
-            if ((unsigned long long)bb[u] >= P.nb) {  // cannot happen for consistent tables
+            if ((uint64_t)bb[u] >= P.nb) {  // cannot happen for consistent tables
               if (act[u]) atomicOr(P.overflow, 0x40000000u);
               act[u] = false;
             }
@@ -1041,7 +1062,7 @@ int launch_hash_caps(ptc_ctx* ctx, const HashCapParams& P) {
 
 size_t pass_lds_bytes(int pass, bool dense, bool col64, uint64_t ncols, uint32_t lds_slots, bool flags,
                       uint32_t threads) {
-  size_t b = (size_t)2 * threads * 20;  // staging: 2 positions per thread
+  size_t b = (size_t)2 * threads * 16 + threads;  // staging: 2 positions per thread, 64 flags per wave
   if (dense) {
     if (pass == PASS_SYMBOLIC) b += ((ncols + 31) / 32) * 4;
     else b += ncols * (flags ? 9 : 8);  // accumulator (+ flag byte)
