@@ -678,6 +678,10 @@ __global__ void k_kt_probe(const uint32_t* __restrict__ q, uint64_t n, KeyCodec 
 
 constexpr int PT_THREADS = 512;
 constexpr int PT_GROUP_THREADS = 1024;     // bucket workgroups: two per CU fill it
+// two 1024-thread workgroups per CU are 8 waves per SIMD, which leaves each
+// lane 64 VGPRs; without the bound the compiler takes 77 and one workgroup
+// per CU fits
+constexpr int PT_GROUP_WAVES_PER_SIMD = 8;
 constexpr uint32_t PT_TAB = 4096;          // LDS table slots per bucket workgroup
 constexpr uint32_t PT_TILE = 4096;         // nonzeros per partition tile (16-bit indices in LDS)
 constexpr uint32_t PT_MAX_BUCKETS = 4096;
@@ -690,6 +694,30 @@ constexpr uint64_t PT_MAX_N = (uint64_t)PT_MAX_BUCKETS * PT_BUCKET_ITEMS;
 // itself (small direct key spaces, one key per bucket).
 __device__ __forceinline__ uint32_t pt_bucket(unsigned long long key, int bbits) {
   return bbits < 0 ? (uint32_t)key : bbits ? (uint32_t)(mix64(key) >> (64 - bbits)) : 0u;
+}
+
+// Keys (and, with rk, linearised non-key tuples) of the nonzeros
+// base + i0 + threadIdx.x + u * blockDim.x, u < PT_U, of a tile of n: mode by
+// mode, so each mode's PT_U loads are in flight together (a loop over the
+// nonzeros that finishes one key before the next load waits on memory PT_U
+// times as often).  Entries past n are 0.
+constexpr int PT_U = 8;  // nonzeros per thread in one sweep: PT_TILE / PT_THREADS
+__device__ __forceinline__ void pt_load_keys(const uint32_t* __restrict__ h, uint32_t d, uint64_t base, uint32_t i0,
+                                             uint32_t n, const KeyCodec& c, unsigned long long (&key)[PT_U]) {
+#pragma unroll
+  for (int u = 0; u < PT_U; ++u) key[u] = 0;
+  for (uint32_t j = 0; j < c.n; ++j) {
+    const uint32_t m = c.mode[j];
+    const unsigned long long st = c.stride[j];
+    uint32_t x[PT_U];
+#pragma unroll
+    for (int u = 0; u < PT_U; ++u) {
+      const uint32_t i = i0 + threadIdx.x + u * blockDim.x;
+      x[u] = i < n ? h[(base + i) * d + m] : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < PT_U; ++u) key[u] += (unsigned long long)x[u] * st;
+  }
 }
 
 // Exclusive scan, in place, of a[0..n) in LDS (n <= 8 * blockDim.x).
@@ -728,11 +756,14 @@ __device__ __forceinline__ void pt_hist_tile(const uint32_t* __restrict__ h, uin
   __shared__ uint32_t lh[PT_MAX_BUCKETS];
   for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) lh[b] = 0;
   __syncthreads();
-  const uint64_t base = t * PT_TILE, end = min(N, base + PT_TILE);
-  for (uint64_t e = base + threadIdx.x; e < end; e += blockDim.x) {
-    unsigned long long key = 0;
-    encode_key(h, d, e, codec, &key);  // the table's own tensor: always within radix
-    atomicAdd(&lh[pt_bucket(key, bbits)], 1u);
+  const uint64_t base = t * PT_TILE;
+  const uint32_t n = (uint32_t)min<uint64_t>(PT_TILE, N - base);
+  for (uint32_t i0 = 0; i0 < n; i0 += PT_U * blockDim.x) {
+    unsigned long long key[PT_U];
+    pt_load_keys(h, d, base, i0, n, codec, key);  // the table's own tensor: always within radix
+#pragma unroll
+    for (int u = 0; u < PT_U; ++u)
+      if (i0 + threadIdx.x + u * blockDim.x < n) atomicAdd(&lh[pt_bucket(key[u], bbits)], 1u);
   }
   __syncthreads();
   for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) hist[(uint64_t)b * ntiles + t] = lh[b];
@@ -783,26 +814,43 @@ __device__ __forceinline__ void pt_scatter_tile(const uint32_t* __restrict__ h, 
   pt_block_scan(lcur, nb, s_wave);
   for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) gdst[b] -= lcur[b];  // modulo 2^32: slot j -> gdst + j
   __syncthreads();
-  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
-    const uint64_t e = base + i;
-    unsigned long long key = 0, rk = 0;
-    encode_key(h, d, e, codec, &key);
-    for (uint32_t q = 0; q < rest.n; ++q) rk += (unsigned long long)h[e * d + rest.mode[q]] * rest.stride[q];
-    lkey[i] = key;
-    lrk[i] = (RestT)rk;
-    ls[atomicAdd(&lcur[pt_bucket(key, bbits)], 1u)] = (uint16_t)i;
+  for (uint32_t i0 = 0; i0 < n; i0 += PT_U * blockDim.x) {
+    unsigned long long key[PT_U], rk[PT_U];
+    pt_load_keys(h, d, base, i0, n, codec, key);
+    pt_load_keys(h, d, base, i0, n, rest, rk);
+#pragma unroll
+    for (int u = 0; u < PT_U; ++u) {
+      const uint32_t i = i0 + threadIdx.x + u * blockDim.x;
+      if (i >= n) continue;
+      lkey[i] = key[u];
+      lrk[i] = (RestT)rk[u];
+      ls[atomicAdd(&lcur[pt_bucket(key[u], bbits)], 1u)] = (uint16_t)i;
+    }
   }
   __syncthreads();
-  for (uint32_t j = threadIdx.x; j < n; j += blockDim.x) {
-    const uint32_t i = ls[j];
-    const unsigned long long key = lkey[i];
-    const uint32_t p = gdst[pt_bucket(key, bbits)] + j;
-    pkey[p] = key;
-    pid[p] = (uint32_t)(base + i);
-    KtPay<RestT, ValT> pl;
-    pl.k = lrk[i];
-    pl.v = val[base + i];
-    ppay[p] = pl;
+  for (uint32_t j0 = 0; j0 < n; j0 += PT_U * blockDim.x) {
+    uint32_t ii[PT_U];
+    ValT v[PT_U];
+#pragma unroll
+    for (int u = 0; u < PT_U; ++u) {  // the value gathers first, all in flight
+      const uint32_t j = j0 + threadIdx.x + u * blockDim.x;
+      ii[u] = j < n ? ls[j] : 0u;
+      v[u] = j < n ? val[base + ii[u]] : ValT(0);
+    }
+#pragma unroll
+    for (int u = 0; u < PT_U; ++u) {
+      const uint32_t j = j0 + threadIdx.x + u * blockDim.x;
+      if (j >= n) continue;
+      const uint32_t i = ii[u];
+      const unsigned long long key = lkey[i];
+      const uint32_t p = gdst[pt_bucket(key, bbits)] + j;
+      pkey[p] = key;
+      pid[p] = (uint32_t)(base + i);
+      KtPay<RestT, ValT> pl;
+      pl.k = lrk[i];
+      pl.v = v[u];
+      ppay[p] = pl;
+    }
   }
 }
 
@@ -903,23 +951,41 @@ __device__ __forceinline__ void pd_scatter_tile(const uint32_t* __restrict__ h, 
   pt_block_scan(lcur, nb, s_wave);
   for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) gdst[b] -= lcur[b];
   __syncthreads();
-  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
-    const uint64_t e = base + i;
-    unsigned long long key = 0, rk = 0;
-    encode_key(h, d, e, codec, &key);  // < nb
-    for (uint32_t q = 0; q < rest.n; ++q) rk += (unsigned long long)h[e * d + rest.mode[q]] * rest.stride[q];
-    lrk[i] = (RestT)rk;
-    lb[i] = (uint16_t)key;
-    ls[atomicAdd(&lcur[key], 1u)] = (uint16_t)i;
+  for (uint32_t i0 = 0; i0 < n; i0 += PT_U * blockDim.x) {
+    unsigned long long key[PT_U], rk[PT_U];
+    pt_load_keys(h, d, base, i0, n, codec, key);  // < nb
+    pt_load_keys(h, d, base, i0, n, rest, rk);
+#pragma unroll
+    for (int u = 0; u < PT_U; ++u) {
+      const uint32_t i = i0 + threadIdx.x + u * blockDim.x;
+      if (i >= n) continue;
+      lrk[i] = (RestT)rk[u];
+      lb[i] = (uint16_t)key[u];
+      ls[atomicAdd(&lcur[key[u]], 1u)] = (uint16_t)i;
+    }
   }
   __syncthreads();
-  for (uint32_t j = threadIdx.x; j < n; j += blockDim.x) {
-    const uint32_t i = ls[j], b = lb[i];
-    const uint32_t p = gdst[b] + j;
-    perm[p] = (uint32_t)(base + i);
-    rest_out[p] = lrk[i];
-    gval[p] = val[base + i];
-    slice_of[p] = cidx[b];
+  for (uint32_t j0 = 0; j0 < n; j0 += PT_U * blockDim.x) {
+    uint32_t ii[PT_U], sl[PT_U];
+    ValT v[PT_U];
+#pragma unroll
+    for (int u = 0; u < PT_U; ++u) {  // the gathers first, all in flight
+      const uint32_t j = j0 + threadIdx.x + u * blockDim.x;
+      ii[u] = j < n ? ls[j] : 0u;
+      v[u] = j < n ? val[base + ii[u]] : ValT(0);
+      sl[u] = j < n ? cidx[lb[ii[u]]] : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < PT_U; ++u) {
+      const uint32_t j = j0 + threadIdx.x + u * blockDim.x;
+      if (j >= n) continue;
+      const uint32_t i = ii[u];
+      const uint32_t p = gdst[lb[i]] + j;
+      perm[p] = (uint32_t)(base + i);
+      rest_out[p] = lrk[i];
+      gval[p] = v[u];
+      slice_of[p] = sl[u];
+    }
   }
 }
 
@@ -1061,14 +1127,15 @@ __device__ bool pt_group_bucket(unsigned long long* tk, uint32_t* tc, uint32_t c
   __syncthreads();
   if (!GLOBAL && pt_ld32(s_ndist) >= PT_LDS_KEYS) return false;  // the same value in every thread
   // counts -> offsets in slot order; (slices << 32 | nonzeros) scanned
-  // together, 8 consecutive slots per thread, a chunk of 8 * blockDim slots
-  // at a time
-  const uint32_t chunk = 8u * blockDim.x;
+  // together, SPT consecutive slots per thread, a chunk of SPT * blockDim
+  // slots at a time (4 x 1024 covers the LDS table in one chunk)
+  constexpr int SPT = 4;
+  const uint32_t chunk = SPT * blockDim.x;
   for (uint32_t c0 = 0; c0 < cap; c0 += chunk) {
-    const uint32_t s0 = c0 + 8u * threadIdx.x;
-    unsigned long long v[8], sum = 0;
+    const uint32_t s0 = c0 + SPT * threadIdx.x;
+    unsigned long long v[SPT], sum = 0;
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
+    for (int j = 0; j < SPT; ++j) {
       const uint32_t s = s0 + j;
       v[j] = 0;
       if (s < cap) {
@@ -1089,7 +1156,7 @@ __device__ bool pt_group_bucket(unsigned long long* tk, uint32_t* tc, uint32_t c
     for (int w = 0; w < wid; ++w) pre += s_wave[w];
     pre += inc - sum;
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
+    for (int j = 0; j < SPT; ++j) {
       const uint32_t s = s0 + j;
       if (v[j]) {
         KtSlice sl;
@@ -1150,12 +1217,13 @@ __device__ bool pt_group_bucket(unsigned long long* tk, uint32_t* tc, uint32_t c
   }
   // place: every nonzero at its key's cursor (loads of four at a time in
   // flight)
-  for (uint32_t i0 = threadIdx.x; i0 < n; i0 += U * blockDim.x) {
-    unsigned long long k[U];
-    uint32_t id[U];
-    KtPay<RestT, ValT> pl[U];
+  constexpr int UP = U;
+  for (uint32_t i0 = threadIdx.x; i0 < n; i0 += UP * blockDim.x) {
+    unsigned long long k[UP];
+    uint32_t id[UP];
+    KtPay<RestT, ValT> pl[UP];
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
+    for (int u = 0; u < UP; ++u) {
       const uint32_t i = i0 + u * blockDim.x;
       const uint64_t q = bs + (i < n ? i : 0u);
       k[u] = pkey[q];
@@ -1163,7 +1231,7 @@ __device__ bool pt_group_bucket(unsigned long long* tk, uint32_t* tc, uint32_t c
       pl[u] = ppay[q];
     }
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
+    for (int u = 0; u < UP; ++u) {
       if (i0 + u * blockDim.x >= n) continue;
       const uint32_t s = pt_slot<GLOBAL>(tk, cap, k[u]);  // present: a read-only probe
       const uint64_t p = bs - obase + pt_add32(&tc[s], 1u);
@@ -1174,6 +1242,17 @@ __device__ bool pt_group_bucket(unsigned long long* tk, uint32_t* tc, uint32_t c
   }
   return true;
 }
+
+// LDS of one bucket workgroup (declared once per kernel, so a kernel that
+// groups either key width still holds one copy)
+struct PtGroupLds {
+  unsigned long long lk[PT_TAB];
+  uint32_t lc[PT_TAB];
+  uint32_t lmap[PT_TAB];
+  unsigned long long s_wave[PT_GROUP_THREADS / 64];
+  unsigned long long s_carry;
+  uint32_t s_ndist;
+};
 
 // Group bucket b of a table whose partitioned nonzeros start at obase and
 // end at oend (positions in the shared partitioned arrays).
@@ -1188,13 +1267,7 @@ __device__ __forceinline__ void pt_group_one(uint32_t b, const unsigned long lon
                                              uint32_t* __restrict__ perm, RestT* __restrict__ rest_out,
                                              ValT* __restrict__ gval, KtEntry* __restrict__ tab, uint64_t cap,
                                              int region, uint32_t* __restrict__ ovf_cnt,
-                                             uint32_t* __restrict__ ovf_list) {
-  __shared__ unsigned long long lk[PT_TAB];
-  __shared__ uint32_t lc[PT_TAB];
-  __shared__ uint32_t lmap[PT_TAB];
-  __shared__ unsigned long long s_wave[PT_GROUP_THREADS / 64];
-  __shared__ unsigned long long s_carry;
-  __shared__ uint32_t s_ndist;
+                                             uint32_t* __restrict__ ovf_list, PtGroupLds& L) {
   const uint64_t bs = off[(uint64_t)b * ntiles];
   const uint64_t be = b + 1 < nb ? off[(uint64_t)(b + 1) * ntiles] : oend;
   const uint32_t n = (uint32_t)(be - bs);
@@ -1204,28 +1277,29 @@ __device__ __forceinline__ void pt_group_one(uint32_t b, const unsigned long lon
   X.region = region;
   X.r0 = region ? (cap / nb) * b : 0;
   X.r1 = region ? X.r0 + cap / nb : 0;
-  X.map = region && cap / nb <= PT_TAB ? lmap : nullptr;
+  X.map = region && cap / nb <= PT_TAB ? L.lmap : nullptr;
   X.ovf_cnt = ovf_cnt;
   X.ovf_list = ovf_list;
   // the LDS table while the bucket's distinct keys fit, else the region
-  if (!pt_group_bucket<false>(lk, lc, PT_TAB, bs, n, obase, pkey, pid, ppay, tmps, nkeys_out, perm, rest_out, gval,
-                              X, &s_carry, s_wave, &s_ndist)) {
+  if (!pt_group_bucket<false>(L.lk, L.lc, PT_TAB, bs, n, obase, pkey, pid, ppay, tmps, nkeys_out, perm, rest_out,
+                              gval, X, &L.s_carry, L.s_wave, &L.s_ndist)) {
     __syncthreads();
     pt_group_bucket<true>(gkey + bs, gcnt + bs, n, bs, n, obase, pkey, pid, ppay, tmps, nkeys_out, perm, rest_out,
-                          gval, X, &s_carry, s_wave, &s_ndist);
+                          gval, X, &L.s_carry, L.s_wave, &L.s_ndist);
   }
 }
 
 template <class RestT, class ValT>
-__global__ void __launch_bounds__(PT_GROUP_THREADS) k_pt_group(
+__global__ void __launch_bounds__(PT_GROUP_THREADS, PT_GROUP_WAVES_PER_SIMD) k_pt_group(
     const unsigned long long* __restrict__ off, uint64_t ntiles, uint32_t nb, uint64_t N,
     const unsigned long long* __restrict__ pkey, const uint32_t* __restrict__ pid,
     const KtPay<RestT, ValT>* __restrict__ ppay, unsigned long long* __restrict__ gkey, uint32_t* __restrict__ gcnt,
     KtSlice* __restrict__ tmps, uint32_t* __restrict__ bkeys, uint32_t* __restrict__ perm,
     RestT* __restrict__ rest_out, ValT* __restrict__ gval, KtEntry* __restrict__ tab, uint64_t cap, int region,
     uint32_t* __restrict__ ovf_cnt, uint32_t* __restrict__ ovf_list) {
+  __shared__ PtGroupLds L;
   pt_group_one<RestT, ValT>(blockIdx.x, off, ntiles, nb, 0, N, pkey, pid, ppay, gkey, gcnt, tmps, &bkeys[blockIdx.x],
-                            perm, rest_out, gval, tab, cap, region, ovf_cnt, ovf_list);
+                            perm, rest_out, gval, tab, cap, region, ovf_cnt, ovf_list, L);
 }
 
 // Slices of bucket b to their compact positions (sbase: the scanned key
@@ -1399,7 +1473,7 @@ __global__ void __launch_bounds__(PT_THREADS) k_mt_scatter(const MtTableG* __res
   }
 }
 
-__global__ void __launch_bounds__(PT_GROUP_THREADS) k_mt_group(
+__global__ void __launch_bounds__(PT_GROUP_THREADS, PT_GROUP_WAVES_PER_SIMD) k_mt_group(
     const MtTableG* __restrict__ tabs, int nt, const unsigned long long* __restrict__ off,
     const unsigned long long* __restrict__ pkey, const uint32_t* __restrict__ pid, const void* __restrict__ ppay,
     unsigned long long* __restrict__ gkey, uint32_t* __restrict__ gcnt, KtSlice* __restrict__ tmps,
@@ -1407,15 +1481,16 @@ __global__ void __launch_bounds__(PT_GROUP_THREADS) k_mt_group(
   const MtTableG& T = tabs[mt_find<MT_BUCKET>(tabs, nt, blockIdx.x)];
   const uint32_t b = (uint32_t)(blockIdx.x - T.bucket_base);
   const unsigned long long* os = off + T.cnt_base;
+  __shared__ PtGroupLds L;
   if (T.wide)
     pt_group_one<unsigned long long, double>(b, os, T.ntiles, T.nb, T.item_base, T.item_base + T.N, pkey, pid,
                                              (const KtPay<unsigned long long, double>*)ppay, gkey, gcnt, tmps,
                                              &bkeys[blockIdx.x], T.perm, T.rest64, T.gval, T.entries, T.cap, T.region,
-                                             ovf_cnt, ovf_list);
+                                             ovf_cnt, ovf_list, L);
   else
     pt_group_one<uint32_t, double>(b, os, T.ntiles, T.nb, T.item_base, T.item_base + T.N, pkey, pid,
                                    (const KtPay<uint32_t, double>*)ppay, gkey, gcnt, tmps, &bkeys[blockIdx.x], T.perm,
-                                   T.rest32, T.gval, T.entries, T.cap, T.region, ovf_cnt, ovf_list);
+                                   T.rest32, T.gval, T.entries, T.cap, T.region, ovf_cnt, ovf_list, L);
 }
 
 __global__ void k_mt_slices(const MtTableG* __restrict__ tabs, int nt, const unsigned long long* __restrict__ off,
@@ -1659,6 +1734,9 @@ extern "C" int ptc_hashtable_build_ex(ptc_ctx* ctx, const ptc_dtensor* t, int gr
   if (!ctx || !t || !out) return PTC_ERR_ARG;
   *out = nullptr;
   if (grouping < PTC_GROUP_AUTO || grouping > PTC_GROUP_PARTITION) return set_err(ctx, PTC_ERR_ARG, "grouping %d", grouping);
+  // slices hold 32-bit nonzero ids, and the row passes address B by 32-bit
+  // positions (a contraction's output can be larger than an upload)
+  if (t->N >= 0xFFFFFFFFull) return set_err(ctx, PTC_ERR_RANGE, "N=%llu exceeds 2^32-2", (unsigned long long)t->N);
   PTC_HIP(ctx, hipSetDevice(ctx->device));
   ptc_hashtable* ht = new ptc_hashtable();
   ht->ctx = ctx;
