@@ -193,26 +193,46 @@ __global__ void k_scan_reduce(const In* __restrict__ in, uint64_t n,
   if (threadIdx.x == 0) block_sums[blockIdx.x] = tot;
 }
 
+// The tile goes through LDS both ways, so global loads and stores are
+// coalesced (element k * SCAN_THREADS + thread) while each thread scans
+// SCAN_ITEMS consecutive elements; one pad word per SCAN_ITEMS keeps a
+// thread's consecutive words in distinct banks across the wave.
+__device__ __forceinline__ uint32_t scan_slot(uint32_t i) { return i + i / SCAN_ITEMS; }
+
 template <class In>
-__global__ void k_scan_apply(const In* __restrict__ in, unsigned long long* __restrict__ out,
-                             uint64_t n, const unsigned long long* __restrict__ block_prefix,
-                             unsigned long long* __restrict__ total_out) {
-  uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
+__global__ void __launch_bounds__(SCAN_THREADS) k_scan_apply(const In* __restrict__ in,
+                                                             unsigned long long* __restrict__ out, uint64_t n,
+                                                             const unsigned long long* __restrict__ block_prefix,
+                                                             unsigned long long* __restrict__ total_out) {
+  __shared__ unsigned long long s_tile[SCAN_TILE + SCAN_TILE / SCAN_ITEMS];
+  const uint64_t b0 = (uint64_t)blockIdx.x * SCAN_TILE;
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    const uint32_t i = k * SCAN_THREADS + threadIdx.x;
+    s_tile[scan_slot(i)] = load_elem(in, b0 + i, n);
+  }
+  __syncthreads();
   unsigned long long v[SCAN_ITEMS];
   unsigned long long s = 0;
 #pragma unroll
   for (int k = 0; k < SCAN_ITEMS; ++k) {
-    v[k] = load_elem(in, base + k, n);
+    v[k] = s_tile[scan_slot(threadIdx.x * SCAN_ITEMS + k)];
     s += v[k];
   }
   unsigned long long tot;
   unsigned long long run = block_exclusive_scan(s, &tot) + (block_prefix ? block_prefix[blockIdx.x] : 0ull);
 #pragma unroll
   for (int k = 0; k < SCAN_ITEMS; ++k) {
-    if (base + k < n) out[base + k] = run;
+    s_tile[scan_slot(threadIdx.x * SCAN_ITEMS + k)] = run;
     run += v[k];
   }
   if (total_out && blockIdx.x == gridDim.x - 1 && threadIdx.x == SCAN_THREADS - 1) *total_out = run;
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    const uint32_t i = k * SCAN_THREADS + threadIdx.x;
+    if (b0 + i < n) out[b0 + i] = s_tile[scan_slot(i)];
+  }
 }
 
 // Inputs of up to SCAN_SMALL elements: one workgroup, one launch.
