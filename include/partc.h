@@ -166,8 +166,9 @@ void ptc_hashtable_free(ptc_hashtable* h);
  * equal keys, the probe index over distinct keys only; key spaces within 32
  * bits), PTC_GROUP_PARTITION (nonzeros partitioned into buckets by key hash,
  * each bucket grouped in an LDS table, the probe index over distinct keys
- * only; fewer than 2^32 nonzeros) or PTC_GROUP_AUTO (sorting up to 24-bit
- * key spaces, partitioning above).  Sorting is the sort-based matricization
+ * only; fewer than 2^32 nonzeros) or PTC_GROUP_AUTO (one counting pass up
+ * to 4096 direct keys, sorting up to 16-bit key spaces, partitioning
+ * above).  Sorting is the sort-based matricization
  * of the reference (matricizedspGEMM_sorting.hpp:194-754, which exits before
  * finishing) applied to the grouping step. */
 enum { PTC_GROUP_AUTO = 0, PTC_GROUP_HASH = 1, PTC_GROUP_SORT = 2, PTC_GROUP_PARTITION = 3 };

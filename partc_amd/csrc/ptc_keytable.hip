@@ -32,12 +32,14 @@
 namespace ptc {
 
 constexpr int KT_THREADS = 256;
-// Automatic grouping sorts key spaces of at most this many bits (three
-// 8-bit radix passes); wider ones hash, where one insert per nonzero beats a
-// fourth pass over keys and ids (measured on MI355X: 30-bit keys of the
-// nips-shape B table hash in 0.52 ms, sort in 0.68 ms; the 20-bit two-tensor
-// B table sorts in 0.56 ms, hashes in 0.98 ms).
-constexpr int SORT_MAX_BITS = 24;
+// Automatic grouping sorts key spaces of at most this many bits (two 8-bit
+// radix passes) and partitions wider ones, measured on MI355X: the 20-bit
+// two-tensor B table (5 M nonzeros, 1 M keys) partitions in 0.31 ms, sorts
+// in 0.51 ms; the 14-bit two-tensor A table (10000 keys) sorts in 0.37 ms,
+// partitions in 0.55 ms, and its contraction runs 0.27 ms faster over the
+// sorted table; 30-bit nips-shape B keys partition in 0.22 ms, sort in
+// 0.63 ms.
+constexpr int SORT_AUTO_BITS = 16;
 
 __device__ __forceinline__ unsigned long long mix64(unsigned long long k) {
   // MurmurHash3 fmix64 finaliser
@@ -1800,7 +1802,7 @@ extern "C" int ptc_hashtable_build_ex(ptc_ctx* ctx, const ptc_dtensor* t, int gr
     *out = ht;
     return PTC_OK;
   }
-  if (sortable && (grouping == PTC_GROUP_SORT || (grouping == PTC_GROUP_AUTO && key_bits <= SORT_MAX_BITS))) {
+  if (sortable && (grouping == PTC_GROUP_SORT || (grouping == PTC_GROUP_AUTO && key_bits <= SORT_AUTO_BITS))) {
     if (!ht->entries || !ht->perm || !(ht->gval || ht->gval32) || !(ht->rest_key32 || ht->rest_key64)) {
       ptc_hashtable_free(ht);
       return PTC_ERR_NOMEM;
@@ -1814,13 +1816,14 @@ extern "C" int ptc_hashtable_build_ex(ptc_ctx* ctx, const ptc_dtensor* t, int gr
     *out = ht;
     return PTC_OK;
   }
-  // partitioned hashing: requested, or the default for wide hashed key spaces
+  // partitioned hashing: requested, or the default for key spaces too wide to
+  // sort (direct ones too: the index is then written by key)
   const bool partitionable = N > 0 && N <= PT_MAX_N;
   if (grouping == PTC_GROUP_PARTITION && !partitionable) {
     ptc_hashtable_free(ht);
     return set_err(ctx, PTC_ERR_UNSUPPORTED, "partitioned grouping needs 1 to %llu nonzeros", (unsigned long long)PT_MAX_N);
   }
-  if (partitionable && (grouping == PTC_GROUP_PARTITION || (grouping == PTC_GROUP_AUTO && !ht->direct))) {
+  if (partitionable && (grouping == PTC_GROUP_PARTITION || grouping == PTC_GROUP_AUTO)) {
     if (!ht->entries || !ht->perm || !(ht->gval || ht->gval32) || !(ht->rest_key32 || ht->rest_key64)) {
       ptc_hashtable_free(ht);
       return PTC_ERR_NOMEM;
