@@ -649,6 +649,7 @@ __device__ __forceinline__ void pass_item(const PP& P, uint32_t widx) {
       if (w < wid) wbase += s_w[w];
       ns += s_w[w];
     }
+    ns = (uint32_t)__builtin_amdgcn_readfirstlane((int)ns);  // the same in every lane: keep it scalar
     uint32_t dst = wbase + incl - nkeep;
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
@@ -668,10 +669,14 @@ __device__ __forceinline__ void pass_item(const PP& P, uint32_t widx) {
     const uint32_t fe = (uint32_t)(min(qend, g1) - g0);
     __syncthreads();
     if (ns) {
-      // each wave expands a contiguous, 64-aligned share of [fb, fe)
+      // each wave expands a contiguous, 64-aligned share of [fb, fe).  The
+      // walk's state is the same in every lane of a wave and is kept in
+      // scalar registers (readfirstlane / readlane), so its tests are
+      // scalar branches rather than exec-masked regions.
+      const uint32_t wv = (uint32_t)__builtin_amdgcn_readfirstlane(wid);
       const uint32_t nf = fe - fb;
       const uint32_t span = ((nf + NT - 1) / NT) * 64;
-      const uint32_t wb = fb + wid * span;
+      const uint32_t wb = fb + wv * span;
       const uint32_t we = min(fe, wb + span);
       // position of the wave's first flop (searched once, by every lane)
       uint32_t qa = 0;
@@ -679,14 +684,14 @@ __device__ __forceinline__ void pass_item(const PP& P, uint32_t widx) {
         uint32_t lo = 0, hi = ns - 1;
         while (lo < hi) {
           uint32_t mid = (lo + hi + 1) >> 1;
-          if (s_start[mid] <= wb) lo = mid; else hi = mid - 1;
+          if ((uint32_t)__builtin_amdgcn_readfirstlane((int)s_start[mid]) <= wb) lo = mid; else hi = mid - 1;
         }
         qa = lo;
       }
       // U groups of 64 flops per step: positions first (LDS only), then all
       // global loads, then the accumulations -- 2U loads in flight per wave
       constexpr int U = PTC_PASS_UNROLL;
-      unsigned char* s_bnd = smem + STAGE * 16 + wid * 64;  // this wave's boundary flags
+      unsigned char* s_bnd = smem + STAGE * 16 + wv * 64;  // this wave's boundary flags
       for (uint32_t f0 = wb; f0 < we; f0 += 64 * U) {
         uint32_t ii[U];
         uint32_t bb[U];
@@ -700,11 +705,20 @@ __device__ __forceinline__ void pass_item(const PP& P, uint32_t widx) {
           if (fu < we) {  // wave-uniform
             // every kept position has at least one flop, so the wave's 64
             // flops lie in positions [qa, qa + 63], and after this step
-            // s_start[qa] <= fu < s_start[qa + 1]
-            if (qa + 1 < ns && s_start[qa + 1] <= fu) ++qa;
+            // s_start[qa] <= fu < s_start[qa + 1] (= nx; past the last
+            // position, no start)
+            const uint32_t n1 = qa + 1 < ns ? (uint32_t)__builtin_amdgcn_readfirstlane((int)s_start[qa + 1])
+                                            : 0xFFFFFFFFu;
+            const uint32_t n2 = qa + 2 < ns ? (uint32_t)__builtin_amdgcn_readfirstlane((int)s_start[qa + 2])
+                                            : 0xFFFFFFFFu;
+            uint32_t nx = n1;
+            if (n1 <= fu) {
+              ++qa;
+              nx = n2;
+            }
             const uint32_t f = min(fu + lane, we - 1);
             uint32_t i = qa;
-            if (qa + 1 < ns && s_start[qa + 1] <= min(fu + 63, we - 1)) {
+            if (nx <= min(fu + 63, we - 1)) {
               // the group crosses positions: lane k flags the flop where
               // position qa + 1 + k starts (a byte per flop of the group),
               // and a lane's position is qa + the flags at or before its flop
@@ -728,7 +742,7 @@ __device__ __forceinline__ void pass_item(const PP& P, uint32_t widx) {
               if (act[u]) atomicOr(P.overflow, 0x40000000u);
               act[u] = false;
             }
-            qa = __shfl(i, 63, 64);
+            qa = (uint32_t)__builtin_amdgcn_readlane((int)i, 63);
           }
         }
         ColT col[U];
