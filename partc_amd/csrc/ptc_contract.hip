@@ -48,8 +48,15 @@
 
 namespace ptc {
 
+// Groups of 64 flops a wave expands per step (2 loads each in flight).
+// Measured on MI355X (numeric row pass): 2 -> 5.85 ms, 4 -> 5.37, 6 -> 5.19,
+// 8 -> 5.07 ms on the two-tensor contraction, whose 64 KB accumulators
+// leave 4 waves per SIMD, so each wave needs many loads in flight; the
+// nips-shape pass (8 waves per SIMD) is flat from 4 to 8 (0.325 ms).  At 8
+// the dense numeric pass takes 58 VGPRs, still 8 waves per SIMD; 12 and 16
+// take 81 and 95, gain 1 % on the two-tensor pass and lose 22 % on nips.
 #ifndef PTC_PASS_UNROLL
-#define PTC_PASS_UNROLL 4
+#define PTC_PASS_UNROLL 8
 #endif
 // A positions staged in LDS at a time: 2 per thread of the launch
 constexpr uint32_t NO_HEAVY = 0xFFFFFFFFu;
