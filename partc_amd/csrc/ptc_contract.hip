@@ -55,9 +55,12 @@ namespace ptc {
 // nips-shape pass (8 waves per SIMD) is flat from 4 to 8 (0.325 ms).  At 8
 // the dense numeric pass takes 58 VGPRs, still 8 waves per SIMD; 12 and 16
 // take 81 and 95, gain 1 % on the two-tensor pass and lose 22 % on nips.
+// FP32 values accumulated in FP64 are the exception: 5.23 ms at 4, 7.00 ms
+// at 8 on the two-tensor pass, so that value type keeps 4.
 #ifndef PTC_PASS_UNROLL
 #define PTC_PASS_UNROLL 8
 #endif
+constexpr int pass_unroll(int val) { return val == 3 ? 4 : PTC_PASS_UNROLL; }
 // A positions staged in LDS at a time: 2 per thread of the launch
 constexpr uint32_t NO_HEAVY = 0xFFFFFFFFu;
 
@@ -697,7 +700,7 @@ __device__ __forceinline__ void pass_item(const PP& P, uint32_t widx) {
       }
       // U groups of 64 flops per step: positions first (LDS only), then all
       // global loads, then the accumulations -- 2U loads in flight per wave
-      constexpr int U = PTC_PASS_UNROLL;
+      constexpr int U = pass_unroll(VAL);
       unsigned char* s_bnd = smem + STAGE * 16 + wv * 64;  // this wave's boundary flags
       for (uint32_t f0 = wb; f0 < we; f0 += 64 * U) {
         uint32_t ii[U];
