@@ -522,6 +522,11 @@ __device__ __forceinline__ void pass_item(const PP& P, uint32_t widx) {
   const uint32_t heavy = P.row_heavy[r];
   const uint64_t pend = (uint64_t)row.off + row.len;
   const uint64_t ncols = P.ncols;
+  // dense rows: the column space (at most a few thousand columns) as 32 bits
+  const uint32_t ncols32 = (uint32_t)min<uint64_t>(ncols, 0xFFFFFFFFull);
+  // a B index or a column out of range (inconsistent tables): the lane drops
+  // the multiply-add and the thread reports it once, after the item
+  bool bad = false;
   const bool flags = P.use_flags != 0;
 
   // accumulator storage
@@ -748,10 +753,9 @@ __device__ __forceinline__ void pass_item(const PP& P, uint32_t widx) {
             ii[u] = i;
             bb[u] = s_bias[i] + f;
             act[u] = fu + lane < we;
-            if ((uint64_t)bb[u] >= P.nb) {  // cannot happen for consistent tables
-              if (act[u]) atomicOr(P.overflow, 0x40000000u);
-              act[u] = false;
-            }
+            const bool inb = (uint64_t)bb[u] < P.nb;  // cannot fail for consistent tables
+            bad |= act[u] && !inb;
+            act[u] = act[u] && inb;
             qa = (uint32_t)__builtin_amdgcn_readlane((int)i, 63);
           }
         }
@@ -779,8 +783,8 @@ __device__ __forceinline__ void pass_item(const PP& P, uint32_t widx) {
           bool ok[U];
 #pragma unroll
           for (int u = 0; u < U; ++u) {
-            ok[u] = act[u] && (uint64_t)col[u] < ncols;
-            if (act[u] && !ok[u]) atomicOr(P.overflow, 0x40000000u);
+            ok[u] = act[u] && (uint32_t)col[u] < ncols32;
+            bad |= act[u] && !ok[u];
             o[u] = 0.f;
             pr[u] = 0.f;
             ep[u] = 0.f;
@@ -808,12 +812,12 @@ __device__ __forceinline__ void pass_item(const PP& P, uint32_t widx) {
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-          if (!act[u]) continue;
           const ColT c = col[u];
           if (DENSE) {
-            if ((uint64_t)c >= ncols) {
-              atomicOr(P.overflow, 0x40000000u);
-            } else if (PASS == PASS_SYMBOLIC) {
+            const bool ok = act[u] && (uint32_t)c < ncols32;
+            bad |= act[u] && !ok;
+            if (!ok) continue;
+            if (PASS == PASS_SYMBOLIC) {
               atomicOr(&bits[c >> 5], 1u << (c & 31));
             } else {
               if (F32) acc_f32_lds<COMP>(&acc_s[c], &acc_c[c], s_aval32[ii[u]], bv32[u]);
@@ -822,6 +826,7 @@ __device__ __forceinline__ void pass_item(const PP& P, uint32_t widx) {
               if (flags) flag[c] = 1;
             }
           } else {
+            if (!act[u]) continue;
             const double v = (PASS != PASS_NUMERIC || F32) ? 0.0
                            : F32IN ? (double)s_aval32[ii[u]] * (double)bv32[u] : s_aval[ii[u]] * bv[u];
             if (S) {
@@ -850,6 +855,8 @@ __device__ __forceinline__ void pass_item(const PP& P, uint32_t widx) {
     __syncthreads();
     q0 = qlim;
   }
+
+  if (bad) atomicOr(P.overflow, 0x40000000u);
 
   // ---- a split row (or a row accumulated in HBM) is finished by the merge launch ----
   if (!MERGE) {
