@@ -35,6 +35,11 @@ struct RowParams {
   const KtSlice* rows;
   const unsigned long long* apre;
   unsigned long long chunk;
+  // non-NULL: the contraction's flop total on the device; chunks then grow
+  // to total / max_items when that is larger (few, large work items for
+  // very large contractions)
+  const unsigned long long* total;
+  uint64_t max_items;
   uint32_t* row_nchunk;
   unsigned long long* row_flops;
   unsigned long long* packed;     // (heavy << 32) | nchunk, scanned later

@@ -206,6 +206,8 @@ int launch_row_features(ptc_ctx* ctx, const uint32_t* row_of_pos, uint64_t npos,
 
 __global__ void k_rows(RowParams P) {
   const uint64_t live = P.live ? (*P.live >> 32) : P.nrows;
+  unsigned long long chunk = P.chunk;
+  if (P.total) chunk = max(chunk, (*P.total + P.max_items - 1) / P.max_items);
   for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < P.nrows; r += (uint64_t)gridDim.x * blockDim.x) {
     if (r >= live) {  // allocated beyond the table's rows: no work, zero sums
       P.row_nchunk[r] = 0;
@@ -216,7 +218,7 @@ __global__ void k_rows(RowParams P) {
     }
     KtSlice s = P.rows[r];
     unsigned long long f = P.apre[(uint64_t)s.off + s.len] - P.apre[s.off];
-    uint32_t nch = f ? (uint32_t)((f + P.chunk - 1) / P.chunk) : 0u;
+    uint32_t nch = f ? (uint32_t)((f + chunk - 1) / chunk) : 0u;
     P.row_nchunk[r] = nch;
     P.row_flops[r] = f;
     P.packed[r] = ((unsigned long long)(nch > 1) << 32) | nch;

@@ -15,6 +15,7 @@ constexpr uint32_t DEFAULT_DENSE_MAX = 8192;
 constexpr uint32_t DEFAULT_LDS_HASH = 2048;
 constexpr uint64_t DEFAULT_CHUNK = 32768;         // multiply-adds per work item, hash accumulators
 constexpr uint64_t DEFAULT_DENSE_CHUNK = 65536;   // floor for dense accumulators
+constexpr uint64_t DENSE_MAX_ITEMS = 3072;         // dense chunks grow to total flops / this
 
 // Hash accumulator plan of every row: LDS slots or an HBM region.
 struct HashPlan {

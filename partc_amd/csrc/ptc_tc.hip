@@ -409,6 +409,15 @@ int prepare_launch(Run& R, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B,
     P.rows = HT_A->slices;
     P.apre = R.apre;
     P.chunk = chunk;
+    // dense rows: at least total / DENSE_MAX_ITEMS multiply-adds per item.
+    // Measured on MI355X: the two-tensor numeric pass (2.6e9 multiply-adds,
+    // 4 waves per SIMD) takes 4.93 ms with 192000 per item, 4.60 ms from
+    // 524288 to 2097152; the nips-shape pass (1.4e8) is fastest at its
+    // 65536 floor (0.326 ms; 0.508 ms at 131072: too few items for the CUs).
+    if (R.dense && !opt.chunk_flops) {
+      P.total = R.apre + R.npos;
+      P.max_items = DENSE_MAX_ITEMS;
+    }
     P.row_nchunk = R.row_nchunk;
     P.row_flops = R.row_flops;
     P.packed = R.packed;
