@@ -724,10 +724,11 @@ __device__ __forceinline__ void pass_item(const PP& P, uint32_t widx) {
             // flops lie in positions [qa, qa + 63], and after this step
             // s_start[qa] <= fu < s_start[qa + 1] (= nx; past the last
             // position, no start)
-            const uint32_t n1 = qa + 1 < ns ? (uint32_t)__builtin_amdgcn_readfirstlane((int)s_start[qa + 1])
-                                            : 0xFFFFFFFFu;
-            const uint32_t n2 = qa + 2 < ns ? (uint32_t)__builtin_amdgcn_readfirstlane((int)s_start[qa + 2])
-                                            : 0xFFFFFFFFu;
+            // both starts read at once (indices clamped into the staged
+            // range, ns >= 1), one LDS round trip per group
+            const uint32_t r1 = s_start[min(qa + 1, ns - 1)], r2 = s_start[min(qa + 2, ns - 1)];
+            const uint32_t n1 = qa + 1 < ns ? (uint32_t)__builtin_amdgcn_readfirstlane((int)r1) : 0xFFFFFFFFu;
+            const uint32_t n2 = qa + 2 < ns ? (uint32_t)__builtin_amdgcn_readfirstlane((int)r2) : 0xFFFFFFFFu;
             uint32_t nx = n1;
             if (n1 <= fu) {
               ++qa;
