@@ -498,7 +498,6 @@ __device__ __forceinline__ void pass_item(const PP& P, uint32_t widx) {
   __shared__ uint32_t s_w[PB_MAX / 64];
   __shared__ uint32_t s_misc[4];
   __shared__ uint32_t s_rowc[PTC_MAX_MODES];
-  __shared__ uint32_t s_ns;
 
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   // dense rows stream 16-bit column keys (their column space is below 2^16)
@@ -521,7 +520,6 @@ __device__ __forceinline__ void pass_item(const PP& P, uint32_t widx) {
   const unsigned long long per = (fr + nch - 1) / nch;
   const unsigned long long g0 = flo + (unsigned long long)k * per;
   const unsigned long long g1 = min(flo + fr, g0 + per);
-  const uint32_t heavy = P.row_heavy[r];
   const uint64_t pend = (uint64_t)row.off + row.len;
   const uint64_t ncols = P.ncols;
   // dense rows: the column space (at most a few thousand columns) as 32 bits

@@ -1747,7 +1747,7 @@ extern "C" int ptc_hashtable_build_ex(ptc_ctx* ctx, const ptc_dtensor* t, int gr
     ptc_hashtable_free(ht);
     return hip_check(ctx, hipGetLastError(), "event");
   }
-  hipEventRecord(ht->ev_start, ctx->stream);
+  (void)hipEventRecord(ht->ev_start, ctx->stream);
   int rc = make_codec(ctx, t, t->dim_tc, t->nk, &ht->codec);
   if (!rc) rc = make_codec(ctx, t, t->dim_tc + t->nk, t->d - t->nk, &ht->rest);
   if (rc) {
@@ -1798,7 +1798,7 @@ extern "C" int ptc_hashtable_build_ex(ptc_ctx* ctx, const ptc_dtensor* t, int gr
       ptc_hashtable_free(ht);
       return rc;
     }
-    hipEventRecord(ht->ev_end, ctx->stream);
+    (void)hipEventRecord(ht->ev_end, ctx->stream);
     *out = ht;
     return PTC_OK;
   }
@@ -1812,7 +1812,7 @@ extern "C" int ptc_hashtable_build_ex(ptc_ctx* ctx, const ptc_dtensor* t, int gr
       ptc_hashtable_free(ht);
       return rc;
     }
-    hipEventRecord(ht->ev_end, ctx->stream);
+    (void)hipEventRecord(ht->ev_end, ctx->stream);
     *out = ht;
     return PTC_OK;
   }
@@ -1833,7 +1833,7 @@ extern "C" int ptc_hashtable_build_ex(ptc_ctx* ctx, const ptc_dtensor* t, int gr
       ptc_hashtable_free(ht);
       return rc;
     }
-    hipEventRecord(ht->ev_end, ctx->stream);
+    (void)hipEventRecord(ht->ev_end, ctx->stream);
     *out = ht;
     return PTC_OK;
   }
@@ -1964,7 +1964,7 @@ extern "C" int ptc_hashtable_build_ex(ptc_ctx* ctx, const ptc_dtensor* t, int gr
     ptc_hashtable_free(ht);
     return hip_check(ctx, e, "hashtable build");
   }
-  hipEventRecord(ht->ev_end, st);
+  (void)hipEventRecord(ht->ev_end, st);
   *out = ht;
   return PTC_OK;
 }
@@ -2262,7 +2262,7 @@ int hashtable_from_rows(ptc_ctx* ctx, const ptc_dtensor* O, const KeyCodec& rowc
     return hip_check(ctx, hipGetLastError(), "event");
   }
   hipStream_t st = ctx->stream;
-  hipEventRecord(ht->ev_start, st);
+  (void)hipEventRecord(ht->ev_start, st);
   int rc = make_codec(ctx, O, O->dim_tc, O->nk, &ht->codec);
   if (!rc) rc = make_codec(ctx, O, O->dim_tc + O->nk, O->d - O->nk, &ht->rest);
   if (rc) {
@@ -2311,7 +2311,7 @@ int hashtable_from_rows(ptc_ctx* ctx, const ptc_dtensor* O, const KeyCodec& rowc
   else
     hipLaunchKernelGGL((k_rest_keys<uint32_t>), dim3(gn), dim3(256), 0, st, (const uint32_t*)O->hedges, O->d, N,
                        ht->rest, ht->rest_key32);
-  hipEventRecord(ht->ev_end, st);
+  (void)hipEventRecord(ht->ev_end, st);
   if (hipGetLastError() != hipSuccess) {
     ptc_hashtable_free(ht);
     return set_err(ctx, PTC_ERR_HIP, "row table kernels");
@@ -2389,7 +2389,7 @@ extern "C" void ptc_hashtable_free(ptc_hashtable* h) {
     }
     dfree(h->ctx, h->d_total);
   }
-  if (h->ev_start) hipEventDestroy(h->ev_start);
-  if (h->ev_end) hipEventDestroy(h->ev_end);
+  if (h->ev_start) (void)hipEventDestroy(h->ev_start);
+  if (h->ev_end) (void)hipEventDestroy(h->ev_end);
   delete h;
 }
