@@ -179,7 +179,7 @@ int read_words64(ptc_batch* b, const std::vector<const unsigned long long*>& src
   return rc;
 }
 
-void stats_mark(ptc_batch* b, int i) { hipEventRecord(b->ev[i], b->ctx->stream); }
+void stats_mark(ptc_batch* b, int i) { (void)hipEventRecord(b->ev[i], b->ctx->stream); }
 double stats_ms(ptc_batch* b, int i, int j) {
   float t = 0.f;
   if (hipEventElapsedTime(&t, b->ev[i], b->ev[j]) != hipSuccess) {
@@ -479,9 +479,9 @@ int plan_rest(ptc_batch* b) {
 
 // Drops what plan_rest made, keeping the key tables.
 void clear_rest(ptc_batch* b) {
-  hipStreamSynchronize(b->ctx->stream);
-  if (b->exec) hipGraphExecDestroy(b->exec);
-  if (b->graph) hipGraphDestroy(b->graph);
+  (void)hipStreamSynchronize(b->ctx->stream);
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
   b->exec = nullptr;
   b->graph = nullptr;
   for (auto*& o : b->out) {
@@ -531,8 +531,10 @@ extern "C" int ptc_batch_plan(ptc_ctx* ctx, int n, const ptc_dtensor* const* A, 
   b->out.assign(n, nullptr);
   b->ws = new Workspace(ctx);
   b->st.jobs = (uint32_t)n;
-  for (auto& e : b->ev) hipEventCreate(&e);
-  int rc = plan_tables(b);
+  int rc = PTC_OK;
+  for (auto& e : b->ev)
+    if (!rc && hipEventCreate(&e) != hipSuccess) rc = set_err(ctx, PTC_ERR_HIP, "batch timing events");
+  if (!rc) rc = plan_tables(b);
   if (!rc) rc = plan_rest(b);
   if (rc) {
     (void)hipGetLastError();
@@ -576,9 +578,9 @@ extern "C" int ptc_batch_output(const ptc_batch* b, int j, const ptc_dtensor** O
 
 extern "C" void ptc_batch_free(ptc_batch* b) {
   if (!b) return;
-  if (b->ctx) hipStreamSynchronize(b->ctx->stream);
-  if (b->exec) hipGraphExecDestroy(b->exec);
-  if (b->graph) hipGraphDestroy(b->graph);
+  if (b->ctx) (void)hipStreamSynchronize(b->ctx->stream);
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
   for (auto* o : b->out) ptc_dtensor_free(o);
   b->runs.clear();
   for (auto* h : b->HA) ptc_hashtable_free(h);
@@ -586,7 +588,7 @@ extern "C" void ptc_batch_free(ptc_batch* b) {
   for (auto* v : b->Ac) delete v;
   delete b->ws;
   for (auto& e : b->ev)
-    if (e) hipEventDestroy(e);
+    if (e) (void)hipEventDestroy(e);
   delete b;
 }
 

@@ -60,8 +60,8 @@ void* dalloc(ptc_ctx* ctx, size_t bytes) {
   if (e != hipSuccess) {
     // release the cache and retry once
     (void)hipGetLastError();
-    hipStreamSynchronize(ctx->stream);
-    for (auto& kv : ctx->free_blocks) hipFree(kv.second);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto& kv : ctx->free_blocks) (void)hipFree(kv.second);
     ctx->free_blocks.clear();
     ctx->cached_bytes = 0;
     e = hipMalloc(&p, sz);
@@ -89,7 +89,7 @@ void dfree(ptc_ctx* ctx, void* p) {
 
 int read_back(ptc_ctx* ctx, void* host, const void* dev, size_t bytes) {
   if (bytes > ctx->host_pinned_bytes) {
-    if (ctx->host_pinned) hipHostFree(ctx->host_pinned);
+    if (ctx->host_pinned) (void)hipHostFree(ctx->host_pinned);
     ctx->host_pinned = nullptr;
     size_t want = std::max<size_t>(bytes, 4096);
     PTC_HIP(ctx, hipHostMalloc(&ctx->host_pinned, want, hipHostMallocDefault));
@@ -379,24 +379,32 @@ extern "C" int ptc_create(ptc_ctx** out, int device) {
     delete ctx;
     return PTC_ERR_HIP;
   }
-  for (auto& e : ctx->ev) hipEventCreate(&e);
-  for (auto& e : ctx->kev) hipEventCreate(&e);
+  bool events_ok = true;
+  for (auto& e : ctx->ev) events_ok = events_ok && hipEventCreate(&e) == hipSuccess;
+  for (auto& e : ctx->kev) events_ok = events_ok && hipEventCreate(&e) == hipSuccess;
+  if (!events_ok) {  // timing events are part of every call's stats
+    (void)hipGetLastError();
+    ptc_destroy(ctx);
+    return PTC_ERR_HIP;
+  }
   *out = ctx;
   return PTC_OK;
 }
 
 extern "C" void ptc_destroy(ptc_ctx* ctx) {
   if (!ctx) return;
-  hipSetDevice(ctx->device);
-  hipStreamSynchronize(ctx->stream);
-  for (auto& kv : ctx->free_blocks) hipFree(kv.second);
-  for (auto& kv : ctx->live) hipFree(kv.first);
+  // teardown: nothing can be done about a failing release, so results are
+  // not checked
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  for (auto& kv : ctx->free_blocks) (void)hipFree(kv.second);
+  for (auto& kv : ctx->live) (void)hipFree(kv.first);
   for (auto& e : ctx->ev)
-    if (e) hipEventDestroy(e);
+    if (e) (void)hipEventDestroy(e);
   for (auto& e : ctx->kev)
-    if (e) hipEventDestroy(e);
-  if (ctx->host_pinned) hipHostFree(ctx->host_pinned);
-  hipStreamDestroy(ctx->stream);
+    if (e) (void)hipEventDestroy(e);
+  if (ctx->host_pinned) (void)hipHostFree(ctx->host_pinned);
+  (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
 

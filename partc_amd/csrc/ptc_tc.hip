@@ -32,12 +32,12 @@ struct Timer {
     for (int i = 0; i < 8; ++i) ev[i] = c->ev[i];
   }
   void mark() {
-    if (n < 8) hipEventRecord(ev[n++], ctx->stream);
+    if (n < 8) (void)hipEventRecord(ev[n++], ctx->stream);  // timing only
   }
   double ms(int a, int b) {
     if (a >= n || b >= n) return 0.0;
     float t = 0.f;
-    hipEventElapsedTime(&t, ev[a], ev[b]);
+    if (hipEventElapsedTime(&t, ev[a], ev[b]) != hipSuccess) t = 0.f;
     return t;
   }
 };
@@ -195,7 +195,7 @@ static int run_pass(Run& R, int pass, HashPlan* hp, const uint4* items, uint64_t
   // one pair of timing events brackets the row launch and its merge launch
   ptc_ctx* ctx = R.ctx;
   const bool timed = ctx->nkev + 2 <= 16;
-  if (timed) hipEventRecord(ctx->kev[ctx->nkev], ctx->stream);
+  if (timed) (void)hipEventRecord(ctx->kev[ctx->nkev], ctx->stream);
   int rc = launch_pass(ctx, pass, R.dense, R.col64, P, nitems, lds, false, R.threads);
   if (!rc && R.dense) {
     if (!merge_rows) {
@@ -210,7 +210,7 @@ static int run_pass(Run& R, int pass, HashPlan* hp, const uint4* items, uint64_t
   }
   if (rc) return rc;
   if (timed) {
-    hipEventRecord(ctx->kev[ctx->nkev + 1], ctx->stream);
+    (void)hipEventRecord(ctx->kev[ctx->nkev + 1], ctx->stream);
     ctx->kev_pass[ctx->nkev / 2] = pass;
     ctx->nkev += 2;
   }
@@ -786,7 +786,7 @@ int tc_impl(ptc_ctx* ctx, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B, 
     if (kr->from_rows) *kr->from_rows = rows_key ? 1 : 0;
   }
   if (stats) {
-    hipEventSynchronize(T.ev[T.n - 1]);
+    (void)hipEventSynchronize(T.ev[T.n - 1]);
     memset(stats, 0, sizeof(*stats));
     if (hto) stats->ms_build_out = hashtable_ms_build(hto);
     stats->nnz_a = A->N;
@@ -809,7 +809,7 @@ int tc_impl(ptc_ctx* ctx, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B, 
     // pass time = its row-pass launch + its merge launch
     for (int i = 0; i + 1 < ctx->nkev; i += 2) {
       float t = 0.f;
-      hipEventElapsedTime(&t, ctx->kev[i], ctx->kev[i + 1]);
+      if (hipEventElapsedTime(&t, ctx->kev[i], ctx->kev[i + 1]) != hipSuccess) t = 0.f;
       if (ctx->kev_pass[i / 2] == PASS_NUMERIC) {
         stats->ms_kernel_numeric += t;
         stats->kernel_launches_numeric++;
