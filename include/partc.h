@@ -64,8 +64,26 @@ typedef struct ptc_dtensor ptc_dtensor;     /* device-resident COO tensor   */
 typedef struct ptc_hashtable ptc_hashtable; /* device key table (FKS role)  */
 typedef struct ptc_mlp ptc_mlp;             /* learned row-size predictor   */
 
-/* Context: one device, one stream.  device = -1 picks the current device. */
+/* Context: one device, one stream.  device = -1 picks the current device.
+ *
+ * Lifetime rule: free every object made on a context (device tensors, key
+ * tables, batch plans, row-size models) before ptc_destroy.  ptc_destroy
+ * releases all of the context's device memory, events and its stream at
+ * once, whatever is still live; an object freed after it only releases its
+ * host record (no HIP call is made), and the context's own host record is
+ * kept until the last such object is freed.  Any other use of an object
+ * after its context was destroyed is an error.  Call ptc_destroy before the
+ * process exits (HIP's own exit handlers may already have run when static
+ * destructors or atexit handlers of the caller run). */
 int ptc_create(ptc_ctx** ctx, int device);
+/* The same with a caller stream: the context orders all its work on
+ * `stream` (a hipStream_t the caller created on that device; not NULL, since
+ * batch plans capture graphs on it) instead of creating its own, so a
+ * caller's kernels before and after a call need no extra synchronisation.
+ * The stream stays the caller's: ptc_destroy does not destroy it, and it
+ * must outlive the context.  No reference counterpart (the reference has no
+ * device work to order). */
+int ptc_create_on_stream(ptc_ctx** ctx, int device, void* stream);
 void ptc_destroy(ptc_ctx* ctx);
 const char* ptc_last_error(const ptc_ctx* ctx);
 int ptc_synchronize(ptc_ctx* ctx);
@@ -73,7 +91,9 @@ int ptc_synchronize(ptc_ctx* ctx);
 void* ptc_stream(ptc_ctx* ctx);
 
 /* Copy a host tensor to the device (the reference keeps every tensor in host
- * malloc memory; main.cpp:59-116).  Per-mode maxima are computed here. */
+ * malloc memory; main.cpp:59-116).  Per-mode maxima are computed here.  A
+ * coordinate of 2^32-1 is PTC_ERR_RANGE (key radices are max index + 1 in 32
+ * bits); the largest index is 2^32-2. */
 int ptc_dtensor_upload(ptc_ctx* ctx, const ptc_tensor_view* t, ptc_dtensor** out);
 /* The same for a tensor with FP32 values (t->vals is ignored; vals holds N
  * floats).  Key tables built on it stream 4-byte values, and contracting two
@@ -102,9 +122,12 @@ int ptc_dtensor_download(ptc_ctx* ctx, const ptc_dtensor* t, uint32_t* hedges, d
  * hi + lo carries it to about twice FP32 precision. */
 int ptc_dtensor_download_f32(ptc_ctx* ctx, const ptc_dtensor* t, float* vals, float* lo);
 /* Overwrite the FP64 values of a device tensor in place (same coordinates):
- * the input of a batch plan's replay (ptc_batch_run).  Values must keep the
- * tensor's class for the single-contraction entry points: nonzero and far
- * from overflow if they were at upload. */
+ * the input of a batch plan's replay (ptc_batch_run), which re-reads them.
+ * Key tables built before the call keep the values they grouped (build new
+ * ones for ptc_tc_fksCuckoo).  The values are rescanned as at upload, so
+ * zero, tiny or huge values make later contractions track touched output
+ * entries with flags (an exactly-zero sum stays in the output, as the
+ * reference keeps it, fksCuckooTC.hpp:318-330). */
 int ptc_dtensor_set_values(ptc_ctx* ctx, ptc_dtensor* t, const double* vals);
 void ptc_dtensor_free(ptc_dtensor* t);
 
@@ -208,7 +231,8 @@ typedef struct {
   int acc_mode;           /* ptc_acc_mode; AUTO lets the tuner decide        */
   uint32_t dense_max_cols;/* largest column space held densely in LDS (0 = default) */
   uint32_t lds_hash_max;  /* largest row nnz held in an LDS hash (0 = default) */
-  uint64_t chunk_flops;   /* target multiply-adds per workgroup (0 = default) */
+  uint64_t chunk_flops;   /* target multiply-adds per workgroup (0 = default; at most
+                             2^31, else PTC_ERR_RANGE) */
   double predict_scale;   /* learned path: capacity = scale * prediction (0 = default 1.15) */
   const ptc_mlp* mlp;     /* learned path: predictor (NULL = built-in)       */
   int deterministic;      /* reserved, must be 0 (PTC_ERR_UNSUPPORTED otherwise) */

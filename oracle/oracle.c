@@ -45,7 +45,9 @@ static double now_s(void) {
   return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
 }
 
-static uint64_t orc_rng_state = 0x1234ABCDull;
+/* per thread, so independent contractions may run on several host threads
+ * at once (orc_seed seeds the calling thread's stream) */
+static _Thread_local uint64_t orc_rng_state = 0x1234ABCDull;
 static uint64_t splitmix64(uint64_t* s) {
   uint64_t z = (*s += 0x9E3779B97F4A7C15ull);
   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;

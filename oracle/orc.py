@@ -121,6 +121,20 @@ def tc_fksCuckoo(A, B, max_rows: int = 0, seed: int = 0x1234ABCD):
     return _collect(res)
 
 
+def contract_meta(A, B, max_rows: int = 0, seed: int = 0x1234ABCD):
+    """The driver path (main.cpp:68-118): both tensors .meta-style
+    (contracted modes first); A_copy is formed free-first inside."""
+    L = lib()
+    L.orc_seed(seed)
+    ka, kb = _Keep(A), _Keep(B)
+    res = OrcResult()
+    rc = L.orc_contract_meta(C.byref(ka.s), C.byref(kb.s), max_rows, C.byref(res))
+    if rc != 0:
+        L.orc_result_free(C.byref(res))
+        raise OracleError(rc)
+    return _collect(res)
+
+
 def matricize_spGEMM(A, B):
     """Oracle of the matricized path (both contracted-first)."""
     L = lib()

@@ -180,6 +180,7 @@ def lib() -> C.CDLL:
     P32, P64, PD = C.POINTER(C.c_uint32), C.POINTER(C.c_uint64), C.POINTER(C.c_double)
     sig = {
         "ptc_create": (i32, [C.POINTER(vp), i32]),
+        "ptc_create_on_stream": (i32, [C.POINTER(vp), i32, vp]),
         "ptc_destroy": (None, [vp]),
         "ptc_last_error": (C.c_char_p, [vp]),
         "ptc_synchronize": (i32, [vp]),
@@ -375,13 +376,19 @@ atexit.register(_close_live_contexts)
 
 
 class Context:
-    """One device and one HIP stream (ptc_create)."""
+    """One device and one HIP stream (ptc_create).  stream: a caller's
+    hipStream_t (an int handle, e.g. torch.cuda.Stream().cuda_stream) that the
+    context then orders all its work on (ptc_create_on_stream); the caller
+    keeps it alive longer than the context."""
 
-    def __init__(self, device: int = -1):
+    def __init__(self, device: int = -1, stream: Optional[int] = None):
         self._h = C.c_void_p()
         # objects holding device memory of this context; freed before it
         self._children = weakref.WeakSet()
-        rc = lib().ptc_create(C.byref(self._h), device)
+        if stream is None:
+            rc = lib().ptc_create(C.byref(self._h), device)
+        else:
+            rc = lib().ptc_create_on_stream(C.byref(self._h), device, C.c_void_p(stream))
         if rc != PTC_OK:
             raise PtcError(rc, "ptc_create")
         _live_contexts.add(self)

@@ -514,11 +514,12 @@ extern "C" int ptc_batch_plan(ptc_ctx* ctx, int n, const ptc_dtensor* const* A, 
   auto w0 = std::chrono::steady_clock::now();
   ptc_batch* b = new ptc_batch();
   b->ctx = ctx;
+  ctx_acquire(ctx);
   b->n = n;
   if (opt) b->opt = *opt; else ptc_options_default(&b->opt);
   b->opt.method = PTC_METHOD_TWO_PASS;
   if (b->opt.value_type != PTC_VAL_F64) {
-    delete b;
+    ptc_batch_free(b);
     return set_err(ctx, PTC_ERR_UNSUPPORTED, "batched contractions compute in FP64");
   }
   b->A.assign(A, A + n);
@@ -578,18 +579,26 @@ extern "C" int ptc_batch_output(const ptc_batch* b, int j, const ptc_dtensor** O
 
 extern "C" void ptc_batch_free(ptc_batch* b) {
   if (!b) return;
-  if (b->ctx) (void)hipStreamSynchronize(b->ctx->stream);
-  if (b->exec) (void)hipGraphExecDestroy(b->exec);
-  if (b->graph) (void)hipGraphDestroy(b->graph);
+  ptc_ctx* ctx = b->ctx;
+  // after ptc_destroy: no HIP call (the context released its memory); the
+  // tables and outputs below release their own references the same way
+  const bool open = ctx_open(ctx);
+  if (open) {
+    (void)hipStreamSynchronize(ctx->stream);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+  }
   for (auto* o : b->out) ptc_dtensor_free(o);
   b->runs.clear();
   for (auto* h : b->HA) ptc_hashtable_free(h);
   for (auto* h : b->HB) ptc_hashtable_free(h);
   for (auto* v : b->Ac) delete v;
   delete b->ws;
-  for (auto& e : b->ev)
-    if (e) (void)hipEventDestroy(e);
+  if (open)
+    for (auto& e : b->ev)
+      if (e) (void)hipEventDestroy(e);
   delete b;
+  ctx_release(ctx);
 }
 
 // Coordinate descent over the knobs that shape a batch's row passes:

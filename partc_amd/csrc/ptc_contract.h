@@ -11,6 +11,10 @@
 namespace ptc {
 
 enum { PASS_SYMBOLIC = 0, PASS_NUMERIC = 1 };
+// Largest work item, in multiply-adds: a row pass keeps flop offsets inside
+// an item as 32-bit values (staged position starts, spans), so an item's
+// range stays below 2^32 with room for the per-wave rounding.
+constexpr unsigned long long MAX_ITEM_FLOPS = 1ull << 31;
 constexpr int PB = PTC_PASS_THREADS;  // default threads per workgroup of the row passes
 constexpr int PB_MAX = 1024;          // largest row-pass workgroup (its launch bound)
 

@@ -208,6 +208,7 @@ __global__ void k_rows(RowParams P) {
   const uint64_t live = P.live ? (*P.live >> 32) : P.nrows;
   unsigned long long chunk = P.chunk;
   if (P.total) chunk = max(chunk, (*P.total + P.max_items - 1) / P.max_items);
+  chunk = min(chunk, MAX_ITEM_FLOPS);  // the growth rule passes 2^32 above ~1.3e13 flops
   for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < P.nrows; r += (uint64_t)gridDim.x * blockDim.x) {
     if (r >= live) {  // allocated beyond the table's rows: no work, zero sums
       P.row_nchunk[r] = 0;

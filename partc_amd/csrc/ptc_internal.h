@@ -79,6 +79,7 @@ struct ptc_block {
 struct ptc_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  bool owns_stream = true;   // false: the caller's stream (ptc_create_on_stream)
   std::string err;
   hipDeviceProp_t prop{};
   // caching allocator: size class -> free blocks
@@ -92,6 +93,13 @@ struct ptc_ctx {
   // pinned host staging for small device->host reads
   void* host_pinned = nullptr;
   size_t host_pinned_bytes = 0;
+  // Lifetime: every device tensor, key table, batch plan and row-size model
+  // made on the context holds a reference.  ptc_destroy releases all device
+  // memory, events and the stream and marks the context closed; the host
+  // record itself stays until the last object is freed, so a free after
+  // ptc_destroy reads a live record, sees `closed` and makes no HIP call.
+  uint64_t refs = 0;
+  bool closed = false;
 };
 
 struct ptc_dtensor {
@@ -171,6 +179,14 @@ int hip_check(ptc_ctx* ctx, hipError_t e, const char* what);
     hipError_t _e = (call);                                \
     if (_e != hipSuccess) return ptc::hip_check(ctx, _e, #call); \
   } while (0)
+
+// Reference of an object on its context (ctx_acquire at creation,
+// ctx_release when the object is freed; the release that drops the last
+// reference of a closed context deletes it).
+void ctx_acquire(ptc_ctx* ctx);
+void ctx_release(ptc_ctx* ctx);
+// True when device work and HIP calls on the context are still allowed.
+inline bool ctx_open(const ptc_ctx* ctx) { return ctx && !ctx->closed; }
 
 void* dalloc(ptc_ctx* ctx, size_t bytes);
 void dfree(ptc_ctx* ctx, void* p);

@@ -1742,6 +1742,7 @@ extern "C" int ptc_hashtable_build_ex(ptc_ctx* ctx, const ptc_dtensor* t, int gr
   PTC_HIP(ctx, hipSetDevice(ctx->device));
   ptc_hashtable* ht = new ptc_hashtable();
   ht->ctx = ctx;
+  ctx_acquire(ctx);
   ht->t = t;
   if (hipEventCreate(&ht->ev_start) != hipSuccess || hipEventCreate(&ht->ev_end) != hipSuccess) {
     ptc_hashtable_free(ht);
@@ -1998,6 +1999,7 @@ int hashtable_build_many(ptc_ctx* ctx, int n, const ptc_dtensor* const* ts, ptc_
     ptc_hashtable* ht = new ptc_hashtable();
     out[i] = ht;
     ht->ctx = ctx;
+    ctx_acquire(ctx);
     ht->t = t;
     if ((rc = make_codec(ctx, t, t->dim_tc, t->nk, &ht->codec)) ||
         (rc = make_codec(ctx, t, t->dim_tc + t->nk, t->d - t->nk, &ht->rest)))
@@ -2133,13 +2135,14 @@ int hashtable_col16(ptc_ctx* ctx, const ptc_hashtable* hc, const uint16_t** out)
     if (!h->rest_key32 || h->rest.space > 65536)
       return set_err(ctx, PTC_ERR_INTERNAL, "16-bit keys of a %llu-key space", (unsigned long long)h->rest.space);
     const uint64_t N = h->t->N;
-    h->rest_key16 = dalloc_n<uint16_t>(ctx, std::max<uint64_t>(N, 1));
-    if (!h->rest_key16) return PTC_ERR_NOMEM;
+    uint16_t* k16 = dalloc_n<uint16_t>(ctx, std::max<uint64_t>(N, 1));
+    if (!k16) return PTC_ERR_NOMEM;
     if (N) {
       hipLaunchKernelGGL(k_narrow16, dim3(grid_for(N, 256, 16384)), dim3(256), 0, ctx->stream, N,
-                         (const uint32_t*)h->rest_key32, h->rest_key16);
+                         (const uint32_t*)h->rest_key32, k16);
       PTC_HIP(ctx, hipGetLastError());
     }
+    h->rest_key16 = k16;
   }
   *out = h->rest_key16;
   return PTC_OK;
@@ -2151,13 +2154,14 @@ int hashtable_slice_of(ptc_ctx* ctx, const ptc_hashtable* hc, const uint32_t** o
   ptc_hashtable* h = const_cast<ptc_hashtable*>(hc);
   if (!h->slice_of) {
     const uint64_t N = h->t->N;
-    h->slice_of = dalloc_n<uint32_t>(ctx, std::max<uint64_t>(N, 1));
-    if (!h->slice_of) return PTC_ERR_NOMEM;
+    uint32_t* so = dalloc_n<uint32_t>(ctx, std::max<uint64_t>(N, 1));
+    if (!so) return PTC_ERR_NOMEM;
     if (N) {
       hipLaunchKernelGGL(k_kt_slice_of, dim3(grid_for(N, 256, 16384)), dim3(256), 0, ctx->stream, N,
-                         (const KtSlice*)h->slices, (const unsigned long long*)h->d_total, h->slice_of);
+                         (const KtSlice*)h->slices, (const unsigned long long*)h->d_total, so);
       PTC_HIP(ctx, hipGetLastError());
     }
+    h->slice_of = so;
   }
   *out = h->slice_of;
   return PTC_OK;
@@ -2255,6 +2259,7 @@ int hashtable_from_rows(ptc_ctx* ctx, const ptc_dtensor* O, const KeyCodec& rowc
     if (O->dim_tc[q] >= rowc.n) return set_err(ctx, PTC_ERR_ARG, "key mode %u is not a row mode", O->dim_tc[q]);
   ptc_hashtable* ht = new ptc_hashtable();
   ht->ctx = ctx;
+  ctx_acquire(ctx);
   ht->t = O;
   ht->shares_vals = true;
   if (hipEventCreate(&ht->ev_start) != hipSuccess || hipEventCreate(&ht->ev_end) != hipSuccess) {
@@ -2375,21 +2380,24 @@ extern "C" int ptc_hashtable_probe(ptc_ctx* ctx, const ptc_hashtable* h, uint64_
 
 extern "C" void ptc_hashtable_free(ptc_hashtable* h) {
   if (!h) return;
-  if (h->ctx) {
-    dfree(h->ctx, h->entries);
-    dfree(h->ctx, h->slices);
-    dfree(h->ctx, h->perm);
-    dfree(h->ctx, h->slice_of);
-    dfree(h->ctx, h->rest_key32);
-    dfree(h->ctx, h->rest_key64);
-    dfree(h->ctx, h->rest_key16);
+  ptc_ctx* ctx = h->ctx;
+  // after ptc_destroy the context released everything: no HIP call here
+  if (ctx_open(ctx)) {
+    dfree(ctx, h->entries);
+    dfree(ctx, h->slices);
+    dfree(ctx, h->perm);
+    dfree(ctx, h->slice_of);
+    dfree(ctx, h->rest_key32);
+    dfree(ctx, h->rest_key64);
+    dfree(ctx, h->rest_key16);
     if (!h->shares_vals) {
-      dfree(h->ctx, h->gval);
-      dfree(h->ctx, h->gval32);
+      dfree(ctx, h->gval);
+      dfree(ctx, h->gval32);
     }
-    dfree(h->ctx, h->d_total);
+    dfree(ctx, h->d_total);
+    if (h->ev_start) (void)hipEventDestroy(h->ev_start);
+    if (h->ev_end) (void)hipEventDestroy(h->ev_end);
   }
-  if (h->ev_start) (void)hipEventDestroy(h->ev_start);
-  if (h->ev_end) (void)hipEventDestroy(h->ev_end);
   delete h;
+  ctx_release(ctx);
 }

@@ -313,6 +313,7 @@ extern "C" int ptc_mlp_create(ptc_ctx* ctx, uint64_t seed, ptc_mlp** out) {
   w[OFF_B3] = 2.0f;  // start near "dense": sigmoid(2) = 0.88
   ptc_mlp* m = new ptc_mlp();
   m->ctx = ctx;
+  ctx_acquire(ctx);
   m->w = dalloc_n<float>(ctx, NPARAM);
   m->m = dalloc_n<float>(ctx, NPARAM);
   m->v = dalloc_n<float>(ctx, NPARAM);
@@ -335,12 +336,14 @@ extern "C" int ptc_mlp_create(ptc_ctx* ctx, uint64_t seed, ptc_mlp** out) {
 extern "C" void ptc_mlp_free(ptc_mlp* m) {
   if (!m) return;
   if (g_default == m) g_default = nullptr;
-  if (m->ctx) {
-    dfree(m->ctx, m->w);
-    dfree(m->ctx, m->m);
-    dfree(m->ctx, m->v);
+  ptc_ctx* ctx = m->ctx;
+  if (ctx_open(ctx)) {
+    dfree(ctx, m->w);
+    dfree(ctx, m->m);
+    dfree(ctx, m->v);
   }
   delete m;
+  ctx_release(ctx);
 }
 
 extern "C" int ptc_mlp_num_params(void) { return NPARAM; }

@@ -75,8 +75,19 @@ void* dalloc(ptc_ctx* ctx, size_t bytes) {
   return p;
 }
 
+void ctx_acquire(ptc_ctx* ctx) {
+  if (ctx) ++ctx->refs;
+}
+
+void ctx_release(ptc_ctx* ctx) {
+  if (!ctx) return;
+  if (ctx->refs) --ctx->refs;
+  if (ctx->closed && ctx->refs == 0) delete ctx;
+}
+
 void dfree(ptc_ctx* ctx, void* p) {
-  if (!p || !ctx) return;
+  // a closed context has already released every block
+  if (!p || !ctx_open(ctx)) return;
   auto it = ctx->live.find(p);
   if (it == ctx->live.end()) return;
   size_t sz = it->second;
@@ -360,7 +371,17 @@ using namespace ptc;
 
 // ================================================================ C ABI
 
-extern "C" int ptc_create(ptc_ctx** out, int device) {
+static int create_ctx(ptc_ctx** out, int device, hipStream_t caller_stream);
+
+extern "C" int ptc_create(ptc_ctx** out, int device) { return create_ctx(out, device, nullptr); }
+
+extern "C" int ptc_create_on_stream(ptc_ctx** out, int device, void* stream) {
+  if (out) *out = nullptr;
+  if (!stream) return PTC_ERR_ARG;
+  return create_ctx(out, device, (hipStream_t)stream);
+}
+
+static int create_ctx(ptc_ctx** out, int device, hipStream_t caller_stream) {
   if (!out) return PTC_ERR_ARG;
   *out = nullptr;
   int n = 0;
@@ -374,8 +395,21 @@ extern "C" int ptc_create(ptc_ctx** out, int device) {
   if (device >= n) return PTC_ERR_ARG;
   ptc_ctx* ctx = new ptc_ctx();
   ctx->device = device;
-  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+  if (caller_stream) {
+    ctx->stream = caller_stream;
+    ctx->owns_stream = false;
+    int sdev = -1;
+    if (hipStreamGetDevice(caller_stream, &sdev) != hipSuccess || sdev != device) {
+      (void)hipGetLastError();
+      delete ctx;
+      return PTC_ERR_ARG;
+    }
+  }
+  if (hipSetDevice(device) != hipSuccess ||
+      (!caller_stream && hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) ||
       hipGetDeviceProperties(&ctx->prop, device) != hipSuccess) {
+    (void)hipGetLastError();
+    if (ctx->owns_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return PTC_ERR_HIP;
   }
@@ -387,25 +421,38 @@ extern "C" int ptc_create(ptc_ctx** out, int device) {
     ptc_destroy(ctx);
     return PTC_ERR_HIP;
   }
+
   *out = ctx;
   return PTC_OK;
 }
 
+// Releases the context's device memory, events and stream at once.  Objects
+// still holding a reference (tensors, tables, plans, models not freed yet)
+// keep the host record alive; their later frees make no HIP call, and the
+// last of them deletes the record (ctx_release).
 extern "C" void ptc_destroy(ptc_ctx* ctx) {
-  if (!ctx) return;
+  if (!ctx || ctx->closed) return;
   // teardown: nothing can be done about a failing release, so results are
   // not checked
   (void)hipSetDevice(ctx->device);
-  (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   for (auto& kv : ctx->free_blocks) (void)hipFree(kv.second);
   for (auto& kv : ctx->live) (void)hipFree(kv.first);
+  ctx->free_blocks.clear();
+  ctx->live.clear();
+  ctx->cached_bytes = 0;
   for (auto& e : ctx->ev)
     if (e) (void)hipEventDestroy(e);
   for (auto& e : ctx->kev)
     if (e) (void)hipEventDestroy(e);
   if (ctx->host_pinned) (void)hipHostFree(ctx->host_pinned);
-  (void)hipStreamDestroy(ctx->stream);
-  delete ctx;
+  ctx->host_pinned = nullptr;
+  ctx->host_pinned_bytes = 0;
+  if (ctx->stream && ctx->owns_stream) (void)hipStreamDestroy(ctx->stream);
+  ctx->stream = nullptr;
+  (void)hipGetLastError();
+  ctx->closed = true;
+  if (ctx->refs == 0) delete ctx;
 }
 
 extern "C" const char* ptc_last_error(const ptc_ctx* ctx) { return ctx ? ctx->err.c_str() : ""; }
@@ -431,6 +478,25 @@ static int check_perm(ptc_ctx* ctx, uint32_t d, uint32_t nk, const uint32_t* dim
   return PTC_OK;
 }
 
+// No product of two such values can be +-0, subnormal or infinite, so an
+// untouched dense accumulator entry keeps its -0.0 initial value.
+static bool values_safe(const double* v64, const float* v32, uint64_t n) {
+  for (uint64_t i = 0; i < n; ++i) {
+    const double a = v64 ? std::fabs(v64[i]) : std::fabs((double)v32[i]);
+    const bool ok = v64 ? (a >= 0x1p-500 && a <= 0x1p500) : (a >= 0x1p-60 && a <= 0x1p60);
+    if (!ok) return false;
+  }
+  return true;
+}
+
+// A coordinate of 2^32-1 in some mode: its key radix (max + 1) would not fit
+// 32 bits.
+static bool max_index_ok(const uint32_t* mode_max, uint32_t d) {
+  for (uint32_t j = 0; j < d; ++j)
+    if (mode_max[j] == 0xFFFFFFFFu) return false;
+  return true;
+}
+
 // Upload of an FP64 (v64) or FP32 (v32) tensor.  vals_safe: every |value| is
 // far enough from 0 and from overflow that no product is 0, subnormal or
 // infinite (then untouched dense accumulator entries can be recognised by
@@ -445,6 +511,7 @@ static int upload(ptc_ctx* ctx, const ptc_tensor_view* v, const double* v64, con
   PTC_HIP(ctx, hipSetDevice(ctx->device));
   ptc_dtensor* t = new ptc_dtensor();
   t->ctx = ctx;
+  ctx_acquire(ctx);
   t->d = v->d;
   t->nk = v->num_key_dimensions;
   t->N = v->N;
@@ -454,15 +521,11 @@ static int upload(ptc_ctx* ctx, const ptc_tensor_view* v, const double* v64, con
   }
   for (uint64_t i = 0; i < v->N; ++i)
     for (uint32_t j = 0; j < v->d; ++j) t->mode_max[j] = std::max(t->mode_max[j], v->hedges[i * v->d + j]);
-  t->vals_safe = true;
-  for (uint64_t i = 0; i < v->N; ++i) {
-    const double a = v64 ? std::fabs(v64[i]) : std::fabs((double)v32[i]);
-    const bool ok = v64 ? (a >= 0x1p-500 && a <= 0x1p500) : (a >= 0x1p-60 && a <= 0x1p60);
-    if (!ok) {
-      t->vals_safe = false;
-      break;
-    }
+  if (!max_index_ok(t->mode_max, t->d)) {
+    ptc_dtensor_free(t);
+    return set_err(ctx, PTC_ERR_RANGE, "coordinate 2^32-1 (the largest index is 2^32-2)");
   }
+  t->vals_safe = values_safe(v64, v32, v->N);
   if (!v->dims)
     for (uint32_t j = 0; j < v->d; ++j) t->dims[j] = v->N ? t->mode_max[j] + 1 : 0;
   t->hedges = dalloc_n<uint32_t>(ctx, v->N * v->d);
@@ -508,6 +571,7 @@ extern "C" int ptc_dtensor_wrap(ptc_ctx* ctx, uint32_t d, uint32_t nk, uint64_t 
   if (N >= 0xFFFFFFFFull) return set_err(ctx, PTC_ERR_RANGE, "N exceeds 2^32-2");
   ptc_dtensor* t = new ptc_dtensor();
   t->ctx = ctx;
+  ctx_acquire(ctx);
   t->d = d;
   t->nk = nk;
   t->N = N;
@@ -521,7 +585,7 @@ extern "C" int ptc_dtensor_wrap(ptc_ctx* ctx, uint32_t d, uint32_t nk, uint64_t 
     Workspace ws(ctx);
     unsigned int* mx = ws.get<unsigned int>(PTC_MAX_MODES);
     if (!mx) {
-      delete t;
+      ptc_dtensor_free(t);
       return PTC_ERR_NOMEM;
     }
     rc = fill_zero(ctx, mx, sizeof(unsigned int) * PTC_MAX_MODES);
@@ -531,9 +595,13 @@ extern "C" int ptc_dtensor_wrap(ptc_ctx* ctx, uint32_t d, uint32_t nk, uint64_t 
       rc = read_back(ctx, t->mode_max, mx, sizeof(uint32_t) * d);
     }
     if (rc) {
-      delete t;
+      ptc_dtensor_free(t);
       return rc;
     }
+  }
+  if (N && !max_index_ok(t->mode_max, d)) {
+    ptc_dtensor_free(t);
+    return set_err(ctx, PTC_ERR_RANGE, "coordinate 2^32-1 (the largest index is 2^32-2)");
   }
   for (uint32_t j = 0; j < d; ++j) t->dims[j] = dims ? dims[j] : (N ? t->mode_max[j] + 1 : 0);
   *out = t;
@@ -618,18 +686,22 @@ extern "C" int ptc_dtensor_set_values(ptc_ctx* ctx, ptc_dtensor* t, const double
     PTC_HIP(ctx, hipMemcpyAsync(t->vals, vals, sizeof(double) * t->N, hipMemcpyHostToDevice, ctx->stream));
     PTC_HIP(ctx, hipStreamSynchronize(ctx->stream));
   }
+  // zero, tiny or huge values make later contractions keep touch flags
+  t->vals_safe = values_safe(vals, nullptr, t->N);
   return PTC_OK;
 }
 
 extern "C" void ptc_dtensor_free(ptc_dtensor* t) {
   if (!t) return;
-  if (t->owns && t->ctx) {
-    dfree(t->ctx, t->hedges);
-    dfree(t->ctx, t->vals);
-    dfree(t->ctx, t->vals32);
-    dfree(t->ctx, t->err32);
+  ptc_ctx* ctx = t->ctx;
+  if (t->owns && ctx_open(ctx)) {
+    dfree(ctx, t->hedges);
+    dfree(ctx, t->vals);
+    dfree(ctx, t->vals32);
+    dfree(ctx, t->err32);
   }
   delete t;
+  ctx_release(ctx);
 }
 
 // --------------------------------------------------------------- .meta

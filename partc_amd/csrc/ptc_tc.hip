@@ -31,13 +31,18 @@ struct Timer {
   explicit Timer(ptc_ctx* c) : ctx(c) {
     for (int i = 0; i < 8; ++i) ev[i] = c->ev[i];
   }
+  // Timing only: a failing event call is ignored, and its error is cleared
+  // so the next launch check does not report it.
   void mark() {
-    if (n < 8) (void)hipEventRecord(ev[n++], ctx->stream);  // timing only
+    if (n < 8 && hipEventRecord(ev[n++], ctx->stream) != hipSuccess) (void)hipGetLastError();
   }
   double ms(int a, int b) {
     if (a >= n || b >= n) return 0.0;
     float t = 0.f;
-    if (hipEventElapsedTime(&t, ev[a], ev[b]) != hipSuccess) t = 0.f;
+    if (hipEventElapsedTime(&t, ev[a], ev[b]) != hipSuccess) {
+      (void)hipGetLastError();
+      t = 0.f;
+    }
     return t;
   }
 };
@@ -195,7 +200,7 @@ static int run_pass(Run& R, int pass, HashPlan* hp, const uint4* items, uint64_t
   // one pair of timing events brackets the row launch and its merge launch
   ptc_ctx* ctx = R.ctx;
   const bool timed = ctx->nkev + 2 <= 16;
-  if (timed) (void)hipEventRecord(ctx->kev[ctx->nkev], ctx->stream);
+  if (timed && hipEventRecord(ctx->kev[ctx->nkev], ctx->stream) != hipSuccess) (void)hipGetLastError();
   int rc = launch_pass(ctx, pass, R.dense, R.col64, P, nitems, lds, false, R.threads);
   if (!rc && R.dense) {
     if (!merge_rows) {
@@ -210,7 +215,7 @@ static int run_pass(Run& R, int pass, HashPlan* hp, const uint4* items, uint64_t
   }
   if (rc) return rc;
   if (timed) {
-    (void)hipEventRecord(ctx->kev[ctx->nkev + 1], ctx->stream);
+    if (hipEventRecord(ctx->kev[ctx->nkev + 1], ctx->stream) != hipSuccess) (void)hipGetLastError();
     ctx->kev_pass[ctx->nkev / 2] = pass;
     ctx->nkev += 2;
   }
@@ -236,6 +241,7 @@ int prepare_heavy(Run& R, int pass) {
 int make_output(Run& R, uint64_t nnz, const ptc_dtensor* A, const ptc_dtensor* B, ptc_dtensor** O) {
   ptc_dtensor* o = new ptc_dtensor();
   o->ctx = R.ctx;
+  ctx_acquire(R.ctx);
   o->d = R.dO;
   o->nk = R.dO;  // tensor.hpp:124
   o->N = nnz;
@@ -286,6 +292,9 @@ int prepare_launch(Run& R, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B,
   if (c == 0 || B->nk != c) return set_err(ctx, PTC_ERR_ARG, "A has %u contracted modes, B %u key modes", c, B->nk);
   if (HT_A->rest.n != c || HT_B->codec.n != c) return set_err(ctx, PTC_ERR_INTERNAL, "codec shape mismatch");
   if (opt.deterministic) return set_err(ctx, PTC_ERR_UNSUPPORTED, "the deterministic (reference-order) mode is not available");
+  if (opt.chunk_flops > MAX_ITEM_FLOPS)
+    return set_err(ctx, PTC_ERR_RANGE, "chunk_flops %llu exceeds 2^31 multiply-adds per work item",
+                   (unsigned long long)opt.chunk_flops);
   {
     const bool a32 = HT_A->gval32 != nullptr, b32 = HT_B->gval32 != nullptr;
     if (a32 != b32) return set_err(ctx, PTC_ERR_ARG, "A and B must both hold FP64 or both FP32 values");
@@ -786,7 +795,7 @@ int tc_impl(ptc_ctx* ctx, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B, 
     if (kr->from_rows) *kr->from_rows = rows_key ? 1 : 0;
   }
   if (stats) {
-    (void)hipEventSynchronize(T.ev[T.n - 1]);
+    if (hipEventSynchronize(T.ev[T.n - 1]) != hipSuccess) (void)hipGetLastError();
     memset(stats, 0, sizeof(*stats));
     if (hto) stats->ms_build_out = hashtable_ms_build(hto);
     stats->nnz_a = A->N;
@@ -809,7 +818,10 @@ int tc_impl(ptc_ctx* ctx, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B, 
     // pass time = its row-pass launch + its merge launch
     for (int i = 0; i + 1 < ctx->nkev; i += 2) {
       float t = 0.f;
-      if (hipEventElapsedTime(&t, ctx->kev[i], ctx->kev[i + 1]) != hipSuccess) t = 0.f;
+      if (hipEventElapsedTime(&t, ctx->kev[i], ctx->kev[i + 1]) != hipSuccess) {
+        (void)hipGetLastError();
+        t = 0.f;
+      }
       if (ctx->kev_pass[i / 2] == PASS_NUMERIC) {
         stats->ms_kernel_numeric += t;
         stats->kernel_launches_numeric++;

@@ -40,6 +40,39 @@ def test_oracle_matches_golden(name):
 
 
 @pytest.mark.parametrize("name", sorted(CASES))
+def test_oracle_driver_path_matches_golden(name):
+    """orc_contract_meta (main.cpp:68-118: A_copy made free-first from the
+    .meta tensor) gives the golden product, in the same emission order as
+    tc_fksCuckoo on the caller-made A_copy."""
+    case = CASES[name]
+    A, B = meta_tensors(case)
+    h, v, info = orc.contract_meta(A, B)
+    hs, vs = sort_coo(h, v)
+    assert np.array_equal(hs, case["coords"])
+    assert np.array_equal(vs, case["vals_ordered"])
+    assert info["flops"] == int(case["flops"][0])
+    h2, v2, _ = orc.tc_fksCuckoo(*tc_inputs(case))
+    assert np.array_equal(h, h2) and np.array_equal(v, v2)
+
+
+def test_oracle_threads_agree_with_serial():
+    """Oracle contractions on several host threads at once (the RNG stream of
+    the hash keys is per thread) give what they give one at a time."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    ins = []
+    for i in range(6):
+        h, v = P.gen_coo(70 + i, [80, 60, 50], 6000, P.DIST_ZIPF)
+        ins.append(P.self_contraction_inputs(h, v, [1, 2]))
+    serial = [sort_coo(*orc.tc_fksCuckoo(A, B)[:2]) for A, B in ins]
+    with ThreadPoolExecutor(6) as ex:
+        par = list(ex.map(lambda ab: sort_coo(*orc.tc_fksCuckoo(*ab)[:2]), ins))
+    for (h1, v1), (h2, v2) in zip(serial, par):
+        assert np.array_equal(h1, h2)
+        assert np.array_equal(v1, v2)
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
 def test_oracle_matricized_matches_golden(name):
     case = CASES[name]
     A, B = meta_tensors(case)
