@@ -288,6 +288,24 @@ int ptc_tc_fksCuckoo_keyed(ptc_ctx* ctx, const ptc_hashtable* HT_A, const ptc_ha
                            uint32_t nk_next, const uint32_t* dim_tc_next, ptc_dtensor** O,
                            ptc_hashtable** HT_O, int* from_rows, ptc_stats* stats);
 
+/* ptc_tc_fksCuckoo that allocates nothing: no counterpart in the reference;
+ * for callers that cannot allocate device memory (serving loops and
+ * harnesses that forbid hipMalloc).  The exact two-pass method in FP64 only
+ * (opt may be NULL; any other method or value type is PTC_ERR_ARG).  The
+ * output goes into caller-owned device arrays: out_hedges (out_cap x d_O
+ * uint32) and out_vals (out_cap doubles); *nnz_out receives the output's
+ * nonzeros, also when out_cap is too small (then PTC_ERR_RANGE and nothing
+ * is written: call again with larger arrays).  Scratch comes from one
+ * 64 MiB device array per device, held for the life of the process; a
+ * contraction that needs more is PTC_ERR_RANGE (use ptc_tc_fksCuckoo).
+ * Every call on a device shares that array: calls that overlap on one
+ * device (several host threads or contexts with work in flight at once)
+ * race.  The key tables are inputs, built beforehand. */
+int ptc_tc_fksCuckoo_noalloc(ptc_ctx* ctx, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B,
+                             const ptc_dtensor* A, const ptc_dtensor* B, const ptc_options* opt,
+                             uint32_t* out_hedges, double* out_vals, uint64_t out_cap, uint64_t* nnz_out,
+                             ptc_stats* stats);
+
 /* Convenience: both tensors carry .meta-style dim_tc (contracted modes
  * first, num_key_dimensions = c, as main.cpp:65 leaves A).  Builds both
  * tables and contracts, as main.cpp:68-118 does.  Replaces the driver path

@@ -205,6 +205,8 @@ def lib() -> C.CDLL:
         "ptc_tc_fksCuckoo_keyed": (i32, [vp, vp, vp, vp, vp, C.POINTER(Options), u32, P32, C.POINTER(vp),
                                          C.POINTER(vp), C.POINTER(i32), C.POINTER(Stats)]),
         "ptc_contract": (i32, [vp, vp, vp, C.POINTER(Options), C.POINTER(vp), C.POINTER(Stats)]),
+        "ptc_tc_fksCuckoo_noalloc": (i32, [vp, vp, vp, vp, vp, C.POINTER(Options), vp, vp, u64, P64,
+                                           C.POINTER(Stats)]),
         "ptc_matricize_spGEMM": (i32, [vp, vp, vp, C.POINTER(vp), C.POINTER(Stats)]),
         "ptc_matricize_sort_spGEMM": (i32, [vp, vp, vp, C.POINTER(vp), C.POINTER(Stats)]),
         "ptc_gen_coo": (i32, [u64, u32, P32, u64, i32, C.c_double, P32, PD]),

@@ -100,6 +100,15 @@ struct ptc_ctx {
   // ptc_destroy reads a live record, sees `closed` and makes no HIP call.
   uint64_t refs = 0;
   bool closed = false;
+  // Device address of the allocation-free entry point's scratch array on
+  // this device (set at creation), and the arena that entry point opens over
+  // it for one call: while `arena` is set, dalloc carves 256-byte aligned
+  // blocks from it with a bump pointer (nothing is freed, nothing is
+  // allocated) and fails past arena_cap.
+  void* scratch = nullptr;
+  unsigned char* arena = nullptr;
+  size_t arena_cap = 0, arena_used = 0;
+  bool arena_overflow = false;
 };
 
 struct ptc_dtensor {
@@ -206,10 +215,14 @@ struct Workspace {
   template <class T>
   T* get(size_t n) {
     T* p = dalloc_n<T>(ctx, n);
-    if (p) blocks.push_back(p);
+    if (p && !ctx->arena) blocks.push_back(p);  // arena blocks are not freed
     return p;
   }
 };
+
+// Scratch of the allocation-free entry point: its device address on the
+// context's device (ptc_create calls this with the device current).
+int scratch_setup(ptc_ctx* ctx);
 
 // Copy a small device value to the host through pinned memory, synchronously
 // on the context stream.

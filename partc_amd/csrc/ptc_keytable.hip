@@ -2142,6 +2142,11 @@ int hashtable_col16(ptc_ctx* ctx, const ptc_hashtable* hc, const uint16_t** out)
                          (const uint32_t*)h->rest_key32, k16);
       PTC_HIP(ctx, hipGetLastError());
     }
+    // a block of the allocation-free scratch lives for one call: not kept
+    if (ctx->arena) {
+      *out = k16;
+      return PTC_OK;
+    }
     h->rest_key16 = k16;
   }
   *out = h->rest_key16;
@@ -2160,6 +2165,10 @@ int hashtable_slice_of(ptc_ctx* ctx, const ptc_hashtable* hc, const uint32_t** o
       hipLaunchKernelGGL(k_kt_slice_of, dim3(grid_for(N, 256, 16384)), dim3(256), 0, ctx->stream, N,
                          (const KtSlice*)h->slices, (const unsigned long long*)h->d_total, so);
       PTC_HIP(ctx, hipGetLastError());
+    }
+    if (ctx->arena) {  // scratch block of one allocation-free call: not kept
+      *out = so;
+      return PTC_OK;
     }
     h->slice_of = so;
   }

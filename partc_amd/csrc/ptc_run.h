@@ -88,6 +88,19 @@ int prepare_heavy(Run& R, int pass);
 int make_output(Run& R, uint64_t nnz, const ptc_dtensor* A, const ptc_dtensor* B, ptc_dtensor** O);
 ptc_dtensor* view_with_keys(const ptc_dtensor* t, uint32_t nk, const uint32_t* dim_tc);
 
+// Output arrays the caller owns (cap entries of d_O coordinates and a value):
+// the exact two-pass contraction writes into them instead of allocating an
+// output.  *nnz receives the output's size, also when cap is too small
+// (PTC_ERR_RANGE, nothing written).
+struct OutInto {
+  uint32_t* hedges;
+  double* vals;
+  uint64_t cap;
+  uint64_t* nnz;
+};
+int tc_into(ptc_ctx* ctx, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B, const ptc_dtensor* A,
+            const ptc_dtensor* B, const ptc_options* opt, const OutInto& into, ptc_stats* stats);
+
 }  // namespace ptc
 
 #endif

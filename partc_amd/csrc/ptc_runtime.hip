@@ -44,6 +44,20 @@ static size_t size_class(size_t bytes) {
 }
 
 void* dalloc(ptc_ctx* ctx, size_t bytes) {
+  if (ctx->arena) {
+    // allocation-free mode: a block of the call's scratch arena
+    // blocks keep the allocator's size classes, so a kernel sees the same
+    // padding behind an array in both modes
+    const size_t off = (ctx->arena_used + 255) & ~size_t(255);
+    const size_t need = size_class(bytes);
+    if (off > ctx->arena_cap || need > ctx->arena_cap - off) {
+      ctx->arena_overflow = true;
+      set_err(ctx, PTC_ERR_RANGE, "scratch of %zu bytes exceeded", ctx->arena_cap);
+      return nullptr;
+    }
+    ctx->arena_used = off + need;
+    return ctx->arena + off;
+  }
   size_t sz = size_class(bytes);
   auto it = ctx->free_blocks.lower_bound(sz);
   // reuse a cached block of the same class or at most 2x larger
@@ -100,6 +114,7 @@ void dfree(ptc_ctx* ctx, void* p) {
 
 int read_back(ptc_ctx* ctx, void* host, const void* dev, size_t bytes) {
   if (bytes > ctx->host_pinned_bytes) {
+    if (ctx->arena) return set_err(ctx, PTC_ERR_INTERNAL, "read-back of %zu bytes in allocation-free mode", bytes);
     if (ctx->host_pinned) (void)hipHostFree(ctx->host_pinned);
     ctx->host_pinned = nullptr;
     size_t want = std::max<size_t>(bytes, 4096);
@@ -421,7 +436,20 @@ static int create_ctx(ptc_ctx** out, int device, hipStream_t caller_stream) {
     ptc_destroy(ctx);
     return PTC_ERR_HIP;
   }
-
+  // pinned staging for read-backs, made now so that small reads never
+  // allocate (the allocation-free entry point reads at most 32 bytes at once)
+  ctx->host_pinned_bytes = 4096;
+  if (hipHostMalloc(&ctx->host_pinned, ctx->host_pinned_bytes, hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
+    ctx->host_pinned = nullptr;
+    ctx->host_pinned_bytes = 0;
+    ptc_destroy(ctx);
+    return PTC_ERR_NOMEM;
+  }
+  if (scratch_setup(ctx) != PTC_OK) {
+    ptc_destroy(ctx);
+    return PTC_ERR_HIP;
+  }
   *out = ctx;
   return PTC_OK;
 }

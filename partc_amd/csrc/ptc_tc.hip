@@ -520,9 +520,23 @@ struct KeyReq {
 };
 
 int tc_impl(ptc_ctx* ctx, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B, const ptc_dtensor* A,
-            const ptc_dtensor* B, const ptc_options* opt_in, ptc_dtensor** O, ptc_stats* stats, const KeyReq* kr);
+            const ptc_dtensor* B, const ptc_options* opt_in, ptc_dtensor** O, ptc_stats* stats, const KeyReq* kr,
+            const OutInto* into = nullptr);
 
 }  // namespace
+
+namespace ptc {
+int tc_into(ptc_ctx* ctx, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B, const ptc_dtensor* A,
+            const ptc_dtensor* B, const ptc_options* opt, const OutInto& into, ptc_stats* stats) {
+  if (!ctx || !HT_A || !HT_B || !A || !B || !into.nnz || (into.cap && (!into.hedges || !into.vals)))
+    return PTC_ERR_ARG;
+  ptc_options o;
+  if (opt) o = *opt; else ptc_options_default(&o);
+  if (o.method != PTC_METHOD_TWO_PASS || o.value_type != PTC_VAL_F64)
+    return set_err(ctx, PTC_ERR_ARG, "caller-provided outputs take the exact two-pass method in FP64");
+  return tc_impl(ctx, HT_A, HT_B, A, B, &o, nullptr, stats, nullptr, &into);
+}
+}  // namespace ptc
 
 extern "C" int ptc_tc_fksCuckoo(ptc_ctx* ctx, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B,
                                 const ptc_dtensor* A, const ptc_dtensor* B, const ptc_options* opt_in,
@@ -557,9 +571,16 @@ extern "C" int ptc_tc_fksCuckoo_keyed(ptc_ctx* ctx, const ptc_hashtable* HT_A, c
 namespace {
 
 int tc_impl(ptc_ctx* ctx, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B, const ptc_dtensor* A,
-            const ptc_dtensor* B, const ptc_options* opt_in, ptc_dtensor** O, ptc_stats* stats, const KeyReq* kr) {
-  if (!ctx || !HT_A || !HT_B || !A || !B || !O) return PTC_ERR_ARG;
-  *O = nullptr;
+            const ptc_dtensor* B, const ptc_options* opt_in, ptc_dtensor** O, ptc_stats* stats, const KeyReq* kr,
+            const OutInto* into) {
+  if (!ctx || !HT_A || !HT_B || !A || !B || !(O || into)) return PTC_ERR_ARG;
+  if (O) *O = nullptr;
+  // caller-provided output arrays (exact two-pass only): a non-owning view
+  ptc_dtensor ext;
+  ext.owns = false;
+  auto drop = [&](ptc_dtensor* o) {
+    if (o != &ext) ptc_dtensor_free(o);
+  };
   ptc_options opt;
   if (opt_in) opt = *opt_in; else ptc_options_default(&opt);
   Run R(ctx);
@@ -593,16 +614,29 @@ int tc_impl(ptc_ctx* ctx, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B, 
     if ((rc = scan_u32_to_u64(ctx, R.row_nnz, out_off, R.nrows, out_off + R.nrows))) return rc;
     if ((rc = read_back(ctx, &hostv[0], out_off + R.nrows, 8))) return rc;
     nnz = hostv[0];
-    if ((rc = make_output(R, nnz, A, B, &out))) return rc;
+    if (into) {
+      *into->nnz = nnz;  // also when the caller's arrays are too small
+      if (nnz > into->cap)
+        return set_err(ctx, PTC_ERR_RANGE, "the output has %llu nonzeros; the arrays hold %llu",
+                       (unsigned long long)nnz, (unsigned long long)into->cap);
+      ext.ctx = ctx;
+      ext.d = R.dO;
+      ext.N = nnz;
+      ext.hedges = into->hedges;
+      ext.vals = into->vals;
+      out = &ext;
+    } else if ((rc = make_output(R, nnz, A, B, &out))) {
+      return rc;
+    }
     // ---------------- numeric
     HashPlan hn;
     if (!R.dense && (rc = plan_hash(R, hn, R.row_nnz, nullptr, nullptr))) {
-      ptc_dtensor_free(out);
+      drop(out);
       return rc;
     }
     if ((rc = prepare_heavy(R, PASS_NUMERIC)) ||
         (!R.dense && (rc = reset_hash_pool(R, hn, true)))) {
-      ptc_dtensor_free(out);
+      drop(out);
       return rc;
     }
     base_params(R, P, R.dense ? nullptr : &hn);
@@ -612,7 +646,7 @@ int tc_impl(ptc_ctx* ctx, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B, 
     P.out_vals32 = out->vals32;
     P.out_err32 = out->err32;
     if ((rc = run_pass(R, PASS_NUMERIC, &hn, R.items, R.nitems, P))) {
-      ptc_dtensor_free(out);
+      drop(out);
       return rc;
     }
     T.mark();  // 3
@@ -772,7 +806,7 @@ int tc_impl(ptc_ctx* ctx, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B, 
     if (!rc && (ovf & 0x80000000u)) rc = set_err(ctx, PTC_ERR_INTERNAL, "row count mismatch between passes");
     if (!rc && (ovf & 0x40000000u)) rc = set_err(ctx, PTC_ERR_INTERNAL, "slice index out of range");
     if (rc) {
-      ptc_dtensor_free(out);
+      drop(out);
       return rc;
     }
   }
@@ -830,7 +864,7 @@ int tc_impl(ptc_ctx* ctx, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B, 
       }
     }
   }
-  *O = out;
+  if (O) *O = out;
   return PTC_OK;
 }
 
