@@ -317,7 +317,13 @@ int ptc_contract(ptc_ctx* ctx, const ptc_dtensor* A, const ptc_dtensor* B,
 /* matricize_spGEMM (matricizedspGEMM.hpp:172) and matricize_sort_spGEMM
  * (matricizedspGEMM_sorting.hpp:194; the reference exits before computing
  * it, :754): both tensors contracted-first; same result as ptc_contract.
- * The sorting variant groups both operands by radix sort (PTC_GROUP_SORT). */
+ * ptc_matricize_spGEMM is a second method: the column-wise product of
+ * cs_multiply (cs_multiply.c:6-38) -- B grouped by its free modes (the
+ * output columns), A by its contracted modes, each output column
+ * accumulated over the output's rows -- so the output comes column by
+ * column; stats->rows counts nonempty output columns.  The sorting variant
+ * is the row-wise method with both operands grouped by radix sort
+ * (PTC_GROUP_SORT). */
 int ptc_matricize_spGEMM(ptc_ctx* ctx, const ptc_dtensor* A, const ptc_dtensor* B,
                          ptc_dtensor** O, ptc_stats* stats);
 int ptc_matricize_sort_spGEMM(ptc_ctx* ctx, const ptc_dtensor* A, const ptc_dtensor* B,

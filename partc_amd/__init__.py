@@ -730,13 +730,18 @@ def contract_batched(As, Bs, ctx: Optional[Context] = None, *, download: bool = 
     return [r.download() for r in res] if download else res
 
 
-def matricize_spGEMM(A, B, ctx: Optional[Context] = None) -> Tensor:
-    """matricizedspGEMM.hpp:172 -- both tensors contracted-first."""
+def matricize_spGEMM(A, B, ctx: Optional[Context] = None, *, download: bool = True,
+                     stats: Optional[Stats] = None):
+    """matricizedspGEMM.hpp:172 -- both tensors contracted-first.  The
+    column-wise product of cs_multiply (cs_multiply.c:6-38): the output comes
+    column by column (B's free tuples), each column's entries over the rows."""
     ctx = ctx or default_context()
     dA, dB = _as_device(A, ctx), _as_device(B, ctx)
     out = C.c_void_p()
-    _check(lib().ptc_matricize_spGEMM(ctx.handle, dA._h, dB._h, C.byref(out), None), ctx)
-    return DeviceTensor(ctx, out).download()
+    st = stats if stats is not None else Stats()
+    _check(lib().ptc_matricize_spGEMM(ctx.handle, dA._h, dB._h, C.byref(out), C.byref(st)), ctx)
+    dO = DeviceTensor(ctx, out)
+    return dO.download() if download else dO
 
 
 def matricize_sort_spGEMM(A, B, ctx: Optional[Context] = None) -> Tensor:

@@ -145,6 +145,7 @@ struct CapParams {
   PTR(float) out_err32;                                                                       \
   int f32; /* FP32 values in, (hi, lo) FP32 pairs out */                                      \
   int val; /* ptc_value_type: accumulation mode of the numeric pass */                        \
+  int col_first; /* 1: an output entry lists its column modes, then its row modes */          \
   uint32_t dO, nrowmodes;                                                                     \
   KeyCodec rowc, colc;
 

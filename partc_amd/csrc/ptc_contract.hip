@@ -329,15 +329,17 @@ __device__ __forceinline__ void emit_entry(const PP& P, uint64_t pos, const uint
     return;
   }
   uint32_t* o = P.out_hedges + pos * P.dO;
-  for (uint32_t j = 0; j < P.nrowmodes; ++j) o[j] = s_rowc[j];
-  o += P.nrowmodes;
+  // row modes then column modes, or (column-wise products) the reverse
+  uint32_t* orow = P.col_first ? o + P.colc.n : o;
+  uint32_t* ocol = P.col_first ? o : o + P.nrowmodes;
+  for (uint32_t j = 0; j < P.nrowmodes; ++j) orow[j] = s_rowc[j];
   if (P.colc.n == 1) {
-    o[0] = (uint32_t)col;
+    ocol[0] = (uint32_t)col;
   } else {
     // constant indices keep the codec in kernel-argument registers
 #pragma unroll
     for (uint32_t j = 0; j < PTC_MAX_MODES; ++j)
-      if (j < P.colc.n) o[j] = (uint32_t)(((unsigned long long)col / P.colc.stride[j]) % P.colc.radix[j]);
+      if (j < P.colc.n) ocol[j] = (uint32_t)(((unsigned long long)col / P.colc.stride[j]) % P.colc.radix[j]);
   }
   if (F32) {
     P.out_vals32[pos] = hi;

@@ -60,6 +60,7 @@ struct Run {
   const uint16_t* bcol16 = nullptr;  // dense rows: B's column keys as 16 bits
   bool f32 = false;            // FP32 values (both tables carry gval32)
   int val = PTC_VAL_F64;       // ptc_value_type of the numeric pass
+  bool col_first = false;      // output modes: column modes, then row modes
   explicit Run(ptc_ctx* c) : ctx(c), ws(c) {}
 };
 

@@ -162,6 +162,7 @@ void base_params(Run& R, PassParams& P, const HashPlan* hp) {
   P.overflow = R.overflow;
   P.dO = R.dO;
   P.nrowmodes = R.nrowmodes;
+  P.col_first = R.col_first ? 1 : 0;
   P.rowc = R.HA->codec;
   P.colc = R.HB->rest;
 }
@@ -250,8 +251,10 @@ int make_output(Run& R, uint64_t nnz, const ptc_dtensor* A, const ptc_dtensor* B
     o->dims[j] = 0;    // zeros (tensor.hpp:121-122)
   }
   // per-mode index bounds (used as key radices if O is contracted again)
-  for (uint32_t j = 0; j < A->nk; ++j) o->mode_max[j] = A->mode_max[A->dim_tc[j]];
-  for (uint32_t j = 0; j < B->d - B->nk; ++j) o->mode_max[A->nk + j] = B->mode_max[B->dim_tc[B->nk + j]];
+  const uint32_t na = A->nk, nb = B->d - B->nk;  // row modes, column modes
+  const uint32_t ra = R.col_first ? nb : 0, cb = R.col_first ? 0 : na;  // where each block starts
+  for (uint32_t j = 0; j < na; ++j) o->mode_max[ra + j] = A->mode_max[A->dim_tc[j]];
+  for (uint32_t j = 0; j < nb; ++j) o->mode_max[cb + j] = B->mode_max[B->dim_tc[B->nk + j]];
   o->hedges = dalloc_n<uint32_t>(R.ctx, nnz * R.dO);
   if (R.f32) {
     o->vals32 = dalloc_n<float>(R.ctx, nnz);
@@ -521,7 +524,7 @@ struct KeyReq {
 
 int tc_impl(ptc_ctx* ctx, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B, const ptc_dtensor* A,
             const ptc_dtensor* B, const ptc_options* opt_in, ptc_dtensor** O, ptc_stats* stats, const KeyReq* kr,
-            const OutInto* into = nullptr);
+            const OutInto* into = nullptr, bool col_first = false);
 
 }  // namespace
 
@@ -572,7 +575,7 @@ namespace {
 
 int tc_impl(ptc_ctx* ctx, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B, const ptc_dtensor* A,
             const ptc_dtensor* B, const ptc_options* opt_in, ptc_dtensor** O, ptc_stats* stats, const KeyReq* kr,
-            const OutInto* into) {
+            const OutInto* into, bool col_first) {
   if (!ctx || !HT_A || !HT_B || !A || !B || !(O || into)) return PTC_ERR_ARG;
   if (O) *O = nullptr;
   // caller-provided output arrays (exact two-pass only): a non-owning view
@@ -584,6 +587,11 @@ int tc_impl(ptc_ctx* ctx, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B, 
   ptc_options opt;
   if (opt_in) opt = *opt_in; else ptc_options_default(&opt);
   Run R(ctx);
+  // col_first: the output's modes are the right operand's free modes, then
+  // the left operand's (the column-wise product computes C^T's rows)
+  R.col_first = col_first;
+  if (col_first && (opt.method != PTC_METHOD_TWO_PASS || kr || into))
+    return set_err(ctx, PTC_ERR_INTERNAL, "column-first output takes the plain two-pass path");
   Timer T(ctx);
   ctx->nkev = 0;
   auto wall0 = std::chrono::steady_clock::now();
@@ -911,9 +919,47 @@ static int contract_grouped(ptc_ctx* ctx, const ptc_dtensor* A, const ptc_dtenso
   return rc;
 }
 
+// The matricized product C = A_mat B_mat computed column by column, as
+// cs_multiply does (cs_multiply.c:6-38 over the CSC operands that
+// matricizedspGEMM.hpp:172-603 builds): for each column j of C (a B free
+// tuple), the columns of A_mat (A grouped by its contracted key) named by
+// B's nonzeros in column j are scaled and accumulated over C's rows.  On the
+// device this is the row pass run on the transposed problem C^T = B_mat^T
+// A_mat^T: B is grouped by its free modes (the output columns: the CSC
+// column pointers of B_mat), A by its contracted modes (the columns of
+// A_mat), every B nonzero probes A's table, and A's slices are streamed and
+// accumulated per output column over C's row space.  The entries of each
+// column are written with A's free modes first, so the result has the
+// modes and meaning of ptc_contract's, column by column (the CSC order of
+// cs_multiply's result) instead of row by row.  Against ptc_contract it
+// exercises other tables (keyed on B's free and A's contracted modes),
+// another probe direction and another accumulator space -- the internal
+// cross-check that main.cpp:131-158 intended between the two methods.
 extern "C" int ptc_matricize_spGEMM(ptc_ctx* ctx, const ptc_dtensor* A, const ptc_dtensor* B, ptc_dtensor** O,
                                     ptc_stats* stats) {
-  return ptc_contract(ctx, A, B, nullptr, O, stats);
+  if (!ctx || !A || !B || !O) return PTC_ERR_ARG;
+  *O = nullptr;
+  const uint32_t c = A->nk;
+  if (c == 0 || B->nk != c) return set_err(ctx, PTC_ERR_ARG, "A and B must have the same number (>0) of contracted modes");
+  // B with its free modes as key: columns of C
+  uint32_t rot[PTC_MAX_MODES];
+  for (uint32_t i = c; i < B->d; ++i) rot[i - c] = B->dim_tc[i];
+  for (uint32_t i = 0; i < c; ++i) rot[B->d - c + i] = B->dim_tc[i];
+  ptc_dtensor* Bc = view_with_keys(B, B->d - c, rot);
+  ptc_hashtable *HA = nullptr, *HBc = nullptr;
+  int rc = ptc_hashtable_build(ctx, A, &HA);  // A keyed on its contracted modes: A_mat's columns
+  if (!rc) rc = ptc_hashtable_build(ctx, Bc, &HBc);
+  ptc_options opt;
+  ptc_options_default(&opt);
+  if (!rc) rc = tc_impl(ctx, HBc, HA, Bc, A, &opt, O, stats, nullptr, nullptr, /*col_first=*/true);
+  if (!rc && stats) {  // report the operands as the caller named them
+    std::swap(stats->nnz_a, stats->nnz_b);
+    std::swap(stats->ms_build_a, stats->ms_build_b);
+  }
+  ptc_hashtable_free(HBc);
+  ptc_hashtable_free(HA);
+  delete Bc;
+  return rc;
 }
 
 // The sorting matricization groups both operands by sorting their keys

@@ -63,6 +63,20 @@ def test_nips_full_output(ctx, nips, method):
     print(f"  nnz {h.shape[0]} fallback rows {st.fallback_rows}")
 
 
+def test_nips_matricized_method(ctx, nips):
+    """The column-wise matricized product (ptc_matricize_spGEMM) on the
+    nips-shape tensor against the oracle's full output."""
+    A, B, (oh, ov), info = nips
+    print("[case] nips full size, column-wise matricized product", flush=True)
+    Am = P.Tensor(A.hedges_array, A.val_array, A.dimensions_array, B.dim_tc, B.num_key_dimensions)
+    st = P.Stats()
+    O = P.matricize_spGEMM(Am, B, ctx, stats=st)
+    h, v = sort_coo(O.hedges_array, O.val_array)
+    assert st.flops == info["flops"]
+    assert np.array_equal(h, oh)
+    assert max_rel(v, ov) <= TOL
+
+
 def two_tensor():
     ha, va = P.gen_coo(3, [10000, 5000, 200], 5_000_000, P.DIST_ZIPF, 1.1)
     hb, vb = P.gen_coo(4, [5000, 200, 8000], 5_000_000, P.DIST_ZIPF, 1.1)
