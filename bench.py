@@ -124,10 +124,48 @@ def f32_errors(P, ctx, dA, dB, method, value_type, ref):
     return {"coords_equal": True, "max_rel_err": float(rel.max()), "mean_rel_err": float(rel.mean())}
 
 
+def host_cores() -> int:
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def cpu_threads_variant(P, A, B, threads):
+    """The oracle on `threads` host threads at once, A's output rows dealt
+    round-robin by row index: each thread builds its own HT_B and contracts
+    its rows (the optional fixed-threads variant; the reference itself runs
+    every phase on one thread)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    from oracle import orc
+
+    rows = A.hedges_array[:, A.dim_tc[0]]
+    parts = []
+    for t in range(threads):
+        sel = rows % threads == t
+        parts.append(P.Tensor(A.hedges_array[sel], A.val_array[sel], A.dimensions_array, A.dim_tc,
+                              A.num_key_dimensions))
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(threads) as ex:
+        res = list(ex.map(lambda a: orc.tc_fksCuckoo(a, B), parts))
+    wall = time.perf_counter() - t0
+    F = sum(r[2]["flops"] for r in res)
+    return {"threads": threads, "value": F / wall, "unit": "MAD/s", "seconds": wall,
+            "nnz_out": int(sum(r[0].shape[0] for r in res)),
+            "note": "rows dealt to threads by row index mod threads; HT_B built once per thread"}
+
+
 def cpu_baseline(P, h, v):
     """The oracle restatement of the reference on this host: the whole
-    contraction, one thread (the reference's numeric loop runs a team of
-    one, fksCuckooTC.hpp:171-173)."""
+    contraction, one thread -- the reference runs every phase on one thread
+    whatever OMP_NUM_THREADS says (its only parallel region, the numeric
+    loop, takes its team size from omp_get_num_threads() called outside a
+    parallel region, which is 1: fksCuckooTC.hpp:171-173), so runs at
+    OMP_NUM_THREADS=4 or =nproc are this run.  Beside it: the fixed-threads
+    variant on the host's cores (at most 16), and the CPU configuration of
+    the reference's own scale (3-mode 200x300x400, 50k uniform nnz,
+    contract {1,2})."""
     from oracle import orc
 
     A, B = P.self_contraction_inputs(h, v, CONTRACTED)
@@ -135,17 +173,39 @@ def cpu_baseline(P, h, v):
     oh, ov, info = orc.tc_fksCuckoo(A, B)
     wall = time.perf_counter() - t0
     F = info["flops"]
-    return {
+    ncores = host_cores()
+    out = {
         "value": F / wall,
         "unit": "MAD/s",
         "cores": 1,
+        "host_cores": ncores,
         "kind": "port",
         "sample": (f"whole workload (all {info['rows']} rows), single thread: HT_B {info['t_build_b']:.2f}s, "
                    f"HT_A {info['t_build_a']:.2f}s, preprocessing {info['t_preprocess']:.2f}s, "
                    f"contraction {info['t_contract']:.2f}s; nnz_out {oh.shape[0]}"),
         "nnz_out": int(oh.shape[0]),
         "seconds": wall,
-    }, _sorted(oh, ov)
+        "omp": "OMP_NUM_THREADS=4 and =nproc give this same single-thread run (fksCuckooTC.hpp:171-173)",
+    }
+    out["threads_variant"] = cpu_threads_variant(P, A, B, max(1, min(ncores, 16)))
+    h1, v1 = P.gen_coo(1, [200, 300, 400], 50000, P.DIST_UNIFORM)
+    A1, B1 = P.self_contraction_inputs(h1, v1, [1, 2])
+    t0 = time.perf_counter()
+    o1h, _, i1 = orc.tc_fksCuckoo(A1, B1)
+    w1 = time.perf_counter() - t0
+    out["c1"] = {"workload": "3-mode 200x300x400, 50k uniform nnz, seed 1, contract {1,2}", "seconds": w1,
+                 "MADps": i1["flops"] / w1, "flops": int(i1["flops"]), "nnz_out": int(o1h.shape[0]),
+                 "phases_s": {k: round(i1[k], 5) for k in ("t_build_b", "t_build_a", "t_preprocess", "t_contract")}}
+    return out, _sorted(oh, ov)
+
+
+def gpu_c1(P, ctx, steps=20):
+    """The CPU configuration on the GPU (launch-bound at this size)."""
+    h, v = P.gen_coo(1, [200, 300, 400], 50000, P.DIST_UNIFORM)
+    A, B = P.self_contraction_inputs(h, v, [1, 2])
+    dA, dB = P.DeviceTensor.upload(A, ctx), P.DeviceTensor.upload(B, ctx)
+    r = run_method(P, ctx, dA, dB, P.METHOD_TWO_PASS, steps, 3)
+    return {"ms_per_step": round(r["ms_per_step"], 4), "MADps": r["stats"]["flops"] / (r["ms_per_step"] * 1e-3)}
 
 
 def _sorted(h, v):
@@ -664,6 +724,7 @@ def main():
         cb = out["cpu_baseline"]
         if cb["nnz_out"] != nnz_c:
             out["cpu_baseline"]["note"] = f"nnz mismatch: cpu {cb['nnz_out']} gpu {nnz_c}"
+        cb["c1"]["gpu_two_pass"] = gpu_c1(P, ctx)
     if not args.no_f32:
         # the same tensors with values rounded to FP32: two-sum compensated
         # FP32 accumulation, exact products accumulated in FP64, and the plain
