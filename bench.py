@@ -375,6 +375,37 @@ def batch_bench(P, args):
     ctx.close()
 
 
+def batch_plan_bench(P, args):
+    """One plan of the 256-job batch under fixed thresholds (--dense-max,
+    --lds-hash, --chunk; 0 = defaults) and `steps` launches of its numeric
+    graph: a short run for kernel traces and counters of the batch's row
+    passes."""
+    ctx = P.Context(0)
+    dev = []
+    for i in range(BATCH_JOBS):
+        d, dims, nnz = P.gen_job_shape(BATCH_SEED0 + i)
+        h, v = P.gen_coo(BATCH_SEED0 + i, dims, nnz, P.DIST_ZIPF, 1.1)
+        perm = np.array(list(range(1, d)) + [0], dtype=np.uint32)
+        dev.append(P.DeviceTensor.upload(P.Tensor(h, v, None, perm, d - 1), ctx))
+    print("# batch inputs ready", file=sys.stderr, flush=True)
+    plan = P.BatchPlan(dev, dev, ctx, dense_max_cols=args.dense_max, lds_hash_max=args.lds_hash,
+                       chunk_flops=args.chunk)
+    st = plan.stats
+    runs = []
+    for _ in range(max(args.steps, 1)):
+        plan.run()
+        runs.append(plan.stats.ms_numeric)
+    out = {"config": {"workload": BATCH_WORKLOAD, "dense_max_cols": args.dense_max, "lds_hash_max": args.lds_hash,
+                      "chunk_flops": args.chunk},
+           "flops": int(st.flops), "nnz_out": int(st.nnz_out), "bins": int(st.bins),
+           "ms_build_device": round(st.ms_build, 3), "ms_symbolic_device": round(st.ms_symbolic, 3),
+           "ms_symbolic_pass": round(st.ms_symbolic_pass, 3), "ms_plan_wall": round(st.ms_plan, 3),
+           "ms_numeric_device": [round(x, 3) for x in runs]}
+    print(json.dumps(out), flush=True)
+    plan.free()
+    ctx.close()
+
+
 TWO_WORKLOAD = ("two-tensor contraction: A 10000x5000x200 (seed 3), B 5000x200x8000 (seed 4), 5M unique nnz "
                 "each, Zipf(1.1) per mode, FP64 U(0,1]; A modes {1,2} with B modes {0,1} -> 10000x8000")
 
@@ -594,7 +625,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--methods", default="two_pass,upper_bound,learned")
     ap.add_argument("--no-f32", action="store_true", help="skip the FP32 compensated leg")
-    ap.add_argument("--config", default="nips", choices=["nips", "batch", "twotensor", "chain"],
+    ap.add_argument("--dense-max", type=int, default=0, help="batchplan: dense LDS column limit (0 = default)")
+    ap.add_argument("--lds-hash", type=int, default=0, help="batchplan: LDS hash rows limit (0 = default)")
+    ap.add_argument("--chunk", type=int, default=0, help="batchplan: multiply-adds per work item (0 = default)")
+    ap.add_argument("--config", default="nips", choices=["nips", "batch", "batchplan", "twotensor", "chain"],
                     help="nips: the headline self-contraction (default); batch: 256 uneven contractions; "
                          "twotensor: the 5M-nnz two-tensor contraction; chain: two contractions in a row")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -613,6 +647,9 @@ def main():
 
     if args.config == "batch":
         batch_bench(P, args)
+        return
+    if args.config == "batchplan":
+        batch_plan_bench(P, args)
         return
     if args.config == "twotensor":
         two_tensor_bench(P, args)
@@ -711,6 +748,8 @@ def main():
                 "kernel_symbolic_ms": round(x["stats"]["ms_kernel_symbolic"], 4),
                 "fallback_rows": int(x["stats"]["fallback_rows"]),
                 "heavy_rows": int(x["stats"]["heavy_rows"]),
+                "out_capacity": int(x["stats"]["out_capacity"]),
+                "device_peak_bytes": int(x["stats"]["device_peak_bytes"]),
             }
             for k, x in results.items()
         },

@@ -255,6 +255,13 @@ typedef struct {
   double ms_kernel_symbolic, ms_kernel_numeric;
   uint64_t kernel_launches_numeric;
   double ms_build_out;    /* ptc_tc_fksCuckoo_keyed: device time of the output's key table */
+  /* output entries allocated before the output's size was known: the
+   * single-pass methods size the output by the rows' capacities (learned:
+   * their sum; upper bound: min(F, rows x columns)); two-pass: nnz_out */
+  uint64_t out_capacity;
+  /* peak device memory the call held beyond what was live when it started
+   * (its scratch and the output) */
+  uint64_t device_peak_bytes;
 } ptc_stats;
 
 void ptc_options_default(ptc_options* opt);

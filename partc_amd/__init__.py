@@ -159,6 +159,8 @@ class Stats(C.Structure):
         ("ms_kernel_numeric", C.c_double),
         ("kernel_launches_numeric", C.c_uint64),
         ("ms_build_out", C.c_double),
+        ("out_capacity", C.c_uint64),
+        ("device_peak_bytes", C.c_uint64),
     ]
 
     def as_dict(self) -> dict:

@@ -136,6 +136,11 @@ struct CapParams {
   PTR(uint32_t) row_nnz;                                                                      \
   PTR(const unsigned long long) out_base;                                                     \
   PTR(const uint32_t) out_cap; /* capacity mode when set */                                   \
+  /* reservation mode (out_count set): a finished row takes its entries with one fetch-add */ \
+  /* on out_count, never past out_limit, and records its first entry in out_rowoff */         \
+  PTR(unsigned long long) out_count;                                                          \
+  uint64_t out_limit;                                                                         \
+  PTR(unsigned long long) out_rowoff;                                                         \
   PTR(uint32_t) overflow;                                                                     \
   PTR(void) stg_col; /* capacity mode staging (col key) */                                    \
   PTR(double) stg_val;                                                                        \

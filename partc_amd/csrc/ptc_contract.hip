@@ -500,6 +500,7 @@ __device__ __forceinline__ void pass_item(const PP& P, uint32_t widx) {
   unsigned char* acc_base = smem + STAGE * 16 + NT;
   __shared__ uint32_t s_w[PB_MAX / 64];
   __shared__ uint32_t s_misc[4];
+  __shared__ unsigned long long s_base;  // reservation mode: the row's first output entry
   __shared__ uint32_t s_rowc[PTC_MAX_MODES];
 
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -950,7 +951,27 @@ __device__ __forceinline__ void pass_item(const PP& P, uint32_t widx) {
   if (full || total > cap) return;
   // never write past the row's allocation (exact mode: the symbolic count)
   const uint32_t limit = min(total, P.out_cap ? cap : P.row_nnz[r]);
-  const uint64_t base = P.out_base[r];
+  uint64_t base;
+  if (P.out_count) {
+    // reservation: the row's entries go straight to the output, placed by
+    // one fetch-add per row (the reference's append, fksCuckooTC.hpp:345-355;
+    // rows land in completion order)
+    if (threadIdx.x == 0) {
+      unsigned long long b = atomicAdd(P.out_count, (unsigned long long)limit);
+      if (b + limit > P.out_limit) {
+        atomicOr(P.overflow, 0x20000000u);  // cannot happen: the output holds every capacity
+        b = ~0ull;
+      } else if (P.out_rowoff) {
+        P.out_rowoff[r] = b;
+      }
+      s_base = b;
+    }
+    __syncthreads();
+    base = s_base;
+    if (base == ~0ull) return;
+  } else {
+    base = P.out_base[r];
+  }
   uint32_t run = 0;
   for (uint64_t c0 = 0; c0 < span; c0 += NT) {
     uint64_t c = c0 + threadIdx.x;

@@ -86,6 +86,8 @@ struct ptc_ctx {
   std::multimap<size_t, void*> free_blocks;
   std::map<void*, size_t> live;
   size_t cached_bytes = 0;
+  size_t live_bytes = 0;     // bytes held by live blocks (cache excluded)
+  size_t peak_bytes = 0;     // high-water mark of live_bytes since a call reset it
   hipEvent_t ev[8] = {};
   hipEvent_t kev[16] = {};   // kernel-level timing events
   int nkev = 0;

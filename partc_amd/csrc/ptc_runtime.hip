@@ -67,6 +67,8 @@ void* dalloc(ptc_ctx* ctx, size_t bytes) {
     ctx->free_blocks.erase(it);
     ctx->cached_bytes -= have;
     ctx->live[p] = have;
+    ctx->live_bytes += have;
+    ctx->peak_bytes = std::max(ctx->peak_bytes, ctx->live_bytes);
     return p;
   }
   void* p = nullptr;
@@ -86,6 +88,8 @@ void* dalloc(ptc_ctx* ctx, size_t bytes) {
     }
   }
   ctx->live[p] = sz;
+  ctx->live_bytes += sz;
+  ctx->peak_bytes = std::max(ctx->peak_bytes, ctx->live_bytes);
   return p;
 }
 
@@ -106,6 +110,7 @@ void dfree(ptc_ctx* ctx, void* p) {
   if (it == ctx->live.end()) return;
   size_t sz = it->second;
   ctx->live.erase(it);
+  ctx->live_bytes -= sz;
   // Blocks are only reused by work queued later on the same stream, so no
   // synchronisation is needed before caching them.
   ctx->free_blocks.emplace(sz, p);
@@ -469,6 +474,7 @@ extern "C" void ptc_destroy(ptc_ctx* ctx) {
   ctx->free_blocks.clear();
   ctx->live.clear();
   ctx->cached_bytes = 0;
+  ctx->live_bytes = 0;
   for (auto& e : ctx->ev)
     if (e) (void)hipEventDestroy(e);
   for (auto& e : ctx->kev)

@@ -512,6 +512,164 @@ int prepare(Run& R, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B, const 
 
 }  // namespace ptc
 
+namespace ptc {
+
+// Output arrays of o reallocated for cap entries, the first keep entries
+// copied over (a single-pass output that outgrew its capacities, or one
+// whose capacities overestimated it by more than half).
+static int resize_output(Run& R, ptc_dtensor* o, uint64_t keep, uint64_t cap) {
+  ptc_ctx* ctx = R.ctx;
+  uint32_t* h = dalloc_n<uint32_t>(ctx, cap * R.dO);
+  double* v = R.f32 ? nullptr : dalloc_n<double>(ctx, cap);
+  float* v32 = R.f32 ? dalloc_n<float>(ctx, cap) : nullptr;
+  float* e32 = R.f32 ? dalloc_n<float>(ctx, cap) : nullptr;
+  if (!h || !(v || (v32 && e32))) {
+    dfree(ctx, h);
+    dfree(ctx, v);
+    dfree(ctx, v32);
+    dfree(ctx, e32);
+    return PTC_ERR_NOMEM;
+  }
+  if (keep) {
+    PTC_HIP(ctx, hipMemcpyAsync(h, o->hedges, sizeof(uint32_t) * keep * R.dO, hipMemcpyDeviceToDevice, ctx->stream));
+    if (R.f32) {
+      PTC_HIP(ctx, hipMemcpyAsync(v32, o->vals32, sizeof(float) * keep, hipMemcpyDeviceToDevice, ctx->stream));
+      PTC_HIP(ctx, hipMemcpyAsync(e32, o->err32, sizeof(float) * keep, hipMemcpyDeviceToDevice, ctx->stream));
+    } else {
+      PTC_HIP(ctx, hipMemcpyAsync(v, o->vals, sizeof(double) * keep, hipMemcpyDeviceToDevice, ctx->stream));
+    }
+  }
+  // the old blocks are only reused by work queued after these copies
+  dfree(ctx, o->hedges);
+  dfree(ctx, o->vals);
+  dfree(ctx, o->vals32);
+  dfree(ctx, o->err32);
+  o->hedges = h;
+  o->vals = v;
+  o->vals32 = v32;
+  o->err32 = e32;
+  return PTC_OK;
+}
+
+// The single-pass methods with direct writes: the output arrays are sized by
+// the rows' capacities (the learned method reads their sum back; the upper
+// bound uses min(F, rows x columns), which bounds it without a round trip),
+// and every row that finishes within its capacity reserves its entries with
+// one fetch-add and writes them in place -- no staging area, no count scan,
+// no copy.  Rows over their capacity (or whose LDS table filled) are redone
+// by the exact two-pass path, appended the same way.
+static int single_pass_direct(Run& R, const ptc_dtensor* A, const ptc_dtensor* B, const uint32_t* cap,
+                              const unsigned long long* cap_off, int method, Timer& T, ptc_dtensor** out_p,
+                              uint64_t* nnz_p, uint64_t* fallback_p, uint64_t* capacity_p) {
+  ptc_ctx* ctx = R.ctx;
+  Workspace& ws = R.ws;
+  int rc = PTC_OK;
+  unsigned long long capsum = R.F;
+  if (R.ncols && R.nrows <= ~0ull / R.ncols) capsum = std::min<unsigned long long>(capsum, R.nrows * R.ncols);
+  if ((method == PTC_METHOD_LEARNED || capsum > STAGING_BOUND) && (rc = read_back(ctx, &capsum, cap_off + R.nrows, 8)))
+    return rc;
+  *capacity_p = capsum;
+  ptc_dtensor* out = nullptr;
+  if ((rc = make_output(R, capsum, A, B, &out))) return rc;
+  auto fail = [&](int code) {
+    ptc_dtensor_free(out);
+    return code;
+  };
+  unsigned long long* counter = ws.get<unsigned long long>(2);
+  unsigned long long* pair = ws.get<unsigned long long>(2);
+  if (!counter || !pair) return fail(PTC_ERR_NOMEM);
+  HashPlan h1;
+  if ((rc = fill_zero(ctx, counter, 16)) || (!R.dense && (rc = plan_hash(R, h1, cap, nullptr, R.row_flops))) ||
+      (rc = fill_zero(ctx, R.row_nnz, (R.nrows + 1) * 4)) || (rc = prepare_heavy(R, PASS_NUMERIC)) ||
+      (!R.dense && (rc = reset_hash_pool(R, h1, true))))
+    return fail(rc);
+  auto direct = [&](PassParams& P, const uint32_t* caps) {
+    P.out_cap = caps;
+    P.out_count = counter;
+    P.out_limit = capsum;
+    P.out_rowoff = R.out_off;
+    P.out_hedges = out->hedges;
+    P.out_vals = out->vals;
+    P.out_vals32 = out->vals32;
+    P.out_err32 = out->err32;
+  };
+  PassParams P;
+  base_params(R, P, R.dense ? nullptr : &h1);
+  direct(P, cap);
+  if ((rc = run_pass(R, PASS_NUMERIC, &h1, R.items, R.nitems, P))) return fail(rc);
+  T.mark();  // 2
+  // the consistency flags, the overflow count and the entries written: one read
+  unsigned long long rb[2] = {0, 0};
+  {
+    WordSources W{};
+    W.n = 2;
+    W.src[0] = R.overflow;
+    W.bytes[0] = 4;
+    W.src[1] = counter;
+    W.bytes[1] = 8;
+    if ((rc = collect_words(ctx, pair, W)) || (rc = read_back(ctx, rb, pair, 16))) return fail(rc);
+  }
+  const uint32_t ovf0 = (uint32_t)rb[0];
+  if (ovf0 & 0x80000000u) return fail(set_err(ctx, PTC_ERR_INTERNAL, "accumulator table overflow"));
+  if (ovf0 & 0x40000000u) return fail(set_err(ctx, PTC_ERR_INTERNAL, "slice index out of range"));
+  if (ovf0 & 0x20000000u) return fail(set_err(ctx, PTC_ERR_INTERNAL, "output reservation past the capacities"));
+  const uint64_t fallback = ovf0 & 0x1FFFFFFFu;
+  uint64_t nnz = rb[1];
+  if (fallback) {
+    // exact two-pass for the rows whose capacity was too small
+    uint32_t* cnt = ws.get<uint32_t>(4);
+    uint4* fitems = ws.get<uint4>(R.nitems);
+    uint32_t* frows = ws.get<uint32_t>(fallback);
+    uint32_t* row_fb = ws.get<uint32_t>(R.nrows);
+    unsigned long long* tot = ws.get<unsigned long long>(R.nrows + 1);
+    if (!cnt || !fitems || !frows || !row_fb || !tot) return fail(PTC_ERR_NOMEM);
+    uint32_t nfitems = 0;
+    if ((rc = fill_zero(ctx, cnt, 16)) || (rc = launch_select_items(ctx, R.items, R.nitems, R.row_nnz, cap, fitems, cnt)) ||
+        (rc = launch_overflow_rows(ctx, R.row_nnz, cap, R.nrows, frows, cnt + 1, row_fb)) ||
+        (rc = read_back(ctx, &nfitems, cnt, 4)) || (rc = launch_reset_rows(ctx, frows, (uint32_t)fallback, R.row_nnz)))
+      return fail(rc);
+    HashPlan hs;
+    if ((!R.dense && (rc = plan_hash(R, hs, nullptr, R.row_flops, nullptr))) || (rc = prepare_heavy(R, PASS_SYMBOLIC)) ||
+        (!R.dense && (rc = reset_hash_pool(R, hs, false))))
+      return fail(rc);
+    PassParams S;
+    base_params(R, S, R.dense ? nullptr : &hs);
+    if ((rc = run_pass(R, PASS_SYMBOLIC, &hs, fitems, nfitems, S, frows, fallback))) return fail(rc);
+    // every row's exact count: the output's size
+    unsigned long long total = 0;
+    if ((rc = scan_u32_to_u64(ctx, R.row_nnz, tot, R.nrows, tot + R.nrows)) ||
+        (rc = read_back(ctx, &total, tot + R.nrows, 8)))
+      return fail(rc);
+    if (total > capsum) {
+      if ((rc = resize_output(R, out, nnz, total))) return fail(rc);
+      capsum = total;
+    }
+    HashPlan hn;
+    if ((!R.dense && (rc = plan_hash(R, hn, R.row_nnz, nullptr, nullptr))) || (rc = prepare_heavy(R, PASS_NUMERIC)) ||
+        (!R.dense && (rc = reset_hash_pool(R, hn, true))))
+      return fail(rc);
+    PassParams Q;
+    base_params(R, Q, R.dense ? nullptr : &hn);
+    direct(Q, nullptr);  // exact mode: each row's count is its symbolic count
+    if ((rc = run_pass(R, PASS_NUMERIC, &hn, fitems, nfitems, Q, frows, fallback))) return fail(rc);
+    uint32_t ovf = 0;
+    if ((rc = read_back(ctx, &ovf, R.overflow, 4))) return fail(rc);
+    if (ovf & 0x80000000u) return fail(set_err(ctx, PTC_ERR_INTERNAL, "row count mismatch between passes"));
+    if (ovf & 0x60000000u) return fail(set_err(ctx, PTC_ERR_INTERNAL, "slice index or reservation out of range"));
+    nnz = total;
+  }
+  // capacities that overestimated the output by more than half: exact arrays
+  if (capsum > 2 * nnz + 65536 && (rc = resize_output(R, out, nnz, nnz))) return fail(rc);
+  out->N = nnz;
+  T.mark();  // 3
+  *out_p = out;
+  *nnz_p = nnz;
+  *fallback_p = fallback;
+  return PTC_OK;
+}
+
+}  // namespace ptc
+
 namespace {
 
 // The key split requested for the output of a contraction chain's step.
@@ -594,6 +752,8 @@ int tc_impl(ptc_ctx* ctx, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B, 
     return set_err(ctx, PTC_ERR_INTERNAL, "column-first output takes the plain two-pass path");
   Timer T(ctx);
   ctx->nkev = 0;
+  const size_t live0 = ctx->live_bytes;
+  ctx->peak_bytes = live0;
   auto wall0 = std::chrono::steady_clock::now();
   T.mark();  // 0
   int rc = prepare(R, HT_A, HT_B, A, B, opt);
@@ -604,6 +764,8 @@ int tc_impl(ptc_ctx* ctx, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B, 
   T.mark();  // 1
   uint64_t nnz = 0;
   uint64_t fallback_rows = 0;
+  uint64_t out_capacity = 0;   // output entries allocated before the count was known
+  bool flags_checked = false;  // the last pass's consistency flags were read already
   ptc_dtensor* out = nullptr;
   const int method = opt.method;
 
@@ -622,6 +784,7 @@ int tc_impl(ptc_ctx* ctx, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B, 
     if ((rc = scan_u32_to_u64(ctx, R.row_nnz, out_off, R.nrows, out_off + R.nrows))) return rc;
     if ((rc = read_back(ctx, &hostv[0], out_off + R.nrows, 8))) return rc;
     nnz = hostv[0];
+    out_capacity = nnz;
     if (into) {
       *into->nnz = nnz;  // also when the caller's arrays are too small
       if (nnz > into->cap)
@@ -688,6 +851,14 @@ int tc_impl(ptc_ctx* ctx, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B, 
       if ((rc = launch_caps(ctx, P))) return rc;
     }
     if ((rc = scan_u32_to_u64(ctx, cap, cap_off, R.nrows, cap_off + R.nrows))) return rc;
+    if (!kr) {
+      if ((rc = single_pass_direct(R, A, B, cap, cap_off, method, T, &out, &nnz, &fallback_rows, &out_capacity)))
+        return rc;
+      flags_checked = true;
+    } else {
+    // A chain step keeps its output rows in row order (its key table is
+    // formed from them): capacity-sized staging, a scan of the true counts,
+    // and a copy into place.
     // Every capacity is at most min(row flops, columns), so the staging area
     // is sized by min(F, rows x columns) without waiting for the sum; only a
     // bound above STAGING_BOUND entries is replaced by the exact sum.
@@ -804,11 +975,13 @@ int tc_impl(ptc_ctx* ctx, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B, 
         return rc;
       }
     }
+    out_capacity = capsum;
     T.mark();  // 3
+    }
   }
   // The consistency flags of the last numeric pass: in the single-pass
   // methods without fallback rows they were read with the counts already.
-  if (method == PTC_METHOD_TWO_PASS || fallback_rows) {
+  if (!flags_checked && (method == PTC_METHOD_TWO_PASS || fallback_rows)) {
     uint32_t ovf = 0;
     rc = read_back(ctx, &ovf, R.overflow, 4);
     if (!rc && (ovf & 0x80000000u)) rc = set_err(ctx, PTC_ERR_INTERNAL, "row count mismatch between passes");
@@ -847,6 +1020,8 @@ int tc_impl(ptc_ctx* ctx, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B, 
     stats->nnz_out = nnz;
     stats->fallback_rows = fallback_rows;
     stats->heavy_rows = R.nheavy;
+    stats->out_capacity = out_capacity;
+    stats->device_peak_bytes = ctx->peak_bytes - live0;
     stats->ms_build_a = hashtable_ms_build(HT_A);
     stats->ms_build_b = hashtable_ms_build(HT_B);
     stats->ms_preprocess = T.ms(0, 1);
