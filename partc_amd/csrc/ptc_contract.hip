@@ -537,7 +537,10 @@ __device__ __forceinline__ void pass_item(const PP& P, uint32_t widx) {
   double* acc = (double*)acc_base;
   float* acc_s = (float*)acc_base;  // FP32: sums [ncols], then compensations [ncols]
   float* acc_c = acc_s + ncols;
-  unsigned char* flag = acc_base + ncols * 8;
+  // touched columns (when values may give zero products): one bit per
+  // column, set by an LDS atomic OR; the finish skips empty 64-column tiles
+  uint32_t* fbits = (uint32_t*)(acc_base + ncols * 8);
+  const uint32_t nfw = (uint32_t)((ncols + 31) / 32);
   uint32_t* bits = (uint32_t*)acc_base;
   ColT* hkeys = (ColT*)acc_base;
   uint32_t S = 0;  // LDS hash slots for this row (0: HBM table)
@@ -549,6 +552,10 @@ __device__ __forceinline__ void pass_item(const PP& P, uint32_t widx) {
   if (DENSE && MERGE) {
     // reduce the row's slabs in chunk order
     const uint64_t slab0 = P.row_slab[r];
+    if (PASS == PASS_NUMERIC && flags) {  // touched bits are OR-ed in below
+      for (uint32_t w = threadIdx.x; w < nfw; w += NT) fbits[w] = 0;
+      __syncthreads();
+    }
     if (PASS == PASS_SYMBOLIC) {
       for (uint32_t w = threadIdx.x; w < nbits; w += NT) {
         uint32_t o = 0;
@@ -572,7 +579,7 @@ __device__ __forceinline__ void pass_item(const PP& P, uint32_t widx) {
           for (uint32_t j = 0; j < nch; ++j) fl |= P.hflag[(slab0 + j) * ncols + c];
         acc_s[c] = S;
         acc_c[c] = Cc;
-        if (flags) flag[c] = fl;
+        if (fl) atomicOr(&fbits[c >> 5], 1u << (c & 31));
       }
     } else {
       for (uint64_t c = threadIdx.x; c < ncols; c += NT) {
@@ -591,7 +598,7 @@ __device__ __forceinline__ void pass_item(const PP& P, uint32_t widx) {
         if (flags)
           for (uint32_t j = 0; j < nch; ++j) fl |= P.hflag[(slab0 + j) * ncols + c];
         acc[c] = sum;
-        if (flags) flag[c] = fl;
+        if (fl) atomicOr(&fbits[c >> 5], 1u << (c & 31));
       }
     }
   } else if (DENSE) {
@@ -603,12 +610,12 @@ __device__ __forceinline__ void pass_item(const PP& P, uint32_t widx) {
         acc_c[c] = 0.f;
       }
       if (flags)
-        for (uint64_t c = threadIdx.x; c < ncols; c += NT) flag[c] = 0;
+        for (uint32_t w = threadIdx.x; w < nfw; w += NT) fbits[w] = 0;
     } else {
       const double nz = __longlong_as_double((long long)PTC_NEG_ZERO_BITS);
       for (uint64_t c = threadIdx.x; c < ncols; c += NT) acc[c] = nz;
       if (flags)
-        for (uint64_t c = threadIdx.x; c < ncols; c += NT) flag[c] = 0;
+        for (uint32_t w = threadIdx.x; w < nfw; w += NT) fbits[w] = 0;
     }
   } else {
     uint32_t hc = P.row_hcap[r];
@@ -802,7 +809,7 @@ __device__ __forceinline__ void pass_item(const PP& P, uint32_t widx) {
               } else {
                 atomicAdd(&acc_s[col[u]], pr[u]);
               }
-              if (flags) flag[col[u]] = 1;
+              if (flags) atomicOr(&fbits[(uint32_t)col[u] >> 5], 1u << ((uint32_t)col[u] & 31));
             }
           }
           if (COMP) {
@@ -828,7 +835,7 @@ __device__ __forceinline__ void pass_item(const PP& P, uint32_t widx) {
               if (F32) acc_f32_lds<COMP>(&acc_s[c], &acc_c[c], s_aval32[ii[u]], bv32[u]);
               else if (F32IN) atomicAdd(&acc[c], (double)s_aval32[ii[u]] * (double)bv32[u]);
               else atomicAdd(&acc[c], s_aval[ii[u]] * bv[u]);
-              if (flags) flag[c] = 1;
+              if (flags) atomicOr(&fbits[(uint32_t)c >> 5], 1u << ((uint32_t)c & 31));
             }
           } else {
             if (!act[u]) continue;
@@ -875,14 +882,14 @@ __device__ __forceinline__ void pass_item(const PP& P, uint32_t widx) {
         for (uint64_t c = threadIdx.x; c < ncols; c += NT) ga[c] = make_float2(acc_s[c], acc_c[c]);
         if (flags) {
           unsigned char* gf = P.hflag + slab * ncols;
-          for (uint64_t c = threadIdx.x; c < ncols; c += NT) gf[c] = flag[c];
+          for (uint64_t c = threadIdx.x; c < ncols; c += NT) gf[c] = (unsigned char)((fbits[c >> 5] >> (c & 31)) & 1u);
         }
       } else {
         double* ga = P.hacc + slab * ncols;
         for (uint64_t c = threadIdx.x; c < ncols; c += NT) ga[c] = acc[c];
         if (flags) {
           unsigned char* gf = P.hflag + slab * ncols;
-          for (uint64_t c = threadIdx.x; c < ncols; c += NT) gf[c] = flag[c];
+          for (uint64_t c = threadIdx.x; c < ncols; c += NT) gf[c] = (unsigned char)((fbits[c >> 5] >> (c & 31)) & 1u);
         }
       }
       return;
@@ -920,22 +927,36 @@ __device__ __forceinline__ void pass_item(const PP& P, uint32_t widx) {
   }
   __syncthreads();
 
+  // The span (columns, LDS slots or HBM slots) is split into 64-entry tiles,
+  // each wave owning a contiguous run of them: a wave counts its entries with
+  // ballots, one block-wide exchange gives every wave its first output
+  // position, and the wave writes its entries in span order (consecutive
+  // lanes, consecutive outputs).  Tiles with nothing in them cost a ballot
+  // (with touched bits, one broadcast LDS word), so a sparse row in a wide
+  // column space is not charged per column.
   const uint64_t span = DENSE ? ncols : (S ? S : (uint64_t)gmask + 1);
-  // count
-  uint32_t cnt = 0;
-  for (uint64_t c = threadIdx.x; c < span; c += NT) {
-    bool f;
-    if (DENSE) f = flags ? flag[c] != 0 : (F32 ? touched32(acc_s[c]) : touched(acc[c]));
-    else if (S) f = hkeys[c] != ColEmpty<ColT>::v;
-    else f = __hip_atomic_load(&gkeys[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != ColEmpty<ColT>::v;
-    cnt += f ? 1u : 0u;
+  const uint64_t ntiles = (span + 63) / 64;
+  const uint64_t tpw = (ntiles + NW - 1) / NW;
+  const uint64_t t_lo = min<uint64_t>(ntiles, (uint64_t)wid * tpw), t_hi = min<uint64_t>(ntiles, t_lo + tpw);
+  auto present = [&](uint64_t c) -> bool {
+    if (c >= span) return false;
+    if (DENSE) {
+      if (flags) return (fbits[c >> 5] >> (c & 31)) & 1u;
+      return F32 ? touched32(acc_s[c]) : touched(acc[c]);
+    }
+    if (S) return hkeys[c] != ColEmpty<ColT>::v;
+    return __hip_atomic_load(&gkeys[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != ColEmpty<ColT>::v;
+  };
+  uint32_t wcnt = 0;
+  for (uint64_t t = t_lo; t < t_hi; ++t) wcnt += (uint32_t)__popcll(__ballot(present(t * 64 + lane)));
+  if (lane == 0) s_w[wid] = wcnt;
+  __syncthreads();
+  uint32_t total = 0, wbase = 0;
+  for (int w = 0; w < NW; ++w) {
+    const uint32_t x = s_w[w];
+    if (w < wid) wbase += x;
+    total += x;
   }
-  for (int o = 32; o > 0; o >>= 1) cnt += __shfl_down(cnt, o, 64);
-  if (lane == 0) s_w[wid] = cnt;
-  __syncthreads();
-  uint32_t total = 0;
-  for (int w = 0; w < NW; ++w) total += s_w[w];
-  __syncthreads();
   uint32_t cap = P.out_cap ? P.out_cap[r] : 0xFFFFFFFFu;
   const bool full = s_misc[3] != 0;  // LDS table filled up: count unknown
   if (threadIdx.x == 0) {
@@ -972,27 +993,27 @@ __device__ __forceinline__ void pass_item(const PP& P, uint32_t widx) {
   } else {
     base = P.out_base[r];
   }
-  uint32_t run = 0;
-  for (uint64_t c0 = 0; c0 < span; c0 += NT) {
-    uint64_t c = c0 + threadIdx.x;
-    bool f = false;
-    ColT col = 0;
-    double v = 0.0;
-    float fs = 0.f, fc = 0.f;
-    if (c < span) {
+  uint32_t run = wbase;
+  for (uint64_t t = t_lo; t < t_hi; ++t) {
+    const uint64_t c = t * 64 + lane;
+    const bool f = present(c);
+    const unsigned long long m = __ballot(f);
+    if (!m) continue;  // wave-uniform
+    const uint32_t rank = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    if (f && run + rank < limit) {
+      ColT col = 0;
+      double v = 0.0;
+      float fs = 0.f, fc = 0.f;
       if (DENSE) {
+        col = (ColT)c;
         if (F32) {
           fs = acc_s[c];
           fc = acc_c[c];
-          f = flags ? flag[c] != 0 : touched32(fs);
         } else {
           v = acc[c];
-          f = flags ? flag[c] != 0 : touched(v);
         }
-        col = (ColT)c;
       } else if (S) {
         col = hkeys[c];
-        f = col != ColEmpty<ColT>::v;
         if (F32) {
           fs = hs32[c];
           fc = hc32[c];
@@ -1001,29 +1022,24 @@ __device__ __forceinline__ void pass_item(const PP& P, uint32_t widx) {
         }
       } else {
         col = __hip_atomic_load(&gkeys[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        f = col != ColEmpty<ColT>::v;
-        if (f) {
-          if (F32) {
-            float* g2 = (float*)gvals;
-            fs = __hip_atomic_load(&g2[2 * c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            fc = __hip_atomic_load(&g2[2 * c + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          } else {
-            v = __hip_atomic_load(&gvals[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          }
+        if (F32) {
+          float* g2 = (float*)gvals;
+          fs = __hip_atomic_load(&g2[2 * c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          fc = __hip_atomic_load(&g2[2 * c + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+          v = __hip_atomic_load(&gvals[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
       }
+      float hi = 0.f, lo = 0.f;
+      if (F32) {
+        f32_finish<COMP>(fs, fc, &hi, &lo);
+      } else if (F32IN) {
+        hi = (float)v;  // the FP64 sum as an FP32 pair
+        lo = (float)(v - (double)hi);
+      }
+      emit_entry<ColT, F32IN>(P, base + run + rank, s_rowc, col, v, hi, lo);
     }
-    float hi = 0.f, lo = 0.f;
-    if (F32) {
-      f32_finish<COMP>(fs, fc, &hi, &lo);
-    } else if (F32IN) {
-      hi = (float)v;  // the FP64 sum as an FP32 pair
-      lo = (float)(v - (double)hi);
-    }
-    uint32_t tile_total;
-    uint32_t rank = block_rank(f, s_w, &tile_total);
-    if (f && run + rank < limit) emit_entry<ColT, F32IN>(P, base + run + rank, s_rowc, col, v, hi, lo);
-    run += tile_total;
+    run += (uint32_t)__popcll(m);
   }
 }
 
@@ -1137,7 +1153,7 @@ size_t pass_lds_bytes(int pass, bool dense, bool col64, uint64_t ncols, uint32_t
   size_t b = (size_t)2 * threads * 16 + threads;  // staging: 2 positions per thread, 64 flags per wave
   if (dense) {
     if (pass == PASS_SYMBOLIC) b += ((ncols + 31) / 32) * 4;
-    else b += ncols * (flags ? 9 : 8);  // accumulator (+ flag byte)
+    else b += ncols * 8 + (flags ? ((ncols + 31) / 32) * 4 : 0);  // accumulator (+ touched bits)
   } else {
     b += (size_t)lds_slots * ((col64 ? 8 : 4) + (pass == PASS_NUMERIC ? 8 : 0));
   }
