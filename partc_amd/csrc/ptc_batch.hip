@@ -113,6 +113,62 @@ struct Bin {
   unsigned grid = 0, mgrid = 0;      // device-filling grids (main, merge)
 };
 
+// Rows of a dense job that are small against its column space: at most
+// SPARSE_FLOPS multiply-adds (so at most that many distinct columns) in a
+// column space of at least SPLIT_MIN_COLS.  Their items run in an LDS hash
+// bin of SPARSE_THREADS-thread workgroups with tables of at most
+// SPARSE_SLOTS slots (about 29 KB of LDS: five workgroups per CU) instead of
+// the job's dense bin, whose accumulator spans every column (up to ~144 KB:
+// one workgroup per CU), so many small rows are in flight per CU at once and
+// none pays for the whole column space.
+constexpr uint64_t SPLIT_MIN_COLS = 4096;
+constexpr unsigned long long SPARSE_FLOPS = 1024;
+constexpr uint32_t SPARSE_SLOTS = 2048;
+constexpr uint32_t SPARSE_THREADS = 128;
+
+// Items of a job split by row kind (order within each list is free).
+// A small row is one work item (a row split over several items needs the
+// dense share's slabs and merge).
+__global__ void k_split_items(const uint4* __restrict__ items, uint64_t nitems, const unsigned long long* __restrict__ row_flops,
+                              const uint32_t* __restrict__ row_nchunk, uint4* __restrict__ dense, uint4* __restrict__ sparse,
+                              uint32_t* __restrict__ cnt) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nitems; i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint4 it = items[i];
+    if (row_flops[it.x] <= SPARSE_FLOPS && row_nchunk[it.x] == 1) sparse[atomicAdd(&cnt[1], 1u)] = it;
+    else dense[atomicAdd(&cnt[0], 1u)] = it;
+  }
+}
+
+// LDS hash slots of the sparse rows: a power of two >= 2 x their flops (>= 16).
+__global__ void k_sparse_caps(const unsigned long long* __restrict__ row_flops, uint64_t nrows, uint32_t* __restrict__ hcap) {
+  for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nrows; r += (uint64_t)gridDim.x * blockDim.x) {
+    const unsigned long long f = row_flops[r];
+    uint32_t c = 16;
+    while (c < 2 * f && c < SPARSE_SLOTS) c <<= 1;
+    hcap[r] = c;
+  }
+}
+
+// One kernel kind's share of a job: its parameters (items pointing at the
+// share's work list), kernel shape and work counts.
+struct Work {
+  int job = 0;
+  bool dense = true, col64 = false;
+  uint32_t threads = PB, lds_slots = 0;
+  PassParams params{};
+  uint64_t nitems = 0, nmerge = 0;
+  double weight = 0.0;  // multiply-adds per item (queue order)
+};
+
+// A dense job whose small rows run in a hash bin (device arrays in b->ws).
+struct Split {
+  uint32_t nd = 0, ns = 0;     // dense and sparse items
+  uint4* dense = nullptr;
+  uint4* sparse = nullptr;
+  uint32_t* hcap = nullptr;    // LDS hash slots of every row
+  unsigned long long* zero = nullptr;  // no row of the share lives in HBM
+};
+
 // LDS footprint classes: a bin's launch opens the class size, so jobs of a
 // bin differ in footprint by at most the class step.
 static size_t lds_class(size_t b) {
@@ -137,6 +193,7 @@ struct ptc_batch {
   std::vector<HashPlan> hn;
   std::vector<ptc_dtensor*> out;
   std::vector<Bin> bins;
+  std::vector<Split> split;          // per job (ns = 0: not split)
   Workspace* ws = nullptr;           // batch-level device arrays
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -189,46 +246,39 @@ double stats_ms(ptc_batch* b, int i, int j) {
   return t;
 }
 
-// Bins of the jobs for one pass and their queues; params[j] must be filled.
-int make_bins(ptc_batch* b, int pass, const std::vector<PassParams>& params, const std::vector<HashPlan*>& hp,
-              std::vector<Bin>& bins) {
+// Bins of the works for one pass and their queues.
+int make_bins(ptc_batch* b, int pass, const std::vector<Work>& works, std::vector<Bin>& bins) {
   std::map<std::tuple<bool, bool, size_t, uint32_t>, int> key;
   bins.clear();
-  for (int j = 0; j < b->n; ++j) {
-    Run& R = *b->runs[j];
-    const size_t need = pass_lds_bytes(pass, R.dense, R.col64, R.ncols, R.lds_slots, R.use_flags, R.threads);
+  for (int w = 0; w < (int)works.size(); ++w) {
+    const Work& W = works[w];
+    const Run& R = *b->runs[W.job];
+    const size_t need = pass_lds_bytes(pass, W.dense, W.col64, R.ncols, W.lds_slots, R.use_flags, W.threads);
     const size_t lds = std::max(need, std::min<size_t>(lds_class(need), lds_limit(b->ctx)));
-    auto k = std::make_tuple(R.dense, R.col64, lds, R.threads);
+    auto k = std::make_tuple(W.dense, W.col64, lds, W.threads);
     auto it = key.find(k);
     if (it == key.end()) {
       it = key.emplace(k, (int)bins.size()).first;
       Bin nb;
-      nb.dense = R.dense;
-      nb.col64 = R.col64;
+      nb.dense = W.dense;
+      nb.col64 = W.col64;
       nb.lds = lds;
-      nb.threads = R.threads;
+      nb.threads = W.threads;
       bins.push_back(nb);
     }
-    bins[it->second].jobs.push_back(j);
+    bins[it->second].jobs.push_back(w);
   }
   for (Bin& bn : bins) {
-    // heaviest work items first: jobs by mean flops per item
-    std::stable_sort(bn.jobs.begin(), bn.jobs.end(), [&](int x, int y) {
-      const Run &a = *b->runs[x], &c = *b->runs[y];
-      return (double)a.F / std::max<uint64_t>(a.nitems, 1) > (double)c.F / std::max<uint64_t>(c.nitems, 1);
-    });
+    // heaviest work items first
+    std::stable_sort(bn.jobs.begin(), bn.jobs.end(),
+                     [&](int x, int y) { return works[x].weight > works[y].weight; });
     std::vector<PassParams> bp;
     std::vector<uint2> q, mq;
     for (size_t s = 0; s < bn.jobs.size(); ++s) {
-      const int j = bn.jobs[s];
-      Run& R = *b->runs[j];
-      bp.push_back(params[j]);
-      for (uint64_t i = 0; i < R.nitems; ++i) q.push_back(make_uint2((uint32_t)s, (uint32_t)i));
-      if (bn.dense) {
-        for (uint64_t h = 0; h < R.nheavy; ++h) mq.push_back(make_uint2((uint32_t)s, (uint32_t)h));
-      } else if (pass == PASS_NUMERIC && hp[j] && hp[j]->pool) {
-        for (uint64_t r = 0; r < R.nrows; ++r) mq.push_back(make_uint2((uint32_t)s, (uint32_t)r));
-      }
+      const Work& W = works[bn.jobs[s]];
+      bp.push_back(W.params);
+      for (uint64_t i = 0; i < W.nitems; ++i) q.push_back(make_uint2((uint32_t)s, (uint32_t)i));
+      for (uint64_t h = 0; h < W.nmerge; ++h) mq.push_back(make_uint2((uint32_t)s, (uint32_t)h));
     }
     if (q.size() > 0xFFFFFFFFull || mq.size() > 0xFFFFFFFFull) return set_err(b->ctx, PTC_ERR_RANGE, "batch queue too long");
     bn.nq = (uint32_t)q.size();
@@ -243,6 +293,56 @@ int make_bins(ptc_batch* b, int pass, const std::vector<PassParams>& params, con
     if (rc) return rc;
   }
   return PTC_OK;
+}
+
+// The works of every job for one pass: the job as one work, or -- for a
+// split dense job -- its dense share (heavy rows, with their merges) and its
+// small rows in an LDS hash share.
+std::vector<Work> make_works(ptc_batch* b, int pass, const std::vector<PassParams>& params,
+                             const std::vector<HashPlan*>& hp) {
+  std::vector<Work> works;
+  for (int j = 0; j < b->n; ++j) {
+    const Run& R = *b->runs[j];
+    const Split& sp = b->split[j];
+    Work W;
+    W.job = j;
+    W.dense = R.dense;
+    W.col64 = R.col64;
+    W.threads = R.threads;
+    W.lds_slots = R.lds_slots;
+    W.params = params[j];
+    W.weight = (double)R.F / std::max<uint64_t>(R.nitems, 1);
+    if (!sp.ns) {
+      W.nitems = R.nitems;
+      W.nmerge = R.dense ? R.nheavy : ((pass == PASS_NUMERIC && hp[j] && hp[j]->pool) ? R.nrows : 0);
+      works.push_back(W);
+      continue;
+    }
+    if (sp.nd || R.nheavy) {
+      W.params.items = sp.dense;
+      W.nitems = sp.nd;
+      W.nmerge = R.nheavy;
+      works.push_back(W);
+    }
+    Work H = W;
+    H.dense = false;
+    H.col64 = false;
+    H.threads = SPARSE_THREADS;
+    H.lds_slots = SPARSE_SLOTS;
+    H.params.items = sp.sparse;
+    H.params.bcol = R.HB->rest_key32;  // 32-bit column keys (the dense share streams 16-bit ones)
+    H.params.row_hcap = sp.hcap;
+    H.params.row_gcap = sp.zero;
+    H.params.row_goff = sp.zero;
+    H.params.gkeys = nullptr;
+    H.params.gvals = nullptr;
+    H.params.merge_rows = nullptr;
+    H.nitems = sp.ns;
+    H.nmerge = 0;
+    H.weight = 0.0;
+    works.push_back(H);
+  }
+  return works;
 }
 
 int launch_bins(ptc_batch* b, int pass, std::vector<Bin>& bins, std::vector<FillSeg>* fills) {
@@ -346,6 +446,39 @@ int plan_rest(ptc_batch* b) {
     b->st.flops += R.F;
     b->st.queue_items += (uint32_t)R.nitems;
   }
+  // ---- small rows of wide dense jobs to LDS hash shares: item lists split
+  // on the device, their sizes read back once
+  b->split.assign(n, Split());
+  {
+    std::vector<const unsigned long long*> csrc(n, nullptr);
+    for (int j = 0; j < n; ++j) {
+      Run& R = *b->runs[j];
+      if (!R.dense || R.ncols < SPLIT_MIN_COLS || R.col64 || !R.nitems) continue;
+      Split& sp = b->split[j];
+      sp.dense = b->ws->get<uint4>(R.nitems);
+      sp.sparse = b->ws->get<uint4>(R.nitems);
+      sp.hcap = b->ws->get<uint32_t>(R.nrows);
+      sp.zero = b->ws->get<unsigned long long>(R.nrows);
+      uint32_t* cnt = b->ws->get<uint32_t>(2);
+      if (!sp.dense || !sp.sparse || !sp.hcap || !sp.zero || !cnt) return PTC_ERR_NOMEM;
+      if ((rc = fill_zero(ctx, cnt, 8)) || (rc = fill_zero(ctx, sp.zero, R.nrows * 8))) return rc;
+      const unsigned gi = (unsigned)std::min<uint64_t>((R.nitems + 255) / 256, 4096);
+      const unsigned gr = (unsigned)std::min<uint64_t>((R.nrows + 255) / 256, 4096);
+      hipLaunchKernelGGL(k_split_items, dim3(gi), dim3(256), 0, ctx->stream, (const uint4*)R.items, R.nitems,
+                         (const unsigned long long*)R.row_flops, (const uint32_t*)R.row_nchunk, sp.dense, sp.sparse, cnt);
+      hipLaunchKernelGGL(k_sparse_caps, dim3(gr), dim3(256), 0, ctx->stream, (const unsigned long long*)R.row_flops,
+                         R.nrows, sp.hcap);
+      PTC_HIP(ctx, hipGetLastError());
+      csrc[j] = (const unsigned long long*)cnt;  // (dense, sparse) as one 64-bit word
+    }
+    std::vector<unsigned long long> cnts;
+    if ((rc = read_words64(b, csrc, cnts))) return rc;
+    for (int j = 0; j < n; ++j) {
+      if (!csrc[j]) continue;
+      b->split[j].nd = (uint32_t)cnts[j];
+      b->split[j].ns = (uint32_t)(cnts[j] >> 32);
+    }
+  }
   // ---- symbolic: HBM pools of hash jobs in one read-back, then one
   // persistent launch per bin
   std::vector<HashPlan> hs(n);
@@ -377,7 +510,9 @@ int plan_rest(ptc_batch* b) {
     params[j].merge_rows = R.dense ? R.heavy_rows : nullptr;
   }
   std::vector<Bin> sbins;
-  if ((rc = make_bins(b, PASS_SYMBOLIC, params, hptr, sbins)) || (rc = reset_heads(b, sbins))) return rc;
+  if ((rc = make_bins(b, PASS_SYMBOLIC, make_works(b, PASS_SYMBOLIC, params, hptr), sbins)) ||
+      (rc = reset_heads(b, sbins)))
+    return rc;
   stats_mark(b, 6);
   if ((rc = launch_bins(b, PASS_SYMBOLIC, sbins, nullptr))) return rc;
   stats_mark(b, 7);
@@ -444,7 +579,7 @@ int plan_rest(ptc_batch* b) {
       rtotal += t->t->N;
     }
   }
-  if ((rc = make_bins(b, PASS_NUMERIC, params, hptr, b->bins))) return rc;
+  if ((rc = make_bins(b, PASS_NUMERIC, make_works(b, PASS_NUMERIC, params, hptr), b->bins))) return rc;
   for (Bin& bn : b->bins) fills.push_back(FillSeg{bn.d_head, 2, 0ull, 0});
   RefreshSeg* d_rseg = upload_vec(b, rseg);
   unsigned long long* d_rstart = upload_vec(b, rstart);
