@@ -113,7 +113,8 @@ struct Bin {
   unsigned grid = 0, mgrid = 0;      // device-filling grids (main, merge)
 };
 
-// Rows of a dense job that are small against its column space: at most
+// Rows of a dense job that are small against its column space (or of a hash
+// job, whose LDS tables are sized for its largest rows): at most
 // SPARSE_FLOPS multiply-adds (so at most that many distinct columns) in a
 // column space of at least SPLIT_MIN_COLS.  Their items run in an LDS hash
 // bin of SPARSE_THREADS-thread workgroups with tables of at most
@@ -155,6 +156,7 @@ struct Work {
   int job = 0;
   bool dense = true, col64 = false;
   uint32_t threads = PB, lds_slots = 0;
+  uint64_t ncols = 0;   // accumulator columns (dense work)
   PassParams params{};
   uint64_t nitems = 0, nmerge = 0;
   double weight = 0.0;  // multiply-adds per item (queue order)
@@ -167,6 +169,27 @@ struct Split {
   uint4* sparse = nullptr;
   uint32_t* hcap = nullptr;    // LDS hash slots of every row
   unsigned long long* zero = nullptr;  // no row of the share lives in HBM
+};
+
+// A hash job whose column space is at most MAX_WINDOWS dense accumulators
+// wide (a 16-bit column space): its large rows run as dense column windows
+// -- each window's workgroups stream the row's whole flop range and
+// accumulate the columns in [lo, lo + w) in LDS -- instead of LDS or HBM hash
+// tables, and its small rows in the sparse share.  Every window (and the
+// sparse share) counts its own entries per row and writes its own part of
+// the job's output.
+constexpr uint32_t MAX_WINDOWS = 4;
+struct Win {
+  uint32_t T = 0;  // windows (0: not windowed)
+  uint32_t threads = PB;
+  uint32_t lo[MAX_WINDOWS] = {}, w[MAX_WINDOWS] = {};
+  const uint16_t* col16 = nullptr;
+  uint32_t* row_nnz[MAX_WINDOWS + 1] = {};  // [T]: the sparse share
+  unsigned long long* out_off[MAX_WINDOWS + 1] = {};
+  uint32_t* hbits[MAX_WINDOWS] = {};
+  double* hacc[MAX_WINDOWS] = {};
+  unsigned char* hflag[MAX_WINDOWS] = {};
+  unsigned long long base[MAX_WINDOWS + 1] = {};  // first output entry of each part
 };
 
 // LDS footprint classes: a bin's launch opens the class size, so jobs of a
@@ -194,6 +217,7 @@ struct ptc_batch {
   std::vector<ptc_dtensor*> out;
   std::vector<Bin> bins;
   std::vector<Split> split;          // per job (ns = 0: not split)
+  std::vector<Win> win;              // per job (T = 0: not windowed)
   Workspace* ws = nullptr;           // batch-level device arrays
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -253,7 +277,8 @@ int make_bins(ptc_batch* b, int pass, const std::vector<Work>& works, std::vecto
   for (int w = 0; w < (int)works.size(); ++w) {
     const Work& W = works[w];
     const Run& R = *b->runs[W.job];
-    const size_t need = pass_lds_bytes(pass, W.dense, W.col64, R.ncols, W.lds_slots, R.use_flags, W.threads);
+    const size_t need = pass_lds_bytes(pass, W.dense, W.col64, W.ncols ? W.ncols : R.ncols, W.lds_slots, R.use_flags,
+                                       W.threads);
     const size_t lds = std::max(need, std::min<size_t>(lds_class(need), lds_limit(b->ctx)));
     auto k = std::make_tuple(W.dense, W.col64, lds, W.threads);
     auto it = key.find(k);
@@ -312,16 +337,53 @@ std::vector<Work> make_works(ptc_batch* b, int pass, const std::vector<PassParam
     W.lds_slots = R.lds_slots;
     W.params = params[j];
     W.weight = (double)R.F / std::max<uint64_t>(R.nitems, 1);
-    if (!sp.ns) {
+    const Win& wn = b->win[j];
+    if (wn.T) {
+      // dense column windows over the large rows
+      for (uint32_t t = 0; t < wn.T && (sp.nd || R.nheavy); ++t) {
+        Work D = W;
+        D.dense = true;
+        D.col64 = false;
+        D.threads = wn.threads;
+        D.lds_slots = 0;
+        D.ncols = wn.w[t];
+        PassParams& P = D.params;
+        P.items = sp.dense;
+        P.bcol = wn.col16;
+        P.ncols = wn.w[t];
+        P.col_lo = wn.lo[t];
+        P.col_win = 1;
+        P.row_nnz = wn.row_nnz[t];
+        P.merge_rows = R.heavy_rows;
+        P.row_hcap = nullptr;
+        P.row_gcap = nullptr;
+        P.row_goff = nullptr;
+        P.gkeys = nullptr;
+        P.gvals = nullptr;
+        if (pass == PASS_SYMBOLIC) {
+          P.hbits = wn.hbits[t];
+        } else {
+          P.hacc = wn.hacc[t];
+          P.hflag = wn.hflag[t];
+          P.out_base = wn.out_off[t];
+          P.out_hedges = b->out[j]->hedges + wn.base[t] * R.dO;
+          P.out_vals = b->out[j]->vals + wn.base[t];
+        }
+        D.nitems = sp.nd;
+        D.nmerge = R.nheavy;
+        works.push_back(D);
+      }
+    }
+    if (!sp.ns && !wn.T) {
       W.nitems = R.nitems;
       W.nmerge = R.dense ? R.nheavy : ((pass == PASS_NUMERIC && hp[j] && hp[j]->pool) ? R.nrows : 0);
       works.push_back(W);
       continue;
     }
-    if (sp.nd || R.nheavy) {
+    if (!wn.T && (sp.nd || R.nheavy)) {
       W.params.items = sp.dense;
       W.nitems = sp.nd;
-      W.nmerge = R.nheavy;
+      W.nmerge = R.dense ? R.nheavy : ((pass == PASS_NUMERIC && hp[j] && hp[j]->pool) ? R.nrows : 0);
       works.push_back(W);
     }
     Work H = W;
@@ -340,7 +402,15 @@ std::vector<Work> make_works(ptc_batch* b, int pass, const std::vector<PassParam
     H.nitems = sp.ns;
     H.nmerge = 0;
     H.weight = 0.0;
-    works.push_back(H);
+    if (wn.T) {  // the sparse share writes its own part of a windowed job's output
+      H.params.row_nnz = wn.row_nnz[wn.T];
+      if (pass == PASS_NUMERIC) {
+        H.params.out_base = wn.out_off[wn.T];
+        H.params.out_hedges = b->out[j]->hedges + wn.base[wn.T] * R.dO;
+        H.params.out_vals = b->out[j]->vals + wn.base[wn.T];
+      }
+    }
+    if (sp.ns) works.push_back(H);
   }
   return works;
 }
@@ -453,7 +523,9 @@ int plan_rest(ptc_batch* b) {
     std::vector<const unsigned long long*> csrc(n, nullptr);
     for (int j = 0; j < n; ++j) {
       Run& R = *b->runs[j];
-      if (!R.dense || R.ncols < SPLIT_MIN_COLS || R.col64 || !R.nitems) continue;
+      // dense jobs in wide column spaces, and every hash job (whose LDS
+      // tables are sized for its largest rows)
+      if ((R.dense && R.ncols < SPLIT_MIN_COLS) || R.col64 || !R.nitems) continue;
       Split& sp = b->split[j];
       sp.dense = b->ws->get<uint4>(R.nitems);
       sp.sparse = b->ws->get<uint4>(R.nitems);
@@ -479,13 +551,45 @@ int plan_rest(ptc_batch* b) {
       b->split[j].ns = (uint32_t)(cnts[j] >> 32);
     }
   }
+  // ---- dense column windows for hash jobs in 16-bit column spaces
+  b->win.assign(n, Win());
+  {
+    const uint64_t lds_cap = lds_limit(ctx);
+    uint64_t wfit = 0;  // widest dense window (touched bits kept) at 512 threads
+    while (wfit < 65536 && pass_lds_bytes(PASS_NUMERIC, true, false, wfit + 256, 0, true, 512) <= lds_cap) wfit += 256;
+    for (int j = 0; j < n && wfit; ++j) {
+      Run& R = *b->runs[j];
+      const Split& sp = b->split[j];
+      if (R.dense || R.col64 || R.ncols > 65536 || !sp.dense) continue;
+      const uint64_t T = (R.ncols + wfit - 1) / wfit;
+      if (T > MAX_WINDOWS) continue;
+      Win& wn = b->win[j];
+      if ((rc = hashtable_col16(ctx, R.HB, &wn.col16))) return rc;
+      wn.T = (uint32_t)T;
+      const uint64_t width = ((R.ncols + T - 1) / T + 63) / 64 * 64;
+      uint64_t wmax = 0;
+      for (uint32_t t = 0; t < wn.T; ++t) {
+        wn.lo[t] = (uint32_t)(t * width);
+        wn.w[t] = (uint32_t)std::min<uint64_t>(width, R.ncols - t * width);
+        wmax = std::max<uint64_t>(wmax, wn.w[t]);
+      }
+      const size_t at512 = pass_lds_bytes(PASS_NUMERIC, true, false, wmax, 0, true, 512);
+      wn.threads = (2 * at512 > lds_cap && pass_lds_bytes(PASS_NUMERIC, true, false, wmax, 0, true, 1024) <= lds_cap)
+                       ? 1024u : 512u;
+      for (uint32_t x = 0; x <= wn.T; ++x) {
+        wn.row_nnz[x] = b->ws->get<uint32_t>(R.nrows + 1);
+        wn.out_off[x] = b->ws->get<unsigned long long>(R.nrows + 1);
+        if (!wn.row_nnz[x] || !wn.out_off[x]) return PTC_ERR_NOMEM;
+      }
+    }
+  }
   // ---- symbolic: HBM pools of hash jobs in one read-back, then one
   // persistent launch per bin
   std::vector<HashPlan> hs(n);
   std::vector<const unsigned long long*> src(n, nullptr);
   for (int j = 0; j < n; ++j) {
     Run& R = *b->runs[j];
-    if (!R.dense) {
+    if (!R.dense && !b->win[j].T) {
       if ((rc = plan_hash_launch(R, hs[j], nullptr, R.row_flops, nullptr))) return rc;
       src[j] = hs[j].gcap + R.nrows;
     }
@@ -496,13 +600,21 @@ int plan_rest(ptc_batch* b) {
   std::vector<HashPlan*> hptr(n, nullptr);
   for (int j = 0; j < n; ++j) {
     Run& R = *b->runs[j];
-    if (!R.dense) {
+    Win& wn = b->win[j];
+    if (!R.dense && !wn.T) {
       if ((rc = plan_hash_finish(R, hs[j], pools[j])) || (rc = reset_hash_pool(R, hs[j], false))) return rc;
       hptr[j] = &hs[j];
     }
     if ((rc = fill_zero(ctx, R.row_nnz, (R.nrows + 1) * 4)) || (rc = prepare_heavy(R, PASS_SYMBOLIC))) return rc;
+    for (uint32_t x = 0; wn.T && x <= wn.T; ++x) {
+      if ((rc = fill_zero(ctx, wn.row_nnz[x], (R.nrows + 1) * 4))) return rc;
+      if (x < wn.T && R.nheavy) {  // symbolic slabs of split rows: one bitmap of the window per chunk
+        wn.hbits[x] = b->ws->get<uint32_t>(R.nslabs * ((wn.w[x] + 31) / 32));
+        if (!wn.hbits[x]) return PTC_ERR_NOMEM;
+      }
+    }
     base_params(R, params[j], R.dense ? nullptr : &hs[j]);
-    if (!R.dense) {
+    if (!R.dense && !wn.T) {
       unsigned long long* flags = nullptr;
       if ((rc = hbm_flags(R, hs[j], &flags))) return rc;
       params[j].row_gcap = flags;
@@ -527,6 +639,30 @@ int plan_rest(ptc_batch* b) {
   }
   std::vector<unsigned long long> nnz, sflags;
   if ((rc = read_words64(b, src, nnz)) || (rc = read_words64(b, fsrc, sflags))) return rc;
+  {  // windowed jobs: each part's size, read back together; parts laid end to end
+    std::vector<const unsigned long long*> wsrc;
+    for (int j = 0; j < n; ++j) {
+      Run& R = *b->runs[j];
+      const Win& wn = b->win[j];
+      for (uint32_t x = 0; wn.T && x <= wn.T; ++x) {
+        if ((rc = scan_u32_to_u64(ctx, wn.row_nnz[x], wn.out_off[x], R.nrows, wn.out_off[x] + R.nrows))) return rc;
+        wsrc.push_back(wn.out_off[x] + R.nrows);
+      }
+    }
+    std::vector<unsigned long long> wtot;
+    if ((rc = read_words64(b, wsrc, wtot))) return rc;
+    size_t k = 0;
+    for (int j = 0; j < n; ++j) {
+      Win& wn = b->win[j];
+      if (!wn.T) continue;
+      unsigned long long run = 0;
+      for (uint32_t x = 0; x <= wn.T; ++x) {
+        wn.base[x] = run;
+        run += wtot[k++];
+      }
+      nnz[j] = run;
+    }
+  }
   for (int j = 0; j < n; ++j)
     if ((uint32_t)sflags[j] & 0xC0000000u)
       return set_err(ctx, PTC_ERR_INTERNAL, "job %d: symbolic pass consistency flags %08x", j, (uint32_t)sflags[j]);
@@ -538,7 +674,7 @@ int plan_rest(ptc_batch* b) {
   for (int j = 0; j < n; ++j) {
     Run& R = *b->runs[j];
     src[j] = nullptr;
-    if (!R.dense) {
+    if (!R.dense && !b->win[j].T) {
       if ((rc = plan_hash_launch(R, b->hn[j], R.row_nnz, nullptr, nullptr))) return rc;
       src[j] = b->hn[j].gcap + R.nrows;
     }
@@ -550,8 +686,14 @@ int plan_rest(ptc_batch* b) {
   std::vector<FillSeg> fills;
   for (int j = 0; j < n; ++j) {
     Run& R = *b->runs[j];
+    Win& wn = b->win[j];
     hptr[j] = nullptr;
-    if (!R.dense) {
+    for (uint32_t x = 0; x < wn.T && R.nheavy; ++x) {  // numeric slabs of split rows per window
+      wn.hacc[x] = b->ws->get<double>(R.nslabs * wn.w[x]);
+      wn.hflag[x] = b->ws->get<unsigned char>(R.nslabs * wn.w[x]);
+      if (!wn.hacc[x] || !wn.hflag[x]) return PTC_ERR_NOMEM;
+    }
+    if (!R.dense && !wn.T) {
       if ((rc = plan_hash_finish(R, b->hn[j], pools[j]))) return rc;
       hptr[j] = &b->hn[j];
       if (b->hn[j].pool) {
@@ -561,7 +703,7 @@ int plan_rest(ptc_batch* b) {
     }
     if ((rc = prepare_heavy(R, PASS_NUMERIC))) return rc;
     base_params(R, params[j], R.dense ? nullptr : &b->hn[j]);
-    if (!R.dense) {
+    if (!R.dense && !wn.T) {
       unsigned long long* flags = nullptr;
       if ((rc = hbm_flags(R, b->hn[j], &flags))) return rc;
       params[j].row_gcap = flags;

@@ -151,6 +151,9 @@ struct CapParams {
   int f32; /* FP32 values in, (hi, lo) FP32 pairs out */                                      \
   int val; /* ptc_value_type: accumulation mode of the numeric pass */                        \
   int col_first; /* 1: an output entry lists its column modes, then its row modes */          \
+  /* dense column window: the accumulator holds columns [col_lo, col_lo + ncols) and the */  \
+  /* row's other columns are left to other windows (col_win = 1) */                          \
+  uint32_t col_lo, col_win;                                                                   \
   uint32_t dO, nrowmodes;                                                                     \
   KeyCodec rowc, colc;
 

@@ -787,6 +787,10 @@ __device__ __forceinline__ void pass_item(const PP& P, uint32_t widx) {
             }
           }
         }
+        if (DENSE && P.col_lo) {  // a column window: window-local columns (others wrap out of range)
+#pragma unroll
+          for (int u = 0; u < U; ++u) col[u] = (ColT)((uint32_t)col[u] - P.col_lo);
+        }
         if constexpr (F32 && DENSE) {
           // all U sum updates first (a returning atomic each when
           // compensating), then the U compensation updates: one wait on LDS
@@ -796,7 +800,7 @@ __device__ __forceinline__ void pass_item(const PP& P, uint32_t widx) {
 #pragma unroll
           for (int u = 0; u < U; ++u) {
             ok[u] = act[u] && (uint32_t)col[u] < ncols32;
-            bad |= act[u] && !ok[u];
+            bad |= act[u] && !ok[u] && !P.col_win;
             o[u] = 0.f;
             pr[u] = 0.f;
             ep[u] = 0.f;
@@ -827,7 +831,7 @@ __device__ __forceinline__ void pass_item(const PP& P, uint32_t widx) {
           const ColT c = col[u];
           if (DENSE) {
             const bool ok = act[u] && (uint32_t)c < ncols32;
-            bad |= act[u] && !ok;
+            bad |= act[u] && !ok && !P.col_win;  // a windowed row's other columns belong to other windows
             if (!ok) continue;
             if (PASS == PASS_SYMBOLIC) {
               atomicOr(&bits[c >> 5], 1u << (c & 31));
@@ -1005,7 +1009,7 @@ __device__ __forceinline__ void pass_item(const PP& P, uint32_t widx) {
       double v = 0.0;
       float fs = 0.f, fc = 0.f;
       if (DENSE) {
-        col = (ColT)c;
+        col = (ColT)(c + P.col_lo);
         if (F32) {
           fs = acc_s[c];
           fc = acc_c[c];

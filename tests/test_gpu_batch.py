@@ -27,8 +27,9 @@ def meta_self(h, v, contracted):
 def job_inputs():
     """Uneven jobs: shapes of the batched workload family (nnz capped so the
     oracle stays fast), plus a dense job with rows split over many
-    workgroups and hash-accumulator jobs (column spaces above the dense
-    limit)."""
+    workgroups and jobs with column spaces above the dense limit (their
+    large rows run as dense column windows -- three for 40000 columns --
+    and their small rows in LDS hash tables)."""
     jobs = []
     for i in range(10):
         d, dims, nnz = P.gen_job_shape(1000 + i)
@@ -41,6 +42,8 @@ def job_inputs():
     jobs.append((h, v, [1, 2]))
     h, v = P.gen_coo(43, [9000, 20, 30, 7], 15000, P.DIST_ZIPF)
     jobs.append((h, v, [1, 2, 3]))
+    h, v = P.gen_coo(46, [40000, 30, 40], 40000, P.DIST_ZIPF)  # 40000 columns: three dense column windows
+    jobs.append((h, v, [1, 2]))
     return jobs
 
 
