@@ -197,8 +197,10 @@ int launch_row_features(ptc_ctx* ctx, const uint32_t* row_of_pos, uint64_t npos,
 int launch_items(ptc_ctx* ctx, const ItemParams& P, const ItemP0Params& Q);
 int launch_hash_caps(ptc_ctx* ctx, const HashCapParams& P);
 int launch_caps(ptc_ctx* ctx, const CapParams& P);
+// val: the numeric pass's ptc_value_type (FP32 accumulators take 12 bytes a
+// dense column: an FP32 sum and an FP64 compensation)
 size_t pass_lds_bytes(int pass, bool dense, bool col64, uint64_t ncols, uint32_t lds_slots, bool flags = true,
-                      uint32_t threads = PB);
+                      uint32_t threads = PB, int val = 0);
 int launch_pass(ptc_ctx* ctx, int pass, bool dense, bool col64, const PassParams& P, uint64_t nitems,
                 size_t lds, bool merge = false, uint32_t threads = PB);
 int launch_emit(ptc_ctx* ctx, const EmitParams& P, bool col64);

@@ -440,17 +440,34 @@ __device__ __forceinline__ void acc_f32_hbm(float* s, float* c, float a, float b
     unsafeAtomicAdd(s, p);
   }
 }
-// (s, c) -> (hi, lo): hi = fl(s + c), lo = the rounding error of that add
+// A dense row's update: the sum word as above (FP32, a returning atomic, the
+// exact error of that very update recomputed locally), the compensation --
+// the TwoSum and TwoProduct errors -- collected in an FP64 word: gfx950's
+// FP64 LDS atomic add runs several times faster than its FP32 one, so the
+// second atomic of every update costs least there.
 template <bool COMP>
-__device__ __forceinline__ void f32_finish(float s, float c, float* hi, float* lo) {
+__device__ __forceinline__ void acc_f32_lds_c64(float* s, double* c, float a, float b) {
+  const float p = a * b;
+  if (COMP) {
+    const float o = atomicAdd(s, p);
+    const float e = two_sum_err(o, p, o + p) + fmaf(a, b, -p);
+    if (e != 0.f) atomicAdd(c, (double)e);
+  } else {
+    atomicAdd(s, p);
+  }
+}
+// (s, c) -> (hi, lo): hi = fl32(s + c), lo = what that rounding left out
+template <bool COMP>
+__device__ __forceinline__ void f32_finish(float s, double c, float* hi, float* lo) {
   if (!COMP) {
     *hi = s;
     *lo = 0.f;
     return;
   }
-  const float h = s + c;
+  const double t = (double)s + c;
+  const float h = (float)t;
   *hi = h;
-  *lo = two_sum_err(s, c, h);
+  *lo = (float)(t - (double)h);
 }
 
 // Orders one wave's LDS accesses around an exchange between its lanes (no
@@ -535,11 +552,12 @@ __device__ __forceinline__ void pass_item(const PP& P, uint32_t widx) {
 
   // accumulator storage
   double* acc = (double*)acc_base;
-  float* acc_s = (float*)acc_base;  // FP32: sums [ncols], then compensations [ncols]
-  float* acc_c = acc_s + ncols;
+  // FP32 accumulators: compensations in FP64 [ncols], then the FP32 sums [ncols]
+  double* acc_c64 = (double*)acc_base;
+  float* acc_s = (float*)(acc_base + ncols * 8);
   // touched columns (when values may give zero products): one bit per
   // column, set by an LDS atomic OR; the finish skips empty 64-column tiles
-  uint32_t* fbits = (uint32_t*)(acc_base + ncols * 8);
+  uint32_t* fbits = (uint32_t*)(acc_base + ncols * (F32 ? 12 : 8));
   const uint32_t nfw = (uint32_t)((ncols + 31) / 32);
   uint32_t* bits = (uint32_t*)acc_base;
   ColT* hkeys = (ColT*)acc_base;
@@ -566,19 +584,20 @@ __device__ __forceinline__ void pass_item(const PP& P, uint32_t widx) {
       // slabs hold (sum, compensation) pairs; combine them in chunk order
       const float2* h2 = (const float2*)P.hacc;
       for (uint64_t c = threadIdx.x; c < ncols; c += NT) {
-        float S = -0.0f, Cc = 0.f;
+        float S = -0.0f;
+        double Cc = 0.0;
         unsigned char fl = 0;
 #pragma unroll 4
         for (uint32_t kk = 0; kk < nch; ++kk) {
           const float2 x = h2[(slab0 + kk) * ncols + c];
           const float sn = S + x.x;
-          if (COMP) Cc += x.y + two_sum_err(S, x.x, sn);
+          if (COMP) Cc += (double)x.y + (double)two_sum_err(S, x.x, sn);
           S = sn;
         }
         if (flags)
           for (uint32_t j = 0; j < nch; ++j) fl |= P.hflag[(slab0 + j) * ncols + c];
         acc_s[c] = S;
-        acc_c[c] = Cc;
+        acc_c64[c] = Cc;
         if (fl) atomicOr(&fbits[c >> 5], 1u << (c & 31));
       }
     } else {
@@ -607,7 +626,7 @@ __device__ __forceinline__ void pass_item(const PP& P, uint32_t widx) {
     } else if (F32) {
       for (uint64_t c = threadIdx.x; c < ncols; c += NT) {
         acc_s[c] = -0.0f;
-        acc_c[c] = 0.f;
+        acc_c64[c] = 0.0;
       }
       if (flags)
         for (uint32_t w = threadIdx.x; w < nfw; w += NT) fbits[w] = 0;
@@ -821,7 +840,7 @@ __device__ __forceinline__ void pass_item(const PP& P, uint32_t widx) {
             for (int u = 0; u < U; ++u) {
               if (!ok[u]) continue;
               const float e = two_sum_err(o[u], pr[u], o[u] + pr[u]) + ep[u];
-              if (e != 0.f) atomicAdd(&acc_c[col[u]], e);
+              if (e != 0.f) atomicAdd(&acc_c64[col[u]], (double)e);
             }
           }
           continue;
@@ -836,7 +855,7 @@ __device__ __forceinline__ void pass_item(const PP& P, uint32_t widx) {
             if (PASS == PASS_SYMBOLIC) {
               atomicOr(&bits[c >> 5], 1u << (c & 31));
             } else {
-              if (F32) acc_f32_lds<COMP>(&acc_s[c], &acc_c[c], s_aval32[ii[u]], bv32[u]);
+              if (F32) acc_f32_lds_c64<COMP>(&acc_s[c], &acc_c64[c], s_aval32[ii[u]], bv32[u]);
               else if (F32IN) atomicAdd(&acc[c], (double)s_aval32[ii[u]] * (double)bv32[u]);
               else atomicAdd(&acc[c], s_aval[ii[u]] * bv[u]);
               if (flags) atomicOr(&fbits[(uint32_t)c >> 5], 1u << ((uint32_t)c & 31));
@@ -883,7 +902,8 @@ __device__ __forceinline__ void pass_item(const PP& P, uint32_t widx) {
         for (uint32_t w = threadIdx.x; w < nbits; w += NT) gb[w] = bits[w];
       } else if (F32) {
         float2* ga = (float2*)P.hacc + slab * ncols;
-        for (uint64_t c = threadIdx.x; c < ncols; c += NT) ga[c] = make_float2(acc_s[c], acc_c[c]);
+        // the compensation rounded to FP32 (a second-order term of the chunk's sum)
+        for (uint64_t c = threadIdx.x; c < ncols; c += NT) ga[c] = make_float2(acc_s[c], (float)acc_c64[c]);
         if (flags) {
           unsigned char* gf = P.hflag + slab * ncols;
           for (uint64_t c = threadIdx.x; c < ncols; c += NT) gf[c] = (unsigned char)((fbits[c >> 5] >> (c & 31)) & 1u);
@@ -1007,12 +1027,13 @@ __device__ __forceinline__ void pass_item(const PP& P, uint32_t widx) {
     if (f && run + rank < limit) {
       ColT col = 0;
       double v = 0.0;
-      float fs = 0.f, fc = 0.f;
+      float fs = 0.f;
+      double fc = 0.0;
       if (DENSE) {
         col = (ColT)(c + P.col_lo);
         if (F32) {
           fs = acc_s[c];
-          fc = acc_c[c];
+          fc = acc_c64[c];
         } else {
           v = acc[c];
         }
@@ -1153,11 +1174,11 @@ int launch_hash_caps(ptc_ctx* ctx, const HashCapParams& P) {
 }
 
 size_t pass_lds_bytes(int pass, bool dense, bool col64, uint64_t ncols, uint32_t lds_slots, bool flags,
-                      uint32_t threads) {
+                      uint32_t threads, int val) {
   size_t b = (size_t)2 * threads * 16 + threads;  // staging: 2 positions per thread, 64 flags per wave
   if (dense) {
     if (pass == PASS_SYMBOLIC) b += ((ncols + 31) / 32) * 4;
-    else b += ncols * 8 + (flags ? ((ncols + 31) / 32) * 4 : 0);  // accumulator (+ touched bits)
+    else b += ncols * ((val == 1 || val == 2) ? 12 : 8) + (flags ? ((ncols + 31) / 32) * 4 : 0);  // accumulator (+ touched bits)
   } else {
     b += (size_t)lds_slots * ((col64 ? 8 : 4) + (pass == PASS_NUMERIC ? 8 : 0));
   }

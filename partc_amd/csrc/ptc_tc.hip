@@ -197,7 +197,8 @@ static int run_pass(Run& R, int pass, HashPlan* hp, const uint4* items, uint64_t
     P.row_gcap = flags;
   }
   P.items = items;
-  size_t lds = pass_lds_bytes(pass, R.dense, R.col64, R.ncols, R.lds_slots, R.use_flags, R.threads);
+  size_t lds = pass_lds_bytes(pass, R.dense, R.col64, R.ncols, R.lds_slots, R.use_flags, R.threads,
+                              pass == PASS_NUMERIC ? R.val : 0);
   // one pair of timing events brackets the row launch and its merge launch
   ptc_ctx* ctx = R.ctx;
   const bool timed = ctx->nkev + 2 <= 16;
@@ -330,7 +331,8 @@ int prepare_launch(Run& R, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B,
   // largest dense column space whose numeric pass fits the workgroup's LDS
   const uint64_t lds_cap = lds_limit(ctx);
   uint64_t dense_fit = 0;
-  while (dense_fit < 65536 && pass_lds_bytes(PASS_NUMERIC, true, false, dense_fit + 256, 0) <= lds_cap) dense_fit += 256;
+  while (dense_fit < 65536 && pass_lds_bytes(PASS_NUMERIC, true, false, dense_fit + 256, 0, true, PB, R.val) <= lds_cap)
+    dense_fit += 256;
   uint64_t dense_max = opt.dense_max_cols ? opt.dense_max_cols : DEFAULT_DENSE_MAX;
   dense_max = std::min<uint64_t>(dense_max, dense_fit);
   R.dense = opt.acc_mode == PTC_ACC_DENSE ? R.ncols <= dense_fit
@@ -349,10 +351,10 @@ int prepare_launch(Run& R, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B,
   // fewer than two per CU, 1024 threads keep twice the waves in flight on
   // the same LDS.
   {
-    const size_t at512 = pass_lds_bytes(PASS_NUMERIC, R.dense, R.col64, R.ncols, R.lds_slots, R.use_flags, 512);
+    const size_t at512 = pass_lds_bytes(PASS_NUMERIC, R.dense, R.col64, R.ncols, R.lds_slots, R.use_flags, 512, R.val);
     uint32_t t = opt.block_threads ? opt.block_threads : (2 * at512 > lds_cap ? 1024u : 512u);
     if (t < 64 || t > (uint32_t)PB_MAX || t % 64) return set_err(ctx, PTC_ERR_ARG, "block_threads %u", t);
-    while (t > 512 && pass_lds_bytes(PASS_NUMERIC, R.dense, R.col64, R.ncols, R.lds_slots, R.use_flags, t) > lds_cap)
+    while (t > 512 && pass_lds_bytes(PASS_NUMERIC, R.dense, R.col64, R.ncols, R.lds_slots, R.use_flags, t, R.val) > lds_cap)
       t /= 2;
     R.threads = t;
   }

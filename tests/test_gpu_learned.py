@@ -125,3 +125,37 @@ def test_learned_vs_oracle_with_fallback(ctx, trained, scale, shape):
     assert max_rel(v1, ov) <= TOL
     if scale <= 0.01:
         assert st.fallback_rows > 0
+
+
+@pytest.mark.parametrize("shape", [([3000, 40, 30, 20], 60000, [1, 2, 3]), ([12000, 30, 40], 20000, [1, 2])])
+def test_learned_allocates_less_than_upper_bound(ctx, shape):
+    """What the network gains over the method without it (PTC_METHOD_UPPER_BOUND,
+    capacity min(flops, columns) per row): on a contraction whose rows hold
+    far fewer distinct columns than multiply-adds, the learned capacities
+    size the output arrays to about the true count instead of the flop
+    bound, with the same result (the built-in model, trained on other
+    tensors).  On dense outputs (the nips-shape and two-tensor workloads) the
+    two capacities coincide, and the learned method only adds its features
+    and inference."""
+    if not P.learned_available():
+        pytest.skip("no trained row-size model in partc_amd/models")
+    dims, nnz, contracted = shape
+    h, v = P.gen_coo(42, dims, nnz, P.DIST_ZIPF)
+    A, B = P.self_contraction_inputs(h, v, contracted)
+    print(f"[case] learned vs upper bound capacity {dims} {nnz}", flush=True)
+    res = {}
+    for name, m in (("upper_bound", P.METHOD_UPPER_BOUND), ("learned", P.METHOD_LEARNED)):
+        st = P.Stats()
+        O = P.tc_fksCuckoo(P.buildHashTable(A, ctx), P.buildHashTable(B, ctx), stats=st, method=m)
+        res[name] = (sort_coo(O.hedges_array, O.val_array), st.out_capacity, st.nnz_out, st.fallback_rows, st.rows)
+    (hu, vu), cap_u, n_u, _, _ = res["upper_bound"]
+    (hl, vl), cap_l, n_l, fb, rows = res["learned"]
+    oh, ov, _ = orc.tc_fksCuckoo(A, B)
+    oh, ov = sort_coo(oh, ov)
+    assert np.array_equal(hu, oh) and np.array_equal(hl, oh)
+    assert max_rel(vu, ov) <= TOL and max_rel(vl, ov) <= TOL
+    print(f"  nnz {n_l}: capacity learned {cap_l} ({cap_l / n_l:.2f}x), upper bound {cap_u} ({cap_u / n_u:.2f}x); "
+          f"fallback rows {fb} of {rows}")
+    assert cap_l <= 1.3 * n_l          # about the true count ...
+    assert cap_l <= 0.85 * cap_u       # ... and well below the flop bound
+    assert fb <= 0.05 * rows
