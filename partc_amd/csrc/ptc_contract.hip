@@ -570,10 +570,9 @@ __device__ __forceinline__ void pass_item(const PP& P, uint32_t widx) {
   if (DENSE && MERGE) {
     // reduce the row's slabs in chunk order
     const uint64_t slab0 = P.row_slab[r];
-    if (PASS == PASS_NUMERIC && flags) {  // touched bits are OR-ed in below
-      for (uint32_t w = threadIdx.x; w < nfw; w += NT) fbits[w] = 0;
-      __syncthreads();
-    }
+    // touched bits: a wave reduces 64 consecutive columns per step (the
+    // columns of a step start at a multiple of 64), so each bitmap word is
+    // assembled by a ballot and stored whole by one lane -- no LDS atomics
     if (PASS == PASS_SYMBOLIC) {
       for (uint32_t w = threadIdx.x; w < nbits; w += NT) {
         uint32_t o = 0;
@@ -598,7 +597,10 @@ __device__ __forceinline__ void pass_item(const PP& P, uint32_t widx) {
           for (uint32_t j = 0; j < nch; ++j) fl |= P.hflag[(slab0 + j) * ncols + c];
         acc_s[c] = S;
         acc_c64[c] = Cc;
-        if (fl) atomicOr(&fbits[c >> 5], 1u << (c & 31));
+        if (flags) {
+          const unsigned long long m = __ballot(fl != 0);
+          if ((lane & 31) == 0) fbits[c >> 5] = (uint32_t)(m >> (lane & 32));
+        }
       }
     } else {
       for (uint64_t c = threadIdx.x; c < ncols; c += NT) {
@@ -617,7 +619,10 @@ __device__ __forceinline__ void pass_item(const PP& P, uint32_t widx) {
         if (flags)
           for (uint32_t j = 0; j < nch; ++j) fl |= P.hflag[(slab0 + j) * ncols + c];
         acc[c] = sum;
-        if (fl) atomicOr(&fbits[c >> 5], 1u << (c & 31));
+        if (flags) {
+          const unsigned long long m = __ballot(fl != 0);
+          if ((lane & 31) == 0) fbits[c >> 5] = (uint32_t)(m >> (lane & 32));
+        }
       }
     }
   } else if (DENSE) {
