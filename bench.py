@@ -406,6 +406,64 @@ def batch_plan_bench(P, args):
     ctx.close()
 
 
+README_WORKLOAD = ("nips-shape tensor (dims 2482x2862x14036x17, 3101609 unique nnz, Zipf(1.1), seed 2) contracted with "
+                   "itself over modes {1,2} (the reference README's 4d.meta): 4-mode output, rows and columns are "
+                   "(mode 0, mode 3) pairs, 42194 each")
+
+
+def readme_bench(P, args):
+    """The shape of the reference's one published run (README.md:25-38:
+    nips with 4d.meta, c = 2) on the synthetic nips-shape tensor: per-phase
+    device times at the reference's boundaries (HT_B build, HT_A build,
+    preprocessing, contraction) for each method, beside the oracle's phases
+    on the host -- its key tables in full, its contraction on a row sample
+    (the synthetic tensor's output is dense-ish, 4.8e8 nonzeros, so the
+    full CPU contraction takes minutes)."""
+    from oracle import orc
+
+    ctx = P.Context(0)
+    h, v = P.gen_coo(SEED, DIMS, NNZ, P.DIST_ZIPF, 1.1)
+    A, B = P.self_contraction_inputs(h, v, [1, 2])
+    dA, dB = P.DeviceTensor.upload(A, ctx), P.DeviceTensor.upload(B, ctx)
+    avail = {"two_pass": P.METHOD_TWO_PASS, "upper_bound": P.METHOD_UPPER_BOUND}
+    if P.learned_available():
+        avail["learned"] = P.METHOD_LEARNED
+    res = {}
+    for name, m in avail.items():
+        r = run_method(P, ctx, dA, dB, m, max(1, args.steps), args.warmup)
+        st = r["stats"]
+        res[name] = {"ms_per_step": round(r["ms_per_step"], 3), "ms_build_b": round(st["ms_build_b"], 3),
+                     "ms_build_a": round(st["ms_build_a"], 3), "ms_preprocess": round(st["ms_preprocess"], 3),
+                     "ms_symbolic": round(st["ms_symbolic"], 3), "ms_numeric": round(st["ms_numeric"], 3),
+                     "flops": int(st["flops"]), "nnz_out": int(st["nnz_out"]), "fallback_rows": int(st["fallback_rows"]),
+                     "spill_rows": int(st["spill_rows"]), "out_capacity": int(st["out_capacity"])}
+        print(f"# readme {name}: {r['ms_per_step']:.2f} ms/step", file=sys.stderr, flush=True)
+    F = next(iter(res.values()))["flops"]
+    out = {"metric": "useful multiply-adds/s, nips-shape tensor contracted over {1,2} (the reference README's run shape)",
+           "config": {"workload": README_WORKLOAD, "flops": F, "nnz_out": next(iter(res.values()))["nnz_out"]},
+           "methods": {k: dict(x, MADps=F / (x["ms_per_step"] * 1e-3)) for k, x in res.items()}}
+    if not args.no_cpu:
+        # the oracle on every 64th row (rows dealt by A's mode-0 and mode-3
+        # pair), its contraction time scaled by the flop share
+        rows = A.hedges_array[:, 0].astype(np.int64) * 17 + A.hedges_array[:, 3]
+        sel = rows % 64 == 0
+        As = P.Tensor(A.hedges_array[sel], A.val_array[sel], A.dimensions_array, A.dim_tc, A.num_key_dimensions)
+        t0 = time.perf_counter()
+        oh, ov, info = orc.tc_fksCuckoo(As, B)
+        wall = time.perf_counter() - t0
+        out["cpu_baseline"] = {
+            "kind": "port", "cores": 1, "unit": "MAD/s", "value": info["flops"] / info["t_contract"],
+            "sample": f"every 64th output row (A's mode-0/mode-3 pair index mod 64 == 0): {info['flops']} of {F} "
+                      f"multiply-adds; HT_B and HT_A built in full",
+            "phases_s": {"ht_b": round(info["t_build_b"], 3), "ht_a_sample": round(info["t_build_a"], 3),
+                         "preprocess_sample": round(info["t_preprocess"], 3),
+                         "contraction_sample": round(info["t_contract"], 3),
+                         "contraction_whole_estimate": round(info["t_contract"] * F / max(info["flops"], 1), 1)},
+            "seconds": wall, "nnz_out_sample": int(oh.shape[0])}
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
 TWO_WORKLOAD = ("two-tensor contraction: A 10000x5000x200 (seed 3), B 5000x200x8000 (seed 4), 5M unique nnz "
                 "each, Zipf(1.1) per mode, FP64 U(0,1]; A modes {1,2} with B modes {0,1} -> 10000x8000")
 
@@ -628,7 +686,7 @@ def main():
     ap.add_argument("--dense-max", type=int, default=0, help="batchplan: dense LDS column limit (0 = default)")
     ap.add_argument("--lds-hash", type=int, default=0, help="batchplan: LDS hash rows limit (0 = default)")
     ap.add_argument("--chunk", type=int, default=0, help="batchplan: multiply-adds per work item (0 = default)")
-    ap.add_argument("--config", default="nips", choices=["nips", "batch", "batchplan", "twotensor", "chain"],
+    ap.add_argument("--config", default="nips", choices=["nips", "batch", "batchplan", "twotensor", "chain", "readme"],
                     help="nips: the headline self-contraction (default); batch: 256 uneven contractions; "
                          "twotensor: the 5M-nnz two-tensor contraction; chain: two contractions in a row")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -650,6 +708,9 @@ def main():
         return
     if args.config == "batchplan":
         batch_plan_bench(P, args)
+        return
+    if args.config == "readme":
+        readme_bench(P, args)
         return
     if args.config == "twotensor":
         two_tensor_bench(P, args)
