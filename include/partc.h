@@ -222,7 +222,15 @@ typedef enum {
   PTC_ACC_AUTO = 0,
   PTC_ACC_DENSE = 1,      /* direct-mapped (perfect-hash) LDS accumulator     */
   PTC_ACC_HASH = 2,       /* open-addressing LDS hash accumulator            */
-  PTC_ACC_SPILL = 3       /* HBM hash accumulator (heavy rows)               */
+  PTC_ACC_SPILL = 3,      /* HBM hash accumulator (heavy rows)               */
+  PTC_ACC_WINDOW = 4      /* dense LDS column windows: each window's workgroups
+                             stream a row's multiply-adds and keep the columns
+                             in their range; rows of at most 1024 multiply-adds
+                             in small LDS hash tables.  Needs a column space
+                             below 2^16 and at most 4 windows; windows are at
+                             most dense_max_cols wide when that is set, else
+                             as wide as the LDS holds.  AUTO takes it for
+                             column spaces too wide for one dense accumulator. */
 } ptc_acc_mode;
 
 typedef struct {
@@ -262,6 +270,10 @@ typedef struct {
   /* peak device memory the call held beyond what was live when it started
    * (its scratch and the output) */
   uint64_t device_peak_bytes;
+  /* dense column windows the numeric pass ran (PTC_ACC_WINDOW; 0: one dense
+   * or hash accumulator per row) and the rows kept in small LDS hash tables
+   * beside them */
+  uint64_t col_windows, small_rows;
 } ptc_stats;
 
 void ptc_options_default(ptc_options* opt);

@@ -47,7 +47,7 @@ ERRORS = {
 
 METHOD_TWO_PASS, METHOD_LEARNED, METHOD_UPPER_BOUND = 0, 1, 2
 VAL_F64, VAL_F32_TWOSUM, VAL_F32_PLAIN, VAL_F32_F64ACC = 0, 1, 2, 3
-ACC_AUTO, ACC_DENSE, ACC_HASH, ACC_SPILL = 0, 1, 2, 3
+ACC_AUTO, ACC_DENSE, ACC_HASH, ACC_SPILL, ACC_WINDOW = 0, 1, 2, 3, 4
 DIST_UNIFORM, DIST_ZIPF = 0, 1
 GROUP_AUTO, GROUP_HASH, GROUP_SORT, GROUP_PARTITION = 0, 1, 2, 3
 
@@ -161,6 +161,8 @@ class Stats(C.Structure):
         ("ms_build_out", C.c_double),
         ("out_capacity", C.c_uint64),
         ("device_peak_bytes", C.c_uint64),
+        ("col_windows", C.c_uint64),
+        ("small_rows", C.c_uint64),
     ]
 
     def as_dict(self) -> dict:

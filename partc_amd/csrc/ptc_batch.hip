@@ -114,41 +114,11 @@ struct Bin {
 };
 
 // Rows of a dense job that are small against its column space (or of a hash
-// job, whose LDS tables are sized for its largest rows): at most
-// SPARSE_FLOPS multiply-adds (so at most that many distinct columns) in a
-// column space of at least SPLIT_MIN_COLS.  Their items run in an LDS hash
-// bin of SPARSE_THREADS-thread workgroups with tables of at most
-// SPARSE_SLOTS slots (about 29 KB of LDS: five workgroups per CU) instead of
-// the job's dense bin, whose accumulator spans every column (up to ~144 KB:
-// one workgroup per CU), so many small rows are in flight per CU at once and
-// none pays for the whole column space.
+// job, whose LDS tables are sized for its largest rows), in a column space of
+// at least SPLIT_MIN_COLS, run in an LDS hash bin of small workgroups (the
+// row-kind split of ptc_contract.h: SPARSE_FLOPS, SPARSE_SLOTS,
+// SPARSE_THREADS).
 constexpr uint64_t SPLIT_MIN_COLS = 4096;
-constexpr unsigned long long SPARSE_FLOPS = 1024;
-constexpr uint32_t SPARSE_SLOTS = 2048;
-constexpr uint32_t SPARSE_THREADS = 128;
-
-// Items of a job split by row kind (order within each list is free).
-// A small row is one work item (a row split over several items needs the
-// dense share's slabs and merge).
-__global__ void k_split_items(const uint4* __restrict__ items, uint64_t nitems, const unsigned long long* __restrict__ row_flops,
-                              const uint32_t* __restrict__ row_nchunk, uint4* __restrict__ dense, uint4* __restrict__ sparse,
-                              uint32_t* __restrict__ cnt) {
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nitems; i += (uint64_t)gridDim.x * blockDim.x) {
-    const uint4 it = items[i];
-    if (row_flops[it.x] <= SPARSE_FLOPS && row_nchunk[it.x] == 1) sparse[atomicAdd(&cnt[1], 1u)] = it;
-    else dense[atomicAdd(&cnt[0], 1u)] = it;
-  }
-}
-
-// LDS hash slots of the sparse rows: a power of two >= 2 x their flops (>= 16).
-__global__ void k_sparse_caps(const unsigned long long* __restrict__ row_flops, uint64_t nrows, uint32_t* __restrict__ hcap) {
-  for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nrows; r += (uint64_t)gridDim.x * blockDim.x) {
-    const unsigned long long f = row_flops[r];
-    uint32_t c = 16;
-    while (c < 2 * f && c < SPARSE_SLOTS) c <<= 1;
-    hcap[r] = c;
-  }
-}
 
 // One kernel kind's share of a job: its parameters (items pointing at the
 // share's work list), kernel shape and work counts.
@@ -178,7 +148,6 @@ struct Split {
 // tables, and its small rows in the sparse share.  Every window (and the
 // sparse share) counts its own entries per row and writes its own part of
 // the job's output.
-constexpr uint32_t MAX_WINDOWS = 4;
 struct Win {
   uint32_t T = 0;  // windows (0: not windowed)
   uint32_t threads = PB;
@@ -505,6 +474,7 @@ int plan_rest(ptc_batch* b) {
   for (int j = 0; j < n; ++j) {
     b->runs[j].reset(new Run(ctx));
     b->runs[j]->force_flags = true;  // replays may bring any values
+    b->runs[j]->no_windows = true;   // the batch plans its own windows
     if ((rc = prepare_launch(*b->runs[j], b->HA[j], b->HB[j], b->Ac[j], b->B[j], b->opt, tot + 4 * j)))
       return set_err(ctx, rc, "job %d: %s", j, ctx->err.c_str());
   }
@@ -534,13 +504,9 @@ int plan_rest(ptc_batch* b) {
       uint32_t* cnt = b->ws->get<uint32_t>(2);
       if (!sp.dense || !sp.sparse || !sp.hcap || !sp.zero || !cnt) return PTC_ERR_NOMEM;
       if ((rc = fill_zero(ctx, cnt, 8)) || (rc = fill_zero(ctx, sp.zero, R.nrows * 8))) return rc;
-      const unsigned gi = (unsigned)std::min<uint64_t>((R.nitems + 255) / 256, 4096);
-      const unsigned gr = (unsigned)std::min<uint64_t>((R.nrows + 255) / 256, 4096);
-      hipLaunchKernelGGL(k_split_items, dim3(gi), dim3(256), 0, ctx->stream, (const uint4*)R.items, R.nitems,
-                         (const unsigned long long*)R.row_flops, (const uint32_t*)R.row_nchunk, sp.dense, sp.sparse, cnt);
-      hipLaunchKernelGGL(k_sparse_caps, dim3(gr), dim3(256), 0, ctx->stream, (const unsigned long long*)R.row_flops,
-                         R.nrows, sp.hcap);
-      PTC_HIP(ctx, hipGetLastError());
+      if ((rc = launch_split_items(ctx, R.items, R.nitems, R.row_flops, R.row_nchunk, sp.dense, sp.sparse, cnt)) ||
+          (rc = launch_sparse_caps(ctx, R.row_flops, R.nrows, sp.hcap)))
+        return rc;
       csrc[j] = (const unsigned long long*)cnt;  // (dense, sparse) as one 64-bit word
     }
     std::vector<unsigned long long> cnts;

@@ -141,6 +141,14 @@ struct CapParams {
   PTR(unsigned long long) out_count;                                                          \
   uint64_t out_limit;                                                                         \
   PTR(unsigned long long) out_rowoff;                                                         \
+  /* out_soft = 1 (with out_count): no per-row capacity; a row part whose reservation would */ \
+  /* pass out_limit writes nothing and is counted in overflow (the caller redoes the run) */  \
+  uint32_t out_soft, out_pad;                                                                 \
+  /* dense symbolic pass: the touched columns of each window [t * win_width, ...) counted */  \
+  /* into win_cnt[t * win_stride + row] (win_n windows; win_cnt NULL: the row total only) */  \
+  PTR(uint32_t) win_cnt;                                                                      \
+  uint64_t win_stride;                                                                        \
+  uint32_t win_width, win_n;                                                                  \
   PTR(uint32_t) overflow;                                                                     \
   PTR(void) stg_col; /* capacity mode staging (col key) */                                    \
   PTR(double) stg_val;                                                                        \
@@ -219,6 +227,27 @@ int launch_overflow_rows(ptc_ctx* ctx, const uint32_t* nnz, const uint32_t* cap,
 int launch_select_items(ptc_ctx* ctx, const uint4* items, uint64_t nitems, const uint32_t* nnz,
                         const uint32_t* cap, uint4* out, uint32_t* n);
 int launch_reset_rows(ptc_ctx* ctx, const uint32_t* list, uint32_t n, uint32_t* counts);
+
+// Small rows in wide column spaces: at most SPARSE_FLOPS multiply-adds (so
+// at most that many distinct columns) and one work item.  They run in an LDS
+// hash pass of SPARSE_THREADS-thread workgroups with tables of at most
+// SPARSE_SLOTS slots (about 29 KB of LDS: five workgroups per CU) instead of
+// a dense accumulator spanning every column (up to ~144 KB: one workgroup
+// per CU), so many small rows are in flight per CU at once and none pays for
+// the whole column space.
+constexpr unsigned long long SPARSE_FLOPS = 1024;
+constexpr uint32_t SPARSE_SLOTS = 2048;
+constexpr uint32_t SPARSE_THREADS = 128;
+// items split by row kind into dense[] and sparse[] (cnt[0], cnt[1]: their
+// counts, zeroed by the caller; order within each list is free)
+int launch_split_items(ptc_ctx* ctx, const uint4* items, uint64_t nitems, const unsigned long long* row_flops,
+                       const uint32_t* row_nchunk, uint4* dense, uint4* sparse, uint32_t* cnt);
+// LDS hash slots of the small rows: a power of two >= 2 x their flops (>= 16)
+int launch_sparse_caps(ptc_ctx* ctx, const unsigned long long* row_flops, uint64_t nrows, uint32_t* hcap);
+// Column windows: off[t * nrows + r] = out_off[r] + sum of cnt[t' * nrows + r]
+// over t' < t (each window's first output entry in its row's range).
+int launch_window_offsets(ptc_ctx* ctx, const unsigned long long* out_off, const uint32_t* cnt, uint64_t nrows,
+                          uint32_t nwin, unsigned long long* off);
 int fill_u64(ptc_ctx* ctx, unsigned long long* p, uint64_t n, unsigned long long v);
 int fill_u32(ptc_ctx* ctx, uint32_t* p, uint64_t n, uint32_t v);
 

@@ -930,16 +930,26 @@ __device__ __forceinline__ void pass_item(const PP& P, uint32_t widx) {
   // ---- finish the row ----
   if (PASS == PASS_SYMBOLIC) {
     if (DENSE) {
-      uint32_t c = 0;
-      for (uint32_t w = threadIdx.x; w < nbits; w += NT) c += __popc(bits[w]);
-      for (int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o, 64);
-      if (lane == 0) s_w[wid] = c;
-      __syncthreads();
-      if (threadIdx.x == 0) {
-        uint32_t t = 0;
-        for (int w = 0; w < NW; ++w) t += s_w[w];
-        P.row_nnz[r] = t;
+      // the row total, or (column windows) each window's count and the total
+      const uint32_t nwin = P.win_cnt ? P.win_n : 1u;
+      const uint32_t wpw = P.win_cnt ? P.win_width / 32 : nbits;  // bitmap words per window
+      uint32_t tot = 0;
+      for (uint32_t t = 0; t < nwin; ++t) {
+        const uint32_t w0 = t * wpw, w1 = t + 1 == nwin ? nbits : min(nbits, w0 + wpw);
+        uint32_t c = 0;
+        for (uint32_t w = w0 + threadIdx.x; w < w1; w += NT) c += __popc(bits[w]);
+        for (int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o, 64);
+        if (lane == 0) s_w[wid] = c;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+          uint32_t s = 0;
+          for (int w = 0; w < NW; ++w) s += s_w[w];
+          if (P.win_cnt) P.win_cnt[(uint64_t)t * P.win_stride + r] = s;
+          tot += s;
+        }
+        __syncthreads();
       }
+      if (threadIdx.x == 0) P.row_nnz[r] = tot;
     } else if (S) {
       if (threadIdx.x == 0) P.row_nnz[r] = s_misc[0];
     }
@@ -986,10 +996,14 @@ __device__ __forceinline__ void pass_item(const PP& P, uint32_t widx) {
     if (w < wid) wbase += x;
     total += x;
   }
-  uint32_t cap = P.out_cap ? P.out_cap[r] : 0xFFFFFFFFu;
+  uint32_t cap = P.out_cap && !P.out_soft ? P.out_cap[r] : 0xFFFFFFFFu;
   const bool full = s_misc[3] != 0;  // LDS table filled up: count unknown
+  const bool soft = P.out_soft != 0;  // only the output's total limit applies
   if (threadIdx.x == 0) {
-    if (P.out_cap) {
+    if (soft) {
+      P.row_nnz[r] = full ? 0xFFFFFFFFu : total;
+      if (full) atomicAdd(P.overflow, 1u);
+    } else if (P.out_cap) {
       // capacity mode: record the count; a row over its capacity (or whose
       // table filled) is redone by the exact two-pass path
       P.row_nnz[r] = full ? 0xFFFFFFFFu : total;
@@ -1000,7 +1014,7 @@ __device__ __forceinline__ void pass_item(const PP& P, uint32_t widx) {
   }
   if (full || total > cap) return;
   // never write past the row's allocation (exact mode: the symbolic count)
-  const uint32_t limit = min(total, P.out_cap ? cap : P.row_nnz[r]);
+  const uint32_t limit = soft ? total : min(total, P.out_cap ? cap : P.row_nnz[r]);
   uint64_t base;
   if (P.out_count) {
     // reservation: the row's entries go straight to the output, placed by
@@ -1009,7 +1023,10 @@ __device__ __forceinline__ void pass_item(const PP& P, uint32_t widx) {
     if (threadIdx.x == 0) {
       unsigned long long b = atomicAdd(P.out_count, (unsigned long long)limit);
       if (b + limit > P.out_limit) {
-        atomicOr(P.overflow, 0x20000000u);  // cannot happen: the output holds every capacity
+        // soft: the output is smaller than the result (redone by the
+        // caller); else it cannot happen: the output holds every capacity
+        if (soft) atomicAdd(P.overflow, 1u);
+        else atomicOr(P.overflow, 0x20000000u);
         b = ~0ull;
       } else if (P.out_rowoff) {
         P.out_rowoff[r] = b;
@@ -1401,6 +1418,59 @@ __global__ void k_reset_rows(const uint32_t* __restrict__ list, uint32_t n, uint
 int launch_reset_rows(ptc_ctx* ctx, const uint32_t* list, uint32_t n, uint32_t* counts) {
   if (!n) return PTC_OK;
   hipLaunchKernelGGL(k_reset_rows, dim3(gridn(n, 256)), dim3(256), 0, ctx->stream, list, n, counts);
+  PTC_HIP(ctx, hipGetLastError());
+  return PTC_OK;
+}
+
+// A small row is one work item (a row split over several items needs the
+// dense share's slabs and merge).
+__global__ void k_split_items(const uint4* __restrict__ items, uint64_t nitems, const unsigned long long* __restrict__ row_flops,
+                              const uint32_t* __restrict__ row_nchunk, uint4* __restrict__ dense, uint4* __restrict__ sparse,
+                              uint32_t* __restrict__ cnt) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nitems; i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint4 it = items[i];
+    if (row_flops[it.x] <= SPARSE_FLOPS && row_nchunk[it.x] == 1) sparse[atomicAdd(&cnt[1], 1u)] = it;
+    else dense[atomicAdd(&cnt[0], 1u)] = it;
+  }
+}
+int launch_split_items(ptc_ctx* ctx, const uint4* items, uint64_t nitems, const unsigned long long* row_flops,
+                       const uint32_t* row_nchunk, uint4* dense, uint4* sparse, uint32_t* cnt) {
+  if (!nitems) return PTC_OK;
+  hipLaunchKernelGGL(k_split_items, dim3(gridn(nitems, 256)), dim3(256), 0, ctx->stream, items, nitems, row_flops,
+                     row_nchunk, dense, sparse, cnt);
+  PTC_HIP(ctx, hipGetLastError());
+  return PTC_OK;
+}
+
+__global__ void k_sparse_caps(const unsigned long long* __restrict__ row_flops, uint64_t nrows, uint32_t* __restrict__ hcap) {
+  for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nrows; r += (uint64_t)gridDim.x * blockDim.x) {
+    const unsigned long long f = row_flops[r];
+    uint32_t c = 16;
+    while (c < 2 * f && c < SPARSE_SLOTS) c <<= 1;
+    hcap[r] = c;
+  }
+}
+int launch_sparse_caps(ptc_ctx* ctx, const unsigned long long* row_flops, uint64_t nrows, uint32_t* hcap) {
+  if (!nrows) return PTC_OK;
+  hipLaunchKernelGGL(k_sparse_caps, dim3(gridn(nrows, 256)), dim3(256), 0, ctx->stream, row_flops, nrows, hcap);
+  PTC_HIP(ctx, hipGetLastError());
+  return PTC_OK;
+}
+
+__global__ void k_window_offsets(const unsigned long long* __restrict__ out_off, const uint32_t* __restrict__ cnt,
+                                 uint64_t nrows, uint32_t nwin, unsigned long long* __restrict__ off) {
+  for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nrows; r += (uint64_t)gridDim.x * blockDim.x) {
+    unsigned long long o = out_off[r];
+    for (uint32_t t = 0; t < nwin; ++t) {
+      off[(uint64_t)t * nrows + r] = o;
+      o += cnt[(uint64_t)t * nrows + r];
+    }
+  }
+}
+int launch_window_offsets(ptc_ctx* ctx, const unsigned long long* out_off, const uint32_t* cnt, uint64_t nrows,
+                          uint32_t nwin, unsigned long long* off) {
+  if (!nrows || !nwin) return PTC_OK;
+  hipLaunchKernelGGL(k_window_offsets, dim3(gridn(nrows, 256)), dim3(256), 0, ctx->stream, out_off, cnt, nrows, nwin, off);
   PTC_HIP(ctx, hipGetLastError());
   return PTC_OK;
 }

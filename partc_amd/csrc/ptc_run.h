@@ -16,6 +16,7 @@ constexpr uint32_t DEFAULT_LDS_HASH = 2048;
 constexpr uint64_t DEFAULT_CHUNK = 32768;         // multiply-adds per work item, hash accumulators
 constexpr uint64_t DEFAULT_DENSE_CHUNK = 65536;   // floor for dense accumulators
 constexpr uint64_t DENSE_MAX_ITEMS = 3072;         // dense chunks grow to total flops / this
+constexpr uint32_t MAX_WINDOWS = 4;                // dense column windows of one contraction
 
 // Hash accumulator plan of every row: LDS slots or an HBM region.
 struct HashPlan {
@@ -61,6 +62,10 @@ struct Run {
   bool f32 = false;            // FP32 values (both tables carry gval32)
   int val = PTC_VAL_F64;       // ptc_value_type of the numeric pass
   bool col_first = false;      // output modes: column modes, then row modes
+  // Dense column windows (a 16-bit column space too wide for one dense LDS
+  // accumulator): window t holds columns [t * win_width, (t + 1) * win_width).
+  bool no_windows = false;     // keep hash accumulators (set before prepare)
+  uint32_t win_n = 0, win_width = 0, win_threads = PB;
   explicit Run(ptc_ctx* c) : ctx(c), ws(c) {}
 };
 

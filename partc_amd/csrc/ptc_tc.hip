@@ -304,6 +304,8 @@ int prepare_launch(Run& R, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B,
     if (a32 != b32) return set_err(ctx, PTC_ERR_ARG, "A and B must both hold FP64 or both FP32 values");
     if (opt.value_type < PTC_VAL_F64 || opt.value_type > PTC_VAL_F32_F64ACC)
       return set_err(ctx, PTC_ERR_ARG, "unknown value type %d", opt.value_type);
+    if (opt.acc_mode < PTC_ACC_AUTO || opt.acc_mode > PTC_ACC_WINDOW)
+      return set_err(ctx, PTC_ERR_ARG, "unknown accumulator mode %d", opt.acc_mode);
     if (opt.value_type == PTC_VAL_F64 && a32)
       return set_err(ctx, PTC_ERR_ARG, "FP32 tensors need an FP32 value type (PTC_VAL_F32_*)");
     if (opt.value_type != PTC_VAL_F64 && !a32)
@@ -338,6 +340,35 @@ int prepare_launch(Run& R, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B,
   R.dense = opt.acc_mode == PTC_ACC_DENSE ? R.ncols <= dense_fit
           : opt.acc_mode == PTC_ACC_AUTO ? R.ncols <= dense_max : false;
   if (opt.acc_mode == PTC_ACC_DENSE && !R.dense) return set_err(ctx, PTC_ERR_ARG, "column space too large for a dense accumulator");
+  // Dense column windows: a 16-bit column space too wide for one dense
+  // accumulator (AUTO), or any when asked for.  The widest window whose
+  // numeric pass fits a 512-thread workgroup's LDS (touched bits kept),
+  // capped by dense_max_cols when windows are asked for explicitly.
+  if (opt.acc_mode == PTC_ACC_WINDOW || (opt.acc_mode == PTC_ACC_AUTO && !R.dense)) {
+    uint64_t wfit = 0;
+    while (wfit < 65536 && pass_lds_bytes(PASS_NUMERIC, true, false, wfit + 256, 0, true, 512, R.val) <= lds_cap)
+      wfit += 256;
+    if (opt.acc_mode == PTC_ACC_WINDOW && opt.dense_max_cols)
+      wfit = std::min<uint64_t>(wfit, std::max<uint64_t>(64, opt.dense_max_cols / 64 * 64));
+    const uint64_t nw = wfit ? (R.ncols + wfit - 1) / wfit : 0;
+    const bool ok = !R.no_windows && !R.col64 && R.ncols <= 65536 && nw >= 1 && nw <= MAX_WINDOWS &&
+                    (opt.acc_mode == PTC_ACC_WINDOW || R.ncols > wfit);
+    if (opt.acc_mode == PTC_ACC_WINDOW && !ok)
+      return set_err(ctx, PTC_ERR_ARG, "column space of %llu not held by %u dense windows below 2^16 columns",
+                     (unsigned long long)R.ncols, MAX_WINDOWS);
+    if (ok) {
+      R.win_n = (uint32_t)nw;
+      R.win_width = (uint32_t)(((R.ncols + nw - 1) / nw + 63) / 64 * 64);
+      const size_t at512 = pass_lds_bytes(PASS_NUMERIC, true, false, R.win_width, 0, true, 512, R.val);
+      R.win_threads = opt.block_threads ? opt.block_threads
+                    : (2 * at512 > lds_cap &&
+                       pass_lds_bytes(PASS_NUMERIC, true, false, R.win_width, 0, true, 1024, R.val) <= lds_cap)
+                        ? 1024u : 512u;
+      if (R.win_threads < 64 || R.win_threads > (uint32_t)PB_MAX || R.win_threads % 64 ||
+          pass_lds_bytes(PASS_NUMERIC, true, false, R.win_width, 0, R.use_flags, R.win_threads, R.val) > lds_cap)
+        return set_err(ctx, PTC_ERR_ARG, "block_threads %u for %u-column windows", R.win_threads, R.win_width);
+    }
+  }
   R.lds_slots = opt.lds_hash_max ? opt.lds_hash_max : DEFAULT_LDS_HASH;
   if (opt.acc_mode == PTC_ACC_SPILL) R.lds_slots = 0;
   {
@@ -364,11 +395,14 @@ int prepare_launch(Run& R, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B,
   // streamed bytes; 65536 is the floor (measured on the nips-shape and
   // two-tensor workloads).  Hash rows keep smaller chunks: their HBM
   // accumulators are shared by all chunks of a row.
+  // Windows: one item runs once per window, its slabs a window wide.
   const uint64_t chunk = opt.chunk_flops ? opt.chunk_flops
-                                         : (R.dense ? std::max<uint64_t>(DEFAULT_DENSE_CHUNK, 24 * R.ncols) : DEFAULT_CHUNK);
+                       : R.dense ? std::max<uint64_t>(DEFAULT_DENSE_CHUNK, 24 * R.ncols)
+                       : R.win_n ? std::max<uint64_t>(DEFAULT_DENSE_CHUNK, 24 * (uint64_t)R.win_width)
+                                 : DEFAULT_CHUNK;
 
   int rc = PTC_OK;
-  if (R.dense && (rc = hashtable_col16(ctx, HT_B, &R.bcol16))) return rc;
+  if ((R.dense || R.win_n) && (rc = hashtable_col16(ctx, HT_B, &R.bcol16))) return rc;
   const uint32_t* row_of_pos = nullptr;  // HT_A's slice index of every A position
   if ((rc = hashtable_slice_of(ctx, HT_A, &row_of_pos))) return rc;
   R.row_of_pos = row_of_pos;
@@ -428,7 +462,7 @@ int prepare_launch(Run& R, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B,
     // 4 waves per SIMD) takes 4.93 ms with 192000 per item, 4.60 ms from
     // 524288 to 2097152; the nips-shape pass (1.4e8) is fastest at its
     // 65536 floor (0.326 ms; 0.508 ms at 131072: too few items for the CUs).
-    if (R.dense && !opt.chunk_flops) {
+    if ((R.dense || R.win_n) && !opt.chunk_flops) {
       P.total = R.apre + R.npos;
       P.max_items = DENSE_MAX_ITEMS;
     }
@@ -670,6 +704,271 @@ static int single_pass_direct(Run& R, const ptc_dtensor* A, const ptc_dtensor* B
   return PTC_OK;
 }
 
+// ---------------- dense column windows
+//
+// Rows of a 16-bit column space too wide for one dense LDS accumulator.
+// Small rows (one work item of at most SPARSE_FLOPS multiply-adds) run in an
+// LDS hash share of small workgroups; the others run as dense column
+// windows: window t's workgroups stream a row's whole flop range and
+// accumulate the columns in [t * W, (t + 1) * W) in LDS.  The symbolic pass
+// needs one bit per column, so it is a single dense pass over the whole
+// column space that also counts each window's columns; the numeric pass runs
+// the windows in turn, each writing at its row's first entry plus the row's
+// counts in earlier windows.  The output is in row order with columns
+// ascending within a row, as with one dense accumulator.  The single-pass
+// methods reserve every row part with a fetch-add bounded only by the
+// output's size (the rows' capacities summed); a run whose result does not
+// fit is redone by the exact path.
+struct WinRun {
+  uint4* dense = nullptr;              // items of the windowed rows
+  uint4* sparse = nullptr;             // items of the small rows
+  uint32_t nd = 0, ns = 0;
+  uint32_t* hcap = nullptr;            // small rows: LDS hash slots
+  unsigned long long* zero = nullptr;  // small rows: none lives in HBM
+  uint32_t* cnt = nullptr;             // [win_n][nrows]: each window's entries per row
+  unsigned long long* off = nullptr;   // [win_n][nrows]: each window's first entry per row
+  uint32_t* hbits = nullptr;           // split rows' symbolic slabs (whole column space)
+  double* hacc = nullptr;              // numeric slabs, a window wide, reused window after window
+  unsigned char* hflag = nullptr;
+};
+
+static uint32_t win_lo(const Run& R, uint32_t t) { return t * R.win_width; }
+static uint32_t win_w(const Run& R, uint32_t t) {
+  return (uint32_t)std::min<uint64_t>(R.win_width, R.ncols - (uint64_t)t * R.win_width);
+}
+
+// Items split by row kind; the two counts read back.
+static int win_split(Run& R, WinRun& W) {
+  ptc_ctx* ctx = R.ctx;
+  Workspace& ws = R.ws;
+  W.dense = ws.get<uint4>(R.nitems);
+  W.sparse = ws.get<uint4>(R.nitems);
+  W.hcap = ws.get<uint32_t>(R.nrows);
+  W.zero = ws.get<unsigned long long>(R.nrows);
+  W.cnt = ws.get<uint32_t>((uint64_t)R.win_n * R.nrows);
+  uint32_t* c2 = ws.get<uint32_t>(2);
+  if (!W.dense || !W.sparse || !W.hcap || !W.zero || !W.cnt || !c2) return PTC_ERR_NOMEM;
+  int rc = PTC_OK;
+  if ((rc = fill_zero(ctx, c2, 8)) || (rc = fill_zero(ctx, W.zero, R.nrows * 8)) ||
+      (rc = fill_zero(ctx, W.cnt, (uint64_t)R.win_n * R.nrows * 4)) ||
+      (rc = launch_split_items(ctx, R.items, R.nitems, R.row_flops, R.row_nchunk, W.dense, W.sparse, c2)) ||
+      (rc = launch_sparse_caps(ctx, R.row_flops, R.nrows, W.hcap)))
+    return rc;
+  uint32_t h[2] = {0, 0};
+  if ((rc = read_back(ctx, h, c2, 8))) return rc;
+  W.nd = h[0];
+  W.ns = h[1];
+  return PTC_OK;
+}
+
+// The windowed rows' pass: the symbolic pass over the whole column space
+// (t unused), or window t of the numeric pass.
+static void win_params(Run& R, const WinRun& W, PassParams& P, int pass, uint32_t t) {
+  base_params(R, P, nullptr);
+  P.items = W.dense;
+  P.bcol = R.bcol16;
+  P.merge_rows = R.heavy_rows;
+  if (pass == PASS_SYMBOLIC) {
+    P.ncols = R.ncols;
+    P.hbits = W.hbits;
+    P.win_cnt = W.cnt;
+    P.win_stride = R.nrows;
+    P.win_width = R.win_width;
+    P.win_n = R.win_n;
+  } else {
+    P.ncols = win_w(R, t);
+    P.col_lo = win_lo(R, t);
+    P.col_win = 1;
+    P.row_nnz = W.cnt + (uint64_t)t * R.nrows;
+    P.hacc = W.hacc;
+    P.hflag = W.hflag;
+  }
+}
+
+// The small rows' pass: LDS hash tables sized from their flops.
+static void sparse_params(Run& R, const WinRun& W, PassParams& P) {
+  base_params(R, P, nullptr);
+  P.items = W.sparse;
+  P.bcol = R.HB->rest_key32;
+  P.row_hcap = W.hcap;
+  P.row_gcap = W.zero;
+  P.row_goff = W.zero;
+  P.merge_rows = nullptr;
+}
+
+// One share's row pass and (dense) the merge launch of its split rows,
+// bracketed by one pair of timing events.
+static int run_share(Run& R, int pass, bool dense, uint32_t threads, uint64_t ncols, uint32_t lds_slots,
+                     PassParams& P, uint64_t nitems, uint64_t nmerge) {
+  if (!nitems) return PTC_OK;
+  ptc_ctx* ctx = R.ctx;
+  const size_t lds = pass_lds_bytes(pass, dense, false, ncols, lds_slots, R.use_flags, threads,
+                                    pass == PASS_NUMERIC ? R.val : 0);
+  const bool timed = ctx->nkev + 2 <= 16;
+  if (timed && hipEventRecord(ctx->kev[ctx->nkev], ctx->stream) != hipSuccess) (void)hipGetLastError();
+  int rc = launch_pass(ctx, pass, dense, false, P, nitems, lds, false, threads);
+  if (!rc && dense && nmerge) rc = launch_pass(ctx, pass, true, false, P, nmerge, lds, true, threads);
+  if (rc) return rc;
+  if (timed) {
+    if (hipEventRecord(ctx->kev[ctx->nkev + 1], ctx->stream) != hipSuccess) (void)hipGetLastError();
+    ctx->kev_pass[ctx->nkev / 2] = pass;
+    ctx->nkev += 2;
+  }
+  return PTC_OK;
+}
+
+static int win_slabs(Run& R, WinRun& W) {
+  if (!R.nheavy || W.hacc) return PTC_OK;
+  W.hacc = R.ws.get<double>(R.nslabs * R.win_width);
+  if (R.use_flags) W.hflag = R.ws.get<unsigned char>(R.nslabs * R.win_width);
+  return W.hacc && (W.hflag || !R.use_flags) ? PTC_OK : PTC_ERR_NOMEM;
+}
+
+// Exact two-pass over windows: *out_p is ext (caller's arrays) or a new output.
+static int win_two_pass(Run& R, WinRun& W, const ptc_dtensor* A, const ptc_dtensor* B, Timer& T,
+                        const OutInto* into, ptc_dtensor* ext, ptc_dtensor** out_p, uint64_t* nnz_p) {
+  ptc_ctx* ctx = R.ctx;
+  Workspace& ws = R.ws;
+  int rc = PTC_OK;
+  const uint32_t sym_threads = R.opt.block_threads ? R.opt.block_threads : 512u;
+  if (R.nheavy && !W.hbits && !(W.hbits = ws.get<uint32_t>(R.nslabs * ((R.ncols + 31) / 32)))) return PTC_ERR_NOMEM;
+  if ((rc = fill_zero(ctx, R.row_nnz, (R.nrows + 1) * 4))) return rc;
+  PassParams P;
+  win_params(R, W, P, PASS_SYMBOLIC, 0);
+  if ((rc = run_share(R, PASS_SYMBOLIC, true, sym_threads, R.ncols, 0, P, W.nd, R.nheavy))) return rc;
+  sparse_params(R, W, P);
+  if ((rc = run_share(R, PASS_SYMBOLIC, false, SPARSE_THREADS, 0, SPARSE_SLOTS, P, W.ns, 0))) return rc;
+  T.mark();  // symbolic done
+  unsigned long long nnz = 0;
+  if ((rc = scan_u32_to_u64(ctx, R.row_nnz, R.out_off, R.nrows, R.out_off + R.nrows)) ||
+      (rc = read_back(ctx, &nnz, R.out_off + R.nrows, 8)))
+    return rc;
+  ptc_dtensor* out = nullptr;
+  if (into) {
+    *into->nnz = nnz;
+    if (nnz > into->cap)
+      return set_err(ctx, PTC_ERR_RANGE, "the output has %llu nonzeros; the arrays hold %llu",
+                     (unsigned long long)nnz, (unsigned long long)into->cap);
+    ext->ctx = ctx;
+    ext->d = R.dO;
+    ext->N = nnz;
+    ext->hedges = into->hedges;
+    ext->vals = into->vals;
+    out = ext;
+  } else if ((rc = make_output(R, nnz, A, B, &out))) {
+    return rc;
+  }
+  auto fail = [&](int code) {
+    if (out != ext) ptc_dtensor_free(out);
+    return code;
+  };
+  if (!(W.off = ws.get<unsigned long long>((uint64_t)R.win_n * R.nrows))) return fail(PTC_ERR_NOMEM);
+  if ((rc = launch_window_offsets(ctx, R.out_off, W.cnt, R.nrows, R.win_n, W.off)) || (rc = win_slabs(R, W)))
+    return fail(rc);
+  auto outputs = [&](PassParams& Q) {
+    Q.out_hedges = out->hedges;
+    Q.out_vals = out->vals;
+    Q.out_vals32 = out->vals32;
+    Q.out_err32 = out->err32;
+  };
+  for (uint32_t t = 0; t < R.win_n; ++t) {
+    win_params(R, W, P, PASS_NUMERIC, t);
+    P.out_base = W.off + (uint64_t)t * R.nrows;
+    outputs(P);
+    if ((rc = run_share(R, PASS_NUMERIC, true, R.win_threads, win_w(R, t), 0, P, W.nd, R.nheavy))) return fail(rc);
+  }
+  sparse_params(R, W, P);
+  P.out_base = R.out_off;
+  outputs(P);
+  if ((rc = run_share(R, PASS_NUMERIC, false, SPARSE_THREADS, 0, SPARSE_SLOTS, P, W.ns, 0))) return fail(rc);
+  T.mark();  // numeric done
+  uint32_t ovf = 0;
+  if ((rc = read_back(ctx, &ovf, R.overflow, 4))) return fail(rc);
+  if (ovf & 0x80000000u) return fail(set_err(ctx, PTC_ERR_INTERNAL, "row count mismatch between passes"));
+  if (ovf & 0x40000000u) return fail(set_err(ctx, PTC_ERR_INTERNAL, "slice index out of range"));
+  *out_p = out;
+  *nnz_p = nnz;
+  return PTC_OK;
+}
+
+// Single pass over windows into an output of the capacities' size.
+static int win_single_pass(Run& R, WinRun& W, const ptc_dtensor* A, const ptc_dtensor* B, const uint32_t* cap,
+                           const unsigned long long* cap_off, int method, Timer& T, ptc_dtensor** out_p,
+                           uint64_t* nnz_p, uint64_t* fallback_p, uint64_t* capacity_p) {
+  (void)cap;
+  ptc_ctx* ctx = R.ctx;
+  Workspace& ws = R.ws;
+  int rc = PTC_OK;
+  unsigned long long capsum = R.F;
+  if (R.ncols && R.nrows <= ~0ull / R.ncols) capsum = std::min<unsigned long long>(capsum, R.nrows * R.ncols);
+  if ((method == PTC_METHOD_LEARNED || capsum > STAGING_BOUND) && (rc = read_back(ctx, &capsum, cap_off + R.nrows, 8)))
+    return rc;
+  *capacity_p = capsum;
+  ptc_dtensor* out = nullptr;
+  if ((rc = make_output(R, capsum, A, B, &out))) return rc;
+  auto fail = [&](int code) {
+    ptc_dtensor_free(out);
+    return code;
+  };
+  unsigned long long* counter = ws.get<unsigned long long>(2);
+  unsigned long long* pair = ws.get<unsigned long long>(2);
+  if (!counter || !pair) return fail(PTC_ERR_NOMEM);
+  if ((rc = fill_zero(ctx, counter, 16)) || (rc = win_slabs(R, W))) return fail(rc);
+  auto direct = [&](PassParams& Q) {
+    Q.out_count = counter;
+    Q.out_limit = capsum;
+    Q.out_soft = 1;
+    Q.out_hedges = out->hedges;
+    Q.out_vals = out->vals;
+    Q.out_vals32 = out->vals32;
+    Q.out_err32 = out->err32;
+  };
+  PassParams P;
+  for (uint32_t t = 0; t < R.win_n; ++t) {
+    win_params(R, W, P, PASS_NUMERIC, t);
+    direct(P);
+    if ((rc = run_share(R, PASS_NUMERIC, true, R.win_threads, win_w(R, t), 0, P, W.nd, R.nheavy))) return fail(rc);
+  }
+  sparse_params(R, W, P);
+  direct(P);
+  if ((rc = run_share(R, PASS_NUMERIC, false, SPARSE_THREADS, 0, SPARSE_SLOTS, P, W.ns, 0))) return fail(rc);
+  unsigned long long rb[2] = {0, 0};
+  {
+    WordSources S{};
+    S.n = 2;
+    S.src[0] = R.overflow;
+    S.bytes[0] = 4;
+    S.src[1] = counter;
+    S.bytes[1] = 8;
+    if ((rc = collect_words(ctx, pair, S)) || (rc = read_back(ctx, rb, pair, 16))) return fail(rc);
+  }
+  const uint32_t ovf0 = (uint32_t)rb[0];
+  if (ovf0 & 0x80000000u) return fail(set_err(ctx, PTC_ERR_INTERNAL, "accumulator table overflow"));
+  if (ovf0 & 0x40000000u) return fail(set_err(ctx, PTC_ERR_INTERNAL, "slice index out of range"));
+  const uint64_t fallback = ovf0 & 0x1FFFFFFFu;
+  if (fallback) {
+    // the result outgrew the capacities' sum: the exact path from the start
+    ptc_dtensor_free(out);
+    out = nullptr;
+    if ((rc = fill_zero(ctx, R.overflow, 4))) return rc;
+    uint64_t nnz = 0;
+    if ((rc = win_two_pass(R, W, A, B, T, nullptr, nullptr, &out, &nnz))) return rc;
+    *out_p = out;
+    *nnz_p = nnz;
+    *fallback_p = fallback;
+    return PTC_OK;
+  }
+  T.mark();  // 2
+  const uint64_t nnz = rb[1];
+  if (capsum > 2 * nnz + 65536 && (rc = resize_output(R, out, nnz, nnz))) return fail(rc);
+  out->N = nnz;
+  T.mark();  // 3
+  *out_p = out;
+  *nnz_p = nnz;
+  *fallback_p = 0;
+  return PTC_OK;
+}
+
 }  // namespace ptc
 
 namespace {
@@ -752,6 +1051,8 @@ int tc_impl(ptc_ctx* ctx, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B, 
   R.col_first = col_first;
   if (col_first && (opt.method != PTC_METHOD_TWO_PASS || kr || into))
     return set_err(ctx, PTC_ERR_INTERNAL, "column-first output takes the plain two-pass path");
+  // a chain step's single pass stages its rows in row order (no windows)
+  R.no_windows = kr && opt.method != PTC_METHOD_TWO_PASS;
   Timer T(ctx);
   ctx->nkev = 0;
   const size_t live0 = ctx->live_bytes;
@@ -771,7 +1072,13 @@ int tc_impl(ptc_ctx* ctx, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B, 
   ptc_dtensor* out = nullptr;
   const int method = opt.method;
 
-  if (method == PTC_METHOD_TWO_PASS) {
+  WinRun W;
+  if (R.win_n && (rc = win_split(R, W))) return rc;
+  if (method == PTC_METHOD_TWO_PASS && R.win_n) {
+    if ((rc = win_two_pass(R, W, A, B, T, into, &ext, &out, &nnz))) return rc;
+    out_capacity = nnz;
+    flags_checked = true;
+  } else if (method == PTC_METHOD_TWO_PASS) {
     // ---------------- symbolic
     HashPlan hs;
     if (!R.dense && (rc = plan_hash(R, hs, nullptr, R.row_flops, nullptr))) return rc;
@@ -854,8 +1161,9 @@ int tc_impl(ptc_ctx* ctx, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B, 
     }
     if ((rc = scan_u32_to_u64(ctx, cap, cap_off, R.nrows, cap_off + R.nrows))) return rc;
     if (!kr) {
-      if ((rc = single_pass_direct(R, A, B, cap, cap_off, method, T, &out, &nnz, &fallback_rows, &out_capacity)))
-        return rc;
+      rc = R.win_n ? win_single_pass(R, W, A, B, cap, cap_off, method, T, &out, &nnz, &fallback_rows, &out_capacity)
+                   : single_pass_direct(R, A, B, cap, cap_off, method, T, &out, &nnz, &fallback_rows, &out_capacity);
+      if (rc) return rc;
       flags_checked = true;
     } else {
     // A chain step keeps its output rows in row order (its key table is
@@ -1024,6 +1332,8 @@ int tc_impl(ptc_ctx* ctx, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B, 
     stats->heavy_rows = R.nheavy;
     stats->out_capacity = out_capacity;
     stats->device_peak_bytes = ctx->peak_bytes - live0;
+    stats->col_windows = R.win_n;
+    stats->small_rows = W.ns;
     stats->ms_build_a = hashtable_ms_build(HT_A);
     stats->ms_build_b = hashtable_ms_build(HT_B);
     stats->ms_preprocess = T.ms(0, 1);
