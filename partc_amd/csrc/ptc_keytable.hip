@@ -1006,6 +1006,320 @@ __global__ void __launch_bounds__(PT_THREADS) k_pd_scatter(const uint32_t* __res
                                slice_of, t);
 }
 
+// ------------------------------------------------ placement by reservation
+//
+// A single table places its tiles without scanning a count matrix: the
+// counting pass counts every tile per bucket (or direct key) in LDS and
+// reserves the tile's run in each bucket it touches with one global
+// fetch-add on that bucket's total, recording (count, offset within the
+// bucket) per (tile, bucket) in tile-major rows; one workgroup turns the
+// totals into bucket starts (and, for a direct key space, into the slices
+// and the whole index); the placing pass reads its tile's row (coalesced)
+// and writes the tile in bucket order.  A bucket's runs land in reservation
+// order rather than tile order: a key's ids form a set, as in the
+// reference's slices (fksCuckoo.hpp:1044-1060 appends ids in insertion
+// order; nothing downstream depends on it).  Against a bucket-major count
+// matrix this drops the matrix's three-launch scan, its strided row reads
+// and, for direct key spaces, the separate slices launch.
+
+// Each bucket's region is split into one part per XCD: the runs that land
+// side by side are written by one XCD and merge in its L2, and the part's
+// counter is only ever touched from that XCD, so its fetch-adds are
+// performed in that XCD's L2 (workgroup scope) instead of at the memory
+// side.  The part is the XCD the counting workgroup actually runs on (its
+// hardware XCC id; gfx950 has eight XCDs), recorded per tile for the
+// placing pass, whose workgroup for the same tile runs on the same XCD under
+// the round-robin dispatch that pt_tile_of assumes (only speed depends on
+// that).
+constexpr uint32_t PC_XCDS = 8;
+__device__ __forceinline__ uint32_t pc_xcc_id() {
+  return __builtin_amdgcn_s_getreg((3 << 11) | 20) & (PC_XCDS - 1);  // hwreg(HW_REG_XCC_ID, 0, 4)
+}
+
+__global__ void __launch_bounds__(PT_THREADS) k_pc_count(const uint32_t* __restrict__ h, uint32_t d, uint64_t N,
+                                                         KeyCodec codec, int bbits, uint32_t nb,
+                                                         uint64_t ntiles, uint32_t* __restrict__ btot,
+                                                         uint32_t* __restrict__ tcnt, uint32_t* __restrict__ toff,
+                                                         uint32_t* __restrict__ txcc) {
+  __shared__ uint32_t lh[PT_MAX_BUCKETS];
+  const uint64_t t = pt_tile_of(blockIdx.x, ntiles);
+  if (t >= ntiles) return;
+  const uint32_t x = pc_xcc_id();
+  if (threadIdx.x == 0) txcc[t] = x;
+  for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) lh[b] = 0;
+  __syncthreads();
+  const uint64_t base = t * PT_TILE;
+  const uint32_t n = (uint32_t)min<uint64_t>(PT_TILE, N - base);
+  for (uint32_t i0 = 0; i0 < n; i0 += PT_U * blockDim.x) {
+    unsigned long long key[PT_U];
+    pt_load_keys(h, d, base, i0, n, codec, key);  // the table's own tensor: always within radix
+#pragma unroll
+    for (int u = 0; u < PT_U; ++u)
+      if (i0 + threadIdx.x + u * blockDim.x < n) atomicAdd(&lh[pt_bucket(key[u], bbits)], 1u);
+  }
+  __syncthreads();
+  uint32_t* rc = tcnt + t * nb;
+  uint32_t* ro = toff + t * nb;
+  // all of a thread's fetch-adds in flight before the first result is used
+  constexpr int PB = PT_MAX_BUCKETS / PT_THREADS;
+  uint32_t c[PB], o[PB];
+#pragma unroll
+  for (int k = 0; k < PB; ++k) {
+    const uint32_t b = threadIdx.x + k * PT_THREADS;
+    c[k] = b < nb ? lh[b] : 0u;
+    o[k] = 0;
+    if (c[k])
+      o[k] = __hip_atomic_fetch_add(&btot[(uint64_t)b * PC_XCDS + x], c[k], __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_WORKGROUP);
+  }
+#pragma unroll
+  for (int k = 0; k < PB; ++k) {
+    const uint32_t b = threadIdx.x + k * PT_THREADS;
+    if (b >= nb) break;
+    rc[b] = c[k];
+    ro[b] = o[k];
+  }
+}
+
+static_assert(PC_XCDS == 8, "k_pc_starts reads a bucket's per-XCD totals as two 16-byte words");
+
+// Bucket starts from the totals, one workgroup of 1024 threads (nb <=
+// PT_MAX_BUCKETS, four consecutive buckets a thread): start[b] and start[nb]
+// = N.  A direct key space (every key its own bucket) also gets its slices
+// (present keys in key order), its whole index, the compact slice index of
+// every key and the (keys << 32 | nonzeros) total.  (present << 32 | count)
+// is scanned as one word: counts sum to N < 2^32.
+__global__ void __launch_bounds__(1024) k_pc_starts(const uint32_t* __restrict__ btot, uint32_t nb, uint64_t N,
+                                                    unsigned long long* __restrict__ start,
+                                                    uint32_t* __restrict__ sstart, int direct,
+                                                    KtSlice* __restrict__ slices, KtEntry* __restrict__ tab,
+                                                    uint32_t* __restrict__ cidx,
+                                                    unsigned long long* __restrict__ total) {
+  __shared__ unsigned long long s_wave[1024 / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const uint32_t b0 = threadIdx.x * 4;
+  // a bucket's eight per-XCD totals are two aligned 16-byte words; all of a
+  // thread's loads in flight together
+  uint4 q[4][2];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const bool in = b0 + j < nb;
+    const uint4* p = reinterpret_cast<const uint4*>(btot + (uint64_t)(b0 + j) * PC_XCDS);
+    q[j][0] = in ? p[0] : make_uint4(0, 0, 0, 0);
+    q[j][1] = in ? p[1] : make_uint4(0, 0, 0, 0);
+  }
+  unsigned long long v[4], sum = 0;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const uint32_t c = q[j][0].x + q[j][0].y + q[j][0].z + q[j][0].w + q[j][1].x + q[j][1].y + q[j][1].z + q[j][1].w;
+    v[j] = c ? (1ull << 32) | c : 0ull;
+    sum += v[j];
+  }
+  unsigned long long inc = sum;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const unsigned long long x = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += x;
+  }
+  if (lane == 63) s_wave[wid] = inc;
+  __syncthreads();
+  unsigned long long pre = inc - sum;
+  for (int w = 0; w < wid; ++w) pre += s_wave[w];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const uint32_t b = b0 + j;
+    if (b >= nb) break;
+    const uint32_t off = (uint32_t)pre, c = (uint32_t)v[j], si = (uint32_t)(pre >> 32);
+    start[b] = off;
+    uint4 s0, s1;  // exclusive prefix over the XCD parts
+    s0.x = off;
+    s0.y = s0.x + q[j][0].x;
+    s0.z = s0.y + q[j][0].y;
+    s0.w = s0.z + q[j][0].z;
+    s1.x = s0.w + q[j][0].w;
+    s1.y = s1.x + q[j][1].x;
+    s1.z = s1.y + q[j][1].y;
+    s1.w = s1.z + q[j][1].z;
+    uint4* ps = reinterpret_cast<uint4*>(sstart + (uint64_t)b * PC_XCDS);
+    ps[0] = s0;
+    ps[1] = s1;
+    if (direct) {
+      KtEntry en;
+      en.key = PTC_EMPTY_KEY;
+      en.off = 0;
+      en.len = 0;
+      if (c) {
+        KtSlice sl;
+        sl.key = b;
+        sl.off = off;
+        sl.len = c;
+        slices[si] = sl;
+        en.key = b;
+        en.off = off;
+        en.len = c;
+      }
+      tab[b] = en;
+      cidx[b] = si;
+    }
+    pre += v[j];
+  }
+  if (threadIdx.x == blockDim.x - 1) {
+    start[nb] = N;
+    if (direct) *total = (pre & 0xFFFFFFFF00000000ull) | N;
+  }
+}
+
+// The tile's counts and reserved offsets: lcur[b] = the bucket's start in
+// the tile (scanned counts), gdst[b] + j = the output position of the
+// tile's bucket-ordered slot j.
+__device__ __forceinline__ void pc_tile_row(uint64_t t, uint32_t nb, const uint32_t* __restrict__ sstart,
+                                            const uint32_t* __restrict__ tcnt, const uint32_t* __restrict__ toff,
+                                            const uint32_t* __restrict__ txcc, uint32_t* lcur, uint32_t* gdst,
+                                            uint32_t* s_wave) {
+  const uint32_t* rc = tcnt + t * nb;
+  const uint32_t* ro = toff + t * nb;
+  const uint32_t x = txcc[t];
+  for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) {
+    const uint32_t c = rc[b];
+    lcur[b] = c;
+    gdst[b] = c ? sstart[(uint64_t)b * PC_XCDS + x] + ro[b] : 0u;  // positions < 2^32
+  }
+  __syncthreads();
+  pt_block_scan(lcur, nb, s_wave);
+  for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) gdst[b] -= lcur[b];  // modulo 2^32: slot j -> gdst + j
+  __syncthreads();
+}
+
+// Direct key spaces: one tile in key order, straight into the grouped layout
+// (ids, non-key tuples, values, slice indices).  Dynamic LDS as
+// k_pd_scatter.
+template <class RestT, class ValT>
+__global__ void __launch_bounds__(PT_THREADS) k_pd_place(const uint32_t* __restrict__ h, uint32_t d, uint64_t N,
+                                                         KeyCodec codec, KeyCodec rest, const ValT* __restrict__ val,
+                                                         uint32_t nb, uint64_t ntiles,
+                                                         const uint32_t* __restrict__ sstart,
+                                                         const uint32_t* __restrict__ tcnt,
+                                                         const uint32_t* __restrict__ toff,
+                                                         const uint32_t* __restrict__ txcc,
+                                                         const uint32_t* __restrict__ cidx,
+                                                         uint32_t* __restrict__ perm, RestT* __restrict__ rest_out,
+                                                         ValT* __restrict__ gval, uint32_t* __restrict__ slice_of) {
+  const uint64_t t = pt_tile_of(blockIdx.x, ntiles);
+  if (t >= ntiles) return;
+  extern __shared__ unsigned long long pt_smem[];
+  RestT* lrk = reinterpret_cast<RestT*>(pt_smem);           // PT_TILE
+  uint16_t* lb = reinterpret_cast<uint16_t*>(lrk + PT_TILE); // PT_TILE: key of every tile item
+  uint16_t* ls = lb + PT_TILE;                               // PT_TILE: tile index of every key-ordered slot
+  uint32_t* lcur = reinterpret_cast<uint32_t*>(ls + PT_TILE); // nb
+  uint32_t* gdst = lcur + nb;                                // nb
+  __shared__ uint32_t s_wave[PT_THREADS / 64];
+  pc_tile_row(t, nb, sstart, tcnt, toff, txcc, lcur, gdst, s_wave);
+  const uint64_t base = t * PT_TILE;
+  const uint32_t n = (uint32_t)min<uint64_t>(PT_TILE, N - base);
+  for (uint32_t i0 = 0; i0 < n; i0 += PT_U * blockDim.x) {
+    unsigned long long key[PT_U], rk[PT_U];
+    pt_load_keys(h, d, base, i0, n, codec, key);  // < nb
+    pt_load_keys(h, d, base, i0, n, rest, rk);
+#pragma unroll
+    for (int u = 0; u < PT_U; ++u) {
+      const uint32_t i = i0 + threadIdx.x + u * blockDim.x;
+      if (i >= n) continue;
+      lrk[i] = (RestT)rk[u];
+      lb[i] = (uint16_t)key[u];
+      ls[atomicAdd(&lcur[key[u]], 1u)] = (uint16_t)i;
+    }
+  }
+  __syncthreads();
+  for (uint32_t j0 = 0; j0 < n; j0 += PT_U * blockDim.x) {
+    uint32_t ii[PT_U], sl[PT_U];
+    ValT v[PT_U];
+#pragma unroll
+    for (int u = 0; u < PT_U; ++u) {  // the gathers first, all in flight
+      const uint32_t j = j0 + threadIdx.x + u * blockDim.x;
+      ii[u] = j < n ? ls[j] : 0u;
+      v[u] = j < n ? val[base + ii[u]] : ValT(0);
+      sl[u] = j < n ? cidx[lb[ii[u]]] : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < PT_U; ++u) {
+      const uint32_t j = j0 + threadIdx.x + u * blockDim.x;
+      if (j >= n) continue;
+      const uint32_t i = ii[u];
+      const uint32_t p = gdst[lb[i]] + j;
+      perm[p] = (uint32_t)(base + i);
+      rest_out[p] = lrk[i];
+      gval[p] = v[u];
+      slice_of[p] = sl[u];
+    }
+  }
+}
+
+// Wide keys: one tile into hash-bucket order (key, id, payload), as
+// k_pt_scatter, its runs placed at the reserved offsets.  Dynamic LDS as
+// k_pt_scatter.
+template <class RestT, class ValT>
+__global__ void __launch_bounds__(PT_THREADS) k_pt_place(const uint32_t* __restrict__ h, uint32_t d, uint64_t N,
+                                                         KeyCodec codec, KeyCodec rest, const ValT* __restrict__ val,
+                                                         int bbits, uint32_t nb, uint64_t ntiles,
+                                                         const uint32_t* __restrict__ sstart,
+                                                         const uint32_t* __restrict__ tcnt,
+                                                         const uint32_t* __restrict__ toff,
+                                                         const uint32_t* __restrict__ txcc,
+                                                         unsigned long long* __restrict__ pkey,
+                                                         uint32_t* __restrict__ pid,
+                                                         KtPay<RestT, ValT>* __restrict__ ppay) {
+  const uint64_t t = pt_tile_of(blockIdx.x, ntiles);
+  if (t >= ntiles) return;
+  extern __shared__ unsigned long long pt_smem[];
+  unsigned long long* lkey = pt_smem;                       // PT_TILE
+  RestT* lrk = reinterpret_cast<RestT*>(lkey + PT_TILE);    // PT_TILE
+  uint16_t* ls = reinterpret_cast<uint16_t*>(lrk + PT_TILE); // PT_TILE: tile index of every bucket-ordered slot
+  uint32_t* lcur = reinterpret_cast<uint32_t*>(ls + PT_TILE); // nb
+  uint32_t* gdst = lcur + nb;                               // nb
+  __shared__ uint32_t s_wave[PT_THREADS / 64];
+  pc_tile_row(t, nb, sstart, tcnt, toff, txcc, lcur, gdst, s_wave);
+  const uint64_t base = t * PT_TILE;
+  const uint32_t n = (uint32_t)min<uint64_t>(PT_TILE, N - base);
+  for (uint32_t i0 = 0; i0 < n; i0 += PT_U * blockDim.x) {
+    unsigned long long key[PT_U], rk[PT_U];
+    pt_load_keys(h, d, base, i0, n, codec, key);
+    pt_load_keys(h, d, base, i0, n, rest, rk);
+#pragma unroll
+    for (int u = 0; u < PT_U; ++u) {
+      const uint32_t i = i0 + threadIdx.x + u * blockDim.x;
+      if (i >= n) continue;
+      lkey[i] = key[u];
+      lrk[i] = (RestT)rk[u];
+      ls[atomicAdd(&lcur[pt_bucket(key[u], bbits)], 1u)] = (uint16_t)i;
+    }
+  }
+  __syncthreads();
+  for (uint32_t j0 = 0; j0 < n; j0 += PT_U * blockDim.x) {
+    uint32_t ii[PT_U];
+    ValT v[PT_U];
+#pragma unroll
+    for (int u = 0; u < PT_U; ++u) {  // the value gathers first, all in flight
+      const uint32_t j = j0 + threadIdx.x + u * blockDim.x;
+      ii[u] = j < n ? ls[j] : 0u;
+      v[u] = j < n ? val[base + ii[u]] : ValT(0);
+    }
+#pragma unroll
+    for (int u = 0; u < PT_U; ++u) {
+      const uint32_t j = j0 + threadIdx.x + u * blockDim.x;
+      if (j >= n) continue;
+      const uint32_t i = ii[u];
+      const unsigned long long key = lkey[i];
+      const uint32_t p = gdst[pt_bucket(key, bbits)] + j;
+      pkey[p] = key;
+      pid[p] = (uint32_t)(base + i);
+      KtPay<RestT, ValT> pl;
+      pl.k = lrk[i];
+      pl.v = v[u];
+      ppay[p] = pl;
+    }
+  }
+}
+
 // Table state of one bucket: LDS arrays, or the bucket's region of a global
 // table.  Either is touched by the bucket's workgroup alone, so every
 // access is a workgroup-scope atomic: global ones are performed in the L2
@@ -1617,8 +1931,10 @@ int build_partitioned(ptc_ctx* ctx, ptc_hashtable* ht, const ptc_dtensor* t, boo
   const bool region = !ht->direct && ht->cap / 8 >= nb;
   Workspace ws(ctx);
   const size_t pay_bytes = rest64 || !f32 ? 16 : 8;
-  uint32_t* hist = ws.get<uint32_t>((uint64_t)nb * ntiles);
-  unsigned long long* off = ws.get<unsigned long long>((uint64_t)nb * ntiles + 1);
+  unsigned long long* off = ws.get<unsigned long long>(nb + 1);  // bucket starts
+  uint32_t* tcnt = ws.get<uint32_t>((uint64_t)nb * ntiles);      // (tile, bucket) counts, tile-major
+  uint32_t* toff = ws.get<uint32_t>((uint64_t)nb * ntiles);      // (tile, bucket) offsets in the bucket
+  uint32_t* txcc = ws.get<uint32_t>(ntiles);                      // XCD of every tile's count
   unsigned long long* pkey = ws.get<unsigned long long>(N);
   uint32_t* pid = ws.get<uint32_t>(N);
   void* ppay = ws.get<unsigned char>(N * pay_bytes);
@@ -1627,18 +1943,23 @@ int build_partitioned(ptc_ctx* ctx, ptc_hashtable* ht, const ptc_dtensor* t, boo
   KtSlice* tmps = ws.get<KtSlice>(N);
   uint32_t* bkeys = ws.get<uint32_t>(nb);
   unsigned long long* sbase = ws.get<unsigned long long>(nb + 1);
-  uint32_t* ovf = ws.get<uint32_t>(N + 1);  // count, then the list
+  uint32_t* btot = ws.get<uint32_t>((uint64_t)nb * PC_XCDS + 1);  // (bucket, XCD) totals, then the overflow count
+  uint32_t* sstart = ws.get<uint32_t>((uint64_t)nb * PC_XCDS);    // (bucket, XCD) starts
+  uint32_t* ovf_list = ws.get<uint32_t>(N);    // keys whose probe ran past their region
   ht->d_total = dalloc_n<unsigned long long>(ctx, 1);
   ht->slices_cap = std::max<uint64_t>(1, std::min<uint64_t>(N, ht->cap));
   ht->slices = dalloc_n<KtSlice>(ctx, ht->slices_cap);
-  if (!hist || !off || !pkey || !pid || !ppay || !gkey || !gcnt || !tmps || !bkeys || !sbase || !ovf ||
+  if (!tcnt || !toff || !txcc || !off || !pkey || !pid || !ppay || !gkey || !gcnt || !tmps || !bkeys || !sbase || !btot || !sstart || !ovf_list ||
       !ht->d_total || !ht->slices)
     return PTC_ERR_NOMEM;
-  int rc = fill_zero(ctx, ovf, 4);
+  // the overflow count and the bucket totals start at zero: one clear
+  int rc = fill_zero(ctx, btot, (size_t)(nb * PC_XCDS + 1) * 4);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_pt_hist, dim3((unsigned)ntiles), dim3(PT_THREADS), 0, st, t->hedges, t->d, N, ht->codec, bbits,
-                     nb, ntiles, hist);
-  if ((rc = scan_u32_to_u64(ctx, hist, off, (uint64_t)nb * ntiles, off + (uint64_t)nb * ntiles))) return rc;
+  uint32_t* ovf = btot + (uint64_t)nb * PC_XCDS;
+  hipLaunchKernelGGL(k_pc_count, dim3((unsigned)(8 * ((ntiles + 7) / 8))), dim3(PT_THREADS), 0, st, t->hedges, t->d,
+                     N, ht->codec, bbits, nb, ntiles, btot, tcnt, toff, txcc);
+  hipLaunchKernelGGL(k_pc_starts, dim3(1), dim3(1024), 0, st, (const uint32_t*)btot, nb, N, off, sstart, 0,
+                     (KtSlice*)nullptr, (KtEntry*)nullptr, (uint32_t*)nullptr, (unsigned long long*)nullptr);
   if (!region)
     hipLaunchKernelGGL(k_kt_clear, dim3(grid_for(ht->cap, 256)), dim3(256), 0, st, ht->entries, (uint32_t*)nullptr,
                        ht->cap);
@@ -1647,19 +1968,20 @@ int build_partitioned(ptc_ctx* ctx, ptc_hashtable* ht, const ptc_dtensor* t, boo
     const size_t lds = (size_t)PT_TILE * (8 + sizeof(RT) + 2) + (size_t)nb * 8;                                   \
     static size_t lds_set = 0;                                                                                       \
     if (lds > 65536 && lds > lds_set) {                                                                              \
-      PTC_HIP(ctx, hipFuncSetAttribute((const void*)k_pt_scatter<RT, VT>,                                           \
+      PTC_HIP(ctx, hipFuncSetAttribute((const void*)k_pt_place<RT, VT>,                                             \
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));                      \
       lds_set = lds;                                                                                                 \
     }                                                                                                                \
-    hipLaunchKernelGGL((k_pt_scatter<RT, VT>), dim3((unsigned)(8 * ((ntiles + 7) / 8))), dim3(PT_THREADS), lds, st,  \
+    hipLaunchKernelGGL((k_pt_place<RT, VT>), dim3((unsigned)(8 * ((ntiles + 7) / 8))), dim3(PT_THREADS), lds, st,    \
                        t->hedges, t->d, N,                                                                          \
-                       ht->codec, ht->rest, (const VT*)(VALS), bbits, nb, ntiles, (const uint32_t*)hist,            \
-                       (const unsigned long long*)off, pkey, pid, (KtPay<RT, VT>*)ppay);                            \
+                       ht->codec, ht->rest, (const VT*)(VALS), bbits, nb, ntiles, (const uint32_t*)sstart,          \
+                       (const uint32_t*)tcnt, (const uint32_t*)toff, (const uint32_t*)txcc, pkey, pid,              \
+                       (KtPay<RT, VT>*)ppay);                                                                       \
     hipLaunchKernelGGL((k_pt_group<RT, VT>), dim3(nb), dim3(PT_GROUP_THREADS), 0, st,                              \
                        (const unsigned long long*)off,                                                              \
-                       ntiles, nb, N, (const unsigned long long*)pkey, (const uint32_t*)pid,                        \
+                       (uint64_t)1, nb, N, (const unsigned long long*)pkey, (const uint32_t*)pid,                   \
                        (const KtPay<RT, VT>*)ppay, gkey, gcnt, tmps, bkeys, ht->perm, RK, GV, ht->entries, ht->cap, \
-                       region ? 1 : 0, ovf, ovf + 1);                                                               \
+                       region ? 1 : 0, ovf, ovf_list);                                                               \
   } while (0)
   if (rest64) {
     if (f32) PTC_PT(unsigned long long, float, t->vals32, ht->rest_key64, ht->gval32);
@@ -1670,11 +1992,11 @@ int build_partitioned(ptc_ctx* ctx, ptc_hashtable* ht, const ptc_dtensor* t, boo
   }
 #undef PTC_PT
   if ((rc = scan_u32_to_u64(ctx, bkeys, sbase, nb, sbase + nb))) return rc;
-  hipLaunchKernelGGL(k_pt_slices, dim3(nb), dim3(256), 0, st, (const unsigned long long*)off, ntiles, nb,
+  hipLaunchKernelGGL(k_pt_slices, dim3(nb), dim3(256), 0, st, (const unsigned long long*)off, (uint64_t)1, nb,
                      (const uint32_t*)bkeys, (const unsigned long long*)sbase, (const KtSlice*)tmps, ht->slices,
                      ht->entries, ht->cap, region ? 0 : 1, ht->direct ? 1 : 0, ht->d_total, N);
   if (region)
-    hipLaunchKernelGGL(k_pt_overflow, dim3(64), dim3(256), 0, st, (const uint32_t*)ovf, (const uint32_t*)(ovf + 1),
+    hipLaunchKernelGGL(k_pt_overflow, dim3(64), dim3(256), 0, st, (const uint32_t*)ovf, (const uint32_t*)ovf_list,
                        (const KtSlice*)tmps, ht->entries, ht->cap);
   PTC_HIP(ctx, hipGetLastError());
   return PTC_OK;
@@ -1687,32 +2009,37 @@ int build_direct_partitioned(ptc_ctx* ctx, ptc_hashtable* ht, const ptc_dtensor*
   const uint32_t nb = (uint32_t)ht->codec.space;
   const uint64_t ntiles = (N + PT_TILE - 1) / PT_TILE;
   Workspace ws(ctx);
-  uint32_t* hist = ws.get<uint32_t>((uint64_t)nb * ntiles);
-  unsigned long long* off = ws.get<unsigned long long>((uint64_t)nb * ntiles + 1);
+  uint32_t* gcnt = ws.get<uint32_t>((uint64_t)nb * PC_XCDS);   // (key, XCD) totals
+  uint32_t* sstart = ws.get<uint32_t>((uint64_t)nb * PC_XCDS); // (key, XCD) starts
+  uint32_t* tcnt = ws.get<uint32_t>((uint64_t)nb * ntiles);  // (tile, key) counts, tile-major
+  uint32_t* toff = ws.get<uint32_t>((uint64_t)nb * ntiles);  // (tile, key) offsets in the slice
+  uint32_t* txcc = ws.get<uint32_t>(ntiles);                  // XCD of every tile's count
+  unsigned long long* off = ws.get<unsigned long long>(nb + 1);
   uint32_t* cidx = ws.get<uint32_t>(nb);
   ht->slice_of = dalloc_n<uint32_t>(ctx, N);
   ht->d_total = dalloc_n<unsigned long long>(ctx, 1);
   ht->slices_cap = std::max<uint64_t>(1, std::min<uint64_t>(N, ht->cap));
   ht->slices = dalloc_n<KtSlice>(ctx, ht->slices_cap);
-  if (!hist || !off || !cidx || !ht->slice_of || !ht->d_total || !ht->slices) return PTC_ERR_NOMEM;
-  hipLaunchKernelGGL(k_pt_hist, dim3((unsigned)ntiles), dim3(PT_THREADS), 0, st, t->hedges, t->d, N, ht->codec, -1, nb,
-                     ntiles, hist);
-  int rc = scan_u32_to_u64(ctx, hist, off, (uint64_t)nb * ntiles, off + (uint64_t)nb * ntiles);
+  if (!gcnt || !sstart || !tcnt || !toff || !txcc || !off || !cidx || !ht->slice_of || !ht->d_total || !ht->slices) return PTC_ERR_NOMEM;
+  int rc = fill_zero(ctx, gcnt, (size_t)nb * PC_XCDS * 4);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_pd_slices, dim3(1), dim3(1024), 0, st, (const unsigned long long*)off, ntiles, nb, N, ht->slices,
+  hipLaunchKernelGGL(k_pc_count, dim3((unsigned)(8 * ((ntiles + 7) / 8))), dim3(PT_THREADS), 0, st, t->hedges, t->d,
+                     N, ht->codec, -1, nb, ntiles, gcnt, tcnt, toff, txcc);
+  hipLaunchKernelGGL(k_pc_starts, dim3(1), dim3(1024), 0, st, (const uint32_t*)gcnt, nb, N, off, sstart, 1, ht->slices,
                      ht->entries, cidx, ht->d_total);
 #define PTC_PD(RT, VT, VALS, RK, GV)                                                                                  \
   do {                                                                                                                \
     const size_t lds = (size_t)PT_TILE * (sizeof(RT) + 4) + (size_t)nb * 8;                                           \
     static size_t lds_set = 0;                                                                                        \
     if (lds > 65536 && lds > lds_set) {                                                                               \
-      PTC_HIP(ctx, hipFuncSetAttribute((const void*)k_pd_scatter<RT, VT>,                                            \
+      PTC_HIP(ctx, hipFuncSetAttribute((const void*)k_pd_place<RT, VT>,                                              \
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));                       \
       lds_set = lds;                                                                                                  \
     }                                                                                                                 \
-    hipLaunchKernelGGL((k_pd_scatter<RT, VT>), dim3((unsigned)(8 * ((ntiles + 7) / 8))), dim3(PT_THREADS), lds, st,   \
-                       t->hedges, t->d, N, ht->codec, ht->rest, (const VT*)(VALS), nb, ntiles, (const uint32_t*)hist, \
-                       (const unsigned long long*)off, (const uint32_t*)cidx, ht->perm, RK, GV, ht->slice_of);      \
+    hipLaunchKernelGGL((k_pd_place<RT, VT>), dim3((unsigned)(8 * ((ntiles + 7) / 8))), dim3(PT_THREADS), lds, st,     \
+                       t->hedges, t->d, N, ht->codec, ht->rest, (const VT*)(VALS), nb, ntiles,                      \
+                       (const uint32_t*)sstart, (const uint32_t*)tcnt, (const uint32_t*)toff, (const uint32_t*)txcc, \
+                       (const uint32_t*)cidx, ht->perm, RK, GV, ht->slice_of);                                      \
   } while (0)
   if (rest64) {
     if (f32) PTC_PD(unsigned long long, float, t->vals32, ht->rest_key64, ht->gval32);
