@@ -48,6 +48,10 @@ struct RowParams {
   unsigned long long* row_flops;
   unsigned long long* packed;     // (heavy << 32) | nchunk, scanned later
   unsigned long long* slabs;      // nchunk of split rows, 0 otherwise (scanned later)
+  // row features (non-NULL): zeroed for rows past the table's key count;
+  // the positions' scan wrote every other row's
+  uint32_t* row_hits;
+  uint32_t* row_maxlen;
 };
 
 struct ItemParams {
@@ -200,8 +204,6 @@ struct EmitParams {
 
 int launch_probe(ptc_ctx* ctx, const ProbeParams& P, bool rest64);
 int launch_rows(ptc_ctx* ctx, const RowParams& P);
-int launch_row_features(ptc_ctx* ctx, const uint32_t* row_of_pos, uint64_t npos, const uint32_t* pos_len,
-                        uint32_t* row_hits, uint32_t* row_maxlen);
 int launch_items(ptc_ctx* ctx, const ItemParams& P, const ItemP0Params& Q);
 int launch_hash_caps(ptc_ctx* ctx, const HashCapParams& P);
 int launch_caps(ptc_ctx* ctx, const CapParams& P);

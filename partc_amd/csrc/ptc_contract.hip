@@ -130,77 +130,6 @@ __global__ void __launch_bounds__(256) k_probe(ProbeParams P) {
   }
 }
 
-// Per-row features for the learned method: contracted keys that hit B and
-// the longest slice.  A positions are row-grouped, so each thread reduces 16
-// consecutive positions, then a wave-segmented reduction leaves one atomic
-// per (wave, row) -- about npos/1024 atomics on the hottest row (one atomic
-// per 64 positions serialises on that row's address and cost 4x the probe).
-// The block's 4096 positions are loaded coalesced into LDS first, padded by
-// one word per 16 so a thread's 16 consecutive words fall in distinct banks
-// across the wave.
-constexpr int RF_ITEMS = 16;
-constexpr int RF_THREADS = 256;
-constexpr int RF_BLOCK = RF_THREADS * RF_ITEMS;
-__device__ __forceinline__ uint32_t rf_slot(uint32_t i) { return i + (i >> 4); }
-__global__ void __launch_bounds__(RF_THREADS) k_row_features(const uint32_t* __restrict__ row_of_pos, uint64_t npos,
-                                                             const uint32_t* __restrict__ pos_len,
-                                                             uint32_t* __restrict__ row_hits,
-                                                             uint32_t* __restrict__ row_maxlen) {
-  __shared__ uint32_t s_row[RF_BLOCK + RF_BLOCK / 16];
-  __shared__ uint32_t s_len[RF_BLOCK + RF_BLOCK / 16];
-  const int lane = threadIdx.x & 63;
-  const uint64_t b0 = (uint64_t)blockIdx.x * RF_BLOCK;
-#pragma unroll
-  for (int k = 0; k < RF_ITEMS; ++k) {
-    const uint32_t i = k * RF_THREADS + threadIdx.x;
-    const uint64_t p = b0 + i;
-    s_row[rf_slot(i)] = p < npos ? row_of_pos[p] : 0xFFFFFFFFu;
-    s_len[rf_slot(i)] = p < npos ? pos_len[p] : 0u;
-  }
-  __syncthreads();
-  uint32_t row = 0xFFFFFFFFu, h = 0, m = 0;
-  for (int k = 0; k < RF_ITEMS; ++k) {
-    const uint32_t i = threadIdx.x * RF_ITEMS + k;
-    const uint32_t r = s_row[rf_slot(i)], l = s_len[rf_slot(i)];
-    if (r == 0xFFFFFFFFu) break;  // past npos
-    if (r != row) {
-      if (row != 0xFFFFFFFFu) {  // flush a finished run (rare: rows are long)
-        if (h) atomicAdd(&row_hits[row], h);
-        if (m) atomicMax(&row_maxlen[row], m);
-      }
-      row = r;
-      h = 0;
-      m = 0;
-    }
-    h += l ? 1u : 0u;
-    m = max(m, l);
-  }
-  // wave-segmented reduction of each thread's last run
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const uint32_t h2 = __shfl_down(h, o, 64);
-    const uint32_t m2 = __shfl_down(m, o, 64);
-    const uint32_t r2 = __shfl_down(row, o, 64);
-    if (lane + o < 64 && r2 == row) {
-      h += h2;
-      m = max(m, m2);
-    }
-  }
-  const uint32_t prev = __shfl_up(row, 1, 64);
-  if (row != 0xFFFFFFFFu && (lane == 0 || prev != row)) {
-    if (h) atomicAdd(&row_hits[row], h);
-    if (m) atomicMax(&row_maxlen[row], m);
-  }
-}
-
-int launch_row_features(ptc_ctx* ctx, const uint32_t* row_of_pos, uint64_t npos, const uint32_t* pos_len,
-                        uint32_t* row_hits, uint32_t* row_maxlen) {
-  if (!npos) return PTC_OK;
-  hipLaunchKernelGGL(k_row_features, dim3((unsigned)((npos + RF_BLOCK - 1) / RF_BLOCK)), dim3(RF_THREADS), 0,
-                     ctx->stream, row_of_pos, npos, pos_len, row_hits, row_maxlen);
-  PTC_HIP(ctx, hipGetLastError());
-  return PTC_OK;
-}
 
 // ------------------------------------------------------------------ rows
 
@@ -215,6 +144,10 @@ __global__ void k_rows(RowParams P) {
       P.row_flops[r] = 0;
       P.packed[r] = 0;
       P.slabs[r] = 0;
+      if (P.row_hits) {  // features of rows past the table (the scan wrote every row with positions)
+        P.row_hits[r] = 0;
+        P.row_maxlen[r] = 0;
+      }
       continue;
     }
     KtSlice s = P.rows[r];

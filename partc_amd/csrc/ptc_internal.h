@@ -239,6 +239,26 @@ int scan_u64(ptc_ctx* ctx, const unsigned long long* in, unsigned long long* out
 
 int fill_zero(ptc_ctx* ctx, void* p, size_t bytes);
 
+// Per scan tile of FEAT_TILE positions: the rows of its first and last
+// positions and their partial counts of positions that hit B (h) and
+// longest slices (m) within the tile.
+constexpr uint32_t FEAT_TILE = 2048;
+struct FeatPart {
+  uint32_t fr, lr, fh, fm, lh, lm;
+};
+struct ScanFeatures {
+  const uint32_t* row_of_pos;  // row of every element (non-decreasing; every row non-empty)
+  uint32_t* row_hits;          // per row: elements with a nonzero value
+  uint32_t* row_maxlen;        // per row: the largest element
+  FeatPart* parts;             // scratch: one per tile
+};
+
+// Exclusive scan of the A positions' slice lengths that also writes the row
+// features (every row that has positions; rows past them are left alone).
+int scan_u32_features(ptc_ctx* ctx, const uint32_t* in, unsigned long long* out, uint64_t n,
+                      unsigned long long* total, const uint32_t* row_of_pos, uint32_t* row_hits,
+                      uint32_t* row_maxlen, FeatPart* parts);
+
 // Device words (4 or 8 bytes, zero-extended to 8) gathered into dst[0..n)
 // by one launch on the context stream.
 struct WordSources {

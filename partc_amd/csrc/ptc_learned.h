@@ -18,6 +18,9 @@ struct LearnedInputs {
 
 // pred[r] = predicted nonzeros of row r (device array of nrows floats).
 int learned_predict(ptc_ctx* ctx, const ptc_mlp* mlp, const LearnedInputs& in, float* pred);
+// cap[r] = the learned capacity of row r (features, inference and the
+// capacity rule of the single-pass methods in one launch).
+int learned_caps(ptc_ctx* ctx, const ptc_mlp* mlp, const LearnedInputs& in, double scale, uint32_t* cap);
 // Feature rows X (nrows x PTC_MLP_IN) and upper bounds ub (device arrays).
 int learned_features(ptc_ctx* ctx, const LearnedInputs& in, float* X, float* ub);
 

@@ -133,6 +133,64 @@ __global__ void __launch_bounds__(256) k_mlp_predict(const float* __restrict__ w
   }
 }
 
+// Features, inference and the row's capacity in one launch (the learned
+// single-pass method): the features of k_mlp_features, the network of
+// k_mlp_predict (the same operations in the same order), then the capacity
+// rule of k_caps: clamp(ceil(scale * prediction), longest slice,
+// min(flops, columns)).  Lane i holds hidden unit i's weights in registers
+// (its first-layer row, its second-layer row W2[i][*], biases and output
+// weight), loaded once per wavefront; each wavefront then runs a few rows.
+constexpr int CAPS_ROWS_PER_WAVE = 4;
+__global__ void __launch_bounds__(256) k_mlp_caps(const float* __restrict__ wg, LearnedInputs in, double scale,
+                                                  uint32_t* __restrict__ cap) {
+  static_assert(MLP_H == 64, "one lane per hidden unit");
+  const int lane = threadIdx.x & 63;
+  float w1r[MLP_IN], w2r[MLP_H];
+#pragma unroll
+  for (int k = 0; k < MLP_IN; ++k) w1r[k] = wg[OFF_W1 + lane * MLP_IN + k];
+  static_assert(OFF_W2 % 4 == 0 && MLP_H % 4 == 0, "second-layer rows are 16-byte aligned");
+  const float4* w2row = reinterpret_cast<const float4*>(wg + OFF_W2 + lane * MLP_H);
+#pragma unroll
+  for (int k = 0; k < MLP_H / 4; ++k) {
+    const float4 q = w2row[k];
+    w2r[4 * k] = q.x;
+    w2r[4 * k + 1] = q.y;
+    w2r[4 * k + 2] = q.z;
+    w2r[4 * k + 3] = q.w;
+  }
+  const float b1 = wg[OFF_B1 + lane], b2 = wg[OFF_B2 + lane], w3 = wg[OFF_W3 + lane], b3 = wg[OFF_B3];
+  const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x / 64);
+  for (uint64_t r = (uint64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64; r < in.nrows; r += waves) {
+    const unsigned long long fl = in.row_flops[r];
+    const uint32_t maxlen = in.row_maxlen[r];
+    const double f = (double)fl;
+    const double u = f < (double)in.ncols ? f : (double)in.ncols;
+    // lane k < MLP_IN computes feature k
+    const double src = lane == 0 ? (double)in.rows[r].len : lane == 1 ? f : lane == 2 ? (double)in.row_hits[r]
+                     : lane == 3 ? (double)maxlen : lane == 4 ? (double)in.ncols : u;
+    const float xl = lane < MLP_IN ? (float)(log1p(src) * 0.125) : 0.f;
+    float a = b1;
+#pragma unroll
+    for (int k = 0; k < MLP_IN; ++k) a += w1r[k] * __shfl(xl, k, 64);
+    const float h1 = a > 0.f ? a : 0.f;
+    float s = b2;
+#pragma unroll
+    for (int k = 0; k < MLP_H; ++k) s += w2r[k] * __shfl(h1, k, 64);
+    const float h2 = s > 0.f ? s : 0.f;
+    float z = w3 * h2;
+    for (int o = 32; o > 0; o >>= 1) z += __shfl_down(z, o, 64);
+    if (lane == 0) {
+      const float pred = (float)u * sigmoidf_(z + b3);
+      const unsigned long long ub = fl < in.ncols ? fl : in.ncols;
+      const double want = ceil((double)pred * scale);
+      const unsigned long long wv = want < 0 ? 0ull : (unsigned long long)fmin(want, 1.8e19);
+      unsigned long long c = wv < maxlen ? maxlen : wv;
+      if (c > ub) c = ub;
+      cap[r] = c > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)c;
+    }
+  }
+}
+
 // training forward: loss and dL/dz per sample (mean over the batch)
 __global__ void __launch_bounds__(256) k_mlp_forward(const float* __restrict__ wg, const float* __restrict__ X,
                                                      const float* __restrict__ ub, const float* __restrict__ y,
@@ -250,6 +308,16 @@ int learned_predict(ptc_ctx* ctx, const ptc_mlp* mlp, const LearnedInputs& in, f
   hipLaunchKernelGGL(k_mlp_features, dim3(gridn(in.nrows, 256)), dim3(256), 0, ctx->stream, in, X, ub);
   hipLaunchKernelGGL(k_mlp_predict, dim3(gridn(in.nrows, PREDICT_ROWS_PER_BLOCK, 2048)), dim3(256), 0, ctx->stream,
                      (const float*)mlp->w, (const float*)X, (const float*)ub, in.nrows, pred);
+  PTC_HIP(ctx, hipGetLastError());
+  return PTC_OK;
+}
+
+int learned_caps(ptc_ctx* ctx, const ptc_mlp* mlp, const LearnedInputs& in, double scale, uint32_t* cap) {
+  if (!mlp) mlp = g_default;
+  if (!mlp || !mlp->w) return set_err(ctx, PTC_ERR_ARG, "no row-size model");
+  if (!in.nrows) return PTC_OK;
+  hipLaunchKernelGGL(k_mlp_caps, dim3(gridn(in.nrows, 4 * CAPS_ROWS_PER_WAVE, 2048)), dim3(256), 0, ctx->stream,
+                     (const float*)mlp->w, in, scale, cap);
   PTC_HIP(ctx, hipGetLastError());
   return PTC_OK;
 }

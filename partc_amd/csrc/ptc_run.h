@@ -56,6 +56,11 @@ struct Run {
   unsigned long long* out_off = nullptr;
   bool want_features = false;  // per-row hit counts and longest slices (learned features)
   const uint32_t* row_of_pos = nullptr;  // HT_A's slice index of every A position
+  // single-pass methods: every row's output capacity, their offsets and sum
+  // (computed with the preprocessing, the sum read back with its totals)
+  uint32_t* cap = nullptr;
+  unsigned long long* cap_off = nullptr;
+  unsigned long long capsum = 0;
   uint32_t threads = PB;       // threads per row-pass workgroup
   bool force_flags = false;    // touch flags whatever the values (set before prepare)
   const uint16_t* bcol16 = nullptr;  // dense rows: B's column keys as 16 bits
@@ -71,9 +76,11 @@ struct Run {
 
 uint64_t lds_limit(ptc_ctx* ctx);
 
-// Preprocessing (key probes, flop prefix, rows, work items).  The launch half
-// copies four totals to tot4 (device): flops, packed item/heavy counts,
-// slabs, HT_A's key total; the finish half takes them on the host.
+// Preprocessing (key probes, flop prefix, rows, work items and, for the
+// single-pass methods, the rows' output capacities).  The launch half copies
+// four totals to tot4 (device): flops, packed item/heavy counts, slabs,
+// HT_A's key total -- and for a single-pass method a fifth, the capacities'
+// sum; the finish half takes them on the host.
 int prepare_launch(Run& R, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B, const ptc_dtensor* A,
                    const ptc_dtensor* B, const ptc_options& opt, unsigned long long* tot4);
 int prepare_finish(Run& R, const unsigned long long* hostv);

@@ -302,6 +302,270 @@ __global__ void __launch_bounds__(SCAN_SMALL_THREADS) k_scan_small(const In* __r
   if (total_out && threadIdx.x == SCAN_SMALL_THREADS - 1) *total_out = run;
 }
 
+// Two-launch scan for long inputs: the per-tile totals (k_scan_reduce), then
+// every tile sums the earlier tiles' totals itself (a few thousand words,
+// read from L2) and scans its own elements.  No atomics and no waiting on
+// other workgroups: on this GPU a chain of same-address device atomics (a
+// ticket counter, a look-back flag) costs tens of nanoseconds per step.
+//
+// With features (the A positions' scan of the learned method) every row
+// (a contiguous, non-empty range of elements) gets its count of elements
+// with a nonzero value and its largest element, written once, by the tile
+// in which the row ends, with no global atomics: the first launch also
+// leaves each tile's FeatPart (its first and last rows and their partial
+// sums in the tile); the second reduces its own rows with LDS atomics,
+// writes the rows that end inside it, and for a row that entered from
+// earlier tiles adds those tiles' partials, found by a parallel walk back
+// over the parts (a row is the only row of every tile strictly inside it).
+
+// per tile: the element total and the FeatPart
+__global__ void __launch_bounds__(SCAN_THREADS) k_scan_reduce_feat(const uint32_t* __restrict__ in, uint64_t n,
+                                                                   unsigned long long* __restrict__ block_sums,
+                                                                   ScanFeatures F) {
+  static_assert(SCAN_TILE == FEAT_TILE, "row features are kept per scan tile");
+  __shared__ unsigned long long s_w[SCAN_THREADS / 64][5];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const uint64_t b0 = (uint64_t)blockIdx.x * SCAN_TILE;
+  const uint64_t last = min<uint64_t>(n, b0 + SCAN_TILE) - 1;
+  const uint32_t fr = F.row_of_pos[b0], lr = F.row_of_pos[last];
+  unsigned long long sum = 0;
+  uint32_t fh = 0, fm = 0, lh = 0, lm = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    const uint64_t e = b0 + (uint64_t)k * SCAN_THREADS + threadIdx.x;
+    if (e > last) continue;
+    const uint32_t l = in[e], r = F.row_of_pos[e];
+    sum += l;
+    if (r == fr) {
+      fh += l ? 1u : 0u;
+      fm = max(fm, l);
+    }
+    if (r == lr) {
+      lh += l ? 1u : 0u;
+      lm = max(lm, l);
+    }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    sum += __shfl_xor(sum, o, 64);
+    fh += __shfl_xor(fh, o, 64);
+    fm = max(fm, (uint32_t)__shfl_xor(fm, o, 64));
+    lh += __shfl_xor(lh, o, 64);
+    lm = max(lm, (uint32_t)__shfl_xor(lm, o, 64));
+  }
+  if (lane == 0) {
+    s_w[wid][0] = sum;
+    s_w[wid][1] = fh;
+    s_w[wid][2] = fm;
+    s_w[wid][3] = lh;
+    s_w[wid][4] = lm;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    FeatPart P;
+    P.fr = fr;
+    P.lr = lr;
+    P.fh = P.fm = P.lh = P.lm = 0;
+    sum = 0;
+    for (int w = 0; w < SCAN_THREADS / 64; ++w) {
+      sum += s_w[w][0];
+      P.fh += (uint32_t)s_w[w][1];
+      P.fm = max(P.fm, (uint32_t)s_w[w][2]);
+      P.lh += (uint32_t)s_w[w][3];
+      P.lm = max(P.lm, (uint32_t)s_w[w][4]);
+    }
+    block_sums[blockIdx.x] = sum;
+    F.parts[blockIdx.x] = P;
+  }
+}
+
+template <class In, bool FEAT>
+__global__ void __launch_bounds__(SCAN_THREADS) k_scan_apply_p(const In* __restrict__ in,
+                                                               unsigned long long* __restrict__ out, uint64_t n,
+                                                               const unsigned long long* __restrict__ block_sums,
+                                                               unsigned long long* __restrict__ total_out,
+                                                               ScanFeatures F) {
+  __shared__ unsigned long long s_tile[SCAN_TILE + SCAN_TILE / SCAN_ITEMS];
+  __shared__ uint32_t s_row[FEAT ? SCAN_TILE + SCAN_TILE / SCAN_ITEMS : 1];
+  // once the tile's results are stored, its LDS holds the per-row sums
+  // (at most one row per element): fewer bytes a workgroup, more of them
+  // per CU
+  static_assert(sizeof(s_tile) >= 2 * SCAN_TILE * sizeof(uint32_t), "row sums fit the tile's LDS");
+  uint32_t* s_h = reinterpret_cast<uint32_t*>(s_tile);
+  uint32_t* s_m = s_h + SCAN_TILE;
+  __shared__ unsigned long long s_red[SCAN_THREADS / 64];
+  __shared__ uint32_t s_stop, s_ch[SCAN_THREADS / 64], s_cm[SCAN_THREADS / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const uint64_t b0 = (uint64_t)blockIdx.x * SCAN_TILE;
+  // this tile's prefix: the earlier tiles' totals
+  unsigned long long pre = 0;
+  for (uint32_t j = threadIdx.x; j < blockIdx.x; j += SCAN_THREADS) pre += block_sums[j];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) pre += __shfl_xor(pre, o, 64);
+  if (lane == 0) s_red[wid] = pre;
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    const uint32_t i = k * SCAN_THREADS + threadIdx.x;
+    s_tile[scan_slot(i)] = load_elem(in, b0 + i, n);
+    if (FEAT) s_row[scan_slot(i)] = b0 + i < n ? F.row_of_pos[b0 + i] : 0xFFFFFFFFu;
+  }
+  __syncthreads();
+  pre = 0;
+#pragma unroll
+  for (int w = 0; w < SCAN_THREADS / 64; ++w) pre += s_red[w];
+  unsigned long long v[SCAN_ITEMS];
+  unsigned long long sum = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    v[k] = s_tile[scan_slot(threadIdx.x * SCAN_ITEMS + k)];
+    sum += v[k];
+  }
+  unsigned long long tot;
+  unsigned long long run = block_exclusive_scan(sum, &tot) + pre;
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    s_tile[scan_slot(threadIdx.x * SCAN_ITEMS + k)] = run;
+    run += v[k];
+  }
+  if (total_out && blockIdx.x == gridDim.x - 1 && threadIdx.x == SCAN_THREADS - 1) *total_out = run;
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    const uint32_t i = k * SCAN_THREADS + threadIdx.x;
+    if (b0 + i < n) out[b0 + i] = s_tile[scan_slot(i)];
+  }
+  if (!FEAT) return;
+  uint32_t rows[SCAN_ITEMS];
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) rows[k] = s_row[scan_slot(threadIdx.x * SCAN_ITEMS + k)];
+  __syncthreads();  // every result is stored: the tile's LDS becomes the row sums
+  for (uint32_t i = threadIdx.x; i < 2 * SCAN_TILE; i += SCAN_THREADS) s_h[i] = 0;
+  __syncthreads();
+  uint32_t fr = 0;
+  {
+    // each thread's run of consecutive elements, into the tile's per-row slots
+    fr = s_row[scan_slot(0)];
+    uint32_t row = 0xFFFFFFFFu, h = 0, m = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+      const uint32_t r = rows[k];
+      if (r == 0xFFFFFFFFu) break;  // past n
+      if (r != row) {
+        if (row != 0xFFFFFFFFu) {
+          if (h) atomicAdd(&s_h[row - fr], h);
+          if (m) atomicMax(&s_m[row - fr], m);
+        }
+        row = r;
+        h = 0;
+        m = 0;
+      }
+      const uint32_t l = (uint32_t)v[k];
+      h += l ? 1u : 0u;
+      m = max(m, l);
+    }
+    if (row != 0xFFFFFFFFu) {
+      if (h) atomicAdd(&s_h[row - fr], h);
+      if (m) atomicMax(&s_m[row - fr], m);
+    }
+  }
+  __syncthreads();
+  {
+    const uint32_t t = blockIdx.x, nt = gridDim.x;
+    const uint32_t lr = s_row[scan_slot((uint32_t)min<uint64_t>(SCAN_TILE, n - b0) - 1)];
+    for (uint32_t i = threadIdx.x + 1; i < lr - fr; i += SCAN_THREADS) {  // rows strictly inside the tile
+      F.row_hits[fr + i] = s_h[i];
+      F.row_maxlen[fr + i] = s_m[i];
+    }
+    // a row ends in this tile unless the next tile starts with it
+    const bool next_same_f = t + 1 < nt && F.parts[t + 1].fr == fr;
+    const bool next_same_l = t + 1 < nt && F.parts[t + 1].fr == lr;
+    if (lr != fr && threadIdx.x == 0 && !next_same_l) {
+      F.row_hits[lr] = s_h[lr - fr];
+      F.row_maxlen[lr] = s_m[lr - fr];
+    }
+    const bool fr_ends = lr != fr || !next_same_f;
+    if (fr_ends) {
+      // the first row's sums from earlier tiles: every tile it fills whole,
+      // then the tile where it starts (its last row there)
+      uint32_t ch = 0, cm = 0;
+      if (t > 0 && F.parts[t - 1].lr == fr) {
+        for (int64_t base = (int64_t)t - 1;; base -= SCAN_THREADS) {
+          if (threadIdx.x == 0) s_stop = SCAN_THREADS;
+          __syncthreads();
+          const int64_t tp = base - (int64_t)threadIdx.x;
+          FeatPart P{};
+          bool edge = tp < 0;
+          if (!edge) {
+            P = F.parts[tp];
+            edge = P.fr != fr;  // the tile where the row starts
+          }
+          if (edge) atomicMin(&s_stop, (uint32_t)threadIdx.x);
+          __syncthreads();
+          const uint32_t stop = s_stop;
+          uint32_t h = 0, m = 0;
+          if (threadIdx.x < stop) {
+            h = P.fh;  // a tile the row fills whole
+            m = P.fm;
+          } else if (threadIdx.x == stop && tp >= 0) {
+            h = P.lh;
+            m = P.lm;
+          }
+#pragma unroll
+          for (int o = 32; o > 0; o >>= 1) {
+            h += __shfl_xor(h, o, 64);
+            m = max(m, (uint32_t)__shfl_xor(m, o, 64));
+          }
+          if (lane == 0) {
+            s_ch[wid] = h;
+            s_cm[wid] = m;
+          }
+          __syncthreads();
+#pragma unroll
+          for (int w = 0; w < SCAN_THREADS / 64; ++w) {
+            ch += s_ch[w];
+            cm = max(cm, s_cm[w]);
+          }
+          if (stop < SCAN_THREADS) break;  // the same value in every thread
+          __syncthreads();
+        }
+      }
+      if (threadIdx.x == 0) {
+        F.row_hits[fr] = ch + s_h[0];
+        F.row_maxlen[fr] = max(cm, s_m[0]);
+      }
+    }
+  }
+}
+
+template <class In, bool FEAT>
+static int scan_two(ptc_ctx* ctx, const In* in, unsigned long long* out, uint64_t n, unsigned long long* total,
+                    const ScanFeatures& F) {
+  const uint64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+  Workspace ws(ctx);
+  unsigned long long* sums = ws.get<unsigned long long>(nb);
+  if (!sums) return PTC_ERR_NOMEM;
+  if constexpr (FEAT)
+    hipLaunchKernelGGL(k_scan_reduce_feat, dim3((unsigned)nb), dim3(SCAN_THREADS), 0, ctx->stream, in, n, sums, F);
+  else
+    hipLaunchKernelGGL((k_scan_reduce<In>), dim3((unsigned)nb), dim3(SCAN_THREADS), 0, ctx->stream, in, n, sums);
+  hipLaunchKernelGGL((k_scan_apply_p<In, FEAT>), dim3((unsigned)nb), dim3(SCAN_THREADS), 0, ctx->stream, in, out, n,
+                     (const unsigned long long*)sums, total, F);
+  PTC_HIP(ctx, hipGetLastError());
+  return PTC_OK;
+}
+// tiles whose prefix a workgroup sums itself (at most 64 words a thread)
+constexpr uint64_t SCAN_TWO_MAX_TILES = 64ull * SCAN_THREADS;
+
+int scan_u32_features(ptc_ctx* ctx, const uint32_t* in, unsigned long long* out, uint64_t n,
+                      unsigned long long* total, const uint32_t* row_of_pos, uint32_t* row_hits,
+                      uint32_t* row_maxlen, FeatPart* parts) {
+  if (n == 0) return total ? fill_zero(ctx, total, sizeof(unsigned long long)) : PTC_OK;
+  if ((n + SCAN_TILE - 1) / SCAN_TILE > SCAN_TWO_MAX_TILES)
+    return set_err(ctx, PTC_ERR_RANGE, "feature scan of %llu elements", (unsigned long long)n);
+  ScanFeatures F{row_of_pos, row_hits, row_maxlen, parts};
+  return scan_two<uint32_t, true>(ctx, in, out, n, total, F);
+}
+
 template <class In>
 static int scan_impl(ptc_ctx* ctx, const In* in, unsigned long long* out, uint64_t n,
                      unsigned long long* total) {
@@ -314,6 +578,7 @@ static int scan_impl(ptc_ctx* ctx, const In* in, unsigned long long* out, uint64
     PTC_HIP(ctx, hipGetLastError());
     return PTC_OK;
   }
+  if ((n + SCAN_TILE - 1) / SCAN_TILE <= SCAN_TWO_MAX_TILES) return scan_two<In, false>(ctx, in, out, n, total, ScanFeatures{});
   uint64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
   if (nb == 1) {
     hipLaunchKernelGGL((k_scan_apply<In>), dim3(1), dim3(SCAN_THREADS), 0, ctx->stream, in, out, n,

@@ -22,8 +22,6 @@
 
 namespace ptc {
 
-static constexpr unsigned long long STAGING_BOUND = 1ull << 28;  // entries (4 GB of column keys + FP64 values)
-
 struct Timer {
   ptc_ctx* ctx;
   hipEvent_t ev[8];
@@ -425,6 +423,7 @@ int prepare_launch(Run& R, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B,
     return PTC_ERR_NOMEM;
 
   // ---------------- preprocessing: probe, flop prefix, rows, items
+  const bool features = opt.method == PTC_METHOD_LEARNED || R.want_features;
   if ((rc = fill_zero(ctx, R.overflow, 16))) return rc;
   {
     ProbeParams P{};
@@ -445,11 +444,17 @@ int prepare_launch(Run& R, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B,
     P.row_maxlen = R.row_maxlen;
     if ((rc = launch_probe(ctx, P, HT_A->rest_key64 != nullptr))) return rc;
   }
-  if (opt.method == PTC_METHOD_LEARNED || R.want_features) {
-    if ((rc = fill_zero(ctx, R.row_hits, R.nrows * 4)) || (rc = fill_zero(ctx, R.row_maxlen, R.nrows * 4))) return rc;
-    if ((rc = launch_row_features(ctx, row_of_pos, R.npos, R.pos_len, R.row_hits, R.row_maxlen))) return rc;
+  // the flop prefix over A positions; the row features ride on the same pass
+  FeatPart* parts = nullptr;
+  if (features) {
+    parts = ws.get<FeatPart>((R.npos + FEAT_TILE - 1) / FEAT_TILE + 1);
+    if (!parts) return PTC_ERR_NOMEM;
+    if ((rc = scan_u32_features(ctx, R.pos_len, R.apre, R.npos, R.apre + R.npos, row_of_pos, R.row_hits,
+                                R.row_maxlen, parts)))
+      return rc;
+  } else if ((rc = scan_u32_to_u64(ctx, R.pos_len, R.apre, R.npos, R.apre + R.npos))) {
+    return rc;
   }
-  if ((rc = scan_u32_to_u64(ctx, R.pos_len, R.apre, R.npos, R.apre + R.npos))) return rc;
   {
     RowParams P{};
     P.nrows = R.nrows;
@@ -470,19 +475,55 @@ int prepare_launch(Run& R, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B,
     P.row_flops = R.row_flops;
     P.packed = R.packed;
     P.slabs = R.row_slab;
+    P.row_hits = parts ? R.row_hits : nullptr;
+    P.row_maxlen = parts ? R.row_maxlen : nullptr;
     if ((rc = launch_rows(ctx, P))) return rc;
   }
   if ((rc = scan_u64(ctx, R.packed, R.packed, R.nrows, R.packed + R.nrows))) return rc;
   if ((rc = scan_u64(ctx, R.row_slab, R.row_slab, R.nrows, R.row_slab + R.nrows))) return rc;
+  // Single-pass methods: every row's capacity (rows past HT_A's key count
+  // have no flops: capacity 0) and their sum, read back with the totals
+  // below, so sizing the output costs no round trip of its own.  Upper
+  // bound: min(flops, columns); learned: features, inference and the
+  // clamped prediction in one launch.
+  const bool single = opt.method != PTC_METHOD_TWO_PASS;
+  if (single) {
+    R.cap = ws.get<uint32_t>(R.nrows + 1);
+    R.cap_off = ws.get<unsigned long long>(R.nrows + 1);
+    if (!R.cap || !R.cap_off) return PTC_ERR_NOMEM;
+    const double scale = opt.predict_scale > 0 ? opt.predict_scale : 1.15;
+    if (opt.method == PTC_METHOD_LEARNED) {
+      LearnedInputs L{};
+      L.nrows = R.nrows;
+      L.rows = HT_A->slices;
+      L.row_flops = R.row_flops;
+      L.row_hits = R.row_hits;
+      L.row_maxlen = R.row_maxlen;
+      L.ncols = R.ncols;
+      if ((rc = learned_caps(ctx, opt.mlp, L, scale, R.cap))) return rc;
+    } else {
+      CapParams P{};
+      P.nrows = R.nrows;
+      P.row_flops = R.row_flops;
+      P.row_maxlen = R.row_maxlen;
+      P.ncols = R.ncols;
+      P.pred = nullptr;
+      P.scale = scale;
+      P.cap = R.cap;
+      if ((rc = launch_caps(ctx, P))) return rc;
+    }
+    if ((rc = scan_u32_to_u64(ctx, R.cap, R.cap_off, R.nrows, R.cap_off + R.nrows))) return rc;
+  }
   // the totals and HT_A's key count, for one host round trip
   {
     WordSources W{};
-    W.n = 4;
+    W.n = single ? 5 : 4;
     W.src[0] = R.apre + R.npos;
     W.src[1] = R.packed + R.nrows;
     W.src[2] = R.row_slab + R.nrows;
     W.src[3] = HT_A->d_total;
-    for (int i = 0; i < 4; ++i) W.bytes[i] = 8;
+    if (single) W.src[4] = R.cap_off + R.nrows;
+    for (uint32_t i = 0; i < W.n; ++i) W.bytes[i] = 8;
     if ((rc = collect_words(ctx, tot, W))) return rc;
   }
   return PTC_OK;
@@ -505,6 +546,7 @@ int prepare_finish(Run& R, const unsigned long long* hostv) {
   R.nrows = HT_A->num_keys;
   R.nslabs = hostv[2];
   R.F = hostv[0];
+  if (R.cap) R.capsum = hostv[4];
   R.nitems = hostv[1] & 0xFFFFFFFFull;
   R.nheavy = hostv[1] >> 32;
   if (R.F == 0) return set_err(ctx, PTC_ERR_EMPTY, "tensor contraction not possible!! (no contributing pair)");
@@ -537,11 +579,11 @@ int prepare_finish(Run& R, const unsigned long long* hostv) {
 
 int prepare(Run& R, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B, const ptc_dtensor* A,
             const ptc_dtensor* B, const ptc_options& opt) {
-  unsigned long long* tot = R.ws.get<unsigned long long>(4);
+  unsigned long long* tot = R.ws.get<unsigned long long>(5);
   if (!tot) return PTC_ERR_NOMEM;
   int rc = prepare_launch(R, HT_A, HT_B, A, B, opt, tot);
-  unsigned long long hostv[4] = {0, 0, 0, 0};
-  if (!rc) rc = read_back(R.ctx, hostv, tot, 32);
+  unsigned long long hostv[5] = {0, 0, 0, 0, 0};
+  if (!rc) rc = read_back(R.ctx, hostv, tot, R.cap ? 40 : 32);
   if (!rc) rc = prepare_finish(R, hostv);
   return rc;
 }
@@ -588,8 +630,7 @@ static int resize_output(Run& R, ptc_dtensor* o, uint64_t keep, uint64_t cap) {
 }
 
 // The single-pass methods with direct writes: the output arrays are sized by
-// the rows' capacities (the learned method reads their sum back; the upper
-// bound uses min(F, rows x columns), which bounds it without a round trip),
+// the sum of the rows' capacities (read back with the preprocessing totals),
 // and every row that finishes within its capacity reserves its entries with
 // one fetch-add and writes them in place -- no staging area, no count scan,
 // no copy.  Rows over their capacity (or whose LDS table filled) are redone
@@ -600,10 +641,7 @@ static int single_pass_direct(Run& R, const ptc_dtensor* A, const ptc_dtensor* B
   ptc_ctx* ctx = R.ctx;
   Workspace& ws = R.ws;
   int rc = PTC_OK;
-  unsigned long long capsum = R.F;
-  if (R.ncols && R.nrows <= ~0ull / R.ncols) capsum = std::min<unsigned long long>(capsum, R.nrows * R.ncols);
-  if ((method == PTC_METHOD_LEARNED || capsum > STAGING_BOUND) && (rc = read_back(ctx, &capsum, cap_off + R.nrows, 8)))
-    return rc;
+  unsigned long long capsum = R.capsum;  // the capacities' sum, read back with the preprocessing totals
   *capacity_p = capsum;
   ptc_dtensor* out = nullptr;
   if ((rc = make_output(R, capsum, A, B, &out))) return rc;
@@ -899,10 +937,7 @@ static int win_single_pass(Run& R, WinRun& W, const ptc_dtensor* A, const ptc_dt
   ptc_ctx* ctx = R.ctx;
   Workspace& ws = R.ws;
   int rc = PTC_OK;
-  unsigned long long capsum = R.F;
-  if (R.ncols && R.nrows <= ~0ull / R.ncols) capsum = std::min<unsigned long long>(capsum, R.nrows * R.ncols);
-  if ((method == PTC_METHOD_LEARNED || capsum > STAGING_BOUND) && (rc = read_back(ctx, &capsum, cap_off + R.nrows, 8)))
-    return rc;
+  unsigned long long capsum = R.capsum;  // the capacities' sum, read back with the preprocessing totals
   *capacity_p = capsum;
   ptc_dtensor* out = nullptr;
   if ((rc = make_output(R, capsum, A, B, &out))) return rc;
@@ -1132,34 +1167,9 @@ int tc_impl(ptc_ctx* ctx, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B, 
     T.mark();  // 3
   } else {
     // ---------------- single pass into capacities
-    uint32_t* cap = ws.get<uint32_t>(R.nrows + 1);
-    unsigned long long* cap_off = ws.get<unsigned long long>(R.nrows + 1);
-    float* pred = nullptr;
-    if (!cap || !cap_off) return PTC_ERR_NOMEM;
-    if (method == PTC_METHOD_LEARNED) {
-      pred = ws.get<float>(R.nrows);
-      if (!pred) return PTC_ERR_NOMEM;
-      LearnedInputs L{};
-      L.nrows = R.nrows;
-      L.rows = HT_A->slices;
-      L.row_flops = R.row_flops;
-      L.row_hits = R.row_hits;
-      L.row_maxlen = R.row_maxlen;
-      L.ncols = R.ncols;
-      if ((rc = learned_predict(ctx, opt.mlp, L, pred))) return rc;
-    }
-    {
-      CapParams P{};
-      P.nrows = R.nrows;
-      P.row_flops = R.row_flops;
-      P.row_maxlen = R.row_maxlen;
-      P.ncols = R.ncols;
-      P.pred = pred;
-      P.scale = opt.predict_scale > 0 ? opt.predict_scale : 1.15;
-      P.cap = cap;
-      if ((rc = launch_caps(ctx, P))) return rc;
-    }
-    if ((rc = scan_u32_to_u64(ctx, cap, cap_off, R.nrows, cap_off + R.nrows))) return rc;
+    uint32_t* cap = R.cap;
+    unsigned long long* cap_off = R.cap_off;
+    if (!cap || !cap_off) return set_err(ctx, PTC_ERR_INTERNAL, "capacities were not computed");
     if (!kr) {
       rc = R.win_n ? win_single_pass(R, W, A, B, cap, cap_off, method, T, &out, &nnz, &fallback_rows, &out_capacity)
                    : single_pass_direct(R, A, B, cap, cap_off, method, T, &out, &nnz, &fallback_rows, &out_capacity);
@@ -1167,14 +1177,9 @@ int tc_impl(ptc_ctx* ctx, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B, 
       flags_checked = true;
     } else {
     // A chain step keeps its output rows in row order (its key table is
-    // formed from them): capacity-sized staging, a scan of the true counts,
-    // and a copy into place.
-    // Every capacity is at most min(row flops, columns), so the staging area
-    // is sized by min(F, rows x columns) without waiting for the sum; only a
-    // bound above STAGING_BOUND entries is replaced by the exact sum.
-    unsigned long long capsum = R.F;
-    if (R.ncols && R.nrows <= ~0ull / R.ncols) capsum = std::min<unsigned long long>(capsum, R.nrows * R.ncols);
-    if (capsum > STAGING_BOUND && (rc = read_back(ctx, &capsum, cap_off + R.nrows, 8))) return rc;
+    // formed from them): staging sized by the capacities' sum, a scan of the
+    // true counts, and a copy into place.
+    const unsigned long long capsum = R.capsum;
     const size_t kb = R.col64 ? 8 : 4;
     void* stg_col = ws.get<unsigned char>(capsum * kb);
     double* stg_val = ws.get<double>(capsum);
