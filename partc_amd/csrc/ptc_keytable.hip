@@ -722,6 +722,58 @@ __device__ __forceinline__ void pt_load_keys(const uint32_t* __restrict__ h, uin
   }
 }
 
+// Keys and linearised non-key tuples of the same nonzeros together.  Four
+// modes (the common 4-mode tensor, when the coordinates are 16-byte aligned)
+// take one 16-byte load a nonzero instead of one strided 4-byte load per
+// mode and codec.
+__device__ __forceinline__ bool pt_vec4(const uint32_t* h, uint32_t d) {
+  return d == 4 && ((uintptr_t)h & 15) == 0;
+}
+__device__ __forceinline__ unsigned long long pt_encode4(const uint4& q, const KeyCodec& c) {
+  unsigned long long k = 0;
+  for (uint32_t j = 0; j < c.n; ++j) {
+    const uint32_t m = c.mode[j];
+    const uint32_t x = m == 0 ? q.x : m == 1 ? q.y : m == 2 ? q.z : q.w;
+    k += (unsigned long long)x * c.stride[j];
+  }
+  return k;
+}
+__device__ __forceinline__ void pt_load_kr(const uint32_t* __restrict__ h, uint32_t d, uint64_t base, uint32_t i0,
+                                           uint32_t n, const KeyCodec& codec, const KeyCodec& rest,
+                                           unsigned long long (&key)[PT_U], unsigned long long (&rk)[PT_U]) {
+  if (pt_vec4(h, d)) {
+    uint4 q[PT_U];
+#pragma unroll
+    for (int u = 0; u < PT_U; ++u) {
+      const uint32_t i = i0 + threadIdx.x + u * blockDim.x;
+      q[u] = i < n ? reinterpret_cast<const uint4*>(h)[base + i] : make_uint4(0, 0, 0, 0);
+    }
+#pragma unroll
+    for (int u = 0; u < PT_U; ++u) {
+      key[u] = pt_encode4(q[u], codec);
+      rk[u] = pt_encode4(q[u], rest);
+    }
+    return;
+  }
+  pt_load_keys(h, d, base, i0, n, codec, key);
+  pt_load_keys(h, d, base, i0, n, rest, rk);
+}
+__device__ __forceinline__ void pt_load_k(const uint32_t* __restrict__ h, uint32_t d, uint64_t base, uint32_t i0,
+                                          uint32_t n, const KeyCodec& codec, unsigned long long (&key)[PT_U]) {
+  if (pt_vec4(h, d)) {
+    uint4 q[PT_U];
+#pragma unroll
+    for (int u = 0; u < PT_U; ++u) {
+      const uint32_t i = i0 + threadIdx.x + u * blockDim.x;
+      q[u] = i < n ? reinterpret_cast<const uint4*>(h)[base + i] : make_uint4(0, 0, 0, 0);
+    }
+#pragma unroll
+    for (int u = 0; u < PT_U; ++u) key[u] = pt_encode4(q[u], codec);
+    return;
+  }
+  pt_load_keys(h, d, base, i0, n, codec, key);
+}
+
 // Exclusive scan, in place, of a[0..n) in LDS (n <= 8 * blockDim.x).
 __device__ void pt_block_scan(uint32_t* a, uint32_t n, uint32_t* s_wave) {
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -1006,112 +1058,110 @@ __global__ void __launch_bounds__(PT_THREADS) k_pd_scatter(const uint32_t* __res
                                slice_of, t);
 }
 
-// ------------------------------------------------ placement by reservation
+// ----------------------------------------------- placement by column scan
 //
-// A single table places its tiles without scanning a count matrix: the
-// counting pass counts every tile per bucket (or direct key) in LDS and
-// reserves the tile's run in each bucket it touches with one global
-// fetch-add on that bucket's total, recording (count, offset within the
-// bucket) per (tile, bucket) in tile-major rows; one workgroup turns the
-// totals into bucket starts (and, for a direct key space, into the slices
-// and the whole index); the placing pass reads its tile's row (coalesced)
-// and writes the tile in bucket order.  A bucket's runs land in reservation
-// order rather than tile order: a key's ids form a set, as in the
-// reference's slices (fksCuckoo.hpp:1044-1060 appends ids in insertion
-// order; nothing downstream depends on it).  Against a bucket-major count
-// matrix this drops the matrix's three-launch scan, its strided row reads
-// and, for direct key spaces, the separate slices launch.
+// A single table places its tiles without global atomics and without a
+// bucket-major count matrix: the counting pass writes every tile's count
+// per bucket (or direct key) as one tile-major row (coalesced); a column
+// scan gives every (tile, bucket) its offset among the bucket's earlier
+// tiles -- first within segments of PC_SEG tiles (one wave per segment and
+// 64 buckets, the segment's rows read coalesced), then across segments in
+// one workgroup that also turns the bucket totals into bucket starts (and,
+// for a direct key space, into the slices and the whole index); the placing
+// pass reads its tile's rows and writes the tile in bucket order.  A
+// bucket's runs stay in tile order, and adjacent tiles run on one XCD
+// (pt_tile_of), so side-by-side runs merge in that XCD's L2.  Measured on
+// MI355X: a count pass with one global fetch-add per (tile, bucket) took
+// 32 us on the nips-shape A table against 10 us without it.
 
-// Each bucket's region is split into one part per XCD: the runs that land
-// side by side are written by one XCD and merge in its L2, and the part's
-// counter is only ever touched from that XCD, so its fetch-adds are
-// performed in that XCD's L2 (workgroup scope) instead of at the memory
-// side.  The part is the XCD the counting workgroup actually runs on (its
-// hardware XCC id; gfx950 has eight XCDs), recorded per tile for the
-// placing pass, whose workgroup for the same tile runs on the same XCD under
-// the round-robin dispatch that pt_tile_of assumes (only speed depends on
-// that).
-constexpr uint32_t PC_XCDS = 8;
-__device__ __forceinline__ uint32_t pc_xcc_id() {
-  return __builtin_amdgcn_s_getreg((3 << 11) | 20) & (PC_XCDS - 1);  // hwreg(HW_REG_XCC_ID, 0, 4)
-}
+constexpr uint32_t PC_SEG = 32;  // tiles per column-scan segment
 
 __global__ void __launch_bounds__(PT_THREADS) k_pc_count(const uint32_t* __restrict__ h, uint32_t d, uint64_t N,
-                                                         KeyCodec codec, int bbits, uint32_t nb,
-                                                         uint64_t ntiles, uint32_t* __restrict__ btot,
-                                                         uint32_t* __restrict__ tcnt, uint32_t* __restrict__ toff,
-                                                         uint32_t* __restrict__ txcc) {
+                                                         KeyCodec codec, int bbits, uint32_t nb, uint64_t ntiles,
+                                                         uint32_t* __restrict__ tcnt) {
   __shared__ uint32_t lh[PT_MAX_BUCKETS];
   const uint64_t t = pt_tile_of(blockIdx.x, ntiles);
   if (t >= ntiles) return;
-  const uint32_t x = pc_xcc_id();
-  if (threadIdx.x == 0) txcc[t] = x;
   for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) lh[b] = 0;
   __syncthreads();
   const uint64_t base = t * PT_TILE;
   const uint32_t n = (uint32_t)min<uint64_t>(PT_TILE, N - base);
   for (uint32_t i0 = 0; i0 < n; i0 += PT_U * blockDim.x) {
     unsigned long long key[PT_U];
-    pt_load_keys(h, d, base, i0, n, codec, key);  // the table's own tensor: always within radix
+    pt_load_k(h, d, base, i0, n, codec, key);  // the table's own tensor: always within radix
 #pragma unroll
     for (int u = 0; u < PT_U; ++u)
       if (i0 + threadIdx.x + u * blockDim.x < n) atomicAdd(&lh[pt_bucket(key[u], bbits)], 1u);
   }
   __syncthreads();
   uint32_t* rc = tcnt + t * nb;
-  uint32_t* ro = toff + t * nb;
-  // all of a thread's fetch-adds in flight before the first result is used
-  constexpr int PB = PT_MAX_BUCKETS / PT_THREADS;
-  uint32_t c[PB], o[PB];
-#pragma unroll
-  for (int k = 0; k < PB; ++k) {
-    const uint32_t b = threadIdx.x + k * PT_THREADS;
-    c[k] = b < nb ? lh[b] : 0u;
-    o[k] = 0;
-    if (c[k])
-      o[k] = __hip_atomic_fetch_add(&btot[(uint64_t)b * PC_XCDS + x], c[k], __ATOMIC_RELAXED,
-                                    __HIP_MEMORY_SCOPE_WORKGROUP);
-  }
-#pragma unroll
-  for (int k = 0; k < PB; ++k) {
-    const uint32_t b = threadIdx.x + k * PT_THREADS;
-    if (b >= nb) break;
-    rc[b] = c[k];
-    ro[b] = o[k];
-  }
+  for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) rc[b] = lh[b];
 }
 
-static_assert(PC_XCDS == 8, "k_pc_starts reads a bucket's per-XCD totals as two 16-byte words");
+// Offsets within segments: wave w of the grid takes buckets [64 g, 64 g +
+// 64) of segment s; toff[t][b] = sum of tcnt[t'][b] over the segment's
+// tiles t' < t, segtot[s][b] = the segment's total.
+__global__ void __launch_bounds__(256) k_pc_colscan(const uint32_t* __restrict__ tcnt, uint32_t nb, uint64_t ntiles,
+                                                    uint32_t* __restrict__ toff, uint32_t* __restrict__ segtot) {
+  const uint32_t ng = (nb + 63) / 64;
+  const uint64_t w = (uint64_t)blockIdx.x * 4 + threadIdx.x / 64;
+  const uint64_t s = w / ng;
+  const uint32_t b = (uint32_t)(w % ng) * 64 + (threadIdx.x & 63);
+  const uint64_t t0 = s * PC_SEG;
+  if (t0 >= ntiles || b >= nb) return;
+  const uint32_t nts = (uint32_t)min<uint64_t>(PC_SEG, ntiles - t0);
+  uint32_t c[PC_SEG];
+#pragma unroll
+  for (uint32_t k = 0; k < PC_SEG; ++k) c[k] = k < nts ? tcnt[(t0 + k) * nb + b] : 0u;  // all in flight
+  uint32_t run = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < PC_SEG; ++k) {
+    if (k < nts) toff[(t0 + k) * nb + b] = run;
+    run += c[k];
+  }
+  segtot[s * nb + b] = run;
+}
+
+// Segment prefixes, one thread per bucket: segpre[s][b] = the bucket's count
+// in segments before s (in place of the segment totals), btot[b] = its total.
+__global__ void __launch_bounds__(256) k_pc_segscan(uint32_t* __restrict__ segpre, uint32_t nb, uint32_t nseg,
+                                                    uint32_t* __restrict__ btot) {
+  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  uint32_t c = 0;
+  for (uint32_t s0 = 0; s0 < nseg; s0 += 8) {
+    uint32_t x[8];
+#pragma unroll
+    for (uint32_t k = 0; k < 8; ++k) x[k] = s0 + k < nseg ? segpre[(uint64_t)(s0 + k) * nb + b] : 0u;  // in flight together
+#pragma unroll
+    for (uint32_t k = 0; k < 8; ++k) {
+      if (s0 + k < nseg) segpre[(uint64_t)(s0 + k) * nb + b] = c;
+      c += x[k];
+    }
+  }
+  btot[b] = c;
+}
 
 // Bucket starts from the totals, one workgroup of 1024 threads (nb <=
-// PT_MAX_BUCKETS, four consecutive buckets a thread): start[b] and start[nb]
-// = N.  A direct key space (every key its own bucket) also gets its slices
+// PT_MAX_BUCKETS, four consecutive buckets a thread): start[b], start[nb] =
+// N.  A direct key space (every key its own bucket) also gets its slices
 // (present keys in key order), its whole index, the compact slice index of
 // every key and the (keys << 32 | nonzeros) total.  (present << 32 | count)
 // is scanned as one word: counts sum to N < 2^32.
 __global__ void __launch_bounds__(1024) k_pc_starts(const uint32_t* __restrict__ btot, uint32_t nb, uint64_t N,
-                                                    unsigned long long* __restrict__ start,
-                                                    uint32_t* __restrict__ sstart, int direct,
+                                                    unsigned long long* __restrict__ start, int direct,
                                                     KtSlice* __restrict__ slices, KtEntry* __restrict__ tab,
                                                     uint32_t* __restrict__ cidx,
-                                                    unsigned long long* __restrict__ total) {
+                                                    unsigned long long* __restrict__ total,
+                                                    uint32_t* __restrict__ zero_word) {
   __shared__ unsigned long long s_wave[1024 / 64];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const uint32_t b0 = threadIdx.x * 4;
-  // a bucket's eight per-XCD totals are two aligned 16-byte words; all of a
-  // thread's loads in flight together
-  uint4 q[4][2];
-#pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const bool in = b0 + j < nb;
-    const uint4* p = reinterpret_cast<const uint4*>(btot + (uint64_t)(b0 + j) * PC_XCDS);
-    q[j][0] = in ? p[0] : make_uint4(0, 0, 0, 0);
-    q[j][1] = in ? p[1] : make_uint4(0, 0, 0, 0);
-  }
+  if (zero_word && threadIdx.x == 0) *zero_word = 0;  // a counter of the next launches
   unsigned long long v[4], sum = 0;
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
-    const uint32_t c = q[j][0].x + q[j][0].y + q[j][0].z + q[j][0].w + q[j][1].x + q[j][1].y + q[j][1].z + q[j][1].w;
+    const uint32_t c = b0 + j < nb ? btot[b0 + j] : 0u;
     v[j] = c ? (1ull << 32) | c : 0ull;
     sum += v[j];
   }
@@ -1131,18 +1181,6 @@ __global__ void __launch_bounds__(1024) k_pc_starts(const uint32_t* __restrict__
     if (b >= nb) break;
     const uint32_t off = (uint32_t)pre, c = (uint32_t)v[j], si = (uint32_t)(pre >> 32);
     start[b] = off;
-    uint4 s0, s1;  // exclusive prefix over the XCD parts
-    s0.x = off;
-    s0.y = s0.x + q[j][0].x;
-    s0.z = s0.y + q[j][0].y;
-    s0.w = s0.z + q[j][0].z;
-    s1.x = s0.w + q[j][0].w;
-    s1.y = s1.x + q[j][1].x;
-    s1.z = s1.y + q[j][1].y;
-    s1.w = s1.z + q[j][1].z;
-    uint4* ps = reinterpret_cast<uint4*>(sstart + (uint64_t)b * PC_XCDS);
-    ps[0] = s0;
-    ps[1] = s1;
     if (direct) {
       KtEntry en;
       en.key = PTC_EMPTY_KEY;
@@ -1169,20 +1207,20 @@ __global__ void __launch_bounds__(1024) k_pc_starts(const uint32_t* __restrict__
   }
 }
 
-// The tile's counts and reserved offsets: lcur[b] = the bucket's start in
-// the tile (scanned counts), gdst[b] + j = the output position of the
-// tile's bucket-ordered slot j.
-__device__ __forceinline__ void pc_tile_row(uint64_t t, uint32_t nb, const uint32_t* __restrict__ sstart,
+// The tile's counts and offsets: lcur[b] = the bucket's start in the tile
+// (scanned counts), gdst[b] + j = the output position of the tile's
+// bucket-ordered slot j.
+__device__ __forceinline__ void pc_tile_row(uint64_t t, uint32_t nb, const unsigned long long* __restrict__ start,
                                             const uint32_t* __restrict__ tcnt, const uint32_t* __restrict__ toff,
-                                            const uint32_t* __restrict__ txcc, uint32_t* lcur, uint32_t* gdst,
+                                            const uint32_t* __restrict__ segpre, uint32_t* lcur, uint32_t* gdst,
                                             uint32_t* s_wave) {
   const uint32_t* rc = tcnt + t * nb;
   const uint32_t* ro = toff + t * nb;
-  const uint32_t x = txcc[t];
+  const uint32_t* rs = segpre + (t / PC_SEG) * nb;
   for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) {
     const uint32_t c = rc[b];
     lcur[b] = c;
-    gdst[b] = c ? sstart[(uint64_t)b * PC_XCDS + x] + ro[b] : 0u;  // positions < 2^32
+    gdst[b] = c ? (uint32_t)start[b] + rs[b] + ro[b] : 0u;  // positions < 2^32
   }
   __syncthreads();
   pt_block_scan(lcur, nb, s_wave);
@@ -1197,10 +1235,10 @@ template <class RestT, class ValT>
 __global__ void __launch_bounds__(PT_THREADS) k_pd_place(const uint32_t* __restrict__ h, uint32_t d, uint64_t N,
                                                          KeyCodec codec, KeyCodec rest, const ValT* __restrict__ val,
                                                          uint32_t nb, uint64_t ntiles,
-                                                         const uint32_t* __restrict__ sstart,
+                                                         const unsigned long long* __restrict__ start,
                                                          const uint32_t* __restrict__ tcnt,
                                                          const uint32_t* __restrict__ toff,
-                                                         const uint32_t* __restrict__ txcc,
+                                                         const uint32_t* __restrict__ segpre,
                                                          const uint32_t* __restrict__ cidx,
                                                          uint32_t* __restrict__ perm, RestT* __restrict__ rest_out,
                                                          ValT* __restrict__ gval, uint32_t* __restrict__ slice_of) {
@@ -1213,13 +1251,12 @@ __global__ void __launch_bounds__(PT_THREADS) k_pd_place(const uint32_t* __restr
   uint32_t* lcur = reinterpret_cast<uint32_t*>(ls + PT_TILE); // nb
   uint32_t* gdst = lcur + nb;                                // nb
   __shared__ uint32_t s_wave[PT_THREADS / 64];
-  pc_tile_row(t, nb, sstart, tcnt, toff, txcc, lcur, gdst, s_wave);
+  pc_tile_row(t, nb, start, tcnt, toff, segpre, lcur, gdst, s_wave);
   const uint64_t base = t * PT_TILE;
   const uint32_t n = (uint32_t)min<uint64_t>(PT_TILE, N - base);
   for (uint32_t i0 = 0; i0 < n; i0 += PT_U * blockDim.x) {
     unsigned long long key[PT_U], rk[PT_U];
-    pt_load_keys(h, d, base, i0, n, codec, key);  // < nb
-    pt_load_keys(h, d, base, i0, n, rest, rk);
+    pt_load_kr(h, d, base, i0, n, codec, rest, key, rk);  // keys < nb
 #pragma unroll
     for (int u = 0; u < PT_U; ++u) {
       const uint32_t i = i0 + threadIdx.x + u * blockDim.x;
@@ -1261,10 +1298,10 @@ template <class RestT, class ValT>
 __global__ void __launch_bounds__(PT_THREADS) k_pt_place(const uint32_t* __restrict__ h, uint32_t d, uint64_t N,
                                                          KeyCodec codec, KeyCodec rest, const ValT* __restrict__ val,
                                                          int bbits, uint32_t nb, uint64_t ntiles,
-                                                         const uint32_t* __restrict__ sstart,
+                                                         const unsigned long long* __restrict__ start,
                                                          const uint32_t* __restrict__ tcnt,
                                                          const uint32_t* __restrict__ toff,
-                                                         const uint32_t* __restrict__ txcc,
+                                                         const uint32_t* __restrict__ segpre,
                                                          unsigned long long* __restrict__ pkey,
                                                          uint32_t* __restrict__ pid,
                                                          KtPay<RestT, ValT>* __restrict__ ppay) {
@@ -1277,13 +1314,12 @@ __global__ void __launch_bounds__(PT_THREADS) k_pt_place(const uint32_t* __restr
   uint32_t* lcur = reinterpret_cast<uint32_t*>(ls + PT_TILE); // nb
   uint32_t* gdst = lcur + nb;                               // nb
   __shared__ uint32_t s_wave[PT_THREADS / 64];
-  pc_tile_row(t, nb, sstart, tcnt, toff, txcc, lcur, gdst, s_wave);
+  pc_tile_row(t, nb, start, tcnt, toff, segpre, lcur, gdst, s_wave);
   const uint64_t base = t * PT_TILE;
   const uint32_t n = (uint32_t)min<uint64_t>(PT_TILE, N - base);
   for (uint32_t i0 = 0; i0 < n; i0 += PT_U * blockDim.x) {
     unsigned long long key[PT_U], rk[PT_U];
-    pt_load_keys(h, d, base, i0, n, codec, key);
-    pt_load_keys(h, d, base, i0, n, rest, rk);
+    pt_load_kr(h, d, base, i0, n, codec, rest, key, rk);
 #pragma unroll
     for (int u = 0; u < PT_U; ++u) {
       const uint32_t i = i0 + threadIdx.x + u * blockDim.x;
@@ -1915,6 +1951,27 @@ int build_sorted(ptc_ctx* ctx, ptc_hashtable* ht, const ptc_dtensor* t, bool f32
   return PTC_OK;
 }
 
+// The counting pass and the column scan of a single table's tiles: every
+// (tile, bucket) gets its count (tcnt) and offset in its segment (toff), every
+// (segment, bucket) its prefix (segpre), every bucket its start (start); a
+// direct table also gets its slices, index, compact slice indices and total.
+int place_offsets(ptc_ctx* ctx, const ptc_dtensor* t, const KeyCodec& codec, int bbits, uint32_t nb, uint64_t ntiles,
+                  uint32_t nseg, uint32_t* tcnt, uint32_t* toff, uint32_t* segpre, uint32_t* btot,
+                  unsigned long long* start, int direct, KtSlice* slices, KtEntry* tab, uint32_t* cidx,
+                  unsigned long long* total, uint32_t* zero_word) {
+  hipStream_t st = ctx->stream;
+  hipLaunchKernelGGL(k_pc_count, dim3((unsigned)(8 * ((ntiles + 7) / 8))), dim3(PT_THREADS), 0, st, t->hedges, t->d,
+                     t->N, codec, bbits, nb, ntiles, tcnt);
+  const uint64_t waves = (uint64_t)nseg * ((nb + 63) / 64);
+  hipLaunchKernelGGL(k_pc_colscan, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, st, (const uint32_t*)tcnt, nb,
+                     ntiles, toff, segpre);
+  hipLaunchKernelGGL(k_pc_segscan, dim3((nb + 255) / 256), dim3(256), 0, st, segpre, nb, nseg, btot);
+  hipLaunchKernelGGL(k_pc_starts, dim3(1), dim3(1024), 0, st, (const uint32_t*)btot, nb, t->N, start, direct, slices,
+                     tab, cidx, total, zero_word);
+  PTC_HIP(ctx, hipGetLastError());
+  return PTC_OK;
+}
+
 // Partitioned hashing (wide keys): partition by hash bucket, group each
 // bucket in LDS, compact the slices and write the probe index.
 int build_partitioned(ptc_ctx* ctx, ptc_hashtable* ht, const ptc_dtensor* t, bool f32, bool rest64) {
@@ -1933,8 +1990,10 @@ int build_partitioned(ptc_ctx* ctx, ptc_hashtable* ht, const ptc_dtensor* t, boo
   const size_t pay_bytes = rest64 || !f32 ? 16 : 8;
   unsigned long long* off = ws.get<unsigned long long>(nb + 1);  // bucket starts
   uint32_t* tcnt = ws.get<uint32_t>((uint64_t)nb * ntiles);      // (tile, bucket) counts, tile-major
-  uint32_t* toff = ws.get<uint32_t>((uint64_t)nb * ntiles);      // (tile, bucket) offsets in the bucket
-  uint32_t* txcc = ws.get<uint32_t>(ntiles);                      // XCD of every tile's count
+  uint32_t* toff = ws.get<uint32_t>((uint64_t)nb * ntiles);      // (tile, bucket) offsets in its segment
+  const uint32_t nseg = (uint32_t)((ntiles + PC_SEG - 1) / PC_SEG);
+  uint32_t* segpre = ws.get<uint32_t>((uint64_t)nb * nseg);       // (segment, bucket) totals, then prefixes
+  uint32_t* btot = ws.get<uint32_t>(nb);                          // bucket totals
   unsigned long long* pkey = ws.get<unsigned long long>(N);
   uint32_t* pid = ws.get<uint32_t>(N);
   void* ppay = ws.get<unsigned char>(N * pay_bytes);
@@ -1943,23 +2002,19 @@ int build_partitioned(ptc_ctx* ctx, ptc_hashtable* ht, const ptc_dtensor* t, boo
   KtSlice* tmps = ws.get<KtSlice>(N);
   uint32_t* bkeys = ws.get<uint32_t>(nb);
   unsigned long long* sbase = ws.get<unsigned long long>(nb + 1);
-  uint32_t* btot = ws.get<uint32_t>((uint64_t)nb * PC_XCDS + 1);  // (bucket, XCD) totals, then the overflow count
-  uint32_t* sstart = ws.get<uint32_t>((uint64_t)nb * PC_XCDS);    // (bucket, XCD) starts
-  uint32_t* ovf_list = ws.get<uint32_t>(N);    // keys whose probe ran past their region
+  uint32_t* ovf = ws.get<uint32_t>(1);         // keys whose probe ran past their region: count,
+  uint32_t* ovf_list = ws.get<uint32_t>(N);    // and list
   ht->d_total = dalloc_n<unsigned long long>(ctx, 1);
   ht->slices_cap = std::max<uint64_t>(1, std::min<uint64_t>(N, ht->cap));
   ht->slices = dalloc_n<KtSlice>(ctx, ht->slices_cap);
-  if (!tcnt || !toff || !txcc || !off || !pkey || !pid || !ppay || !gkey || !gcnt || !tmps || !bkeys || !sbase || !btot || !sstart || !ovf_list ||
+  if (!tcnt || !toff || !segpre || !btot || !off || !pkey || !pid || !ppay || !gkey || !gcnt || !tmps || !bkeys || !sbase || !ovf ||
+      !ovf_list ||
       !ht->d_total || !ht->slices)
     return PTC_ERR_NOMEM;
-  // the overflow count and the bucket totals start at zero: one clear
-  int rc = fill_zero(ctx, btot, (size_t)(nb * PC_XCDS + 1) * 4);
+  // (the overflow count is cleared by the offsets' last launch)
+  int rc = place_offsets(ctx, t, ht->codec, bbits, nb, ntiles, nseg, tcnt, toff, segpre, btot, off, 0, nullptr, nullptr,
+                         nullptr, nullptr, ovf);
   if (rc) return rc;
-  uint32_t* ovf = btot + (uint64_t)nb * PC_XCDS;
-  hipLaunchKernelGGL(k_pc_count, dim3((unsigned)(8 * ((ntiles + 7) / 8))), dim3(PT_THREADS), 0, st, t->hedges, t->d,
-                     N, ht->codec, bbits, nb, ntiles, btot, tcnt, toff, txcc);
-  hipLaunchKernelGGL(k_pc_starts, dim3(1), dim3(1024), 0, st, (const uint32_t*)btot, nb, N, off, sstart, 0,
-                     (KtSlice*)nullptr, (KtEntry*)nullptr, (uint32_t*)nullptr, (unsigned long long*)nullptr);
   if (!region)
     hipLaunchKernelGGL(k_kt_clear, dim3(grid_for(ht->cap, 256)), dim3(256), 0, st, ht->entries, (uint32_t*)nullptr,
                        ht->cap);
@@ -1974,8 +2029,8 @@ int build_partitioned(ptc_ctx* ctx, ptc_hashtable* ht, const ptc_dtensor* t, boo
     }                                                                                                                \
     hipLaunchKernelGGL((k_pt_place<RT, VT>), dim3((unsigned)(8 * ((ntiles + 7) / 8))), dim3(PT_THREADS), lds, st,    \
                        t->hedges, t->d, N,                                                                          \
-                       ht->codec, ht->rest, (const VT*)(VALS), bbits, nb, ntiles, (const uint32_t*)sstart,          \
-                       (const uint32_t*)tcnt, (const uint32_t*)toff, (const uint32_t*)txcc, pkey, pid,              \
+                       ht->codec, ht->rest, (const VT*)(VALS), bbits, nb, ntiles, (const unsigned long long*)off,   \
+                       (const uint32_t*)tcnt, (const uint32_t*)toff, (const uint32_t*)segpre, pkey, pid,            \
                        (KtPay<RT, VT>*)ppay);                                                                       \
     hipLaunchKernelGGL((k_pt_group<RT, VT>), dim3(nb), dim3(PT_GROUP_THREADS), 0, st,                              \
                        (const unsigned long long*)off,                                                              \
@@ -2009,24 +2064,21 @@ int build_direct_partitioned(ptc_ctx* ctx, ptc_hashtable* ht, const ptc_dtensor*
   const uint32_t nb = (uint32_t)ht->codec.space;
   const uint64_t ntiles = (N + PT_TILE - 1) / PT_TILE;
   Workspace ws(ctx);
-  uint32_t* gcnt = ws.get<uint32_t>((uint64_t)nb * PC_XCDS);   // (key, XCD) totals
-  uint32_t* sstart = ws.get<uint32_t>((uint64_t)nb * PC_XCDS); // (key, XCD) starts
   uint32_t* tcnt = ws.get<uint32_t>((uint64_t)nb * ntiles);  // (tile, key) counts, tile-major
-  uint32_t* toff = ws.get<uint32_t>((uint64_t)nb * ntiles);  // (tile, key) offsets in the slice
-  uint32_t* txcc = ws.get<uint32_t>(ntiles);                  // XCD of every tile's count
+  uint32_t* toff = ws.get<uint32_t>((uint64_t)nb * ntiles);  // (tile, key) offsets in its segment
+  const uint32_t nseg = (uint32_t)((ntiles + PC_SEG - 1) / PC_SEG);
+  uint32_t* segpre = ws.get<uint32_t>((uint64_t)nb * nseg);   // (segment, key) totals, then prefixes
+  uint32_t* btot = ws.get<uint32_t>(nb);                      // key totals
   unsigned long long* off = ws.get<unsigned long long>(nb + 1);
   uint32_t* cidx = ws.get<uint32_t>(nb);
   ht->slice_of = dalloc_n<uint32_t>(ctx, N);
   ht->d_total = dalloc_n<unsigned long long>(ctx, 1);
   ht->slices_cap = std::max<uint64_t>(1, std::min<uint64_t>(N, ht->cap));
   ht->slices = dalloc_n<KtSlice>(ctx, ht->slices_cap);
-  if (!gcnt || !sstart || !tcnt || !toff || !txcc || !off || !cidx || !ht->slice_of || !ht->d_total || !ht->slices) return PTC_ERR_NOMEM;
-  int rc = fill_zero(ctx, gcnt, (size_t)nb * PC_XCDS * 4);
+  if (!tcnt || !toff || !segpre || !btot || !off || !cidx || !ht->slice_of || !ht->d_total || !ht->slices) return PTC_ERR_NOMEM;
+  int rc = place_offsets(ctx, t, ht->codec, -1, nb, ntiles, nseg, tcnt, toff, segpre, btot, off, 1, ht->slices,
+                         ht->entries, cidx, ht->d_total, nullptr);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_pc_count, dim3((unsigned)(8 * ((ntiles + 7) / 8))), dim3(PT_THREADS), 0, st, t->hedges, t->d,
-                     N, ht->codec, -1, nb, ntiles, gcnt, tcnt, toff, txcc);
-  hipLaunchKernelGGL(k_pc_starts, dim3(1), dim3(1024), 0, st, (const uint32_t*)gcnt, nb, N, off, sstart, 1, ht->slices,
-                     ht->entries, cidx, ht->d_total);
 #define PTC_PD(RT, VT, VALS, RK, GV)                                                                                  \
   do {                                                                                                                \
     const size_t lds = (size_t)PT_TILE * (sizeof(RT) + 4) + (size_t)nb * 8;                                           \
@@ -2038,7 +2090,8 @@ int build_direct_partitioned(ptc_ctx* ctx, ptc_hashtable* ht, const ptc_dtensor*
     }                                                                                                                 \
     hipLaunchKernelGGL((k_pd_place<RT, VT>), dim3((unsigned)(8 * ((ntiles + 7) / 8))), dim3(PT_THREADS), lds, st,     \
                        t->hedges, t->d, N, ht->codec, ht->rest, (const VT*)(VALS), nb, ntiles,                      \
-                       (const uint32_t*)sstart, (const uint32_t*)tcnt, (const uint32_t*)toff, (const uint32_t*)txcc, \
+                       (const unsigned long long*)off, (const uint32_t*)tcnt, (const uint32_t*)toff,              \
+                       (const uint32_t*)segpre,                                                                     \
                        (const uint32_t*)cidx, ht->perm, RK, GV, ht->slice_of);                                      \
   } while (0)
   if (rest64) {
