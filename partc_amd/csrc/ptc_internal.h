@@ -155,7 +155,8 @@ struct ptc_hashtable {
   KeyCodec rest;
   uint32_t* rest_key32 = nullptr;        // when rest.space <= 2^32
   unsigned long long* rest_key64 = nullptr;
-  uint16_t* rest_key16 = nullptr;        // 16-bit copy (rest.space <= 2^16), made on first use
+  uint16_t* rest_key16 = nullptr;        // 16-bit copy (rest.space <= 2^16): written by the partitioned builds,
+                                         // else made on first use
   double* gval = nullptr;      // device, N: values in grouped order
   float* gval32 = nullptr;     // device, N: values in grouped order, FP32 tensors (gval is then NULL)
   bool shares_vals = false;    // gval/gval32 are the tensor's own values (grouped order = tensor order)

@@ -1241,7 +1241,8 @@ __global__ void __launch_bounds__(PT_THREADS) k_pd_place(const uint32_t* __restr
                                                          const uint32_t* __restrict__ segpre,
                                                          const uint32_t* __restrict__ cidx,
                                                          uint32_t* __restrict__ perm, RestT* __restrict__ rest_out,
-                                                         ValT* __restrict__ gval, uint32_t* __restrict__ slice_of) {
+                                                         ValT* __restrict__ gval, uint32_t* __restrict__ slice_of,
+                                                         uint16_t* __restrict__ rest16) {
   const uint64_t t = pt_tile_of(blockIdx.x, ntiles);
   if (t >= ntiles) return;
   extern __shared__ unsigned long long pt_smem[];
@@ -1285,6 +1286,7 @@ __global__ void __launch_bounds__(PT_THREADS) k_pd_place(const uint32_t* __restr
       const uint32_t p = gdst[lb[i]] + j;
       perm[p] = (uint32_t)(base + i);
       rest_out[p] = lrk[i];
+      if (rest16) rest16[p] = (uint16_t)lrk[i];
       gval[p] = v[u];
       slice_of[p] = sl[u];
     }
@@ -1420,6 +1422,7 @@ struct PtIndex {
   uint32_t* map;       // LDS image of the region (slot -> local slice), or NULL
   uint32_t* ovf_cnt;   // keys whose probe ran past their region
   uint32_t* ovf_list;  // their slice records (indices into tmps)
+  uint16_t* rest16;    // non-NULL: a 16-bit copy of the non-key tuples as well
 };
 
 __device__ __forceinline__ uint32_t pt_cas32(uint32_t* p, uint32_t cmp, uint32_t v) {
@@ -1460,8 +1463,12 @@ __device__ bool pt_group_bucket(unsigned long long* tk, uint32_t* tc, uint32_t c
     *s_ndist = 0;
   }
   __syncthreads();
-  // counts per key (four keys per thread loaded at a time)
+  // counts per key (four keys per thread loaded at a time); a bucket of at
+  // most four nonzeros a thread keeps each one's slot in registers for the
+  // placing pass (no second read of its key, no second probe)
   constexpr int U = 4;
+  const bool kept = n <= U * blockDim.x;
+  uint32_t slot[U];
   for (uint32_t i0 = threadIdx.x; i0 < n; i0 += U * blockDim.x) {
     unsigned long long k[U];
 #pragma unroll
@@ -1470,11 +1477,14 @@ __device__ bool pt_group_bucket(unsigned long long* tk, uint32_t* tc, uint32_t c
       k[u] = i < n ? pkey[bs + i] : 0ull;
     }
 #pragma unroll
-    for (int u = 0; u < U; ++u)
+    for (int u = 0; u < U; ++u) {
+      slot[u] = PT_FULL;
       if (i0 + u * blockDim.x < n) {
         const uint32_t s = pt_slot<GLOBAL>(tk, cap, k[u], GLOBAL ? nullptr : s_ndist);
         if (s != PT_FULL) pt_add32(&tc[s], 1u);
+        slot[u] = s;
       }
+    }
   }
   __syncthreads();
   if (!GLOBAL && pt_ld32(s_ndist) >= PT_LDS_KEYS) return false;  // the same value in every thread
@@ -1578,17 +1588,18 @@ __device__ bool pt_group_bucket(unsigned long long* tk, uint32_t* tc, uint32_t c
     for (int u = 0; u < UP; ++u) {
       const uint32_t i = i0 + u * blockDim.x;
       const uint64_t q = bs + (i < n ? i : 0u);
-      k[u] = pkey[q];
+      k[u] = kept ? 0ull : pkey[q];
       id[u] = pid[q];
       pl[u] = ppay[q];
     }
 #pragma unroll
     for (int u = 0; u < UP; ++u) {
       if (i0 + u * blockDim.x >= n) continue;
-      const uint32_t s = pt_slot<GLOBAL>(tk, cap, k[u]);  // present: a read-only probe
+      const uint32_t s = kept ? slot[u] : pt_slot<GLOBAL>(tk, cap, k[u]);  // present: a read-only probe
       const uint64_t p = bs - obase + pt_add32(&tc[s], 1u);
       perm[p] = id[u];
       rest_out[p] = pl[u].k;
+      if (X.rest16) X.rest16[p] = (uint16_t)pl[u].k;
       gval[p] = pl[u].v;
     }
   }
@@ -1619,7 +1630,8 @@ __device__ __forceinline__ void pt_group_one(uint32_t b, const unsigned long lon
                                              uint32_t* __restrict__ perm, RestT* __restrict__ rest_out,
                                              ValT* __restrict__ gval, KtEntry* __restrict__ tab, uint64_t cap,
                                              int region, uint32_t* __restrict__ ovf_cnt,
-                                             uint32_t* __restrict__ ovf_list, PtGroupLds& L) {
+                                             uint32_t* __restrict__ ovf_list, PtGroupLds& L,
+                                             uint16_t* __restrict__ rest16 = nullptr) {
   const uint64_t bs = off[(uint64_t)b * ntiles];
   const uint64_t be = b + 1 < nb ? off[(uint64_t)(b + 1) * ntiles] : oend;
   const uint32_t n = (uint32_t)(be - bs);
@@ -1632,6 +1644,7 @@ __device__ __forceinline__ void pt_group_one(uint32_t b, const unsigned long lon
   X.map = region && cap / nb <= PT_TAB ? L.lmap : nullptr;
   X.ovf_cnt = ovf_cnt;
   X.ovf_list = ovf_list;
+  X.rest16 = rest16;
   // the LDS table while the bucket's distinct keys fit, else the region
   if (!pt_group_bucket<false>(L.lk, L.lc, PT_TAB, bs, n, obase, pkey, pid, ppay, tmps, nkeys_out, perm, rest_out,
                               gval, X, &L.s_carry, L.s_wave, &L.s_ndist)) {
@@ -1648,10 +1661,10 @@ __global__ void __launch_bounds__(PT_GROUP_THREADS, PT_GROUP_WAVES_PER_SIMD) k_p
     const KtPay<RestT, ValT>* __restrict__ ppay, unsigned long long* __restrict__ gkey, uint32_t* __restrict__ gcnt,
     KtSlice* __restrict__ tmps, uint32_t* __restrict__ bkeys, uint32_t* __restrict__ perm,
     RestT* __restrict__ rest_out, ValT* __restrict__ gval, KtEntry* __restrict__ tab, uint64_t cap, int region,
-    uint32_t* __restrict__ ovf_cnt, uint32_t* __restrict__ ovf_list) {
+    uint32_t* __restrict__ ovf_cnt, uint32_t* __restrict__ ovf_list, uint16_t* __restrict__ rest16) {
   __shared__ PtGroupLds L;
   pt_group_one<RestT, ValT>(blockIdx.x, off, ntiles, nb, 0, N, pkey, pid, ppay, gkey, gcnt, tmps, &bkeys[blockIdx.x],
-                            perm, rest_out, gval, tab, cap, region, ovf_cnt, ovf_list, L);
+                            perm, rest_out, gval, tab, cap, region, ovf_cnt, ovf_list, L, rest16);
 }
 
 // Slices of bucket b to their compact positions (sbase: the scanned key
@@ -2007,6 +2020,10 @@ int build_partitioned(ptc_ctx* ctx, ptc_hashtable* ht, const ptc_dtensor* t, boo
   ht->d_total = dalloc_n<unsigned long long>(ctx, 1);
   ht->slices_cap = std::max<uint64_t>(1, std::min<uint64_t>(N, ht->cap));
   ht->slices = dalloc_n<KtSlice>(ctx, ht->slices_cap);
+  // non-key tuples of at most 16 bits (the column keys of a dense
+  // contraction) are also written as 16-bit words
+  if (!rest64 && ht->rest.space <= 65536 && !(ht->rest_key16 = dalloc_n<uint16_t>(ctx, std::max<uint64_t>(N, 1))))
+    return PTC_ERR_NOMEM;
   if (!tcnt || !toff || !segpre || !btot || !off || !pkey || !pid || !ppay || !gkey || !gcnt || !tmps || !bkeys || !sbase || !ovf ||
       !ovf_list ||
       !ht->d_total || !ht->slices)
@@ -2018,7 +2035,7 @@ int build_partitioned(ptc_ctx* ctx, ptc_hashtable* ht, const ptc_dtensor* t, boo
   if (!region)
     hipLaunchKernelGGL(k_kt_clear, dim3(grid_for(ht->cap, 256)), dim3(256), 0, st, ht->entries, (uint32_t*)nullptr,
                        ht->cap);
-#define PTC_PT(RT, VT, VALS, RK, GV)                                                                                 \
+#define PTC_PT(RT, VT, VALS, RK, GV, R16)                                                                              \
   do {                                                                                                               \
     const size_t lds = (size_t)PT_TILE * (8 + sizeof(RT) + 2) + (size_t)nb * 8;                                   \
     static size_t lds_set = 0;                                                                                       \
@@ -2036,14 +2053,14 @@ int build_partitioned(ptc_ctx* ctx, ptc_hashtable* ht, const ptc_dtensor* t, boo
                        (const unsigned long long*)off,                                                              \
                        (uint64_t)1, nb, N, (const unsigned long long*)pkey, (const uint32_t*)pid,                   \
                        (const KtPay<RT, VT>*)ppay, gkey, gcnt, tmps, bkeys, ht->perm, RK, GV, ht->entries, ht->cap, \
-                       region ? 1 : 0, ovf, ovf_list);                                                               \
+                       region ? 1 : 0, ovf, ovf_list, R16);                                                          \
   } while (0)
   if (rest64) {
-    if (f32) PTC_PT(unsigned long long, float, t->vals32, ht->rest_key64, ht->gval32);
-    else PTC_PT(unsigned long long, double, t->vals, ht->rest_key64, ht->gval);
+    if (f32) PTC_PT(unsigned long long, float, t->vals32, ht->rest_key64, ht->gval32, (uint16_t*)nullptr);
+    else PTC_PT(unsigned long long, double, t->vals, ht->rest_key64, ht->gval, (uint16_t*)nullptr);
   } else {
-    if (f32) PTC_PT(uint32_t, float, t->vals32, ht->rest_key32, ht->gval32);
-    else PTC_PT(uint32_t, double, t->vals, ht->rest_key32, ht->gval);
+    if (f32) PTC_PT(uint32_t, float, t->vals32, ht->rest_key32, ht->gval32, ht->rest_key16);
+    else PTC_PT(uint32_t, double, t->vals, ht->rest_key32, ht->gval, ht->rest_key16);
   }
 #undef PTC_PT
   if ((rc = scan_u32_to_u64(ctx, bkeys, sbase, nb, sbase + nb))) return rc;
@@ -2075,11 +2092,13 @@ int build_direct_partitioned(ptc_ctx* ctx, ptc_hashtable* ht, const ptc_dtensor*
   ht->d_total = dalloc_n<unsigned long long>(ctx, 1);
   ht->slices_cap = std::max<uint64_t>(1, std::min<uint64_t>(N, ht->cap));
   ht->slices = dalloc_n<KtSlice>(ctx, ht->slices_cap);
+  if (!rest64 && ht->rest.space <= 65536 && !(ht->rest_key16 = dalloc_n<uint16_t>(ctx, std::max<uint64_t>(N, 1))))
+    return PTC_ERR_NOMEM;
   if (!tcnt || !toff || !segpre || !btot || !off || !cidx || !ht->slice_of || !ht->d_total || !ht->slices) return PTC_ERR_NOMEM;
   int rc = place_offsets(ctx, t, ht->codec, -1, nb, ntiles, nseg, tcnt, toff, segpre, btot, off, 1, ht->slices,
                          ht->entries, cidx, ht->d_total, nullptr);
   if (rc) return rc;
-#define PTC_PD(RT, VT, VALS, RK, GV)                                                                                  \
+#define PTC_PD(RT, VT, VALS, RK, GV, R16)                                                                              \
   do {                                                                                                                \
     const size_t lds = (size_t)PT_TILE * (sizeof(RT) + 4) + (size_t)nb * 8;                                           \
     static size_t lds_set = 0;                                                                                        \
@@ -2092,14 +2111,14 @@ int build_direct_partitioned(ptc_ctx* ctx, ptc_hashtable* ht, const ptc_dtensor*
                        t->hedges, t->d, N, ht->codec, ht->rest, (const VT*)(VALS), nb, ntiles,                      \
                        (const unsigned long long*)off, (const uint32_t*)tcnt, (const uint32_t*)toff,              \
                        (const uint32_t*)segpre,                                                                     \
-                       (const uint32_t*)cidx, ht->perm, RK, GV, ht->slice_of);                                      \
+                       (const uint32_t*)cidx, ht->perm, RK, GV, ht->slice_of, R16);                                 \
   } while (0)
   if (rest64) {
-    if (f32) PTC_PD(unsigned long long, float, t->vals32, ht->rest_key64, ht->gval32);
-    else PTC_PD(unsigned long long, double, t->vals, ht->rest_key64, ht->gval);
+    if (f32) PTC_PD(unsigned long long, float, t->vals32, ht->rest_key64, ht->gval32, (uint16_t*)nullptr);
+    else PTC_PD(unsigned long long, double, t->vals, ht->rest_key64, ht->gval, (uint16_t*)nullptr);
   } else {
-    if (f32) PTC_PD(uint32_t, float, t->vals32, ht->rest_key32, ht->gval32);
-    else PTC_PD(uint32_t, double, t->vals, ht->rest_key32, ht->gval);
+    if (f32) PTC_PD(uint32_t, float, t->vals32, ht->rest_key32, ht->gval32, ht->rest_key16);
+    else PTC_PD(uint32_t, double, t->vals, ht->rest_key32, ht->gval, ht->rest_key16);
   }
 #undef PTC_PD
   PTC_HIP(ctx, hipGetLastError());
