@@ -766,6 +766,10 @@ def main():
     streamed = sb / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
     roof["streamed_bytes_per_launch"] = int(sb)
     roof["streamed_GBps"] = round(streamed, 1)
+    if traffic and kms > 0:  # the HBM bytes the counters measured, over the same live kernel time
+        tgbps = traffic["bytes_per_launch"] / (kms * 1e-3) / 1e9
+        roof["traffic_GBps"] = round(tgbps, 1)
+        roof["traffic_frac"] = round(tgbps / HBM_PEAK_GBS, 4)
     if copy_gbps > 0:  # what a plain device copy reaches on this GPU
         roof["achievable_copy"] = round(copy_gbps, 1)
         roof["frac_of_achievable"] = round(achieved / copy_gbps, 4)
