@@ -31,6 +31,7 @@ struct ProbeParams {
   uint32_t* pos_off;
   uint32_t* row_hits;
   uint32_t* row_maxlen;
+  uint32_t* zero4;                // non-NULL: four words cleared by the launch (the passes' flags)
 };
 
 struct RowParams {
@@ -128,6 +129,8 @@ struct CapParams {
   PTR(double) hacc;                                                                           \
   PTR(unsigned char) hflag;                                                                   \
   PTR(const uint32_t) merge_rows; /* merge launch: rows to finish (NULL: all rows) */         \
+  /* a grid sized by a bound: (split rows << 32 | items) on the device; work past it exits */ \
+  PTR(const unsigned long long) live_items;                                                   \
   PTR(const unsigned long long) row_slab;                                                     \
   int use_flags; /* 1: a flag byte per column marks touches */                                \
   /* hash accumulators */                                                                     \

@@ -78,6 +78,7 @@ __device__ __forceinline__ unsigned long long mix64d(unsigned long long k) {
 template <class RestT>
 __global__ void __launch_bounds__(256) k_probe(ProbeParams P) {
   const RestT* __restrict__ akey = (const RestT*)P.akey;
+  if (P.zero4 && blockIdx.x == 0 && threadIdx.x < 4) P.zero4[threadIdx.x] = 0;
   for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < P.npos; p += (uint64_t)gridDim.x * blockDim.x) {
     uint32_t len = 0, off = 0;
     {
@@ -1025,6 +1026,10 @@ __device__ __forceinline__ void pass_item(const PP& P, uint32_t widx) {
 
 template <int PASS, bool DENSE, class ColT, bool MERGE, int VAL>
 __global__ void __launch_bounds__(PB_MAX) k_pass(PassParams P) {
+  if (P.live_items) {  // a grid sized by a bound: work items (split rows) past the count exit
+    const unsigned long long L = *P.live_items;
+    if (blockIdx.x >= (MERGE ? (uint32_t)(L >> 32) : (uint32_t)L)) return;
+  }
   pass_item<PASS, DENSE, ColT, MERGE, VAL>(P, blockIdx.x);
 }
 

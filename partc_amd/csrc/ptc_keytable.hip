@@ -1562,19 +1562,35 @@ __device__ bool pt_group_bucket(unsigned long long* tk, uint32_t* tc, uint32_t c
   }
   if (threadIdx.x == 0) *nkeys_out = (uint32_t)(*s_carry >> 32);
   if (X.region && X.map) {  // the region of the probe index, written in order
-    for (uint32_t s = threadIdx.x; s < (uint32_t)(X.r1 - X.r0); s += blockDim.x) {
-      const uint32_t j = X.map[s];
-      KtEntry en;
-      en.key = PTC_EMPTY_KEY;
-      en.off = 0;
-      en.len = 0;
-      if (j != 0xFFFFFFFFu) {
-        const KtSlice sl = tmps[bs + j];
-        en.key = sl.key;
-        en.off = sl.off;
-        en.len = sl.len;
+    // (R <= PT_TAB: at most four slots a thread, their slice reads in flight
+    // together)
+    const uint32_t R = (uint32_t)(X.r1 - X.r0);
+    constexpr int W = PT_TAB / PT_GROUP_THREADS;
+    for (uint32_t s0 = threadIdx.x; s0 < R; s0 += W * blockDim.x) {
+      uint32_t j[W];
+      KtSlice sl[W];
+#pragma unroll
+      for (int w = 0; w < W; ++w) {
+        const uint32_t s = s0 + w * blockDim.x;
+        j[w] = s < R ? X.map[s] : 0xFFFFFFFFu;
       }
-      X.tab[X.r0 + s] = en;
+#pragma unroll
+      for (int w = 0; w < W; ++w) {
+        sl[w].key = PTC_EMPTY_KEY;
+        sl[w].off = 0;
+        sl[w].len = 0;
+        if (j[w] != 0xFFFFFFFFu) sl[w] = tmps[bs + j[w]];
+      }
+#pragma unroll
+      for (int w = 0; w < W; ++w) {
+        const uint32_t s = s0 + w * blockDim.x;
+        if (s >= R) continue;
+        KtEntry en;
+        en.key = sl[w].key;
+        en.off = sl[w].off;
+        en.len = sl[w].len;
+        X.tab[X.r0 + s] = en;
+      }
     }
   }
   // place: every nonzero at its key's cursor (loads of four at a time in

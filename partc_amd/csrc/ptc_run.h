@@ -55,12 +55,16 @@ struct Run {
   uint32_t* overflow = nullptr;
   unsigned long long* out_off = nullptr;
   bool want_features = false;  // per-row hit counts and longest slices (learned features)
+  bool early_items = false;    // work items built before the preprocessing read-back (bounded count)
+  bool allow_ahead = false;    // the caller lets a single pass run ahead of the preprocessing totals
+  bool ahead = false;          // ... and this one does
   const uint32_t* row_of_pos = nullptr;  // HT_A's slice index of every A position
   // single-pass methods: every row's output capacity, their offsets and sum
   // (computed with the preprocessing, the sum read back with its totals)
   uint32_t* cap = nullptr;
   unsigned long long* cap_off = nullptr;
   unsigned long long capsum = 0;
+  WordSources totw{};  // the preprocessing totals' device words
   uint32_t threads = PB;       // threads per row-pass workgroup
   bool force_flags = false;    // touch flags whatever the values (set before prepare)
   const uint16_t* bcol16 = nullptr;  // dense rows: B's column keys as 16 bits
@@ -84,6 +88,9 @@ uint64_t lds_limit(ptc_ctx* ctx);
 int prepare_launch(Run& R, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B, const ptc_dtensor* A,
                    const ptc_dtensor* B, const ptc_options& opt, unsigned long long* tot4);
 int prepare_finish(Run& R, const unsigned long long* hostv);
+// Work items and split-row list into arrays of the given sizes (npos = 0:
+// no chunk starts).
+int build_items(Run& R, uint64_t nitems, uint64_t nheavy, uint64_t npos);
 int prepare(Run& R, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B, const ptc_dtensor* A,
             const ptc_dtensor* B, const ptc_options& opt);
 

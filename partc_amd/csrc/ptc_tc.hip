@@ -22,6 +22,11 @@
 
 namespace ptc {
 
+// Output and split-row slab bytes below which a single pass over dense rows
+// runs ahead of the preprocessing totals (the allocations then follow the
+// bounds rows x columns and rows + DENSE_MAX_ITEMS work items).
+static constexpr uint64_t AHEAD_BYTES = 512ull << 20;
+
 struct Timer {
   ptc_ctx* ctx;
   hipEvent_t ev[8];
@@ -424,7 +429,7 @@ int prepare_launch(Run& R, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B,
 
   // ---------------- preprocessing: probe, flop prefix, rows, items
   const bool features = opt.method == PTC_METHOD_LEARNED || R.want_features;
-  if ((rc = fill_zero(ctx, R.overflow, 16))) return rc;
+  if (!R.npos && (rc = fill_zero(ctx, R.overflow, 16))) return rc;  // else cleared by the probe
   {
     ProbeParams P{};
     P.npos = R.npos;
@@ -442,6 +447,7 @@ int prepare_launch(Run& R, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B,
     P.pos_off = R.pos_off;
     P.row_hits = R.row_hits;
     P.row_maxlen = R.row_maxlen;
+    P.zero4 = R.overflow;
     if ((rc = launch_probe(ctx, P, HT_A->rest_key64 != nullptr))) return rc;
   }
   // the flop prefix over A positions; the row features ride on the same pass
@@ -481,6 +487,23 @@ int prepare_launch(Run& R, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B,
   }
   if ((rc = scan_u64(ctx, R.packed, R.packed, R.nrows, R.packed + R.nrows))) return rc;
   if ((rc = scan_u64(ctx, R.row_slab, R.row_slab, R.nrows, R.row_slab + R.nrows))) return rc;
+  // Dense rows with the default work-item rule have at most rows +
+  // DENSE_MAX_ITEMS items (each row's chunk count is at most its flops /
+  // chunk + 1 and the chunk is at least total / DENSE_MAX_ITEMS): the items
+  // are built now, in arrays of that bound, so the GPU has them to do while
+  // the host waits for the totals.
+  R.early_items = (R.dense || R.win_n) && !opt.chunk_flops;
+  if (R.early_items && (rc = build_items(R, R.nrows + DENSE_MAX_ITEMS, R.nrows, R.npos))) return rc;
+  // A single pass over dense rows whose output and split-row slabs have
+  // host-known bounds within AHEAD_BYTES runs ahead of these totals (one
+  // round trip for the whole contraction); its capacities then bound rows,
+  // not the output, and need no sum.
+  R.ahead = false;
+  if (R.allow_ahead && R.early_items && R.dense && !R.win_n && R.ncols && R.nrows <= (1ull << 40) / R.ncols) {
+    const uint64_t out_bytes = R.nrows * R.ncols * (4 * R.dO + 8);
+    const uint64_t slab_bytes = (R.nrows + DENSE_MAX_ITEMS) * R.ncols * (R.f32 ? 12 : 8 + (R.use_flags ? 1 : 0));
+    R.ahead = out_bytes <= AHEAD_BYTES && slab_bytes <= AHEAD_BYTES;
+  }
   // Single-pass methods: every row's capacity (rows past HT_A's key count
   // have no flops: capacity 0) and their sum, read back with the totals
   // below, so sizing the output costs no round trip of its own.  Upper
@@ -512,19 +535,21 @@ int prepare_launch(Run& R, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B,
       P.cap = R.cap;
       if ((rc = launch_caps(ctx, P))) return rc;
     }
-    if ((rc = scan_u32_to_u64(ctx, R.cap, R.cap_off, R.nrows, R.cap_off + R.nrows))) return rc;
+    if (!R.ahead && (rc = scan_u32_to_u64(ctx, R.cap, R.cap_off, R.nrows, R.cap_off + R.nrows))) return rc;
   }
-  // the totals and HT_A's key count, for one host round trip
+  // the totals and HT_A's key count, for one host round trip (tot NULL: the
+  // caller collects them from R.totw with words of its own)
   {
-    WordSources W{};
-    W.n = single ? 5 : 4;
+    WordSources& W = R.totw;
+    W = WordSources{};
+    W.n = single && !R.ahead ? 5 : 4;
     W.src[0] = R.apre + R.npos;
     W.src[1] = R.packed + R.nrows;
     W.src[2] = R.row_slab + R.nrows;
     W.src[3] = HT_A->d_total;
-    if (single) W.src[4] = R.cap_off + R.nrows;
+    if (W.n == 5) W.src[4] = R.cap_off + R.nrows;
     for (uint32_t i = 0; i < W.n; ++i) W.bytes[i] = 8;
-    if ((rc = collect_words(ctx, tot, W))) return rc;
+    if (tot && (rc = collect_words(ctx, tot, W))) return rc;
   }
   return PTC_OK;
 }
@@ -546,35 +571,45 @@ int prepare_finish(Run& R, const unsigned long long* hostv) {
   R.nrows = HT_A->num_keys;
   R.nslabs = hostv[2];
   R.F = hostv[0];
-  if (R.cap) R.capsum = hostv[4];
+  if (R.cap && !R.ahead) R.capsum = hostv[4];
   R.nitems = hostv[1] & 0xFFFFFFFFull;
   R.nheavy = hostv[1] >> 32;
   if (R.F == 0) return set_err(ctx, PTC_ERR_EMPTY, "tensor contraction not possible!! (no contributing pair)");
-  R.items = ws.get<uint4>(R.nitems);
-  R.heavy_rows = ws.get<uint32_t>(R.nheavy);
-  if (!R.items || !R.heavy_rows) return PTC_ERR_NOMEM;
-  {
-    ItemParams P{};
-    P.nrows = R.nrows;
-    P.rows = HT_A->slices;
-    P.row_nchunk = R.row_nchunk;
-    P.packed = R.packed;
-    P.row_heavy = R.row_heavy;
-    P.heavy_rows = R.heavy_rows;
-    P.items = R.items;
-    ItemP0Params Q{};
-    Q.npos = R.nheavy ? R.npos : 0;
-    Q.row_of_pos = R.row_of_pos;
-    Q.row_nchunk = R.row_nchunk;
-    Q.pos_len = R.pos_len;
-    Q.rows = HT_A->slices;
-    Q.apre = R.apre;
-    Q.row_flops = R.row_flops;
-    Q.packed = R.packed;
-    Q.items = R.items;
-    if ((rc = launch_items(ctx, P, Q))) return rc;
+  if (R.early_items) {  // built before the read-back, within their bound
+    if (R.nitems > R.nrows + DENSE_MAX_ITEMS)
+      return set_err(ctx, PTC_ERR_INTERNAL, "%llu work items past their bound", (unsigned long long)R.nitems);
+    return PTC_OK;
   }
-  return PTC_OK;
+  return build_items(R, R.nitems, R.nheavy, R.nheavy ? R.npos : 0);
+}
+
+// Work items (row, chunk, first position) and the split rows' list into
+// arrays of nitems and nheavy entries; npos = 0 skips the chunk starts.
+int build_items(Run& R, uint64_t nitems, uint64_t nheavy, uint64_t npos) {
+  ptc_ctx* ctx = R.ctx;
+  const ptc_hashtable* HT_A = R.HA;
+  R.items = R.ws.get<uint4>(std::max<uint64_t>(nitems, 1));
+  R.heavy_rows = R.ws.get<uint32_t>(std::max<uint64_t>(nheavy, 1));
+  if (!R.items || !R.heavy_rows) return PTC_ERR_NOMEM;
+  ItemParams P{};
+  P.nrows = R.nrows;
+  P.rows = HT_A->slices;
+  P.row_nchunk = R.row_nchunk;
+  P.packed = R.packed;
+  P.row_heavy = R.row_heavy;
+  P.heavy_rows = R.heavy_rows;
+  P.items = R.items;
+  ItemP0Params Q{};
+  Q.npos = npos;
+  Q.row_of_pos = R.row_of_pos;
+  Q.row_nchunk = R.row_nchunk;
+  Q.pos_len = R.pos_len;
+  Q.rows = HT_A->slices;
+  Q.apre = R.apre;
+  Q.row_flops = R.row_flops;
+  Q.packed = R.packed;
+  Q.items = R.items;
+  return launch_items(ctx, P, Q);
 }
 
 int prepare(Run& R, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B, const ptc_dtensor* A,
@@ -635,13 +670,27 @@ static int resize_output(Run& R, ptc_dtensor* o, uint64_t keep, uint64_t cap) {
 // one fetch-add and writes them in place -- no staging area, no count scan,
 // no copy.  Rows over their capacity (or whose LDS table filled) are redone
 // by the exact two-pass path, appended the same way.
+// Ahead mode: the pass runs before the preprocessing totals
+// reach the host -- dense rows with the default work-item rule, whose items
+// are already built within their bound -- on a grid of that bound (work
+// past the device's count exits), with split-row slabs and the output sized
+// by bounds known on the host (rows x columns); the totals come back with
+// the pass's own flags and count, one round trip for the whole contraction.
 static int single_pass_direct(Run& R, const ptc_dtensor* A, const ptc_dtensor* B, const uint32_t* cap,
                               const unsigned long long* cap_off, int method, Timer& T, ptc_dtensor** out_p,
-                              uint64_t* nnz_p, uint64_t* fallback_p, uint64_t* capacity_p) {
+                              uint64_t* nnz_p, uint64_t* fallback_p, uint64_t* capacity_p, bool ahead = false) {
   ptc_ctx* ctx = R.ctx;
   Workspace& ws = R.ws;
   int rc = PTC_OK;
-  unsigned long long capsum = R.capsum;  // the capacities' sum, read back with the preprocessing totals
+  // the capacities' sum, read back with the preprocessing totals; ahead: the
+  // bound rows x columns, which every row's capacity min(flops, columns) sums below
+  unsigned long long capsum = ahead ? R.nrows * R.ncols : R.capsum;
+  uint64_t nitems = R.nitems;
+  if (ahead) {  // bounds of the counts the pass reads on the device
+    nitems = R.nrows + DENSE_MAX_ITEMS;
+    R.nheavy = R.nrows;
+    R.nslabs = nitems;
+  }
   *capacity_p = capsum;
   ptc_dtensor* out = nullptr;
   if ((rc = make_output(R, capsum, A, B, &out))) return rc;
@@ -649,12 +698,16 @@ static int single_pass_direct(Run& R, const ptc_dtensor* A, const ptc_dtensor* B
     ptc_dtensor_free(out);
     return code;
   };
-  unsigned long long* counter = ws.get<unsigned long long>(2);
-  unsigned long long* pair = ws.get<unsigned long long>(2);
-  if (!counter || !pair) return fail(PTC_ERR_NOMEM);
+  // the reservation counter rides behind the row counts: one clear for both
+  const uint64_t cnt_words = (R.nrows + 1 + 3) / 4 * 4;  // 16-byte aligned end of the row counts
+  uint32_t* row_nnz = ws.get<uint32_t>(cnt_words + 4);
+  unsigned long long* pair = ws.get<unsigned long long>(8);
+  if (!row_nnz || !pair) return fail(PTC_ERR_NOMEM);
+  R.row_nnz = row_nnz;
+  unsigned long long* counter = reinterpret_cast<unsigned long long*>(row_nnz + cnt_words);
   HashPlan h1;
-  if ((rc = fill_zero(ctx, counter, 16)) || (!R.dense && (rc = plan_hash(R, h1, cap, nullptr, R.row_flops))) ||
-      (rc = fill_zero(ctx, R.row_nnz, (R.nrows + 1) * 4)) || (rc = prepare_heavy(R, PASS_NUMERIC)) ||
+  if ((!R.dense && (rc = plan_hash(R, h1, cap, nullptr, R.row_flops))) ||
+      (rc = fill_zero(ctx, row_nnz, (cnt_words + 4) * 4)) || (rc = prepare_heavy(R, PASS_NUMERIC)) ||
       (!R.dense && (rc = reset_hash_pool(R, h1, true))))
     return fail(rc);
   auto direct = [&](PassParams& P, const uint32_t* caps) {
@@ -670,10 +723,12 @@ static int single_pass_direct(Run& R, const ptc_dtensor* A, const ptc_dtensor* B
   PassParams P;
   base_params(R, P, R.dense ? nullptr : &h1);
   direct(P, cap);
-  if ((rc = run_pass(R, PASS_NUMERIC, &h1, R.items, R.nitems, P))) return fail(rc);
+  if (ahead) P.live_items = R.packed + R.nrows;
+  if ((rc = run_pass(R, PASS_NUMERIC, &h1, R.items, nitems, P))) return fail(rc);
   T.mark();  // 2
-  // the consistency flags, the overflow count and the entries written: one read
-  unsigned long long rb[2] = {0, 0};
+  // the consistency flags, the overflow count and the entries written (and,
+  // ahead, the preprocessing totals): one read
+  unsigned long long rb[7] = {0, 0, 0, 0, 0, 0, 0};
   {
     WordSources W{};
     W.n = 2;
@@ -681,7 +736,14 @@ static int single_pass_direct(Run& R, const ptc_dtensor* A, const ptc_dtensor* B
     W.bytes[0] = 4;
     W.src[1] = counter;
     W.bytes[1] = 8;
-    if ((rc = collect_words(ctx, pair, W)) || (rc = read_back(ctx, rb, pair, 16))) return fail(rc);
+    if (ahead) {  // and the preprocessing totals
+      for (uint32_t i = 0; i < R.totw.n; ++i) {
+        W.src[W.n] = R.totw.src[i];
+        W.bytes[W.n++] = R.totw.bytes[i];
+      }
+    }
+    if ((rc = collect_words(ctx, pair, W)) || (rc = read_back(ctx, rb, pair, 8 * W.n))) return fail(rc);
+    if (ahead && (rc = prepare_finish(R, rb + 2))) return fail(rc);
   }
   const uint32_t ovf0 = (uint32_t)rb[0];
   if (ovf0 & 0x80000000u) return fail(set_err(ctx, PTC_ERR_INTERNAL, "accumulator table overflow"));
@@ -1094,8 +1156,22 @@ int tc_impl(ptc_ctx* ctx, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B, 
   ctx->peak_bytes = live0;
   auto wall0 = std::chrono::steady_clock::now();
   T.mark();  // 0
-  int rc = prepare(R, HT_A, HT_B, A, B, opt);
+  // Preprocessing.  A single-pass contraction over dense rows whose output
+  // and split-row slabs have host-known bounds within AHEAD_BYTES runs its
+  // numeric pass ahead of the preprocessing totals (R.ahead, one round trip
+  // in all); every other contraction reads the totals back first.
+  unsigned long long* tot = R.ws.get<unsigned long long>(5);
+  if (!tot) return PTC_ERR_NOMEM;
+  R.allow_ahead = opt.method != PTC_METHOD_TWO_PASS && !kr && !into && !col_first;
+  int rc = prepare_launch(R, HT_A, HT_B, A, B, opt, nullptr);
   if (rc) return rc;
+  const bool ahead = R.ahead;
+  if (!ahead) {
+    unsigned long long hostv5[5] = {0, 0, 0, 0, 0};
+    if ((rc = collect_words(ctx, tot, R.totw)) || (rc = read_back(R.ctx, hostv5, tot, 8 * R.totw.n)) ||
+        (rc = prepare_finish(R, hostv5)))
+      return rc;
+  }
   Workspace& ws = R.ws;
   unsigned long long* out_off = R.out_off;
   unsigned long long hostv[2] = {0, 0};
@@ -1172,7 +1248,8 @@ int tc_impl(ptc_ctx* ctx, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B, 
     if (!cap || !cap_off) return set_err(ctx, PTC_ERR_INTERNAL, "capacities were not computed");
     if (!kr) {
       rc = R.win_n ? win_single_pass(R, W, A, B, cap, cap_off, method, T, &out, &nnz, &fallback_rows, &out_capacity)
-                   : single_pass_direct(R, A, B, cap, cap_off, method, T, &out, &nnz, &fallback_rows, &out_capacity);
+                   : single_pass_direct(R, A, B, cap, cap_off, method, T, &out, &nnz, &fallback_rows, &out_capacity,
+                                        ahead);
       if (rc) return rc;
       flags_checked = true;
     } else {
