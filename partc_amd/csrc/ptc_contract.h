@@ -66,6 +66,8 @@ struct ItemParams {
 };
 
 struct ItemP0Params {
+  uint64_t nitems;                   // work items (a bound when live is set)
+  const unsigned long long* live;    // non-NULL: the item count on the device (low 32 bits)
   uint64_t npos;
   const uint32_t* row_of_pos;
   const uint32_t* row_nchunk;

@@ -179,28 +179,6 @@ __global__ void k_items(ItemParams P) {
   }
 }
 
-// start position of every chunk k > 0 of a heavy row: the position whose
-// flop range contains the chunk's first flop
-__global__ void k_item_p0(ItemP0Params P) {
-  for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < P.npos; p += (uint64_t)gridDim.x * blockDim.x) {
-    uint32_t r = P.row_of_pos[p];
-    uint32_t nch = P.row_nchunk[r];
-    if (nch <= 1) continue;
-    uint32_t len = P.pos_len[p];
-    if (!len) continue;
-    KtSlice s = P.rows[r];
-    unsigned long long flo = P.apre[s.off];
-    unsigned long long f = P.row_flops[r];
-    unsigned long long per = (f + nch - 1) / nch;
-    unsigned long long a = P.apre[p] - flo, bnd = a + len;  // [a, bnd) relative to row
-    // chunks k with k*per in [a, bnd), k >= 1
-    unsigned long long k0 = (a + per - 1) / per;
-    if (k0 == 0) k0 = 1;
-    uint64_t io = P.packed[r] & 0xFFFFFFFFull;
-    for (unsigned long long k = k0; k < nch && k * per < bnd; ++k) P.items[io + k].z = (uint32_t)p;
-  }
-}
-
 // hash capacities: pow2 >= 2*bound (min 16); rows that must live in HBM
 // (several chunks, or too big for the LDS table) get a global capacity
 __global__ void k_hash_caps(HashCapParams P) {
@@ -1119,9 +1097,41 @@ int launch_rows(ptc_ctx* ctx, const RowParams& P) {
   return PTC_OK;
 }
 
+// First position of every chunk k > 0 of a split row, one wavefront per
+// work item: the last position p of the row with apre[p] <= the chunk's first
+// flop (the position whose flop range holds it; zero-length positions share
+// the apre of the position after them), found by a 64-way search -- a few
+// rounds of 64 loads instead of a pass over every A position.
+__global__ void __launch_bounds__(256) k_item_starts(ItemP0Params P) {
+  const uint64_t n = P.live ? (*P.live & 0xFFFFFFFFull) : P.nitems;
+  const int lane = threadIdx.x & 63;
+  const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x / 64);
+  for (uint64_t i = (uint64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64; i < n; i += waves) {
+    const uint4 it = P.items[i];
+    if (it.y == 0) continue;  // a row's first chunk starts at its first position
+    const uint32_t r = it.x, nch = P.row_nchunk[r];
+    const KtSlice row = P.rows[r];
+    const unsigned long long flo = P.apre[row.off], f = P.row_flops[r];
+    const unsigned long long target = flo + (unsigned long long)it.y * ((f + nch - 1) / nch);
+    uint64_t lo = row.off, hi = (uint64_t)row.off + row.len;  // apre[lo] <= target throughout
+    while (hi - lo > 64) {
+      const uint64_t step = (hi - lo + 63) / 64;
+      const uint64_t q = lo + (uint64_t)lane * step;
+      const unsigned long long m = __ballot(q < hi && P.apre[q] <= target);
+      const uint64_t nlo = lo + (uint64_t)(63 - __clzll(m)) * step;  // lane 0 holds: m != 0
+      hi = min(hi, nlo + step);
+      lo = nlo;
+    }
+    const uint64_t q = lo + lane;
+    const unsigned long long m = __ballot(q < hi && P.apre[q] <= target);
+    if (lane == 0) P.items[i].z = (uint32_t)(lo + (uint64_t)(63 - __clzll(m)));
+  }
+}
+
 int launch_items(ptc_ctx* ctx, const ItemParams& P, const ItemP0Params& Q) {
   if (P.nrows) hipLaunchKernelGGL(k_items, dim3(gridn(P.nrows, 256)), dim3(256), 0, ctx->stream, P);
-  if (Q.npos) hipLaunchKernelGGL(k_item_p0, dim3(gridn(Q.npos, 256)), dim3(256), 0, ctx->stream, Q);
+  if (Q.npos && Q.nitems)
+    hipLaunchKernelGGL(k_item_starts, dim3(gridn(Q.nitems, 4, 4096)), dim3(256), 0, ctx->stream, Q);
   PTC_HIP(ctx, hipGetLastError());
   return PTC_OK;
 }

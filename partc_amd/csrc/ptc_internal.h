@@ -237,6 +237,9 @@ int scan_u32_to_u64(ptc_ctx* ctx, const uint32_t* in, unsigned long long* out, u
                     unsigned long long* total);
 int scan_u64(ptc_ctx* ctx, const unsigned long long* in, unsigned long long* out, uint64_t n,
              unsigned long long* total);
+// Two arrays of n + 1 words scanned in place, each total written at [n]
+// (one launch while n is small).
+int scan_u64_two_inplace(ptc_ctx* ctx, unsigned long long* a, unsigned long long* b, uint64_t n);
 
 int fill_zero(ptc_ctx* ctx, void* p, size_t bytes);
 

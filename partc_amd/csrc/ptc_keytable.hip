@@ -1743,6 +1743,68 @@ __global__ void k_pt_overflow(const uint32_t* __restrict__ cnt, const uint32_t* 
   }
 }
 
+// The last step of a single partitioned table, one workgroup per bucket:
+// the bucket's slices to their compact positions (the bucket's place among
+// the compact slices is the sum of the earlier buckets' key counts, which
+// the workgroup adds up itself), the probe index without region mode, the
+// key total (the last bucket); then, spread over all workgroups, the keys
+// whose probe ran past their region (ovf_cnt non-NULL), inserted from their
+// home line now that every region is written.  One launch for what a scan,
+// pt_slices_one and k_pt_overflow do in three.
+__global__ void __launch_bounds__(256) k_pt_finish(const unsigned long long* __restrict__ start, uint32_t nb,
+                                                   const uint32_t* __restrict__ bkeys,
+                                                   const KtSlice* __restrict__ tmps, KtSlice* __restrict__ slices,
+                                                   KtEntry* __restrict__ tab, uint64_t cap, int index, int direct,
+                                                   unsigned long long* __restrict__ total, uint64_t N,
+                                                   const uint32_t* __restrict__ ovf_cnt,
+                                                   const uint32_t* __restrict__ ovf_list) {
+  __shared__ unsigned long long s_red[256 / 64];
+  const uint32_t b = blockIdx.x;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  unsigned long long pre = 0;
+  for (uint32_t j = threadIdx.x; j < b; j += blockDim.x) pre += bkeys[j];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) pre += __shfl_xor(pre, o, 64);
+  if (lane == 0) s_red[wid] = pre;
+  __syncthreads();
+  pre = s_red[0] + s_red[1] + s_red[2] + s_red[3];
+  const uint64_t bs = start[b];
+  const uint32_t nk = bkeys[b];
+  if (b + 1 == nb && threadIdx.x == 0) *total = ((pre + nk) << 32) | N;
+  for (uint32_t j = threadIdx.x; j < nk; j += blockDim.x) {
+    const KtSlice sl = tmps[bs + j];
+    slices[pre + j] = sl;
+    if (!index) continue;
+    if (direct) {
+      tab[sl.key].key = sl.key;
+      tab[sl.key].off = sl.off;
+      tab[sl.key].len = sl.len;
+      continue;
+    }
+    uint64_t i = kt_home(mix64(sl.key), cap);
+    while (true) {
+      const unsigned long long old = atomicCAS(&tab[i].key, PTC_EMPTY_KEY, sl.key);
+      if (old == PTC_EMPTY_KEY) break;  // keys are distinct: a claimed entry is never this key's
+      i = (i + 1) & (cap - 1);
+    }
+    tab[i].off = sl.off;
+    tab[i].len = sl.len;
+  }
+  if (!ovf_cnt) return;
+  const uint32_t no = *ovf_cnt;
+  for (uint32_t k = b * blockDim.x + threadIdx.x; k < no; k += nb * blockDim.x) {
+    const KtSlice sl = tmps[ovf_list[k]];
+    uint64_t i = kt_home(mix64(sl.key), cap);
+    while (true) {
+      const unsigned long long old = atomicCAS(&tab[i].key, PTC_EMPTY_KEY, sl.key);
+      if (old == PTC_EMPTY_KEY) break;
+      i = (i + 1) & (cap - 1);
+    }
+    tab[i].off = sl.off;
+    tab[i].len = sl.len;
+  }
+}
+
 // ---------------------------------------------------------- many tables
 //
 // A set of FP64 tables built together (a batch plan): the same counting
@@ -2030,7 +2092,6 @@ int build_partitioned(ptc_ctx* ctx, ptc_hashtable* ht, const ptc_dtensor* t, boo
   uint32_t* gcnt = ws.get<uint32_t>(N);
   KtSlice* tmps = ws.get<KtSlice>(N);
   uint32_t* bkeys = ws.get<uint32_t>(nb);
-  unsigned long long* sbase = ws.get<unsigned long long>(nb + 1);
   uint32_t* ovf = ws.get<uint32_t>(1);         // keys whose probe ran past their region: count,
   uint32_t* ovf_list = ws.get<uint32_t>(N);    // and list
   ht->d_total = dalloc_n<unsigned long long>(ctx, 1);
@@ -2040,7 +2101,7 @@ int build_partitioned(ptc_ctx* ctx, ptc_hashtable* ht, const ptc_dtensor* t, boo
   // contraction) are also written as 16-bit words
   if (!rest64 && ht->rest.space <= 65536 && !(ht->rest_key16 = dalloc_n<uint16_t>(ctx, std::max<uint64_t>(N, 1))))
     return PTC_ERR_NOMEM;
-  if (!tcnt || !toff || !segpre || !btot || !off || !pkey || !pid || !ppay || !gkey || !gcnt || !tmps || !bkeys || !sbase || !ovf ||
+  if (!tcnt || !toff || !segpre || !btot || !off || !pkey || !pid || !ppay || !gkey || !gcnt || !tmps || !bkeys || !ovf ||
       !ovf_list ||
       !ht->d_total || !ht->slices)
     return PTC_ERR_NOMEM;
@@ -2079,13 +2140,10 @@ int build_partitioned(ptc_ctx* ctx, ptc_hashtable* ht, const ptc_dtensor* t, boo
     else PTC_PT(uint32_t, double, t->vals, ht->rest_key32, ht->gval, ht->rest_key16);
   }
 #undef PTC_PT
-  if ((rc = scan_u32_to_u64(ctx, bkeys, sbase, nb, sbase + nb))) return rc;
-  hipLaunchKernelGGL(k_pt_slices, dim3(nb), dim3(256), 0, st, (const unsigned long long*)off, (uint64_t)1, nb,
-                     (const uint32_t*)bkeys, (const unsigned long long*)sbase, (const KtSlice*)tmps, ht->slices,
-                     ht->entries, ht->cap, region ? 0 : 1, ht->direct ? 1 : 0, ht->d_total, N);
-  if (region)
-    hipLaunchKernelGGL(k_pt_overflow, dim3(64), dim3(256), 0, st, (const uint32_t*)ovf, (const uint32_t*)ovf_list,
-                       (const KtSlice*)tmps, ht->entries, ht->cap);
+  hipLaunchKernelGGL(k_pt_finish, dim3(nb), dim3(256), 0, st, (const unsigned long long*)off, nb,
+                     (const uint32_t*)bkeys, (const KtSlice*)tmps, ht->slices, ht->entries, ht->cap,
+                     region ? 0 : 1, ht->direct ? 1 : 0, ht->d_total, N, region ? (const uint32_t*)ovf : nullptr,
+                     (const uint32_t*)ovf_list);
   PTC_HIP(ctx, hipGetLastError());
   return PTC_OK;
 }

@@ -566,6 +566,37 @@ int scan_u32_features(ptc_ctx* ctx, const uint32_t* in, unsigned long long* out,
   return scan_two<uint32_t, true>(ctx, in, out, n, total, F);
 }
 
+// Two independent small scans in one launch: workgroup i scans array i.
+__global__ void __launch_bounds__(SCAN_SMALL_THREADS) k_scan_small2(unsigned long long* __restrict__ a,
+                                                                    unsigned long long* __restrict__ b, uint64_t n) {
+  unsigned long long* x = blockIdx.x ? b : a;
+  __shared__ unsigned long long wsum[SCAN_SMALL_THREADS / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const uint64_t base = (uint64_t)threadIdx.x * SCAN_SMALL_ITEMS;
+  unsigned long long v[SCAN_SMALL_ITEMS], s = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_SMALL_ITEMS; ++k) {
+    v[k] = load_elem(x, base + k, n);
+    s += v[k];
+  }
+  unsigned long long y = s;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const unsigned long long t = __shfl_up(y, o, 64);
+    if (lane >= o) y += t;
+  }
+  if (lane == 63) wsum[wid] = y;
+  __syncthreads();
+  unsigned long long run = y - s;
+  for (int w = 0; w < wid; ++w) run += wsum[w];
+#pragma unroll
+  for (int k = 0; k < SCAN_SMALL_ITEMS; ++k) {
+    if (base + k < n) x[base + k] = run;
+    run += v[k];
+  }
+  if (threadIdx.x == SCAN_SMALL_THREADS - 1) x[n] = run;  // the total after the last element
+}
+
 template <class In>
 static int scan_impl(ptc_ctx* ctx, const In* in, unsigned long long* out, uint64_t n,
                      unsigned long long* total) {
@@ -607,6 +638,15 @@ int scan_u32_to_u64(ptc_ctx* ctx, const uint32_t* in, unsigned long long* out, u
 int scan_u64(ptc_ctx* ctx, const unsigned long long* in, unsigned long long* out, uint64_t n,
              unsigned long long* total) {
   return scan_impl<unsigned long long>(ctx, in, out, n, total);
+}
+int scan_u64_two_inplace(ptc_ctx* ctx, unsigned long long* a, unsigned long long* b, uint64_t n) {
+  if (n == 0 || n > SCAN_SMALL) {
+    int rc = scan_u64(ctx, a, a, n, a + n);
+    return rc ? rc : scan_u64(ctx, b, b, n, b + n);
+  }
+  hipLaunchKernelGGL(k_scan_small2, dim3(2), dim3(SCAN_SMALL_THREADS), 0, ctx->stream, a, b, n);
+  PTC_HIP(ctx, hipGetLastError());
+  return PTC_OK;
 }
 
 // ------------------------------------------------------------- key codec

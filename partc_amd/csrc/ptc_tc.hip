@@ -485,8 +485,7 @@ int prepare_launch(Run& R, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B,
     P.row_maxlen = parts ? R.row_maxlen : nullptr;
     if ((rc = launch_rows(ctx, P))) return rc;
   }
-  if ((rc = scan_u64(ctx, R.packed, R.packed, R.nrows, R.packed + R.nrows))) return rc;
-  if ((rc = scan_u64(ctx, R.row_slab, R.row_slab, R.nrows, R.row_slab + R.nrows))) return rc;
+  if ((rc = scan_u64_two_inplace(ctx, R.packed, R.row_slab, R.nrows))) return rc;
   // Dense rows with the default work-item rule have at most rows +
   // DENSE_MAX_ITEMS items (each row's chunk count is at most its flops /
   // chunk + 1 and the chunk is at least total / DENSE_MAX_ITEMS): the items
@@ -600,6 +599,8 @@ int build_items(Run& R, uint64_t nitems, uint64_t nheavy, uint64_t npos) {
   P.heavy_rows = R.heavy_rows;
   P.items = R.items;
   ItemP0Params Q{};
+  Q.nitems = nitems;
+  Q.live = R.early_items ? R.packed + R.nrows : nullptr;
   Q.npos = npos;
   Q.row_of_pos = R.row_of_pos;
   Q.row_nchunk = R.row_nchunk;
