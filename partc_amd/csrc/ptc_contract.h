@@ -21,7 +21,6 @@ constexpr int PB_MAX = 1024;          // largest row-pass workgroup (its launch 
 struct ProbeParams {
   uint64_t npos;
   const void* akey;               // HT_A rest keys (A's contracted tuple)
-  const uint32_t* row_of_pos;     // HT_A slice_of
   KeyCodec acodec, bcodec;        // A's contracted codec, B's key codec
   int same_codec;                 // the two codecs linearise identically
   const KtEntry* tab;             // HT_B entries
@@ -29,8 +28,6 @@ struct ProbeParams {
   int direct;                     // HT_B addressed by key
   uint32_t* pos_len;
   uint32_t* pos_off;
-  uint32_t* row_hits;
-  uint32_t* row_maxlen;
   uint32_t* zero4;                // non-NULL: four words cleared by the launch (the passes' flags)
 };
 

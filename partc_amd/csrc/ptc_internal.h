@@ -257,6 +257,17 @@ struct ScanFeatures {
   FeatPart* parts;             // scratch: one per tile
 };
 
+// Per TILE_PART elements (half a scan tile): the count of nonzero elements
+// and the largest element.
+constexpr uint32_t TILE_PART = 1024;
+struct TilePart {
+  uint32_t h, m;
+};
+// Exclusive scan of n u32 elements that also leaves the TilePart of every
+// TILE_PART elements (tiles: 2 * ceil(n / (2 * TILE_PART)) entries).
+int scan_u32_tiles(ptc_ctx* ctx, const uint32_t* in, unsigned long long* out, uint64_t n,
+                   unsigned long long* total, TilePart* tiles);
+
 // Exclusive scan of the A positions' slice lengths that also writes the row
 // features (every row that has positions; rows past them are left alone).
 int scan_u32_features(ptc_ctx* ctx, const uint32_t* in, unsigned long long* out, uint64_t n,

@@ -14,6 +14,11 @@ struct LearnedInputs {
   const uint32_t* row_hits;            // distinct contracted keys that hit B
   const uint32_t* row_maxlen;          // longest slice
   uint64_t ncols;                      // column space
+  // with row_hits NULL (capacities only) the kernel finds the hits and the
+  // longest slice itself: from every position's slice length and the
+  // TilePart of every TILE_PART positions (parts strictly inside a row)
+  const uint32_t* pos_len;
+  const TilePart* tiles;
 };
 
 // pred[r] = predicted nonzeros of row r (device array of nrows floats).

@@ -137,56 +137,167 @@ __global__ void __launch_bounds__(256) k_mlp_predict(const float* __restrict__ w
 // single-pass method): the features of k_mlp_features, the network of
 // k_mlp_predict (the same operations in the same order), then the capacity
 // rule of k_caps: clamp(ceil(scale * prediction), longest slice,
-// min(flops, columns)).  Lane i holds hidden unit i's weights in registers
-// (its first-layer row, its second-layer row W2[i][*], biases and output
-// weight), loaded once per wavefront; each wavefront then runs a few rows.
-constexpr int CAPS_ROWS_PER_WAVE = 4;
-__global__ void __launch_bounds__(256) k_mlp_caps(const float* __restrict__ wg, LearnedInputs in, double scale,
-                                                  uint32_t* __restrict__ cap) {
-  static_assert(MLP_H == 64, "one lane per hidden unit");
-  const int lane = threadIdx.x & 63;
-  float w1r[MLP_IN], w2r[MLP_H];
+// min(flops, columns)).  Lane i computes hidden unit i from the weights,
+// staged once per workgroup in LDS (W2 transposed, so the 64 lanes read 64
+// consecutive words); each wavefront runs one row at a time, its features'
+// loads all in flight together.
+constexpr int CAPS_THREADS = 512;
+constexpr int CAPS_WAVES_PER_SIMD = 4;  // two workgroups a CU: every workgroup of a few thousand rows resident at once
+__device__ __forceinline__ float lane_value(float x, int k) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), k));
+}
+
+// Row r's count of positions that hit B and its longest slice, over the
+// wavefront (the same values in every lane), for a row of positions [a, b):
+// the positions of the parts (TILE_PART positions each) the row only partly
+// covers, at most two, in 16-byte words, all loads of a lane in flight
+// together (pos_len holds three words past its last position, so the word
+// of the last position is in bounds; positions outside the row are
+// masked); the parts strictly inside the row by their TilePart.
+constexpr int FEAT_HEAD_Q = (TILE_PART / 4 + 1 + 63) / 64;  // 16-byte words a lane, unaligned start
+constexpr int FEAT_TAIL_Q = (TILE_PART / 4 + 63) / 64;      // 16-byte words a lane, part-aligned start
+__device__ __forceinline__ void feat_word(const uint4& w, uint64_t q, uint64_t lo, uint64_t hi, uint32_t& h,
+                                          uint32_t& m) {
+  const uint32_t v[4] = {w.x, w.y, w.z, w.w};
 #pragma unroll
-  for (int k = 0; k < MLP_IN; ++k) w1r[k] = wg[OFF_W1 + lane * MLP_IN + k];
-  static_assert(OFF_W2 % 4 == 0 && MLP_H % 4 == 0, "second-layer rows are 16-byte aligned");
-  const float4* w2row = reinterpret_cast<const float4*>(wg + OFF_W2 + lane * MLP_H);
-#pragma unroll
-  for (int k = 0; k < MLP_H / 4; ++k) {
-    const float4 q = w2row[k];
-    w2r[4 * k] = q.x;
-    w2r[4 * k + 1] = q.y;
-    w2r[4 * k + 2] = q.z;
-    w2r[4 * k + 3] = q.w;
+  for (int c = 0; c < 4; ++c) {
+    const uint64_t e = 4 * q + c;
+    const uint32_t l = e >= lo && e < hi ? v[c] : 0u;
+    h += l ? 1u : 0u;
+    m = max(m, l);
   }
-  const float b1 = wg[OFF_B1 + lane], b2 = wg[OFF_B2 + lane], w3 = wg[OFF_W3 + lane], b3 = wg[OFF_B3];
+}
+__device__ __forceinline__ void row_features(const LearnedInputs& in, const KtSlice& sl, int lane, uint32_t& hits,
+                                             uint32_t& maxlen) {
+  const uint64_t a = sl.off, b = a + sl.len;  // len >= 1: the row has flops
+  const uint64_t ta = a / TILE_PART, tb = (b - 1) / TILE_PART;
+  const uint64_t e0 = ta == tb ? b : (ta + 1) * TILE_PART;  // the head: [a, e0)
+  const uint64_t s1 = tb * TILE_PART;                         // the tail: [s1, b) when ta < tb
+  const uint4* __restrict__ w4 = reinterpret_cast<const uint4*>(in.pos_len);
+  const uint64_t hq0 = a >> 2, hq1 = (e0 + 3) >> 2, tq0 = s1 >> 2, tq1 = ta == tb ? tq0 : (b + 3) >> 2;
+  uint4 hw[FEAT_HEAD_Q], tw[FEAT_TAIL_Q];
+#pragma unroll
+  for (int j = 0; j < FEAT_HEAD_Q; ++j) {
+    const uint64_t q = hq0 + lane + 64 * j;
+    hw[j] = q < hq1 ? w4[q] : make_uint4(0, 0, 0, 0);
+  }
+#pragma unroll
+  for (int j = 0; j < FEAT_TAIL_Q; ++j) {
+    const uint64_t q = tq0 + lane + 64 * j;
+    tw[j] = q < tq1 ? w4[q] : make_uint4(0, 0, 0, 0);
+  }
+  uint32_t h = 0, m = 0;
+  for (uint64_t t0 = ta + 1 + lane; t0 < tb; t0 += 4 * 64) {  // four tile loads a lane in flight
+    TilePart p[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) p[u] = t0 + u * 64 < tb ? in.tiles[t0 + u * 64] : TilePart{0, 0};
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      h += p[u].h;
+      m = max(m, p[u].m);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < FEAT_HEAD_Q; ++j) feat_word(hw[j], hq0 + lane + 64 * j, a, e0, h, m);
+#pragma unroll
+  for (int j = 0; j < FEAT_TAIL_Q; ++j) feat_word(tw[j], tq0 + lane + 64 * j, s1, ta == tb ? s1 : b, h, m);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    h += __shfl_xor(h, o, 64);
+    m = max(m, (uint32_t)__shfl_xor(m, o, 64));
+  }
+  hits = h;
+  maxlen = m;
+}
+
+__global__ void __launch_bounds__(CAPS_THREADS, CAPS_WAVES_PER_SIMD) k_mlp_caps(const float* __restrict__ wg, LearnedInputs in,
+                                                           double scale, uint32_t* __restrict__ cap) {
+  static_assert(MLP_H == 64, "one lane per hidden unit");
+  __shared__ float w2t[MLP_H * MLP_H];  // W2 transposed: w2t[k][i] = W2[i][k]
+  __shared__ float w1s[MLP_H * MLP_IN], b1s[MLP_H], b2s[MLP_H], w3s[MLP_H];
+  const int lane = threadIdx.x & 63;
   const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x / 64);
-  for (uint64_t r = (uint64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64; r < in.nrows; r += waves) {
-    const unsigned long long fl = in.row_flops[r];
-    const uint32_t maxlen = in.row_maxlen[r];
-    const double f = (double)fl;
-    const double u = f < (double)in.ncols ? f : (double)in.ncols;
-    // lane k < MLP_IN computes feature k
-    const double src = lane == 0 ? (double)in.rows[r].len : lane == 1 ? f : lane == 2 ? (double)in.row_hits[r]
-                     : lane == 3 ? (double)maxlen : lane == 4 ? (double)in.ncols : u;
-    const float xl = lane < MLP_IN ? (float)(log1p(src) * 0.125) : 0.f;
-    float a = b1;
+  uint64_t r = (uint64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+  // every staging load in flight before the first LDS write
+  static_assert(OFF_W2 % 4 == 0, "W2 is 16-byte aligned");
+  static_assert(MLP_H * MLP_H % (4 * CAPS_THREADS) == 0 && MLP_H * MLP_IN <= CAPS_THREADS, "staging shape");
+  constexpr int W2Q = MLP_H * MLP_H / (4 * CAPS_THREADS);
+  float4 q[W2Q];
 #pragma unroll
-    for (int k = 0; k < MLP_IN; ++k) a += w1r[k] * __shfl(xl, k, 64);
-    const float h1 = a > 0.f ? a : 0.f;
-    float s = b2;
+  for (int j = 0; j < W2Q; ++j) q[j] = reinterpret_cast<const float4*>(wg + OFF_W2)[threadIdx.x + j * CAPS_THREADS];
+  const float w1v = threadIdx.x < MLP_H * MLP_IN ? wg[OFF_W1 + threadIdx.x] : 0.f;
+  const int hu = threadIdx.x & (MLP_H - 1);
+  const float b1v = wg[OFF_B1 + hu], b2v = wg[OFF_B2 + hu], w3v = wg[OFF_W3 + hu];
+  // the first row's flops and slice, in flight with the weights (the row's
+  // slice is read with its flops: r < nrows <= the slices allocated; used
+  // only when the row has flops)
+  unsigned long long fl = 0;
+  KtSlice sl{};
+  if (r < in.nrows) {
+    fl = in.row_flops[r];
+    sl = in.rows[r];
+  }
 #pragma unroll
-    for (int k = 0; k < MLP_H; ++k) s += w2r[k] * __shfl(h1, k, 64);
-    const float h2 = s > 0.f ? s : 0.f;
-    float z = w3 * h2;
-    for (int o = 32; o > 0; o >>= 1) z += __shfl_down(z, o, 64);
-    if (lane == 0) {
-      const float pred = (float)u * sigmoidf_(z + b3);
-      const unsigned long long ub = fl < in.ncols ? fl : in.ncols;
-      const double want = ceil((double)pred * scale);
-      const unsigned long long wv = want < 0 ? 0ull : (unsigned long long)fmin(want, 1.8e19);
-      unsigned long long c = wv < maxlen ? maxlen : wv;
-      if (c > ub) c = ub;
-      cap[r] = c > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)c;
+  for (int j = 0; j < W2Q; ++j) {
+    const uint32_t t = threadIdx.x + j * CAPS_THREADS;
+    const uint32_t i = (4 * t) / MLP_H, k = (4 * t) % MLP_H;
+    w2t[(k + 0) * MLP_H + i] = q[j].x;
+    w2t[(k + 1) * MLP_H + i] = q[j].y;
+    w2t[(k + 2) * MLP_H + i] = q[j].z;
+    w2t[(k + 3) * MLP_H + i] = q[j].w;
+  }
+  if (threadIdx.x < MLP_H * MLP_IN) w1s[threadIdx.x] = w1v;
+  if (threadIdx.x < MLP_H) {
+    b1s[threadIdx.x] = b1v;
+    b2s[threadIdx.x] = b2v;
+    w3s[threadIdx.x] = w3v;
+  }
+  const float b3 = wg[OFF_B3];
+  __syncthreads();
+  for (; r < in.nrows; r += waves) {
+    uint32_t hits = 0, maxlen = 0;
+    if (in.row_hits) {
+      hits = in.row_hits[r];
+      maxlen = in.row_maxlen[r];
+    } else if (fl) {
+      row_features(in, sl, lane, hits, maxlen);
+    }
+    if (fl) {
+      const double f = (double)fl;
+      const double u = f < (double)in.ncols ? f : (double)in.ncols;
+      // lane k < MLP_IN computes feature k
+      const double src = lane == 0 ? (double)sl.len : lane == 1 ? f : lane == 2 ? (double)hits
+                       : lane == 3 ? (double)maxlen : lane == 4 ? (double)in.ncols : u;
+      const float xl = lane < MLP_IN ? (float)(log1p(src) * 0.125) : 0.f;
+      // lane k's value to every lane by readlane (a scalar register): the
+      // 64-step sum below is one chain of adds, so each step must not wait
+      // on a cross-lane LDS permute
+      float a = b1s[lane];
+#pragma unroll
+      for (int k = 0; k < MLP_IN; ++k) a += w1s[lane * MLP_IN + k] * lane_value(xl, k);
+      const float h1 = a > 0.f ? a : 0.f;
+      float s = b2s[lane];
+#pragma unroll
+      for (int k = 0; k < MLP_H; ++k) s += w2t[k * MLP_H + lane] * lane_value(h1, k);
+      const float h2 = s > 0.f ? s : 0.f;
+      float z = w3s[lane] * h2;
+      for (int o = 32; o > 0; o >>= 1) z += __shfl_down(z, o, 64);
+      if (lane == 0) {
+        const float pred = (float)u * sigmoidf_(z + b3);
+        const unsigned long long ub = fl < in.ncols ? fl : in.ncols;
+        const double want = ceil((double)pred * scale);
+        const unsigned long long wv = want < 0 ? 0ull : (unsigned long long)fmin(want, 1.8e19);
+        unsigned long long c = wv < maxlen ? maxlen : wv;
+        if (c > ub) c = ub;
+        cap[r] = c > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)c;
+      }
+    } else if (lane == 0) {
+      cap[r] = 0;  // no flops (rows past the table too): capacity 0 whatever the network says
+    }
+    const uint64_t rn = r + waves;
+    if (rn < in.nrows) {
+      fl = in.row_flops[rn];
+      sl = in.rows[rn];
     }
   }
 }
@@ -316,7 +427,7 @@ int learned_caps(ptc_ctx* ctx, const ptc_mlp* mlp, const LearnedInputs& in, doub
   if (!mlp) mlp = g_default;
   if (!mlp || !mlp->w) return set_err(ctx, PTC_ERR_ARG, "no row-size model");
   if (!in.nrows) return PTC_OK;
-  hipLaunchKernelGGL(k_mlp_caps, dim3(gridn(in.nrows, 4 * CAPS_ROWS_PER_WAVE, 2048)), dim3(256), 0, ctx->stream,
+  hipLaunchKernelGGL(k_mlp_caps, dim3(gridn(in.nrows, CAPS_THREADS / 64, 1024)), dim3(CAPS_THREADS), 0, ctx->stream,
                      (const float*)mlp->w, in, scale, cap);
   PTC_HIP(ctx, hipGetLastError());
   return PTC_OK;

@@ -556,6 +556,85 @@ static int scan_two(ptc_ctx* ctx, const In* in, unsigned long long* out, uint64_
 // tiles whose prefix a workgroup sums itself (at most 64 words a thread)
 constexpr uint64_t SCAN_TWO_MAX_TILES = 64ull * SCAN_THREADS;
 
+// per tile: the element total; per half tile: the count of nonzero
+// elements and the largest
+__global__ void __launch_bounds__(SCAN_THREADS) k_scan_reduce_tiles(const uint32_t* __restrict__ in, uint64_t n,
+                                                                    unsigned long long* __restrict__ block_sums,
+                                                                    TilePart* __restrict__ tiles) {
+  static_assert(SCAN_TILE == 2 * TILE_PART && SCAN_ITEMS % 2 == 0, "two tile parts a scan tile");
+  __shared__ unsigned long long s_w[SCAN_THREADS / 64][5];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const uint64_t b0 = (uint64_t)blockIdx.x * SCAN_TILE;
+  unsigned long long sum = 0;
+  uint32_t h[2] = {0, 0}, m[2] = {0, 0};
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) {  // element k * SCAN_THREADS + thread: the first half for k < SCAN_ITEMS / 2
+    const uint64_t e = b0 + (uint64_t)k * SCAN_THREADS + threadIdx.x;
+    const uint32_t l = e < n ? in[e] : 0u;
+    const int half = k < SCAN_ITEMS / 2 ? 0 : 1;
+    sum += l;
+    h[half] += l ? 1u : 0u;
+    m[half] = max(m[half], l);
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    sum += __shfl_xor(sum, o, 64);
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      h[j] += __shfl_xor(h[j], o, 64);
+      m[j] = max(m[j], (uint32_t)__shfl_xor(m[j], o, 64));
+    }
+  }
+  if (lane == 0) {
+    s_w[wid][0] = sum;
+    s_w[wid][1] = h[0];
+    s_w[wid][2] = m[0];
+    s_w[wid][3] = h[1];
+    s_w[wid][4] = m[1];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    TilePart P0{0, 0}, P1{0, 0};
+    sum = 0;
+    for (int w = 0; w < SCAN_THREADS / 64; ++w) {
+      sum += s_w[w][0];
+      P0.h += (uint32_t)s_w[w][1];
+      P0.m = max(P0.m, (uint32_t)s_w[w][2]);
+      P1.h += (uint32_t)s_w[w][3];
+      P1.m = max(P1.m, (uint32_t)s_w[w][4]);
+    }
+    block_sums[blockIdx.x] = sum;
+    tiles[2 * blockIdx.x] = P0;
+    tiles[2 * blockIdx.x + 1] = P1;
+  }
+}
+
+template <class In>
+static int scan_impl(ptc_ctx* ctx, const In* in, unsigned long long* out, uint64_t n,
+                     unsigned long long* total);
+
+int scan_u32_tiles(ptc_ctx* ctx, const uint32_t* in, unsigned long long* out, uint64_t n,
+                   unsigned long long* total, TilePart* tiles) {
+  if (n == 0) return total ? fill_zero(ctx, total, sizeof(unsigned long long)) : PTC_OK;
+  const uint64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+  Workspace ws(ctx);
+  unsigned long long* sums = ws.get<unsigned long long>(nb);
+  if (!sums) return PTC_ERR_NOMEM;
+  hipLaunchKernelGGL(k_scan_reduce_tiles, dim3((unsigned)nb), dim3(SCAN_THREADS), 0, ctx->stream, in, n, sums, tiles);
+  if (nb <= SCAN_TWO_MAX_TILES) {
+    hipLaunchKernelGGL((k_scan_apply_p<uint32_t, false>), dim3((unsigned)nb), dim3(SCAN_THREADS), 0, ctx->stream, in,
+                       out, n, (const unsigned long long*)sums, total, ScanFeatures{});
+  } else {
+    PTC_HIP(ctx, hipGetLastError());
+    const int rc = scan_impl<unsigned long long>(ctx, sums, sums, nb, nullptr);
+    if (rc) return rc;
+    hipLaunchKernelGGL((k_scan_apply<uint32_t>), dim3((unsigned)nb), dim3(SCAN_THREADS), 0, ctx->stream, in, out, n,
+                       (const unsigned long long*)sums, total);
+  }
+  PTC_HIP(ctx, hipGetLastError());
+  return PTC_OK;
+}
+
 int scan_u32_features(ptc_ctx* ctx, const uint32_t* in, unsigned long long* out, uint64_t n,
                       unsigned long long* total, const uint32_t* row_of_pos, uint32_t* row_hits,
                       uint32_t* row_maxlen, FeatPart* parts) {

@@ -410,7 +410,7 @@ int prepare_launch(Run& R, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B,
   if ((rc = hashtable_slice_of(ctx, HT_A, &row_of_pos))) return rc;
   R.row_of_pos = row_of_pos;
   Workspace& ws = R.ws;
-  R.pos_len = ws.get<uint32_t>(R.npos);
+  R.pos_len = ws.get<uint32_t>(R.npos + 4);  // + the rest of the last 16-byte word (read whole by the learned features)
   R.pos_off = ws.get<uint32_t>(R.npos);
   R.apre = ws.get<unsigned long long>(R.npos + 1);
   R.row_hits = ws.get<uint32_t>(R.nrows);
@@ -428,13 +428,15 @@ int prepare_launch(Run& R, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B,
     return PTC_ERR_NOMEM;
 
   // ---------------- preprocessing: probe, flop prefix, rows, items
-  const bool features = opt.method == PTC_METHOD_LEARNED || R.want_features;
+  // per-row features as arrays (the training profile); the learned method's
+  // capacity kernel finds them itself from the positions and tile parts
+  const bool features = R.want_features;
+  const bool tile_parts = opt.method == PTC_METHOD_LEARNED && !features;
   if (!R.npos && (rc = fill_zero(ctx, R.overflow, 16))) return rc;  // else cleared by the probe
   {
     ProbeParams P{};
     P.npos = R.npos;
     P.akey = HT_A->rest_key64 ? (const void*)HT_A->rest_key64 : (const void*)HT_A->rest_key32;
-    P.row_of_pos = row_of_pos;
     P.acodec = HT_A->rest;
     P.bcodec = HT_B->codec;
     P.same_codec = 1;
@@ -445,14 +447,17 @@ int prepare_launch(Run& R, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B,
     P.direct = HT_B->direct ? 1 : 0;
     P.pos_len = R.pos_len;
     P.pos_off = R.pos_off;
-    P.row_hits = R.row_hits;
-    P.row_maxlen = R.row_maxlen;
     P.zero4 = R.overflow;
     if ((rc = launch_probe(ctx, P, HT_A->rest_key64 != nullptr))) return rc;
   }
   // the flop prefix over A positions; the row features ride on the same pass
   FeatPart* parts = nullptr;
-  if (features) {
+  TilePart* tiles = nullptr;
+  if (tile_parts) {
+    tiles = ws.get<TilePart>(2 * ((R.npos + 2 * TILE_PART - 1) / (2 * TILE_PART)) + 2);
+    if (!tiles) return PTC_ERR_NOMEM;
+    if ((rc = scan_u32_tiles(ctx, R.pos_len, R.apre, R.npos, R.apre + R.npos, tiles))) return rc;
+  } else if (features) {
     parts = ws.get<FeatPart>((R.npos + FEAT_TILE - 1) / FEAT_TILE + 1);
     if (!parts) return PTC_ERR_NOMEM;
     if ((rc = scan_u32_features(ctx, R.pos_len, R.apre, R.npos, R.apre + R.npos, row_of_pos, R.row_hits,
@@ -519,9 +524,11 @@ int prepare_launch(Run& R, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B,
       L.nrows = R.nrows;
       L.rows = HT_A->slices;
       L.row_flops = R.row_flops;
-      L.row_hits = R.row_hits;
-      L.row_maxlen = R.row_maxlen;
+      L.row_hits = parts ? R.row_hits : nullptr;
+      L.row_maxlen = parts ? R.row_maxlen : nullptr;
       L.ncols = R.ncols;
+      L.pos_len = R.pos_len;
+      L.tiles = tiles;
       if ((rc = learned_caps(ctx, opt.mlp, L, scale, R.cap))) return rc;
     } else {
       CapParams P{};
