@@ -1443,7 +1443,8 @@ __device__ bool pt_group_bucket(unsigned long long* tk, uint32_t* tc, uint32_t c
                                 const KtPay<RestT, ValT>* __restrict__ ppay, KtSlice* __restrict__ tmps,
                                 uint32_t* __restrict__ nkeys_out, uint32_t* __restrict__ perm,
                                 RestT* __restrict__ rest_out, ValT* __restrict__ gval, const PtIndex& X,
-                                unsigned long long* s_carry, unsigned long long* s_wave, uint32_t* s_ndist) {
+                                unsigned long long* s_carry, unsigned long long* s_wave, uint32_t* s_ndist,
+                                unsigned long long* stage8 = nullptr, uint32_t* stage4 = nullptr) {
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
   for (uint32_t s = threadIdx.x; s < cap; s += blockDim.x) {
     pt_st64(&tk[s], PTC_EMPTY_KEY);
@@ -1593,6 +1594,58 @@ __device__ bool pt_group_bucket(unsigned long long* tk, uint32_t* tc, uint32_t c
       }
     }
   }
+  if (stage8 && kept && n <= PT_TAB) {
+    // place through LDS: every nonzero's value and id go to its key's
+    // cursor in LDS (the key table's words, free once the slices are out,
+    // and the region map's), then out in order with coalesced stores; the
+    // non-key tuples the same way in a second round.  Scattered stores of
+    // 2-8 bytes to the bucket's range cost one cache-line access per lane.
+    static_assert(PT_TAB * sizeof(unsigned long long) >= PT_TAB * sizeof(RestT) &&
+                      PT_TAB * sizeof(unsigned long long) >= PT_TAB * sizeof(ValT),
+                  "a bucket's values and tuples fit the staging words");
+    ValT* sv = reinterpret_cast<ValT*>(stage8);
+    RestT* sr = reinterpret_cast<RestT*>(stage8);
+    uint32_t lp[U];
+    RestT rk[U];
+    {
+      uint32_t id[U];
+      KtPay<RestT, ValT> pl[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const uint32_t i = threadIdx.x + u * blockDim.x;
+        const uint64_t q = bs + (i < n ? i : 0u);
+        id[u] = pid[q];
+        pl[u] = ppay[q];
+      }
+      __syncthreads();  // the region write-out has read the map
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        lp[u] = 0xFFFFFFFFu;
+        if (threadIdx.x + u * blockDim.x >= n) continue;
+        lp[u] = pt_add32(&tc[slot[u]], 1u);
+        sv[lp[u]] = pl[u].v;
+        stage4[lp[u]] = id[u];
+        rk[u] = pl[u].k;
+      }
+    }
+    __syncthreads();
+    const uint64_t base = bs - obase;
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+      perm[base + i] = stage4[i];
+      gval[base + i] = sv[i];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (lp[u] != 0xFFFFFFFFu) sr[lp[u]] = rk[u];
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+      const RestT k = sr[i];
+      rest_out[base + i] = k;
+      if (X.rest16) X.rest16[base + i] = (uint16_t)k;
+    }
+    return true;
+  }
   // place: every nonzero at its key's cursor (loads of four at a time in
   // flight)
   constexpr int UP = U;
@@ -1663,7 +1716,7 @@ __device__ __forceinline__ void pt_group_one(uint32_t b, const unsigned long lon
   X.rest16 = rest16;
   // the LDS table while the bucket's distinct keys fit, else the region
   if (!pt_group_bucket<false>(L.lk, L.lc, PT_TAB, bs, n, obase, pkey, pid, ppay, tmps, nkeys_out, perm, rest_out,
-                              gval, X, &L.s_carry, L.s_wave, &L.s_ndist)) {
+                              gval, X, &L.s_carry, L.s_wave, &L.s_ndist, L.lk, L.lmap)) {
     __syncthreads();
     pt_group_bucket<true>(gkey + bs, gcnt + bs, n, bs, n, obase, pkey, pid, ppay, tmps, nkeys_out, perm, rest_out,
                           gval, X, &L.s_carry, L.s_wave, &L.s_ndist);
