@@ -65,6 +65,8 @@ struct ItemParams {
 struct ItemP0Params {
   uint64_t nitems;                   // work items (a bound when live is set)
   const unsigned long long* live;    // non-NULL: the item count on the device (low 32 bits)
+  int find_rows;                     // write whole items, each row found in packed (k_items did not run)
+  uint64_t nrows;                    // rows of packed (find_rows)
   uint64_t npos;
   const uint32_t* row_of_pos;
   const uint32_t* row_nchunk;
@@ -206,6 +208,13 @@ struct EmitParams {
 
 int launch_probe(ptc_ctx* ctx, const ProbeParams& P, bool rest64);
 int launch_rows(ptc_ctx* ctx, const RowParams& P);
+// k_rows, the scans of packed and slabs, k_items and (cap non-NULL) the
+// upper-bound capacities in one workgroup: for nrows + 1 <= ROWS_SMALL_MAX
+constexpr int ROWS_SMALL_MAX_K = 8;
+constexpr uint64_t ROWS_SMALL_MAX = 1024ull * ROWS_SMALL_MAX_K;
+int launch_rows_small(ptc_ctx* ctx, const RowParams& P, const ItemParams& Q, uint32_t* cap, unsigned long long ncols);
+// k_item_starts alone (the items written by launch_rows_small)
+int launch_item_starts(ptc_ctx* ctx, const ItemP0Params& Q);
 int launch_items(ptc_ctx* ctx, const ItemParams& P, const ItemP0Params& Q);
 int launch_hash_caps(ptc_ctx* ctx, const HashCapParams& P);
 int launch_caps(ptc_ctx* ctx, const CapParams& P);

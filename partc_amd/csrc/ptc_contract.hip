@@ -161,6 +161,109 @@ __global__ void k_rows(RowParams P) {
   }
 }
 
+// Rows, their two scans, their split-row list and (upper bound) their
+// capacities in one workgroup, for at most ROWS_SMALL_THREADS * K rows: what
+// k_rows, the in-place scans of packed and slabs (totals at [nrows]),
+// k_items (but the items) and k_caps write, in one launch instead of four.  Thread t takes
+// the K consecutive rows from t * K, all their loads in flight together.
+constexpr int ROWS_SMALL_THREADS = 1024;
+template <int K>
+__global__ void __launch_bounds__(ROWS_SMALL_THREADS) k_rows_small(RowParams P, ItemParams Q, uint32_t* cap,
+                                                                   unsigned long long ncols) {
+  __shared__ unsigned long long s_w[2][ROWS_SMALL_THREADS / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const uint64_t r0 = (uint64_t)threadIdx.x * K;
+  // every load unconditional (a load under a condition is waited for
+  // before the branch joins): the slices of rows clamped to the last one,
+  // read before the row count arrives (rows < nrows are allocated); rows
+  // past the count get the empty slice, whose flops apre[0] - apre[0] are 0
+  KtSlice sl[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) sl[k] = P.rows[min<uint64_t>(r0 + k, P.nrows - 1)];
+  const uint64_t live = P.live ? (*P.live >> 32) : P.nrows;
+  unsigned long long chunk = P.chunk;
+  if (P.total) chunk = max(chunk, (*P.total + P.max_items - 1) / P.max_items);
+  chunk = min(chunk, MAX_ITEM_FLOPS);
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const uint64_t r = r0 + k;
+    if (r >= live || r >= P.nrows) {
+      sl[k].off = 0;
+      sl[k].len = 0;
+    }
+  }
+  unsigned long long f[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) f[k] = P.apre[(uint64_t)sl[k].off + sl[k].len] - P.apre[sl[k].off];
+  uint32_t nch[K];
+  unsigned long long pk_sum = 0, sb_sum = 0;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const uint64_t r = r0 + k;
+    nch[k] = f[k] ? (uint32_t)((f[k] + chunk - 1) / chunk) : 0u;
+    if (r < P.nrows) {
+      P.row_nchunk[r] = nch[k];
+      P.row_flops[r] = f[k];
+      if (P.row_hits && r >= live) {
+        P.row_hits[r] = 0;
+        P.row_maxlen[r] = 0;
+      }
+      if (cap) cap[r] = (uint32_t)min(min(f[k], ncols), 0xFFFFFFFFull);
+    }
+    pk_sum += ((unsigned long long)(nch[k] > 1) << 32) | nch[k];
+    sb_sum += nch[k] > 1 ? nch[k] : 0ull;
+  }
+  // exclusive scans of (heavy << 32 | chunks) and of the split rows' slabs
+  unsigned long long x = pk_sum, y = sb_sum;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const unsigned long long a = __shfl_up(x, o, 64), b = __shfl_up(y, o, 64);
+    if (lane >= o) {
+      x += a;
+      y += b;
+    }
+  }
+  if (lane == 63) {
+    s_w[0][wid] = x;
+    s_w[1][wid] = y;
+  }
+  __syncthreads();
+  unsigned long long pk = x - pk_sum, sb = y - sb_sum;
+  for (int w = 0; w < wid; ++w) {
+    pk += s_w[0][w];
+    sb += s_w[1][w];
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const uint64_t r = r0 + k;
+    if (r < P.nrows) {
+      P.packed[r] = pk;
+      P.slabs[r] = sb;
+      // the split-row list (k_items); the items themselves are written by
+      // k_item_starts, which finds each item's row in the scanned counts
+      const uint32_t hv = (uint32_t)(pk >> 32);
+      Q.row_heavy[r] = nch[k] > 1 ? hv : NO_HEAVY;
+      if (nch[k] > 1) Q.heavy_rows[hv] = (uint32_t)r;
+    } else if (r == P.nrows) {  // the totals after the last row
+      P.packed[r] = pk;
+      P.slabs[r] = sb;
+    }
+    pk += ((unsigned long long)(nch[k] > 1) << 32) | nch[k];
+    sb += nch[k] > 1 ? nch[k] : 0ull;
+  }
+}
+int launch_rows_small(ptc_ctx* ctx, const RowParams& P, const ItemParams& Q, uint32_t* cap, unsigned long long ncols) {
+  if (P.nrows + 1 <= (uint64_t)ROWS_SMALL_THREADS * 4)
+    hipLaunchKernelGGL(k_rows_small<4>, dim3(1), dim3(ROWS_SMALL_THREADS), 0, ctx->stream, P, Q, cap, ncols);
+  else if (P.nrows + 1 <= (uint64_t)ROWS_SMALL_THREADS * ROWS_SMALL_MAX_K)
+    hipLaunchKernelGGL(k_rows_small<ROWS_SMALL_MAX_K>, dim3(1), dim3(ROWS_SMALL_THREADS), 0, ctx->stream, P, Q, cap,
+                       ncols);
+  else
+    return set_err(ctx, PTC_ERR_INTERNAL, "%llu rows for the one-workgroup row pass", (unsigned long long)P.nrows);
+  PTC_HIP(ctx, hipGetLastError());
+  return PTC_OK;
+}
+
 // fills items: (row, chunk, first position, heavy index)
 __global__ void k_items(ItemParams P) {
   for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < P.nrows; r += (uint64_t)gridDim.x * blockDim.x) {
@@ -1107,8 +1210,31 @@ __global__ void __launch_bounds__(256) k_item_starts(ItemP0Params P) {
   const int lane = threadIdx.x & 63;
   const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x / 64);
   for (uint64_t i = (uint64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64; i < n; i += waves) {
-    const uint4 it = P.items[i];
-    if (it.y == 0) continue;  // a row's first chunk starts at its first position
+    uint4 it;
+    if (P.find_rows) {
+      // the item's row: the last row whose first item (the low word of its
+      // scanned count) is at or before i, found 64 rows at a time
+      uint64_t lo = 0, hi = P.nrows;  // (packed[lo] & 0xFFFFFFFF) <= i throughout
+      while (hi - lo > 64) {
+        const uint64_t step = (hi - lo + 63) / 64;
+        const uint64_t q = lo + (uint64_t)lane * step;
+        const unsigned long long m = __ballot(q < hi && (P.packed[q] & 0xFFFFFFFFull) <= i);
+        const uint64_t nlo = lo + (uint64_t)(63 - __clzll(m)) * step;
+        hi = min(hi, nlo + step);
+        lo = nlo;
+      }
+      const uint64_t q = lo + lane;
+      const unsigned long long m = __ballot(q < hi && (P.packed[q] & 0xFFFFFFFFull) <= i);
+      const uint32_t r = (uint32_t)(lo + (uint64_t)(63 - __clzll(m)));
+      it = make_uint4(r, (uint32_t)(i - (P.packed[r] & 0xFFFFFFFFull)), P.rows[r].off, 0u);
+      if (it.y == 0) {
+        if (lane == 0) P.items[i] = it;
+        continue;
+      }
+    } else {
+      it = P.items[i];
+      if (it.y == 0) continue;  // a row's first chunk starts at its first position
+    }
     const uint32_t r = it.x, nch = P.row_nchunk[r];
     const KtSlice row = P.rows[r];
     const unsigned long long flo = P.apre[row.off], f = P.row_flops[r];
@@ -1124,8 +1250,19 @@ __global__ void __launch_bounds__(256) k_item_starts(ItemP0Params P) {
     }
     const uint64_t q = lo + lane;
     const unsigned long long m = __ballot(q < hi && P.apre[q] <= target);
-    if (lane == 0) P.items[i].z = (uint32_t)(lo + (uint64_t)(63 - __clzll(m)));
+    it.z = (uint32_t)(lo + (uint64_t)(63 - __clzll(m)));
+    if (lane == 0) {
+      if (P.find_rows) P.items[i] = it;
+      else P.items[i].z = it.z;
+    }
   }
+}
+
+int launch_item_starts(ptc_ctx* ctx, const ItemP0Params& Q) {
+  if (Q.npos && Q.nitems)
+    hipLaunchKernelGGL(k_item_starts, dim3(gridn(Q.nitems, 4, 4096)), dim3(256), 0, ctx->stream, Q);
+  PTC_HIP(ctx, hipGetLastError());
+  return PTC_OK;
 }
 
 int launch_items(ptc_ctx* ctx, const ItemParams& P, const ItemP0Params& Q) {

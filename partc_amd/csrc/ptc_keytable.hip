@@ -1469,6 +1469,9 @@ __device__ bool pt_group_bucket(unsigned long long* tk, uint32_t* tc, uint32_t c
   // placing pass (no second read of its key, no second probe)
   constexpr int U = 4;
   const bool kept = n <= U * blockDim.x;
+  // a bucket of at most PT_LDS_KEYS nonzeros cannot fill the LDS table: its
+  // inserts skip the distinct-key count (an LDS atomic on one word per key)
+  const bool track = n > PT_LDS_KEYS;
   uint32_t slot[U];
   for (uint32_t i0 = threadIdx.x; i0 < n; i0 += U * blockDim.x) {
     unsigned long long k[U];
@@ -1481,14 +1484,14 @@ __device__ bool pt_group_bucket(unsigned long long* tk, uint32_t* tc, uint32_t c
     for (int u = 0; u < U; ++u) {
       slot[u] = PT_FULL;
       if (i0 + u * blockDim.x < n) {
-        const uint32_t s = pt_slot<GLOBAL>(tk, cap, k[u], GLOBAL ? nullptr : s_ndist);
+        const uint32_t s = pt_slot<GLOBAL>(tk, cap, k[u], GLOBAL || !track ? nullptr : s_ndist);
         if (s != PT_FULL) pt_add32(&tc[s], 1u);
         slot[u] = s;
       }
     }
   }
   __syncthreads();
-  if (!GLOBAL && pt_ld32(s_ndist) >= PT_LDS_KEYS) return false;  // the same value in every thread
+  if (!GLOBAL && track && pt_ld32(s_ndist) >= PT_LDS_KEYS) return false;  // the same value in every thread
   // counts -> offsets in slot order; (slices << 32 | nonzeros) scanned
   // together, SPT consecutive slots per thread, a chunk of SPT * blockDim
   // slots at a time (4 x 1024 covers the LDS table in one chunk)

@@ -91,6 +91,8 @@ int prepare_finish(Run& R, const unsigned long long* hostv);
 // Work items and split-row list into arrays of the given sizes (npos = 0:
 // no chunk starts).
 int build_items(Run& R, uint64_t nitems, uint64_t nheavy, uint64_t npos);
+// The arrays of build_items and the parameters of its two launches.
+int item_params(Run& R, uint64_t nitems, uint64_t nheavy, uint64_t npos, ItemParams& P, ItemP0Params& Q);
 int prepare(Run& R, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B, const ptc_dtensor* A,
             const ptc_dtensor* B, const ptc_options& opt);
 

@@ -488,16 +488,29 @@ int prepare_launch(Run& R, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B,
     P.slabs = R.row_slab;
     P.row_hits = parts ? R.row_hits : nullptr;
     P.row_maxlen = parts ? R.row_maxlen : nullptr;
-    if ((rc = launch_rows(ctx, P))) return rc;
+    // Dense rows with the default work-item rule have at most rows +
+    // DENSE_MAX_ITEMS items (each row's chunk count is at most its flops /
+    // chunk + 1 and the chunk is at least total / DENSE_MAX_ITEMS): the items
+    // are built now, in arrays of that bound, so the GPU has them to do while
+    // the host waits for the totals.  A few thousand rows take one
+    // workgroup for their flops, scans, items and upper-bound capacities.
+    R.early_items = (R.dense || R.win_n) && !opt.chunk_flops;
+    if (R.early_items && R.nrows + 1 <= ROWS_SMALL_MAX) {
+      ItemParams Q;
+      ItemP0Params Q0;
+      if ((rc = item_params(R, R.nrows + DENSE_MAX_ITEMS, R.nrows, R.npos, Q, Q0))) return rc;
+      if (opt.method == PTC_METHOD_UPPER_BOUND) {
+        R.cap = ws.get<uint32_t>(R.nrows + 1);
+        if (!R.cap) return PTC_ERR_NOMEM;
+      }
+      Q0.find_rows = 1;
+      Q0.nrows = R.nrows;
+      if ((rc = launch_rows_small(ctx, P, Q, R.cap, R.ncols)) || (rc = launch_item_starts(ctx, Q0))) return rc;
+    } else {
+      if ((rc = launch_rows(ctx, P)) || (rc = scan_u64_two_inplace(ctx, R.packed, R.row_slab, R.nrows))) return rc;
+      if (R.early_items && (rc = build_items(R, R.nrows + DENSE_MAX_ITEMS, R.nrows, R.npos))) return rc;
+    }
   }
-  if ((rc = scan_u64_two_inplace(ctx, R.packed, R.row_slab, R.nrows))) return rc;
-  // Dense rows with the default work-item rule have at most rows +
-  // DENSE_MAX_ITEMS items (each row's chunk count is at most its flops /
-  // chunk + 1 and the chunk is at least total / DENSE_MAX_ITEMS): the items
-  // are built now, in arrays of that bound, so the GPU has them to do while
-  // the host waits for the totals.
-  R.early_items = (R.dense || R.win_n) && !opt.chunk_flops;
-  if (R.early_items && (rc = build_items(R, R.nrows + DENSE_MAX_ITEMS, R.nrows, R.npos))) return rc;
   // A single pass over dense rows whose output and split-row slabs have
   // host-known bounds within AHEAD_BYTES runs ahead of these totals (one
   // round trip for the whole contraction); its capacities then bound rows,
@@ -515,7 +528,8 @@ int prepare_launch(Run& R, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B,
   // clamped prediction in one launch.
   const bool single = opt.method != PTC_METHOD_TWO_PASS;
   if (single) {
-    R.cap = ws.get<uint32_t>(R.nrows + 1);
+    const bool caps_done = R.cap != nullptr;  // written with the rows (upper bound, few rows)
+    if (!R.cap) R.cap = ws.get<uint32_t>(R.nrows + 1);
     R.cap_off = ws.get<unsigned long long>(R.nrows + 1);
     if (!R.cap || !R.cap_off) return PTC_ERR_NOMEM;
     const double scale = opt.predict_scale > 0 ? opt.predict_scale : 1.15;
@@ -539,7 +553,7 @@ int prepare_launch(Run& R, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B,
       P.pred = nullptr;
       P.scale = scale;
       P.cap = R.cap;
-      if ((rc = launch_caps(ctx, P))) return rc;
+      if (!caps_done && (rc = launch_caps(ctx, P))) return rc;
     }
     if (!R.ahead && (rc = scan_u32_to_u64(ctx, R.cap, R.cap_off, R.nrows, R.cap_off + R.nrows))) return rc;
   }
@@ -591,13 +605,12 @@ int prepare_finish(Run& R, const unsigned long long* hostv) {
 
 // Work items (row, chunk, first position) and the split rows' list into
 // arrays of nitems and nheavy entries; npos = 0 skips the chunk starts.
-int build_items(Run& R, uint64_t nitems, uint64_t nheavy, uint64_t npos) {
-  ptc_ctx* ctx = R.ctx;
+int item_params(Run& R, uint64_t nitems, uint64_t nheavy, uint64_t npos, ItemParams& P, ItemP0Params& Q) {
   const ptc_hashtable* HT_A = R.HA;
   R.items = R.ws.get<uint4>(std::max<uint64_t>(nitems, 1));
   R.heavy_rows = R.ws.get<uint32_t>(std::max<uint64_t>(nheavy, 1));
   if (!R.items || !R.heavy_rows) return PTC_ERR_NOMEM;
-  ItemParams P{};
+  P = ItemParams{};
   P.nrows = R.nrows;
   P.rows = HT_A->slices;
   P.row_nchunk = R.row_nchunk;
@@ -605,7 +618,7 @@ int build_items(Run& R, uint64_t nitems, uint64_t nheavy, uint64_t npos) {
   P.row_heavy = R.row_heavy;
   P.heavy_rows = R.heavy_rows;
   P.items = R.items;
-  ItemP0Params Q{};
+  Q = ItemP0Params{};
   Q.nitems = nitems;
   Q.live = R.early_items ? R.packed + R.nrows : nullptr;
   Q.npos = npos;
@@ -617,7 +630,14 @@ int build_items(Run& R, uint64_t nitems, uint64_t nheavy, uint64_t npos) {
   Q.row_flops = R.row_flops;
   Q.packed = R.packed;
   Q.items = R.items;
-  return launch_items(ctx, P, Q);
+  return PTC_OK;
+}
+
+int build_items(Run& R, uint64_t nitems, uint64_t nheavy, uint64_t npos) {
+  ItemParams P;
+  ItemP0Params Q;
+  const int rc = item_params(R, nitems, nheavy, npos, P, Q);
+  return rc ? rc : launch_items(R.ctx, P, Q);
 }
 
 int prepare(Run& R, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B, const ptc_dtensor* A,
