@@ -64,8 +64,11 @@ def streamed_bytes(npos, F, nnz_c, d_o):
     return npos * 20 + F * (2 + 8) + nnz_c * (4 * d_o + 8)
 
 
-def run_method(P, ctx, dA, dB, method, steps, warmup, value_type=0):
-    """Times `steps` whole contractions; returns per-step ms and stats."""
+def run_method(P, ctx, dA, dB, method, steps, warmup, value_type=0, stat_steps=3):
+    """Times `steps` whole contractions with phase timing off (as a caller
+    runs them: no HIP events between phases); then `stat_steps` more with
+    phase timing on give the per-phase statistics and the numeric kernel's
+    event-bracketed duration.  Returns per-step ms and stats."""
     import ctypes as C
 
     L = P.lib()
@@ -85,20 +88,26 @@ def run_method(P, ctx, dA, dB, method, steps, warmup, value_type=0):
         if rc != 0:
             raise P.PtcError(rc, ctx.last_error())
 
+    ctx.set_timing(False)
     for _ in range(warmup):
         one()
     ctx.synchronize()
-    kern_ms, kern_launch = 0.0, 0
     t0 = time.perf_counter()
     per = []
     for _ in range(steps):
         s0 = time.perf_counter()
         one()
         per.append(time.perf_counter() - s0)
-        kern_ms += st.ms_kernel_numeric
-        kern_launch += st.kernel_launches_numeric
     ctx.synchronize()
     total = time.perf_counter() - t0
+    # phase statistics and kernel durations, outside the timed steps
+    ctx.set_timing(True)
+    kern_ms, kern_launch = 0.0, 0
+    for _ in range(max(1, stat_steps)):
+        one()
+        kern_ms += st.ms_kernel_numeric
+        kern_launch += st.kernel_launches_numeric
+    ctx.set_timing(False)
     return {
         "ms_per_step": 1e3 * total / steps,
         "ms_min": 1e3 * min(per),
@@ -601,12 +610,14 @@ def chain_bench(P, args):
         for _ in range(args.warmup):
             step(keyed)
         ctx.synchronize()
-        tab = []
         t0 = time.perf_counter()
         for _ in range(args.steps):
-            tab.append(step(keyed))
+            step(keyed)
         ctx.synchronize()
         ms = 1e3 * (time.perf_counter() - t0) / args.steps
+        ctx.set_timing(True)  # the O table's device time, outside the timed steps
+        tab = [step(keyed) for _ in range(2)]
+        ctx.set_timing(False)
         res[name] = {"ms_per_step": round(ms, 4), "ms_table_O": round(sum(tab) / len(tab), 4)}
         print(f"# {name}: {ms:.3f} ms/step, O table {res[name]['ms_table_O']:.3f} ms", file=sys.stderr, flush=True)
     F = int(st1.flops) + int(st2.flops)

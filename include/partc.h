@@ -89,6 +89,15 @@ const char* ptc_last_error(const ptc_ctx* ctx);
 int ptc_synchronize(ptc_ctx* ctx);
 /* Raw hipStream_t of the context, for callers that order their own work. */
 void* ptc_stream(ptc_ctx* ctx);
+/* Phase timing on (1) or off (0, the default).  On, key-table builds and
+ * contractions record HIP events between their phases and fill the ms_*
+ * fields of ptc_stats and ms_build of key tables; off, those read 0.  Each
+ * event leaves the GPU idle for several microseconds between two kernels
+ * (on MI355X a nips-shape contraction with its two key tables runs 0.72 ms
+ * untimed against 0.77 ms timed), so timing is for profiling runs.  Batch
+ * plans always time their phases (the autotuner needs them).  No reference
+ * counterpart (main.cpp times its phases with host clocks). */
+int ptc_set_timing(ptc_ctx* ctx, int on);
 
 /* Copy a host tensor to the device (the reference keeps every tensor in host
  * malloc memory; main.cpp:59-116).  Per-mode maxima are computed here.  A

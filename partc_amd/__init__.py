@@ -188,6 +188,7 @@ def lib() -> C.CDLL:
         "ptc_destroy": (None, [vp]),
         "ptc_last_error": (C.c_char_p, [vp]),
         "ptc_synchronize": (i32, [vp]),
+        "ptc_set_timing": (i32, [vp, i32]),
         "ptc_stream": (vp, [vp]),
         "ptc_dtensor_upload": (i32, [vp, C.POINTER(TensorView), C.POINTER(vp)]),
         "ptc_dtensor_upload_f32": (i32, [vp, C.POINTER(TensorView), C.POINTER(C.c_float), C.POINTER(vp)]),
@@ -413,6 +414,12 @@ class Context:
 
     def synchronize(self) -> None:
         _check(lib().ptc_synchronize(self._h), self)
+
+    def set_timing(self, on: bool) -> None:
+        """Phase timing by HIP events (ptc_set_timing): the ms_* statistics
+        of builds and contractions; off by default, since every event idles
+        the GPU for a few microseconds between kernels."""
+        _check(lib().ptc_set_timing(self._h, 1 if on else 0), self)
 
     def stream(self) -> int:
         return int(lib().ptc_stream(self._h) or 0)

@@ -208,11 +208,13 @@ struct EmitParams {
 
 int launch_probe(ptc_ctx* ctx, const ProbeParams& P, bool rest64);
 int launch_rows(ptc_ctx* ctx, const RowParams& P);
-// k_rows, the scans of packed and slabs, k_items and (cap non-NULL) the
-// upper-bound capacities in one workgroup: for nrows + 1 <= ROWS_SMALL_MAX
+// k_rows, the scans of packed and slabs, the split-row list of k_items,
+// (cap non-NULL) the upper-bound capacities and the clear of nzero words at
+// zero, in one workgroup: for nrows + 1 <= ROWS_SMALL_MAX
 constexpr int ROWS_SMALL_MAX_K = 8;
 constexpr uint64_t ROWS_SMALL_MAX = 1024ull * ROWS_SMALL_MAX_K;
-int launch_rows_small(ptc_ctx* ctx, const RowParams& P, const ItemParams& Q, uint32_t* cap, unsigned long long ncols);
+int launch_rows_small(ptc_ctx* ctx, const RowParams& P, const ItemParams& Q, uint32_t* cap, unsigned long long ncols,
+                      uint32_t* zero, uint32_t nzero);
 // k_item_starts alone (the items written by launch_rows_small)
 int launch_item_starts(ptc_ctx* ctx, const ItemP0Params& Q);
 int launch_items(ptc_ctx* ctx, const ItemParams& P, const ItemP0Params& Q);

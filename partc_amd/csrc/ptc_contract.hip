@@ -161,15 +161,17 @@ __global__ void k_rows(RowParams P) {
   }
 }
 
-// Rows, their two scans, their split-row list and (upper bound) their
-// capacities in one workgroup, for at most ROWS_SMALL_THREADS * K rows: what
+// Rows, their two scans, their split-row list, (upper bound) their
+// capacities and (zero non-NULL) the clear of nzero words in one workgroup, for at most ROWS_SMALL_THREADS * K rows: what
 // k_rows, the in-place scans of packed and slabs (totals at [nrows]),
 // k_items (but the items) and k_caps write, in one launch instead of four.  Thread t takes
 // the K consecutive rows from t * K, all their loads in flight together.
 constexpr int ROWS_SMALL_THREADS = 1024;
 template <int K>
 __global__ void __launch_bounds__(ROWS_SMALL_THREADS) k_rows_small(RowParams P, ItemParams Q, uint32_t* cap,
-                                                                   unsigned long long ncols) {
+                                                                   unsigned long long ncols, uint32_t* zero,
+                                                                   uint32_t nzero) {
+  for (uint32_t i = threadIdx.x; i < nzero; i += ROWS_SMALL_THREADS) zero[i] = 0;
   __shared__ unsigned long long s_w[2][ROWS_SMALL_THREADS / 64];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const uint64_t r0 = (uint64_t)threadIdx.x * K;
@@ -252,12 +254,14 @@ __global__ void __launch_bounds__(ROWS_SMALL_THREADS) k_rows_small(RowParams P, 
     sb += nch[k] > 1 ? nch[k] : 0ull;
   }
 }
-int launch_rows_small(ptc_ctx* ctx, const RowParams& P, const ItemParams& Q, uint32_t* cap, unsigned long long ncols) {
+int launch_rows_small(ptc_ctx* ctx, const RowParams& P, const ItemParams& Q, uint32_t* cap, unsigned long long ncols,
+                      uint32_t* zero, uint32_t nzero) {
   if (P.nrows + 1 <= (uint64_t)ROWS_SMALL_THREADS * 4)
-    hipLaunchKernelGGL(k_rows_small<4>, dim3(1), dim3(ROWS_SMALL_THREADS), 0, ctx->stream, P, Q, cap, ncols);
+    hipLaunchKernelGGL(k_rows_small<4>, dim3(1), dim3(ROWS_SMALL_THREADS), 0, ctx->stream, P, Q, cap, ncols, zero,
+                       nzero);
   else if (P.nrows + 1 <= (uint64_t)ROWS_SMALL_THREADS * ROWS_SMALL_MAX_K)
     hipLaunchKernelGGL(k_rows_small<ROWS_SMALL_MAX_K>, dim3(1), dim3(ROWS_SMALL_THREADS), 0, ctx->stream, P, Q, cap,
-                       ncols);
+                       ncols, zero, nzero);
   else
     return set_err(ctx, PTC_ERR_INTERNAL, "%llu rows for the one-workgroup row pass", (unsigned long long)P.nrows);
   PTC_HIP(ctx, hipGetLastError());

@@ -89,6 +89,9 @@ struct ptc_ctx {
   size_t live_bytes = 0;     // bytes held by live blocks (cache excluded)
   size_t peak_bytes = 0;     // high-water mark of live_bytes since a call reset it
   hipEvent_t ev[8] = {};
+  // phase timing (ptc_set_timing): HIP events between the phases of key
+  // table builds and contractions, off by default
+  int timing = 0;
   hipEvent_t kev[16] = {};   // kernel-level timing events
   int nkev = 0;
   int kev_pass[8] = {};
@@ -161,7 +164,7 @@ struct ptc_hashtable {
   float* gval32 = nullptr;     // device, N: values in grouped order, FP32 tensors (gval is then NULL)
   bool shares_vals = false;    // gval/gval32 are the tensor's own values (grouped order = tensor order)
   hipEvent_t ev_start = nullptr, ev_end = nullptr;  // bracket the build on the stream
-  double ms_build = -1.0;      // from the events, on first request (hashtable_ms_build)
+  double ms_build = -1.0;      // from the events, on first request (hashtable_ms_build; 0 without timing)
 };
 
 #endif  // __HIPCC__

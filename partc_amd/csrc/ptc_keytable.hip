@@ -2273,11 +2273,13 @@ extern "C" int ptc_hashtable_build_ex(ptc_ctx* ctx, const ptc_dtensor* t, int gr
   ht->ctx = ctx;
   ctx_acquire(ctx);
   ht->t = t;
-  if (hipEventCreate(&ht->ev_start) != hipSuccess || hipEventCreate(&ht->ev_end) != hipSuccess) {
-    ptc_hashtable_free(ht);
-    return hip_check(ctx, hipGetLastError(), "event");
+  if (ctx->timing) {
+    if (hipEventCreate(&ht->ev_start) != hipSuccess || hipEventCreate(&ht->ev_end) != hipSuccess) {
+      ptc_hashtable_free(ht);
+      return hip_check(ctx, hipGetLastError(), "event");
+    }
+    (void)hipEventRecord(ht->ev_start, ctx->stream);
   }
-  (void)hipEventRecord(ht->ev_start, ctx->stream);
   int rc = make_codec(ctx, t, t->dim_tc, t->nk, &ht->codec);
   if (!rc) rc = make_codec(ctx, t, t->dim_tc + t->nk, t->d - t->nk, &ht->rest);
   if (rc) {
@@ -2328,7 +2330,7 @@ extern "C" int ptc_hashtable_build_ex(ptc_ctx* ctx, const ptc_dtensor* t, int gr
       ptc_hashtable_free(ht);
       return rc;
     }
-    (void)hipEventRecord(ht->ev_end, ctx->stream);
+    if (ht->ev_end) (void)hipEventRecord(ht->ev_end, ctx->stream);
     *out = ht;
     return PTC_OK;
   }
@@ -2342,7 +2344,7 @@ extern "C" int ptc_hashtable_build_ex(ptc_ctx* ctx, const ptc_dtensor* t, int gr
       ptc_hashtable_free(ht);
       return rc;
     }
-    (void)hipEventRecord(ht->ev_end, ctx->stream);
+    if (ht->ev_end) (void)hipEventRecord(ht->ev_end, ctx->stream);
     *out = ht;
     return PTC_OK;
   }
@@ -2363,7 +2365,7 @@ extern "C" int ptc_hashtable_build_ex(ptc_ctx* ctx, const ptc_dtensor* t, int gr
       ptc_hashtable_free(ht);
       return rc;
     }
-    (void)hipEventRecord(ht->ev_end, ctx->stream);
+    if (ht->ev_end) (void)hipEventRecord(ht->ev_end, ctx->stream);
     *out = ht;
     return PTC_OK;
   }
@@ -2494,7 +2496,7 @@ extern "C" int ptc_hashtable_build_ex(ptc_ctx* ctx, const ptc_dtensor* t, int gr
     ptc_hashtable_free(ht);
     return hip_check(ctx, e, "hashtable build");
   }
-  (void)hipEventRecord(ht->ev_end, st);
+  if (ht->ev_end) (void)hipEventRecord(ht->ev_end, st);
   *out = ht;
   return PTC_OK;
 }
@@ -2730,7 +2732,7 @@ double hashtable_ms_build(const ptc_hashtable* hc) {
     else
       (void)hipGetLastError();
   }
-  return h->ms_build;
+  return h->ms_build < 0 ? 0.0 : h->ms_build;
 }
 
 // ------------------------------------------------- tables from output rows
@@ -2800,12 +2802,14 @@ int hashtable_from_rows(ptc_ctx* ctx, const ptc_dtensor* O, const KeyCodec& rowc
   ctx_acquire(ctx);
   ht->t = O;
   ht->shares_vals = true;
-  if (hipEventCreate(&ht->ev_start) != hipSuccess || hipEventCreate(&ht->ev_end) != hipSuccess) {
-    ptc_hashtable_free(ht);
-    return hip_check(ctx, hipGetLastError(), "event");
-  }
   hipStream_t st = ctx->stream;
-  (void)hipEventRecord(ht->ev_start, st);
+  if (ctx->timing) {
+    if (hipEventCreate(&ht->ev_start) != hipSuccess || hipEventCreate(&ht->ev_end) != hipSuccess) {
+      ptc_hashtable_free(ht);
+      return hip_check(ctx, hipGetLastError(), "event");
+    }
+    (void)hipEventRecord(ht->ev_start, st);
+  }
   int rc = make_codec(ctx, O, O->dim_tc, O->nk, &ht->codec);
   if (!rc) rc = make_codec(ctx, O, O->dim_tc + O->nk, O->d - O->nk, &ht->rest);
   if (rc) {
@@ -2854,7 +2858,7 @@ int hashtable_from_rows(ptc_ctx* ctx, const ptc_dtensor* O, const KeyCodec& rowc
   else
     hipLaunchKernelGGL((k_rest_keys<uint32_t>), dim3(gn), dim3(256), 0, st, (const uint32_t*)O->hedges, O->d, N,
                        ht->rest, ht->rest_key32);
-  (void)hipEventRecord(ht->ev_end, st);
+  if (ht->ev_end) (void)hipEventRecord(ht->ev_end, st);
   if (hipGetLastError() != hipSuccess) {
     ptc_hashtable_free(ht);
     return set_err(ctx, PTC_ERR_HIP, "row table kernels");

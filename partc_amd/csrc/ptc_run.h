@@ -58,6 +58,9 @@ struct Run {
   bool early_items = false;    // work items built before the preprocessing read-back (bounded count)
   bool allow_ahead = false;    // the caller lets a single pass run ahead of the preprocessing totals
   bool ahead = false;          // ... and this one does
+  // the single pass's row counts and reservation counter (cnt_words + 4
+  // words), already cleared on the device by the preprocessing (or NULL)
+  uint32_t* cnt_block = nullptr;
   const uint32_t* row_of_pos = nullptr;  // HT_A's slice index of every A position
   // single-pass methods: every row's output capacity, their offsets and sum
   // (computed with the preprocessing, the sum read back with its totals)
@@ -91,6 +94,8 @@ int prepare_finish(Run& R, const unsigned long long* hostv);
 // Work items and split-row list into arrays of the given sizes (npos = 0:
 // no chunk starts).
 int build_items(Run& R, uint64_t nitems, uint64_t nheavy, uint64_t npos);
+// Words of a single pass's row counts before its reservation counter.
+uint64_t count_words(uint64_t nrows);
 // The arrays of build_items and the parameters of its two launches.
 int item_params(Run& R, uint64_t nitems, uint64_t nheavy, uint64_t npos, ItemParams& P, ItemP0Params& Q);
 int prepare(Run& R, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B, const ptc_dtensor* A,

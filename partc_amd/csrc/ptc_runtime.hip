@@ -883,6 +883,12 @@ extern "C" int ptc_synchronize(ptc_ctx* ctx) {
 
 extern "C" void* ptc_stream(ptc_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
 
+extern "C" int ptc_set_timing(ptc_ctx* ctx, int on) {
+  if (!ctx) return PTC_ERR_ARG;
+  ctx->timing = on ? 1 : 0;
+  return PTC_OK;
+}
+
 static int check_perm(ptc_ctx* ctx, uint32_t d, uint32_t nk, const uint32_t* dim_tc) {
   if (d == 0 || d > PTC_MAX_MODES) return set_err(ctx, PTC_ERR_ARG, "d=%u not in [1,%d]", d, PTC_MAX_MODES);
   if (nk > d) return set_err(ctx, PTC_ERR_ARG, "num_key_dimensions %u > d %u", nk, d);

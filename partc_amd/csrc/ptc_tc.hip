@@ -37,7 +37,7 @@ struct Timer {
   // Timing only: a failing event call is ignored, and its error is cleared
   // so the next launch check does not report it.
   void mark() {
-    if (n < 8 && hipEventRecord(ev[n++], ctx->stream) != hipSuccess) (void)hipGetLastError();
+    if (ctx->timing && n < 8 && hipEventRecord(ev[n++], ctx->stream) != hipSuccess) (void)hipGetLastError();
   }
   double ms(int a, int b) {
     if (a >= n || b >= n) return 0.0;
@@ -204,7 +204,7 @@ static int run_pass(Run& R, int pass, HashPlan* hp, const uint4* items, uint64_t
                               pass == PASS_NUMERIC ? R.val : 0);
   // one pair of timing events brackets the row launch and its merge launch
   ptc_ctx* ctx = R.ctx;
-  const bool timed = ctx->nkev + 2 <= 16;
+  const bool timed = ctx->timing && ctx->nkev + 2 <= 16;
   if (timed && hipEventRecord(ctx->kev[ctx->nkev], ctx->stream) != hipSuccess) (void)hipGetLastError();
   int rc = launch_pass(ctx, pass, R.dense, R.col64, P, nitems, lds, false, R.threads);
   if (!rc && R.dense) {
@@ -503,9 +503,16 @@ int prepare_launch(Run& R, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B,
         R.cap = ws.get<uint32_t>(R.nrows + 1);
         if (!R.cap) return PTC_ERR_NOMEM;
       }
+      uint32_t nzero = 0;
+      if (opt.method != PTC_METHOD_TWO_PASS) {  // the single pass's counts, cleared here
+        nzero = (uint32_t)(count_words(R.nrows) + 4);
+        if (!(R.cnt_block = ws.get<uint32_t>(nzero))) return PTC_ERR_NOMEM;
+      }
       Q0.find_rows = 1;
       Q0.nrows = R.nrows;
-      if ((rc = launch_rows_small(ctx, P, Q, R.cap, R.ncols)) || (rc = launch_item_starts(ctx, Q0))) return rc;
+      if ((rc = launch_rows_small(ctx, P, Q, R.cap, R.ncols, R.cnt_block, nzero)) ||
+          (rc = launch_item_starts(ctx, Q0)))
+        return rc;
     } else {
       if ((rc = launch_rows(ctx, P)) || (rc = scan_u64_two_inplace(ctx, R.packed, R.row_slab, R.nrows))) return rc;
       if (R.early_items && (rc = build_items(R, R.nrows + DENSE_MAX_ITEMS, R.nrows, R.npos))) return rc;
@@ -605,6 +612,8 @@ int prepare_finish(Run& R, const unsigned long long* hostv) {
 
 // Work items (row, chunk, first position) and the split rows' list into
 // arrays of nitems and nheavy entries; npos = 0 skips the chunk starts.
+uint64_t count_words(uint64_t nrows) { return (nrows + 1 + 3) / 4 * 4; }  // 16-byte aligned end of nrows + 1 counts
+
 int item_params(Run& R, uint64_t nitems, uint64_t nheavy, uint64_t npos, ItemParams& P, ItemP0Params& Q) {
   const ptc_hashtable* HT_A = R.HA;
   R.items = R.ws.get<uint4>(std::max<uint64_t>(nitems, 1));
@@ -726,16 +735,18 @@ static int single_pass_direct(Run& R, const ptc_dtensor* A, const ptc_dtensor* B
     ptc_dtensor_free(out);
     return code;
   };
-  // the reservation counter rides behind the row counts: one clear for both
-  const uint64_t cnt_words = (R.nrows + 1 + 3) / 4 * 4;  // 16-byte aligned end of the row counts
-  uint32_t* row_nnz = ws.get<uint32_t>(cnt_words + 4);
+  // the reservation counter rides behind the row counts: one clear for
+  // both (done by the preprocessing when it allocated them)
+  const uint64_t cnt_words = count_words(R.nrows);
+  const bool cleared = R.cnt_block != nullptr;
+  uint32_t* row_nnz = cleared ? R.cnt_block : ws.get<uint32_t>(cnt_words + 4);
   unsigned long long* pair = ws.get<unsigned long long>(8);
   if (!row_nnz || !pair) return fail(PTC_ERR_NOMEM);
   R.row_nnz = row_nnz;
   unsigned long long* counter = reinterpret_cast<unsigned long long*>(row_nnz + cnt_words);
   HashPlan h1;
   if ((!R.dense && (rc = plan_hash(R, h1, cap, nullptr, R.row_flops))) ||
-      (rc = fill_zero(ctx, row_nnz, (cnt_words + 4) * 4)) || (rc = prepare_heavy(R, PASS_NUMERIC)) ||
+      (!cleared && (rc = fill_zero(ctx, row_nnz, (cnt_words + 4) * 4))) || (rc = prepare_heavy(R, PASS_NUMERIC)) ||
       (!R.dense && (rc = reset_hash_pool(R, h1, true))))
     return fail(rc);
   auto direct = [&](PassParams& P, const uint32_t* caps) {
@@ -932,7 +943,7 @@ static int run_share(Run& R, int pass, bool dense, uint32_t threads, uint64_t nc
   ptc_ctx* ctx = R.ctx;
   const size_t lds = pass_lds_bytes(pass, dense, false, ncols, lds_slots, R.use_flags, threads,
                                     pass == PASS_NUMERIC ? R.val : 0);
-  const bool timed = ctx->nkev + 2 <= 16;
+  const bool timed = ctx->timing && ctx->nkev + 2 <= 16;
   if (timed && hipEventRecord(ctx->kev[ctx->nkev], ctx->stream) != hipSuccess) (void)hipGetLastError();
   int rc = launch_pass(ctx, pass, dense, false, P, nitems, lds, false, threads);
   if (!rc && dense && nmerge) rc = launch_pass(ctx, pass, true, false, P, nmerge, lds, true, threads);
@@ -1430,7 +1441,7 @@ int tc_impl(ptc_ctx* ctx, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B, 
     if (kr->from_rows) *kr->from_rows = rows_key ? 1 : 0;
   }
   if (stats) {
-    if (hipEventSynchronize(T.ev[T.n - 1]) != hipSuccess) (void)hipGetLastError();
+    if (T.n && hipEventSynchronize(T.ev[T.n - 1]) != hipSuccess) (void)hipGetLastError();
     memset(stats, 0, sizeof(*stats));
     if (hto) stats->ms_build_out = hashtable_ms_build(hto);
     stats->nnz_a = A->N;

@@ -114,3 +114,30 @@ def test_zero_values_after_set_values(ctx, method):
     h, v = sort_coo(O.hedges_array, O.val_array)
     assert np.array_equal(h, oh)
     assert max_rel(v, ov) <= 1e-12
+
+
+def test_phase_timing_switch():
+    """ptc_set_timing: off (the default) the phase times read 0 and the
+    result is the one timing on gives, with the phase times filled."""
+    A, B = two_operands(31)
+    ctx = P.Context(0)
+    try:
+        dA, dB = P.DeviceTensor.upload(A, ctx), P.DeviceTensor.upload(B, ctx)
+        outs = []
+        for on in (False, True):
+            ctx.set_timing(on)
+            st = P.Stats()
+            HB, HA = P.buildHashTable(dB, ctx), P.buildHashTable(dA, ctx)
+            O = P.tc_fksCuckoo(HA, HB, stats=st, method=P.METHOD_UPPER_BOUND)
+            outs.append(sort_coo(O.hedges_array, O.val_array))
+            times = [st.ms_build_b, st.ms_build_a, st.ms_preprocess, st.ms_numeric, st.ms_kernel_numeric]
+            if on:
+                assert all(t > 0 for t in times), times
+            else:
+                assert all(t == 0 for t in times), times
+        # coordinates bit-exact; sums within the FP64 tolerance (atomic
+        # accumulation order differs between runs)
+        assert np.array_equal(outs[0][0], outs[1][0])
+        assert max_rel(outs[0][1], outs[1][1]) <= 1e-12
+    finally:
+        ctx.close()
