@@ -622,25 +622,45 @@ __device__ __forceinline__ void pass_item(const PP& P, uint32_t widx) {
         }
       }
     } else {
-      for (uint64_t c = threadIdx.x; c < ncols; c += NT) {
-        double sum = __longlong_as_double((long long)PTC_NEG_ZERO_BITS);
-        unsigned char fl = 0;
-        uint32_t kk = 0;
-        for (; kk + 4 <= nch; kk += 4) {  // four independent loads in flight
-          const double* q = P.hacc + (slab0 + kk) * ncols + c;
-          const double x0 = q[0], x1 = q[ncols], x2 = q[2 * ncols], x3 = q[3 * ncols];
-          sum += x0;
-          sum += x1;
-          sum += x2;
-          sum += x3;
+      // CQ columns a thread at a time, four chunks of each in flight
+      // together (sixteen loads); every column's sum in chunk order
+      constexpr int CQ = 4;
+      for (uint64_t c0 = threadIdx.x; c0 < ncols; c0 += CQ * NT) {
+        double sum[CQ];
+        unsigned char fl[CQ];
+#pragma unroll
+        for (int q = 0; q < CQ; ++q) {
+          sum[q] = __longlong_as_double((long long)PTC_NEG_ZERO_BITS);
+          fl[q] = 0;
         }
-        for (; kk < nch; ++kk) sum += P.hacc[(slab0 + kk) * ncols + c];
-        if (flags)
-          for (uint32_t j = 0; j < nch; ++j) fl |= P.hflag[(slab0 + j) * ncols + c];
-        acc[c] = sum;
-        if (flags) {
-          const unsigned long long m = __ballot(fl != 0);
-          if ((lane & 31) == 0) fbits[c >> 5] = (uint32_t)(m >> (lane & 32));
+        uint32_t kk = 0;
+        for (; kk + 4 <= nch; kk += 4) {
+          double x[CQ][4];
+#pragma unroll
+          for (int q = 0; q < CQ; ++q) {
+            const uint64_t c = c0 + (uint64_t)q * NT;
+            const double* a = P.hacc + (slab0 + kk) * ncols + min(c, ncols - 1);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) x[q][j] = a[j * ncols];  // (a clamped column: read, not used)
+          }
+#pragma unroll
+          for (int q = 0; q < CQ; ++q)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) sum[q] += x[q][j];
+        }
+        for (; kk < nch; ++kk)
+#pragma unroll
+          for (int q = 0; q < CQ; ++q) sum[q] += P.hacc[(slab0 + kk) * ncols + min(c0 + (uint64_t)q * NT, ncols - 1)];
+#pragma unroll
+        for (int q = 0; q < CQ; ++q) {
+          const uint64_t c = c0 + (uint64_t)q * NT;
+          if (flags && c < ncols)
+            for (uint32_t j = 0; j < nch; ++j) fl[q] |= P.hflag[(slab0 + j) * ncols + c];
+          if (c < ncols) acc[c] = sum[q];
+          if (flags) {  // lanes past the columns hold no flag
+            const unsigned long long m = __ballot(fl[q] != 0);
+            if ((lane & 31) == 0 && c < ncols) fbits[c >> 5] = (uint32_t)(m >> (lane & 32));
+          }
         }
       }
     }
