@@ -49,6 +49,8 @@ int main(int argc, char** argv) {
 
   ptc_ctx* ctx = nullptr;
   if ((rc = ptc_create(&ctx, 0))) return fail(nullptr, rc, "opening the GPU");
+  // the per-phase times printed below (main.cpp prints its phases too)
+  if ((rc = ptc_set_timing(ctx, 1))) return fail(ctx, rc, "phase timing");
   ptc_options opt;
   ptc_options_default(&opt);
   if (method == "upper_bound") {
