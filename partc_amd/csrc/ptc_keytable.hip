@@ -1229,8 +1229,8 @@ __device__ __forceinline__ void pc_tile_row(uint64_t t, uint32_t nb, const unsig
 }
 
 // Direct key spaces: one tile in key order, straight into the grouped layout
-// (ids, non-key tuples, values, slice indices).  Dynamic LDS as
-// k_pd_scatter.
+// (ids, non-key tuples, values; slice indices when slice_of is non-NULL).
+// Dynamic LDS as k_pd_scatter.
 template <class RestT, class ValT>
 __global__ void __launch_bounds__(PT_THREADS) k_pd_place(const uint32_t* __restrict__ h, uint32_t d, uint64_t N,
                                                          KeyCodec codec, KeyCodec rest, const ValT* __restrict__ val,
@@ -1276,7 +1276,7 @@ __global__ void __launch_bounds__(PT_THREADS) k_pd_place(const uint32_t* __restr
       const uint32_t j = j0 + threadIdx.x + u * blockDim.x;
       ii[u] = j < n ? ls[j] : 0u;
       v[u] = j < n ? val[base + ii[u]] : ValT(0);
-      sl[u] = j < n ? cidx[lb[ii[u]]] : 0u;
+      sl[u] = j < n && slice_of ? cidx[lb[ii[u]]] : 0u;
     }
 #pragma unroll
     for (int u = 0; u < PT_U; ++u) {
@@ -1288,7 +1288,7 @@ __global__ void __launch_bounds__(PT_THREADS) k_pd_place(const uint32_t* __restr
       rest_out[p] = lrk[i];
       if (rest16) rest16[p] = (uint16_t)lrk[i];
       gval[p] = v[u];
-      slice_of[p] = sl[u];
+      if (slice_of) slice_of[p] = sl[u];
     }
   }
 }
@@ -2218,13 +2218,14 @@ int build_direct_partitioned(ptc_ctx* ctx, ptc_hashtable* ht, const ptc_dtensor*
   uint32_t* btot = ws.get<uint32_t>(nb);                      // key totals
   unsigned long long* off = ws.get<unsigned long long>(nb + 1);
   uint32_t* cidx = ws.get<uint32_t>(nb);
-  ht->slice_of = dalloc_n<uint32_t>(ctx, N);
+  // (the slice index of every position is built on first use: only a
+  // contraction that wants per-row features reads it)
   ht->d_total = dalloc_n<unsigned long long>(ctx, 1);
   ht->slices_cap = std::max<uint64_t>(1, std::min<uint64_t>(N, ht->cap));
   ht->slices = dalloc_n<KtSlice>(ctx, ht->slices_cap);
   if (!rest64 && ht->rest.space <= 65536 && !(ht->rest_key16 = dalloc_n<uint16_t>(ctx, std::max<uint64_t>(N, 1))))
     return PTC_ERR_NOMEM;
-  if (!tcnt || !toff || !segpre || !btot || !off || !cidx || !ht->slice_of || !ht->d_total || !ht->slices) return PTC_ERR_NOMEM;
+  if (!tcnt || !toff || !segpre || !btot || !off || !cidx || !ht->d_total || !ht->slices) return PTC_ERR_NOMEM;
   int rc = place_offsets(ctx, t, ht->codec, -1, nb, ntiles, nseg, tcnt, toff, segpre, btot, off, 1, ht->slices,
                          ht->entries, cidx, ht->d_total, nullptr);
   if (rc) return rc;
@@ -2241,7 +2242,7 @@ int build_direct_partitioned(ptc_ctx* ctx, ptc_hashtable* ht, const ptc_dtensor*
                        t->hedges, t->d, N, ht->codec, ht->rest, (const VT*)(VALS), nb, ntiles,                      \
                        (const unsigned long long*)off, (const uint32_t*)tcnt, (const uint32_t*)toff,              \
                        (const uint32_t*)segpre,                                                                     \
-                       (const uint32_t*)cidx, ht->perm, RK, GV, ht->slice_of, R16);                                 \
+                       (const uint32_t*)cidx, ht->perm, RK, GV, (uint32_t*)nullptr, R16);                           \
   } while (0)
   if (rest64) {
     if (f32) PTC_PD(unsigned long long, float, t->vals32, ht->rest_key64, ht->gval32, (uint16_t*)nullptr);

@@ -406,8 +406,10 @@ int prepare_launch(Run& R, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B,
 
   int rc = PTC_OK;
   if ((R.dense || R.win_n) && (rc = hashtable_col16(ctx, HT_B, &R.bcol16))) return rc;
-  const uint32_t* row_of_pos = nullptr;  // HT_A's slice index of every A position
-  if ((rc = hashtable_slice_of(ctx, HT_A, &row_of_pos))) return rc;
+  // HT_A's slice index of every A position: only the per-row features of
+  // the training profile read it
+  const uint32_t* row_of_pos = nullptr;
+  if (R.want_features && (rc = hashtable_slice_of(ctx, HT_A, &row_of_pos))) return rc;
   R.row_of_pos = row_of_pos;
   Workspace& ws = R.ws;
   R.pos_len = ws.get<uint32_t>(R.npos + 4);  // + the rest of the last 16-byte word (read whole by the learned features)
