@@ -1296,7 +1296,7 @@ __global__ void __launch_bounds__(PT_THREADS) k_pd_place(const uint32_t* __restr
 // Wide keys: one tile into hash-bucket order (key, id, payload), as
 // k_pt_scatter, its runs placed at the reserved offsets.  Dynamic LDS as
 // k_pt_scatter.
-template <class RestT, class ValT>
+template <class RestT, class ValT, class KeyT>
 __global__ void __launch_bounds__(PT_THREADS) k_pt_place(const uint32_t* __restrict__ h, uint32_t d, uint64_t N,
                                                          KeyCodec codec, KeyCodec rest, const ValT* __restrict__ val,
                                                          int bbits, uint32_t nb, uint64_t ntiles,
@@ -1310,7 +1310,9 @@ __global__ void __launch_bounds__(PT_THREADS) k_pt_place(const uint32_t* __restr
   const uint64_t t = pt_tile_of(blockIdx.x, ntiles);
   if (t >= ntiles) return;
   extern __shared__ unsigned long long pt_smem[];
-  unsigned long long* lkey = pt_smem;                       // PT_TILE
+  // keys of a space within 32 bits take 4 bytes of LDS: a tile then needs
+  // 48 KB (three workgroups a CU, the whole nips-shape B tensor at once)
+  KeyT* lkey = reinterpret_cast<KeyT*>(pt_smem);            // PT_TILE
   RestT* lrk = reinterpret_cast<RestT*>(lkey + PT_TILE);    // PT_TILE
   uint16_t* ls = reinterpret_cast<uint16_t*>(lrk + PT_TILE); // PT_TILE: tile index of every bucket-ordered slot
   uint32_t* lcur = reinterpret_cast<uint32_t*>(ls + PT_TILE); // nb
@@ -1326,7 +1328,7 @@ __global__ void __launch_bounds__(PT_THREADS) k_pt_place(const uint32_t* __restr
     for (int u = 0; u < PT_U; ++u) {
       const uint32_t i = i0 + threadIdx.x + u * blockDim.x;
       if (i >= n) continue;
-      lkey[i] = key[u];
+      lkey[i] = (KeyT)key[u];
       lrk[i] = (RestT)rk[u];
       ls[atomicAdd(&lcur[pt_bucket(key[u], bbits)], 1u)] = (uint16_t)i;
     }
@@ -2168,20 +2170,25 @@ int build_partitioned(ptc_ctx* ctx, ptc_hashtable* ht, const ptc_dtensor* t, boo
   if (!region)
     hipLaunchKernelGGL(k_kt_clear, dim3(grid_for(ht->cap, 256)), dim3(256), 0, st, ht->entries, (uint32_t*)nullptr,
                        ht->cap);
-#define PTC_PT(RT, VT, VALS, RK, GV, R16)                                                                              \
+#define PTC_PLACE(RT, VT, KT, VALS)                                                                                    \
   do {                                                                                                               \
-    const size_t lds = (size_t)PT_TILE * (8 + sizeof(RT) + 2) + (size_t)nb * 8;                                   \
+    const size_t lds = (size_t)PT_TILE * (sizeof(KT) + sizeof(RT) + 2) + (size_t)nb * 8;                          \
     static size_t lds_set = 0;                                                                                       \
     if (lds > 65536 && lds > lds_set) {                                                                              \
-      PTC_HIP(ctx, hipFuncSetAttribute((const void*)k_pt_place<RT, VT>,                                             \
+      PTC_HIP(ctx, hipFuncSetAttribute((const void*)k_pt_place<RT, VT, KT>,                                         \
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));                      \
       lds_set = lds;                                                                                                 \
     }                                                                                                                \
-    hipLaunchKernelGGL((k_pt_place<RT, VT>), dim3((unsigned)(8 * ((ntiles + 7) / 8))), dim3(PT_THREADS), lds, st,    \
-                       t->hedges, t->d, N,                                                                          \
+    hipLaunchKernelGGL((k_pt_place<RT, VT, KT>), dim3((unsigned)(8 * ((ntiles + 7) / 8))), dim3(PT_THREADS), lds,    \
+                       st, t->hedges, t->d, N,                                                                      \
                        ht->codec, ht->rest, (const VT*)(VALS), bbits, nb, ntiles, (const unsigned long long*)off,   \
                        (const uint32_t*)tcnt, (const uint32_t*)toff, (const uint32_t*)segpre, pkey, pid,            \
                        (KtPay<RT, VT>*)ppay);                                                                       \
+  } while (0)
+#define PTC_PT(RT, VT, VALS, RK, GV, R16)                                                                              \
+  do {                                                                                                               \
+    if (ht->codec.space <= (1ull << 32)) PTC_PLACE(RT, VT, uint32_t, VALS);                                         \
+    else PTC_PLACE(RT, VT, unsigned long long, VALS);                                                                \
     hipLaunchKernelGGL((k_pt_group<RT, VT>), dim3(nb), dim3(PT_GROUP_THREADS), 0, st,                              \
                        (const unsigned long long*)off,                                                              \
                        (uint64_t)1, nb, N, (const unsigned long long*)pkey, (const uint32_t*)pid,                   \
@@ -2196,6 +2203,7 @@ int build_partitioned(ptc_ctx* ctx, ptc_hashtable* ht, const ptc_dtensor* t, boo
     else PTC_PT(uint32_t, double, t->vals, ht->rest_key32, ht->gval, ht->rest_key16);
   }
 #undef PTC_PT
+#undef PTC_PLACE
   hipLaunchKernelGGL(k_pt_finish, dim3(nb), dim3(256), 0, st, (const unsigned long long*)off, nb,
                      (const uint32_t*)bkeys, (const KtSlice*)tmps, ht->slices, ht->entries, ht->cap,
                      region ? 0 : 1, ht->direct ? 1 : 0, ht->d_total, N, region ? (const uint32_t*)ovf : nullptr,
