@@ -77,6 +77,16 @@ struct KtPay {
   RestT k;
   ValT v;
 };
+// 4-byte tuples with 8-byte values: the padding word carries the nonzero's
+// id through the partitioned builds (no separate id array to write and read)
+template <>
+struct KtPay<uint32_t, double> {
+  uint32_t k;
+  uint32_t id;
+  double v;
+};
+template <class RestT, class ValT>
+constexpr bool kPayId = sizeof(RestT) == 4 && sizeof(ValT) == 8;
 
 // Pass 1 (fused, the default for hashed tables): every lane inserts its own
 // key with atomicCAS along the probe order above; then the wavefront sorts
@@ -899,8 +909,9 @@ __device__ __forceinline__ void pt_scatter_tile(const uint32_t* __restrict__ h, 
       const unsigned long long key = lkey[i];
       const uint32_t p = gdst[pt_bucket(key, bbits)] + j;
       pkey[p] = key;
-      pid[p] = (uint32_t)(base + i);
       KtPay<RestT, ValT> pl;
+      if constexpr (kPayId<RestT, ValT>) pl.id = (uint32_t)(base + i);
+      else pid[p] = (uint32_t)(base + i);
       pl.k = lrk[i];
       pl.v = v[u];
       ppay[p] = pl;
@@ -1351,8 +1362,9 @@ __global__ void __launch_bounds__(PT_THREADS) k_pt_place(const uint32_t* __restr
       const unsigned long long key = lkey[i];
       const uint32_t p = gdst[pt_bucket(key, bbits)] + j;
       pkey[p] = key;
-      pid[p] = (uint32_t)(base + i);
       KtPay<RestT, ValT> pl;
+      if constexpr (kPayId<RestT, ValT>) pl.id = (uint32_t)(base + i);
+      else pid[p] = (uint32_t)(base + i);
       pl.k = lrk[i];
       pl.v = v[u];
       ppay[p] = pl;
@@ -1603,7 +1615,8 @@ __device__ bool pt_group_bucket(unsigned long long* tk, uint32_t* tc, uint32_t c
     // place through LDS: every nonzero's value and id go to its key's
     // cursor in LDS (the key table's words, free once the slices are out,
     // and the region map's), then out in order with coalesced stores; the
-    // non-key tuples the same way in a second round.  Scattered stores of
+    // non-key tuples in the cursor words (4 bytes) or in a second round
+    // (8 bytes).  Scattered stores of
     // 2-8 bytes to the bucket's range cost one cache-line access per lane.
     static_assert(PT_TAB * sizeof(unsigned long long) >= PT_TAB * sizeof(RestT) &&
                       PT_TAB * sizeof(unsigned long long) >= PT_TAB * sizeof(ValT),
@@ -1619,8 +1632,9 @@ __device__ bool pt_group_bucket(unsigned long long* tk, uint32_t* tc, uint32_t c
       for (int u = 0; u < U; ++u) {
         const uint32_t i = threadIdx.x + u * blockDim.x;
         const uint64_t q = bs + (i < n ? i : 0u);
-        id[u] = pid[q];
         pl[u] = ppay[q];
+        if constexpr (kPayId<RestT, ValT>) id[u] = pl[u].id;
+        else id[u] = pid[q];
       }
       __syncthreads();  // the region write-out has read the map
 #pragma unroll
@@ -1635,6 +1649,22 @@ __device__ bool pt_group_bucket(unsigned long long* tk, uint32_t* tc, uint32_t c
     }
     __syncthreads();
     const uint64_t base = bs - obase;
+    if constexpr (sizeof(RestT) == sizeof(uint32_t)) {
+      // 4-byte tuples: staged in the cursor words, free once every cursor
+      // has been taken; one round out for all three arrays
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (lp[u] != 0xFFFFFFFFu) tc[lp[u]] = (uint32_t)rk[u];
+      __syncthreads();
+      for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+        const RestT k = (RestT)tc[i];
+        perm[base + i] = stage4[i];
+        gval[base + i] = sv[i];
+        rest_out[base + i] = k;
+        if (X.rest16) X.rest16[base + i] = (uint16_t)k;
+      }
+      return true;
+    }
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
       perm[base + i] = stage4[i];
       gval[base + i] = sv[i];
@@ -1663,8 +1693,9 @@ __device__ bool pt_group_bucket(unsigned long long* tk, uint32_t* tc, uint32_t c
       const uint32_t i = i0 + u * blockDim.x;
       const uint64_t q = bs + (i < n ? i : 0u);
       k[u] = kept ? 0ull : pkey[q];
-      id[u] = pid[q];
       pl[u] = ppay[q];
+      if constexpr (kPayId<RestT, ValT>) id[u] = pl[u].id;
+      else id[u] = pid[q];
     }
 #pragma unroll
     for (int u = 0; u < UP; ++u) {
