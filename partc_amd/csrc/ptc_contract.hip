@@ -1141,22 +1141,35 @@ __global__ void __launch_bounds__(PB_MAX) k_pass(PassParams P) {
 // Persistent row pass over the work of many contractions: each workgroup
 // takes (job, work index) pairs from the queue until it is exhausted.  The
 // queue lists the heaviest work first, so the tail of the launch is short
-// items whatever the mix of job sizes.
+// items whatever the mix of job sizes.  The symbolic pass claims
+// QUEUE_CLAIM consecutive entries per fetch-add: fetch-adds on one word
+// serialise (tens of nanoseconds each on this GPU), and its light entries
+// (column bitmaps only) came out bounded by the claims (256-job batch:
+// 22.4 -> 16.5 ms of symbolic device time with the split-item counters
+// aggregated per wave).  The numeric pass claims one entry at a time: its
+// entries are long enough, and several heavy entries in one workgroup left
+// a longer tail (15.7 -> 17.0 ms).
+constexpr uint32_t QUEUE_CLAIM = 4;
 template <int PASS, bool DENSE, class ColT, bool MERGE, int VAL>
 __global__ void __launch_bounds__(PB_MAX) k_pass_batch(const PassParams* __restrict__ jobs, const uint2* __restrict__ queue,
                                                    uint32_t nq, uint32_t* __restrict__ head) {
   __shared__ uint32_t s_q;
+  constexpr uint32_t claim = PASS == PASS_SYMBOLIC ? QUEUE_CLAIM : 1u;
   for (;;) {
     __syncthreads();  // the previous item is done with LDS
-    if (threadIdx.x == 0) s_q = atomicAdd(head, 1u);
+    if (threadIdx.x == 0) s_q = atomicAdd(head, claim);
     __syncthreads();
-    const uint32_t q = s_q;
-    if (q >= nq) return;
-    const uint2 w = queue[q];
-    // a private copy: the job's parameters are loaded once per item (into
-    // scalar registers), not re-read after every atomic of the item
-    const PassParamsG P = reinterpret_cast<const PassParamsG*>(jobs)[w.x];
-    pass_item<PASS, DENSE, ColT, MERGE, VAL>(P, w.y);
+    const uint32_t q0 = s_q;
+    if (q0 >= nq) return;
+    const uint32_t q1 = min(nq, q0 + claim);
+    for (uint32_t q = q0; q < q1; ++q) {
+      if (q > q0) __syncthreads();  // the previous item is done with LDS
+      const uint2 w = queue[q];
+      // a private copy: the job's parameters are loaded once per item (into
+      // scalar registers), not re-read after every atomic of the item
+      const PassParamsG P = reinterpret_cast<const PassParamsG*>(jobs)[w.x];
+      pass_item<PASS, DENSE, ColT, MERGE, VAL>(P, w.y);
+    }
   }
 }
 
@@ -1532,14 +1545,28 @@ int launch_reset_rows(ptc_ctx* ctx, const uint32_t* list, uint32_t n, uint32_t* 
 }
 
 // A small row is one work item (a row split over several items needs the
-// dense share's slabs and merge).
+// dense share's slabs and merge).  One fetch-add per wave and list (the
+// wave's items ranked by ballot): fetch-adds on one word serialise, tens of
+// nanoseconds each on this GPU.
 __global__ void k_split_items(const uint4* __restrict__ items, uint64_t nitems, const unsigned long long* __restrict__ row_flops,
                               const uint32_t* __restrict__ row_nchunk, uint4* __restrict__ dense, uint4* __restrict__ sparse,
                               uint32_t* __restrict__ cnt) {
+  const int lane = threadIdx.x & 63;
+  const unsigned long long below = (1ull << lane) - 1ull;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nitems; i += (uint64_t)gridDim.x * blockDim.x) {
     const uint4 it = items[i];
-    if (row_flops[it.x] <= SPARSE_FLOPS && row_nchunk[it.x] == 1) sparse[atomicAdd(&cnt[1], 1u)] = it;
-    else dense[atomicAdd(&cnt[0], 1u)] = it;
+    const bool sp = row_flops[it.x] <= SPARSE_FLOPS && row_nchunk[it.x] == 1;
+    const unsigned long long ms = __ballot(sp), md = __ballot(!sp), act = ms | md;
+    const int leader = __ffsll((long long)act) - 1;  // the lowest active lane
+    uint32_t bs = 0, bd = 0;
+    if (lane == leader) {
+      if (ms) bs = atomicAdd(&cnt[1], (uint32_t)__popcll(ms));
+      if (md) bd = atomicAdd(&cnt[0], (uint32_t)__popcll(md));
+    }
+    bs = __shfl(bs, leader, 64);
+    bd = __shfl(bd, leader, 64);
+    if (sp) sparse[bs + __popcll(ms & below)] = it;
+    else dense[bd + __popcll(md & below)] = it;
   }
 }
 int launch_split_items(ptc_ctx* ctx, const uint4* items, uint64_t nitems, const unsigned long long* row_flops,
