@@ -998,7 +998,8 @@ __device__ __forceinline__ void pd_scatter_tile(const uint32_t* __restrict__ h, 
                                                 const unsigned long long* __restrict__ off, uint64_t obase,
                                                 const uint32_t* __restrict__ cidx, uint32_t* __restrict__ perm,
                                                 RestT* __restrict__ rest_out, ValT* __restrict__ gval,
-                                                uint32_t* __restrict__ slice_of, uint64_t t) {
+                                                uint32_t* __restrict__ slice_of, uint64_t t,
+                                                uint16_t* __restrict__ rest16 = nullptr) {
   extern __shared__ unsigned long long pt_smem[];
   RestT* lrk = reinterpret_cast<RestT*>(pt_smem);           // PT_TILE
   uint16_t* lb = reinterpret_cast<uint16_t*>(lrk + PT_TILE); // PT_TILE: key of every tile item
@@ -1038,7 +1039,7 @@ __device__ __forceinline__ void pd_scatter_tile(const uint32_t* __restrict__ h, 
       const uint32_t j = j0 + threadIdx.x + u * blockDim.x;
       ii[u] = j < n ? ls[j] : 0u;
       v[u] = j < n ? val[base + ii[u]] : ValT(0);
-      sl[u] = j < n ? cidx[lb[ii[u]]] : 0u;
+      sl[u] = j < n && slice_of ? cidx[lb[ii[u]]] : 0u;
     }
 #pragma unroll
     for (int u = 0; u < PT_U; ++u) {
@@ -1048,8 +1049,9 @@ __device__ __forceinline__ void pd_scatter_tile(const uint32_t* __restrict__ h, 
       const uint32_t p = gdst[lb[i]] + j;
       perm[p] = (uint32_t)(base + i);
       rest_out[p] = lrk[i];
+      if (rest16) rest16[p] = (uint16_t)lrk[i];
       gval[p] = v[u];
-      slice_of[p] = sl[u];
+      if (slice_of) slice_of[p] = sl[u];
     }
   }
 }
@@ -1913,6 +1915,7 @@ __global__ void __launch_bounds__(256) k_pt_finish(const unsigned long long* __r
   PTR(KtEntry) entries;                                                                         \
   PTR(KtSlice) slices;                                                                          \
   PTR(uint32_t) slice_of;                                                                       \
+  PTR(uint16_t) rest16;                                                                         \
   PTR(unsigned long long) total;                                                                \
   PTR(uint32_t) cidx;                                                                           \
   KeyCodec codec;                                                                               \
@@ -1994,7 +1997,7 @@ __global__ void __launch_bounds__(PT_THREADS) k_mt_scatter(const MtTableG* __res
                                                   T.item_base, T.cidx, T.perm, T.rest64, T.gval, T.slice_of, lt);
     else
       pd_scatter_tile<uint32_t, double>(T.hedges, T.d, T.N, codec, rest, T.vals, T.nb, T.ntiles, hs, os, T.item_base,
-                                        T.cidx, T.perm, T.rest32, T.gval, T.slice_of, lt);
+                                        T.cidx, T.perm, T.rest32, T.gval, T.slice_of, lt, T.rest16);
   } else {
     if (T.wide)
       pt_scatter_tile<unsigned long long, double>(T.hedges, T.d, T.N, codec, rest, T.vals, T.bbits, T.nb, T.ntiles, hs,
@@ -2022,7 +2025,7 @@ __global__ void __launch_bounds__(PT_GROUP_THREADS, PT_GROUP_WAVES_PER_SIMD) k_m
   else
     pt_group_one<uint32_t, double>(b, os, T.ntiles, T.nb, T.item_base, T.item_base + T.N, pkey, pid,
                                    (const KtPay<uint32_t, double>*)ppay, gkey, gcnt, tmps, &bkeys[blockIdx.x], T.perm,
-                                   T.rest32, T.gval, T.entries, T.cap, T.region, ovf_cnt, ovf_list, L);
+                                   T.rest32, T.gval, T.entries, T.cap, T.region, ovf_cnt, ovf_list, L, T.rest16);
 }
 
 __global__ void k_mt_slices(const MtTableG* __restrict__ tabs, int nt, const unsigned long long* __restrict__ off,
@@ -2592,9 +2595,13 @@ int hashtable_build_many(ptc_ctx* ctx, int n, const ptc_dtensor* const* ts, ptc_
     ht->d_total = dalloc_n<unsigned long long>(ctx, 1);
     ht->slices_cap = std::max<uint64_t>(1, std::min<uint64_t>(N, cap));
     ht->slices = dalloc_n<KtSlice>(ctx, ht->slices_cap);
-    if (counting) ht->slice_of = dalloc_n<uint32_t>(ctx, N);
+    // (the slice index of every position is built on first use) and the
+    // 16-bit copy of non-key tuples of at most 16 bits (the column keys that
+    // dense contractions stream) is written while grouping
+    if (!wide && ht->rest.space <= 65536 && !(ht->rest_key16 = dalloc_n<uint16_t>(ctx, std::max<uint64_t>(N, 1))))
+      return fail(PTC_ERR_NOMEM);
     if (!ht->entries || !ht->perm || !ht->gval || !(ht->rest_key32 || ht->rest_key64) || !ht->d_total ||
-        !ht->slices || (counting && !ht->slice_of))
+        !ht->slices)
       return fail(PTC_ERR_NOMEM);
     MtTable m{};
     m.hedges = t->hedges;
@@ -2605,7 +2612,8 @@ int hashtable_build_many(ptc_ctx* ctx, int n, const ptc_dtensor* const* ts, ptc_
     m.gval = ht->gval;
     m.entries = ht->entries;
     m.slices = ht->slices;
-    m.slice_of = ht->slice_of;
+    m.slice_of = nullptr;
+    m.rest16 = ht->rest_key16;
     m.total = ht->d_total;
     m.codec = ht->codec;
     m.rest = ht->rest;
