@@ -67,10 +67,8 @@ struct ItemP0Params {
   const unsigned long long* live;    // non-NULL: the item count on the device (low 32 bits)
   int find_rows;                     // write whole items, each row found in packed (k_items did not run)
   uint64_t nrows;                    // rows of packed (find_rows)
-  uint64_t npos;
-  const uint32_t* row_of_pos;
+  uint64_t npos;                     // A positions (none: no launch)
   const uint32_t* row_nchunk;
-  const uint32_t* pos_len;
   const KtSlice* rows;
   const unsigned long long* apre;
   const unsigned long long* row_flops;

@@ -633,9 +633,7 @@ int item_params(Run& R, uint64_t nitems, uint64_t nheavy, uint64_t npos, ItemPar
   Q.nitems = nitems;
   Q.live = R.early_items ? R.packed + R.nrows : nullptr;
   Q.npos = npos;
-  Q.row_of_pos = R.row_of_pos;
   Q.row_nchunk = R.row_nchunk;
-  Q.pos_len = R.pos_len;
   Q.rows = HT_A->slices;
   Q.apre = R.apre;
   Q.row_flops = R.row_flops;
