@@ -256,6 +256,8 @@ __global__ void __launch_bounds__(ROWS_SMALL_THREADS) k_rows_small(RowParams P, 
 }
 int launch_rows_small(ptc_ctx* ctx, const RowParams& P, const ItemParams& Q, uint32_t* cap, unsigned long long ncols,
                       uint32_t* zero, uint32_t nzero) {
+  // the kernel clamps its row loads to row nrows - 1
+  if (P.nrows == 0) return set_err(ctx, PTC_ERR_INTERNAL, "no rows for the one-workgroup row pass");
   if (P.nrows + 1 <= (uint64_t)ROWS_SMALL_THREADS * 4)
     hipLaunchKernelGGL(k_rows_small<4>, dim3(1), dim3(ROWS_SMALL_THREADS), 0, ctx->stream, P, Q, cap, ncols, zero,
                        nzero);

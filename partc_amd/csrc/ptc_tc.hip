@@ -333,6 +333,11 @@ int prepare_launch(Run& R, const ptc_hashtable* HT_A, const ptc_hashtable* HT_B,
   R.dO = A->nk + (B->d - B->nk);  // tensor.hpp:113
   R.use_flags = R.force_flags || !(A->vals_safe && B->vals_safe);
   if (R.dO == 0) return set_err(ctx, PTC_ERR_EMPTY, "tensor contraction not possible!! (output has no modes)");
+  // No nonzero on one side: no pair contributes (what the flop total read
+  // back below would show).  Returned here because an A table known to hold
+  // no key has no row for the row passes to read.
+  if (A->N == 0 || B->N == 0 || (HT_A->keys_known && HT_A->num_keys == 0))
+    return set_err(ctx, PTC_ERR_EMPTY, "tensor contraction not possible!! (no contributing pair)");
   // largest dense column space whose numeric pass fits the workgroup's LDS
   const uint64_t lds_cap = lds_limit(ctx);
   uint64_t dense_fit = 0;

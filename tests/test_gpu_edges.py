@@ -8,6 +8,8 @@
   every coordinate with a contributing pair (fksCuckooTC.hpp:318-330 adds
   without testing the sum): signed values that cancel, and zero values
   brought in by ptc_dtensor_set_values after a safe upload.
+* An operand with no nonzero is PTC_ERR_EMPTY, also on tables used again
+  once A's key count (0) is known.
 """
 import numpy as np
 import pytest
@@ -111,6 +113,37 @@ def test_zero_values_after_set_values(ctx, method):
     print(f"[case] set_values with zeros, {int((ov == 0).sum())} zero sums, {method}", flush=True)
     st = P.Stats()
     O = P.tc_fksCuckoo(P.buildHashTable(dA, ctx), P.buildHashTable(dB, ctx), stats=st, method=METHODS[method])
+    h, v = sort_coo(O.hedges_array, O.val_array)
+    assert np.array_equal(h, oh)
+    assert max_rel(v, ov) <= 1e-12
+
+
+@pytest.mark.parametrize("method", ["two_pass", "upper_bound", "learned"])
+def test_empty_operand_tables_reused(ctx, method):
+    """A with no nonzero (and, separately, B with none): no pair contributes,
+    so every call is PTC_ERR_EMPTY, as the reference's "tensor contraction
+    not possible!!" (fksCuckooTC.hpp).  The same tables are contracted again
+    after the first call and after the key count is read, when A's key
+    count of 0 is known: the row passes then have no row to read."""
+    if method == "learned" and not P.learned_available():
+        pytest.skip("no trained row-size model")
+    A, B = two_operands(35)
+    empty_a = P.Tensor(np.zeros((0, 2), np.uint32), np.zeros(0), np.array([300, 50], np.uint32),
+                       np.array([0, 1], np.uint32), 1)
+    empty_b = P.Tensor(np.zeros((0, 2), np.uint32), np.zeros(0), np.array([50, 400], np.uint32),
+                       np.array([0, 1], np.uint32), 1)
+    for a, b, name in ((empty_a, B, "empty A"), (A, empty_b, "empty B")):
+        print(f"[case] {name}, {method}", flush=True)
+        HA, HB = P.buildHashTable(a, ctx), P.buildHashTable(b, ctx)
+        for step in range(3):
+            if step == 1:
+                assert HA.info()[1] == a.N
+            with pytest.raises(P.PtcError) as e:
+                P.tc_fksCuckoo(HA, HB, method=METHODS[method])
+            assert e.value.code == -4, (name, step)
+    # the context still contracts after the empty calls
+    (oh, ov), _ = oracle(A, B)
+    O = P.tc_fksCuckoo(P.buildHashTable(A, ctx), P.buildHashTable(B, ctx), method=METHODS[method])
     h, v = sort_coo(O.hedges_array, O.val_array)
     assert np.array_equal(h, oh)
     assert max_rel(v, ov) <= 1e-12
