@@ -186,6 +186,23 @@ def test_oracle_empty_contraction_reports_error():
     assert e.value.code == -4
 
 
+@pytest.mark.parametrize("side", ["A", "B"])
+def test_oracle_empty_operand_reports_error(side):
+    """An operand with no nonzero has no contributing pair either."""
+    h, _ = P.gen_coo(3, [300, 50], 200, P.DIST_UNIFORM)
+    A = P.Tensor(h, np.ones(h.shape[0]), np.array([300, 50], np.uint32), np.array([0, 1], np.uint32), 1)
+    h, _ = P.gen_coo(4, [50, 400], 200, P.DIST_UNIFORM)
+    B = P.Tensor(h, np.ones(h.shape[0]), np.array([50, 400], np.uint32), np.array([0, 1], np.uint32), 1)
+    empty = lambda T: P.Tensor(np.zeros((0, 2), np.uint32), np.zeros(0), T.dimensions_array, T.dim_tc, 1)
+    if side == "A":
+        A = empty(A)
+    else:
+        B = empty(B)
+    with pytest.raises(orc.OracleError) as e:
+        orc.tc_fksCuckoo(A, B)
+    assert e.value.code == -4
+
+
 def test_fastmod_matches_modulo():
     rng = np.random.default_rng(3)
     nums = np.concatenate([np.arange(0, 4096), rng.integers(0, 2 ** 32, 4096),
