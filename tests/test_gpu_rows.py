@@ -11,6 +11,10 @@ covers and from per-part records for the parts inside it: rows here cover
 one part, straddle two, and span many.  Every case runs each method and
 compares with the oracle (coordinates bit-exact after sorting, values
 within 1e-12 relative) and the flop count.
+
+Key tables used again: the first contraction reads A's key count back, and
+later ones size every row array by that count instead of by the bound on
+distinct keys (the row passes then read no count from the device).
 """
 import numpy as np
 import pytest
@@ -72,3 +76,24 @@ def test_rows_spanning_many_parts(ctx):
     lens = [20000, 1, 5000, 3, 1023, 1025, 7000, 2, 2048, 12000]
     A, B = matrices(len(lens), lambda i: lens[i], nkeys=20000, ncols=2000, slice_len=4, seed=7)
     check(ctx, A, B, "long rows")
+
+
+@pytest.mark.parametrize("nrows,ncols", [(4095, 300), (9000, 300), (3000, 200000)])
+def test_tables_used_again(ctx, nrows, ncols):
+    """The same two key tables contracted by every method in turn, twice:
+    dense rows in one workgroup (4095) and in the multi-launch form (9000),
+    and hash-accumulated rows (200000 columns)."""
+    A, B = matrices(nrows, lambda i: 1 + i % 3, nkeys=400, ncols=ncols, slice_len=3, seed=nrows + 1)
+    oh, ov, info = orc.tc_fksCuckoo(A, B)
+    oh, ov = sort_coo(oh, ov)
+    names = [n for n in METHODS if n != "learned" or P.learned_available()]
+    HB, HA = P.buildHashTable(B, ctx), P.buildHashTable(A, ctx)
+    for rep in range(2):
+        for name in names:
+            print(f"[case] tables used again, {nrows} rows x {ncols} columns, call {rep} {name}", flush=True)
+            st = P.Stats()
+            O = P.tc_fksCuckoo(HA, HB, stats=st, method=METHODS[name])
+            h, v = sort_coo(O.hedges_array, O.val_array)
+            assert st.flops == info["flops"], name
+            assert h.shape == oh.shape and np.array_equal(h, oh), name
+            assert max_rel(v, ov) <= TOL, name
