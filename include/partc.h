@@ -266,9 +266,12 @@ typedef struct {
   uint64_t fallback_rows; /* learned path: rows whose prediction overflowed  */
   uint64_t heavy_rows;    /* rows split over several workgroups              */
   uint64_t spill_rows;    /* rows accumulated in HBM                          */
+  /* phase times (ms): filled with phase timing on (ptc_set_timing), else 0;
+   * ms_total is the call's wall time either way */
   double ms_build_b, ms_build_a, ms_preprocess, ms_symbolic, ms_numeric, ms_total;
   /* HIP-event durations of the row-pass kernels alone (k_pass launches on
-   * the context stream), summed over their launches in this call */
+   * the context stream), summed over their launches in this call; with
+   * phase timing on, else 0 */
   double ms_kernel_symbolic, ms_kernel_numeric;
   uint64_t kernel_launches_numeric;
   double ms_build_out;    /* ptc_tc_fksCuckoo_keyed: device time of the output's key table */
